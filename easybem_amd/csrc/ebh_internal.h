@@ -1,0 +1,131 @@
+// Internal definitions shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "easybem_hip.h"
+#include "host_geometry.h"
+
+namespace ebh {
+
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define EBH_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      ::ebh::set_error(std::string(#call) + " failed: " +                 \
+                       hipGetErrorString(e_));                            \
+      return EBH_ERR_HIP;                                                 \
+    }                                                                     \
+  } while (0)
+
+#define EBH_TRY(expr)                                                     \
+  do {                                                                    \
+    int rc_ = (expr);                                                     \
+    if (rc_ != EBH_OK) return rc_;                                        \
+  } while (0)
+
+// Owning device buffer (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count)
+  {
+    if (count == n && p) return EBH_OK;
+    release();
+    if (count == 0) return EBH_OK;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      set_error("device allocation of " + std::to_string(count * sizeof(T)) +
+                " bytes failed");
+      return EBH_ERR_NOMEM;
+    }
+    n = count;
+    return EBH_OK;
+  }
+  int upload(const T *h, size_t count)
+  {
+    EBH_TRY(alloc(count));
+    if (count) EBH_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return EBH_OK;
+  }
+  int upload(const std::vector<T> &v) { return upload(v.data(), v.size()); }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace ebh
+
+// Device-resident mesh: topology on the host, the arrays kernels read on the
+// device.
+struct ebh_mesh {
+  int device = 0;
+  ebh::Topology T;
+  ebh::DevBuf<double> nodes;                    // np*3
+  ebh::DevBuf<int> tri;                         // ne*3 (0-based)
+  ebh::DevBuf<int> touch_ptr, touch;            // exclusion lists
+  ebh::DevBuf<int> node_tri_ptr, node_tri, node_tri_k;
+  size_t bytes() const
+  {
+    return nodes.bytes() + tri.bytes() + touch_ptr.bytes() + touch.bytes() +
+           node_tri_ptr.bytes() + node_tri.bytes() + node_tri_k.bytes();
+  }
+};
+
+// Far-field rule data passed by value to kernels.
+struct RuleArgs {
+  int nt;                 // points per triangle
+  double x[16], y[16];    // barycentric-parameter coordinates of the points
+  double w[16];           // weights (sum 1/2)
+};
+
+struct ebh_op {
+  ebh_mesh *mesh = nullptr;
+  ebh_op_desc d{};
+  int nq = 0;                    // near-field Gauss points per dimension
+  RuleArgs rule{};
+  int nc = 1;                    // components (3 for Kelvin)
+  int nrows = 0, ncols = 0;      // test / trial DOFs (per component)
+  int nout = 1;                  // far outputs per target triangle (x nc)
+  // near field (FP64 CSR; Kelvin stores 6 symmetric entries per nnz)
+  ebh::DevBuf<int> near_ptr, near_col;
+  ebh::DevBuf<double> near_val;
+  int64_t near_nnz = 0;
+  // far-field geometry
+  ebh::DevBuf<float> tq32, sg32;     // target [nt][3][ne], source [ne*nt][3]
+  ebh::DevBuf<double> tq64, sg64;
+  ebh::DevBuf<double> tw, swt;       // w_q J per target / source point
+  ebh::DevBuf<double> normals;       // ne*3 unit normals
+  // per-apply workspace
+  ebh::DevBuf<float> src32;
+  ebh::DevBuf<double> src64;
+  ebh::DevBuf<double> partial, tri_out;
+  // launch configuration
+  int block = 256, tile = 64, chunks = 1, chunk = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t bytes() const;
+};
+
+namespace ebh {
+int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
+                 bool timed);
+int op_build_near(ebh_op *op);
+int op_build_far(ebh_op *op);
+int op_configure(ebh_op *op, int block, int tile, int chunks);
+}  // namespace ebh

@@ -1,0 +1,439 @@
+// Host-side mesh preparation: quadrature, Sauter-Schwab tables, topology,
+// near-field CSR patterns.  See host_geometry.h.
+#include "host_geometry.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace ebh {
+
+// ---------------------------------------------------------------- quadrature
+static const int kLineSizes[19] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12,
+                                   16, 20, 24, 32, 40, 48, 64};
+
+int line_rule_size(int idx)
+{
+  return (idx >= 0 && idx < 19) ? kLineSizes[idx] : -1;
+}
+
+bool gauss01(int n, std::vector<double> &p, std::vector<double> &w)
+{
+  if (n < 1 || n > 128) return false;
+  p.assign(n, 0.0);
+  w.assign(n, 0.0);
+  auto legendre = [n](long double x, long double &pn, long double &pn1) {
+    long double a = 1.0L, b = x;
+    for (int k = 2; k <= n; k++) {
+      long double c = ((2 * k - 1) * x * b - (k - 1) * a) / k;
+      a = b;
+      b = c;
+    }
+    if (n == 1) { a = 1.0L; b = x; }
+    pn = b;
+    pn1 = a;
+  };
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    long double x = cosl(3.14159265358979323846264338L * (i + 0.75L) /
+                         (n + 0.5L));
+    long double pn, pn1, dp = 1.0L;
+    for (int it = 0; it < 100; it++) {
+      legendre(x, pn, pn1);
+      dp = n * (x * pn - pn1) / (x * x - 1.0L);
+      long double dx = pn / dp;
+      x -= dx;
+      if (fabsl(dx) < 1e-19L) break;
+    }
+    legendre(x, pn, pn1);
+    dp = n * (x * pn - pn1) / (x * x - 1.0L);
+    long double wt = 1.0L / ((1.0L - x * x) * dp * dp);
+    p[i] = (double)((1.0L - x) / 2.0L);
+    p[n - 1 - i] = (double)((1.0L + x) / 2.0L);
+    w[i] = w[n - 1 - i] = (double)wt;
+  }
+  if (n % 2) p[n / 2] = 0.5;
+  return true;
+}
+
+bool line_rule(int idx, std::vector<double> &p, std::vector<double> &w)
+{
+  int n = line_rule_size(idx);
+  return n > 0 && gauss01(n, p, w);
+}
+
+namespace {
+struct TriRuleBuilder {
+  std::vector<double> &x, &y, &w;
+  void one(double a, double b, double wt) { x.push_back(a); y.push_back(b); w.push_back(wt); }
+  void orbit3(double a, double wt)  // (a,a), (a,1-2a), (1-2a,a)
+  {
+    double b = 1.0 - 2.0 * a;
+    one(a, a, wt); one(a, b, wt); one(b, a, wt);
+  }
+  void orbit6(double a, double b, double wt)  // permutations of (a,b,1-a-b)
+  {
+    double c = 1.0 - a - b;
+    one(a, b, wt); one(b, c, wt); one(c, a, wt);
+    one(b, a, wt); one(c, b, wt); one(a, c, wt);
+  }
+};
+}  // namespace
+
+// Published symmetric rules: Strang & Fix (1, 3, 4 points), Radon's closed
+// form (7 points, degree 5), Dunavant 1985 (6, 12, 13 points).
+bool tri_rule(int idx, std::vector<double> &x, std::vector<double> &y,
+              std::vector<double> &w)
+{
+  x.clear(); y.clear(); w.clear();
+  TriRuleBuilder R{x, y, w};
+  switch (idx) {
+  case 0: R.one(1.0 / 3.0, 1.0 / 3.0, 0.5); return true;
+  case 1: R.orbit3(1.0 / 6.0, 1.0 / 6.0); return true;
+  case 2:
+    R.one(1.0 / 3.0, 1.0 / 3.0, -27.0 / 96.0);
+    R.orbit3(0.2, 25.0 / 96.0);
+    return true;
+  case 3:
+    R.orbit3(0.445948490915965, 0.223381589678011 / 2);
+    R.orbit3(0.091576213509771, 0.109951743655322 / 2);
+    return true;
+  case 4: {
+    double s = std::sqrt(15.0);
+    R.one(1.0 / 3.0, 1.0 / 3.0, 9.0 / 80.0);
+    R.orbit3((6.0 - s) / 21.0, (155.0 - s) / 2400.0);
+    R.orbit3((6.0 + s) / 21.0, (155.0 + s) / 2400.0);
+    return true;
+  }
+  case 5:
+    R.orbit3(0.249286745170910, 0.116786275726379 / 2);
+    R.orbit3(0.063089014491502, 0.050844906370207 / 2);
+    R.orbit6(0.053145049844817, 0.310352451033784, 0.082851075618374 / 2);
+    return true;
+  case 6:
+    R.one(1.0 / 3.0, 1.0 / 3.0, -0.149570044467682 / 2);
+    R.orbit3(0.260345966079040, 0.175615257433208 / 2);
+    R.orbit3(0.065130102902216, 0.053347235608838 / 2);
+    R.orbit6(0.048690315425316, 0.312865496004874, 0.077113760890257 / 2);
+    return true;
+  default: return false;
+  }
+}
+
+// ------------------------------------------------------- Sauter-Schwab maps
+int ss_nregions(int cls) { return cls == 1 ? 2 : cls == 2 ? 5 : 3; }
+int ss_nouter(int cls, int nq)
+{
+  return cls == 1 ? nq * nq * nq : cls == 2 ? nq * nq : nq;
+}
+
+// Maps (inner, outer) -> (s, t) on the reference triangle {0<=s2<=s1<=1},
+// Jacobian and homogeneity factor h (x - y = h * d_m).  Vertex- and
+// edge-adjacent maps: Sauter & Schwab, Boundary Element Methods, sec. 5.2.1,
+// regions ordered so that d_m matches the reduced difference vectors of
+// src/BEM3dLaplace.cpp:76-99,133-142.  Coincident: the difference z = s - t
+// runs along six rays z = rho*zhat_m(eta) of the hexagon T - T; for fixed z,
+// s runs over T n (T+z), a triangle of side 1 - rho.
+static void ss_map(int cls, int m, const double *in, const double *out,
+                   double *s, double *t, double *jac, double *h)
+{
+  if (cls == 1) {
+    double xi = in[0], e1 = out[0], e2 = out[1], e3 = out[2];
+    double a0 = xi, a1 = xi * e1, b0 = xi * e2, b1 = xi * e2 * e3;
+    if (m == 0) { s[0] = a0; s[1] = a1; t[0] = b0; t[1] = b1; }
+    else { s[0] = b0; s[1] = b1; t[0] = a0; t[1] = a1; }
+    *jac = xi * xi * xi * e2;
+    *h = xi;
+  } else if (cls == 2) {
+    double xi = in[0], e1 = in[1], e2 = out[0], e3 = out[1];
+    switch (m) {
+    case 0:
+      s[0] = xi; s[1] = xi * e1;
+      t[0] = xi * (1 - e1 * e2 * e3); t[1] = xi * e1 * e2 * (1 - e3); break;
+    case 1:
+      s[0] = xi * (1 - e1 * e2); s[1] = xi * e1 * (1 - e2);
+      t[0] = xi; t[1] = xi * e1 * e2 * e3; break;
+    case 2:
+      s[0] = xi * (1 - e1 * e2 * e3); s[1] = xi * e1 * e2 * (1 - e3);
+      t[0] = xi; t[1] = xi * e1; break;
+    case 3:
+      s[0] = xi * (1 - e1 * e2 * e3); s[1] = xi * e1 * (1 - e2 * e3);
+      t[0] = xi; t[1] = xi * e1 * e2; break;
+    default:
+      s[0] = xi; s[1] = xi * e1 * e3;
+      t[0] = xi * (1 - e1 * e2); t[1] = xi * e1 * (1 - e2); break;
+    }
+    *jac = xi * xi * xi * e1 * e1 * (m < 4 ? e2 : 1.0);
+    *h = xi * e1;
+  } else {
+    double rho = in[0], l1 = in[1], l2 = in[2], eta = out[0], zh0, zh1;
+    switch (m % 3) {
+    case 0: zh0 = eta; zh1 = 1.0; break;
+    case 1: zh0 = 1.0; zh1 = eta; break;
+    default: zh0 = eta; zh1 = eta - 1.0; break;
+    }
+    if (m >= 3) { zh0 = -zh0; zh1 = -zh1; }
+    double z1 = rho * zh0, z2 = rho * zh1;
+    double a = std::max(z2, 0.0), b = 1.0 + std::min(z1, 0.0);
+    double c = std::max(z1 - z2, 0.0);
+    double V1x = a + c, V1y = a, V2x = b, V2y = a, V3x = b, V3y = b - c;
+    s[0] = V1x + l1 * (V2x - V1x) + l2 * (V3x - V1x);
+    s[1] = V1y + l1 * (V2y - V1y) + l2 * (V3y - V1y);
+    t[0] = s[0] - z1;
+    t[1] = s[1] - z2;
+    double side = b - a - c;
+    *jac = rho * side * side;
+    *h = rho;
+  }
+}
+
+// Exact inner integration: the inner integrands are polynomials of degree
+// <= 5 per variable, integrated by 4-point Gauss-Legendre and (coincident
+// simplex) the degree-4 six-point rule.  The coincident table folds the
+// opposite rays m and m+3, valid for kernels even in x - y.
+void ss_table(int cls, int kdeg, int nq, std::vector<double> &W)
+{
+  std::vector<double> L, Lw, G, Gw, Sx, Sy, Sw;
+  gauss01(nq, L, Lw);
+  gauss01(4, G, Gw);
+  tri_rule(3, Sx, Sy, Sw);
+  const int nreg = ss_nregions(cls), nout = ss_nouter(cls, nq);
+  W.assign((size_t)9 * nreg * nout, 0.0);
+  auto add = [&](double *Wm, double c, const double *s, const double *t) {
+    double fs[3] = {1.0 - s[0], s[0] - s[1], s[1]};
+    double ft[3] = {1.0 - t[0], t[0] - t[1], t[1]};
+    for (int k = 0; k < 3; k++)
+      for (int l = 0; l < 3; l++) Wm[3 * k + l] += c * fs[k] * ft[l];
+  };
+  for (int q = 0; q < nout; q++) {
+    double out[3] = {0, 0, 0};
+    if (cls == 1) {
+      out[0] = L[q / (nq * nq)]; out[1] = L[(q / nq) % nq]; out[2] = L[q % nq];
+    } else if (cls == 2) {
+      out[0] = L[q / nq]; out[1] = L[q % nq];
+    } else {
+      out[0] = L[q];
+    }
+    const int mreg = cls == 3 ? 6 : nreg;
+    for (int m = 0; m < mreg; m++) {
+      double *Wm = W.data() + ((size_t)(m % nreg) * nout + q) * 9;
+      double s[2], t[2], jac, h;
+      if (cls == 1) {
+        for (size_t g = 0; g < G.size(); g++) {
+          double in[1] = {G[g]};
+          ss_map(cls, m, in, out, s, t, &jac, &h);
+          add(Wm, Gw[g] * jac / std::pow(h, kdeg), s, t);
+        }
+      } else if (cls == 2) {
+        for (size_t g = 0; g < G.size(); g++)
+          for (size_t g2 = 0; g2 < G.size(); g2++) {
+            double in[2] = {G[g], G[g2]};
+            ss_map(cls, m, in, out, s, t, &jac, &h);
+            add(Wm, Gw[g] * Gw[g2] * jac / std::pow(h, kdeg), s, t);
+          }
+      } else {
+        for (size_t g = 0; g < G.size(); g++)
+          for (size_t r = 0; r < Sw.size(); r++) {
+            double in[3] = {G[g], Sx[r], Sy[r]};
+            ss_map(cls, m, in, out, s, t, &jac, &h);
+            add(Wm, Gw[g] * Sw[r] * jac / std::pow(h, kdeg), s, t);
+          }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- topology
+std::string build_topology(int np, const double *nodes, int ne,
+                           const int *elems, Topology &T)
+{
+  if (np < 3 || ne < 4) return "mesh too small (need >= 3 nodes, >= 4 triangles)";
+  T = Topology();
+  T.np = np;
+  T.ne = ne;
+  T.nodes.assign(nodes, nodes + (size_t)3 * np);
+  for (size_t k = 0; k < T.nodes.size(); k++)
+    if (!std::isfinite(T.nodes[k])) return "non-finite node coordinate";
+  T.tri.resize((size_t)3 * ne);
+  for (int i = 0; i < ne; i++) {
+    for (int k = 0; k < 3; k++) {
+      int n = elems[3 * i + k];
+      if (n < 1 || n > np)
+        return "element " + std::to_string(i) + " references node " +
+               std::to_string(n) + " outside 1.." + std::to_string(np);
+      T.tri[3 * i + k] = n - 1;
+    }
+    const int *f = &T.tri[3 * i];
+    if (f[0] == f[1] || f[1] == f[2] || f[0] == f[2])
+      return "element " + std::to_string(i) + " repeats a node";
+    const double *a = &T.nodes[3 * f[0]], *b = &T.nodes[3 * f[1]],
+                 *c = &T.nodes[3 * f[2]];
+    double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2],
+           n2 = u[0] * v[1] - u[1] * v[0];
+    double J = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    double L2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + v[0] * v[0] +
+                v[1] * v[1] + v[2] * v[2];
+    if (!(J > 1e-12 * L2)) return "element " + std::to_string(i) + " is degenerate";
+  }
+  // closed, consistently oriented 2-manifold: each undirected edge appears
+  // exactly twice, once in each direction
+  {
+    std::vector<std::pair<int64_t, int>> e;
+    e.reserve((size_t)3 * ne);
+    for (int i = 0; i < ne; i++)
+      for (int k = 0; k < 3; k++) {
+        int a = T.tri[3 * i + k], b = T.tri[3 * i + (k + 1) % 3];
+        int64_t key = (int64_t)std::min(a, b) * np + std::max(a, b);
+        e.push_back({key, a < b ? 1 : -1});
+      }
+    std::sort(e.begin(), e.end());
+    for (size_t k = 0; k < e.size();) {
+      size_t m = k;
+      int sum = 0;
+      while (m < e.size() && e[m].first == e[k].first) { sum += e[m].second; m++; }
+      if (m - k != 2)
+        return "edge (" + std::to_string(e[k].first / np + 1) + "," +
+               std::to_string(e[k].first % np + 1) + ") is shared by " +
+               std::to_string(m - k) +
+               " triangles: the mesh must be a closed 2-manifold";
+      if (sum != 0)
+        return "edge (" + std::to_string(e[k].first / np + 1) + "," +
+               std::to_string(e[k].first % np + 1) +
+               ") has the same direction in both triangles: inconsistent "
+               "orientation";
+      k = m;
+    }
+  }
+  // node -> triangles
+  T.node_tri_ptr.assign(np + 1, 0);
+  for (int i = 0; i < 3 * ne; i++) T.node_tri_ptr[T.tri[i] + 1]++;
+  for (int n = 0; n < np; n++) T.node_tri_ptr[n + 1] += T.node_tri_ptr[n];
+  T.node_tri.resize((size_t)3 * ne);
+  T.node_tri_k.resize((size_t)3 * ne);
+  {
+    std::vector<int> fill(T.node_tri_ptr.begin(), T.node_tri_ptr.end() - 1);
+    for (int i = 0; i < ne; i++)
+      for (int k = 0; k < 3; k++) {
+        int n = T.tri[3 * i + k];
+        T.node_tri[fill[n]] = i;
+        T.node_tri_k[fill[n]] = k;
+        fill[n]++;
+      }
+  }
+  for (int n = 0; n < np; n++)
+    if (T.node_tri_ptr[n + 1] == T.node_tri_ptr[n])
+      return "node " + std::to_string(n + 1) + " belongs to no triangle";
+  // touching lists and classified pairs
+  T.touch_ptr.assign(ne + 1, 0);
+  T.touch.clear();
+  T.pairs.clear();
+  std::vector<int> cand;
+  for (int i = 0; i < ne; i++) {
+    cand.clear();
+    for (int k = 0; k < 3; k++) {
+      int n = T.tri[3 * i + k];
+      for (int p = T.node_tri_ptr[n]; p < T.node_tri_ptr[n + 1]; p++)
+        cand.push_back(T.node_tri[p]);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    T.max_touch = std::max<int>(T.max_touch, (int)cand.size());
+    const int *fi = &T.tri[3 * i];
+    for (int j : cand) {
+      T.touch.push_back(j);
+      const int *fj = &T.tri[3 * j];
+      NearPair P;
+      P.i = i;
+      P.j = j;
+      if (i == j) {
+        P.cls = kCoincident;
+        for (int k = 0; k < 3; k++) P.oi[k] = P.oj[k] = fi[k];
+      } else {
+        // shared nodes scanned in j's order (src/BEM3dLaplace.cpp:692-709)
+        int ci[3], cj[3], cnt = 0;
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++)
+            if (fj[k] == fi[l]) { ci[cnt] = l; cj[cnt] = k; cnt++; break; }
+        if (cnt == 1) {
+          P.cls = kVertex;
+          for (int k = 0; k < 3; k++) {
+            P.oi[k] = fi[(ci[0] + k) % 3];
+            P.oj[k] = fj[(cj[0] + k) % 3];
+          }
+        } else if (cnt == 2) {
+          // orient so the shared edge runs cj0 -> cj0+1 in j
+          if (cj[0] == 0 && cj[1] == 2) {
+            std::swap(cj[0], cj[1]);
+            std::swap(ci[0], ci[1]);
+          }
+          int remi = 3 - ci[0] - ci[1], remj = 3 - cj[0] - cj[1];
+          P.cls = kEdge;
+          P.oi[0] = fi[ci[0]]; P.oi[1] = fi[ci[1]]; P.oi[2] = fi[remi];
+          P.oj[0] = fj[cj[0]]; P.oj[1] = fj[cj[1]]; P.oj[2] = fj[remj];
+        } else {
+          return "triangles " + std::to_string(i) + " and " +
+                 std::to_string(j) + " share all three nodes";
+        }
+      }
+      T.n_class[P.cls]++;
+      T.pairs.push_back(P);
+    }
+    T.touch_ptr[i + 1] = (int)T.touch.size();
+  }
+  T.n_class[kDisjoint] = (int64_t)ne * ne - (int64_t)T.pairs.size();
+  return "";
+}
+
+void build_near_csr(const Topology &T, int test_basis, int trial_basis,
+                    NearCSR &out)
+{
+  out = NearCSR();
+  out.nrows = test_basis ? T.np : T.ne;
+  out.ncols = trial_basis ? T.np : T.ne;
+  struct Item { int col; int64_t c; };
+  // bucket contributions by row, then sort within each row
+  std::vector<int> cnt(out.nrows + 1, 0);
+  const int nk = test_basis ? 3 : 1, nl = trial_basis ? 3 : 1;
+  for (const NearPair &P : T.pairs)
+    for (int k = 0; k < nk; k++) cnt[(test_basis ? P.oi[k] : P.i) + 1] += nl;
+  for (int r = 0; r < out.nrows; r++) cnt[r + 1] += cnt[r];
+  std::vector<Item> items(cnt[out.nrows]);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (size_t p = 0; p < T.pairs.size(); p++) {
+    const NearPair &P = T.pairs[p];
+    for (int k = 0; k < nk; k++) {
+      int row = test_basis ? P.oi[k] : P.i;
+      for (int l = 0; l < nl; l++) {
+        int col = trial_basis ? P.oj[l] : P.j;
+        int code;
+        if (test_basis && trial_basis) code = 3 * k + l;
+        else if (!test_basis && trial_basis) code = 9 + l;
+        else if (test_basis && !trial_basis) code = 12 + k;
+        else code = 15;
+        items[fill[row]++] = Item{col, (int64_t)p * 16 + code};
+      }
+    }
+  }
+  out.row_ptr.assign(out.nrows + 1, 0);
+  out.contrib_ptr.clear();
+  out.contrib_ptr.push_back(0);
+  out.contrib.reserve(items.size());
+  for (int r = 0; r < out.nrows; r++) {
+    auto b = items.begin() + cnt[r], e = items.begin() + cnt[r + 1];
+    std::sort(b, e, [](const Item &x, const Item &y) {
+      return x.col != y.col ? x.col < y.col : x.c < y.c;
+    });
+    for (auto it = b; it != e;) {
+      auto jt = it;
+      while (jt != e && jt->col == it->col) { out.contrib.push_back(jt->c); ++jt; }
+      out.col.push_back(it->col);
+      out.contrib_ptr.push_back((int)out.contrib.size());
+      it = jt;
+    }
+    out.row_ptr[r + 1] = (int)out.col.size();
+  }
+}
+
+}  // namespace ebh

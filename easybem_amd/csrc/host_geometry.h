@@ -1,0 +1,75 @@
+// Host-side mesh preparation for the layer-potential operators: quadrature
+// rules, reduced Sauter-Schwab weight tables, mesh validation, touching-pair
+// classification and near-field CSR patterns.  All of it runs once per mesh
+// (or once per operator) on the CPU; nothing here is on the per-apply path.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ebh {
+
+// ---------------------------------------------------------------- quadrature
+// Gauss-Legendre on [0,1]; sizes indexed like the reference table
+// (src/quadrature.cpp:286-287).
+int line_rule_size(int idx);
+bool gauss01(int n, std::vector<double> &p, std::vector<double> &w);
+bool line_rule(int idx, std::vector<double> &p, std::vector<double> &w);
+// Symmetric triangle rules, index d-1 of polynomial degree d, weights summing
+// to 1/2 (src/quadrature.cpp:1047-1048).  Indices 0..6 are supported.
+bool tri_rule(int idx, std::vector<double> &x, std::vector<double> &y,
+              std::vector<double> &w);
+
+// Reduced Sauter-Schwab weight tables.  cls: 1 vertex-adjacent, 2
+// edge-adjacent, 3 coincident.  kdeg: homogeneity degree of the kernel
+// (1 single layer / Kelvin, 2 double layer).  Layout
+// W[((m * nout) + q) * 9 + 3k + l], k test / l trial shape function in the
+// pair's local vertex order.
+int ss_nregions(int cls);
+int ss_nouter(int cls, int nq);
+void ss_table(int cls, int kdeg, int nq, std::vector<double> &W);
+
+// ---------------------------------------------------------------- topology
+enum PairClass : int { kDisjoint = 0, kVertex = 1, kEdge = 2, kCoincident = 3 };
+
+struct NearPair {
+  int i, j;       // test / trial triangle (0-based)
+  int cls;        // PairClass
+  int oi[3];      // global 0-based nodes of i in the pair's local order
+  int oj[3];      // same for j
+};
+
+struct Topology {
+  int np = 0, ne = 0;
+  std::vector<double> nodes;            // np*3
+  std::vector<int> tri;                 // ne*3, 0-based
+  std::vector<int> node_tri_ptr, node_tri;    // node -> triangles (CSR)
+  std::vector<int> node_tri_k;                // local index of node in tri
+  std::vector<int> touch_ptr, touch;          // triangle -> touching triangles
+                                              // (sorted ascending, incl. self)
+  std::vector<NearPair> pairs;                // all touching pairs, (i, j) order
+  int64_t n_class[4] = {0, 0, 0, 0};
+  int max_touch = 0;
+};
+
+// Validates the mesh (index range, non-degenerate triangles, every edge
+// shared by exactly two triangles with opposite orientation) and builds the
+// topology.  Returns an empty string on success, else the error.
+std::string build_topology(int np, const double *nodes, int ne,
+                           const int *elems_1based, Topology &T);
+
+// Near-field CSR pattern for a (test basis, trial basis) combination with
+// contribution lists: entry e sums pairbuf values listed in
+// contrib[contrib_ptr[e] .. contrib_ptr[e+1]); a contribution is
+// pair * 16 + code, code 0..8 = slot 3k+l, 9+l = sum over k (P0 test),
+// 12+k = sum over l (P0 trial), 15 = sum of all nine (P0 x P0).
+struct NearCSR {
+  int nrows = 0, ncols = 0;
+  std::vector<int> row_ptr, col;
+  std::vector<int> contrib_ptr;
+  std::vector<int64_t> contrib;
+};
+void build_near_csr(const Topology &T, int test_basis, int trial_basis,
+                    NearCSR &out);
+
+}  // namespace ebh

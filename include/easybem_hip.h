@@ -1,0 +1,135 @@
+/* easybem-hip: C-ABI of the MI355X (gfx950) matrix-free layer-potential
+ * apply, its Krylov solvers and the solve server.
+ *
+ * Plain C types only: host/device pointers, sizes, opaque handles and int
+ * status codes (0 = success, < 0 = error; ebh_last_error() explains).
+ *
+ * Which reference interface each entry point replaces:
+ *   ebh_mesh_create  initGPU's mesh upload and createMeshTopology
+ *                    (include/BEM3dLaplaceGPU.h:8-10; src/BEM3dLaplaceGPU.cu:830-921;
+ *                    src/meshTopology.cpp:196-261)
+ *   ebh_op_create    initGPU's quadrature upload (src/BEM3dLaplaceGPU.cu:867-917)
+ *                    plus the near-field Sauter-Schwab integrals that
+ *                    applyGPULaplace3d1Layer/2Layer recompute on every call
+ *                    (src/BEM3dLaplaceGPU.cu:997-1222,1340-1460)
+ *   ebh_op_apply     applyGPULaplace3d1Layer (ansatz 0/1) and
+ *                    applyGPULaplace3d2Layer (include/BEM3dLaplaceGPU.h:14-16;
+ *                    src/BEM3dLaplaceGPU.cu:985-1551); the Kelvin single layer
+ *                    of getElasticity3d (src/BEM3dElasticity.cpp:838-846)
+ *                    applied matrix-free
+ *   ebh_op_destroy / ebh_mesh_destroy   closeGPU (src/BEM3dLaplaceGPU.cu:924-957)
+ *   ebh_mass_apply   getLaplace3dMass (src/BEM3dLaplace.cpp:448-486), P0 x P1,
+ *                    and the P1 x P1 mass matrix
+ *   ebh_solve*       the scipy Krylov loop of
+ *                    python/examples/laplace3dDirichletGPU.py:60-83
+ *                    (GMRES instead of CG; the reference has no solver)
+ *   ebh_server_*     no reference counterpart (solve service)
+ */
+#ifndef EASYBEM_HIP_H
+#define EASYBEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  EBH_OK = 0,
+  EBH_ERR_ARG = -1,         /* invalid argument or size                 */
+  EBH_ERR_MESH = -2,        /* mesh rejected (open, degenerate, ...)    */
+  EBH_ERR_HIP = -3,         /* HIP runtime error                        */
+  EBH_ERR_UNSUPPORTED = -4, /* operator/basis/rule combination          */
+  EBH_ERR_NOMEM = -5,       /* device allocation failed / over budget   */
+  EBH_ERR_NOCONV = -6,      /* solver hit its iteration limit           */
+  EBH_ERR_STATE = -7,       /* handle in the wrong state                */
+  EBH_ERR_NONFINITE = -8    /* NaN/Inf in an input vector               */
+};
+
+enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2 };
+enum { EBH_P0 = 0, EBH_P1 = 1 };
+enum { EBH_FP64 = 0, EBH_FP32 = 1 };
+
+typedef struct ebh_mesh ebh_mesh;
+typedef struct ebh_op ebh_op;
+typedef struct ebh_solver ebh_solver;
+typedef struct ebh_server ebh_server;
+
+/* Operator descriptor.  line_rule / tri_rule are indices into the
+ * reference's Gauss-Legendre and triangle-rule tables
+ * (src/quadrature.cpp:286-287,1047-1048); the reference's single `order`
+ * argument corresponds to line_rule == tri_rule == order. */
+typedef struct {
+  int kernel;        /* EBH_LAPLACE_SL, EBH_LAPLACE_DL, EBH_KELVIN_SL     */
+  int test_basis;    /* EBH_P0 (one DOF per triangle) or EBH_P1 (nodes)   */
+  int trial_basis;   /* DL requires EBH_P1; Kelvin requires test==trial   */
+  int far_precision; /* EBH_FP64 or EBH_FP32 (near field is always FP64) */
+  int line_rule;     /* near-field Gauss index, 0..18                     */
+  int tri_rule;      /* far-field triangle rule index, 0..6               */
+  double nu;         /* Poisson ratio for Kelvin (E = 1)                  */
+  int max_rhs;       /* largest nrhs passed to ebh_op_apply (>= 1)        */
+} ebh_op_desc;
+
+const char *ebh_last_error(void);
+int ebh_version(int *major, int *minor);
+
+/* ---- device memory helpers (so callers need no HIP headers) ---------- */
+int ebh_malloc(void **ptr, size_t bytes);
+int ebh_free(void *ptr);
+int ebh_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ebh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ebh_memset(void *dst, int value, size_t bytes);
+int ebh_synchronize(void);
+int ebh_device_info(int *n_devices, int *n_cu, size_t *free_bytes,
+                    size_t *total_bytes);
+
+/* ---- mesh ------------------------------------------------------------- */
+/* nodes: np x 3 FP64 row-major; elems: ne x 3 int32, 1-based, outward
+ * orientation (P2-P1)x(P3-P1).  The mesh must be a closed, consistently
+ * oriented 2-manifold; otherwise EBH_ERR_MESH. */
+int ebh_mesh_create(int np, const double *nodes, int ne, const int *elems,
+                    ebh_mesh **out);
+/* info[0..7] = np, ne, coincident, edge-adjacent, vertex-adjacent and
+ * disjoint pair counts, max touching triangles per triangle, bytes held. */
+int ebh_mesh_info(const ebh_mesh *mesh, int64_t *info);
+void ebh_mesh_destroy(ebh_mesh *mesh);
+
+/* ---- operator --------------------------------------------------------- */
+int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out);
+/* Per-RHS vector lengths including Kelvin's three components:
+ * rows = test DOFs, cols = trial DOFs. */
+int ebh_op_dims(const ebh_op *op, int *nrows, int *ncols);
+/* y = A x on device memory; x holds nrhs vectors of ncols entries
+ * back-to-back, y receives nrhs vectors of nrows entries.  Kelvin vectors
+ * are component-blocked (all x components, then y, then z), like the
+ * reference's 3ne x 3ne layout.  stream: a hipStream_t or NULL. */
+int ebh_op_apply(ebh_op *op, const double *x, double *y, int nrhs,
+                 void *stream);
+/* Same on host memory (copies in and out, synchronous). */
+int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs);
+/* Far-field launch configuration: block threads, source tile (triangles),
+ * source chunks.  Zero keeps the current value. */
+int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks);
+int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks);
+/* Averages over the last `ebh_op_time` calls: milliseconds per apply of
+ * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
+ * measured with HIP events on the op's stream over `reps` applies of the
+ * given device vectors. */
+int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
+                double *ms);
+/* Work counts: [0] far interactions per RHS, [1] near nnz (entries),
+ * [2] near rows, [3] near cols. */
+int ebh_op_work(const ebh_op *op, int64_t *work);
+void ebh_op_destroy(ebh_op *op);
+
+/* ---- mass matrices ---------------------------------------------------- */
+/* y = M u with M the P0 x P1 (test_basis 0) or P1 x P1 (test_basis 1) mass
+ * matrix; u has np entries.  Host pointers. */
+int ebh_mass_apply(const ebh_mesh *mesh, int test_basis, const double *u,
+                   double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

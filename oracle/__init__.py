@@ -1,0 +1,198 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker (never as the measured or shipped path).
+
+``Oracle`` wraps oracle/liboracle.so (the C restatement of the reference
+operator, oracle/oracle.c).  ``Reference`` wraps oracle/_ref/libeasybem_ref.so,
+the reference's own CPU sources compiled by oracle/Makefile; it exists only
+where the reference sources were present at build time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libeasybem_ref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def _mesh(nodes, elems):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    return nodes, elems, nodes.shape[0], elems.shape[0]
+
+
+class Oracle:
+    def __init__(self, path: str = LIB):
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_line_rule.argtypes = [C.c_int, _dp, _dp]
+        L.orc_tri_rule.argtypes = [C.c_int, _dp, _dp]
+        L.orc_gauss01.argtypes = [C.c_int, _dp, _dp]
+        L.orc_laplace_dense.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int,
+                                        C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
+        L.orc_kelvin_dense.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_double,
+                                       C.c_int, C.c_int, _dp]
+        L.orc_apply.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                C.c_int, _ip, C.c_int, C.c_int, C.c_double,
+                                _dp, C.c_int, _ip, _dp, C.c_int]
+        L.orc_pair_block.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _ip,
+                                     C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _dp, _ip, _ip]
+        L.orc_ss_full.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int]
+        L.orc_ss_full.restype = C.c_double
+        L.orc_ss_table.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        self.L = L
+
+    # quadrature -------------------------------------------------------
+    def line_rule(self, idx):
+        p = np.zeros(128); w = np.zeros(128)
+        n = self.L.orc_line_rule(idx, p, w)
+        if n < 0:
+            raise ValueError(idx)
+        return p[:n].copy(), w[:n].copy()
+
+    def tri_rule(self, idx):
+        p = np.zeros(256); w = np.zeros(128)
+        n = self.L.orc_tri_rule(idx, p, w)
+        if n < 0:
+            raise ValueError(idx)
+        return p[:2 * n].reshape(n, 2).copy(), w[:n].copy()
+
+    def gauss01(self, n):
+        p = np.zeros(n); w = np.zeros(n)
+        self.L.orc_gauss01(n, p, w)
+        return p, w
+
+    # operators ----------------------------------------------------------
+    def laplace_dense(self, nodes, elems, line_idx, tri_idx=None):
+        """Dense (V[test,trial], K[test,node], M[test,node]) in P0 x P0 / P0 x P1."""
+        tri_idx = line_idx if tri_idx is None else tri_idx
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        V = np.zeros((ne, ne)); K = np.zeros((ne, np_)); M = np.zeros((ne, np_))
+        rc = self.L.orc_laplace_dense(np_, nodes, ne, elems, line_idx, tri_idx,
+                                      V.ctypes.data, K.ctypes.data,
+                                      M.ctypes.data)
+        if rc:
+            raise RuntimeError(rc)
+        return V, K, M
+
+    def kelvin_dense(self, nodes, elems, nu, line_idx, tri_idx=None):
+        tri_idx = line_idx if tri_idx is None else tri_idx
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        V = np.zeros((3 * ne, 3 * ne))
+        if self.L.orc_kelvin_dense(np_, nodes, ne, elems, nu, line_idx,
+                                   tri_idx, V):
+            raise RuntimeError("kelvin_dense")
+        return V
+
+    def apply(self, op, test_basis, trial_basis, nodes, elems, x, rows=None,
+              line_idx=4, tri_idx=None, nu=0.3, nthreads=0):
+        """Row-sampled apply.  op: 'V', 'K' or 'kelvin'.  Returns y[rows]
+        (shape (nrows,) or (3, nrows) for Kelvin)."""
+        opc = {"V": 0, "K": 1, "kelvin": 2}[op]
+        tri_idx = line_idx if tri_idx is None else tri_idx
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        ntest = np_ if test_basis else ne
+        rows = np.arange(ntest, dtype=np.int32) if rows is None else \
+            np.ascontiguousarray(rows, dtype=np.int32)
+        nc = 3 if opc == 2 else 1
+        y = np.zeros(nc * len(rows))
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        rc = self.L.orc_apply(opc, test_basis, trial_basis, np_, nodes, ne,
+                              elems, line_idx, tri_idx, nu, x, len(rows), rows,
+                              y, nthreads)
+        if rc:
+            raise RuntimeError(rc)
+        return y.reshape(nc, len(rows)) if nc == 3 else y
+
+    def pair_block(self, op, nodes, elems, i, j, line_idx=4, tri_idx=None,
+                   use_tables=True):
+        opc = {"V": 0, "K": 1, "kelvin": 2}[op]
+        tri_idx = line_idx if tri_idx is None else tri_idx
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        out = np.zeros(63)
+        oi = np.zeros(3, np.int32); oj = np.zeros(3, np.int32)
+        cls = self.L.orc_pair_block(opc, np_, nodes, ne, elems, i, j, line_idx,
+                                    tri_idx, int(use_tables), out, oi, oj)
+        return cls, out, oi, oj
+
+    def ss_full(self, cls, kind, exps, geom, n):
+        return self.L.orc_ss_full(cls, kind, np.asarray(exps, np.int32),
+                                  np.ascontiguousarray(geom, np.float64), n)
+
+    def ss_table(self, cls, kdeg, nq):
+        nreg = {1: 2, 2: 5, 3: 3}[cls]
+        nout = {1: nq ** 3, 2: nq ** 2, 3: nq}[cls]
+        W = np.zeros(9 * nreg * nout)
+        self.L.orc_ss_table(cls, kdeg, nq, W)
+        return W.reshape(nreg, nout, 3, 3)
+
+
+class Reference:
+    """The reference's own CPU operator, compiled from its sources."""
+
+    def __init__(self, path: str = REFLIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        L.ebref_laplace3d.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int,
+                                      _dp, _dp, _dp, _dp]
+        L.ebref_laplace3d_mass.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp]
+        L.ebref_elasticity3d.argtypes = [C.c_int, _dp, C.c_int, _ip,
+                                         C.c_double, C.c_int, _dp, C.c_void_p]
+        L.ebref_line_rule.argtypes = [C.c_int, _dp, _dp]
+        L.ebref_tri_rule.argtypes = [C.c_int, _dp, _dp]
+        self.L = L
+
+    @staticmethod
+    def available(path: str = REFLIB) -> bool:
+        return os.path.exists(path)
+
+    def laplace3d(self, nodes, elems, order):
+        """(V[test,trial], K[test,node], D, M[test,node]) like getLaplace3d."""
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        V = np.zeros(ne * ne); K = np.zeros(ne * np_); D = np.zeros(np_ * np_)
+        M = np.zeros(ne * np_)
+        self.L.ebref_laplace3d(np_, nodes, ne, elems, order, V, K, D, M)
+        # V is written row-major [test*ne+trial]; K, D, M column-major
+        return (V.reshape(ne, ne), K.reshape(np_, ne).T.copy(),
+                D.reshape(np_, np_).T.copy(), M.reshape(np_, ne).T.copy())
+
+    def mass(self, nodes, elems):
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        M = np.zeros(ne * np_)
+        self.L.ebref_laplace3d_mass(np_, nodes, ne, elems, M)
+        return M.reshape(np_, ne).T.copy()
+
+    def elasticity3d_V(self, nodes, elems, nu, order):
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        V = np.zeros(9 * ne * ne)
+        self.L.ebref_elasticity3d(np_, nodes, ne, elems, nu, order, V, None)
+        return V.reshape(3 * ne, 3 * ne).T.copy()  # column-major -> [row, col]
+
+    def line_rule(self, idx):
+        p = np.zeros(128); w = np.zeros(128)
+        n = self.L.ebref_line_rule(idx, p, w)
+        return p[:n].copy(), w[:n].copy()
+
+    def tri_rule(self, idx):
+        p = np.zeros(256); w = np.zeros(128)
+        n = self.L.ebref_tri_rule(idx, p, w)
+        return p[:2 * n].reshape(n, 2).copy(), w[:n].copy()

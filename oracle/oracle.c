@@ -1,0 +1,1142 @@
+/* CPU oracle for the matrix-free layer-potential apply -- TEST INFRASTRUCTURE.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this file's library (oracle/liboracle.so); the product never does.
+ *
+ * What pins each part (tests named in tests/test_oracle_*.py):
+ *   quadrature tables     equality with the reference tables exported by the
+ *                         compiled reference (oracle/_ref) and the golden
+ *                         fixture tests/golden/quadrature.npz, plus exactness
+ *                         on monomials up to each rule's degree
+ *   P0 V, P0xP1 K, M      entrywise against the compiled reference's dense
+ *                         getLaplace3d / getLaplace3dMass and the golden
+ *                         fixtures (<= 1e-13 relative), plus Gauss's identity
+ *                         K*1 = -1/2 M*1 and the unit-sphere eigenvalue 1/5
+ *   P0 Kelvin V           entrywise against getElasticity3d and fixtures
+ *   Sauter-Schwab maps    polynomial moments against closed forms
+ *   P1 blocks (tables)    block sums against the reference reduced formulas,
+ *                         full 4-D integration of the maps at high order, and
+ *                         symmetry of the assembled P1 single layer
+ * No function of this file is left "parity unpinned".
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle.h"
+
+#define ORC_PI 3.14159265358979323846
+#define MAXQ 64
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Sizes of the reference's Gauss-Legendre table (src/quadrature.cpp:286-287). */
+static const int kLineSizes[19] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12,
+                                   16, 20, 24, 32, 40, 48, 64};
+
+/* Gauss-Legendre on [0,1] by Newton iteration on P_n (long double). */
+int orc_gauss01(int n, double *pts, double *wts)
+{
+  if (n < 1 || n > 128) return -1;
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    long double x = cosl((long double)ORC_PI * (i + 0.75L) / (n + 0.5L));
+    long double dp = 1.0L;
+    for (int it = 0; it < 100; it++) {
+      long double p0 = 1.0L, p1 = x;
+      for (int k = 2; k <= n; k++) {
+        long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) { p1 = x; p0 = 1.0L; }
+      dp = n * (x * p1 - p0) / (x * x - 1.0L);
+      long double dx = p1 / dp;
+      x -= dx;
+      if (fabsl(dx) < 1e-19L) break;
+    }
+    {
+      long double p0 = 1.0L, p1 = x;
+      for (int k = 2; k <= n; k++) {
+        long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) { p1 = x; p0 = 1.0L; }
+      dp = n * (x * p1 - p0) / (x * x - 1.0L);
+    }
+    long double w = 2.0L / ((1.0L - x * x) * dp * dp);
+    pts[i] = (double)((1.0L - x) / 2.0L);
+    pts[n - 1 - i] = (double)((1.0L + x) / 2.0L);
+    wts[i] = wts[n - 1 - i] = (double)(w / 2.0L);
+  }
+  if (n % 2 == 1) pts[n / 2] = 0.5;
+  return n;
+}
+
+int orc_line_rule(int idx, double *pts, double *wts)
+{
+  if (idx < 0 || idx >= 19) return -1;
+  return orc_gauss01(kLineSizes[idx], pts, wts);
+}
+
+/* Published symmetric triangle rules on {x,y >= 0, x+y <= 1}, weights
+ * summing to 1/2, indexed like the reference table
+ * (src/quadrature.cpp:1047-1048): index d-1 has polynomial degree d.
+ * 1, 3, 4 points: Strang & Fix; 6, 12, 13 points: Dunavant (1985);
+ * 7 points: Radon's degree-5 rule in closed form. */
+static int push3(double *p, double *w, int n, double a, double wt)
+{ /* orbit (a, a), (a, 1-2a), (1-2a, a) */
+  double b = 1.0 - 2.0 * a;
+  p[2 * n] = a; p[2 * n + 1] = a; w[n++] = wt;
+  p[2 * n] = a; p[2 * n + 1] = b; w[n++] = wt;
+  p[2 * n] = b; p[2 * n + 1] = a; w[n++] = wt;
+  return n;
+}
+static int push6(double *p, double *w, int n, double a, double b, double wt)
+{ /* all permutations of barycentrics (a, b, 1-a-b) */
+  double c = 1.0 - a - b;
+  double q[6][2] = {{a, b}, {b, c}, {c, a}, {b, a}, {c, b}, {a, c}};
+  for (int k = 0; k < 6; k++) {
+    p[2 * n] = q[k][0]; p[2 * n + 1] = q[k][1]; w[n++] = wt;
+  }
+  return n;
+}
+
+int orc_tri_rule(int idx, double *p, double *w)
+{
+  int n = 0;
+  switch (idx) {
+  case 0:
+    p[0] = p[1] = 1.0 / 3.0; w[0] = 0.5; return 1;
+  case 1:
+    return push3(p, w, 0, 1.0 / 6.0, 1.0 / 6.0);
+  case 2:
+    p[0] = p[1] = 1.0 / 3.0; w[0] = -27.0 / 96.0; n = 1;
+    return push3(p, w, n, 0.2, 25.0 / 96.0);
+  case 3:
+    n = push3(p, w, 0, 0.445948490915965, 0.223381589678011 / 2);
+    return push3(p, w, n, 0.091576213509771, 0.109951743655322 / 2);
+  case 4: {
+    double s = sqrt(15.0);
+    p[0] = p[1] = 1.0 / 3.0; w[0] = 9.0 / 80.0; n = 1;
+    n = push3(p, w, n, (6.0 - s) / 21.0, (155.0 - s) / 2400.0);
+    return push3(p, w, n, (6.0 + s) / 21.0, (155.0 + s) / 2400.0);
+  }
+  case 5:
+    n = push3(p, w, 0, 0.249286745170910, 0.116786275726379 / 2);
+    n = push3(p, w, n, 0.063089014491502, 0.050844906370207 / 2);
+    return push6(p, w, n, 0.053145049844817, 0.310352451033784,
+                 0.082851075618374 / 2);
+  case 6:
+    p[0] = p[1] = 1.0 / 3.0; w[0] = -0.149570044467682 / 2; n = 1;
+    n = push3(p, w, n, 0.260345966079040, 0.175615257433208 / 2);
+    n = push3(p, w, n, 0.065130102902216, 0.053347235608838 / 2);
+    return push6(p, w, n, 0.048690315425316, 0.312865496004874,
+                 0.077113760890257 / 2);
+  default:
+    return -1;
+  }
+}
+
+typedef struct { int n; double p[MAXQ], w[MAXQ]; } line_rule;
+typedef struct { int n; double x[MAXQ], y[MAXQ], w[MAXQ]; } tri_rule;
+
+static int load_line(int idx, line_rule *L)
+{
+  L->n = orc_line_rule(idx, L->p, L->w);
+  return L->n > 0 ? 0 : -1;
+}
+static int load_tri(int idx, tri_rule *T)
+{
+  double p[2 * MAXQ];
+  T->n = orc_tri_rule(idx, p, T->w);
+  if (T->n <= 0) return -1;
+  for (int i = 0; i < T->n; i++) { T->x[i] = p[2 * i]; T->y[i] = p[2 * i + 1]; }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Small vector helpers                                                     */
+/* ------------------------------------------------------------------------ */
+static inline double dot3(const double *a, const double *b)
+{ return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3(const double *a, const double *b, double *c)
+{
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double ncross(const double *a, const double *b)
+{ double c[3]; cross3(a, b, c); return sqrt(dot3(c, c)); }
+static inline void sub3(const double *a, const double *b, double *c)
+{ c[0] = a[0] - b[0]; c[1] = a[1] - b[1]; c[2] = a[2] - b[2]; }
+/* d = a*p + b*q + c*r + e*s */
+static inline void lin4(double a, const double *p, double b, const double *q,
+                        double c, const double *r, double e, const double *s,
+                        double *d)
+{
+  for (int k = 0; k < 3; k++) d[k] = a * p[k] + b * q[k] + c * r[k] + e * s[k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference integrators restated (FP64)                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Coincident single layer, 1-D reduced: src/BEM3dLaplace.cpp:22-52.
+ * u = P2-P1, v = P3-P2 of the one triangle. */
+static double sl_id(const double *u, const double *v, const line_rule *L)
+{
+  double c[3]; cross3(u, v, c);
+  double jsq = dot3(c, c), s = 0.0, d[3];
+  for (int q = 0; q < L->n; q++) {
+    double e = L->p[q], t = 0.0;
+    for (int k = 0; k < 3; k++) d[k] = e * u[k] + v[k];
+    t += 1.0 / sqrt(dot3(d, d));
+    for (int k = 0; k < 3; k++) d[k] = e * v[k] + u[k];
+    t += 1.0 / sqrt(dot3(d, d));
+    for (int k = 0; k < 3; k++) d[k] = e * (u[k] + v[k]) - v[k];
+    t += 1.0 / sqrt(dot3(d, d));
+    s += L->w[q] * t;
+  }
+  return jsq * s / ORC_PI / 12.0;
+}
+
+/* The five edge-adjacent difference vectors, in the reference's order
+ * (src/BEM3dLaplace.cpp:76-99): the first four carry a factor eta2. */
+static void edge_dvec(int m, double e2, double e3, const double *u,
+                      const double *v, const double *w, double *d)
+{
+  for (int k = 0; k < 3; k++) {
+    double uk = u[k], vk = v[k], wk = w[k];
+    switch (m) {
+    case 0: d[k] = e2 * (e3 * (uk + wk) - wk) + vk; break;
+    case 1: d[k] = -e2 * (uk + vk + e3 * wk) + vk; break;
+    case 2: d[k] = e2 * (vk - e3 * (uk + vk)) - wk; break;
+    case 3: d[k] = -e2 * (wk + e3 * (uk + vk)) + vk; break;
+    default: d[k] = e2 * (uk + wk) + e3 * vk - wk; break;
+    }
+  }
+}
+
+/* Edge-adjacent single layer, 2-D reduced: src/BEM3dLaplace.cpp:55-102. */
+static double sl_edge(const double *u, const double *v, const double *w,
+                      const line_rule *L)
+{
+  double J = ncross(u, v) * ncross(u, w), s = 0.0, d[3];
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++) {
+      double e2 = L->p[a], e3 = L->p[b], t = 0.0;
+      for (int m = 0; m < 4; m++) {
+        edge_dvec(m, e2, e3, u, v, w, d);
+        t += 1.0 / sqrt(dot3(d, d));
+      }
+      t *= e2;
+      edge_dvec(4, e2, e3, u, v, w, d);
+      t += 1.0 / sqrt(dot3(d, d));
+      s += L->w[a] * L->w[b] * t;
+    }
+  return J * s / ORC_PI / 24.0;
+}
+
+/* Vertex-adjacent single layer, 3-D reduced: src/BEM3dLaplace.cpp:105-148. */
+static double sl_vertex(const double *u, const double *v, const double *w,
+                        const double *z, const line_rule *L)
+{
+  double J = ncross(u, v) * ncross(w, z), s = 0.0, d[3];
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++)
+      for (int c = 0; c < L->n; c++) {
+        double e1 = L->p[a], e2 = L->p[b], e3 = L->p[c], t;
+        lin4(1.0, u, e1, v, -e2, w, -e2 * e3, z, d);
+        t = 1.0 / sqrt(dot3(d, d));
+        lin4(1.0, w, e1, z, -e2, u, -e2 * e3, v, d);
+        t += 1.0 / sqrt(dot3(d, d));
+        s += L->w[a] * L->w[b] * L->w[c] * e2 * t;
+      }
+  return J * s / ORC_PI / 12.0;
+}
+
+/* Disjoint single layer, tensor triangle rule: src/BEM3dLaplace.cpp:151-192. */
+static double sl_far(const double *A, const double *u, const double *v,
+                     const double *B, const double *w, const double *z,
+                     const tri_rule *T)
+{
+  double J = ncross(u, v) * ncross(w, z), s = 0.0;
+  for (int a = 0; a < T->n; a++) {
+    double x[3];
+    for (int k = 0; k < 3; k++) x[k] = A[k] + T->x[a] * u[k] + T->y[a] * v[k];
+    for (int b = 0; b < T->n; b++) {
+      double d[3];
+      for (int k = 0; k < 3; k++)
+        d[k] = x[k] - B[k] - T->x[b] * w[k] - T->y[b] * z[k];
+      s += T->w[a] * T->w[b] / sqrt(dot3(d, d));
+    }
+  }
+  return J * s / ORC_PI / 4.0;
+}
+
+/* Edge-adjacent double layer (P0 test x P1 trial): src/BEM3dLaplace.cpp:195-309.
+ * The radial variable uses the 2-point Gauss rule and eta1 is the midpoint. */
+static void dl_edge(const double *u, const double *v, const double *w,
+                    const line_rule *L, const line_rule *L2, double *K)
+{
+  double c[3], n[3];
+  cross3(u, w, c);
+  double JT = sqrt(dot3(c, c));
+  for (int k = 0; k < 3; k++) n[k] = c[k] / JT;
+  double J = ncross(u, v) * JT;
+  const double e1 = 0.5;
+  double I[3] = {0, 0, 0}, d[3];
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++) {
+      double e2 = L->p[a], e3 = L->p[b];
+      double e12 = e1 * e2, e123 = e12 * e3;
+      double t[3] = {0, 0, 0};
+      for (int m = 0; m < 5; m++) {
+        edge_dvec(m, e2, e3, u, v, w, d);
+        double r = sqrt(dot3(d, d));
+        double H = dot3(d, n) / (r * r * r);
+        if (m == 4) { t[0] *= e2; t[1] *= e2; t[2] *= e2; }
+        for (int g = 0; g < 2; g++) {
+          double P = L2->p[g], W = L2->w[g] * P;
+          double f0, f1, f2;
+          switch (m) {
+          case 0: f0 = 1.0 - P * (1.0 - e123); f1 = P * (1.0 - e12);
+                  f2 = P * e12 * (1.0 - e3); break;
+          case 1: f0 = 1.0 - P; f1 = P * (1.0 - e123); f2 = P * e123; break;
+          case 2: f0 = 1.0 - P; f1 = P * (1.0 - e1); f2 = P * e1; break;
+          case 3: f0 = 1.0 - P; f1 = P * (1.0 - e12); f2 = P * e12; break;
+          default: f0 = 1.0 - P * (1.0 - e12); f1 = P * (1.0 - e1);
+                   f2 = P * e1 * (1.0 - e2); break;
+          }
+          t[0] += W * f0 * H; t[1] += W * f1 * H; t[2] += W * f2 * H;
+        }
+      }
+      for (int k = 0; k < 3; k++) I[k] += L->w[a] * L->w[b] * t[k];
+    }
+  for (int k = 0; k < 3; k++) K[k] = J * I[k] / 4.0 / ORC_PI;
+}
+
+/* Vertex-adjacent double layer: src/BEM3dLaplace.cpp:312-383. */
+static void dl_vertex(const double *u, const double *v, const double *w,
+                      const double *z, const line_rule *L,
+                      const line_rule *L2, double *K)
+{
+  double c[3], n[3];
+  cross3(w, z, c);
+  double JT = sqrt(dot3(c, c));
+  for (int k = 0; k < 3; k++) n[k] = c[k] / JT;
+  double J = ncross(u, v) * JT;
+  double I[3] = {0, 0, 0}, d[3];
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++)
+      for (int cc = 0; cc < L->n; cc++) {
+        double e1 = L->p[a], e2 = L->p[b], e3 = L->p[cc], t[3] = {0, 0, 0};
+        for (int m = 0; m < 2; m++) {
+          if (m == 0) lin4(1.0, u, e1, v, -e2, w, -e2 * e3, z, d);
+          else lin4(e2, u, e2 * e3, v, -1.0, w, -e1, z, d);
+          double r = sqrt(dot3(d, d));
+          double H = dot3(d, n) / (r * r * r);
+          for (int g = 0; g < 2; g++) {
+            double P = L2->p[g], W = L2->w[g] * P;
+            double f0, f1, f2;
+            if (m == 0) { f0 = 1.0 - P * e2; f1 = P * e2 * (1.0 - e3);
+                          f2 = P * e2 * e3; }
+            else { f0 = 1.0 - P; f1 = P * (1.0 - e1); f2 = P * e1; }
+            t[0] += W * f0 * H; t[1] += W * f1 * H; t[2] += W * f2 * H;
+          }
+        }
+        double ww = L->w[a] * L->w[b] * L->w[cc] * e2;
+        for (int k = 0; k < 3; k++) I[k] += ww * t[k];
+      }
+  for (int k = 0; k < 3; k++) K[k] = J * I[k] / 4.0 / ORC_PI;
+}
+
+/* Disjoint double layer: src/BEM3dLaplace.cpp:386-437. */
+static void dl_far(const double *A, const double *u, const double *v,
+                   const double *B, const double *w, const double *z,
+                   const tri_rule *T, double *K)
+{
+  double c[3], n[3];
+  cross3(w, z, c);
+  double JT = sqrt(dot3(c, c));
+  for (int k = 0; k < 3; k++) n[k] = c[k] / JT;
+  double J = ncross(u, v) * JT, I[3] = {0, 0, 0};
+  for (int a = 0; a < T->n; a++) {
+    double x[3];
+    for (int k = 0; k < 3; k++) x[k] = A[k] + T->x[a] * u[k] + T->y[a] * v[k];
+    for (int b = 0; b < T->n; b++) {
+      double d[3], e1 = T->x[b], e2 = T->y[b];
+      for (int k = 0; k < 3; k++) d[k] = x[k] - B[k] - e1 * w[k] - e2 * z[k];
+      double r = sqrt(dot3(d, d));
+      double H = T->w[a] * T->w[b] * dot3(d, n) / (r * r * r);
+      I[0] += (1.0 - e1 - e2) * H; I[1] += e1 * H; I[2] += e2 * H;
+    }
+  }
+  for (int k = 0; k < 3; k++) K[k] = J * I[k] / ORC_PI / 4.0;
+}
+
+/* Kelvin integrands at difference vector d: I, I11, I22, I33, I12, I13, I23
+ * (src/BEM3dElasticity.cpp:46-83). */
+static inline void kelvin_terms(const double *d, double s, double *I)
+{
+  double r = sqrt(dot3(d, d)), r3 = r * r * r;
+  I[0] += s / r;
+  I[1] += s * d[0] * d[0] / r3; I[2] += s * d[1] * d[1] / r3;
+  I[3] += s * d[2] * d[2] / r3; I[4] += s * d[0] * d[1] / r3;
+  I[5] += s * d[0] * d[2] / r3; I[6] += s * d[1] * d[2] / r3;
+}
+
+/* Kelvin reductions: src/BEM3dElasticity.cpp:30-100 (coincident),
+ * :103-207 (edge), :210-286 (vertex), :289-348 (disjoint). */
+static void ke_id(const double *u, const double *v, const line_rule *L,
+                  double *I)
+{
+  double c[3]; cross3(u, v, c);
+  double jsq = dot3(c, c), d[3];
+  memset(I, 0, 7 * sizeof(double));
+  for (int q = 0; q < L->n; q++) {
+    double e = L->p[q];
+    for (int k = 0; k < 3; k++) d[k] = e * u[k] + v[k];
+    kelvin_terms(d, L->w[q], I);
+    for (int k = 0; k < 3; k++) d[k] = e * v[k] + u[k];
+    kelvin_terms(d, L->w[q], I);
+    for (int k = 0; k < 3; k++) d[k] = e * (u[k] + v[k]) - v[k];
+    kelvin_terms(d, L->w[q], I);
+  }
+  for (int k = 0; k < 7; k++) I[k] *= jsq / ORC_PI / 12.0;
+}
+static void ke_edge(const double *u, const double *v, const double *w,
+                    const line_rule *L, double *I)
+{
+  double J = ncross(u, v) * ncross(u, w), d[3];
+  memset(I, 0, 7 * sizeof(double));
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++) {
+      double e2 = L->p[a], e3 = L->p[b], ww = L->w[a] * L->w[b];
+      for (int m = 0; m < 5; m++) {
+        edge_dvec(m, e2, e3, u, v, w, d);
+        kelvin_terms(d, m < 4 ? ww * e2 : ww, I);
+      }
+    }
+  for (int k = 0; k < 7; k++) I[k] *= J / ORC_PI / 24.0;
+}
+static void ke_vertex(const double *u, const double *v, const double *w,
+                      const double *z, const line_rule *L, double *I)
+{
+  double J = ncross(u, v) * ncross(w, z), d[3];
+  memset(I, 0, 7 * sizeof(double));
+  for (int a = 0; a < L->n; a++)
+    for (int b = 0; b < L->n; b++)
+      for (int c = 0; c < L->n; c++) {
+        double e1 = L->p[a], e2 = L->p[b], e3 = L->p[c];
+        double ww = L->w[a] * L->w[b] * L->w[c] * e2;
+        lin4(1.0, u, e1, v, -e2, w, -e2 * e3, z, d);
+        kelvin_terms(d, ww, I);
+        lin4(1.0, w, e1, z, -e2, u, -e2 * e3, v, d);
+        kelvin_terms(d, ww, I);
+      }
+  for (int k = 0; k < 7; k++) I[k] *= J / ORC_PI / 12.0;
+}
+static void ke_far(const double *A, const double *u, const double *v,
+                   const double *B, const double *w, const double *z,
+                   const tri_rule *T, double *I)
+{
+  double J = ncross(u, v) * ncross(w, z);
+  memset(I, 0, 7 * sizeof(double));
+  for (int a = 0; a < T->n; a++) {
+    double x[3];
+    for (int k = 0; k < 3; k++) x[k] = A[k] + T->x[a] * u[k] + T->y[a] * v[k];
+    for (int b = 0; b < T->n; b++) {
+      double d[3];
+      for (int k = 0; k < 3; k++)
+        d[k] = x[k] - B[k] - T->x[b] * w[k] - T->y[b] * z[k];
+      kelvin_terms(d, T->w[a] * T->w[b], I);
+    }
+  }
+  for (int k = 0; k < 7; k++) I[k] *= J / ORC_PI / 4.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pair classification and local geometry (src/BEM3dLaplace.cpp:602-765)   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int cls;            /* 3 coincident, 2 edge, 1 vertex, 0 disjoint */
+  int oi[3], oj[3];   /* local vertex orders (global 0-based node ids)    */
+  double A[3], B[3], u[3], v[3], w[3], z[3];
+  int kscatter[3];    /* reference K scatter targets (0-based nodes)      */
+} pair_geom;
+
+static void pair_setup(const double *nodes, const int *elems, int i, int j,
+                       pair_geom *g)
+{
+  const int *fi = elems + 3 * i, *fj = elems + 3 * j;
+  double Pi[3][3], Pj[3][3];
+  for (int k = 0; k < 3; k++) {
+    memcpy(Pi[k], nodes + 3 * (fi[k] - 1), 3 * sizeof(double));
+    memcpy(Pj[k], nodes + 3 * (fj[k] - 1), 3 * sizeof(double));
+  }
+  if (i == j) {
+    g->cls = 3;
+    for (int k = 0; k < 3; k++) {
+      g->oi[k] = g->oj[k] = fi[k] - 1;
+      g->u[k] = Pi[1][k] - Pi[0][k];
+      g->v[k] = Pi[2][k] - Pi[1][k];
+    }
+    return;
+  }
+  int ci[2], cj[2], cnt = 0;
+  for (int k = 0; k < 3; k++)
+    for (int l = 0; l < 3; l++)
+      if (fj[k] == fi[l]) {
+        if (cnt < 2) { ci[cnt] = l; cj[cnt] = k; }
+        cnt++;
+        break;
+      }
+  if (cnt == 2 && cj[0] == 0 && cj[1] == 2) { /* orient the shared edge */
+    int t = cj[0]; cj[0] = cj[1]; cj[1] = t;
+    t = ci[0]; ci[0] = ci[1]; ci[1] = t;
+  }
+  if (cnt == 0) {
+    g->cls = 0;
+    for (int k = 0; k < 3; k++) {
+      g->A[k] = Pi[0][k]; g->B[k] = Pj[0][k];
+      g->u[k] = Pi[1][k] - Pi[0][k]; g->v[k] = Pi[2][k] - Pi[0][k];
+      g->w[k] = Pj[1][k] - Pj[0][k]; g->z[k] = Pj[2][k] - Pj[0][k];
+      g->oi[k] = fi[k] - 1; g->oj[k] = fj[k] - 1;
+      g->kscatter[k] = fj[k] - 1;
+    }
+  } else if (cnt == 1) {
+    int c = ci[0], d = cj[0];
+    g->cls = 1;
+    for (int k = 0; k < 3; k++) {
+      g->u[k] = Pi[(c + 1) % 3][k] - Pi[c][k];
+      g->v[k] = Pi[(c + 2) % 3][k] - Pi[(c + 1) % 3][k];
+      g->w[k] = Pj[(d + 1) % 3][k] - Pj[d][k];
+      g->z[k] = Pj[(d + 2) % 3][k] - Pj[(d + 1) % 3][k];
+    }
+    for (int k = 0; k < 3; k++) {
+      g->oi[k] = fi[(c + k) % 3] - 1;
+      g->oj[k] = fj[(d + k) % 3] - 1;
+      g->kscatter[k] = g->oj[k];
+    }
+  } else {
+    int remi = 3 - ci[0] - ci[1], remj = 3 - cj[0] - cj[1];
+    g->cls = 2;
+    for (int k = 0; k < 3; k++) {
+      g->u[k] = Pj[cj[1]][k] - Pj[cj[0]][k];
+      g->v[k] = Pi[remi][k] - Pj[cj[1]][k];
+      g->w[k] = Pj[remj][k] - Pj[cj[1]][k];
+    }
+    g->oi[0] = fi[ci[0]] - 1; g->oi[1] = fi[ci[1]] - 1; g->oi[2] = fi[remi] - 1;
+    g->oj[0] = fj[cj[0]] - 1; g->oj[1] = fj[cj[1]] - 1; g->oj[2] = fj[remj] - 1;
+    for (int k = 0; k < 3; k++) g->kscatter[k] = g->oj[k];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sauter-Schwab maps (reference triangle T = {0 <= s2 <= s1 <= 1})        */
+/* ------------------------------------------------------------------------ */
+/* Vertex-adjacent (2 regions) and edge-adjacent (5 regions): the maps of
+ * Sauter & Schwab, "Boundary Element Methods", sec. 5.2.1, written so that
+ * (x - y)/h reproduces the reference difference vectors above.
+ * Coincident: z = s - t runs over the hexagon T - T along six rays
+ * z = rho*zhat_m(eta); for fixed z the inner point s ranges over the triangle
+ * T n (T + z), whose area is (1-rho)^2/2, parameterised by barycentric
+ * (l1, l2) on the unit simplex.
+ * ss_map returns s, t, the Jacobian and the homogeneity factor h with
+ * x - y = h * d_m(outer). */
+static void ss_map(int cls, int m, const double *in, const double *out,
+                   double *s, double *t, double *jac, double *h)
+{
+  if (cls == 1) { /* vertex: inner xi; outer e1, e2, e3 */
+    double xi = in[0], e1 = out[0], e2 = out[1], e3 = out[2];
+    double a0 = xi, a1 = xi * e1, b0 = xi * e2, b1 = xi * e2 * e3;
+    if (m == 0) { s[0] = a0; s[1] = a1; t[0] = b0; t[1] = b1; }
+    else { s[0] = b0; s[1] = b1; t[0] = a0; t[1] = a1; }
+    *jac = xi * xi * xi * e2;
+    *h = xi;
+  } else if (cls == 2) { /* edge: inner xi, e1; outer e2, e3 */
+    double xi = in[0], e1 = in[1], e2 = out[0], e3 = out[1];
+    switch (m) {
+    case 0: /* difference vector 0: eta2(eta3(u+w)-w)+v */
+      s[0] = xi; s[1] = xi * e1;
+      t[0] = xi * (1 - e1 * e2 * e3); t[1] = xi * e1 * e2 * (1 - e3); break;
+    case 1: /* -eta2(u+v+eta3 w)+v */
+      s[0] = xi * (1 - e1 * e2); s[1] = xi * e1 * (1 - e2);
+      t[0] = xi; t[1] = xi * e1 * e2 * e3; break;
+    case 2: /* eta2(v-eta3(u+v))-w */
+      s[0] = xi * (1 - e1 * e2 * e3); s[1] = xi * e1 * e2 * (1 - e3);
+      t[0] = xi; t[1] = xi * e1; break;
+    case 3: /* -eta2(w+eta3(u+v))+v */
+      s[0] = xi * (1 - e1 * e2 * e3); s[1] = xi * e1 * (1 - e2 * e3);
+      t[0] = xi; t[1] = xi * e1 * e2; break;
+    default: /* eta2(u+w)+eta3 v-w, no eta2 factor */
+      s[0] = xi; s[1] = xi * e1 * e3;
+      t[0] = xi * (1 - e1 * e2); t[1] = xi * e1 * (1 - e2); break;
+    }
+    *jac = xi * xi * xi * e1 * e1 * (m < 4 ? e2 : 1.0);
+    *h = xi * e1;
+  } else { /* coincident: inner rho, l1, l2; outer eta */
+    double rho = in[0], l1 = in[1], l2 = in[2], eta = out[0];
+    double zh[2];
+    switch (m % 3) {
+    case 0: zh[0] = eta; zh[1] = 1.0; break;
+    case 1: zh[0] = 1.0; zh[1] = eta; break;
+    default: zh[0] = eta; zh[1] = eta - 1.0; break;
+    }
+    if (m >= 3) { zh[0] = -zh[0]; zh[1] = -zh[1]; }
+    double z1 = rho * zh[0], z2 = rho * zh[1];
+    double a = z2 > 0 ? z2 : 0.0;
+    double b = 1.0 + (z1 < 0 ? z1 : 0.0);
+    double c = (z1 - z2) > 0 ? (z1 - z2) : 0.0;
+    double V1[2] = {a + c, a}, V2[2] = {b, a}, V3[2] = {b, b - c};
+    s[0] = V1[0] + l1 * (V2[0] - V1[0]) + l2 * (V3[0] - V1[0]);
+    s[1] = V1[1] + l1 * (V2[1] - V1[1]) + l2 * (V3[1] - V1[1]);
+    t[0] = s[0] - z1;
+    t[1] = s[1] - z2;
+    double side = b - a - c;
+    *jac = rho * side * side;
+    *h = rho;
+  }
+}
+
+/* P1 shape functions on the Sauter triangle, vertex order (a0, a1, a2). */
+static inline void sauter_phi(const double *s, double *f)
+{ f[0] = 1.0 - s[0]; f[1] = s[0] - s[1]; f[2] = s[1]; }
+
+static int ss_nregions(int cls) { return cls == 1 ? 2 : cls == 2 ? 5 : 3; }
+static int ss_nouter(int cls, int nq)
+{ return cls == 1 ? nq * nq * nq : cls == 2 ? nq * nq : nq; }
+
+/* Reduced weight tables: for each region m and outer point q,
+ * W[(m*nout + q)*9 + 3k + l] = sum over the inner variables of
+ * jac / h^deg * phi_k(s) phi_l(t), integrated exactly (inner polynomial
+ * degree <= 5 is integrated by 4-point Gauss and a degree-4 simplex rule).
+ * The coincident table folds the opposite rays m and m+3 together since
+ * every kernel used is even in x - y there (the double layer vanishes). */
+int orc_ss_table(int cls, int kdeg, int nq, double *W)
+{
+  line_rule L, G;
+  double gp[8], gw[8];
+  if (orc_gauss01(nq, L.p, L.w) < 0) return -1;
+  L.n = nq;
+  orc_gauss01(4, gp, gw);
+  memcpy(G.p, gp, sizeof(gp)); memcpy(G.w, gw, sizeof(gw)); G.n = 4;
+  tri_rule S;
+  load_tri(3, &S);
+  int nreg = ss_nregions(cls), nout = ss_nouter(cls, nq);
+  memset(W, 0, sizeof(double) * 9 * nreg * nout);
+  for (int q = 0; q < nout; q++) {
+    double out[3];
+    if (cls == 1) {
+      out[0] = L.p[q / (nq * nq)]; out[1] = L.p[(q / nq) % nq]; out[2] = L.p[q % nq];
+    } else if (cls == 2) {
+      out[0] = L.p[q / nq]; out[1] = L.p[q % nq];
+    } else out[0] = L.p[q];
+    int mreg = cls == 3 || cls == 0 ? 6 : nreg;
+    for (int m = 0; m < mreg; m++) {
+      double *Wm = W + ((m % nreg) * nout + q) * 9;
+      if (cls == 1) {
+        for (int g = 0; g < G.n; g++) {
+          double in[1] = {G.p[g]}, s[2], t[2], jac, h, fs[3], ft[3];
+          ss_map(cls, m, in, out, s, t, &jac, &h);
+          sauter_phi(s, fs); sauter_phi(t, ft);
+          double c = G.w[g] * jac / pow(h, kdeg);
+          for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) Wm[3 * k + l] += c * fs[k] * ft[l];
+        }
+      } else if (cls == 2) {
+        for (int g = 0; g < G.n; g++)
+          for (int g2 = 0; g2 < G.n; g2++) {
+            double in[2] = {G.p[g], G.p[g2]}, s[2], t[2], jac, h, fs[3], ft[3];
+            ss_map(cls, m, in, out, s, t, &jac, &h);
+            sauter_phi(s, fs); sauter_phi(t, ft);
+            double c = G.w[g] * G.w[g2] * jac / pow(h, kdeg);
+            for (int k = 0; k < 3; k++)
+              for (int l = 0; l < 3; l++) Wm[3 * k + l] += c * fs[k] * ft[l];
+          }
+      } else {
+        for (int g = 0; g < G.n; g++)
+          for (int r = 0; r < S.n; r++) {
+            double in[3] = {G.p[g], S.x[r], S.y[r]}, s[2], t[2], jac, h;
+            double fs[3], ft[3];
+            ss_map(3, m, in, out, s, t, &jac, &h);
+            sauter_phi(s, fs); sauter_phi(t, ft);
+            double c = G.w[g] * S.w[r] * jac / pow(h, kdeg);
+            for (int k = 0; k < 3; k++)
+              for (int l = 0; l < 3; l++) Wm[3 * k + l] += c * fs[k] * ft[l];
+          }
+      }
+    }
+  }
+  return nreg * nout;
+}
+
+/* Difference vector d_m at outer point q (same formulas as the reference
+ * reductions; coincident and vertex-adjacent region 1 keep the x - y sign). */
+static void ss_dvec(int cls, int m, const double *out, const pair_geom *g,
+                    double *d)
+{
+  if (cls == 1) {
+    double e1 = out[0], e2 = out[1], e3 = out[2];
+    if (m == 0) lin4(1.0, g->u, e1, g->v, -e2, g->w, -e2 * e3, g->z, d);
+    else lin4(e2, g->u, e2 * e3, g->v, -1.0, g->w, -e1, g->z, d);
+  } else if (cls == 2) {
+    edge_dvec(m, out[0], out[1], g->u, g->v, g->w, d);
+  } else {
+    double e = out[0];
+    for (int k = 0; k < 3; k++) {
+      double uk = g->u[k], vk = g->v[k];
+      d[k] = m == 0 ? e * uk + vk : m == 1 ? uk + e * vk : e * (uk + vk) - vk;
+    }
+  }
+}
+
+/* Table-driven near-field pair block.  op 0: SL (9), op 1: DL (9),
+ * op 2: Kelvin (63). */
+static void near_block_tables(int op, const pair_geom *g, int nq,
+                              const double *W, const double *WD, double *out)
+{
+  int cls = g->cls == 3 ? 3 : g->cls == 2 ? 2 : 1;
+  int nreg = ss_nregions(cls), nout = ss_nouter(cls, nq);
+  double gp[MAXQ], gw[MAXQ];
+  orc_gauss01(nq, gp, gw);
+  double J, n[3] = {0, 0, 0}, c[3];
+  if (cls == 3) { J = ncross(g->u, g->v); J = J * J; }
+  else if (cls == 2) {
+    cross3(g->u, g->w, c);
+    double jt = sqrt(dot3(c, c));
+    for (int k = 0; k < 3; k++) n[k] = c[k] / jt;
+    J = ncross(g->u, g->v) * jt;
+  } else {
+    cross3(g->w, g->z, c);
+    double jt = sqrt(dot3(c, c));
+    for (int k = 0; k < 3; k++) n[k] = c[k] / jt;
+    J = ncross(g->u, g->v) * jt;
+  }
+  int nval = op == 2 ? 63 : 9;
+  memset(out, 0, sizeof(double) * nval);
+  if (op == 1 && cls == 3) return; /* (x-y).n_y = 0 on a flat panel */
+  for (int q = 0; q < nout; q++) {
+    double o[3], wq;
+    if (cls == 1) {
+      int a = q / (nq * nq), b = (q / nq) % nq, cc = q % nq;
+      o[0] = gp[a]; o[1] = gp[b]; o[2] = gp[cc]; wq = gw[a] * gw[b] * gw[cc];
+    } else if (cls == 2) {
+      int a = q / nq, b = q % nq;
+      o[0] = gp[a]; o[1] = gp[b]; wq = gw[a] * gw[b];
+    } else { o[0] = gp[q]; wq = gw[q]; }
+    for (int m = 0; m < nreg; m++) {
+      double d[3];
+      ss_dvec(cls, m, o, g, d);
+      double r = sqrt(dot3(d, d));
+      if (op == 0) {
+        const double *Wm = W + (m * nout + q) * 9;
+        double kv = wq / r;
+        for (int e = 0; e < 9; e++) out[e] += Wm[e] * kv;
+      } else if (op == 1) {
+        const double *Wm = WD + (m * nout + q) * 9;
+        double kv = wq * dot3(d, n) / (r * r * r);
+        for (int e = 0; e < 9; e++) out[e] += Wm[e] * kv;
+      } else {
+        const double *Wm = W + (m * nout + q) * 9;
+        double kv[7] = {0, 0, 0, 0, 0, 0, 0};
+        kelvin_terms(d, wq, kv);
+        for (int e = 0; e < 9; e++)
+          for (int cc = 0; cc < 7; cc++) out[7 * e + cc] += Wm[e] * kv[cc];
+      }
+    }
+  }
+  for (int e = 0; e < nval; e++) out[e] *= J / (4.0 * ORC_PI);
+}
+
+/* Far-field pair block with P1 shape functions (1 - x1 - x2, x1, x2). */
+static void far_block(int op, const pair_geom *g, const tri_rule *T,
+                      double *out)
+{
+  double c[3], n[3];
+  cross3(g->w, g->z, c);
+  double jt = sqrt(dot3(c, c));
+  for (int k = 0; k < 3; k++) n[k] = c[k] / jt;
+  double J = ncross(g->u, g->v) * jt;
+  int nval = op == 2 ? 63 : 9;
+  memset(out, 0, sizeof(double) * nval);
+  for (int a = 0; a < T->n; a++) {
+    double x[3], fa[3] = {1.0 - T->x[a] - T->y[a], T->x[a], T->y[a]};
+    for (int k = 0; k < 3; k++)
+      x[k] = g->A[k] + T->x[a] * g->u[k] + T->y[a] * g->v[k];
+    for (int b = 0; b < T->n; b++) {
+      double d[3], e1 = T->x[b], e2 = T->y[b];
+      double fb[3] = {1.0 - e1 - e2, e1, e2};
+      for (int k = 0; k < 3; k++)
+        d[k] = x[k] - g->B[k] - e1 * g->w[k] - e2 * g->z[k];
+      double r = sqrt(dot3(d, d)), ww = T->w[a] * T->w[b];
+      if (op == 0) {
+        double kv = ww / r;
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++) out[3 * k + l] += fa[k] * fb[l] * kv;
+      } else if (op == 1) {
+        double kv = ww * dot3(d, n) / (r * r * r);
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++) out[3 * k + l] += fa[k] * fb[l] * kv;
+      } else {
+        double kv[7] = {0, 0, 0, 0, 0, 0, 0};
+        kelvin_terms(d, ww, kv);
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++)
+            for (int cc = 0; cc < 7; cc++)
+              out[7 * (3 * k + l) + cc] += fa[k] * fb[l] * kv[cc];
+      }
+    }
+  }
+  for (int e = 0; e < nval; e++) out[e] *= J / (4.0 * ORC_PI);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shared context for assembly and applies                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  line_rule L, L2;
+  tri_rule T;
+  int nq;
+  double *W[4], *WD[4]; /* indexed by cls 1..3 */
+} ctx_t;
+
+static int ctx_init(ctx_t *c, int line_idx, int tri_idx, int tables)
+{
+  memset(c, 0, sizeof(*c));
+  if (load_line(line_idx, &c->L) || load_line(1, &c->L2) ||
+      load_tri(tri_idx, &c->T))
+    return -1;
+  c->nq = c->L.n;
+  if (tables) {
+    for (int cls = 1; cls <= 3; cls++) {
+      size_t sz = 9 * (size_t)ss_nregions(cls) * ss_nouter(cls, c->nq);
+      c->W[cls] = (double *)malloc(sz * sizeof(double));
+      c->WD[cls] = (double *)malloc(sz * sizeof(double));
+      orc_ss_table(cls, 1, c->nq, c->W[cls]);
+      orc_ss_table(cls, 2, c->nq, c->WD[cls]);
+    }
+  }
+  return 0;
+}
+static void ctx_free(ctx_t *c)
+{
+  for (int k = 0; k < 4; k++) { free(c->W[k]); free(c->WD[k]); }
+}
+
+/* Block of a pair in pair-local orders via the tables (near) or the tensor
+ * rule (far). */
+static void pair_block(const ctx_t *c, int op, const pair_geom *g, double *out)
+{
+  if (g->cls == 0) far_block(op, g, &c->T, out);
+  else {
+    int cls = g->cls == 3 ? 3 : g->cls == 2 ? 2 : 1;
+    near_block_tables(op, g, c->nq, c->W[cls], c->WD[cls], out);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dense P0 assembly restating getLaplace3d / getElasticity3d                */
+/* ------------------------------------------------------------------------ */
+int orc_laplace_dense(int np, const double *nodes, int ne, const int *elems,
+                      int line_idx, int tri_idx, double *V, double *K,
+                      double *M)
+{
+  ctx_t c;
+  if (ctx_init(&c, line_idx, tri_idx, 0)) return -1;
+  if (K) memset(K, 0, sizeof(double) * (size_t)ne * np);
+  if (M) memset(M, 0, sizeof(double) * (size_t)ne * np);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < ne; i++) {
+    const int *fi = elems + 3 * i;
+    if (M) {
+      double a[3], b[3];
+      sub3(nodes + 3 * (fi[1] - 1), nodes + 3 * (fi[0] - 1), a);
+      sub3(nodes + 3 * (fi[2] - 1), nodes + 3 * (fi[0] - 1), b);
+      double Ji = ncross(a, b);
+      for (int k = 0; k < 3; k++) M[(size_t)i * np + fi[k] - 1] += Ji / 6.0;
+    }
+    for (int j = 0; j < ne; j++) {
+      pair_geom g;
+      pair_setup(nodes, elems, i, j, &g);
+      double v = 0.0, k3[3] = {0, 0, 0};
+      switch (g.cls) {
+      case 3: v = sl_id(g.u, g.v, &c.L); break;
+      case 0:
+        v = sl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c.T);
+        if (K) dl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c.T, k3);
+        break;
+      case 1:
+        v = sl_vertex(g.u, g.v, g.w, g.z, &c.L);
+        if (K) dl_vertex(g.u, g.v, g.w, g.z, &c.L, &c.L2, k3);
+        break;
+      default:
+        v = sl_edge(g.u, g.v, g.w, &c.L);
+        if (K) dl_edge(g.u, g.v, g.w, &c.L, &c.L2, k3);
+        break;
+      }
+      if (V) V[(size_t)i * ne + j] = v;
+      if (K && g.cls != 3)
+        for (int k = 0; k < 3; k++) K[(size_t)i * np + g.kscatter[k]] += k3[k];
+    }
+  }
+  ctx_free(&c);
+  return 0;
+}
+
+static void kelvin_p0(const ctx_t *c, const pair_geom *g, double *I)
+{
+  switch (g->cls) {
+  case 3: ke_id(g->u, g->v, &c->L, I); break;
+  case 0: ke_far(g->A, g->u, g->v, g->B, g->w, g->z, &c->T, I); break;
+  case 1: ke_vertex(g->u, g->v, g->w, g->z, &c->L, I); break;
+  default: ke_edge(g->u, g->v, g->w, &c->L, I); break;
+  }
+}
+
+/* Kelvin block from the 7 integrals (src/BEM3dElasticity.cpp:612,838-846). */
+static void kelvin_block(const double *I, double nu, double *B /* 3x3 */)
+{
+  double f = (1.0 + nu) / 2.0 / (1.0 - nu), t = 3.0 - 4.0 * nu;
+  B[0] = f * (t * I[0] + I[1]); B[4] = f * (t * I[0] + I[2]);
+  B[8] = f * (t * I[0] + I[3]);
+  B[1] = B[3] = f * I[4]; B[2] = B[6] = f * I[5]; B[5] = B[7] = f * I[6];
+}
+
+int orc_kelvin_dense(int np, const double *nodes, int ne, const int *elems,
+                     double nu, int line_idx, int tri_idx, double *V)
+{
+  (void)np;
+  ctx_t c;
+  if (ctx_init(&c, line_idx, tri_idx, 0)) return -1;
+  size_t n3 = 3 * (size_t)ne;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < ne; j++) {
+      pair_geom g;
+      double I[7], B[9];
+      pair_setup(nodes, elems, i, j, &g);
+      kelvin_p0(&c, &g, I);
+      kelvin_block(I, nu, B);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          V[(a * (size_t)ne + i) * n3 + b * (size_t)ne + j] = B[3 * a + b];
+    }
+  ctx_free(&c);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Row-sampled matrix-free applies                                          */
+/* ------------------------------------------------------------------------ */
+static int local_index(const int *fi, int node0)
+{
+  for (int k = 0; k < 3; k++) if (fi[k] - 1 == node0) return k;
+  return -1;
+}
+
+/* Contribution of test triangle i against every trial triangle: out[3*nc]
+ * indexed by i's own local vertex k (P1 test) or out[c] (P0 test, k=0),
+ * nc = 1 (Laplace) or 3 (Kelvin components). */
+static void tri_row(const ctx_t *c, int op, int test_basis, int trial_basis,
+                    int np, const double *nodes, int ne, const int *elems,
+                    double nu, const double *x, int i, double *out)
+{
+  const int *fi = elems + 3 * i;
+  int nc = op == 2 ? 3 : 1;
+  int ntrial = trial_basis ? np : ne;
+  memset(out, 0, sizeof(double) * 3 * nc);
+  for (int j = 0; j < ne; j++) {
+    pair_geom g;
+    pair_setup(nodes, elems, i, j, &g);
+    const int *fj = elems + 3 * j;
+    if (op == 0 && test_basis == 0 && trial_basis == 0) {
+      /* reference P0 single layer */
+      double v;
+      switch (g.cls) {
+      case 3: v = sl_id(g.u, g.v, &c->L); break;
+      case 0: v = sl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c->T); break;
+      case 1: v = sl_vertex(g.u, g.v, g.w, g.z, &c->L); break;
+      default: v = sl_edge(g.u, g.v, g.w, &c->L); break;
+      }
+      out[0] += v * x[j];
+      continue;
+    }
+    if (op == 1 && test_basis == 0) {
+      /* reference P0-test x P1-trial double layer */
+      double k3[3] = {0, 0, 0};
+      switch (g.cls) {
+      case 3: break;
+      case 0: dl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c->T, k3); break;
+      case 1: dl_vertex(g.u, g.v, g.w, g.z, &c->L, &c->L2, k3); break;
+      default: dl_edge(g.u, g.v, g.w, &c->L, &c->L2, k3); break;
+      }
+      if (g.cls != 3)
+        for (int k = 0; k < 3; k++) out[0] += k3[k] * x[g.kscatter[k]];
+      continue;
+    }
+    if (op == 2 && test_basis == 0 && trial_basis == 0) {
+      /* reference P0 Kelvin */
+      double I[7], B[9];
+      kelvin_p0(c, &g, I);
+      kelvin_block(I, nu, B);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          out[a] += B[3 * a + b] * x[(size_t)b * ne + j];
+      continue;
+    }
+    /* P1 test and/or trial via the pair block */
+    double blk[63];
+    pair_block(c, op, &g, blk);
+    for (int kk = 0; kk < 3; kk++) {
+      int ki = test_basis ? local_index(fi, g.oi[kk]) : 0;
+      for (int ll = 0; ll < 3; ll++) {
+        int col = trial_basis ? g.oj[ll] : j;
+        if (op == 2) {
+          double I[7], B[9];
+          for (int cc = 0; cc < 7; cc++) I[cc] = blk[7 * (3 * kk + ll) + cc];
+          kelvin_block(I, nu, B);
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+              out[3 * ki + a] += B[3 * a + b] * x[(size_t)b * ntrial + col];
+        } else {
+          out[3 * ki] += blk[3 * kk + ll] * x[col];
+        }
+      }
+    }
+    (void)fj;
+  }
+}
+
+int orc_apply(int op, int test_basis, int trial_basis, int np,
+              const double *nodes, int ne, const int *elems, int line_idx,
+              int tri_idx, double nu, const double *x, int nrows,
+              const int *rows, double *y, int nthreads)
+{
+  if (op == 1 && trial_basis != 1) return -2;
+  if (op == 2 && test_basis != trial_basis) return -2;
+  ctx_t c;
+  if (ctx_init(&c, line_idx, tri_idx, 1)) return -1;
+  int nc = op == 2 ? 3 : 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  if (!test_basis) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < nrows; r++) {
+      double out[9];
+      tri_row(&c, op, test_basis, trial_basis, np, nodes, ne, elems, nu, x,
+              rows[r], out);
+      for (int a = 0; a < nc; a++) y[(size_t)a * nrows + r] = out[a];
+    }
+  } else {
+    /* node rows: sum the node's share over the triangles containing it */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < nrows; r++) {
+      int n0 = rows[r];
+      double acc[3] = {0, 0, 0};
+      for (int i = 0; i < ne; i++) {
+        int k = local_index(elems + 3 * i, n0);
+        if (k < 0) continue;
+        double out[9];
+        tri_row(&c, op, test_basis, trial_basis, np, nodes, ne, elems, nu, x,
+                i, out);
+        for (int a = 0; a < nc; a++) acc[a] += out[3 * k + a];
+      }
+      for (int a = 0; a < nc; a++) y[(size_t)a * nrows + r] = acc[a];
+    }
+  }
+  ctx_free(&c);
+  return 0;
+}
+
+int orc_pair_block(int op, int np, const double *nodes, int ne,
+                   const int *elems, int i, int j, int line_idx, int tri_idx,
+                   int use_tables, double *out, int *order_i, int *order_j)
+{
+  (void)np; (void)ne;
+  ctx_t c;
+  if (ctx_init(&c, line_idx, tri_idx, use_tables)) return -1;
+  pair_geom g;
+  pair_setup(nodes, elems, i, j, &g);
+  for (int k = 0; k < 3; k++) { order_i[k] = g.oi[k]; order_j[k] = g.oj[k]; }
+  if (use_tables) pair_block(&c, op, &g, out);
+  else {
+    memset(out, 0, sizeof(double) * (op == 2 ? 63 : 9));
+    if (op == 0) {
+      switch (g.cls) {
+      case 3: out[0] = sl_id(g.u, g.v, &c.L); break;
+      case 0: out[0] = sl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c.T); break;
+      case 1: out[0] = sl_vertex(g.u, g.v, g.w, g.z, &c.L); break;
+      default: out[0] = sl_edge(g.u, g.v, g.w, &c.L); break;
+      }
+    } else if (op == 1) {
+      switch (g.cls) {
+      case 3: break;
+      case 0: dl_far(g.A, g.u, g.v, g.B, g.w, g.z, &c.T, out); break;
+      case 1: dl_vertex(g.u, g.v, g.w, g.z, &c.L, &c.L2, out); break;
+      default: dl_edge(g.u, g.v, g.w, &c.L, &c.L2, out); break;
+      }
+    } else kelvin_p0(&c, &g, out);
+  }
+  ctx_free(&c);
+  return g.cls;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Full 4-D integration of the maps (test only)                             */
+/* ------------------------------------------------------------------------ */
+double orc_ss_full(int cls, int kind, const int *e, const double *geom, int n)
+{
+  double gp[128], gw[128];
+  if (orc_gauss01(n, gp, gw) < 0) return NAN;
+  int nreg = cls == 0 ? 6 : cls == 1 ? 5 : 2;
+  int mcls = cls == 0 ? 3 : cls == 1 ? 2 : 1;
+  const double *u = geom, *v = geom + 3, *w = geom + 6, *z = geom + 9;
+  double total = 0.0;
+  for (int m = 0; m < nreg; m++)
+    for (int a = 0; a < n; a++)
+      for (int b = 0; b < n; b++)
+        for (int c = 0; c < n; c++)
+          for (int d = 0; d < n; d++) {
+            double in[3], out[3], wt = gw[a] * gw[b] * gw[c] * gw[d], s[2],
+                   t[2], jac, h;
+            if (mcls == 1) {
+              in[0] = gp[a]; out[0] = gp[b]; out[1] = gp[c]; out[2] = gp[d];
+            } else if (mcls == 2) {
+              in[0] = gp[a]; in[1] = gp[b]; out[0] = gp[c]; out[1] = gp[d];
+            } else { /* conical product on the simplex for (l1, l2) */
+              in[0] = gp[a]; in[1] = gp[b]; in[2] = gp[c] * (1.0 - gp[b]);
+              wt *= (1.0 - gp[b]);
+              out[0] = gp[d];
+            }
+            ss_map(mcls, m, in, out, s, t, &jac, &h);
+            double f;
+            if (kind == 0) {
+              f = pow(s[0], e[0]) * pow(s[1], e[1]) * pow(t[0], e[2]) *
+                  pow(t[1], e[3]);
+            } else {
+              double dx[3], fs[3], ft[3];
+              for (int k = 0; k < 3; k++) {
+                if (mcls == 1)
+                  dx[k] = s[0] * u[k] + s[1] * v[k] - t[0] * w[k] - t[1] * z[k];
+                else if (mcls == 2)
+                  dx[k] = (s[0] - t[0]) * u[k] + s[1] * v[k] - t[1] * w[k];
+                else
+                  dx[k] = (s[0] - t[0]) * u[k] + (s[1] - t[1]) * v[k];
+              }
+              sauter_phi(s, fs); sauter_phi(t, ft);
+              f = fs[e[0]] * ft[e[1]] / sqrt(dot3(dx, dx));
+            }
+            total += wt * jac * f;
+          }
+  return total;
+}
