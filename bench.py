@@ -1,0 +1,182 @@
+"""Benchmark: Laplace Dirichlet solves on one MI355X.
+
+Workload (one step = one solve): icosphere subdivided 5 times (20480
+triangles, 10242 vertices), P1 single layer V and double layer K, interior
+Dirichlet data u = 1/|x - (0,0,2)|, right-hand side b = (1/2 M + K) u,
+solved for the P1 Neumann trace with GMRES(50) by mixed-precision iterative
+refinement (FP32 far field inside, FP64 outer residual) to a relative FP64
+residual of 1e-8.  Near field: 5-point Gauss Sauter-Schwab; far field:
+7-point triangle rule.
+
+Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
+build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
+PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
+FLOP_SL = 11               # per far interaction, single layer, one RHS
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(P, T, n_matvec, seconds=10.0):
+    """The oracle (FP64 C restatement, OpenMP) on sampled P1 rows of the same
+    operator, extrapolated to full matvecs x the FP64 solve's matvec count."""
+    from oracle import Oracle
+    o = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, P.shape[0])
+    nrows, elapsed = 16, 0.0
+    while True:
+        rows = rng.choice(P.shape[0], size=nrows, replace=False)
+        t0 = time.perf_counter()
+        o.apply("V", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4,
+                nthreads=threads)
+        elapsed = time.perf_counter() - t0
+        if elapsed > seconds / 4 or nrows >= P.shape[0]:
+            break
+        nrows = min(P.shape[0], int(nrows * max(2.0, seconds / 4 / max(elapsed, 1e-3))))
+    t_matvec = elapsed / nrows * P.shape[0]
+    return {"value": 1.0 / (t_matvec * n_matvec), "unit": "solves/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle P1 V apply on {nrows} random node rows "
+                      f"({elapsed:.1f} s), extrapolated to a full matvec "
+                      f"({t_matvec:.1f} s) x {n_matvec} FP64 matvecs per solve"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must "
+                          "be 1 (multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    from easybem_amd import _lib as L, meshes
+
+    P, T = meshes.icosphere(args.level)
+    x0 = np.array([0.0, 0.0, 2.0])
+    u = 1.0 / np.linalg.norm(P - x0, axis=1)
+    mesh = L.Mesh(P, T)
+    V64 = L.Operator(mesh, "V", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    V32 = L.Operator(mesh, "V", 1, 1, "fp32", line_rule=4, tri_rule=4)
+    K64 = L.Operator(mesh, "K", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    b = 0.5 * mesh.mass(u, 1) + K64(u)
+    db = L.DeviceArray.from_host(b)
+    dx = L.DeviceArray(b.size)
+    zeros = np.zeros(b.size)
+    solver = L.Solver(V64, V32, restart=50)
+
+    def solve(mixed=True):
+        L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))
+        return solver.solve_device(db, dx, tol=1e-8, mixed=mixed,
+                                   max_iters=3000, inner_tol=1e-5)
+
+    for _ in range(args.warmup):
+        solve()
+    L.check(L.lib().ebh_synchronize())
+    L.op_profile(V32, True, 1 << 15)
+    lat, stats = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        st = solve()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        stats.append(st)
+    L.check(L.lib().ebh_synchronize())
+    wall = time.perf_counter() - t0
+    far_ms, far_launches = L.op_profile(V32, False)
+    if not all(s["converged"] and s["relres"] <= 1e-8 for s in stats):
+        log("solve did not converge:", stats)
+        sys.exit(1)
+    solves_per_s = args.steps / wall
+
+    # matvec throughput of both far-field precisions
+    xv = L.DeviceArray.from_host(np.random.default_rng(1).uniform(-1, 1, b.size))
+    yv = L.DeviceArray(b.size)
+    w = V32.work()
+    t32 = V32.time(xv, yv, reps=5)
+    t64 = V64.time(xv, yv, reps=3)
+    far_flop = FLOP_SL * w["far_interactions"]
+
+    out_fp64 = {}
+    if not args.no_fp64_compare:
+        solve(mixed=False)
+        n64 = 2
+        ts = time.perf_counter()
+        for _ in range(n64):
+            st64 = solve(mixed=False)
+        t64s = (time.perf_counter() - ts) / n64
+        out_fp64 = {"fp64_solves_per_s": 1.0 / t64s,
+                    "fp64_iters": st64["iters"],
+                    "mixed_speedup_vs_fp64": solves_per_s * t64s}
+    else:
+        st64 = {"matvec64": stats[-1]["iters"]}
+
+    avg_far_ms = far_ms / max(far_launches, 1)
+    achieved = far_flop / (avg_far_ms * 1e-3) / 1e12
+    lat_sorted = np.sort(lat)
+    res = {
+        "metric": "solves/sec to 1e-8 FP64 residual (GMRES(50), mixed-precision "
+                  "iterative refinement)",
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 far field / f64 near field, Krylov and residual",
+        "data": "synthetic",
+        "config": {"workload": f"icosphere L{args.level} P1 Laplace Dirichlet "
+                               "solve, u=1/|x-(0,0,2)|, b=(M/2+K)u, "
+                               "GMRES(50) mixed IR to 1e-8",
+                   "triangles": int(T.shape[0]), "vertices": int(P.shape[0]),
+                   "near_rule": "5-point Gauss (line index 4)",
+                   "far_rule": "7-point triangle (index 4)"},
+        "p50_ms": float(np.percentile(lat_sorted, 50)),
+        "p99_ms": float(np.percentile(lat_sorted, 99)),
+        "iters": stats[-1]["iters"], "outer": stats[-1]["outer"],
+        "matvec32_per_solve": stats[-1]["matvec32"],
+        "matvec64_per_solve": stats[-1]["matvec64"],
+        **out_fp64,
+        "matvec_ms": {"fp32": t32["total"], "fp64": t64["total"]},
+        "matvec_far_tflops": {"fp32": far_flop / (t32["far"] * 1e-3) / 1e12,
+                              "fp64": far_flop / (t64["far"] * 1e-3) / 1e12},
+        "roofline": {"kernel": "k_far<float, single layer, 7 points>",
+                     "bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_launch": far_flop,
+                     "avg_launch_ms": avg_far_ms, "launches": far_launches},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(P, T, st64.get("matvec64", 1))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

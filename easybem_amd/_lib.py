@@ -1,0 +1,308 @@
+"""ctypes binding of the C-ABI in include/easybem_hip.h.
+
+The HIP library is required: importing this module raises if
+libeasybem_hip.so has not been built (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libeasybem_hip.so")
+
+KERNELS = {"V": 0, "K": 1, "kelvin": 2}
+EBH_OK = 0
+
+
+class EbhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"easybem-hip error {code}: {msg}")
+        self.code = code
+
+
+class OpDesc(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("test_basis", C.c_int),
+                ("trial_basis", C.c_int), ("far_precision", C.c_int),
+                ("line_rule", C.c_int), ("tri_rule", C.c_int),
+                ("nu", C.c_double), ("max_rhs", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(
+                f"{LIBPATH} is missing: build it with `python -m easybem_amd.build`")
+        L = C.CDLL(LIBPATH)
+        vp, i64p = C.c_void_p, C.POINTER(C.c_int64)
+        L.ebh_last_error.restype = C.c_char_p
+        L.ebh_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.ebh_free.argtypes = [vp]
+        L.ebh_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
+        L.ebh_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
+        L.ebh_memset.argtypes = [vp, C.c_int, C.c_size_t]
+        L.ebh_device_info.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_size_t)]
+        L.ebh_mesh_create.argtypes = [C.c_int, vp, C.c_int, vp,
+                                      C.POINTER(C.c_void_p)]
+        L.ebh_mesh_info.argtypes = [vp, i64p]
+        L.ebh_mesh_destroy.argtypes = [vp]
+        L.ebh_mesh_destroy.restype = None
+        L.ebh_op_create.argtypes = [vp, C.POINTER(OpDesc), C.POINTER(C.c_void_p)]
+        L.ebh_op_dims.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ebh_op_apply.argtypes = [vp, vp, vp, C.c_int, vp]
+        L.ebh_op_apply_host.argtypes = [vp, vp, vp, C.c_int]
+        L.ebh_op_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.ebh_op_get_launch.argtypes = [vp, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
+                                  C.POINTER(C.c_double)]
+        L.ebh_op_work.argtypes = [vp, i64p]
+        L.ebh_op_destroy.argtypes = [vp]
+        L.ebh_op_destroy.restype = None
+        L.ebh_mass_apply.argtypes = [vp, C.c_int, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != EBH_OK:
+        raise EbhError(rc, lib().ebh_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A float64 device buffer (ebh_malloc) with host copies."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        check(lib().ebh_malloc(C.byref(p), max(1, self.n) * 8))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(a.size)
+        check(lib().ebh_memcpy_h2d(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n)
+        check(lib().ebh_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ebh_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_info():
+    n, cu, fr, tot = C.c_int(), C.c_int(), C.c_size_t(), C.c_size_t()
+    check(lib().ebh_device_info(C.byref(n), C.byref(cu), C.byref(fr),
+                                C.byref(tot)))
+    return {"devices": n.value, "cu": cu.value, "free": fr.value,
+            "total": tot.value}
+
+
+class Mesh:
+    """Device-resident mesh (ebh_mesh).  elements are 1-based int32."""
+
+    def __init__(self, nodes, elements):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.elements = np.ascontiguousarray(elements, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().ebh_mesh_create(self.nodes.shape[0], _ptr(self.nodes),
+                                    self.elements.shape[0], _ptr(self.elements),
+                                    C.byref(h)))
+        self.h = h
+
+    def info(self):
+        a = (C.c_int64 * 8)()
+        check(lib().ebh_mesh_info(self.h, a))
+        keys = ["np", "ne", "coincident", "edge", "vertex", "far",
+                "max_touch", "bytes"]
+        return dict(zip(keys, list(a)))
+
+    def mass(self, u, test_basis=0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        n = self.nodes.shape[0] if test_basis else self.elements.shape[0]
+        y = np.empty(n)
+        check(lib().ebh_mass_apply(self.h, test_basis, _ptr(u), _ptr(y)))
+        return y
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebh_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    """Matrix-free layer-potential operator (ebh_op) on a Mesh."""
+
+    def __init__(self, mesh: Mesh, kernel="V", test_basis=0, trial_basis=0,
+                 precision="fp64", line_rule=4, tri_rule=None, nu=0.3,
+                 max_rhs=1):
+        self.mesh = mesh
+        d = OpDesc(KERNELS[kernel], int(test_basis), int(trial_basis),
+                   0 if precision == "fp64" else 1, int(line_rule),
+                   int(line_rule if tri_rule is None else tri_rule), float(nu),
+                   int(max_rhs))
+        self.desc = d
+        h = C.c_void_p()
+        check(lib().ebh_op_create(mesh.h, C.byref(d), C.byref(h)))
+        self.h = h
+        r, c = C.c_int(), C.c_int()
+        check(lib().ebh_op_dims(h, C.byref(r), C.byref(c)))
+        self.nrows, self.ncols = r.value, c.value
+
+    def apply_host(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nrhs = x.size // self.ncols
+        y = np.empty(nrhs * self.nrows)
+        check(lib().ebh_op_apply_host(self.h, _ptr(x), _ptr(y), nrhs))
+        return y if nrhs > 1 or x.ndim == 1 else y
+
+    __call__ = apply_host
+
+    def apply(self, x: DeviceArray, y: DeviceArray, nrhs=1, stream=None):
+        check(lib().ebh_op_apply(self.h, x.ptr, y.ptr, nrhs, stream))
+
+    def set_launch(self, block=0, tile=0, chunks=0):
+        check(lib().ebh_op_set_launch(self.h, block, tile, chunks))
+
+    def launch(self):
+        b, t, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().ebh_op_get_launch(self.h, C.byref(b), C.byref(t),
+                                      C.byref(c)))
+        return b.value, t.value, c.value
+
+    def time(self, x: DeviceArray, y: DeviceArray, reps=10, nrhs=1):
+        ms = (C.c_double * 4)()
+        check(lib().ebh_op_time(self.h, x.ptr, y.ptr, nrhs, reps, ms))
+        return {"fold": ms[0], "far": ms[1], "near": ms[2], "total": ms[3]}
+
+    def work(self):
+        w = (C.c_int64 * 4)()
+        check(lib().ebh_op_work(self.h, w))
+        return {"far_interactions": w[0], "near_nnz": w[1], "rows": w[2],
+                "cols": w[3]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebh_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("restart", C.c_int), ("max_iters", C.c_int),
+                ("tol", C.c_double), ("mixed", C.c_int),
+                ("inner_tol", C.c_double), ("max_outer", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("matvec64", C.c_int),
+                ("matvec32", C.c_int), ("relres", C.c_double),
+                ("ms", C.c_double), ("converged", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _solver_sigs(L):
+    if getattr(L, "_solver_sigs", False):
+        return
+    vp = C.c_void_p
+    L.ebh_solver_create.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_void_p)]
+    L.ebh_solver_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts),
+                                   C.POINTER(SolveStats)]
+    L.ebh_solver_destroy.argtypes = [vp]
+    L.ebh_solver_destroy.restype = None
+    L._solver_sigs = True
+
+
+class Solver:
+    """GMRES(restart) on device; mixed-precision refinement when a32 given."""
+
+    def __init__(self, a64: Operator, a32: Operator = None, restart=50):
+        L = lib()
+        _solver_sigs(L)
+        h = C.c_void_p()
+        check(L.ebh_solver_create(a64.h, a32.h if a32 else None, restart,
+                                  C.byref(h)))
+        self.h = h
+        self.n = a64.nrows
+        self.a64, self.a32 = a64, a32
+        self.restart = restart
+
+    def solve_device(self, b: DeviceArray, x: DeviceArray, tol=1e-8,
+                     mixed=False, max_iters=2000, inner_tol=1e-5,
+                     max_outer=10, raise_on_noconv=True):
+        o = SolveOpts(self.restart, max_iters, tol, int(mixed), inner_tol,
+                      max_outer)
+        st = SolveStats()
+        rc = lib().ebh_solver_solve(self.h, b.ptr, x.ptr, C.byref(o),
+                                    C.byref(st))
+        if rc != EBH_OK and (raise_on_noconv or rc != -6):
+            check(rc)
+        return st.as_dict()
+
+    def solve(self, b, x0=None, **kw):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        db = DeviceArray.from_host(b)
+        dx = DeviceArray.from_host(np.zeros_like(b) if x0 is None else x0)
+        st = self.solve_device(db, dx, **kw)
+        return dx.to_host(), st
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebh_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def op_profile(op: Operator, enable: bool, max_launches=4096):
+    """Start/stop far-kernel event timing; returns (total_ms, launches)
+    recorded since the previous call."""
+    L = lib()
+    L.ebh_op_profile.argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    ms, n = C.c_double(), C.c_int64()
+    check(L.ebh_op_profile(op.h, int(enable), max_launches, C.byref(ms),
+                           C.byref(n)))
+    return ms.value, n.value

@@ -1,0 +1,117 @@
+// pybind11 module `easybem`: the reference module's GPU entry points
+// (python/bindings.cpp:186-261) and its mass matrix (:42-60), with the same
+// names, arguments and return shapes, over the HIP library.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "BEM3dLaplaceGPU.h"
+
+namespace py = pybind11;
+
+namespace {
+
+using darr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+using iarr = py::array_t<int, py::array::c_style | py::array::forcecast>;
+
+void check_mesh(const darr &nodes, const iarr &elements)
+{
+  if (nodes.ndim() != 2 || nodes.shape(1) != 3)
+    throw std::invalid_argument("nodes must have shape (nNodes, 3)");
+  if (elements.ndim() != 2 || elements.shape(1) != 3)
+    throw std::invalid_argument("elements must have shape (nElements, 3)");
+}
+
+void need_live()
+{
+  if (gpuNumElements() == 0) throw std::runtime_error("call init_gpu first");
+}
+
+py::array_t<double> apply(const darr &x, size_t n_in, size_t n_out,
+                          void (*fn)(const double *, double *, int), int ansatz)
+{
+  if ((size_t)x.size() != n_in)
+    throw std::invalid_argument("vector has " + std::to_string(x.size()) +
+                                " entries, expected " + std::to_string(n_in));
+  py::array_t<double> out(n_out);
+  const double *xp = x.data();
+  double *yp = out.mutable_data();
+  {
+    py::gil_scoped_release nogil;
+    fn(xp, yp, ansatz);
+  }
+  return out;
+}
+
+void v_apply(const double *x, double *y, int ansatz) { applyGPULaplace3d1Layer(x, y, ansatz); }
+void k_apply(const double *x, double *y, int) { applyGPULaplace3d2Layer(x, y); }
+void d_apply(const double *x, double *y, int) { applyGPULaplace3dHyperSing(x, y); }
+
+}  // namespace
+
+PYBIND11_MODULE(easybem, m)
+{
+  m.doc() = "easybem drop-in module over the MI355X HIP layer-potential apply";
+
+  // P0 x P1 mass matrix, (nElements, nNodes), like getLaplace3dMass
+  // (src/BEM3dLaplace.cpp:448-486): each triangle adds J/6 to its nodes.
+  m.def("get_laplace3d_mass", [](darr nodes, iarr elements) {
+    check_mesh(nodes, elements);
+    const ssize_t np = nodes.shape(0), ne = elements.shape(0);
+    py::array_t<double> M({ne, np});
+    auto o = M.mutable_unchecked<2>();
+    for (ssize_t i = 0; i < ne; i++)
+      for (ssize_t j = 0; j < np; j++) o(i, j) = 0.0;
+    auto P = nodes.unchecked<2>();
+    auto T = elements.unchecked<2>();
+    for (ssize_t i = 0; i < ne; i++) {
+      int a = T(i, 0) - 1, b = T(i, 1) - 1, c = T(i, 2) - 1;
+      if (a < 0 || b < 0 || c < 0 || a >= np || b >= np || c >= np)
+        throw std::invalid_argument("element index out of range");
+      double u[3], v[3];
+      for (int k = 0; k < 3; k++) { u[k] = P(b, k) - P(a, k); v[k] = P(c, k) - P(a, k); }
+      double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2],
+             n2 = u[0] * v[1] - u[1] * v[0];
+      double J = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      o(i, a) += J / 6.0; o(i, b) += J / 6.0; o(i, c) += J / 6.0;
+    }
+    return M;
+  });
+
+  m.def("init_gpu", [](darr nodes, iarr elements, int order, int nblocks, int nthreads) {
+    check_mesh(nodes, elements);
+    const double *P = nodes.data();
+    const int *T = elements.data();
+    const int np = (int)nodes.shape(0), ne = (int)elements.shape(0);
+    py::gil_scoped_release nogil;
+    initGPU(np, P, ne, T, order, nblocks, nthreads);
+  }, py::arg("nodes"), py::arg("elements"), py::arg("order"),
+     py::arg("nblocks") = 0, py::arg("nthreads") = 0);
+
+  m.def("close_gpu", []() { closeGPU(); });
+
+  m.def("set_far_precision", [](int fp32) { setGPUFarPrecision(fp32); },
+        "0: FP64 far field (default); 1: FP32 far field");
+
+  m.def("apply_gpu_laplace3d_1layer", [](darr x) {
+    need_live();
+    return apply(x, gpuNumElements(), gpuNumElements(), v_apply, 0);
+  });
+  m.def("apply_gpu_laplace3d_1layer_linears", [](darr x) {
+    need_live();
+    return apply(x, gpuNumNodes(), gpuNumNodes(), v_apply, 1);
+  });
+  m.def("apply_gpu_laplace3d_2layer", [](darr u) {
+    need_live();
+    return apply(u, gpuNumNodes(), gpuNumElements(), k_apply, 0);
+  });
+  m.def("apply_gpu_laplace3d_hypersing", [](darr u) {
+    need_live();
+    return apply(u, gpuNumNodes(), gpuNumNodes(), d_apply, 0);
+  });
+
+  m.def("version", []() { return "1.0"; });
+}

@@ -1,0 +1,359 @@
+// C-ABI entry points (include/easybem_hip.h): error reporting, device memory
+// helpers, mesh and operator handles, mass matrices.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ebh_internal.h"
+
+namespace ebh {
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+const char *get_error() { return g_error.c_str(); }
+}  // namespace ebh
+
+using namespace ebh;
+
+size_t ebh_op::bytes() const
+{
+  return near_ptr.bytes() + near_col.bytes() + near_val.bytes() + tq32.bytes() +
+         sg32.bytes() + tq64.bytes() + sg64.bytes() + tw.bytes() + swt.bytes() +
+         normals.bytes() + src32.bytes() + src64.bytes() + partial.bytes() +
+         tri_out.bytes();
+}
+
+extern "C" {
+
+const char *ebh_last_error(void) { return get_error(); }
+
+int ebh_version(int *major, int *minor)
+{
+  if (major) *major = 1;
+  if (minor) *minor = 0;
+  return EBH_OK;
+}
+
+int ebh_malloc(void **ptr, size_t bytes)
+{
+  if (!ptr) { set_error("null pointer"); return EBH_ERR_ARG; }
+  *ptr = nullptr;
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+    *ptr = nullptr;
+    set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return EBH_ERR_NOMEM;
+  }
+  return EBH_OK;
+}
+int ebh_free(void *ptr)
+{
+  if (ptr) EBH_HIP(hipFree(ptr));
+  return EBH_OK;
+}
+int ebh_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+  EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return EBH_OK;
+}
+int ebh_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+  EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return EBH_OK;
+}
+int ebh_memset(void *dst, int value, size_t bytes)
+{
+  EBH_HIP(hipMemset(dst, value, bytes));
+  return EBH_OK;
+}
+int ebh_synchronize(void)
+{
+  EBH_HIP(hipDeviceSynchronize());
+  return EBH_OK;
+}
+int ebh_device_info(int *n_devices, int *n_cu, size_t *free_bytes,
+                    size_t *total_bytes)
+{
+  int n = 0;
+  EBH_HIP(hipGetDeviceCount(&n));
+  if (n_devices) *n_devices = n;
+  if (n == 0) return EBH_OK;
+  hipDeviceProp_t prop;
+  EBH_HIP(hipGetDeviceProperties(&prop, 0));
+  if (n_cu) *n_cu = prop.multiProcessorCount;
+  size_t fr = 0, tot = 0;
+  EBH_HIP(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = fr;
+  if (total_bytes) *total_bytes = tot;
+  return EBH_OK;
+}
+
+// ------------------------------------------------------------------ mesh
+int ebh_mesh_create(int np, const double *nodes, int ne, const int *elems,
+                    ebh_mesh **out)
+{
+  if (!out || !nodes || !elems) { set_error("null argument"); return EBH_ERR_ARG; }
+  *out = nullptr;
+  ebh_mesh *m = new ebh_mesh();
+  std::string err = build_topology(np, nodes, ne, elems, m->T);
+  if (!err.empty()) {
+    delete m;
+    set_error("mesh rejected: " + err);
+    return EBH_ERR_MESH;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete m;
+    set_error("no HIP device");
+    return EBH_ERR_HIP;
+  }
+  m->device = dev;
+  const Topology &T = m->T;
+  int rc = EBH_OK;
+  if ((rc = m->nodes.upload(T.nodes)) || (rc = m->tri.upload(T.tri)) ||
+      (rc = m->touch_ptr.upload(T.touch_ptr)) || (rc = m->touch.upload(T.touch)) ||
+      (rc = m->node_tri_ptr.upload(T.node_tri_ptr)) ||
+      (rc = m->node_tri.upload(T.node_tri)) ||
+      (rc = m->node_tri_k.upload(T.node_tri_k))) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return EBH_OK;
+}
+
+int ebh_mesh_info(const ebh_mesh *m, int64_t *info)
+{
+  if (!m || !info) { set_error("null argument"); return EBH_ERR_ARG; }
+  info[0] = m->T.np;
+  info[1] = m->T.ne;
+  info[2] = m->T.n_class[kCoincident];
+  info[3] = m->T.n_class[kEdge];
+  info[4] = m->T.n_class[kVertex];
+  info[5] = m->T.n_class[kDisjoint];
+  info[6] = m->T.max_touch;
+  info[7] = (int64_t)m->bytes();
+  return EBH_OK;
+}
+
+void ebh_mesh_destroy(ebh_mesh *m) { delete m; }
+
+// -------------------------------------------------------------- operator
+int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
+{
+  if (!mesh || !desc || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  *out = nullptr;
+  const ebh_op_desc &d = *desc;
+  if (d.kernel < 0 || d.kernel > 2) { set_error("unknown kernel"); return EBH_ERR_ARG; }
+  if ((d.test_basis != 0 && d.test_basis != 1) ||
+      (d.trial_basis != 0 && d.trial_basis != 1)) {
+    set_error("basis must be EBH_P0 or EBH_P1");
+    return EBH_ERR_ARG;
+  }
+  if (d.kernel == EBH_LAPLACE_DL && d.trial_basis != EBH_P1) {
+    set_error("the double layer needs a P1 trial space (as the reference's K)");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_KELVIN_SL && d.test_basis != d.trial_basis) {
+    set_error("Kelvin single layer needs equal test and trial bases");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.far_precision != EBH_FP64 && d.far_precision != EBH_FP32) {
+    set_error("far_precision must be EBH_FP64 or EBH_FP32");
+    return EBH_ERR_ARG;
+  }
+  if (d.kernel == EBH_KELVIN_SL && !(d.nu > -1.0 && d.nu < 0.5)) {
+    set_error("Poisson ratio must lie in (-1, 0.5)");
+    return EBH_ERR_ARG;
+  }
+  const int nq = line_rule_size(d.line_rule);
+  if (nq <= 0) { set_error("line_rule index must be 0..18"); return EBH_ERR_ARG; }
+  std::vector<double> rx, ry, rw;
+  if (!tri_rule(d.tri_rule, rx, ry, rw)) {
+    set_error("tri_rule index must be 0..6");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  ebh_op *op = new ebh_op();
+  op->mesh = mesh;
+  op->d = d;
+  if (op->d.max_rhs < 1) op->d.max_rhs = 1;
+  op->nq = nq;
+  op->rule.nt = (int)rw.size();
+  for (int q = 0; q < op->rule.nt; q++) {
+    op->rule.x[q] = rx[q];
+    op->rule.y[q] = ry[q];
+    op->rule.w[q] = rw[q];
+  }
+  op->nc = d.kernel == EBH_KELVIN_SL ? 3 : 1;
+  op->nrows = d.test_basis ? mesh->T.np : mesh->T.ne;
+  op->ncols = d.trial_basis ? mesh->T.np : mesh->T.ne;
+  int rc = EBH_OK;
+  if (hipStreamCreate(&op->stream) != hipSuccess) {
+    delete op;
+    set_error("hipStreamCreate failed");
+    return EBH_ERR_HIP;
+  }
+  for (auto &e : op->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      ebh_op_destroy(op);
+      set_error("hipEventCreate failed");
+      return EBH_ERR_HIP;
+    }
+  if ((rc = op_build_near(op)) || (rc = op_build_far(op))) {
+    ebh_op_destroy(op);
+    return rc;
+  }
+  *out = op;
+  return EBH_OK;
+}
+
+int ebh_op_dims(const ebh_op *op, int *nrows, int *ncols)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (nrows) *nrows = op->nrows * op->nc;
+  if (ncols) *ncols = op->ncols * op->nc;
+  return EBH_OK;
+}
+
+int ebh_op_apply(ebh_op *op, const double *x, double *y, int nrhs, void *stream)
+{
+  if (!op || !x || !y || nrhs < 1) { set_error("bad apply arguments"); return EBH_ERR_ARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : op->stream;
+  const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
+  for (int r = 0; r < nrhs; r++)
+    EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, s, false));
+  return EBH_OK;
+}
+
+int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs)
+{
+  if (!op || !x || !y || nrhs < 1) { set_error("bad apply arguments"); return EBH_ERR_ARG; }
+  const size_t cx = (size_t)op->ncols * op->nc * nrhs, cy = (size_t)op->nrows * op->nc * nrhs;
+  for (size_t k = 0; k < cx; k++)
+    if (!std::isfinite(x[k])) { set_error("non-finite input vector"); return EBH_ERR_NONFINITE; }
+  DevBuf<double> dx, dy;
+  EBH_TRY(dx.upload(x, cx));
+  EBH_TRY(dy.alloc(cy));
+  EBH_TRY(ebh_op_apply(op, dx.p, dy.p, nrhs, nullptr));
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  EBH_HIP(hipMemcpy(y, dy.p, cy * sizeof(double), hipMemcpyDeviceToHost));
+  return EBH_OK;
+}
+
+int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  return op_configure(op, block, tile, chunks);
+}
+
+int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (block) *block = op->block;
+  if (tile) *tile = op->tile;
+  if (chunks) *chunks = op->chunks;
+  return EBH_OK;
+}
+
+int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
+                double *ms)
+{
+  if (!op || !x || !y || !ms || reps < 1 || nrhs < 1) {
+    set_error("bad timing arguments");
+    return EBH_ERR_ARG;
+  }
+  double acc[4] = {0, 0, 0, 0};
+  const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
+  for (int k = 0; k < reps; k++)
+    for (int r = 0; r < nrhs; r++) {
+      EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, op->stream, true));
+      EBH_HIP(hipEventSynchronize(op->ev[3]));
+      float t01, t12, t23, t03;
+      EBH_HIP(hipEventElapsedTime(&t01, op->ev[0], op->ev[1]));
+      EBH_HIP(hipEventElapsedTime(&t12, op->ev[1], op->ev[2]));
+      EBH_HIP(hipEventElapsedTime(&t23, op->ev[2], op->ev[3]));
+      EBH_HIP(hipEventElapsedTime(&t03, op->ev[0], op->ev[3]));
+      acc[0] += t01; acc[1] += t12; acc[2] += t23; acc[3] += t03;
+    }
+  for (int k = 0; k < 4; k++) ms[k] = acc[k] / reps;
+  return EBH_OK;
+}
+
+int ebh_op_work(const ebh_op *op, int64_t *work)
+{
+  if (!op || !work) { set_error("null argument"); return EBH_ERR_ARG; }
+  const int64_t nt = op->rule.nt;
+  work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt;
+  work[1] = op->near_nnz;
+  work[2] = op->nrows;
+  work[3] = op->ncols;
+  return EBH_OK;
+}
+
+int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
+                   int64_t *launches)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  if (far_ms || launches) {
+    double tot = 0.0;
+    for (int k = 0; k < op->prof_count; k++) {
+      float ms = 0.f;
+      EBH_HIP(hipEventElapsedTime(&ms, op->prof_ev[2 * k], op->prof_ev[2 * k + 1]));
+      tot += ms;
+    }
+    if (far_ms) *far_ms = tot;
+    if (launches) *launches = op->prof_count;
+  }
+  if (enable) {
+    if (max_launches < 1) max_launches = 4096;
+    while ((int)op->prof_ev.size() < 2 * max_launches) {
+      hipEvent_t e;
+      EBH_HIP(hipEventCreate(&e));
+      op->prof_ev.push_back(e);
+    }
+  }
+  op->prof_on = enable != 0;
+  op->prof_count = 0;
+  return EBH_OK;
+}
+
+void ebh_op_destroy(ebh_op *op)
+{
+  if (!op) return;
+  if (op->stream) (void)hipStreamSynchronize(op->stream);
+  for (auto &e : op->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : op->prof_ev) (void)hipEventDestroy(e);
+  if (op->stream) (void)hipStreamDestroy(op->stream);
+  delete op;
+}
+
+// -------------------------------------------------------- mass matrices
+// P0 x P1: each triangle adds J/6 to its three nodes
+// (src/BEM3dLaplace.cpp:464-477).  P1 x P1: the element matrix
+// (J/24) [[2,1,1],[1,2,1],[1,1,2]].
+int ebh_mass_apply(const ebh_mesh *m, int test_basis, const double *u, double *y)
+{
+  if (!m || !u || !y) { set_error("null argument"); return EBH_ERR_ARG; }
+  const Topology &T = m->T;
+  const int rows = test_basis ? T.np : T.ne;
+  std::fill(y, y + rows, 0.0);
+  for (int i = 0; i < T.ne; i++) {
+    const int *f = &T.tri[3 * i];
+    const double *a = &T.nodes[3 * f[0]], *b = &T.nodes[3 * f[1]], *c = &T.nodes[3 * f[2]];
+    double p[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double q[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double n0 = p[1] * q[2] - p[2] * q[1], n1 = p[2] * q[0] - p[0] * q[2],
+           n2 = p[0] * q[1] - p[1] * q[0];
+    double J = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    if (!test_basis) {
+      y[i] += J / 6.0 * (u[f[0]] + u[f[1]] + u[f[2]]);
+    } else {
+      const double s = u[f[0]] + u[f[1]] + u[f[2]];
+      for (int k = 0; k < 3; k++) y[f[k]] += J / 24.0 * (s + u[f[k]]);
+    }
+  }
+  return EBH_OK;
+}
+
+}  // extern "C"

@@ -119,6 +119,10 @@ struct ebh_op {
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // far-kernel profiling over a timed region: event pairs around each launch
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;   // 2 per recorded launch
+  int prof_count = 0;
   size_t bytes() const;
 };
 

@@ -1,0 +1,502 @@
+// Far-field apply: an N-body sum over triangle quadrature points.
+//
+//   y_i = sum_{a in i} W_a phi_k(a) sum_{j not touching i} sum_{b in j}
+//         G(x_a - y_b) sigma_b
+//
+// with W_a = w_a |J_i| and the trial density folded into the source
+// strength sigma_b = w_b |J_j| (density at y_b) / (4 pi) (src/BEM3dLaplace.cpp:
+// 151-192 single layer, :386-437 double layer; src/BEM3dElasticity.cpp:
+// 289-348 Kelvin).  Touching pairs are excluded here and handled by the
+// FP64 near-field CSR.
+//
+// Kernel layout (gfx950, wave64):
+//   * one lane = one target triangle; its NT quadrature points live in
+//     registers, so each source point read from LDS feeds NT interactions;
+//   * a block walks one contiguous chunk of source triangles in LDS tiles of
+//     up to 64 triangles; all lanes read the same LDS record (broadcast);
+//   * touching source triangles are masked per lane with a 64-bit tile mask
+//     built from the sorted touching list (a moving cursor, no inner-loop
+//     memory traffic);
+//   * FP32 variant: per-tile FP32 partial sums, folded into FP64
+//     accumulators after every tile;
+//   * partial row sums of every chunk go to a [chunk][out][target] buffer and
+//     a second kernel reduces them in a fixed order (deterministic, no
+//     atomics).
+// Block b works on chunk b % chunks, so with chunks a multiple of 8 every
+// block of one source chunk runs on the same XCD (blocks are dealt to the 8
+// XCDs round-robin).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "ebh_internal.h"
+
+namespace ebh {
+namespace {
+
+constexpr double kInv4Pi = 0.07957747154594766788;
+
+template <typename R> struct Eps;
+// Added to r^2 so an excluded coincident point (x_a == y_b, strength 0)
+// yields 0 * finite; far below the squared distance of any two distinct
+// quadrature points of non-touching triangles.
+template <> struct Eps<float> { static constexpr float v = 1e-20f; };
+template <> struct Eps<double> { static constexpr double v = 1e-30; };
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+// FP64 reciprocal square root: FP32 hardware estimate (relative error about
+// 2^-23) refined by two Newton-Raphson steps y <- y + y(1 - x y^2)/2, each of
+// which squares the relative error.
+__device__ __forceinline__ double rsq(double x)
+{
+  double y = (double)__builtin_amdgcn_rsqf((float)x);
+  double h = x * y;
+  double e = fma(-h, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  h = x * y;
+  e = fma(-h, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+template <typename R>
+struct FarArgs {
+  int ne, chunks, chunk, tile, nout;  // nout: 1 (P0 test) or 3 (P1 test)
+  R kappa;                            // Kelvin: 3 - 4 nu
+  const R *tq;                        // [nt][3][ne]
+  const R *src;                       // [ne*nt][REC]
+  const int *touch_ptr, *touch;
+  const double *tw;                   // [nt][ne]
+  double *partial;                    // [chunks][nout*nc][ne]
+  double phi[16][3];                  // test shape functions at the points
+};
+
+template <int KIND> struct Rec { static constexpr int n = KIND == 0 ? 4 : 8; };
+
+template <typename R, int KIND, int NT>
+__global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
+{
+  constexpr int REC = Rec<KIND>::n;
+  constexpr int NC = KIND == 2 ? 3 : 1;
+  constexpr bool F32 = sizeof(R) == 4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R *tile = reinterpret_cast<R *>(smem_raw);
+  const int ne = a.ne;
+  const int s = blockIdx.x % a.chunks;
+  const int t = (blockIdx.x / a.chunks) * blockDim.x + threadIdx.x;
+  const int tt = t < ne ? t : ne - 1;
+  const int j0 = s * a.chunk;
+  const int j1 = min(ne, j0 + a.chunk);
+
+  R X[NT], Y[NT], Z[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
+    Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
+    Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+  }
+  double acc64[NT * NC];
+#pragma unroll
+  for (int k = 0; k < NT * NC; k++) acc64[k] = 0.0;
+
+  int cur = a.touch_ptr[tt];
+  const int cend = a.touch_ptr[tt + 1];
+  while (cur < cend && a.touch[cur] < j0) cur++;
+  int nextx = cur < cend ? a.touch[cur] : INT_MAX;
+  const R eps = Eps<R>::v;
+
+  for (int js = j0; js < j1; js += a.tile) {
+    const int nt = min(a.tile, j1 - js);
+    __syncthreads();
+    {
+      const uint4 *g = reinterpret_cast<const uint4 *>(a.src + (size_t)js * NT * REC);
+      uint4 *l = reinterpret_cast<uint4 *>(tile);
+      const int nv = nt * NT * REC * (int)sizeof(R) / 16;
+      for (int k = threadIdx.x; k < nv; k += blockDim.x) l[k] = g[k];
+    }
+    __syncthreads();
+    uint64_t mask = 0;
+    while (nextx < js + nt) {
+      mask |= 1ull << (nextx - js);
+      ++cur;
+      nextx = cur < cend ? a.touch[cur] : INT_MAX;
+    }
+    R acc[NT * NC];
+#pragma unroll
+    for (int k = 0; k < NT * NC; k++) acc[k] = R(0);
+    for (int jj = 0; jj < nt; jj++) {
+      const bool ex = (mask >> jj) & 1ull;
+      const R *rec = tile + jj * NT * REC;
+#pragma unroll
+      for (int b = 0; b < NT; b++, rec += REC) {
+        const R sx = rec[0], sy = rec[1], sz = rec[2];
+        if constexpr (KIND == 0) {
+          const R sg = ex ? R(0) : rec[3];
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
+            const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+            acc[q] = fma(sg, rsq(r2), acc[q]);
+          }
+        } else if constexpr (KIND == 1) {
+          const R px = ex ? R(0) : rec[3], py = ex ? R(0) : rec[4],
+                  pz = ex ? R(0) : rec[5];
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
+            const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+            const R ri = rsq(r2);
+            const R dp = fma(dx, px, fma(dy, py, dz * pz));
+            acc[q] = fma(dp, ri * ri * ri, acc[q]);
+          }
+        } else {
+          const R gx = ex ? R(0) : rec[3], gy = ex ? R(0) : rec[4],
+                  gz = ex ? R(0) : rec[5];
+          const R kx = a.kappa * gx, ky = a.kappa * gy, kz = a.kappa * gz;
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
+            const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+            const R ri = rsq(r2);
+            const R ds = fma(dx, gx, fma(dy, gy, dz * gz)) * (ri * ri);
+            acc[3 * q + 0] = fma(ri, fma(dx, ds, kx), acc[3 * q + 0]);
+            acc[3 * q + 1] = fma(ri, fma(dy, ds, ky), acc[3 * q + 1]);
+            acc[3 * q + 2] = fma(ri, fma(dz, ds, kz), acc[3 * q + 2]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NT * NC; k++) acc64[k] += (double)acc[k];
+  }
+  if (t >= ne) return;
+  const size_t stride = (size_t)ne;
+  double *out = a.partial + (size_t)s * a.nout * NC * stride + t;
+  if (a.nout == 1) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < NT; q++) v = fma(a.tw[q * stride + t], acc64[q * NC + c], v);
+      out[c * stride] = v;
+    }
+  } else {
+    double wq[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + t];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < NT; q++)
+          v = fma(wq[q] * a.phi[q][k], acc64[q * NC + c], v);
+        out[(k * NC + c) * stride] = v;
+      }
+  }
+  (void)F32;
+}
+
+// Source strengths.  One lane per source quadrature point (j, b).
+template <typename R, int KIND>
+__global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
+                       const double *__restrict__ swt,
+                       const double *__restrict__ normals,
+                       const int *__restrict__ tri, int trial_p1,
+                       RuleArgs rule, const double *__restrict__ x, int ncols,
+                       double scale, R *__restrict__ src)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ne * nt) return;
+  const int j = idx / nt, b = idx - j * nt;
+  constexpr int NC = KIND == 2 ? 3 : 1;
+  double dens[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const double *xc = x + (size_t)c * ncols;
+    if (trial_p1) {
+      const double e1 = rule.x[b], e2 = rule.y[b];
+      dens[c] = (1.0 - e1 - e2) * xc[tri[3 * j]] + e1 * xc[tri[3 * j + 1]] +
+                e2 * xc[tri[3 * j + 2]];
+    } else {
+      dens[c] = xc[j];
+    }
+  }
+  const double f = swt[idx] * scale;
+  constexpr int REC = Rec<KIND>::n;
+  R *o = src + (size_t)idx * REC;
+  o[0] = (R)sg[3 * idx];
+  o[1] = (R)sg[3 * idx + 1];
+  o[2] = (R)sg[3 * idx + 2];
+  if (KIND == 0) {
+    o[3] = (R)(f * dens[0]);
+  } else if (KIND == 1) {
+    const double m = f * dens[0];
+    o[3] = (R)(m * normals[3 * j]);
+    o[4] = (R)(m * normals[3 * j + 1]);
+    o[5] = (R)(m * normals[3 * j + 2]);
+    o[6] = o[7] = (R)0;
+  } else {
+    o[3] = (R)(f * dens[0]);
+    o[4] = (R)(f * dens[NC > 1 ? 1 : 0]);
+    o[5] = (R)(f * dens[NC > 2 ? 2 : 0]);
+    o[6] = o[7] = (R)0;
+  }
+}
+
+__global__ void k_reduce(int n, int chunks, const double *__restrict__ partial,
+                         double *__restrict__ out)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < chunks; c++) s += partial[(size_t)c * n + k];
+  out[k] = s;
+}
+
+// y = near CSR * x + far; one lane per test DOF.
+__global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
+                         const int *__restrict__ ptr, const int *__restrict__ col,
+                         const double *__restrict__ val,
+                         const double *__restrict__ x,
+                         const double *__restrict__ far,
+                         const int *__restrict__ ntp, const int *__restrict__ nt,
+                         const int *__restrict__ ntk, double *__restrict__ y)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  if (nc == 1) {
+    double s = 0.0;
+    for (int e = ptr[r]; e < ptr[r + 1]; e++) s = fma(val[e], x[col[e]], s);
+    if (test_p1) {
+      for (int p = ntp[r]; p < ntp[r + 1]; p++) s += far[(size_t)ntk[p] * ne + nt[p]];
+    } else {
+      s += far[r];
+    }
+    y[r] = s;
+    return;
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int e = ptr[r]; e < ptr[r + 1]; e++) {
+    const double *B = val + (size_t)e * 6;  // xx yy zz xy xz yz
+    const int c = col[e];
+    const double x0 = x[c], x1 = x[(size_t)ncols + c], x2 = x[2 * (size_t)ncols + c];
+    s0 += B[0] * x0 + B[3] * x1 + B[4] * x2;
+    s1 += B[3] * x0 + B[1] * x1 + B[5] * x2;
+    s2 += B[4] * x0 + B[5] * x1 + B[2] * x2;
+  }
+  if (test_p1) {
+    for (int p = ntp[r]; p < ntp[r + 1]; p++) {
+      const int k = ntk[p], t = nt[p];
+      s0 += far[(size_t)(3 * k + 0) * ne + t];
+      s1 += far[(size_t)(3 * k + 1) * ne + t];
+      s2 += far[(size_t)(3 * k + 2) * ne + t];
+    }
+  } else {
+    s0 += far[r];
+    s1 += far[(size_t)ne + r];
+    s2 += far[2 * (size_t)ne + r];
+  }
+  y[r] = s0;
+  y[(size_t)nrows + r] = s1;
+  y[2 * (size_t)nrows + r] = s2;
+}
+
+template <typename R, int KIND>
+int launch_far(int nt, dim3 grid, dim3 blk, size_t lds, hipStream_t s,
+               const FarArgs<R> &a)
+{
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((k_far<R, KIND, 1>), grid, blk, lds, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far<R, KIND, 3>), grid, blk, lds, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far<R, KIND, 4>), grid, blk, lds, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far<R, KIND, 6>), grid, blk, lds, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far<R, KIND, 7>), grid, blk, lds, s, a); break;
+  case 12: hipLaunchKernelGGL((k_far<R, KIND, 12>), grid, blk, lds, s, a); break;
+  case 13: hipLaunchKernelGGL((k_far<R, KIND, 13>), grid, blk, lds, s, a); break;
+  default:
+    set_error("unsupported far-field rule size " + std::to_string(nt));
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+template <typename R>
+int run_far(ebh_op *op, hipStream_t s)
+{
+  FarArgs<R> a;
+  const int ne = op->mesh->T.ne, nt = op->rule.nt;
+  a.ne = ne;
+  a.chunks = op->chunks;
+  a.chunk = op->chunk;
+  a.tile = op->tile;
+  a.nout = op->d.test_basis ? 3 : 1;
+  a.kappa = (R)(3.0 - 4.0 * op->d.nu);
+  if constexpr (sizeof(R) == 4) {
+    a.tq = op->tq32.p;
+    a.src = op->src32.p;
+  } else {
+    a.tq = op->tq64.p;
+    a.src = op->src64.p;
+  }
+  a.touch_ptr = op->mesh->touch_ptr.p;
+  a.touch = op->mesh->touch.p;
+  a.tw = op->tw.p;
+  a.partial = op->partial.p;
+  for (int q = 0; q < 16; q++) {
+    const bool in = q < nt;
+    a.phi[q][0] = in ? 1.0 - op->rule.x[q] - op->rule.y[q] : 0.0;
+    a.phi[q][1] = in ? op->rule.x[q] : 0.0;
+    a.phi[q][2] = in ? op->rule.y[q] : 0.0;
+  }
+  const int tb = (ne + op->block - 1) / op->block;
+  dim3 grid((unsigned)(tb * op->chunks)), blk(op->block);
+  const size_t lds = (size_t)op->tile * nt * (op->d.kernel == 0 ? 4 : 8) * sizeof(R);
+  switch (op->d.kernel) {
+  case EBH_LAPLACE_SL: return launch_far<R, 0>(nt, grid, blk, lds, s, a);
+  case EBH_LAPLACE_DL: return launch_far<R, 1>(nt, grid, blk, lds, s, a);
+  default: return launch_far<R, 2>(nt, grid, blk, lds, s, a);
+  }
+}
+
+template <typename R>
+int run_fold(ebh_op *op, const double *x, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne, nt = op->rule.nt;
+  const int n = ne * nt;
+  R *src;
+  if constexpr (sizeof(R) == 4) src = op->src32.p; else src = op->src64.p;
+  const double scale = op->d.kernel == EBH_KELVIN_SL
+                           ? (1.0 + op->d.nu) / (8.0 * M_PI * (1.0 - op->d.nu))
+                           : kInv4Pi;
+  dim3 grid((n + 255) / 256), blk(256);
+  const int p1 = op->d.trial_basis;
+  switch (op->d.kernel) {
+  case EBH_LAPLACE_SL:
+    hipLaunchKernelGGL((k_fold<R, 0>), grid, blk, 0, s, ne, nt, op->sg64.p,
+                       op->swt.p, op->normals.p, op->mesh->tri.p, p1, op->rule,
+                       x, op->ncols, scale, src);
+    break;
+  case EBH_LAPLACE_DL:
+    hipLaunchKernelGGL((k_fold<R, 1>), grid, blk, 0, s, ne, nt, op->sg64.p,
+                       op->swt.p, op->normals.p, op->mesh->tri.p, p1, op->rule,
+                       x, op->ncols, scale, src);
+    break;
+  default:
+    hipLaunchKernelGGL((k_fold<R, 2>), grid, blk, 0, s, ne, nt, op->sg64.p,
+                       op->swt.p, op->normals.p, op->mesh->tri.p, p1, op->rule,
+                       x, op->ncols, scale, src);
+    break;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+}  // namespace
+
+int op_configure(ebh_op *op, int block, int tile, int chunks)
+{
+  const int ne = op->mesh->T.ne;
+  if (block) {
+    if (block != 64 && block != 128 && block != 256) {
+      set_error("far-field block size must be 64, 128 or 256");
+      return EBH_ERR_ARG;
+    }
+    op->block = block;
+  }
+  if (tile) {
+    if (tile < 1 || tile > 64) {
+      set_error("far-field tile must be 1..64 source triangles");
+      return EBH_ERR_ARG;
+    }
+    op->tile = tile;
+  }
+  if (chunks <= 0) {
+    // aim for ~4096 blocks: 256 CUs x several resident blocks
+    const int tb = (ne + op->block - 1) / op->block;
+    chunks = std::max(1, (4096 + tb - 1) / tb);
+    if (chunks >= 8) chunks = (chunks + 7) / 8 * 8;
+  }
+  chunks = std::min(chunks, ne);
+  op->chunk = (ne + chunks - 1) / chunks;
+  op->chunks = (ne + op->chunk - 1) / op->chunk;
+  const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
+  EBH_TRY(op->partial.alloc((size_t)op->chunks * nout * ne));
+  return EBH_OK;
+}
+
+int op_build_far(ebh_op *op)
+{
+  const Topology &T = op->mesh->T;
+  const int ne = T.ne, nt = op->rule.nt;
+  std::vector<double> tq((size_t)nt * 3 * ne), sg((size_t)ne * nt * 3),
+      tw((size_t)nt * ne), swt((size_t)ne * nt), nrm((size_t)3 * ne);
+  for (int t = 0; t < ne; t++) {
+    const double *P0 = &T.nodes[3 * T.tri[3 * t]];
+    const double *P1 = &T.nodes[3 * T.tri[3 * t + 1]];
+    const double *P2 = &T.nodes[3 * T.tri[3 * t + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
+                 c2 = u[0] * v[1] - u[1] * v[0];
+    const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    nrm[3 * t] = c0 / J; nrm[3 * t + 1] = c1 / J; nrm[3 * t + 2] = c2 / J;
+    for (int q = 0; q < nt; q++) {
+      const double e1 = op->rule.x[q], e2 = op->rule.y[q];
+      for (int k = 0; k < 3; k++) {
+        // x = A + ksi1 u + ksi2 v, src/BEM3dLaplace.cpp:171-173
+        const double xk = P0[k] + e1 * u[k] + e2 * v[k];
+        tq[(size_t)(3 * q + k) * ne + t] = xk;
+        sg[((size_t)t * nt + q) * 3 + k] = xk;
+      }
+      tw[(size_t)q * ne + t] = op->rule.w[q] * J;
+      swt[(size_t)t * nt + q] = op->rule.w[q] * J;
+    }
+  }
+  if (op->d.far_precision == EBH_FP32) {
+    std::vector<float> t32(tq.begin(), tq.end());
+    EBH_TRY(op->tq32.upload(t32));
+    EBH_TRY(op->src32.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
+  } else {
+    EBH_TRY(op->tq64.upload(tq));
+    EBH_TRY(op->src64.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
+  }
+  EBH_TRY(op->sg64.upload(sg));
+  EBH_TRY(op->tw.upload(tw));
+  EBH_TRY(op->swt.upload(swt));
+  EBH_TRY(op->normals.upload(nrm));
+  const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
+  EBH_TRY(op->tri_out.alloc(nout * ne));
+  const int block = ne >= 16384 ? 256 : ne >= 4096 ? 128 : 64;
+  return op_configure(op, block, 64, 0);
+}
+
+int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
+                 bool timed)
+{
+  const int ne = op->mesh->T.ne;
+  const bool f32 = op->d.far_precision == EBH_FP32;
+  if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
+  EBH_TRY(f32 ? run_fold<float>(op, x, s) : run_fold<double>(op, x, s));
+  if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
+  const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
+  EBH_TRY(f32 ? run_far<float>(op, s) : run_far<double>(op, s));
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
+  if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
+  const int nred = (op->d.test_basis ? 3 : 1) * op->nc * ne;
+  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
+                     op->chunks, op->partial.p, op->tri_out.p);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+                     op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
+                     op->near_ptr.p, op->near_col.p, op->near_val.p, x,
+                     op->tri_out.p, op->mesh->node_tri_ptr.p,
+                     op->mesh->node_tri.p, op->mesh->node_tri_k.p, y);
+  EBH_HIP(hipGetLastError());
+  if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
+  return EBH_OK;
+}
+
+}  // namespace ebh

@@ -118,6 +118,12 @@ int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks);
  * given device vectors. */
 int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
                 double *ms);
+/* Far-kernel timing over a region of ordinary applies: enable=1 starts
+ * recording HIP events around every far-field launch (up to max_launches);
+ * each call first returns the total milliseconds and launch count recorded
+ * since the previous call (events on the op's stream), then resets. */
+int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
+                   int64_t *launches);
 /* Work counts: [0] far interactions per RHS, [1] near nnz (entries),
  * [2] near rows, [3] near cols. */
 int ebh_op_work(const ebh_op *op, int64_t *work);
@@ -128,6 +134,43 @@ void ebh_op_destroy(ebh_op *op);
  * matrix; u has np entries.  Host pointers. */
 int ebh_mass_apply(const ebh_mesh *mesh, int test_basis, const double *u,
                    double *y);
+
+/* ---- Krylov solver ---------------------------------------------------- */
+/* GMRES(restart) with FP64 Arnoldi (classical Gram-Schmidt, twice) and
+ * FP64 Givens rotations, on device vectors.  With `mixed` set and an FP32
+ * far-field operator a32 given, the solve is mixed-precision iterative
+ * refinement: the residual r = b - A64 x is formed with the FP64 operator,
+ * each correction solves A32 d = r by GMRES to inner_tol, x += d, until
+ * ||b - A64 x|| / ||b|| <= tol. */
+typedef struct {
+  int restart;      /* GMRES restart length (default 50)                  */
+  int max_iters;    /* cap on Arnoldi steps over the whole solve          */
+  double tol;       /* target FP64 relative residual                      */
+  int mixed;        /* 1: iterative refinement with the FP32 operator     */
+  double inner_tol; /* relative tolerance of each inner solve (mixed)     */
+  int max_outer;    /* cap on refinement steps (mixed)                    */
+} ebh_solve_opts;
+
+typedef struct {
+  int iters;        /* Arnoldi steps (operator applies inside GMRES)      */
+  int outer;        /* refinement steps (mixed) or restarts               */
+  int matvec64;     /* FP64 operator applies                              */
+  int matvec32;     /* FP32 operator applies                              */
+  double relres;    /* final ||b - A64 x|| / ||b|| (true residual, FP64)  */
+  double ms;        /* wall time of the solve on the stream (HIP events)  */
+  int converged;
+} ebh_solve_stats;
+
+/* a64: FP64 operator (square: rows == cols).  a32: optional FP32-far
+ * operator of the same shape (needed when opts->mixed).  Allocates its
+ * Krylov basis once; one solver handle serves one solve at a time. */
+int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out);
+/* b, x: device vectors of length rows; x holds the initial guess on entry.
+ * Returns EBH_ERR_NOCONV (with x the last iterate) if the tolerance is not
+ * reached within the caps. */
+int ebh_solver_solve(ebh_solver *s, const double *b, double *x,
+                     const ebh_solve_opts *opts, ebh_solve_stats *stats);
+void ebh_solver_destroy(ebh_solver *s);
 
 #ifdef __cplusplus
 }

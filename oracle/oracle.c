@@ -951,7 +951,7 @@ static void tri_row(const ctx_t *c, int op, int test_basis, int trial_basis,
   const int *fi = elems + 3 * i;
   int nc = op == 2 ? 3 : 1;
   int ntrial = trial_basis ? np : ne;
-  memset(out, 0, sizeof(double) * 3 * nc);
+  memset(out, 0, sizeof(double) * 9);
   for (int j = 0; j < ne; j++) {
     pair_geom g;
     pair_setup(nodes, elems, i, j, &g);
