@@ -306,3 +306,21 @@ def op_profile(op: Operator, enable: bool, max_launches=4096):
     check(L.ebh_op_profile(op.h, int(enable), max_launches, C.byref(ms),
                            C.byref(n)))
     return ms.value, n.value
+
+
+def mesh_check(nodes, elements):
+    """Host-only validation and classification (no GPU needed)."""
+    L = lib()
+    L.ebh_mesh_check.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    elements = np.ascontiguousarray(elements, dtype=np.int32)
+    info = (C.c_int64 * 8)()
+    nnz = (C.c_int64 * 4)()
+    check(L.ebh_mesh_check(nodes.shape[0], _ptr(nodes), elements.shape[0],
+                           _ptr(elements), info, nnz))
+    keys = ["np", "ne", "coincident", "edge", "vertex", "far", "max_touch"]
+    d = dict(zip(keys, list(info)[:7]))
+    d["nnz"] = {"P0xP0": nnz[0], "P0xP1": nnz[1], "P1xP0": nnz[2],
+                "P1xP1": nnz[3]}
+    return d

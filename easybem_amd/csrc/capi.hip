@@ -136,6 +136,34 @@ int ebh_mesh_info(const ebh_mesh *m, int64_t *info)
 
 void ebh_mesh_destroy(ebh_mesh *m) { delete m; }
 
+int ebh_mesh_check(int np, const double *nodes, int ne, const int *elems,
+                   int64_t *info, int64_t *nnz)
+{
+  if (!nodes || !elems) { set_error("null argument"); return EBH_ERR_ARG; }
+  Topology T;
+  std::string err = build_topology(np, nodes, ne, elems, T);
+  if (!err.empty()) {
+    set_error("mesh rejected: " + err);
+    return EBH_ERR_MESH;
+  }
+  if (info) {
+    info[0] = T.np;
+    info[1] = T.ne;
+    info[2] = T.n_class[kCoincident];
+    info[3] = T.n_class[kEdge];
+    info[4] = T.n_class[kVertex];
+    info[5] = T.n_class[kDisjoint];
+    info[6] = T.max_touch;
+  }
+  if (nnz)
+    for (int k = 0; k < 4; k++) {
+      NearCSR c;
+      build_near_csr(T, k >> 1, k & 1, c);
+      nnz[k] = (int64_t)c.col.size();
+    }
+  return EBH_OK;
+}
+
 // -------------------------------------------------------------- operator
 int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
 {

@@ -93,6 +93,13 @@ int ebh_mesh_create(int np, const double *nodes, int ne, const int *elems,
 /* info[0..7] = np, ne, coincident, edge-adjacent, vertex-adjacent and
  * disjoint pair counts, max touching triangles per triangle, bytes held. */
 int ebh_mesh_info(const ebh_mesh *mesh, int64_t *info);
+/* Host-only validation and pair classification (no device needed): fills
+ * info[0..6] like ebh_mesh_info and, when nnz is non-NULL, nnz[0..3] = the
+ * near-field CSR entries of the P0xP0, P0xP1, P1xP0 and P1xP1 patterns.
+ * Returns EBH_ERR_MESH with the reason in ebh_last_error() for a rejected
+ * mesh. */
+int ebh_mesh_check(int np, const double *nodes, int ne, const int *elems,
+                   int64_t *info, int64_t *nnz);
 void ebh_mesh_destroy(ebh_mesh *mesh);
 
 /* ---- operator --------------------------------------------------------- */

@@ -1,0 +1,100 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled reference
+CPU operator (oracle/_ref/libeasybem_ref.so, built by oracle/Makefile from
+the reference's own sources).  Run in a checkout where the reference sources
+were present at build time:
+
+    python tests/golden/make_golden.py
+
+Fixtures (inputs and reference outputs only):
+  quadrature.npz        the reference's Gauss-Legendre (indices 0..18) and
+                        triangle (0..6) tables
+  laplace_<mesh>_o<k>.npz  dense V[test,trial], K[test,node], M[test,node]
+                        of getLaplace3d at order k
+  kelvin_ico1_o4.npz    dense Kelvin V (nu = 0.3) of getElasticity3d
+  hybrid_ico3.npz       config-1 golden vectors: y = V_hyb x for x ~ U(-1,1)
+                        (seed 20261203) and x = Y_2^0(centroids), with V_hyb
+                        the far entries of V(order 4) and the touching entries
+                        of V(order 3)
+  calibration.json      single-thread time of the reference getLaplace3d vs
+                        the oracle's dense assembly on icosphere level 3
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from easybem_amd import meshes  # noqa: E402
+from oracle import Oracle, Reference  # noqa: E402
+
+
+def touching_mask(T):
+    """Boolean ne x ne matrix: triangles sharing at least one node."""
+    ne = T.shape[0]
+    np_ = T.max()
+    inc = np.zeros((ne, np_ + 1), dtype=np.int8)
+    for k in range(3):
+        inc[np.arange(ne), T[:, k]] = 1
+    return (inc.astype(np.int32) @ inc.T.astype(np.int32)) > 0
+
+
+def main():
+    ref = Reference()
+    out = {}
+    q = {}
+    for i in range(19):
+        p, w = ref.line_rule(i)
+        q[f"line{i}_p"], q[f"line{i}_w"] = p, w
+    for i in range(7):
+        p, w = ref.tri_rule(i)
+        q[f"tri{i}_p"], q[f"tri{i}_w"] = p, w
+    np.savez_compressed(os.path.join(HERE, "quadrature.npz"), **q)
+
+    cases = {"ico1": meshes.icosphere(1), "torus8x4": meshes.torus(1.0, 0.4, 8, 4)}
+    for name, (P, T) in cases.items():
+        for order in (3, 4):
+            V, K, D, M = ref.laplace3d(P, T, order)
+            np.savez_compressed(os.path.join(HERE, f"laplace_{name}_o{order}.npz"),
+                                nodes=P, elements=T, V=V, K=K, M=M)
+    P, T = meshes.icosphere(1)
+    Vk = ref.elasticity3d_V(P, T, 0.3, 4)
+    np.savez_compressed(os.path.join(HERE, "kelvin_ico1_o4.npz"), nodes=P,
+                        elements=T, nu=0.3, V=Vk)
+
+    P, T = meshes.icosphere(3)
+    t0 = time.perf_counter()
+    V4 = ref.laplace3d(P, T, 4)[0]
+    t_ref = time.perf_counter() - t0
+    V3 = ref.laplace3d(P, T, 3)[0]
+    near = touching_mask(T)
+    Vh = np.where(near, V3, V4)
+    rng = np.random.default_rng(20261203)
+    x = rng.uniform(-1, 1, T.shape[0])
+    c = meshes.centroids(P, T)
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    y20 = 3 * c[:, 2] ** 2 - 1
+    np.savez_compressed(os.path.join(HERE, "hybrid_ico3.npz"), nodes=P,
+                        elements=T, x=x, y=Vh @ x, t=y20, Vt=Vh @ y20,
+                        near_rows=np.array([0, 1, 100, 641, 1279]),
+                        V_near_rows=Vh[[0, 1, 100, 641, 1279]])
+
+    o = Oracle()
+    t0 = time.perf_counter()
+    o.laplace_dense(P, T, 4)
+    t_orc = time.perf_counter() - t0
+    with open(os.path.join(HERE, "calibration.json"), "w") as f:
+        json.dump({"mesh": "icosphere level 3 (1280 triangles)", "order": 4,
+                   "reference_getLaplace3d_s": t_ref,
+                   "oracle_dense_s": t_orc,
+                   "note": "reference: 1 thread, V+K+D+M; oracle: V+K+M, "
+                           "OpenMP threads as available"}, f, indent=1)
+    print("fixtures written; reference", t_ref, "s, oracle", t_orc, "s")
+
+
+if __name__ == "__main__":
+    main()
