@@ -124,13 +124,14 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NT * NC; k++) acc[k] = R(0);
     for (int jj = 0; jj < nt; jj++) {
-      const bool ex = (mask >> jj) & 1ull;
+      // 0 for a touching source triangle (handled by the near field), else 1
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
       const R *rec = tile + jj * NT * REC;
 #pragma unroll
       for (int b = 0; b < NT; b++, rec += REC) {
         const R sx = rec[0], sy = rec[1], sz = rec[2];
         if constexpr (KIND == 0) {
-          const R sg = ex ? R(0) : rec[3];
+          const R sg = rec[3] * keep;
 #pragma unroll
           for (int q = 0; q < NT; q++) {
             const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
@@ -138,8 +139,7 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
             acc[q] = fma(sg, rsq(r2), acc[q]);
           }
         } else if constexpr (KIND == 1) {
-          const R px = ex ? R(0) : rec[3], py = ex ? R(0) : rec[4],
-                  pz = ex ? R(0) : rec[5];
+          const R px = rec[3] * keep, py = rec[4] * keep, pz = rec[5] * keep;
 #pragma unroll
           for (int q = 0; q < NT; q++) {
             const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
@@ -149,8 +149,7 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
             acc[q] = fma(dp, ri * ri * ri, acc[q]);
           }
         } else {
-          const R gx = ex ? R(0) : rec[3], gy = ex ? R(0) : rec[4],
-                  gz = ex ? R(0) : rec[5];
+          const R gx = rec[3] * keep, gy = rec[4] * keep, gz = rec[5] * keep;
           const R kx = a.kappa * gx, ky = a.kappa * gy, kz = a.kappa * gz;
 #pragma unroll
           for (int q = 0; q < NT; q++) {
