@@ -59,6 +59,71 @@ def cpu_baseline(P, T, n_matvec, seconds=10.0):
                       f"({t_matvec:.1f} s) x {n_matvec} FP64 matvecs per solve"}
 
 
+def secondary(L, meshes, solve_batched=False):
+    """Configs 1, 3 and 4 at full size: matvec times (HIP events) and, for
+    config 4, optionally the batched 64-RHS mixed-precision solve."""
+    out = {}
+    rng = np.random.default_rng(20261203)
+    # config 1: icosphere L3, P0 V, 7-point far / 4-point near
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    c = meshes.centroids(P, T)
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    t = 3 * c[:, 2] ** 2 - 1
+    r1 = {}
+    for prec in ("fp64", "fp32"):
+        op = L.Operator(m, "V", 0, 0, prec, line_rule=3, tri_rule=4)
+        x = L.DeviceArray.from_host(t)
+        y = L.DeviceArray(op.nrows)
+        tm = op.time(x, y, reps=20)
+        Vt = y.to_host()
+        A = meshes.areas(P, T)
+        r1[prec] = {"matvec_ms": tm["total"], "far_ms": tm["far"],
+                    "far_tflops": 11 * op.work()["far_interactions"] / tm["far"] / 1e9,
+                    "rayleigh_Y20": float(t @ Vt / np.sum(A * t * t))}
+    out["cfg1_ico3_p0_V"] = r1
+    # config 3: torus 256x128, Kelvin P1, FP32 far field
+    P, T = meshes.torus(1.0, 0.4, 256, 128)
+    m = L.Mesh(P, T)
+    r3 = {}
+    for prec in ("fp32", "fp64"):
+        op = L.Operator(m, "kelvin", 1, 1, prec, line_rule=4, tri_rule=4, nu=0.3)
+        x = L.DeviceArray.from_host(rng.standard_normal(op.ncols))
+        y = L.DeviceArray(op.nrows)
+        tm = op.time(x, y, reps=2)
+        r3[prec] = {"matvec_ms": tm["total"],
+                    "far_tflops": 30 * op.work()["far_interactions"] / tm["far"] / 1e9}
+    out["cfg3_torus_p1_kelvin"] = r3
+    # config 4: icosphere L6, P0 V, 64 right-hand sides in one launch
+    P, T = meshes.icosphere(6)
+    m = L.Mesh(P, T)
+    r4 = {}
+    ops = {}
+    for prec in ("fp32", "fp64"):
+        op = L.Operator(m, "V", 0, 0, prec, line_rule=4, tri_rule=4, max_rhs=64)
+        ops[prec] = op
+        B = rng.uniform(-1, 1, 64 * op.ncols)
+        x = L.DeviceArray.from_host(B)
+        y = L.DeviceArray(op.nrows * 64)
+        tm = op.time(x, y, reps=1, nrhs=64)
+        r4[prec] = {"block_matvec_ms": tm["total"],
+                    "far_tflops": (9 + 2 * 64) * op.work()["far_interactions"]
+                    / tm["far"] / 1e9}
+    if solve_batched:
+        s = L.Solver(ops["fp64"], ops["fp32"], restart=50, batch=64)
+        B = rng.uniform(-1, 1, 64 * ops["fp64"].ncols)
+        db = L.DeviceArray.from_host(B)
+        dx = L.DeviceArray.from_host(np.zeros_like(B))
+        t0 = time.perf_counter()
+        st = s.solve_device(db, dx, tol=1e-8, mixed=True, nb=64, max_iters=3000)
+        dt = time.perf_counter() - t0
+        r4["batched_mixed_solve"] = {"seconds": dt, "solves_per_s": 64 / dt,
+                                     "iters_max": max(v["iters"] for v in st),
+                                     "relres_max": max(v["relres"] for v in st)}
+    out["cfg4_ico6_p0_V_64rhs"] = r4
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +132,9 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--secondary", action="store_true",
+                    help="also time configs 1, 3 and 4 (matvecs, batched solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -91,12 +159,11 @@ def main():
     def solve(mixed=True):
         L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))
         return solver.solve_device(db, dx, tol=1e-8, mixed=mixed,
-                                   max_iters=3000, inner_tol=1e-5)
+                                   max_iters=3000, inner_tol=args.inner_tol)
 
     for _ in range(args.warmup):
         solve()
     L.check(L.lib().ebh_synchronize())
-    L.op_profile(V32, True, 1 << 15)
     lat, stats = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -106,7 +173,6 @@ def main():
         stats.append(st)
     L.check(L.lib().ebh_synchronize())
     wall = time.perf_counter() - t0
-    far_ms, far_launches = L.op_profile(V32, False)
     if not all(s["converged"] and s["relres"] <= 1e-8 for s in stats):
         log("solve did not converge:", stats)
         sys.exit(1)
@@ -118,6 +184,11 @@ def main():
     w = V32.work()
     t32 = V32.time(xv, yv, reps=5)
     t64 = V64.time(xv, yv, reps=3)
+    # far-kernel launch time with HIP events on the operator's stream
+    L.op_profile(V32, True, 64)
+    for _ in range(10):
+        V32.apply(xv, yv)
+    far_ms, far_launches = L.op_profile(V32, False)
     far_flop = FLOP_SL * w["far_interactions"]
 
     out_fp64 = {}
@@ -175,6 +246,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(P, T, st64.get("matvec64", 1))
+    if args.secondary:
+        res["secondary"] = secondary(L, meshes, solve_batched=True)
     print(json.dumps(res))
 
 

@@ -244,45 +244,60 @@ def _solver_sigs(L):
         return
     vp = C.c_void_p
     L.ebh_solver_create.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_void_p)]
+    L.ebh_solver_create_batched.argtypes = [vp, vp, C.c_int, C.c_int,
+                                            C.POINTER(C.c_void_p)]
+    L.ebh_solver_set.argtypes = [vp, C.c_int, C.c_int]
     L.ebh_solver_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts),
                                    C.POINTER(SolveStats)]
+    L.ebh_solver_solve_batched.argtypes = [vp, C.c_int, vp, vp,
+                                           C.POINTER(SolveOpts),
+                                           C.POINTER(SolveStats)]
     L.ebh_solver_destroy.argtypes = [vp]
     L.ebh_solver_destroy.restype = None
     L._solver_sigs = True
 
 
 class Solver:
-    """GMRES(restart) on device; mixed-precision refinement when a32 given."""
+    """GMRES(restart) on device for up to `batch` systems sharing the
+    operator; mixed-precision refinement when a32 is given."""
 
-    def __init__(self, a64: Operator, a32: Operator = None, restart=50):
+    def __init__(self, a64: Operator, a32: Operator = None, restart=50,
+                 batch=1, graphs=True, check_every=0):
         L = lib()
         _solver_sigs(L)
         h = C.c_void_p()
-        check(L.ebh_solver_create(a64.h, a32.h if a32 else None, restart,
-                                  C.byref(h)))
+        check(L.ebh_solver_create_batched(a64.h, a32.h if a32 else None,
+                                          restart, batch, C.byref(h)))
         self.h = h
         self.n = a64.nrows
         self.a64, self.a32 = a64, a32
         self.restart = restart
+        self.batch = batch
+        check(L.ebh_solver_set(h, int(graphs), check_every))
+
+    def set(self, graphs=True, check_every=0):
+        check(lib().ebh_solver_set(self.h, int(graphs), check_every))
 
     def solve_device(self, b: DeviceArray, x: DeviceArray, tol=1e-8,
                      mixed=False, max_iters=2000, inner_tol=1e-5,
-                     max_outer=10, raise_on_noconv=True):
+                     max_outer=10, raise_on_noconv=True, nb=1):
         o = SolveOpts(self.restart, max_iters, tol, int(mixed), inner_tol,
                       max_outer)
-        st = SolveStats()
-        rc = lib().ebh_solver_solve(self.h, b.ptr, x.ptr, C.byref(o),
-                                    C.byref(st))
+        st = (SolveStats * nb)()
+        rc = lib().ebh_solver_solve_batched(self.h, nb, b.ptr, x.ptr,
+                                            C.byref(o), st)
         if rc != EBH_OK and (raise_on_noconv or rc != -6):
             check(rc)
-        return st.as_dict()
+        out = [s.as_dict() for s in st]
+        return out[0] if nb == 1 else out
 
     def solve(self, b, x0=None, **kw):
         b = np.ascontiguousarray(b, dtype=np.float64)
+        nb = b.size // self.n
         db = DeviceArray.from_host(b)
         dx = DeviceArray.from_host(np.zeros_like(b) if x0 is None else x0)
-        st = self.solve_device(db, dx, **kw)
-        return dx.to_host(), st
+        st = self.solve_device(db, dx, nb=nb, **kw)
+        return dx.to_host().reshape(b.shape), st
 
     def close(self):
         if getattr(self, "h", None):
@@ -324,3 +339,83 @@ def mesh_check(nodes, elements):
     d["nnz"] = {"P0xP0": nnz[0], "P0xP1": nnz[1], "P1xP0": nnz[2],
                 "P1xP1": nnz[3]}
     return d
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("max_slots", C.c_int), ("hbm_budget", C.c_size_t),
+                ("restart", C.c_int), ("tol", C.c_double), ("mixed", C.c_int),
+                ("line_rule", C.c_int), ("tri_rule", C.c_int),
+                ("max_iters", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("relres", C.c_double),
+                ("queue_ms", C.c_double), ("setup_ms", C.c_double),
+                ("solve_ms", C.c_double), ("total_ms", C.c_double),
+                ("ne", C.c_int), ("footprint", C.c_size_t),
+                ("cache_hit", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """Continuous-batching Dirichlet solve server (ebh_server_*)."""
+
+    def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
+                 mixed=True, line_rule=4, tri_rule=4, max_iters=2000):
+        L = lib()
+        vp = C.c_void_p
+        L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
+        L.ebh_server_submit.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp,
+                                        C.c_uint64, C.POINTER(C.c_int64)]
+        L.ebh_server_wait.argtypes = [vp, C.c_int64, C.c_double,
+                                      C.POINTER(RequestResult), vp]
+        L.ebh_server_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.ebh_server_destroy.argtypes = [vp]
+        L.ebh_server_destroy.restype = None
+        o = ServerOpts(max_slots, hbm_budget, restart, tol, int(mixed),
+                       line_rule, tri_rule, max_iters)
+        h = vp()
+        check(L.ebh_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self.ne = {}
+
+    def submit(self, nodes, elements, x0, key=0):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        elements = np.ascontiguousarray(elements, dtype=np.int32)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        rid = C.c_int64()
+        check(lib().ebh_server_submit(self.h, nodes.shape[0], _ptr(nodes),
+                                      elements.shape[0], _ptr(elements),
+                                      _ptr(x0), key, C.byref(rid)))
+        self.ne[rid.value] = elements.shape[0]
+        return rid.value
+
+    def wait(self, rid, timeout=0.0, raise_on_error=False):
+        res = RequestResult()
+        t = np.empty(self.ne.pop(rid))
+        rc = lib().ebh_server_wait(self.h, rid, timeout, C.byref(res), _ptr(t))
+        if rc != EBH_OK and raise_on_error:
+            check(rc)
+        d = res.as_dict()
+        d["error"] = "" if rc == EBH_OK else lib().ebh_last_error().decode()
+        return d, (t if rc == EBH_OK else None)
+
+    def stats(self):
+        a = (C.c_int64 * 8)()
+        check(lib().ebh_server_stats(self.h, a))
+        keys = ["submitted", "completed", "failed", "active", "queued",
+                "bytes_held", "peak_bytes", "cache_hits"]
+        return dict(zip(keys, list(a)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

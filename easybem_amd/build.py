@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-fgpu-flush-denormals-to-zero"]
 
 LIB_SOURCES = ["host_geometry.cpp", "near_field.hip", "far_field.hip",
-               "capi.hip", "solver.hip", "legacy_api.cpp"]
+               "capi.hip", "solver.hip", "legacy_api.cpp", "server.cpp"]
 
 
 def _run(cmd):
@@ -56,7 +56,8 @@ def build_lib(jobs: int = 8) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
+          "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -19,7 +19,8 @@ size_t ebh_op::bytes() const
   return near_ptr.bytes() + near_col.bytes() + near_val.bytes() + tq32.bytes() +
          sg32.bytes() + tq64.bytes() + sg64.bytes() + tw.bytes() + swt.bytes() +
          normals.bytes() + src32.bytes() + src64.bytes() + partial.bytes() +
-         tri_out.bytes();
+         tri_out.bytes() + mgeom32.bytes() + msig32.bytes() + mgeom64.bytes() +
+         msig64.bytes();
 }
 
 extern "C" {
@@ -49,19 +50,25 @@ int ebh_free(void *ptr)
   if (ptr) EBH_HIP(hipFree(ptr));
   return EBH_OK;
 }
+// Operator and solver streams are non-blocking, so these helpers first wait
+// for all device work: they are for callers that do not manage streams.
 int ebh_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
+  EBH_HIP(hipDeviceSynchronize());
   EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return EBH_OK;
 }
 int ebh_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
+  EBH_HIP(hipDeviceSynchronize());
   EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return EBH_OK;
 }
 int ebh_memset(void *dst, int value, size_t bytes)
 {
+  EBH_HIP(hipDeviceSynchronize());
   EBH_HIP(hipMemset(dst, value, bytes));
+  EBH_HIP(hipDeviceSynchronize());
   return EBH_OK;
 }
 int ebh_synchronize(void)
@@ -214,7 +221,10 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   op->nrows = d.test_basis ? mesh->T.np : mesh->T.ne;
   op->ncols = d.trial_basis ? mesh->T.np : mesh->T.ne;
   int rc = EBH_OK;
-  if (hipStreamCreate(&op->stream) != hipSuccess) {
+  // non-blocking: no implicit ordering with the legacy default stream, so
+  // other host threads' default-stream calls never touch this stream (or a
+  // graph being captured on it)
+  if (hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking) != hipSuccess) {
     delete op;
     set_error("hipStreamCreate failed");
     return EBH_ERR_HIP;
@@ -245,10 +255,7 @@ int ebh_op_apply(ebh_op *op, const double *x, double *y, int nrhs, void *stream)
 {
   if (!op || !x || !y || nrhs < 1) { set_error("bad apply arguments"); return EBH_ERR_ARG; }
   hipStream_t s = stream ? (hipStream_t)stream : op->stream;
-  const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
-  for (int r = 0; r < nrhs; r++)
-    EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, s, false));
-  return EBH_OK;
+  return op_apply_multi(op, x, y, nrhs, s, false);
 }
 
 int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs)
@@ -291,9 +298,11 @@ int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
   }
   double acc[4] = {0, 0, 0, 0};
   const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
+  const bool multi = op->rb && nrhs > 1;
   for (int k = 0; k < reps; k++)
-    for (int r = 0; r < nrhs; r++) {
-      EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, op->stream, true));
+    for (int r = 0; r < (multi ? 1 : nrhs); r++) {
+      if (multi) EBH_TRY(op_apply_multi(op, x, y, nrhs, op->stream, true));
+      else EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, op->stream, true));
       EBH_HIP(hipEventSynchronize(op->ev[3]));
       float t01, t12, t23, t03;
       EBH_HIP(hipEventElapsedTime(&t01, op->ev[0], op->ev[1]));

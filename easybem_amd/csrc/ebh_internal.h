@@ -108,6 +108,7 @@ struct ebh_op {
   int64_t near_nnz = 0;
   // far-field geometry
   ebh::DevBuf<float> tq32, sg32;     // target [nt][3][ne], source [ne*nt][3]
+  ebh::DevBuf<float> tc32;           // FP32 double layer: centroids hi/lo [6][ne]
   ebh::DevBuf<double> tq64, sg64;
   ebh::DevBuf<double> tw, swt;       // w_q J per target / source point
   ebh::DevBuf<double> normals;       // ne*3 unit normals
@@ -115,8 +116,13 @@ struct ebh_op {
   ebh::DevBuf<float> src32;
   ebh::DevBuf<double> src64;
   ebh::DevBuf<double> partial, tri_out;
+  // multi-RHS single layer (rb = RHS per lane, 0 when unused)
+  int rb = 0, mchunks = 1, mchunk = 0;
+  ebh::DevBuf<float> mgeom32, msig32;
+  ebh::DevBuf<double> mgeom64, msig64;
   // launch configuration
   int block = 256, tile = 64, chunks = 1, chunk = 0;
+  int gen = 0;  // bumped when the launch shape changes (captured graphs key on it)
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // far-kernel profiling over a timed region: event pairs around each launch
@@ -129,6 +135,8 @@ struct ebh_op {
 namespace ebh {
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed);
+int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
+                   hipStream_t s, bool timed);
 int op_build_near(ebh_op *op);
 int op_build_far(ebh_op *op);
 int op_configure(ebh_op *op, int block, int tile, int chunks);

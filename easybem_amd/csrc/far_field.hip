@@ -62,7 +62,9 @@ template <typename R>
 struct FarArgs {
   int ne, chunks, chunk, tile, nout;  // nout: 1 (P0 test) or 3 (P1 test)
   R kappa;                            // Kelvin: 3 - 4 nu
-  const R *tq;                        // [nt][3][ne]
+  const R *tq;                        // [nt][3][ne] (FP32 double layer: offsets
+                                      // from the triangle centroid)
+  const R *tc;                        // FP32 double layer: centroid hi/lo [6][ne]
   const R *src;                       // [ne*nt][REC]
   const int *touch_ptr, *touch;
   const double *tw;                   // [nt][ne]
@@ -70,14 +72,23 @@ struct FarArgs {
   double phi[16][3];                  // test shape functions at the points
 };
 
-template <int KIND> struct Rec { static constexpr int n = KIND == 0 ? 4 : 8; };
+// Source record lengths: single layer (x, y, z, sigma); double layer and
+// Kelvin (x, y, z, s0, s1, s2, -, -); FP32 double layer keeps the source
+// point as a hi + lo float pair: (hi xyz, lo xyz, p xyz, -, -, -).
+template <typename R, int KIND> struct Rec {
+  static constexpr int n = KIND == 0 ? 4 : (KIND == 1 && sizeof(R) == 4 ? 12 : 8);
+};
 
 template <typename R, int KIND, int NT>
 __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 {
-  constexpr int REC = Rec<KIND>::n;
+  constexpr int REC = Rec<R, KIND>::n;
   constexpr int NC = KIND == 2 ? 3 : 1;
   constexpr bool F32 = sizeof(R) == 4;
+  // FP32 double layer: (x - y).n is a near-cancelling difference for nearby
+  // panels of a smooth surface, so x - y is formed as
+  // (c_hi - y_hi) + (c_lo - y_lo) + (x - c) with c the target centroid.
+  constexpr bool COMP = F32 && KIND == 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R *tile = reinterpret_cast<R *>(smem_raw);
   const int ne = a.ne;
@@ -93,6 +104,14 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
     X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
     Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
     Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+  }
+  R Ch[3] = {0, 0, 0}, Cl[3] = {0, 0, 0};
+  if constexpr (COMP) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Ch[k] = a.tc[k * (size_t)ne + tt];
+      Cl[k] = a.tc[(3 + k) * (size_t)ne + tt];
+    }
   }
   double acc64[NT * NC];
 #pragma unroll
@@ -137,6 +156,19 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
             const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
             const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
             acc[q] = fma(sg, rsq(r2), acc[q]);
+          }
+        } else if constexpr (COMP) {
+          const R px = rec[6] * keep, py = rec[7] * keep, pz = rec[8] * keep;
+          const R Dx = (Ch[0] - sx) + (Cl[0] - rec[3]);
+          const R Dy = (Ch[1] - sy) + (Cl[1] - rec[4]);
+          const R Dz = (Ch[2] - sz) + (Cl[2] - rec[5]);
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const R dx = Dx + X[q], dy = Dy + Y[q], dz = Dz + Z[q];
+            const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+            const R ri = rsq(r2);
+            const R dp = fma(dx, px, fma(dy, py, dz * pz));
+            acc[q] = fma(dp, ri * ri * ri, acc[q]);
           }
         } else if constexpr (KIND == 1) {
           const R px = rec[3] * keep, py = rec[4] * keep, pz = rec[5] * keep;
@@ -196,6 +228,141 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   (void)F32;
 }
 
+// Multi-RHS single layer.  One lane per target quadrature point, RB
+// right-hand sides per lane; a block holds TB = blockDim/NT whole triangles.
+// Source records are read with wave-uniform addresses (scalar loads: the
+// strengths arrive as SGPR operands of v_fmac), so there is no LDS staging
+// and no block barrier in the main loop.  Per interaction: 3 sub, 3 FMA,
+// rsqrt, one multiply by the 0/1 exclusion factor, RB FMAs.
+template <typename R>
+struct FarMArgs {
+  int ne, chunks, chunk, nout, groups, nrhs;
+  const R *tq;                  // [nt][3][ne]
+  const R *geom;                // [ne*nt][4]
+  const R *sig;                 // [groups][ne*nt][RB]
+  const int *touch_ptr, *touch;
+  const double *tw;             // [nt][ne]
+  double *partial;              // [chunks][nrhs][nout][ne]
+  double phi[16][3];
+};
+
+template <typename R, int NT, int RB>
+__global__ __launch_bounds__(256) void k_far_mrhs(FarMArgs<R> a)
+{
+  constexpr int TB = 256 / NT;  // whole triangles per block
+  __shared__ double red[256];
+  const int ne = a.ne;
+  int b = blockIdx.x;
+  const int s = b % a.chunks;
+  b /= a.chunks;
+  const int g = b % a.groups;
+  const int tb = b / a.groups;
+  const int lt = threadIdx.x / NT, q = threadIdx.x - lt * NT;
+  const bool lane_ok = lt < TB;
+  const int t = tb * TB + lt;
+  const int tt = (lane_ok && t < ne) ? t : ne - 1;
+  const int qq = lane_ok ? q : 0;
+  const int j0 = s * a.chunk, j1 = min(ne, j0 + a.chunk);
+  const R X = a.tq[(3 * qq + 0) * (size_t)ne + tt];
+  const R Y = a.tq[(3 * qq + 1) * (size_t)ne + tt];
+  const R Z = a.tq[(3 * qq + 2) * (size_t)ne + tt];
+  double acc64[RB];
+#pragma unroll
+  for (int r = 0; r < RB; r++) acc64[r] = 0.0;
+  int cur = a.touch_ptr[tt];
+  const int cend = a.touch_ptr[tt + 1];
+  while (cur < cend && a.touch[cur] < j0) cur++;
+  int nextx = cur < cend ? a.touch[cur] : INT_MAX;
+  const R eps = Eps<R>::v;
+  const R *__restrict__ geom = a.geom;
+  const R *__restrict__ sig = a.sig + (size_t)g * ne * NT * RB;
+  for (int js = j0; js < j1; js += 64) {
+    const int nt = min(64, j1 - js);
+    uint64_t mask = 0;
+    while (nextx < js + nt) {
+      mask |= 1ull << (nextx - js);
+      ++cur;
+      nextx = cur < cend ? a.touch[cur] : INT_MAX;
+    }
+    R acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) acc[r] = R(0);
+    for (int jj = 0; jj < nt; jj++) {
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      const int p0 = (js + jj) * NT;
+#pragma unroll
+      for (int bq = 0; bq < NT; bq++) {
+        const R *gp = geom + (size_t)(p0 + bq) * 4;
+        const R dx = X - gp[0], dy = Y - gp[1], dz = Z - gp[2];
+        const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+        const R ri = rsq(r2) * keep;
+        const R *sp = sig + (size_t)(p0 + bq) * RB;
+#pragma unroll
+        for (int r = 0; r < RB; r++) acc[r] = fma(sp[r], ri, acc[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; r++) acc64[r] += (double)acc[r];
+  }
+  // reduce the NT points of each triangle through LDS, one RHS at a time
+  const double wq = lane_ok ? a.tw[(size_t)qq * ne + tt] : 0.0;
+  const int nrhs_here = min(RB, a.nrhs - g * RB);
+  const int nred = TB * a.nout;
+  for (int r = 0; r < nrhs_here; r++) {
+    __syncthreads();
+    red[threadIdx.x] = acc64[r] * wq;
+    __syncthreads();
+    if (threadIdx.x < nred) {
+      const int k = threadIdx.x / TB, lt2 = threadIdx.x - k * TB;
+      const int t2 = tb * TB + lt2;
+      if (t2 < ne) {
+        double v = 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < NT; q2++)
+          v += red[lt2 * NT + q2] * (a.nout == 1 ? 1.0 : a.phi[q2][k]);
+        a.partial[(((size_t)s * a.nrhs + g * RB + r) * a.nout + k) * ne + t2] = v;
+      }
+    }
+  }
+}
+
+template <typename R, int RB>
+__global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
+                            const double *__restrict__ sg,
+                            const double *__restrict__ swt,
+                            const int *__restrict__ tri, int trial_p1,
+                            RuleArgs rule, const double *__restrict__ x,
+                            int ncols, double scale, R *__restrict__ geom,
+                            R *__restrict__ sig)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npts = ne * nt;
+  if (idx >= npts * groups) return;
+  const int g = idx / npts, p = idx - g * npts;
+  const int j = p / nt, b = p - j * nt;
+  if (g == 0) {
+    R *o = geom + (size_t)p * 4;
+    o[0] = (R)sg[3 * p];
+    o[1] = (R)sg[3 * p + 1];
+    o[2] = (R)sg[3 * p + 2];
+    o[3] = (R)0;
+  }
+  const double f = swt[p] * scale;
+  const double e1 = rule.x[b], e2 = rule.y[b];
+  R *o = sig + ((size_t)g * npts + p) * RB;
+  for (int r = 0; r < RB; r++) {
+    const int rr = g * RB + r;
+    double d = 0.0;
+    if (rr < nrhs) {
+      const double *xc = x + (size_t)rr * ncols;
+      d = trial_p1 ? (1.0 - e1 - e2) * xc[tri[3 * j]] + e1 * xc[tri[3 * j + 1]] +
+                         e2 * xc[tri[3 * j + 2]]
+                   : xc[j];
+    }
+    o[r] = (R)(f * d);
+  }
+}
+
 // Source strengths.  One lane per source quadrature point (j, b).
 template <typename R, int KIND>
 __global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
@@ -222,13 +389,20 @@ __global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
     }
   }
   const double f = swt[idx] * scale;
-  constexpr int REC = Rec<KIND>::n;
+  constexpr int REC = Rec<R, KIND>::n;
   R *o = src + (size_t)idx * REC;
   o[0] = (R)sg[3 * idx];
   o[1] = (R)sg[3 * idx + 1];
   o[2] = (R)sg[3 * idx + 2];
   if (KIND == 0) {
     o[3] = (R)(f * dens[0]);
+  } else if (KIND == 1 && REC == 12) {
+    const double m = f * dens[0];
+    for (int k = 0; k < 3; k++) {
+      o[3 + k] = (R)(sg[3 * idx + k] - (double)o[k]);
+      o[6 + k] = (R)(m * normals[3 * j + k]);
+      o[9 + k] = (R)0;
+    }
   } else if (KIND == 1) {
     const double m = f * dens[0];
     o[3] = (R)(m * normals[3 * j]);
@@ -254,6 +428,7 @@ __global__ void k_reduce(int n, int chunks, const double *__restrict__ partial,
 }
 
 // y = near CSR * x + far; one lane per test DOF.
+// blockIdx.y selects the right-hand side (vectors back to back).
 __global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
                          const int *__restrict__ ptr, const int *__restrict__ col,
                          const double *__restrict__ val,
@@ -264,6 +439,10 @@ __global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
+  const int rhs = blockIdx.y;
+  x += (size_t)rhs * ncols * nc;
+  y += (size_t)rhs * nrows * nc;
+  far += (size_t)rhs * (test_p1 ? 3 : 1) * nc * ne;
   if (nc == 1) {
     double s = 0.0;
     for (int e = ptr[r]; e < ptr[r + 1]; e++) s = fma(val[e], x[col[e]], s);
@@ -334,9 +513,11 @@ int run_far(ebh_op *op, hipStream_t s)
   a.kappa = (R)(3.0 - 4.0 * op->d.nu);
   if constexpr (sizeof(R) == 4) {
     a.tq = op->tq32.p;
+    a.tc = op->tc32.p;
     a.src = op->src32.p;
   } else {
     a.tq = op->tq64.p;
+    a.tc = nullptr;
     a.src = op->src64.p;
   }
   a.touch_ptr = op->mesh->touch_ptr.p;
@@ -351,7 +532,8 @@ int run_far(ebh_op *op, hipStream_t s)
   }
   const int tb = (ne + op->block - 1) / op->block;
   dim3 grid((unsigned)(tb * op->chunks)), blk(op->block);
-  const size_t lds = (size_t)op->tile * nt * (op->d.kernel == 0 ? 4 : 8) * sizeof(R);
+  const int rec = op->d.kernel == 0 ? 4 : (op->d.kernel == 1 && sizeof(R) == 4 ? 12 : 8);
+  const size_t lds = (size_t)op->tile * nt * rec * sizeof(R);
   switch (op->d.kernel) {
   case EBH_LAPLACE_SL: return launch_far<R, 0>(nt, grid, blk, lds, s, a);
   case EBH_LAPLACE_DL: return launch_far<R, 1>(nt, grid, blk, lds, s, a);
@@ -392,6 +574,87 @@ int run_fold(ebh_op *op, const double *x, hipStream_t s)
   return EBH_OK;
 }
 
+template <typename R, int RB>
+int launch_mrhs_nt(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
+{
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((k_far_mrhs<R, 1, RB>), grid, dim3(256), 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_mrhs<R, 3, RB>), grid, dim3(256), 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_mrhs<R, 4, RB>), grid, dim3(256), 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_mrhs<R, 6, RB>), grid, dim3(256), 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_mrhs<R, 7, RB>), grid, dim3(256), 0, s, a); break;
+  case 12: hipLaunchKernelGGL((k_far_mrhs<R, 12, RB>), grid, dim3(256), 0, s, a); break;
+  case 13: hipLaunchKernelGGL((k_far_mrhs<R, 13, RB>), grid, dim3(256), 0, s, a); break;
+  default:
+    set_error("unsupported far-field rule size " + std::to_string(nt));
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+template <typename R>
+int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne, nt = op->rule.nt, rb = op->rb;
+  const int groups = (nrhs + rb - 1) / rb;
+  R *geom, *sig;
+  if constexpr (sizeof(R) == 4) { geom = op->mgeom32.p; sig = op->msig32.p; }
+  else { geom = op->mgeom64.p; sig = op->msig64.p; }
+  const int n = ne * nt * groups;
+  const int p1 = op->d.trial_basis;
+  switch (rb) {
+  case 8:
+    hipLaunchKernelGGL((k_fold_mrhs<R, 8>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
+                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
+                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
+    break;
+  case 16:
+    hipLaunchKernelGGL((k_fold_mrhs<R, 16>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
+                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
+                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
+    break;
+  default:
+    hipLaunchKernelGGL((k_fold_mrhs<R, 32>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
+                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
+                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
+    break;
+  }
+  EBH_HIP(hipGetLastError());
+  FarMArgs<R> a;
+  a.ne = ne;
+  a.chunks = op->mchunks;
+  a.chunk = op->mchunk;
+  a.nout = op->d.test_basis ? 3 : 1;
+  a.groups = groups;
+  a.nrhs = nrhs;
+  a.tq = sizeof(R) == 4 ? (const R *)op->tq32.p : (const R *)op->tq64.p;
+  a.geom = geom;
+  a.sig = sig;
+  a.touch_ptr = op->mesh->touch_ptr.p;
+  a.touch = op->mesh->touch.p;
+  a.tw = op->tw.p;
+  a.partial = op->partial.p;
+  for (int q = 0; q < 16; q++) {
+    const bool in = q < nt;
+    a.phi[q][0] = in ? 1.0 - op->rule.x[q] - op->rule.y[q] : 0.0;
+    a.phi[q][1] = in ? op->rule.x[q] : 0.0;
+    a.phi[q][2] = in ? op->rule.y[q] : 0.0;
+  }
+  const int tbn = (ne + 256 / nt - 1) / (256 / nt);
+  dim3 grid((unsigned)(tbn * groups * op->mchunks));
+  const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
+  int rc;
+  switch (rb) {
+  case 8: rc = launch_mrhs_nt<R, 8>(nt, grid, s, a); break;
+  case 16: rc = launch_mrhs_nt<R, 16>(nt, grid, s, a); break;
+  default: rc = launch_mrhs_nt<R, 32>(nt, grid, s, a); break;
+  }
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
+  return rc;
+}
+
 }  // namespace
 
 int op_configure(ebh_op *op, int block, int tile, int chunks)
@@ -421,7 +684,20 @@ int op_configure(ebh_op *op, int block, int tile, int chunks)
   op->chunk = (ne + chunks - 1) / chunks;
   op->chunks = (ne + op->chunk - 1) / op->chunk;
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
-  EBH_TRY(op->partial.alloc((size_t)op->chunks * nout * ne));
+  size_t need = (size_t)op->chunks * nout * ne;
+  if (op->rb) {
+    // multi-RHS launch: one lane per target point, 256/nt triangles a block
+    const int tbn = (ne + 256 / op->rule.nt - 1) / (256 / op->rule.nt);
+    const int groups = (op->d.max_rhs + op->rb - 1) / op->rb;
+    int mc = std::max(1, (4096 + tbn * groups - 1) / (tbn * groups));
+    if (mc >= 8) mc = (mc + 7) / 8 * 8;
+    mc = std::min(mc, ne);
+    op->mchunk = (ne + mc - 1) / mc;
+    op->mchunks = (ne + op->mchunk - 1) / op->mchunk;
+    need = std::max(need, (size_t)op->mchunks * op->d.max_rhs * nout * ne);
+  }
+  EBH_TRY(op->partial.alloc(need));
+  op->gen++;
   return EBH_OK;
 }
 
@@ -455,8 +731,27 @@ int op_build_far(ebh_op *op)
   }
   if (op->d.far_precision == EBH_FP32) {
     std::vector<float> t32(tq.begin(), tq.end());
+    if (op->d.kernel == EBH_LAPLACE_DL) {
+      // target points as offsets from the centroid; centroid as hi + lo
+      std::vector<float> tc((size_t)6 * ne);
+      for (int t = 0; t < ne; t++) {
+        double c[3] = {0, 0, 0};
+        for (int k = 0; k < 3; k++)
+          c[k] = (T.nodes[3 * T.tri[3 * t] + k] + T.nodes[3 * T.tri[3 * t + 1] + k] +
+                  T.nodes[3 * T.tri[3 * t + 2] + k]) / 3.0;
+        for (int k = 0; k < 3; k++) {
+          const float hi = (float)c[k];
+          tc[(size_t)k * ne + t] = hi;
+          tc[(size_t)(3 + k) * ne + t] = (float)(c[k] - (double)hi);
+          for (int q = 0; q < nt; q++)
+            t32[(size_t)(3 * q + k) * ne + t] = (float)(tq[(size_t)(3 * q + k) * ne + t] - c[k]);
+        }
+      }
+      EBH_TRY(op->tc32.upload(tc));
+    }
     EBH_TRY(op->tq32.upload(t32));
-    EBH_TRY(op->src32.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
+    EBH_TRY(op->src32.alloc((size_t)ne * nt *
+                            (op->d.kernel == 0 ? 4 : op->d.kernel == 1 ? 12 : 8)));
   } else {
     EBH_TRY(op->tq64.upload(tq));
     EBH_TRY(op->src64.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
@@ -466,7 +761,19 @@ int op_build_far(ebh_op *op)
   EBH_TRY(op->swt.upload(swt));
   EBH_TRY(op->normals.upload(nrm));
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
-  EBH_TRY(op->tri_out.alloc(nout * ne));
+  EBH_TRY(op->tri_out.alloc(nout * ne * (size_t)op->d.max_rhs));
+  if (op->d.kernel == EBH_LAPLACE_SL && op->d.max_rhs > 1) {
+    op->rb = op->d.max_rhs >= 32 ? 32 : op->d.max_rhs >= 16 ? 16 : 8;
+    const size_t groups = (op->d.max_rhs + op->rb - 1) / op->rb;
+    const size_t npts = (size_t)ne * nt;
+    if (op->d.far_precision == EBH_FP32) {
+      EBH_TRY(op->mgeom32.alloc(npts * 4));
+      EBH_TRY(op->msig32.alloc(npts * groups * op->rb));
+    } else {
+      EBH_TRY(op->mgeom64.alloc(npts * 4));
+      EBH_TRY(op->msig64.alloc(npts * groups * op->rb));
+    }
+  }
   const int block = ne >= 16384 ? 256 : ne >= 4096 ? 128 : 64;
   return op_configure(op, block, 64, 0);
 }
@@ -489,6 +796,40 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                      op->chunks, op->partial.p, op->tri_out.p);
   EBH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+                     op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
+                     op->near_ptr.p, op->near_col.p, op->near_val.p, x,
+                     op->tri_out.p, op->mesh->node_tri_ptr.p,
+                     op->mesh->node_tri.p, op->mesh->node_tri_k.p, y);
+  EBH_HIP(hipGetLastError());
+  if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
+  return EBH_OK;
+}
+
+int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
+                   hipStream_t s, bool timed)
+{
+  if (nrhs == 1 || !op->rb) {
+    const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
+    for (int r = 0; r < nrhs; r++)
+      EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, s, timed));
+    return EBH_OK;
+  }
+  if (nrhs > op->d.max_rhs) {
+    set_error("nrhs " + std::to_string(nrhs) + " exceeds the operator's max_rhs " +
+              std::to_string(op->d.max_rhs));
+    return EBH_ERR_ARG;
+  }
+  const int ne = op->mesh->T.ne;
+  const bool f32 = op->d.far_precision == EBH_FP32;
+  if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
+  if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
+  EBH_TRY(f32 ? run_mrhs<float>(op, x, nrhs, s) : run_mrhs<double>(op, x, nrhs, s));
+  if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
+  const int nred = (op->d.test_basis ? 3 : 1) * ne * nrhs;
+  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
+                     op->mchunks, op->partial.p, op->tri_out.p);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256, nrhs), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,
                      op->tri_out.p, op->mesh->node_tri_ptr.p,

@@ -1,0 +1,365 @@
+// Solve server: interior Dirichlet Laplace solves served as they arrive,
+// with continuous batching on one GPU (see include/easybem_hip.h,
+// ebh_server_*).  No reference counterpart: the reference runs one scipy
+// solve at a time (python/examples/laplace3dDirichletGPU.py:60-81).
+//
+// Design:
+//   * max_slots worker threads; each admitted request runs its set-up and its
+//     solve on its own HIP streams (its operators' streams), so the device
+//     interleaves the work of all admitted requests;
+//   * admission is FIFO and by device footprint: the head of the queue is
+//     admitted when a slot is free and its estimated bytes fit in the
+//     budget; idle cached meshes are evicted (least recently used) to make
+//     room;
+//   * a request's result depends only on its own inputs: no state or
+//     reduction is shared between requests, so a result is identical alone
+//     and in a mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ebh_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_between(Clock::time_point a, Clock::time_point b)
+{
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+  int64_t id = 0;
+  int np = 0, ne = 0;
+  std::vector<double> nodes;
+  std::vector<int> elems;
+  double x0[3] = {0, 0, 0};
+  uint64_t key = 0;
+  size_t footprint = 0;
+  Clock::time_point t_submit, t_admit, t_done;
+  ebh_request_result res{};
+  std::vector<double> t;
+  std::string err;
+  bool done = false;
+};
+
+// A resident mesh with the operators and solver of one request type.
+struct Entry {
+  uint64_t key = 0;
+  ebh_mesh *mesh = nullptr;
+  ebh_op *V64 = nullptr, *V32 = nullptr, *K64 = nullptr;
+  ebh_solver *solver = nullptr;
+  size_t bytes = 0;
+  uint64_t last_use = 0;
+  ~Entry()
+  {
+    ebh_solver_destroy(solver);
+    ebh_op_destroy(V64);
+    ebh_op_destroy(V32);
+    ebh_op_destroy(K64);
+    ebh_mesh_destroy(mesh);
+  }
+};
+
+// Device bytes of one request, estimated before anything is allocated:
+// near-field pairs (about 13 per triangle for V, with their FP64 blocks
+// during set-up) and CSR, far-field quadrature data and partial sums, the
+// Krylov basis and work vectors.
+size_t estimate_bytes(int np, int ne, int restart, bool mixed)
+{
+  const double pairs = 14.0 * ne;
+  const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
+  const double far_one = ne * 7.0 * (3 * 12 + 64 + 16) + 4096.0 * 256 * 8 * 2;
+  const double ops = (mixed ? 3 : 2) * (near / 2 + far_one);
+  const double krylov = (restart + 6.0) * ne * 8;
+  const double mesh = np * 24.0 + ne * 100.0;
+  return (size_t)(1.25 * (near + ops + krylov + mesh)) + (16u << 20);
+}
+
+}  // namespace
+
+struct ebh_server {
+  ebh_server_opts o{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::map<int64_t, std::shared_ptr<Request>> reqs;
+  std::multimap<uint64_t, std::unique_ptr<Entry>> idle;  // resident, not in use
+  std::vector<std::thread> workers;
+  bool stopping = false;
+  size_t budget = 0, held = 0, peak = 0;
+  int64_t next_id = 1, n_sub = 0, n_done = 0, n_fail = 0, n_active = 0, n_hits = 0;
+  uint64_t tick = 0;
+
+  // must hold mu
+  bool evict_for(size_t need)
+  {
+    while (held + need > budget && !idle.empty()) {
+      auto lru = idle.begin();
+      for (auto it = idle.begin(); it != idle.end(); ++it)
+        if (it->second->last_use < lru->second->last_use) lru = it;
+      held -= lru->second->bytes;
+      idle.erase(lru);
+    }
+    return held + need <= budget;
+  }
+
+  void worker()
+  {
+    while (true) {
+      std::shared_ptr<Request> r;
+      std::unique_ptr<Entry> entry;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] {
+          if (queue.empty()) return stopping;
+          auto &h = queue.front();
+          if (h->key) {
+            auto it = idle.find(h->key);
+            if (it != idle.end()) return true;
+          }
+          return evict_for(h->footprint) || (held == 0);
+        });
+        if (queue.empty()) return;
+        r = queue.front();
+        queue.pop_front();
+        auto it = r->key ? idle.find(r->key) : idle.end();
+        if (it != idle.end()) {
+          entry = std::move(it->second);
+          idle.erase(it);
+          r->res.cache_hit = 1;
+          n_hits++;
+        } else {
+          evict_for(r->footprint);
+          held += r->footprint;
+        }
+        peak = std::max(peak, held);
+        n_active++;
+        r->t_admit = Clock::now();
+      }
+      const bool hit = entry != nullptr;
+      int rc = run(*r, entry);
+      if (rc != EBH_OK) r->err = ebh_last_error();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        r->t_done = Clock::now();
+        r->res.status = rc;
+        r->res.queue_ms = ms_between(r->t_submit, r->t_admit);
+        r->res.total_ms = ms_between(r->t_submit, r->t_done);
+        r->res.footprint = r->footprint;
+        r->done = true;
+        n_active--;
+        (rc == EBH_OK ? n_done : n_fail)++;
+        // bytes stay accounted while the mesh stays resident
+        const size_t bytes = hit ? entry->bytes : r->footprint;
+        if (entry && r->key && rc == EBH_OK) {
+          entry->last_use = ++tick;
+          entry->bytes = bytes;
+          idle.emplace(r->key, std::move(entry));
+        } else {
+          held -= bytes;
+        }
+      }
+      entry.reset();
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+
+  int build(Request &r, std::unique_ptr<Entry> &e)
+  {
+    e.reset(new Entry());
+    e->key = r.key;
+    EBH_TRY(ebh_mesh_create(r.np, r.nodes.data(), r.ne, r.elems.data(), &e->mesh));
+    ebh_op_desc d{};
+    d.kernel = EBH_LAPLACE_SL;
+    d.test_basis = EBH_P0;
+    d.trial_basis = EBH_P0;
+    d.far_precision = EBH_FP64;
+    d.line_rule = o.line_rule;
+    d.tri_rule = o.tri_rule;
+    d.max_rhs = 1;
+    EBH_TRY(ebh_op_create(e->mesh, &d, &e->V64));
+    if (o.mixed) {
+      d.far_precision = EBH_FP32;
+      EBH_TRY(ebh_op_create(e->mesh, &d, &e->V32));
+    }
+    d.kernel = EBH_LAPLACE_DL;
+    d.trial_basis = EBH_P1;
+    d.far_precision = EBH_FP64;
+    EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
+    EBH_TRY(ebh_solver_create(e->V64, e->V32, o.restart, &e->solver));
+    // eager launches: stream capture must not overlap other threads'
+    // allocations and copies, which the other workers make at any time
+    EBH_TRY(ebh_solver_set(e->solver, 0, 0));
+    return EBH_OK;
+  }
+
+  int run(Request &r, std::unique_ptr<Entry> &e)
+  {
+    const auto t0 = Clock::now();
+    r.res.ne = r.ne;
+    if (!e) EBH_TRY(build(r, e));
+    // boundary data u = 1/|x - x0| and b = (1/2 M + K) u
+    std::vector<double> u(r.np), mu_(r.ne), ku(r.ne), b(r.ne);
+    for (int n = 0; n < r.np; n++) {
+      const double dx = r.nodes[3 * n] - r.x0[0], dy = r.nodes[3 * n + 1] - r.x0[1],
+                   dz = r.nodes[3 * n + 2] - r.x0[2];
+      u[n] = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (!std::isfinite(u[n])) {
+        ebh::set_error("source point lies on the surface");
+        return EBH_ERR_NONFINITE;
+      }
+    }
+    EBH_TRY(ebh_mass_apply(e->mesh, EBH_P0, u.data(), mu_.data()));
+    EBH_TRY(ebh_op_apply_host(e->K64, u.data(), ku.data(), 1));
+    for (int i = 0; i < r.ne; i++) b[i] = 0.5 * mu_[i] + ku[i];
+    ebh::DevBuf<double> db, dx;
+    EBH_TRY(db.upload(b));
+    std::vector<double> zero(r.ne, 0.0);
+    EBH_TRY(dx.upload(zero));
+    const auto t1 = Clock::now();
+    ebh_solve_opts so{};
+    so.restart = o.restart;
+    so.max_iters = o.max_iters;
+    so.tol = o.tol;
+    so.mixed = o.mixed;
+    so.inner_tol = 1e-5;
+    so.max_outer = 10;
+    ebh_solve_stats st{};
+    const int rc = ebh_solver_solve(e->solver, db.p, dx.p, &so, &st);
+    r.t.resize(r.ne);
+    EBH_HIP(hipMemcpy(r.t.data(), dx.p, r.ne * sizeof(double), hipMemcpyDeviceToHost));
+    const auto t2 = Clock::now();
+    r.res.iters = st.iters;
+    r.res.relres = st.relres;
+    r.res.setup_ms = ms_between(t0, t1);
+    r.res.solve_ms = ms_between(t1, t2);
+    return rc;
+  }
+};
+
+extern "C" {
+
+int ebh_server_create(const ebh_server_opts *opts, ebh_server **out)
+{
+  if (!out) { ebh::set_error("null argument"); return EBH_ERR_ARG; }
+  *out = nullptr;
+  ebh_server *s = new ebh_server();
+  if (opts) s->o = *opts;
+  ebh_server_opts &o = s->o;
+  if (o.max_slots <= 0) o.max_slots = 4;
+  if (o.restart <= 0) o.restart = 50;
+  if (o.tol <= 0) o.tol = 1e-8;
+  if (o.max_iters <= 0) o.max_iters = 2000;
+  if (!opts) o.mixed = 1;
+  if (o.line_rule <= 0) o.line_rule = 4;
+  if (o.tri_rule <= 0) o.tri_rule = 4;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    delete s;
+    ebh::set_error("no HIP device");
+    return EBH_ERR_HIP;
+  }
+  s->budget = o.hbm_budget ? o.hbm_budget : (size_t)(0.8 * fr);
+  for (int k = 0; k < o.max_slots; k++) s->workers.emplace_back([s] { s->worker(); });
+  *out = s;
+  return EBH_OK;
+}
+
+int ebh_server_submit(ebh_server *s, int np, const double *nodes, int ne,
+                      const int *elems, const double *x0, uint64_t mesh_key,
+                      int64_t *request_id)
+{
+  if (!s || !nodes || !elems || !x0 || !request_id || np < 3 || ne < 4) {
+    ebh::set_error("bad submit arguments");
+    return EBH_ERR_ARG;
+  }
+  auto r = std::make_shared<Request>();
+  r->np = np;
+  r->ne = ne;
+  r->nodes.assign(nodes, nodes + (size_t)3 * np);
+  r->elems.assign(elems, elems + (size_t)3 * ne);
+  for (int k = 0; k < 3; k++) r->x0[k] = x0[k];
+  r->key = mesh_key;
+  r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0);
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->footprint > s->budget) {
+      ebh::set_error("request needs about " + std::to_string(r->footprint) +
+                     " device bytes, more than the server budget " +
+                     std::to_string(s->budget));
+      return EBH_ERR_NOMEM;
+    }
+    r->id = s->next_id++;
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    s->n_sub++;
+  }
+  s->cv_work.notify_all();
+  *request_id = r->id;
+  return EBH_OK;
+}
+
+int ebh_server_wait(ebh_server *s, int64_t id, double timeout_s,
+                    ebh_request_result *res, double *t)
+{
+  if (!s) { ebh::set_error("null server"); return EBH_ERR_ARG; }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) { ebh::set_error("unknown request id"); return EBH_ERR_ARG; }
+  auto r = it->second;
+  auto pred = [&] { return r->done; };
+  if (timeout_s > 0) {
+    if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) {
+      ebh::set_error("timed out waiting for the request");
+      return EBH_ERR_STATE;
+    }
+  } else {
+    s->cv_done.wait(lk, pred);
+  }
+  if (res) *res = r->res;
+  if (t && r->res.status == EBH_OK) std::copy(r->t.begin(), r->t.end(), t);
+  if (r->res.status != EBH_OK) ebh::set_error(r->err);
+  s->reqs.erase(it);
+  return r->res.status;
+}
+
+int ebh_server_stats(ebh_server *s, int64_t *st)
+{
+  if (!s || !st) { ebh::set_error("null argument"); return EBH_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(s->mu);
+  st[0] = s->n_sub;
+  st[1] = s->n_done;
+  st[2] = s->n_fail;
+  st[3] = s->n_active;
+  st[4] = (int64_t)s->queue.size();
+  st[5] = (int64_t)s->held;
+  st[6] = (int64_t)s->peak;
+  st[7] = s->n_hits;
+  return EBH_OK;
+}
+
+void ebh_server_destroy(ebh_server *s)
+{
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  for (auto &w : s->workers) w.join();
+  s->idle.clear();
+  delete s;
+}
+
+}  // extern "C"

@@ -1,15 +1,24 @@
-// Device-resident GMRES(m) in FP64 and mixed-precision iterative refinement
-// (FP32 far-field operator inside, FP64 operator for the outer residual).
+// Device-resident GMRES(m) in FP64 for a batch of nb systems sharing one
+// operator, and mixed-precision iterative refinement (FP32 far-field operator
+// inside, FP64 operator for the outer residual).
 //
 // The reference has no Krylov solver: its examples call scipy's CG on the
 // GPU apply (python/examples/laplace3dDirichletGPU.py:64-81).  Here the
-// vectors, the Krylov basis and the operator stay on the device; only the
-// small Hessenberg column (<= restart+1 numbers) crosses to the host for the
-// Givens update and the convergence test.
-//
-// All reductions are deterministic: fixed per-block partial sums reduced in
-// a fixed order by one block.
+// operator, the Krylov bases, the Hessenberg matrices, the Givens rotations
+// and the residual estimates all stay on the device:
+//   * one Arnoldi step of all nb systems = one multi-RHS operator apply,
+//     classical Gram-Schmidt twice (batched dot / update kernels), a norm,
+//     a Givens update and a normalisation -- a fixed kernel sequence that is
+//     captured once per (operator, step j, nb) as a hipGraph and replayed;
+//   * the host reads the per-system convergence flags every `check` steps
+//     (one small copy), never per kernel;
+//   * systems that converge go inactive (their Krylov vectors are zeroed)
+//     while the others continue in lockstep.
+// Reductions are deterministic: fixed per-block partial sums, reduced in a
+// fixed order by one wave per output.
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "ebh_internal.h"
@@ -17,76 +26,181 @@
 namespace ebh {
 namespace {
 
-constexpr int kDotBlocks = 120;
+constexpr int kDotBlocks = 64;   // partial sums per dot product
 constexpr int kThreads = 256;
 
-// partial[b*k + j] = sum over block b's rows of V[j][i] * w[i]
-__global__ __launch_bounds__(kThreads) void k_dots(int n, int k,
-                                                   const double *__restrict__ V,
-                                                   const double *__restrict__ w,
-                                                   double *__restrict__ partial)
+// partial[((r * k) + j) * kDotBlocks + b] = sum over block b's rows of
+// V_j[r][i] * W[r][i];  V_j[r] = V0 + j*jstride + r*n, W[r] = W + r*n.
+__global__ __launch_bounds__(kThreads) void k_bdots(int n, int k,
+                                                    const double *__restrict__ V0,
+                                                    size_t jstride,
+                                                    const double *__restrict__ W,
+                                                    const int *__restrict__ active,
+                                                    double *__restrict__ partial)
 {
   __shared__ double red[kThreads / 64];
+  const int r = blockIdx.y;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double *w = W + (size_t)r * n;
+  const bool on = active[r] != 0;
   for (int j = 0; j < k; j++) {
-    const double *v = V + (size_t)j * n;
+    const double *v = V0 + j * jstride + (size_t)r * n;
     double s = 0.0;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
-      s = fma(v[i], w[i], s);
+    if (on)
+      for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+        s = fma(v[i], w[i], s);
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) red[wv] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (int q = 0; q < kThreads / 64; q++) t += red[q];
-      partial[(size_t)blockIdx.x * k + j] = t;
+      partial[((size_t)r * k + j) * kDotBlocks + blockIdx.x] = t;
     }
     __syncthreads();
   }
 }
 
-// out[j] = sum_b partial[b*k + j] (fixed order)
-__global__ void k_dots_finish(int nb, int k, const double *__restrict__ partial,
-                              double *__restrict__ out)
+// out[r*ldo + j] (= or +=) sum_b partial; one wave per (r, j).
+__global__ void k_bdots_finish(int k, int nb, const double *__restrict__ partial,
+                               double *__restrict__ out, int ldo, int add)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; b++) s += partial[(size_t)b * k + j];
-  out[j] = s;
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wid >= k * nb) return;
+  const int r = wid / k, j = wid - r * k;
+  double s = lane < kDotBlocks ? partial[(size_t)wid * kDotBlocks + lane] : 0.0;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) {
+    double *o = out + (size_t)r * ldo + j;
+    *o = add ? *o + s : s;
+  }
 }
 
-// w[i] += alpha * sum_j h[j] V[j][i]
-__global__ void k_gemv_n(int n, int k, const double *__restrict__ V,
-                         const double *__restrict__ h, double alpha,
-                         double *__restrict__ w)
+// W[r][i] += alpha * sum_j C[r*ldc + j] V_j[r][i], for j < kr (kr per system
+// from kvec when given, else k).
+__global__ void k_bgemv(int n, int k, const int *__restrict__ kvec,
+                        const double *__restrict__ V0, size_t jstride,
+                        const double *__restrict__ C, int ldc, double alpha,
+                        double *__restrict__ W, const int *__restrict__ active)
 {
+  const int r = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !active[r]) return;
+  const int kr = kvec ? kvec[r] : k;
+  const double *c = C + (size_t)r * ldc;
+  const double *v = V0 + (size_t)r * n + i;
+  double s = 0.0;
+  for (int j = 0; j < kr; j++) s = fma(c[j], v[j * jstride], s);
+  W[(size_t)r * n + i] = fma(alpha, s, W[(size_t)r * n + i]);
+}
+
+// One Arnoldi step's small work for every system: h = hc[r][0..j] (after
+// CGS2), hc[r][j+1] = ||w||^2.  Applies the stored rotations, builds the new
+// one, updates g and the residual estimate, stores the column of R, sets the
+// normalisation factor of w and retires converged systems.
+__global__ void k_bgivens(int nb, int m, int j, double *__restrict__ hc,
+                          double *__restrict__ Rm, double *__restrict__ cs,
+                          double *__restrict__ sn, double *__restrict__ g,
+                          double *__restrict__ est, const double *__restrict__ atol,
+                          double *__restrict__ scale, int *__restrict__ active,
+                          int *__restrict__ jdone)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb) return;
+  if (!active[r]) { scale[r] = 0.0; return; }
+  double *h = hc + (size_t)r * (m + 2);
+  const double hn = sqrt(fmax(h[j + 1], 0.0));
+  h[j + 1] = hn;
+  double *c = cs + (size_t)r * m, *s = sn + (size_t)r * m, *gg = g + (size_t)r * (m + 1);
+  for (int k = 0; k < j; k++) {
+    const double a = h[k], b = h[k + 1];
+    h[k] = c[k] * a + s[k] * b;
+    h[k + 1] = -s[k] * a + c[k] * b;
+  }
+  const double a = h[j], b = h[j + 1];
+  const double rr = hypot(a, b);
+  c[j] = rr > 0.0 ? a / rr : 1.0;
+  s[j] = rr > 0.0 ? b / rr : 0.0;
+  h[j] = rr;
+  gg[j + 1] = -s[j] * gg[j];
+  gg[j] = c[j] * gg[j];
+  double *col = Rm + (size_t)r * (m + 1) * m;
+  for (int k = 0; k <= j; k++) col[(size_t)k * m + j] = h[k];
+  est[r] = fabs(gg[j + 1]);
+  scale[r] = hn > 0.0 ? 1.0 / hn : 0.0;
+  jdone[r] = j + 1;
+  if (est[r] <= atol[r] || !(hn > 0.0)) active[r] = 0;
+}
+
+// dst[r] = (src ? src[r] : dst[r]) * scale[r]
+__global__ void k_bscale(int n, const double *__restrict__ src,
+                         double *__restrict__ dst, const double *__restrict__ scale)
+{
+  const int r = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
-  for (int j = 0; j < k; j++) s = fma(h[j], V[(size_t)j * n + i], s);
-  w[i] = fma(alpha, s, w[i]);
+  const double f = scale[r];
+  dst[(size_t)r * n + i] = (src ? src[(size_t)r * n + i] : dst[(size_t)r * n + i]) * f;
 }
 
-// y[i] = a*x[i] + b*y[i]
-__global__ void k_axpby(int n, double a, const double *__restrict__ x, double b,
+// Back substitution R y = g for each system over its jdone[r] columns.
+__global__ void k_bbacksolve(int nb, int m, const double *__restrict__ Rm,
+                             const double *__restrict__ g, const int *__restrict__ jdone,
+                             double *__restrict__ y)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb) return;
+  const int jn = jdone[r];
+  const double *R = Rm + (size_t)r * (m + 1) * m;
+  const double *gg = g + (size_t)r * (m + 1);
+  double *yy = y + (size_t)r * m;
+  for (int k = jn - 1; k >= 0; k--) {
+    double s = gg[k];
+    for (int l = k + 1; l < jn; l++) s -= R[(size_t)k * m + l] * yy[l];
+    yy[k] = s / R[(size_t)k * m + k];
+  }
+}
+
+// y = a*x + b*y over n entries
+__global__ void k_axpby(size_t n, double a, const double *__restrict__ x, double b,
                         double *__restrict__ y)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
-__global__ void k_fill(int n, double v, double *__restrict__ y)
+__global__ void k_copy(size_t n, const double *__restrict__ x, double *__restrict__ y)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+__global__ void k_fill(size_t n, double v, double *__restrict__ y)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = v;
 }
 
-__global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y)
+// Restart set-up from beta2 = ||r||^2 per system.
+__global__ void k_brestart(int nb, int m, const double *__restrict__ beta2,
+                           const double *__restrict__ atol, double *__restrict__ g,
+                           double *__restrict__ est, double *__restrict__ scale,
+                           int *__restrict__ active, int *__restrict__ jdone)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = x[i];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nb) return;
+  const double b = sqrt(fmax(beta2[r], 0.0));
+  double *gg = g + (size_t)r * (m + 1);
+  for (int k = 0; k <= m; k++) gg[k] = 0.0;
+  gg[0] = b;
+  est[r] = b;
+  jdone[r] = 0;
+  active[r] = b > atol[r] ? 1 : 0;
+  scale[r] = b > 0.0 ? 1.0 / b : 0.0;
 }
+
+inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
 }  // namespace ebh
@@ -95,142 +209,319 @@ using namespace ebh;
 
 struct ebh_solver {
   ebh_op *a64 = nullptr, *a32 = nullptr;
-  int n = 0, m = 50;
+  int n = 0, m = 50, nbmax = 1;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  DevBuf<double> V;         // (m+1) x n Krylov basis
-  DevBuf<double> w, r, d, t;
-  DevBuf<double> partial, hd;
-  std::vector<double> hh;   // host staging
+  DevBuf<double> V;                  // (m+1) x nbmax x n
+  DevBuf<double> Rv, D, X2;          // nbmax x n
+  DevBuf<double> hc, Rm, cs, sn, g, est, atol, scale, yv, beta2, partial;
+  DevBuf<int> active, jdone, allon;
+  double *h_est = nullptr;           // pinned host mirrors
+  int *h_active = nullptr, *h_jdone = nullptr;
+  std::map<std::tuple<const ebh_op *, int, int, int>, hipGraphExec_t> graphs;
+  bool use_graphs = true;
+  int check = 4;
   int matvec64 = 0, matvec32 = 0;
 
-  int dots(int k, const double *Vp, const double *wp, double *out)
+  ~ebh_solver()
   {
-    hipLaunchKernelGGL(k_dots, dim3(kDotBlocks), dim3(kThreads), 0, stream, n, k,
-                       Vp, wp, partial.p);
-    hipLaunchKernelGGL(k_dots_finish, dim3((k + 63) / 64), dim3(64), 0, stream,
-                       kDotBlocks, k, partial.p, hd.p);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (h_est) (void)hipHostFree(h_est);
+    if (h_active) (void)hipHostFree(h_active);
+    if (h_jdone) (void)hipHostFree(h_jdone);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+
+  size_t vstride() const { return (size_t)nbmax * n; }
+
+  int apply(ebh_op *op, const double *x, double *y, int nb)
+  {
+    EBH_TRY(op_apply_multi(op, x, y, nb, stream, false));
+    if (op == a64) matvec64 += nb; else matvec32 += nb;
+    return EBH_OK;
+  }
+
+  // dots of W[r] against k vectors V0 + j*jstride, into out[r*ldo + j]
+  int bdots(int nb, int k, const double *V0, size_t jstride, const double *W,
+            const int *act, double *out, int ldo, int add)
+  {
+    hipLaunchKernelGGL(k_bdots, dim3(kDotBlocks, nb), dim3(kThreads), 0, stream, n, k,
+                       V0, jstride, W, act, partial.p);
+    const int waves = k * nb;
+    hipLaunchKernelGGL(k_bdots_finish, dim3((waves + 3) / 4), dim3(256), 0, stream, k,
+                       nb, partial.p, out, ldo, add);
     EBH_HIP(hipGetLastError());
-    EBH_HIP(hipMemcpyAsync(out, hd.p, k * sizeof(double), hipMemcpyDeviceToHost,
+    return EBH_OK;
+  }
+
+  // second CGS coefficients live after the first in hc
+  double *h2() { return hc.p + (size_t)nbmax * (m + 2); }
+
+  // One Arnoldi step j for nb systems (kernels only; graph-capturable).
+  int step_kernels(ebh_op *A, int j, int nb)
+  {
+    const size_t vs = vstride();
+    double *vj = V.p + (size_t)j * vs, *w = V.p + (size_t)(j + 1) * vs;
+    EBH_TRY(op_apply_multi(A, vj, w, nb, stream, false));
+    const int ld = m + 2;
+    const dim3 gv((n + 255) / 256, nb);
+    // classical Gram-Schmidt, twice
+    EBH_TRY(bdots(nb, j + 1, V.p, vs, w, active.p, hc.p, ld, 0));
+    hipLaunchKernelGGL(k_bgemv, gv, dim3(256), 0, stream, n, j + 1, (const int *)nullptr,
+                       V.p, vs, hc.p, ld, -1.0, w, active.p);
+    EBH_TRY(bdots(nb, j + 1, V.p, vs, w, active.p, h2(), ld, 0));
+    hipLaunchKernelGGL(k_bgemv, gv, dim3(256), 0, stream, n, j + 1, (const int *)nullptr,
+                       V.p, vs, h2(), ld, -1.0, w, active.p);
+    hipLaunchKernelGGL(k_axpby, grid1((size_t)nb * ld), dim3(256), 0, stream,
+                       (size_t)nb * ld, 1.0, h2(), 1.0, hc.p);
+    // ||w||^2 into hc[r][j+1]
+    EBH_TRY(bdots(nb, 1, w, 0, w, active.p, hc.p + j + 1, ld, 0));
+    hipLaunchKernelGGL(k_bgivens, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, m, j,
+                       hc.p, Rm.p, cs.p, sn.p, g.p, est.p, atol.p, scale.p, active.p,
+                       jdone.p);
+    hipLaunchKernelGGL(k_bscale, gv, dim3(256), 0, stream, n, (const double *)nullptr, w,
+                       scale.p);
+    EBH_HIP(hipGetLastError());
+    return EBH_OK;
+  }
+
+  int step(ebh_op *A, int j, int nb)
+  {
+    // per-launch event profiling of the operator needs eager launches
+    if (!use_graphs || A->prof_on) {
+      EBH_TRY(step_kernels(A, j, nb));
+    } else {
+      auto key = std::make_tuple((const ebh_op *)A, j, nb, A->gen);
+      auto it = graphs.find(key);
+      if (it == graphs.end()) {
+        hipGraph_t gr;
+        EBH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int rc = step_kernels(A, j, nb);
+        const hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc != EBH_OK) return rc;
+        if (e != hipSuccess) {
+          set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+          return EBH_ERR_HIP;
+        }
+        hipGraphExec_t ex;
+        const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e2 != hipSuccess) {
+          set_error(std::string("graph instantiate failed: ") + hipGetErrorString(e2));
+          return EBH_ERR_HIP;
+        }
+        it = graphs.emplace(key, ex).first;
+      }
+      EBH_HIP(hipGraphLaunch(it->second, stream));
+    }
+    if (A == a64) matvec64 += nb; else matvec32 += nb;
+    return EBH_OK;
+  }
+
+  int read_state(int nb)
+  {
+    EBH_HIP(hipMemcpyAsync(h_active, active.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                           stream));
+    EBH_HIP(hipMemcpyAsync(h_est, est.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                           stream));
+    EBH_HIP(hipMemcpyAsync(h_jdone, jdone.p, nb * sizeof(int), hipMemcpyDeviceToHost,
                            stream));
     EBH_HIP(hipStreamSynchronize(stream));
     return EBH_OK;
   }
-  int norm(const double *x, double *out)
+
+  // Rv = B - A Z (Z zero when z_zero), beta2 = ||Rv||^2 per system
+  int residual(ebh_op *A, const double *B, const double *Z, bool z_zero, int nb)
   {
-    double v;
-    EBH_TRY(dots(1, x, x, &v));
-    *out = std::sqrt(v);
-    return EBH_OK;
-  }
-  int gemv_n(int k, const double *h_host, double alpha, double *wp)
-  {
-    EBH_HIP(hipMemcpyAsync(hd.p, h_host, k * sizeof(double), hipMemcpyHostToDevice,
-                           stream));
-    hipLaunchKernelGGL(k_gemv_n, dim3((n + 255) / 256), dim3(256), 0, stream, n, k,
-                       V.p, hd.p, alpha, wp);
-    EBH_HIP(hipGetLastError());
-    // hd is reused by the next dots(): keep the copy ordered before it
-    return EBH_OK;
-  }
-  int axpby(double a, const double *x, double b, double *y)
-  {
-    hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, stream, n, a,
-                       x, b, y);
-    EBH_HIP(hipGetLastError());
-    return EBH_OK;
-  }
-  int apply(ebh_op *op, const double *x, double *y)
-  {
-    EBH_TRY(op_apply_one(op, x, y, stream, false));
-    if (op == a64) matvec64++; else matvec32++;
-    return EBH_OK;
+    const size_t tot = (size_t)nb * n;
+    if (z_zero) {
+      hipLaunchKernelGGL(k_copy, grid1(tot), dim3(256), 0, stream, tot, B, Rv.p);
+    } else {
+      EBH_TRY(apply(A, Z, Rv.p, nb));
+      hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, stream, tot, 1.0, B, -1.0,
+                         Rv.p);
+    }
+    return bdots(nb, 1, Rv.p, 0, Rv.p, allon.p, beta2.p, 1, 0);
   }
 
-  // GMRES(m) on A z = rhs from z (in/out).  Stops when the Arnoldi residual
-  // estimate drops below atol, or after maxit Arnoldi steps.
-  int gmres(ebh_op *A, const double *rhs, double *z, bool z_zero,
-            bool true_residual, double atol, int maxit, int *iters, double *est)
+  // GMRES(m) on A Z = B for nb systems with absolute tolerances atol_h;
+  // iters receives the Arnoldi steps per system.
+  int gmres(ebh_op *A, const double *B, double *Z, bool z_zero, int nb,
+            const std::vector<double> &atol_h, int maxit, std::vector<int> &iters)
   {
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1),
-        h2(m + 1), y(m);
-    int it = 0;
-    double beta = 0.0;
+    EBH_HIP(hipMemcpyAsync(atol.p, atol_h.data(), nb * sizeof(double),
+                           hipMemcpyHostToDevice, stream));
+    EBH_HIP(hipStreamSynchronize(stream));
+    iters.assign(nb, 0);
+    int total = 0;
     while (true) {
-      // r = rhs - A z
-      if (z_zero) {
-        hipLaunchKernelGGL(k_copy, dim3((n + 255) / 256), dim3(256), 0, stream,
-                           n, rhs, r.p);
-        z_zero = false;
-      } else {
-        EBH_TRY(apply(A, z, r.p));
-        EBH_TRY(axpby(1.0, rhs, -1.0, r.p));
-      }
-      EBH_TRY(norm(r.p, &beta));
-      *est = beta;
-      if (beta <= atol || it >= maxit || !(beta > 0.0)) break;
-      hipLaunchKernelGGL(k_copy, dim3((n + 255) / 256), dim3(256), 0, stream, n,
-                         r.p, V.p);
-      EBH_TRY(axpby(0.0, r.p, 1.0 / beta, V.p));
-      std::fill(g.begin(), g.end(), 0.0);
-      g[0] = beta;
-      int j = 0;
-      for (; j < m && it < maxit; j++, it++) {
-        double *vj1 = V.p + (size_t)(j + 1) * n;
-        EBH_TRY(apply(A, V.p + (size_t)j * n, vj1));
-        // classical Gram-Schmidt, twice
-        EBH_TRY(dots(j + 1, V.p, vj1, h.data()));
-        EBH_TRY(gemv_n(j + 1, h.data(), -1.0, vj1));
-        EBH_TRY(dots(j + 1, V.p, vj1, h2.data()));
-        EBH_TRY(gemv_n(j + 1, h2.data(), -1.0, vj1));
-        for (int k = 0; k <= j; k++) h[k] += h2[k];
-        double hn;
-        EBH_TRY(norm(vj1, &hn));
-        h[j + 1] = hn;
-        if (hn > 0.0) EBH_TRY(axpby(0.0, vj1, 1.0 / hn, vj1));
-        // Givens rotations on the new column
-        for (int k = 0; k < j; k++) {
-          const double a = h[k], b = h[k + 1];
-          h[k] = cs[k] * a + sn[k] * b;
-          h[k + 1] = -sn[k] * a + cs[k] * b;
+      EBH_TRY(residual(A, B, Z, z_zero, nb));
+      z_zero = false;
+      hipLaunchKernelGGL(k_brestart, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, m,
+                         beta2.p, atol.p, g.p, est.p, scale.p, active.p, jdone.p);
+      hipLaunchKernelGGL(k_bscale, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n,
+                         Rv.p, V.p, scale.p);
+      EBH_HIP(hipGetLastError());
+      EBH_TRY(read_state(nb));
+      bool any = false;
+      for (int r = 0; r < nb; r++) any |= h_active[r] != 0;
+      if (!any || total >= maxit) break;
+      for (int j = 0; j < m && total < maxit; j++) {
+        EBH_TRY(step(A, j, nb));
+        total++;
+        if ((j + 1) % check == 0 || j + 1 == m) {
+          EBH_TRY(read_state(nb));
+          bool on = false;
+          for (int r = 0; r < nb; r++) on |= h_active[r] != 0;
+          if (!on) break;
         }
-        const double a = h[j], b = h[j + 1];
-        const double rr = std::hypot(a, b);
-        cs[j] = rr > 0.0 ? a / rr : 1.0;
-        sn[j] = rr > 0.0 ? b / rr : 0.0;
-        h[j] = rr;
-        h[j + 1] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = h[k];
-        *est = std::fabs(g[j + 1]);
-        if (*est <= atol || hn == 0.0) { j++; it++; break; }
       }
-      // back substitution and update z += V_j y
-      for (int k = j - 1; k >= 0; k--) {
-        double s = g[k];
-        for (int l = k + 1; l < j; l++) s -= H[(size_t)k * m + l] * y[l];
-        y[k] = s / H[(size_t)k * m + k];
+      EBH_TRY(read_state(nb));
+      bool all_met = true;
+      for (int r = 0; r < nb; r++) {
+        iters[r] += h_jdone[r];
+        all_met &= h_est[r] <= atol_h[r];
       }
-      EBH_TRY(gemv_n(j, y.data(), 1.0, z));
-      if (*est <= atol || it >= maxit) {
-        if (!true_residual) break;
-        // true residual of the returned iterate
-        EBH_TRY(apply(A, z, r.p));
-        EBH_TRY(axpby(1.0, rhs, -1.0, r.p));
-        EBH_TRY(norm(r.p, &beta));
-        *est = beta;
-        break;
-      }
+      // Z += V y with each system's own number of steps
+      hipLaunchKernelGGL(k_bbacksolve, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, m,
+                         Rm.p, g.p, jdone.p, yv.p);
+      hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
+                         jdone.p, V.p, vstride(), yv.p, m, 1.0, Z, allon.p);
+      EBH_HIP(hipGetLastError());
+      // the Arnoldi estimate met every tolerance: no restart residual needed
+      // (callers that need the true residual compute it themselves)
+      if (all_met || total >= maxit) break;
     }
-    *iters = it;
     return EBH_OK;
   }
 };
 
+namespace {
+
+int read_norms(ebh_solver *s, int nb, std::vector<double> &out)
+{
+  out.resize(nb);
+  EBH_HIP(hipMemcpyAsync(out.data(), s->beta2.p, nb * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream));
+  EBH_HIP(hipStreamSynchronize(s->stream));
+  for (double &v : out) v = std::sqrt(std::fmax(v, 0.0));
+  return EBH_OK;
+}
+
+int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
+               const ebh_solve_opts *o, ebh_solve_stats *st)
+{
+  ebh_solve_opts opt = *o;
+  if (opt.tol <= 0) opt.tol = 1e-8;
+  if (opt.max_iters <= 0) opt.max_iters = 1000;
+  if (opt.inner_tol <= 0) opt.inner_tol = 1e-5;
+  if (opt.max_outer <= 0) opt.max_outer = 10;
+  if (opt.mixed && !s->a32) {
+    set_error("mixed-precision solve needs an FP32 operator");
+    return EBH_ERR_ARG;
+  }
+  if (nb < 1 || nb > s->nbmax) {
+    set_error("batch size must be 1.." + std::to_string(s->nbmax));
+    return EBH_ERR_ARG;
+  }
+  s->matvec64 = s->matvec32 = 0;
+  EBH_HIP(hipEventRecord(s->e0, s->stream));
+  const size_t tot = (size_t)nb * s->n;
+  std::vector<double> bn;
+  EBH_TRY(s->bdots(nb, 1, b, 0, b, s->allon.p, s->beta2.p, 1, 0));
+  EBH_TRY(read_norms(s, nb, bn));
+  for (double v : bn)
+    if (!std::isfinite(v)) { set_error("non-finite right-hand side"); return EBH_ERR_NONFINITE; }
+  std::vector<int> iters(nb, 0), it;
+  int outer = 0;
+  // a zero initial guess needs no operator apply for its residual
+  std::vector<double> xn;
+  EBH_TRY(s->bdots(nb, 1, x, 0, x, s->allon.p, s->beta2.p, 1, 0));
+  EBH_TRY(read_norms(s, nb, xn));
+  bool x_zero = true;
+  for (double v : xn) x_zero &= v == 0.0;
+  std::vector<double> rn;   // FP64 residual norms of the returned iterates
+  if (!opt.mixed) {
+    std::vector<double> atol(nb);
+    for (int r = 0; r < nb; r++) atol[r] = opt.tol * bn[r];
+    EBH_TRY(s->gmres(s->a64, b, x, x_zero, nb, atol, opt.max_iters, iters));
+    // true FP64 residual of the returned iterates
+    EBH_TRY(s->apply(s->a64, x, s->D.p, nb));
+    hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, s->stream, tot, 1.0, b, -1.0,
+                       s->D.p);
+    EBH_TRY(s->bdots(nb, 1, s->D.p, 0, s->D.p, s->allon.p, s->beta2.p, 1, 0));
+    EBH_TRY(read_norms(s, nb, rn));
+  } else {
+    int total = 0;
+    while (true) {
+      // FP64 residual of the current iterates into D (the loop always ends
+      // right after this, so rn belongs to the returned iterates)
+      if (x_zero) {
+        hipLaunchKernelGGL(k_copy, grid1(tot), dim3(256), 0, s->stream, tot, b, s->D.p);
+        x_zero = false;
+      } else {
+        EBH_TRY(s->apply(s->a64, x, s->D.p, nb));
+        hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, s->stream, tot, 1.0, b,
+                           -1.0, s->D.p);
+      }
+      EBH_TRY(s->bdots(nb, 1, s->D.p, 0, s->D.p, s->allon.p, s->beta2.p, 1, 0));
+      EBH_TRY(read_norms(s, nb, rn));
+      bool all = true;
+      std::vector<double> atol(nb);
+      for (int r = 0; r < nb; r++) {
+        const double rel = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
+        all &= rel <= opt.tol;
+        // systems already at the target get a tolerance their residual meets
+        const double itol =
+            rel <= opt.tol ? 2.0 : std::max(opt.inner_tol, 0.5 * opt.tol / rel);
+        atol[r] = itol * rn[r];
+      }
+      if (all || outer >= opt.max_outer || total >= opt.max_iters) break;
+      hipLaunchKernelGGL(k_fill, grid1(tot), dim3(256), 0, s->stream, tot, 0.0, s->X2.p);
+      EBH_TRY(s->gmres(s->a32, s->D.p, s->X2.p, true, nb, atol, opt.max_iters - total, it));
+      int mx = 0;
+      for (int r = 0; r < nb; r++) { iters[r] += it[r]; mx = std::max(mx, it[r]); }
+      total += mx;
+      hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, s->stream, tot, 1.0, s->X2.p,
+                         1.0, x);
+      outer++;
+    }
+  }
+  EBH_HIP(hipEventRecord(s->e1, s->stream));
+  EBH_HIP(hipEventSynchronize(s->e1));
+  float ms = 0.f;
+  EBH_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
+  bool conv = true;
+  double worst = 0.0;
+  for (int r = 0; r < nb; r++) {
+    const double rel = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
+    worst = std::max(worst, rel);
+    conv &= rel <= opt.tol;
+    if (st) {
+      st[r].iters = iters[r];
+      st[r].outer = outer;
+      st[r].matvec64 = s->matvec64 / nb;
+      st[r].matvec32 = s->matvec32 / nb;
+      st[r].relres = rel;
+      st[r].ms = ms;
+      st[r].converged = rel <= opt.tol;
+    }
+  }
+  if (!conv) {
+    set_error("solver stopped at relative residual " + std::to_string(worst));
+    return EBH_ERR_NOCONV;
+  }
+  return EBH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
-int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out)
+int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart, int max_batch,
+                              ebh_solver **out)
 {
   if (!a64 || !out) { set_error("null argument"); return EBH_ERR_ARG; }
   *out = nullptr;
@@ -244,111 +535,87 @@ int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out)
     set_error("a32 and a64 differ in shape");
     return EBH_ERR_ARG;
   }
+  if (max_batch < 1) max_batch = 1;
+  if (max_batch > a64->d.max_rhs || (a32 && max_batch > a32->d.max_rhs)) {
+    set_error("batch size exceeds the operators' max_rhs");
+    return EBH_ERR_ARG;
+  }
   if (restart < 1) restart = 50;
   ebh_solver *s = new ebh_solver();
   s->a64 = a64;
   s->a32 = a32;
   s->n = n;
   s->m = restart;
+  s->nbmax = max_batch;
   s->stream = a64->stream;
+  // convergence polling interval: every step when a step is expensive
+  // (large meshes), every 4th when launch latency dominates
+  s->check = a64->mesh->T.ne >= 4096 ? 1 : 4;
+  const size_t nb = max_batch, m = restart;
   int rc = EBH_OK;
-  if ((rc = s->V.alloc((size_t)(restart + 1) * n)) || (rc = s->w.alloc(n)) ||
-      (rc = s->r.alloc(n)) || (rc = s->d.alloc(n)) || (rc = s->t.alloc(n)) ||
-      (rc = s->partial.alloc((size_t)kDotBlocks * (restart + 2))) ||
-      (rc = s->hd.alloc(restart + 2))) {
+  if ((rc = s->V.alloc((m + 1) * nb * n)) || (rc = s->Rv.alloc(nb * n)) ||
+      (rc = s->D.alloc(nb * n)) || (rc = s->X2.alloc(nb * n)) ||
+      (rc = s->hc.alloc(2 * nb * (m + 2))) || (rc = s->Rm.alloc(nb * (m + 1) * m)) ||
+      (rc = s->cs.alloc(nb * m)) || (rc = s->sn.alloc(nb * m)) ||
+      (rc = s->g.alloc(nb * (m + 1))) || (rc = s->est.alloc(nb)) ||
+      (rc = s->atol.alloc(nb)) || (rc = s->scale.alloc(nb)) ||
+      (rc = s->yv.alloc(nb * m)) || (rc = s->beta2.alloc(nb)) ||
+      (rc = s->partial.alloc((size_t)kDotBlocks * nb * (m + 2))) ||
+      (rc = s->active.alloc(nb)) || (rc = s->jdone.alloc(nb)) ||
+      (rc = s->allon.alloc(nb))) {
     delete s;
     return rc;
   }
-  if (hipEventCreate(&s->e0) != hipSuccess || hipEventCreate(&s->e1) != hipSuccess) {
-    ebh_solver_destroy(s);
-    set_error("hipEventCreate failed");
+  std::vector<int> ones(nb, 1);
+  if ((rc = s->allon.upload(ones))) { delete s; return rc; }
+  // defined contents for the Hessenberg scratch (entries past the current
+  // step are carried along unused)
+  if (hipMemsetAsync(s->hc.p, 0, s->hc.bytes(), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->V.p, 0, s->V.bytes(), s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    delete s;
+    set_error("solver initialisation failed");
+    return EBH_ERR_HIP;
+  }
+  if (hipHostMalloc((void **)&s->h_est, nb * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&s->h_active, nb * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&s->h_jdone, nb * sizeof(int)) != hipSuccess ||
+      hipEventCreate(&s->e0) != hipSuccess || hipEventCreate(&s->e1) != hipSuccess) {
+    delete s;
+    set_error("solver host/event allocation failed");
     return EBH_ERR_HIP;
   }
   *out = s;
   return EBH_OK;
 }
 
-int ebh_solver_solve(ebh_solver *s, const double *b, double *x,
-                     const ebh_solve_opts *o, ebh_solve_stats *st)
+int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out)
 {
-  if (!s || !b || !x || !o) { set_error("null argument"); return EBH_ERR_ARG; }
-  ebh_solve_opts opt = *o;
-  if (opt.tol <= 0) opt.tol = 1e-8;
-  if (opt.max_iters <= 0) opt.max_iters = 1000;
-  if (opt.inner_tol <= 0) opt.inner_tol = 1e-5;
-  if (opt.max_outer <= 0) opt.max_outer = 10;
-  if (opt.mixed && !s->a32) {
-    set_error("mixed-precision solve needs an FP32 operator");
-    return EBH_ERR_ARG;
-  }
-  s->matvec64 = s->matvec32 = 0;
-  EBH_HIP(hipEventRecord(s->e0, s->stream));
-  double bn;
-  EBH_TRY(s->norm(b, &bn));
-  if (!std::isfinite(bn)) { set_error("non-finite right-hand side"); return EBH_ERR_NONFINITE; }
-  ebh_solve_stats stats{};
-  double relres = 0.0;
-  if (bn == 0.0) {
-    hipLaunchKernelGGL(k_fill, dim3((s->n + 255) / 256), dim3(256), 0, s->stream,
-                       s->n, 0.0, x);
-    stats.converged = 1;
-  } else if (!opt.mixed) {
-    int it = 0;
-    double est = 0.0;
-    EBH_TRY(s->gmres(s->a64, b, x, false, true, opt.tol * bn, opt.max_iters,
-                     &it, &est));
-    stats.iters = it;
-    relres = est / bn;
-    stats.converged = relres <= opt.tol;
-  } else {
-    int total = 0, outer = 0;
-    while (true) {
-      // FP64 residual of the current iterate
-      EBH_TRY(s->apply(s->a64, x, s->w.p));
-      EBH_TRY(s->axpby(1.0, b, -1.0, s->w.p));
-      double rn;
-      EBH_TRY(s->norm(s->w.p, &rn));
-      relres = rn / bn;
-      if (relres <= opt.tol || outer >= opt.max_outer || total >= opt.max_iters) break;
-      // inner FP32-operator solve of A d = r from d = 0
-      hipLaunchKernelGGL(k_fill, dim3((s->n + 255) / 256), dim3(256), 0,
-                         s->stream, s->n, 0.0, s->d.p);
-      int it = 0;
-      double est = 0.0;
-      // no need to solve further than the target demands
-      const double itol = std::max(opt.inner_tol, 0.5 * opt.tol / relres);
-      EBH_TRY(s->gmres(s->a32, s->w.p, s->d.p, true, false, itol * rn,
-                       opt.max_iters - total, &it, &est));
-      total += it;
-      EBH_TRY(s->axpby(1.0, s->d.p, 1.0, x));
-      outer++;
-    }
-    stats.iters = total;
-    stats.outer = outer;
-    stats.converged = relres <= opt.tol;
-  }
-  EBH_HIP(hipEventRecord(s->e1, s->stream));
-  EBH_HIP(hipEventSynchronize(s->e1));
-  float ms = 0.f;
-  EBH_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
-  stats.relres = relres;
-  stats.ms = ms;
-  stats.matvec64 = s->matvec64;
-  stats.matvec32 = s->matvec32;
-  if (st) *st = stats;
-  if (!stats.converged) {
-    set_error("solver stopped at relative residual " + std::to_string(relres));
-    return EBH_ERR_NOCONV;
-  }
+  return ebh_solver_create_batched(a64, a32, restart, 1, out);
+}
+
+int ebh_solver_set(ebh_solver *s, int use_graphs, int check_every)
+{
+  if (!s) { set_error("null solver"); return EBH_ERR_ARG; }
+  s->use_graphs = use_graphs != 0;
+  if (check_every > 0) s->check = check_every;
   return EBH_OK;
 }
 
-void ebh_solver_destroy(ebh_solver *s)
+int ebh_solver_solve(ebh_solver *s, const double *b, double *x,
+                     const ebh_solve_opts *opts, ebh_solve_stats *stats)
 {
-  if (!s) return;
-  if (s->e0) (void)hipEventDestroy(s->e0);
-  if (s->e1) (void)hipEventDestroy(s->e1);
-  delete s;
+  if (!s || !b || !x || !opts) { set_error("null argument"); return EBH_ERR_ARG; }
+  return solve_impl(s, 1, b, x, opts, stats);
 }
+
+int ebh_solver_solve_batched(ebh_solver *s, int nb, const double *b, double *x,
+                             const ebh_solve_opts *opts, ebh_solve_stats *stats)
+{
+  if (!s || !b || !x || !opts) { set_error("null argument"); return EBH_ERR_ARG; }
+  return solve_impl(s, nb, b, x, opts, stats);
+}
+
+void ebh_solver_destroy(ebh_solver *s) { delete s; }
 
 }  // extern "C"

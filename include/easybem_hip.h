@@ -172,12 +172,84 @@ typedef struct {
  * operator of the same shape (needed when opts->mixed).  Allocates its
  * Krylov basis once; one solver handle serves one solve at a time. */
 int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out);
+/* Batched variant: up to max_batch systems with the same operator solved
+ * together (one Krylov space per system, one multi-RHS apply per Arnoldi
+ * step); max_batch must not exceed the operators' max_rhs. */
+int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart,
+                              int max_batch, ebh_solver **out);
+/* use_graphs: capture each Arnoldi step as a hipGraph and replay it
+ * (default 1); check_every: read convergence flags every k steps (default:
+ * every step for meshes of 4096+ triangles, else every 4th).  A solver runs
+ * on its FP64 operator's stream.  Stream capture must not overlap HIP
+ * allocation or copy calls made by other host threads (the capture is
+ * invalidated), so multithreaded callers -- the solve server among them --
+ * set use_graphs = 0. */
+int ebh_solver_set(ebh_solver *s, int use_graphs, int check_every);
+/* nb systems: b and x hold nb vectors of rows entries back to back; stats
+ * (if non-NULL) receives nb entries. */
+int ebh_solver_solve_batched(ebh_solver *s, int nb, const double *b,
+                             double *x, const ebh_solve_opts *opts,
+                             ebh_solve_stats *stats);
 /* b, x: device vectors of length rows; x holds the initial guess on entry.
  * Returns EBH_ERR_NOCONV (with x the last iterate) if the tolerance is not
  * reached within the caps. */
 int ebh_solver_solve(ebh_solver *s, const double *b, double *x,
                      const ebh_solve_opts *opts, ebh_solve_stats *stats);
 void ebh_solver_destroy(ebh_solver *s);
+
+/* ---- solve server (no reference counterpart) -------------------------- */
+/* Interior Dirichlet Laplace solves served as they arrive.  Each request
+ * brings a mesh and an exterior point source x0; the server forms
+ * u = 1/|x - x0| at the nodes, b = (1/2 M + K) u with the P0-test x
+ * P1-trial double layer and mass matrix, and solves V t = b for the P0
+ * Neumann trace t -- the formulation of the reference example
+ * python/examples/laplace3dDirichletGPU.py:60-81 -- by GMRES with
+ * mixed-precision refinement.
+ * Continuous batching: up to max_slots requests are on the device at once,
+ * each on its own HIP streams, so their work overlaps; a queued request is
+ * admitted as soon as a slot is free and its estimated device footprint
+ * (mesh, near-field CSR, far-field workspace, Krylov basis) fits in the
+ * remaining budget.  Meshes submitted with the same non-zero mesh_key stay
+ * resident (with their near-field blocks) and are reused. */
+typedef struct {
+  int max_slots;       /* concurrent requests on the device (default 4)     */
+  size_t hbm_budget;   /* bytes the server may hold; 0 = 80% of free HBM     */
+  int restart;         /* GMRES restart (default 50)                         */
+  double tol;          /* FP64 relative residual target (default 1e-8)       */
+  int mixed;           /* 1 = mixed-precision refinement (default), 0 = FP64 */
+  int line_rule;       /* near-field rule index (default 4)                  */
+  int tri_rule;        /* far-field rule index (default 4)                   */
+  int max_iters;       /* per request (default 2000)                         */
+} ebh_server_opts;
+
+typedef struct {
+  int status;          /* EBH_OK or the request's error code                 */
+  int iters;           /* Arnoldi steps                                      */
+  double relres;       /* final FP64 relative residual                       */
+  double queue_ms;     /* submit -> admitted                                 */
+  double setup_ms;     /* mesh / operators / right-hand side                 */
+  double solve_ms;     /* the Krylov solve                                   */
+  double total_ms;     /* submit -> done                                     */
+  int ne;              /* unknowns (triangles)                               */
+  size_t footprint;    /* estimated device bytes while admitted              */
+  int cache_hit;       /* 1 when the mesh was already resident               */
+} ebh_request_result;
+
+int ebh_server_create(const ebh_server_opts *opts, ebh_server **out);
+/* Copies the inputs and returns at once with a request id. */
+int ebh_server_submit(ebh_server *s, int np, const double *nodes, int ne,
+                      const int *elems, const double *x0, uint64_t mesh_key,
+                      int64_t *request_id);
+/* Blocks until the request finishes (timeout_s <= 0: no limit); t (ne
+ * entries, may be NULL) receives the Neumann trace.  Each id can be waited
+ * on once. */
+int ebh_server_wait(ebh_server *s, int64_t request_id, double timeout_s,
+                    ebh_request_result *res, double *t);
+/* [0] submitted, [1] completed, [2] failed, [3] active, [4] queued,
+ * [5] bytes admitted now, [6] peak bytes admitted, [7] cache hits. */
+int ebh_server_stats(ebh_server *s, int64_t *stats);
+/* Finishes queued work, then stops the workers. */
+void ebh_server_destroy(ebh_server *s);
 
 #ifdef __cplusplus
 }

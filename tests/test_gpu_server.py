@@ -1,0 +1,91 @@
+"""Solve server: continuous batching, admission by device footprint,
+per-request failure isolation, results identical alone and in a mix."""
+import numpy as np
+import pytest
+
+from easybem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def stream(n, seed=0, levels=(2, 3, 4)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        lev = levels[k % len(levels)]
+        P, T, x0 = meshes.perturbed_icosphere(lev, int(rng.integers(1 << 30)))
+        out.append((P, T, x0))
+    return out
+
+
+def test_server_result_matches_direct_solve():
+    from easybem_amd import _lib as L
+    P, T, x0 = stream(1, seed=5, levels=(3,))[0]
+    srv = L.Server(max_slots=1)
+    rid = srv.submit(P, T, x0)
+    res, t = srv.wait(rid)
+    srv.close()
+    assert res["status"] == 0 and res["relres"] <= 1e-8
+    m = L.Mesh(P, T)
+    V64 = L.Operator(m, "V", 0, 0, "fp64")
+    V32 = L.Operator(m, "V", 0, 0, "fp32")
+    K = L.Operator(m, "K", 0, 1, "fp64")
+    u = 1.0 / np.linalg.norm(P - x0, axis=1)
+    b = 0.5 * m.mass(u, 0) + K(u)
+    t2, st = L.Solver(V64, V32).solve(b, tol=1e-8, mixed=True)
+    assert np.array_equal(t, t2)
+
+
+def test_results_identical_alone_and_in_mix():
+    from easybem_amd import _lib as L
+    reqs = stream(9, seed=1)
+    alone = []
+    srv = L.Server(max_slots=1)
+    for P, T, x0 in reqs:
+        res, t = srv.wait(srv.submit(P, T, x0))
+        assert res["status"] == 0, res
+        alone.append(t)
+    srv.close()
+    srv = L.Server(max_slots=4)
+    ids = [srv.submit(P, T, x0) for P, T, x0 in reqs]
+    for rid, t0 in zip(ids, alone):
+        res, t = srv.wait(rid)
+        print(res)
+        assert res["status"] == 0 and res["relres"] <= 1e-8
+        assert np.array_equal(t, t0)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 9 and st["failed"] == 0
+
+
+def test_admission_respects_budget_and_bad_mesh_is_isolated():
+    from easybem_amd import _lib as L
+    reqs = stream(6, seed=2, levels=(3, 4))
+    P, T, x0 = reqs[0]
+    srv = L.Server(max_slots=4, hbm_budget=260 << 20)
+    ids = [srv.submit(P, T, x0) for P, T, x0 in reqs]
+    Pb, Tb, xb = reqs[1]
+    bad = srv.submit(Pb, Tb[:-1], xb)          # open surface
+    ids2 = [srv.submit(P, T, x0) for P, T, x0 in reqs[:2]]
+    res_bad, t_bad = srv.wait(bad)
+    assert res_bad["status"] == -2 and "closed 2-manifold" in res_bad["error"]
+    for rid in ids + ids2:
+        res, _ = srv.wait(rid)
+        assert res["status"] == 0 and res["relres"] <= 1e-8
+    st = srv.stats()
+    srv.close()
+    assert st["peak_bytes"] <= 260 << 20
+    assert st["failed"] == 1
+
+
+def test_resident_mesh_reused():
+    from easybem_amd import _lib as L
+    P, T, x0 = stream(1, seed=7, levels=(3,))[0]
+    srv = L.Server(max_slots=2)
+    r1, t1 = srv.wait(srv.submit(P, T, x0, key=42))
+    r2, t2 = srv.wait(srv.submit(P, T, x0 + 0.5, key=42))
+    r3, t3 = srv.wait(srv.submit(P, T, x0, key=42))
+    srv.close()
+    assert r1["cache_hit"] == 0 and r2["cache_hit"] == 1 and r3["cache_hit"] == 1
+    assert np.array_equal(t1, t3) and not np.array_equal(t1, t2)
+    assert r3["setup_ms"] < r1["setup_ms"]

@@ -74,3 +74,58 @@ def test_dirichlet_solution_converges_to_exact_flux():
         errs.append(np.sqrt(np.sum((t - tex) ** 2 * A) / np.sum(tex ** 2 * A)))
     print(errs)
     assert errs[1] < 0.6 * errs[0] and errs[2] < 0.6 * errs[1]
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_multi_rhs_apply_matches_single(basis, prec, tol):
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    one = L.Operator(m, "V", basis, basis, prec)
+    many = L.Operator(m, "V", basis, basis, prec, max_rhs=64)
+    n = one.ncols
+    rng = np.random.default_rng(3)
+    for nrhs in (2, 3, 8, 20, 64):
+        print(f"case basis={basis} {prec} nrhs={nrhs}", flush=True)
+        X = rng.uniform(-1, 1, (nrhs, n))
+        Y = many(X.ravel()).reshape(nrhs, n)
+        for r in range(nrhs):
+            ref = one(X[r])
+            assert np.linalg.norm(Y[r] - ref) / np.linalg.norm(ref) <= tol
+
+
+def test_batched_solve_matches_individual_solves():
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    V = L.Operator(m, "V", 0, 0, "fp64", max_rhs=8)
+    rng = np.random.default_rng(9)
+    B = rng.uniform(-1, 1, (5, T.shape[0]))
+    X, st = L.Solver(V, None, batch=8).solve(B.ravel(), tol=1e-9)
+    X = X.reshape(5, -1)
+    single = L.Solver(V, None, batch=1)
+    for r in range(5):
+        x, s1 = single.solve(B[r], tol=1e-9)
+        assert st[r]["converged"] and s1["converged"]
+        assert np.linalg.norm(X[r] - x) / np.linalg.norm(x) < 1e-7
+        assert np.linalg.norm(B[r] - V(X[r])) / np.linalg.norm(B[r]) <= 1e-9
+
+
+def test_graph_replays_match_eager_bitwise():
+    """Each Arnoldi step is a captured hipGraph; replays (interleaved with
+    eager work) must reproduce the eager solve exactly."""
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    V64 = L.Operator(m, "V", 0, 0, "fp64")
+    V32 = L.Operator(m, "V", 0, 0, "fp32")
+    b = np.random.default_rng(4).uniform(-1, 1, T.shape[0])
+    eager = L.Solver(V64, V32, graphs=False)
+    x0, s0 = eager.solve(b, tol=1e-8, mixed=True)
+    graph = L.Solver(V64, V32, graphs=True)
+    for rep in range(3):
+        x1, s1 = graph.solve(b, tol=1e-8, mixed=True)
+        assert s1["iters"] == s0["iters"] and s1["outer"] == s0["outer"]
+        assert np.array_equal(x1, x0), rep
+        V64(b)  # eager work between replays
