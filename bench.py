@@ -124,6 +124,63 @@ def secondary(L, meshes, solve_batched=False):
     return out
 
 
+def serving(L, meshes, n_requests=12, levels=(4, 5), rate=None, slots=4,
+            seed=20261203):
+    """Config 5: a stream of Dirichlet solves with Poisson arrivals.  Each
+    mesh is a seeded icosphere (level drawn from `levels`) with radius
+    1 + 0.1 sum_{l<=4} c_lm Y_lm and axes scaled by U(0.5, 1.5); the
+    boundary data comes from a point source outside the mesh.  Latency is
+    submit -> result."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_requests):
+        lev = int(rng.choice(levels))
+        P, T, x0 = meshes.perturbed_icosphere(lev, int(rng.integers(1 << 30)))
+        reqs.append((lev, P, T, x0))
+    srv = L.Server(max_slots=slots, tol=1e-8, mixed=True)
+    # warm the runtime (kernels loaded, streams created) on a small request
+    P, T, x0 = meshes.perturbed_icosphere(2, 1)
+    srv.wait(srv.submit(P, T, x0))
+    if rate is None:
+        # offered load about 0.7 of the single-stream service rate
+        rate = 0.7 / np.mean([{4: 0.08, 5: 0.45, 6: 5.0}[lv] for lv, *_ in reqs])
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    t0 = time.perf_counter()
+    ids, arrivals = [], []
+    t_next = t0
+    for (lev, P, T, x0), gap in zip(reqs, gaps):
+        t_next += gap
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        arrivals.append(time.perf_counter() - t0)
+        ids.append((lev, srv.submit(P, T, x0)))
+    res = []
+    for lev, rid in ids:
+        r, _ = srv.wait(rid)
+        r["level"] = lev
+        res.append(r)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in res if r["status"] == 0 and r["relres"] <= 1e-8]
+    lat = np.array([r["total_ms"] for r in ok])
+    by_level = {}
+    for lev in sorted(set(levels)):
+        v = np.array([r["total_ms"] for r in ok if r["level"] == lev])
+        if len(v):
+            by_level[f"L{lev}"] = {"n": int(len(v)), "p50_ms": float(np.percentile(v, 50)),
+                                   "max_ms": float(v.max())}
+    return {"requests": n_requests, "completed": len(ok), "levels": list(levels),
+            "arrival_rate_per_s": rate, "slots": slots,
+            "solves_per_s": len(ok) / wall,
+            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "mean_queue_ms": float(np.mean([r["queue_ms"] for r in ok])) if ok else None,
+            "by_level": by_level, "peak_device_bytes": st["peak_bytes"],
+            "max_relres": max(r["relres"] for r in ok) if ok else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +192,10 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--secondary", action="store_true",
                     help="also time configs 1, 3 and 4 (matvecs, batched solve)")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="serving stream of N requests on icosphere levels 4-6 "
+                         "(the default run serves 12 requests on levels 4-5)")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -248,6 +309,10 @@ def main():
         res["cpu_baseline"] = cpu_baseline(P, T, st64.get("matvec64", 1))
     if args.secondary:
         res["secondary"] = secondary(L, meshes, solve_batched=True)
+    if args.serve:
+        res["serving"] = serving(L, meshes, n_requests=args.serve, levels=(4, 5, 6))
+    elif not args.no_serve:
+        res["serving"] = serving(L, meshes)
     print(json.dumps(res))
 
 
