@@ -109,6 +109,8 @@ struct ebh_op {
   // far-field geometry
   ebh::DevBuf<float> tq32, sg32;     // target [nt][3][ne], source [ne*nt][3]
   ebh::DevBuf<float> tc32;           // FP32 double layer: centroids hi/lo [6][ne]
+  ebh::DevBuf<float> tsl32;          // single layer: centroid form [3+4nt][ne]
+  ebh::DevBuf<double> tsl64;
   ebh::DevBuf<double> tq64, sg64;
   ebh::DevBuf<double> tw, swt;       // w_q J per target / source point
   ebh::DevBuf<double> normals;       // ne*3 unit normals

@@ -65,6 +65,8 @@ struct FarArgs {
   const R *tq;                        // [nt][3][ne] (FP32 double layer: offsets
                                       // from the triangle centroid)
   const R *tc;                        // FP32 double layer: centroid hi/lo [6][ne]
+  const R *tsl;                       // single layer: [3 + 4 nt][ne] = centroid C,
+                                      // then per point (2(x-C), |x-C|^2)
   const R *src;                       // [ne*nt][REC]
   const int *touch_ptr, *touch;
   const double *tw;                   // [nt][ne]
@@ -98,12 +100,30 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   const int j0 = s * a.chunk;
   const int j1 = min(ne, j0 + a.chunk);
 
-  R X[NT], Y[NT], Z[NT];
+  // Single layer: r^2 = |D|^2 + (2 D.d_q + |d_q|^2) with D = C - y formed
+  // once per source point and d_q = x_q - C; double layer and Kelvin keep
+  // the points (x_q) themselves.
+  R X[NT], Y[NT], Z[NT], E[NT];
+  R Cx = 0, Cy = 0, Cz = 0;
+  if constexpr (KIND == 0) {
+    Cx = a.tsl[0 * (size_t)ne + tt];
+    Cy = a.tsl[1 * (size_t)ne + tt];
+    Cz = a.tsl[2 * (size_t)ne + tt];
 #pragma unroll
-  for (int q = 0; q < NT; q++) {
-    X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
-    Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
-    Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+    for (int q = 0; q < NT; q++) {
+      X[q] = a.tsl[(3 + 4 * q) * (size_t)ne + tt];
+      Y[q] = a.tsl[(4 + 4 * q) * (size_t)ne + tt];
+      Z[q] = a.tsl[(5 + 4 * q) * (size_t)ne + tt];
+      E[q] = a.tsl[(6 + 4 * q) * (size_t)ne + tt];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
+      Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
+      Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+      E[q] = 0;
+    }
   }
   R Ch[3] = {0, 0, 0}, Cl[3] = {0, 0, 0};
   if constexpr (COMP) {
@@ -151,11 +171,14 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
         const R sx = rec[0], sy = rec[1], sz = rec[2];
         if constexpr (KIND == 0) {
           const R sg = rec[3] * keep;
+          const R Dx = Cx - sx, Dy = Cy - sy, Dz = Cz - sz;
+          // an excluded (touching) triangle gets |D|^2 + 1: its strength is
+          // 0 and r^2 stays positive, also at a coincident point
+          const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
 #pragma unroll
           for (int q = 0; q < NT; q++) {
-            const R dx = X[q] - sx, dy = Y[q] - sy, dz = Z[q] - sz;
-            const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
-            acc[q] = fma(sg, rsq(r2), acc[q]);
+            const R B = fma(Dx, X[q], fma(Dy, Y[q], fma(Dz, Z[q], E[q])));
+            acc[q] = fma(sg, rsq(A + B), acc[q]);
           }
         } else if constexpr (COMP) {
           const R px = rec[6] * keep, py = rec[7] * keep, pz = rec[8] * keep;
@@ -514,10 +537,12 @@ int run_far(ebh_op *op, hipStream_t s)
   if constexpr (sizeof(R) == 4) {
     a.tq = op->tq32.p;
     a.tc = op->tc32.p;
+    a.tsl = op->tsl32.p;
     a.src = op->src32.p;
   } else {
     a.tq = op->tq64.p;
     a.tc = nullptr;
+    a.tsl = op->tsl64.p;
     a.src = op->src64.p;
   }
   a.touch_ptr = op->mesh->touch_ptr.p;
@@ -755,6 +780,32 @@ int op_build_far(ebh_op *op)
   } else {
     EBH_TRY(op->tq64.upload(tq));
     EBH_TRY(op->src64.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
+  }
+  if (op->d.kernel == EBH_LAPLACE_SL) {
+    // centroid form of the target points for the single layer
+    std::vector<double> tsl((size_t)(3 + 4 * nt) * ne);
+    for (int t = 0; t < ne; t++) {
+      double c[3];
+      for (int k = 0; k < 3; k++)
+        c[k] = (T.nodes[3 * T.tri[3 * t] + k] + T.nodes[3 * T.tri[3 * t + 1] + k] +
+                T.nodes[3 * T.tri[3 * t + 2] + k]) / 3.0;
+      for (int k = 0; k < 3; k++) tsl[(size_t)k * ne + t] = c[k];
+      for (int q = 0; q < nt; q++) {
+        double e = 0.0;
+        for (int k = 0; k < 3; k++) {
+          const double d = tq[(size_t)(3 * q + k) * ne + t] - c[k];
+          tsl[(size_t)(3 + 4 * q + k) * ne + t] = 2.0 * d;
+          e += d * d;
+        }
+        tsl[(size_t)(6 + 4 * q) * ne + t] = e;
+      }
+    }
+    if (op->d.far_precision == EBH_FP32) {
+      std::vector<float> t32(tsl.begin(), tsl.end());
+      EBH_TRY(op->tsl32.upload(t32));
+    } else {
+      EBH_TRY(op->tsl64.upload(tsl));
+    }
   }
   EBH_TRY(op->sg64.upload(sg));
   EBH_TRY(op->tw.upload(tw));
