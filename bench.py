@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
 FLOP_SL = 11               # per far interaction, single layer, one RHS
+# HBM bytes per launch of k_far<float,0,7> on the level-5 icosphere: two
+# rocprofv3 passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --pmc-probe`, each
+# scaled by the same counter on a 512 MiB device copy
+# (profiles/k_far_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5.
+TRAFFIC_FAR32_L5_BYTES = 58.08e6
 
 
 def log(*a):
@@ -181,6 +186,29 @@ def serving(L, meshes, n_requests=12, levels=(4, 5), rate=None, slots=4,
             "max_relres": max(r["relres"] for r in ok) if ok else None}
 
 
+def pmc_probe(L, meshes, level):
+    """Fixed workload for hardware-counter passes: a plain device copy of
+    512 MiB (the calibration of FETCH_SIZE / WRITE_SIZE) three times, then
+    three FP32-far P1 single-layer applies at the bench mesh."""
+    import torch
+    n = 64 << 20
+    src = torch.empty(n, dtype=torch.float64, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    P, T = meshes.icosphere(level)
+    mesh = L.Mesh(P, T)
+    V32 = L.Operator(mesh, "V", 1, 1, "fp32", line_rule=4, tri_rule=4)
+    x = L.DeviceArray.from_host(np.random.default_rng(1).uniform(-1, 1, V32.ncols))
+    y = L.DeviceArray(V32.nrows)
+    for _ in range(3):
+        V32.apply(x, y)
+    L.check(L.lib().ebh_synchronize())
+    print(json.dumps({"probe": "pmc", "copy_bytes_each_way": n * 8,
+                      "far_interactions": V32.work()["far_interactions"]}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +224,9 @@ def main():
                     help="serving stream of N requests on icosphere levels 4-6 "
                          "(the default run serves 12 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run only the fixed copy + apply workload profiled "
+                         "with hardware counters (no JSON bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -204,6 +235,9 @@ def main():
 
     from easybem_amd import _lib as L, meshes
 
+    if args.pmc_probe:
+        pmc_probe(L, meshes, args.level)
+        return
     P, T = meshes.icosphere(args.level)
     x0 = np.array([0.0, 0.0, 2.0])
     u = 1.0 / np.linalg.norm(P - x0, axis=1)
@@ -301,7 +335,8 @@ def main():
         "roofline": {"kernel": "k_far<float, single layer, 7 points>",
                      "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_FAR32_L5_BYTES if args.level == 5 else None,
                      "flop_per_launch": far_flop,
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
     }

@@ -217,6 +217,12 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     op->rule.y[q] = ry[q];
     op->rule.w[q] = rw[q];
   }
+  op->ray = ray_order(op->rule.nt, op->rule.x, op->rule.y, op->rule.w, op->ray_c);
+  if (op->rule.nt <= 7 && !op->ray) {
+    delete op;
+    set_error("far-field rules of up to 7 points must be median-symmetric");
+    return EBH_ERR_UNSUPPORTED;
+  }
   op->nc = d.kernel == EBH_KELVIN_SL ? 3 : 1;
   op->nrows = d.test_basis ? mesh->T.np : mesh->T.ne;
   op->ncols = d.trial_basis ? mesh->T.np : mesh->T.ne;

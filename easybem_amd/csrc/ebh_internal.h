@@ -99,6 +99,9 @@ struct ebh_op {
   ebh_op_desc d{};
   int nq = 0;                    // near-field Gauss points per dimension
   RuleArgs rule{};
+  // rule in centroid-median order (ray_order): point q = C + ray_c[q] (P_k - C)
+  bool ray = false;
+  double ray_c[16] = {};
   int nc = 1;                    // components (3 for Kelvin)
   int nrows = 0, ncols = 0;      // test / trial DOFs (per component)
   int nout = 1;                  // far outputs per target triangle (x nc)
@@ -109,7 +112,7 @@ struct ebh_op {
   // far-field geometry
   ebh::DevBuf<float> tq32, sg32;     // target [nt][3][ne], source [ne*nt][3]
   ebh::DevBuf<float> tc32;           // FP32 double layer: centroids hi/lo [6][ne]
-  ebh::DevBuf<float> tsl32;          // single layer: centroid form [3+4nt][ne]
+  ebh::DevBuf<float> tsl32;          // single layer target frame (see op_build_far)
   ebh::DevBuf<double> tsl64;
   ebh::DevBuf<double> tq64, sg64;
   ebh::DevBuf<double> tw, swt;       // w_q J per target / source point

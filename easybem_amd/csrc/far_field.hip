@@ -65,8 +65,11 @@ struct FarArgs {
   const R *tq;                        // [nt][3][ne] (FP32 double layer: offsets
                                       // from the triangle centroid)
   const R *tc;                        // FP32 double layer: centroid hi/lo [6][ne]
-  const R *tsl;                       // single layer: [3 + 4 nt][ne] = centroid C,
-                                      // then per point (2(x-C), |x-C|^2)
+  const R *tsl;                       // single layer target frame, [rows][ne]:
+                                      // median rules (nt <= 7): C, 2(P0-C),
+                                      // 2(P1-C), c_q^2 |P_k-C|^2 per point;
+                                      // others: C, then 2(x_q-C), |x_q-C|^2
+  R rc[16];                           // median rules: c_q of point q
   const R *src;                       // [ne*nt][REC]
   const int *touch_ptr, *touch;
   const double *tw;                   // [nt][ne]
@@ -84,6 +87,13 @@ template <typename R, int KIND> struct Rec {
 template <typename R, int KIND, int NT>
 __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 {
+  // Single layer with a median-symmetric rule (the point set is the
+  // centroid C and/or orbits C + c (P_k - C), k = 0, 1, 2): with D = C - y,
+  // r^2 = |D|^2 + c G_k + c^2 |P_k - C|^2, G_k = 2 D.(P_k - C), and
+  // G_0 + G_1 + G_2 = 0.  Per source point: D, |D|^2, G_0, G_1, G_2; per
+  // interaction: one add and one FMA (none for the centroid) before rsqrt.
+  constexpr bool RAY = KIND == 0 && NT <= 7;
+  constexpr int C0 = NT % 3;  // 1 when the centroid leads the rule
   constexpr int REC = Rec<R, KIND>::n;
   constexpr int NC = KIND == 2 ? 3 : 1;
   constexpr bool F32 = sizeof(R) == 4;
@@ -105,7 +115,23 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   // the points (x_q) themselves.
   R X[NT], Y[NT], Z[NT], E[NT];
   R Cx = 0, Cy = 0, Cz = 0;
-  if constexpr (KIND == 0) {
+  R F0x = 0, F0y = 0, F0z = 0, F1x = 0, F1y = 0, F1z = 0;
+  if constexpr (RAY) {
+    Cx = a.tsl[0 * (size_t)ne + tt];
+    Cy = a.tsl[1 * (size_t)ne + tt];
+    Cz = a.tsl[2 * (size_t)ne + tt];
+    F0x = a.tsl[3 * (size_t)ne + tt];
+    F0y = a.tsl[4 * (size_t)ne + tt];
+    F0z = a.tsl[5 * (size_t)ne + tt];
+    F1x = a.tsl[6 * (size_t)ne + tt];
+    F1y = a.tsl[7 * (size_t)ne + tt];
+    F1z = a.tsl[8 * (size_t)ne + tt];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      X[q] = Y[q] = Z[q] = 0;
+      E[q] = a.tsl[(9 + q) * (size_t)ne + tt];
+    }
+  } else if constexpr (KIND == 0) {
     Cx = a.tsl[0 * (size_t)ne + tt];
     Cy = a.tsl[1 * (size_t)ne + tt];
     Cz = a.tsl[2 * (size_t)ne + tt];
@@ -169,7 +195,21 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 #pragma unroll
       for (int b = 0; b < NT; b++, rec += REC) {
         const R sx = rec[0], sy = rec[1], sz = rec[2];
-        if constexpr (KIND == 0) {
+        if constexpr (RAY) {
+          const R sg = rec[3] * keep;
+          const R Dx = Cx - sx, Dy = Cy - sy, Dz = Cz - sz;
+          // an excluded (touching) triangle gets r^2 + 1 with strength 0
+          const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
+          R G[3];
+          G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
+          G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
+          G[2] = -(G[0] + G[1]);
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
+            acc[q] = fma(sg, rsq(r2), acc[q]);
+          }
+        } else if constexpr (KIND == 0) {
           const R sg = rec[3] * keep;
           const R Dx = Cx - sx, Dy = Cy - sy, Dz = Cz - sz;
           // an excluded (touching) triangle gets |D|^2 + 1: its strength is
@@ -551,6 +591,7 @@ int run_far(ebh_op *op, hipStream_t s)
   a.partial = op->partial.p;
   for (int q = 0; q < 16; q++) {
     const bool in = q < nt;
+    a.rc[q] = (R)op->ray_c[q];
     a.phi[q][0] = in ? 1.0 - op->rule.x[q] - op->rule.y[q] : 0.0;
     a.phi[q][1] = in ? op->rule.x[q] : 0.0;
     a.phi[q][2] = in ? op->rule.y[q] : 0.0;
@@ -782,14 +823,36 @@ int op_build_far(ebh_op *op)
     EBH_TRY(op->src64.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
   }
   if (op->d.kernel == EBH_LAPLACE_SL) {
-    // centroid form of the target points for the single layer
-    std::vector<double> tsl((size_t)(3 + 4 * nt) * ne);
+    // single-layer target frame: centroid, then the median form (ray rules)
+    // or the offsets of every point from the centroid
+    const bool ray = op->ray && nt <= 7;
+    std::vector<double> tsl((size_t)(ray ? 9 + nt : 3 + 4 * nt) * ne);
     for (int t = 0; t < ne; t++) {
       double c[3];
       for (int k = 0; k < 3; k++)
         c[k] = (T.nodes[3 * T.tri[3 * t] + k] + T.nodes[3 * T.tri[3 * t + 1] + k] +
                 T.nodes[3 * T.tri[3 * t + 2] + k]) / 3.0;
       for (int k = 0; k < 3; k++) tsl[(size_t)k * ne + t] = c[k];
+      if (ray) {
+        double e[3][3], e2[3];
+        for (int v = 0; v < 3; v++) {
+          e2[v] = 0.0;
+          for (int k = 0; k < 3; k++) {
+            e[v][k] = T.nodes[3 * T.tri[3 * t + v] + k] - c[k];
+            e2[v] += e[v][k] * e[v][k];
+          }
+        }
+        for (int k = 0; k < 3; k++) {
+          tsl[(size_t)(3 + k) * ne + t] = 2.0 * e[0][k];
+          tsl[(size_t)(6 + k) * ne + t] = 2.0 * e[1][k];
+        }
+        const int c0 = nt % 3;
+        for (int q = 0; q < nt; q++) {
+          const double cq = op->ray_c[q];
+          tsl[(size_t)(9 + q) * ne + t] = q < c0 ? 0.0 : cq * cq * e2[(q - c0) % 3];
+        }
+        continue;
+      }
       for (int q = 0; q < nt; q++) {
         double e = 0.0;
         for (int k = 0; k < 3; k++) {

@@ -120,6 +120,56 @@ bool tri_rule(int idx, std::vector<double> &x, std::vector<double> &y,
   }
 }
 
+bool ray_order(int nt, double *x, double *y, double *w, double *c)
+{
+  const double tol = 1e-13;
+  int cen = -1;
+  std::vector<int> kk(nt, -1);
+  std::vector<double> cc(nt, 0.0);
+  for (int q = 0; q < nt; q++) {
+    const double mu[3] = {1.0 - x[q] - y[q] - 1.0 / 3.0, x[q] - 1.0 / 3.0,
+                          y[q] - 1.0 / 3.0};
+    if (std::fabs(mu[0]) < tol && std::fabs(mu[1]) < tol && std::fabs(mu[2]) < tol) {
+      if (cen >= 0) return false;
+      cen = q;
+      continue;
+    }
+    for (int k = 0; k < 3; k++) {
+      const int i = (k + 1) % 3, j = (k + 2) % 3;
+      if (std::fabs(mu[i] - mu[j]) < tol) { kk[q] = k; cc[q] = mu[k] - mu[i]; }
+    }
+    if (kk[q] < 0) return false;
+  }
+  // orbits in order of first appearance; each needs k = 0, 1, 2 once
+  std::vector<int> order;
+  if (cen >= 0) order.push_back(cen);
+  std::vector<bool> used(nt, false);
+  if (cen >= 0) used[cen] = true;
+  for (int q = 0; q < nt; q++) {
+    if (used[q]) continue;
+    int mem[3] = {-1, -1, -1};
+    for (int p = q; p < nt; p++)
+      if (!used[p] && kk[p] >= 0 && std::fabs(cc[p] - cc[q]) < 1e-12) {
+        if (mem[kk[p]] >= 0) return false;
+        mem[kk[p]] = p;
+      }
+    for (int k = 0; k < 3; k++) {
+      if (mem[k] < 0) return false;
+      used[mem[k]] = true;
+      order.push_back(mem[k]);
+    }
+  }
+  std::vector<double> ox(nt), oy(nt), ow(nt);
+  for (int q = 0; q < nt; q++) {
+    ox[q] = x[order[q]];
+    oy[q] = y[order[q]];
+    ow[q] = w[order[q]];
+    c[q] = order[q] == cen ? 0.0 : cc[order[q]];
+  }
+  for (int q = 0; q < nt; q++) { x[q] = ox[q]; y[q] = oy[q]; w[q] = ow[q]; }
+  return true;
+}
+
 // ------------------------------------------------------- Sauter-Schwab maps
 int ss_nregions(int cls) { return cls == 1 ? 2 : cls == 2 ? 5 : 3; }
 int ss_nouter(int cls, int nq)

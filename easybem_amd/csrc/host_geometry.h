@@ -19,6 +19,13 @@ bool line_rule(int idx, std::vector<double> &p, std::vector<double> &w);
 // to 1/2 (src/quadrature.cpp:1047-1048).  Indices 0..6 are supported.
 bool tri_rule(int idx, std::vector<double> &x, std::vector<double> &y,
               std::vector<double> &w);
+// Reorders a rule whose points are the centroid or lie on the medians
+// (barycentric (a, a, 1-2a) and permutations) into: the centroid first, if
+// present, then each orbit with its special vertex k = 0, 1, 2.  Point q is
+// then C + c[q] (P_k - C), C the centroid and P_k the k-th vertex, with
+// k = (q - has_centroid) % 3.  Returns false (rule untouched) for any other
+// rule.
+bool ray_order(int nt, double *x, double *y, double *w, double *c);
 
 // Reduced Sauter-Schwab weight tables.  cls: 1 vertex-adjacent, 2
 // edge-adjacent, 3 coincident.  kdeg: homogeneity degree of the kernel
