@@ -30,8 +30,9 @@ FLOP_SL = 11               # per far interaction, single layer, one RHS
 # HBM bytes per launch of k_far<float,0,7> on the level-5 icosphere: two
 # rocprofv3 passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --pmc-probe`, each
 # scaled by the same counter on a 512 MiB device copy
-# (profiles/k_far_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5.
-TRAFFIC_FAR32_L5_BYTES = 58.08e6
+# (profiles/k_far_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
+# with the packaged launch shape.
+TRAFFIC_FAR32_L5_BYTES = 46.28e6
 
 
 def log(*a):
@@ -224,6 +225,9 @@ def main():
                     help="serving stream of N requests on icosphere levels 4-6 "
                          "(the default run serves 12 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the launch-shape autotuner on the bench operators "
+                         "instead of using the packaged table")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only the fixed copy + apply workload profiled "
                          "with hardware counters (no JSON bench line)")
@@ -236,6 +240,7 @@ def main():
     from easybem_amd import _lib as L, meshes
 
     if args.pmc_probe:
+        # same launch shapes as the bench line (the packaged table)
         pmc_probe(L, meshes, args.level)
         return
     P, T = meshes.icosphere(args.level)
@@ -245,6 +250,21 @@ def main():
     V64 = L.Operator(mesh, "V", 1, 1, "fp64", line_rule=4, tri_rule=4)
     V32 = L.Operator(mesh, "V", 1, 1, "fp32", line_rule=4, tri_rule=4)
     K64 = L.Operator(mesh, "K", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    # launch shapes: the autotuner's packaged table (loaded with the
+    # library), or a fresh search with --retune (untimed setup)
+    tune = {"source": "autotuner table " + os.path.basename(L.TUNE_FILE)
+            if os.path.exists(L.TUNE_FILE) else "default shapes"}
+    if args.retune:
+        tune["source"] = "autotuner run"
+        for name, op in (("fp32", V32), ("fp64", V64)):
+            r = op.autotune(budget_ms=3000.0)
+            tune[name + "_search"] = {
+                "apply_ms": r["ms"], "apply_ms_default_shape": r["ms_default"],
+                "shapes_timed": r["tried"]}
+    for name, op in (("fp32", V32), ("fp64", V64)):
+        blk, tile, chunks = op.launch()
+        tune[name] = {"block": blk, "tile": tile, "chunks": chunks,
+                      "workgroups": chunks * -(-int(T.shape[0]) // blk)}
     b = 0.5 * mesh.mass(u, 1) + K64(u)
     db = L.DeviceArray.from_host(b)
     dx = L.DeviceArray(b.size)
@@ -330,13 +350,15 @@ def main():
         "matvec64_per_solve": stats[-1]["matvec64"],
         **out_fp64,
         "matvec_ms": {"fp32": t32["total"], "fp64": t64["total"]},
+        "autotune": tune,
         "matvec_far_tflops": {"fp32": far_flop / (t32["far"] * 1e-3) / 1e12,
                               "fp64": far_flop / (t64["far"] * 1e-3) / 1e12},
         "roofline": {"kernel": "k_far<float, single layer, 7 points>",
                      "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_FAR32_L5_BYTES if args.level == 5 else None,
+                     "traffic": (TRAFFIC_FAR32_L5_BYTES if args.level == 5
+                                 and not args.retune else None),
                      "flop_per_launch": far_flop,
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
     }

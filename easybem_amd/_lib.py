@@ -12,6 +12,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libeasybem_hip.so")
+TUNE_FILE = os.path.join(PKG, "tuning_mi355x.txt")
 
 KERNELS = {"V": 0, "K": 1, "kelvin": 2}
 EBH_OK = 0
@@ -69,6 +70,12 @@ def lib():
         L.ebh_op_destroy.restype = None
         L.ebh_mass_apply.argtypes = [vp, C.c_int, vp, vp]
         _lib = L
+        # packaged launch shapes from the autotuner (EBH_TUNE_FILE overrides
+        # the path; an empty value starts with an empty table)
+        tf = os.environ.get("EBH_TUNE_FILE", TUNE_FILE)
+        if tf and os.path.exists(tf):
+            L.ebh_tune_load.argtypes = [C.c_char_p]
+            check(L.ebh_tune_load(os.fsencode(tf)))
     return _lib
 
 
@@ -206,6 +213,16 @@ class Operator:
         check(lib().ebh_op_time(self.h, x.ptr, y.ptr, nrhs, reps, ms))
         return {"fold": ms[0], "far": ms[1], "near": ms[2], "total": ms[3]}
 
+    def autotune(self, budget_ms=2000.0):
+        """Search far-field launch shapes (ebh_op_autotune); the fastest is
+        kept on this operator and recorded for later operators of its key."""
+        L = lib()
+        L.ebh_op_autotune.argtypes = [C.c_void_p, C.c_double,
+                                      C.POINTER(TuneResult)]
+        r = TuneResult()
+        check(L.ebh_op_autotune(self.h, float(budget_ms), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
     def work(self):
         w = (C.c_int64 * 4)()
         check(lib().ebh_op_work(self.h, w))
@@ -309,6 +326,27 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_int), ("tile", C.c_int), ("chunks", C.c_int),
+                ("grid", C.c_int), ("mgrid", C.c_int), ("ms", C.c_double),
+                ("ms_default", C.c_double), ("mms", C.c_double),
+                ("mms_default", C.c_double), ("tried", C.c_int)]
+
+
+def tune_save(path):
+    lib().ebh_tune_save.argtypes = [C.c_char_p]
+    check(lib().ebh_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    lib().ebh_tune_load.argtypes = [C.c_char_p]
+    check(lib().ebh_tune_load(os.fsencode(path)))
+
+
+def tune_clear():
+    check(lib().ebh_tune_clear())
 
 
 def op_profile(op: Operator, enable: bool, max_launches=4096):

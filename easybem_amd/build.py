@@ -28,7 +28,8 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-fgpu-flush-denormals-to-zero"]
 
 LIB_SOURCES = ["host_geometry.cpp", "near_field.hip", "far_field.hip",
-               "capi.hip", "solver.hip", "legacy_api.cpp", "server.cpp"]
+               "capi.hip", "solver.hip", "legacy_api.cpp", "server.cpp",
+               "autotune.cpp"]
 
 
 def _run(cmd):

@@ -1,0 +1,259 @@
+// Launch-shape autotuner for the far-field kernels.
+//
+// The far-field apply has three shape parameters: the workgroup size (target
+// triangles per workgroup, 64/128/256), the LDS source tile (source
+// triangles staged per barrier, up to 64 so one 64-bit exclusion mask covers
+// a tile) and the number of source chunks, expressed here as the total
+// workgroup count of the launch.  The tuner times whole applies with HIP
+// events on the operator's stream and walks the three parameters by
+// coordinate descent from the current shape.  Results go into a process-wide
+// table keyed by device, operator kind and mesh size class; operators created
+// later under the same key start from the tuned shape.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ebh_internal.h"
+
+namespace ebh {
+namespace {
+
+struct Entry {
+  int block, tile, grid, mgrid;
+  double ms;
+};
+
+std::mutex g_mu;
+std::map<std::string, Entry> g_table;
+
+std::string device_tag(int dev)
+{
+  static std::mutex mu;
+  static std::map<int, std::string> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  hipDeviceProp_t p;
+  std::string tag = "unknown";
+  if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+    std::string arch = p.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    tag = arch + "x" + std::to_string(p.multiProcessorCount);
+  }
+  cache[dev] = tag;
+  return tag;
+}
+
+// Size class: half-octaves of the triangle count.
+std::string key_of(const ebh_op *op)
+{
+  const int ne = op->mesh->T.ne;
+  const int cls = (int)std::lround(2.0 * std::log2((double)std::max(ne, 1)));
+  char buf[192];
+  std::snprintf(buf, sizeof buf, "%s/k%d/p%d/nt%d/test%d/rb%d/ne%d",
+                device_tag(op->mesh->device).c_str(), op->d.kernel,
+                op->d.far_precision, op->rule.nt, op->d.test_basis, op->rb, cls);
+  return buf;
+}
+
+// Best of `reps` whole applies (ms), single or multi-RHS.
+int time_apply(ebh_op *op, const double *x, double *y, bool multi, int reps,
+               double *ms)
+{
+  double best = 1e300;
+  for (int k = 0; k < reps; k++) {
+    if (multi) EBH_TRY(op_apply_multi(op, x, y, op->d.max_rhs, op->stream, true));
+    else EBH_TRY(op_apply_one(op, x, y, op->stream, true));
+    EBH_HIP(hipEventSynchronize(op->ev[3]));
+    float t = 0.f;
+    EBH_HIP(hipEventElapsedTime(&t, op->ev[0], op->ev[3]));
+    best = std::min(best, (double)t);
+  }
+  *ms = best;
+  return EBH_OK;
+}
+
+int grid_of(const ebh_op *op)
+{
+  const int tb = (op->mesh->T.ne + op->block - 1) / op->block;
+  return tb * op->chunks;
+}
+
+}  // namespace
+
+void tune_apply(ebh_op *op)
+{
+  Entry e;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find(key_of(op));
+    if (it == g_table.end()) return;
+    e = it->second;
+  }
+  if (e.mgrid > 0) op->mgrid = e.mgrid;
+  // a recorded shape was valid when it was tuned; on failure the operator
+  // keeps its default shape
+  if (op_configure(op, e.block, e.tile, chunks_for_grid(op, e.block, e.grid)) != EBH_OK)
+    (void)op_configure(op, 0, 0, 0);
+}
+
+}  // namespace ebh
+
+using namespace ebh;
+
+extern "C" {
+
+int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
+{
+  if (!op || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  EBH_HIP(hipSetDevice(op->mesh->device));
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  const int nrhs = op->rb ? op->d.max_rhs : 1;
+  DevBuf<double> x, y;
+  EBH_TRY(x.alloc((size_t)op->ncols * op->nc * nrhs));
+  EBH_TRY(y.alloc((size_t)op->nrows * op->nc * nrhs));
+  // zero input: the far-field kernels do the same work for any values
+  EBH_HIP(hipMemsetAsync(x.p, 0, x.bytes(), op->stream));
+  EBH_HIP(hipStreamSynchronize(op->stream));
+
+  const int reps = 3;
+  double spent = 0.0;
+  int tried = 0;
+  // distinct effective shapes (block, tile, chunks) already timed
+  std::map<std::tuple<int, int, int>, double> seen;
+  auto eval = [&](int block, int tile, int grid, double *ms) -> int {
+    const int chunks = chunks_for_grid(op, block, grid);
+    auto key = std::make_tuple(block, tile, chunks);
+    auto it = seen.find(key);
+    if (it != seen.end()) { *ms = it->second; return EBH_OK; }
+    EBH_TRY(op_configure(op, block, tile, chunks));
+    EBH_TRY(time_apply(op, x.p, y.p, false, reps, ms));
+    spent += *ms * reps;
+    tried++;
+    seen[key] = *ms;
+    return EBH_OK;
+  };
+
+  int bb = op->block, bt = op->tile, bg = grid_of(op);
+  double best = 0.0;
+  EBH_TRY(eval(bb, bt, bg, &best));
+  const double ms0 = best;
+  const int blocks[] = {64, 128, 256};
+  const int tiles[] = {16, 32, 64};
+  const int grids[] = {1024, 2048, 4096, 8192, 16384, 32768};
+  for (int round = 0; round < 2 && spent < budget_ms; round++) {
+    for (int g : grids) {
+      if (spent >= budget_ms) break;
+      double ms;
+      EBH_TRY(eval(bb, bt, g, &ms));
+      if (ms < best) { best = ms; bg = g; }
+    }
+    for (int b : blocks) {
+      if (spent >= budget_ms) break;
+      double ms;
+      EBH_TRY(eval(b, bt, bg, &ms));
+      if (ms < best) { best = ms; bb = b; }
+    }
+    for (int t : tiles) {
+      if (spent >= budget_ms) break;
+      double ms;
+      EBH_TRY(eval(bb, t, bg, &ms));
+      if (ms < best) { best = ms; bt = t; }
+    }
+  }
+  EBH_TRY(op_configure(op, bb, bt, chunks_for_grid(op, bb, bg)));
+
+  // multi-RHS kernel: its workgroup count, with the single-RHS shape fixed
+  int bm = 0;
+  double mbest = 0.0, mms0 = 0.0;
+  if (op->rb) {
+    bm = op->mgrid;
+    EBH_TRY(time_apply(op, x.p, y.p, true, reps, &mbest));
+    mms0 = mbest;
+    spent += mbest * reps;
+    tried++;
+    for (int g : grids) {
+      if (spent >= budget_ms) break;
+      if (g == bm) continue;
+      op->mgrid = g;
+      EBH_TRY(op_configure(op, bb, bt, op->chunks));
+      double ms;
+      EBH_TRY(time_apply(op, x.p, y.p, true, reps, &ms));
+      spent += ms * reps;
+      tried++;
+      if (ms < mbest) { mbest = ms; bm = g; }
+    }
+    op->mgrid = bm;
+    EBH_TRY(op_configure(op, bb, bt, op->chunks));
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[key_of(op)] = Entry{bb, bt, bg, bm, best};
+  }
+  out->block = bb;
+  out->tile = bt;
+  out->chunks = op->chunks;
+  out->grid = bg;
+  out->mgrid = bm;
+  out->ms = best;
+  out->ms_default = ms0;
+  out->mms = mbest;
+  out->mms_default = mms0;
+  out->tried = tried;
+  return EBH_OK;
+}
+
+int ebh_tune_save(const char *path)
+{
+  if (!path) { set_error("null path"); return EBH_ERR_ARG; }
+  std::ofstream f(path);
+  if (!f) { set_error(std::string("cannot write ") + path); return EBH_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (const auto &kv : g_table)
+    f << kv.first << ' ' << kv.second.block << ' ' << kv.second.tile << ' '
+      << kv.second.grid << ' ' << kv.second.mgrid << ' ' << kv.second.ms << '\n';
+  return f ? EBH_OK : EBH_ERR_ARG;
+}
+
+int ebh_tune_load(const char *path)
+{
+  if (!path) { set_error("null path"); return EBH_ERR_ARG; }
+  std::ifstream f(path);
+  if (!f) { set_error(std::string("cannot read ") + path); return EBH_ERR_ARG; }
+  std::map<std::string, Entry> add;
+  std::string line;
+  int n = 0;
+  while (std::getline(f, line)) {
+    n++;
+    if (line.empty()) continue;
+    std::istringstream is(line);
+    std::string key;
+    Entry e;
+    if (!(is >> key >> e.block >> e.tile >> e.grid >> e.mgrid >> e.ms) ||
+        (e.block != 64 && e.block != 128 && e.block != 256) || e.tile < 1 ||
+        e.tile > 64 || e.grid < 1 || e.mgrid < 0) {
+      set_error(std::string(path) + ": malformed line " + std::to_string(n));
+      return EBH_ERR_ARG;
+    }
+    add[key] = e;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (const auto &kv : add) g_table[kv.first] = kv.second;
+  return EBH_OK;
+}
+
+int ebh_tune_clear(void)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_table.clear();
+  return EBH_OK;
+}
+
+}  // extern "C"

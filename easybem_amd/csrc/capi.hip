@@ -245,6 +245,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     ebh_op_destroy(op);
     return rc;
   }
+  tune_apply(op);
   *out = op;
   return EBH_OK;
 }

@@ -123,6 +123,7 @@ struct ebh_op {
   ebh::DevBuf<double> partial, tri_out;
   // multi-RHS single layer (rb = RHS per lane, 0 when unused)
   int rb = 0, mchunks = 1, mchunk = 0;
+  int mgrid = 4096;  // multi-RHS launch: target workgroup count
   ebh::DevBuf<float> mgeom32, msig32;
   ebh::DevBuf<double> mgeom64, msig64;
   // launch configuration
@@ -145,4 +146,8 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
 int op_build_near(ebh_op *op);
 int op_build_far(ebh_op *op);
 int op_configure(ebh_op *op, int block, int tile, int chunks);
+// source chunks that give about `grid` workgroups at this block size
+int chunks_for_grid(const ebh_op *op, int block, int grid);
+// applies a tuned launch shape recorded for this operator's key, if any
+void tune_apply(ebh_op *op);
 }  // namespace ebh

@@ -723,6 +723,15 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
 
 }  // namespace
 
+int chunks_for_grid(const ebh_op *op, int block, int grid)
+{
+  const int ne = op->mesh->T.ne;
+  const int tb = (ne + block - 1) / block;
+  int chunks = std::max(1, (grid + tb - 1) / tb);
+  if (chunks >= 8) chunks = (chunks + 7) / 8 * 8;
+  return std::min(chunks, ne);
+}
+
 int op_configure(ebh_op *op, int block, int tile, int chunks)
 {
   const int ne = op->mesh->T.ne;
@@ -740,12 +749,8 @@ int op_configure(ebh_op *op, int block, int tile, int chunks)
     }
     op->tile = tile;
   }
-  if (chunks <= 0) {
-    // aim for ~4096 blocks: 256 CUs x several resident blocks
-    const int tb = (ne + op->block - 1) / op->block;
-    chunks = std::max(1, (4096 + tb - 1) / tb);
-    if (chunks >= 8) chunks = (chunks + 7) / 8 * 8;
-  }
+  // default: ~4096 workgroups, 256 CUs x several resident blocks
+  if (chunks <= 0) chunks = chunks_for_grid(op, op->block, 4096);
   chunks = std::min(chunks, ne);
   op->chunk = (ne + chunks - 1) / chunks;
   op->chunks = (ne + op->chunk - 1) / op->chunk;
@@ -755,7 +760,7 @@ int op_configure(ebh_op *op, int block, int tile, int chunks)
     // multi-RHS launch: one lane per target point, 256/nt triangles a block
     const int tbn = (ne + 256 / op->rule.nt - 1) / (256 / op->rule.nt);
     const int groups = (op->d.max_rhs + op->rb - 1) / op->rb;
-    int mc = std::max(1, (4096 + tbn * groups - 1) / (tbn * groups));
+    int mc = std::max(1, (op->mgrid + tbn * groups - 1) / (tbn * groups));
     if (mc >= 8) mc = (mc + 7) / 8 * 8;
     mc = std::min(mc, ne);
     op->mchunk = (ne + mc - 1) / mc;

@@ -131,6 +131,33 @@ int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
  * since the previous call (events on the op's stream), then resets. */
 int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
                    int64_t *launches);
+/* ---- launch-shape autotuner ----------------------------------------- */
+/* No counterpart in the reference, whose initGPU takes nblocks/nthreads by
+ * hand (include/BEM3dLaplaceGPU.h:8-10).  Times whole applies with HIP
+ * events on the op's stream over far-field launch shapes -- workgroup size
+ * 64/128/256 target triangles, LDS source tile 16/32/64 triangles, 1024 to
+ * 32768 workgroups -- by coordinate descent from the current shape, keeps
+ * the fastest and records it in a process-wide table under the operator's
+ * key (device, kernel, precision, rule size, test basis, RHS block, mesh
+ * size class in half-octaves); operators created later under the same key
+ * start from it.  Multi-RHS operators also tune the workgroup count of the
+ * multi-RHS kernel.  budget_ms bounds the GPU time spent timing (the
+ * starting shape is always timed).  The op must not be used by another
+ * thread meanwhile. */
+typedef struct {
+  int block, tile, chunks, grid;  /* chosen single-RHS shape */
+  int mgrid;                      /* chosen multi-RHS workgroup count, 0 if none */
+  double ms, ms_default;          /* apply ms: chosen / starting shape */
+  double mms, mms_default;        /* multi-RHS apply ms: chosen / starting */
+  int tried;                      /* shapes timed */
+} ebh_tune_result;
+int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out);
+/* Tuning table as text, one line per key: "key block tile grid mgrid ms".
+ * Loading merges into the table (later lines win). */
+int ebh_tune_save(const char *path);
+int ebh_tune_load(const char *path);
+int ebh_tune_clear(void);
+
 /* Work counts: [0] far interactions per RHS, [1] near nnz (entries),
  * [2] near rows, [3] near cols. */
 int ebh_op_work(const ebh_op *op, int64_t *work);

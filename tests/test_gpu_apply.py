@@ -75,6 +75,79 @@ def test_hybrid_rule_icosphere3(L, oracle, prec):
     assert err <= (TOL64 if prec == "fp64" else TOL32)
 
 
+@pytest.mark.parametrize("tri_rule", range(7))
+@pytest.mark.parametrize("kernel,test,trial", [("V", 0, 0), ("V", 1, 1),
+                                               ("K", 0, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_every_triangle_rule(L, oracle, tri_rule, kernel, test, trial, prec):
+    """Far-field rules 0..6 (1, 3, 4, 6, 7, 12, 13 points).  Rules of up to
+    7 points run the single layer in the centroid-median form after the
+    operator reorders their points; 12 and 13 points use the general form."""
+    P, T = meshes.perturbed_icosphere(1, 5)[:2]
+    print(f"case rule={tri_rule} {kernel}{test}{trial} {prec}", flush=True)
+    x = _x(kernel, trial, P, T, 3)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, kernel, test, trial, prec, line_rule=3,
+                    tri_rule=tri_rule)
+    yr = oracle.apply(kernel, test, trial, P, T, x, line_idx=3,
+                      tri_idx=tri_rule).ravel()
+    err = rel(op(x), yr)
+    print("rel err", err)
+    assert err <= (TOL64 if prec == "fp64" else TOL32)
+
+
+@pytest.fixture
+def tune_table(L, tmp_path):
+    """Saves the process-wide tuning table and restores it afterwards."""
+    saved = tmp_path / "table.txt"
+    L.tune_save(str(saved))
+    yield
+    L.tune_clear()
+    L.tune_load(str(saved))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_autotune_keeps_results_and_records_shape(L, prec, tune_table):
+    """The tuner's chosen shape gives the same apply (FP64: to rounding of
+    the chunk summation order) and is picked up by a new operator of the
+    same key."""
+    L.tune_clear()
+    P, T = meshes.icosphere(3)
+    x = np.random.default_rng(8).uniform(-1, 1, P.shape[0])
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "V", 1, 1, prec)
+    y0 = op(x)
+    r = L.Operator.autotune(op, budget_ms=500.0)
+    print(r)
+    assert r["block"] in (64, 128, 256) and 1 <= r["tile"] <= 64
+    assert r["tried"] >= 1 and r["ms"] <= r["ms_default"]
+    assert op.launch() == (r["block"], r["tile"], r["chunks"])
+    assert rel(op(x), y0) < (1e-14 if prec == "fp64" else 1e-6)
+    op2 = L.Operator(m, "V", 1, 1, prec)
+    assert op2.launch() == op.launch()
+    L.tune_clear()
+    op3 = L.Operator(m, "V", 1, 1, prec)
+    assert op3.launch()[1] == 64
+
+
+def test_autotune_multi_rhs(L, tune_table):
+    L.tune_clear()
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "V", 0, 0, "fp32", max_rhs=16)
+    X = np.random.default_rng(9).uniform(-1, 1, (16, T.shape[0]))
+    dx = L.DeviceArray.from_host(X.ravel())
+    dy = L.DeviceArray(X.size)
+    op.apply(dx, dy, nrhs=16)
+    y0 = dy.to_host()
+    r = op.autotune(budget_ms=500.0)
+    print(r)
+    assert r["mgrid"] > 0 and r["mms"] <= r["mms_default"]
+    op.apply(dx, dy, nrhs=16)
+    assert rel(dy.to_host(), y0) < 1e-6
+    L.tune_clear()
+
+
 def test_launch_configurations_agree(L):
     """Every launch shape gives the same FP64 result to rounding."""
     P, T = meshes.icosphere(3)

@@ -90,3 +90,42 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"),
                         "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_tuning_table_round_trip(L, tmp_path):
+    """The autotuner's table persists as text and rejects malformed lines
+    (host-only: no device call)."""
+    saved = tmp_path / "saved.txt"
+    L.tune_save(str(saved))
+    L.tune_clear()
+    src = tmp_path / "in.txt"
+    src.write_text("gfx950x256/k0/p1/nt7/test1/rb0/ne29 256 64 4096 4096 3.2\n"
+                   "gfx950x256/k0/p0/nt7/test1/rb0/ne29 128 32 8192 0 10.5\n")
+    L.tune_load(str(src))
+    out = tmp_path / "out.txt"
+    L.tune_save(str(out))
+    lines = sorted(out.read_text().split("\n"))
+    assert lines[1:] == sorted(src.read_text().split("\n"))[1:]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("key 96 64 4096 0 1.0\n")
+    with pytest.raises(L.EbhError):
+        L.tune_load(str(bad))
+    L.tune_clear()
+    L.tune_save(str(out))
+    assert out.read_text() == ""
+    L.tune_load(str(saved))
+
+
+def test_packaged_tuning_table_is_well_formed(L, tmp_path):
+    from easybem_amd import _lib
+    if not os.path.exists(_lib.TUNE_FILE):
+        pytest.skip("no packaged table")
+    saved = tmp_path / "saved.txt"
+    L.tune_save(str(saved))
+    L.tune_clear()
+    L.tune_load(_lib.TUNE_FILE)
+    out = tmp_path / "out.txt"
+    L.tune_save(str(out))
+    assert len(out.read_text().split()) >= 6
+    L.tune_clear()
+    L.tune_load(str(saved))
