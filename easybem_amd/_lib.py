@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libeasybem_hip.so")
 TUNE_FILE = os.path.join(PKG, "tuning_mi355x.txt")
 
-KERNELS = {"V": 0, "K": 1, "kelvin": 2}
+KERNELS = {"V": 0, "K": 1, "kelvin": 2, "D": 3}
 EBH_OK = 0
 
 
@@ -28,7 +28,7 @@ class OpDesc(C.Structure):
     _fields_ = [("kernel", C.c_int), ("test_basis", C.c_int),
                 ("trial_basis", C.c_int), ("far_precision", C.c_int),
                 ("line_rule", C.c_int), ("tri_rule", C.c_int),
-                ("nu", C.c_double), ("max_rhs", C.c_int)]
+                ("nu", C.c_double), ("max_rhs", C.c_int), ("stab", C.c_double)]
 
 
 _lib = None
@@ -173,12 +173,12 @@ class Operator:
 
     def __init__(self, mesh: Mesh, kernel="V", test_basis=0, trial_basis=0,
                  precision="fp64", line_rule=4, tri_rule=None, nu=0.3,
-                 max_rhs=1):
+                 max_rhs=1, stab=0.0):
         self.mesh = mesh
         d = OpDesc(KERNELS[kernel], int(test_basis), int(trial_basis),
                    0 if precision == "fp64" else 1, int(line_rule),
                    int(line_rule if tri_rule is None else tri_rule), float(nu),
-                   int(max_rhs))
+                   int(max_rhs), float(stab))
         self.desc = d
         h = C.c_void_p()
         check(lib().ebh_op_create(mesh.h, C.byref(d), C.byref(h)))

@@ -113,6 +113,10 @@ extern "C" {
 int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
 {
   if (!op || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  if (op->inner) {
+    EBH_HIP(hipStreamSynchronize(op->stream));
+    return ebh_op_autotune(op->inner, budget_ms, out);
+  }
   EBH_HIP(hipSetDevice(op->mesh->device));
   EBH_HIP(hipStreamSynchronize(op->stream));
   const int nrhs = op->rb ? op->d.max_rhs : 1;

@@ -16,7 +16,9 @@ using namespace ebh;
 
 size_t ebh_op::bytes() const
 {
-  return near_ptr.bytes() + near_col.bytes() + near_val.bytes() + tq32.bytes() +
+  return (inner ? inner->bytes() : 0) + curl.bytes() + mvec.bytes() + hsw.bytes() +
+         hsz.bytes() + hsdot.bytes() +
+         near_ptr.bytes() + near_col.bytes() + near_val.bytes() + tq32.bytes() +
          sg32.bytes() + tq64.bytes() + sg64.bytes() + tw.bytes() + swt.bytes() +
          normals.bytes() + src32.bytes() + src64.bytes() + partial.bytes() +
          tri_out.bytes() + mgeom32.bytes() + msig32.bytes() + mgeom64.bytes() +
@@ -177,7 +179,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (!mesh || !desc || !out) { set_error("null argument"); return EBH_ERR_ARG; }
   *out = nullptr;
   const ebh_op_desc &d = *desc;
-  if (d.kernel < 0 || d.kernel > 2) { set_error("unknown kernel"); return EBH_ERR_ARG; }
+  if (d.kernel < 0 || d.kernel > 3) { set_error("unknown kernel"); return EBH_ERR_ARG; }
   if ((d.test_basis != 0 && d.test_basis != 1) ||
       (d.trial_basis != 0 && d.trial_basis != 1)) {
     set_error("basis must be EBH_P0 or EBH_P1");
@@ -186,6 +188,15 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (d.kernel == EBH_LAPLACE_DL && d.trial_basis != EBH_P1) {
     set_error("the double layer needs a P1 trial space (as the reference's K)");
     return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_LAPLACE_HS &&
+      (d.test_basis != EBH_P1 || d.trial_basis != EBH_P1)) {
+    set_error("the hypersingular operator needs P1 test and trial spaces");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_LAPLACE_HS ? !std::isfinite(d.stab) : d.stab != 0.0) {
+    set_error("stab must be finite, and 0 for kernels other than EBH_LAPLACE_HS");
+    return EBH_ERR_ARG;
   }
   if (d.kernel == EBH_KELVIN_SL && d.test_basis != d.trial_basis) {
     set_error("Kelvin single layer needs equal test and trial bases");
@@ -241,11 +252,24 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
       set_error("hipEventCreate failed");
       return EBH_ERR_HIP;
     }
-  if ((rc = op_build_near(op)) || (rc = op_build_far(op))) {
-    ebh_op_destroy(op);
-    return rc;
+  if (d.kernel == EBH_LAPLACE_HS) {
+    // the wrapped P0 single layer takes three curl components per RHS
+    ebh_op_desc vd = d;
+    vd.kernel = EBH_LAPLACE_SL;
+    vd.test_basis = vd.trial_basis = EBH_P0;
+    vd.stab = 0.0;
+    vd.max_rhs = op->d.max_rhs > 1 ? 3 * op->d.max_rhs : 1;
+    if ((rc = ebh_op_create(mesh, &vd, &op->inner)) || (rc = hs_build(op))) {
+      ebh_op_destroy(op);
+      return rc;
+    }
+  } else {
+    if ((rc = op_build_near(op)) || (rc = op_build_far(op))) {
+      ebh_op_destroy(op);
+      return rc;
+    }
+    tune_apply(op);
   }
-  tune_apply(op);
   *out = op;
   return EBH_OK;
 }
@@ -284,15 +308,16 @@ int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }
   EBH_HIP(hipStreamSynchronize(op->stream));
-  return op_configure(op, block, tile, chunks);
+  return op_configure(far_op(op), block, tile, chunks);
 }
 
 int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }
-  if (block) *block = op->block;
-  if (tile) *tile = op->tile;
-  if (chunks) *chunks = op->chunks;
+  const ebh_op *f = far_op(op);
+  if (block) *block = f->block;
+  if (tile) *tile = f->tile;
+  if (chunks) *chunks = f->chunks;
   return EBH_OK;
 }
 
@@ -326,8 +351,9 @@ int ebh_op_work(const ebh_op *op, int64_t *work)
 {
   if (!op || !work) { set_error("null argument"); return EBH_ERR_ARG; }
   const int64_t nt = op->rule.nt;
-  work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt;
-  work[1] = op->near_nnz;
+  // hypersingular: three single-layer applies per RHS
+  work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt * (op->inner ? 3 : 1);
+  work[1] = far_op(op)->near_nnz;
   work[2] = op->nrows;
   work[3] = op->ncols;
   return EBH_OK;
@@ -337,6 +363,11 @@ int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
                    int64_t *launches)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (op->inner) {
+    // the wrapped single layer's far launches run on this op's stream
+    EBH_HIP(hipStreamSynchronize(op->stream));
+    return ebh_op_profile(op->inner, enable, max_launches, far_ms, launches);
+  }
   EBH_HIP(hipStreamSynchronize(op->stream));
   if (far_ms || launches) {
     double tot = 0.0;
@@ -365,6 +396,8 @@ void ebh_op_destroy(ebh_op *op)
 {
   if (!op) return;
   if (op->stream) (void)hipStreamSynchronize(op->stream);
+  ebh_op_destroy(op->inner);
+  op->inner = nullptr;
   for (auto &e : op->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : op->prof_ev) (void)hipEventDestroy(e);

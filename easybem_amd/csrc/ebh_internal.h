@@ -126,6 +126,11 @@ struct ebh_op {
   int mgrid = 4096;  // multi-RHS launch: target workgroup count
   ebh::DevBuf<float> mgeom32, msig32;
   ebh::DevBuf<double> mgeom64, msig64;
+  // hypersingular (EBH_LAPLACE_HS): the wrapped P0 single layer, the curls
+  // [ne][3][3], lumped node areas, curl densities / single-layer outputs
+  // [nrhs][3][ne] and the partial m.x sums
+  ebh_op *inner = nullptr;
+  ebh::DevBuf<double> curl, mvec, hsw, hsz, hsdot;
   // launch configuration
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   int gen = 0;  // bumped when the launch shape changes (captured graphs key on it)
@@ -146,6 +151,13 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
 int op_build_near(ebh_op *op);
 int op_build_far(ebh_op *op);
 int op_configure(ebh_op *op, int block, int tile, int chunks);
+int hs_build(ebh_op *op);
+int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
+             bool timed);
+// the operator whose far-field kernels run (the wrapped single layer for
+// the hypersingular operator)
+inline ebh_op *far_op(ebh_op *op) { return op->inner ? op->inner : op; }
+inline const ebh_op *far_op(const ebh_op *op) { return op->inner ? op->inner : op; }
 // source chunks that give about `grid` workgroups at this block size
 int chunks_for_grid(const ebh_op *op, int block, int grid);
 // applies a tuned launch shape recorded for this operator's key, if any

@@ -900,6 +900,7 @@ int op_build_far(ebh_op *op)
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
+  if (op->inner) return hs_apply(op, x, y, 1, s, timed);
   const int ne = op->mesh->T.ne;
   const bool f32 = op->d.far_precision == EBH_FP32;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
@@ -927,6 +928,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
 int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed)
 {
+  if (op->inner) return hs_apply(op, x, y, nrhs, s, timed);
   if (nrhs == 1 || !op->rb) {
     const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
     for (int r = 0; r < nrhs; r++)

@@ -486,4 +486,38 @@ void build_near_csr(const Topology &T, int test_basis, int trial_basis,
   }
 }
 
+// The surface gradient of the hat functions solves the 2x2 metric system of
+// the edges e1 = P1 - P0, e2 = P2 - P0; its rotation by the unit normal n,
+// b = e x n per edge, gives the curl (src/BEM3dLaplace.cpp:508-547).
+void surface_curls(const Topology &T, std::vector<double> &curl)
+{
+  curl.assign((size_t)9 * T.ne, 0.0);
+  for (int i = 0; i < T.ne; i++) {
+    const double *P[3];
+    for (int k = 0; k < 3; k++) P[k] = &T.nodes[3 * T.tri[3 * i + k]];
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; k++) { e1[k] = P[1][k] - P[0][k]; e2[k] = P[2][k] - P[0][k]; }
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                   e1[0] * e2[1] - e1[1] * e2[0]};
+    const double J = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (double &c : n) c /= J;
+    const double a11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const double a22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    const double a12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+    const double det = a11 * a22 - a12 * a12;
+    const double b1[3] = {-e1[1] * n[2] + e1[2] * n[1], e1[0] * n[2] - e1[2] * n[0],
+                          -e1[0] * n[1] + e1[1] * n[0]};
+    const double b2[3] = {-e2[1] * n[2] + e2[2] * n[1], e2[0] * n[2] - e2[2] * n[0],
+                          -e2[0] * n[1] + e2[1] * n[0]};
+    for (int c = 0; c < 3; c++) {
+      const double y1 = (b1[c] * a22 - b2[c] * a12) / det;
+      const double y2 = (a11 * b2[c] - a12 * b1[c]) / det;
+      double *o = &curl[(3 * (size_t)i + c) * 3];
+      o[0] = -y1 - y2;
+      o[1] = y1;
+      o[2] = y2;
+    }
+  }
+}
+
 }  // namespace ebh

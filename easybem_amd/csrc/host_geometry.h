@@ -65,6 +65,11 @@ struct Topology {
 std::string build_topology(int np, const double *nodes, int ne,
                            const int *elems_1based, Topology &T);
 
+// Surface curls of the P1 hat functions, restating getLaplace3dCurl
+// (src/BEM3dLaplace.cpp:489-558): curl[(3i + c) 3 + k] is Cartesian
+// component c of curl_Gamma phi_k on triangle i, k its local vertex.
+void surface_curls(const Topology &T, std::vector<double> &curl);
+
 // Near-field CSR pattern for a (test basis, trial basis) combination with
 // contribution lists: entry e sums pairbuf values listed in
 // contrib[contrib_ptr[e] .. contrib_ptr[e+1]); a contribution is

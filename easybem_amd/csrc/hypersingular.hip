@@ -1,0 +1,147 @@
+// Laplace hypersingular operator on P1 through surface curls:
+//
+//   D = sum_{c=1..3} C_c^T V C_c   (+ stab m m^T)
+//
+// with V the P0 single layer, (C_c)_{i,n} the c-th Cartesian component of
+// the surface curl of hat function n on triangle i (getLaplace3dCurl,
+// src/BEM3dLaplace.cpp:489-558, and the dense D assembly at :652,769), and m
+// the lumped node areas, m_n = sum_{i ni n} |T_i|/3
+// (python/examples/laplace3dNeumannGPU.py:36-49 adds m (m . x) to fix the
+// constants).  One apply: a curl kernel (three P0 densities per right-hand
+// side), the wrapped single layer on 3 nrhs right-hand sides, and a gather
+// over each node's triangles; no atomics, a fixed summation order.
+#include <algorithm>
+#include <cmath>
+
+#include "ebh_internal.h"
+
+namespace ebh {
+namespace {
+
+constexpr int kDotBlocks = 64;
+
+// w[(3r + c) ne + i] = sum_k curl[(3i + c) 3 + k] x[r np + tri[3i + k]]
+__global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
+                                                 const double *__restrict__ curl,
+                                                 const int *__restrict__ tri,
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ w)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i >= ne) return;
+  const double *xr = x + (size_t)r * np;
+  const double x0 = xr[tri[3 * i]], x1 = xr[tri[3 * i + 1]], x2 = xr[tri[3 * i + 2]];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double *o = curl + (3 * (size_t)i + c) * 3;
+    w[(3 * (size_t)r + c) * ne + i] = fma(o[0], x0, fma(o[1], x1, o[2] * x2));
+  }
+}
+
+// part[r kDotBlocks + b] = sum over block b's strided nodes of m_n x_n
+__global__ __launch_bounds__(256) void k_hs_dot(int np, const double *__restrict__ m,
+                                                const double *__restrict__ x,
+                                                double *__restrict__ part)
+{
+  __shared__ double red[256];
+  const int r = blockIdx.y;
+  const double *xr = x + (size_t)r * np;
+  double s = 0.0;
+  for (int n = blockIdx.x * 256 + threadIdx.x; n < np; n += kDotBlocks * 256)
+    s = fma(m[n], xr[n], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[r * kDotBlocks + blockIdx.x] = red[0];
+}
+
+// y[r np + n] = sum_{(i,k) at n} sum_c curl[(3i + c) 3 + k] z[(3r + c) ne + i]
+//               + stab m_n (m . x_r)
+__global__ __launch_bounds__(256) void k_hs_gather(
+    int ne, int np, double stab, const double *__restrict__ curl,
+    const int *__restrict__ ptr, const int *__restrict__ ntri,
+    const int *__restrict__ nk, const double *__restrict__ z,
+    const double *__restrict__ m, const double *__restrict__ part,
+    double *__restrict__ y)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (n >= np) return;
+  const double *zr = z + 3 * (size_t)r * ne;
+  double acc = 0.0;
+  for (int e = ptr[n]; e < ptr[n + 1]; e++) {
+    const int i = ntri[e], k = nk[e];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      acc = fma(curl[(3 * (size_t)i + c) * 3 + k], zr[(size_t)c * ne + i], acc);
+  }
+  if (stab != 0.0) {
+    double s = 0.0;
+    for (int b = 0; b < kDotBlocks; b++) s += part[r * kDotBlocks + b];
+    acc = fma(stab * m[n], s, acc);
+  }
+  y[(size_t)r * np + n] = acc;
+}
+
+}  // namespace
+
+int hs_build(ebh_op *op)
+{
+  const Topology &T = op->mesh->T;
+  std::vector<double> curl, m((size_t)T.np, 0.0);
+  surface_curls(T, curl);
+  for (int i = 0; i < T.ne; i++) {
+    const double *P0 = &T.nodes[3 * T.tri[3 * i]];
+    const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
+    const double *P2 = &T.nodes[3 * T.tri[3 * i + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
+                 c2 = u[0] * v[1] - u[1] * v[0];
+    const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    for (int k = 0; k < 3; k++) m[T.tri[3 * i + k]] += J / 6.0;
+  }
+  EBH_TRY(op->curl.upload(curl));
+  EBH_TRY(op->mvec.upload(m));
+  const size_t nb = (size_t)std::max(1, op->d.max_rhs);
+  EBH_TRY(op->hsw.alloc(3 * (size_t)T.ne * nb));
+  EBH_TRY(op->hsz.alloc(3 * (size_t)T.ne * nb));
+  EBH_TRY(op->hsdot.alloc(kDotBlocks * nb));
+  return EBH_OK;
+}
+
+int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
+             bool timed)
+{
+  if (nrhs > std::max(1, op->d.max_rhs)) {
+    set_error("nrhs " + std::to_string(nrhs) + " exceeds the operator's max_rhs " +
+              std::to_string(op->d.max_rhs));
+    return EBH_ERR_ARG;
+  }
+  const int ne = op->mesh->T.ne, np = op->mesh->T.np;
+  if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
+  hipLaunchKernelGGL(k_hs_curl, dim3((ne + 255) / 256, nrhs), dim3(256), 0, s, ne, np,
+                     op->curl.p, op->mesh->tri.p, x, op->hsw.p);
+  EBH_HIP(hipGetLastError());
+  if (op->d.stab != 0.0) {
+    hipLaunchKernelGGL(k_hs_dot, dim3(kDotBlocks, nrhs), dim3(256), 0, s, np,
+                       op->mvec.p, x, op->hsdot.p);
+    EBH_HIP(hipGetLastError());
+  }
+  if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
+  EBH_TRY(op_apply_multi(op->inner, op->hsw.p, op->hsz.p, 3 * nrhs, s, false));
+  if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
+  hipLaunchKernelGGL(k_hs_gather, dim3((np + 255) / 256, nrhs), dim3(256), 0, s, ne,
+                     np, op->d.stab, op->curl.p, op->mesh->node_tri_ptr.p,
+                     op->mesh->node_tri.p, op->mesh->node_tri_k.p, op->hsz.p,
+                     op->mvec.p, op->hsdot.p, y);
+  EBH_HIP(hipGetLastError());
+  if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
+  return EBH_OK;
+}
+
+}  // namespace ebh

@@ -293,7 +293,7 @@ struct ebh_solver {
     if (!use_graphs || A->prof_on) {
       EBH_TRY(step_kernels(A, j, nb));
     } else {
-      auto key = std::make_tuple((const ebh_op *)A, j, nb, A->gen);
+      auto key = std::make_tuple((const ebh_op *)A, j, nb, far_op(A)->gen);
       auto it = graphs.find(key);
       if (it == graphs.end()) {
         hipGraph_t gr;

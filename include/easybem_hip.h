@@ -47,7 +47,12 @@ enum {
   EBH_ERR_NONFINITE = -8    /* NaN/Inf in an input vector               */
 };
 
-enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2 };
+/* EBH_LAPLACE_HS: the hypersingular operator on P1 through surface curls,
+ * D = sum_c C_c^T V C_c (+ stab m m^T), as getLaplace3d's dense D
+ * (src/BEM3dLaplace.cpp:652,769) and the stub applyGPULaplace3dHyperSing
+ * (src/BEM3dLaplaceGPU.cu:1554-1555); test and trial basis must be P1. */
+enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2,
+       EBH_LAPLACE_HS = 3 };
 enum { EBH_P0 = 0, EBH_P1 = 1 };
 enum { EBH_FP64 = 0, EBH_FP32 = 1 };
 
@@ -61,7 +66,7 @@ typedef struct ebh_server ebh_server;
  * (src/quadrature.cpp:286-287,1047-1048); the reference's single `order`
  * argument corresponds to line_rule == tri_rule == order. */
 typedef struct {
-  int kernel;        /* EBH_LAPLACE_SL, EBH_LAPLACE_DL, EBH_KELVIN_SL     */
+  int kernel;        /* EBH_LAPLACE_SL/_DL/_HS, EBH_KELVIN_SL             */
   int test_basis;    /* EBH_P0 (one DOF per triangle) or EBH_P1 (nodes)   */
   int trial_basis;   /* DL requires EBH_P1; Kelvin requires test==trial   */
   int far_precision; /* EBH_FP64 or EBH_FP32 (near field is always FP64) */
@@ -69,6 +74,8 @@ typedef struct {
   int tri_rule;      /* far-field triangle rule index, 0..6               */
   double nu;         /* Poisson ratio for Kelvin (E = 1)                  */
   int max_rhs;       /* largest nrhs passed to ebh_op_apply (>= 1)        */
+  double stab;       /* EBH_LAPLACE_HS: adds stab m (m.x), m_n the lumped
+                        node areas (laplace3dNeumannGPU.py:36-49); else 0  */
 } ebh_op_desc;
 
 const char *ebh_last_error(void);

@@ -36,6 +36,17 @@ def _mesh(nodes, elems):
     return nodes, elems, nodes.shape[0], elems.shape[0]
 
 
+def lumped_areas(nodes, elems):
+    """m_n = sum of |T|/3 over the triangles at node n."""
+    P = np.asarray(nodes, dtype=np.float64)
+    T = np.asarray(elems) - 1
+    a = 0.5 * np.linalg.norm(np.cross(P[T[:, 1]] - P[T[:, 0]],
+                                      P[T[:, 2]] - P[T[:, 0]]), axis=1)
+    m = np.zeros(P.shape[0])
+    np.add.at(m, T, a[:, None] / 3)
+    return m
+
+
 class Oracle:
     def __init__(self, path: str = LIB):
         if not os.path.exists(path):
@@ -58,6 +69,7 @@ class Oracle:
         L.orc_ss_full.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int]
         L.orc_ss_full.restype = C.c_double
         L.orc_ss_table.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_surface_curls.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp]
         self.L = L
 
     # quadrature -------------------------------------------------------
@@ -137,6 +149,36 @@ class Oracle:
         return self.L.orc_ss_full(cls, kind, np.asarray(exps, np.int32),
                                   np.ascontiguousarray(geom, np.float64), n)
 
+    def curl(self, nodes, elems):
+        """(C1, C2, C3), each dense [triangle, node], restating
+        getLaplace3dCurl (src/BEM3dLaplace.cpp:489-558)."""
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        cu = np.zeros(9 * ne)
+        self.L.orc_surface_curls(np_, nodes, ne, elems, cu)
+        cu = cu.reshape(ne, 3, 3)
+        Cs = [np.zeros((ne, np_)) for _ in range(3)]
+        rows = np.arange(ne)
+        for c in range(3):
+            for k in range(3):
+                Cs[c][rows, elems[:, k] - 1] = cu[:, c, k]
+        return tuple(Cs)
+
+    def hypersingular_apply(self, nodes, elems, x, line_idx=4, tri_idx=None,
+                            stab=0.0):
+        """y = sum_c C_c^T V C_c x (+ stab m (m.x)): the dense D of
+        getLaplace3d (src/BEM3dLaplace.cpp:652,769) applied through three P0
+        single-layer applies; m = lumped node areas as in
+        python/examples/laplace3dNeumannGPU.py:36-49."""
+        Cs = self.curl(nodes, elems)
+        y = np.zeros(np.asarray(nodes).shape[0])
+        for Cc in Cs:
+            y += Cc.T @ self.apply("V", 0, 0, nodes, elems, Cc @ x,
+                                   line_idx=line_idx, tri_idx=tri_idx).ravel()
+        if stab:
+            m = lumped_areas(nodes, elems)
+            y += stab * m * (m @ x)
+        return y
+
     def ss_table(self, cls, kdeg, nq):
         nreg = {1: 2, 2: 5, 3: 3}[cls]
         nout = {1: nq ** 3, 2: nq ** 2, 3: nq}[cls]
@@ -155,6 +197,8 @@ class Reference:
         L.ebref_laplace3d.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int,
                                       _dp, _dp, _dp, _dp]
         L.ebref_laplace3d_mass.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp]
+        L.ebref_laplace3d_curl.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp,
+                                           _dp, _dp]
         L.ebref_elasticity3d.argtypes = [C.c_int, _dp, C.c_int, _ip,
                                          C.c_double, C.c_int, _dp, C.c_void_p]
         L.ebref_line_rule.argtypes = [C.c_int, _dp, _dp]
@@ -180,6 +224,13 @@ class Reference:
         M = np.zeros(ne * np_)
         self.L.ebref_laplace3d_mass(np_, nodes, ne, elems, M)
         return M.reshape(np_, ne).T.copy()
+
+    def curl(self, nodes, elems):
+        """(C1, C2, C3), each [triangle, node], like getLaplace3dCurl."""
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        Cs = [np.zeros(ne * np_) for _ in range(3)]
+        self.L.ebref_laplace3d_curl(np_, nodes, ne, elems, *Cs)
+        return tuple(c.reshape(np_, ne).T.copy() for c in Cs)
 
     def elasticity3d_V(self, nodes, elems, nu, order):
         nodes, elems, np_, ne = _mesh(nodes, elems)

@@ -842,6 +842,51 @@ static void pair_block(const ctx_t *c, int op, const pair_geom *g, double *out)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Surface curls, src/BEM3dLaplace.cpp:489-558: the 2x2 metric system of the  */
+/* edges e1 = P2 - P1, e2 = P3 - P1 against b = e x n per edge (:523-547).    */
+/* ------------------------------------------------------------------------ */
+int orc_surface_curls(int np, const double *nodes, int ne, const int *elems,
+                      double *curl)
+{
+  (void)np;
+  for (int i = 0; i < ne; i++) {
+    const double *P[3];
+    for (int k = 0; k < 3; k++) P[k] = nodes + 3 * (elems[3 * i + k] - 1);
+    double e1[3], e2[3], n[3];
+    for (int k = 0; k < 3; k++) {
+      e1[k] = P[1][k] - P[0][k];
+      e2[k] = P[2][k] - P[0][k];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const double J = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    double a11 = 0, a12 = 0, a22 = 0;
+    for (int k = 0; k < 3; k++) {
+      n[k] /= J;
+      a11 += e1[k] * e1[k];
+      a22 += e2[k] * e2[k];
+      a12 += e1[k] * e2[k];
+    }
+    const double det = a11 * a22 - a12 * a12;
+    /* rows c = x, y, z of b1 = e1 x n and b2 = e2 x n */
+    const double b1[3] = {-e1[1] * n[2] + e1[2] * n[1], e1[0] * n[2] - e1[2] * n[0],
+                          -e1[0] * n[1] + e1[1] * n[0]};
+    const double b2[3] = {-e2[1] * n[2] + e2[2] * n[1], e2[0] * n[2] - e2[2] * n[0],
+                          -e2[0] * n[1] + e2[1] * n[0]};
+    for (int c = 0; c < 3; c++) {
+      const double y1 = (b1[c] * a22 - b2[c] * a12) / det;
+      const double y2 = (a11 * b2[c] - a12 * b1[c]) / det;
+      double *o = curl + (3 * (size_t)i + c) * 3;
+      o[0] = -y1 - y2;
+      o[1] = y1;
+      o[2] = y2;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Dense P0 assembly restating getLaplace3d / getElasticity3d                */
 /* ------------------------------------------------------------------------ */
 int orc_laplace_dense(int np, const double *nodes, int ne, const int *elems,

@@ -53,6 +53,12 @@ int orc_apply(int op, int test_basis, int trial_basis, int np,
               int tri_idx, double nu, const double *x, int nrows,
               const int *rows, double *y, int nthreads);
 
+/* Surface curls of the P1 hat functions restating getLaplace3dCurl
+ * (src/BEM3dLaplace.cpp:489-558): curl[(3i + c)*3 + k] is component c of
+ * the curl of triangle i's local hat function k.  elems are 1-based. */
+int orc_surface_curls(int np, const double *nodes, int ne, const int *elems,
+                      double *curl);
+
 /* One pair's local block in the pair's local vertex orders, for tests:
  * out[9] for op 0/1 (k test, l trial, out[3k+l]); out[63] for op 2
  * (out[7*(3k+l) + c], c = I, I11, I22, I33, I12, I13, I23).

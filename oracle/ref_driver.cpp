@@ -30,6 +30,19 @@ int ebref_laplace3d(int np, const double *nodes, int ne, const int *elems,
   return 0;
 }
 
+// C1, C2, C3 (ne x np, column-major as the reference writes them).
+int ebref_laplace3d_curl(int np, const double *nodes, int ne, const int *elems,
+                         double *C1, double *C2, double *C3)
+{
+  double *c1 = nullptr, *c2 = nullptr, *c3 = nullptr;
+  getLaplace3dCurl(np, nodes, ne, elems, c1, c2, c3);
+  std::memcpy(C1, c1, sizeof(double) * (size_t)ne * np);
+  std::memcpy(C2, c2, sizeof(double) * (size_t)ne * np);
+  std::memcpy(C3, c3, sizeof(double) * (size_t)ne * np);
+  delete[] c1; delete[] c2; delete[] c3;
+  return 0;
+}
+
 int ebref_laplace3d_mass(int np, const double *nodes, int ne,
                          const int *elems, double *M)
 {

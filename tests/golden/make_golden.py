@@ -59,8 +59,10 @@ def main():
     for name, (P, T) in cases.items():
         for order in (3, 4):
             V, K, D, M = ref.laplace3d(P, T, order)
+            C1, C2, C3 = ref.curl(P, T)
             np.savez_compressed(os.path.join(HERE, f"laplace_{name}_o{order}.npz"),
-                                nodes=P, elements=T, V=V, K=K, M=M)
+                                nodes=P, elements=T, V=V, K=K, M=M, D=D,
+                                C1=C1, C2=C2, C3=C3)
     P, T = meshes.icosphere(1)
     Vk = ref.elasticity3d_V(P, T, 0.3, 4)
     np.savez_compressed(os.path.join(HERE, "kelvin_ico1_o4.npz"), nodes=P,

@@ -80,6 +80,29 @@ def test_laplace_dense_matches_reference(oracle, mesh, order):
     assert relmax(M, g["M"]) < 1e-15
 
 
+@pytest.mark.parametrize("mesh", ["ico1", "torus8x4"])
+def test_surface_curls_match_reference(oracle, mesh):
+    """orc_surface_curls against getLaplace3dCurl's C1, C2, C3."""
+    z = gold(f"laplace_{mesh}_o4.npz")
+    Cs = oracle.curl(z["nodes"], z["elements"])
+    for c in range(3):
+        assert relmax(Cs[c], z[f"C{c + 1}"]) <= 1e-14
+
+
+@pytest.mark.parametrize("mesh", ["ico1", "torus8x4"])
+@pytest.mark.parametrize("order", [3, 4])
+def test_hypersingular_matches_reference_dense_d(oracle, mesh, order):
+    """sum_c C_c^T V C_c through three oracle P0 applies equals the
+    reference's dense D; D annihilates constants (surface curls of a
+    constant vanish)."""
+    z = gold(f"laplace_{mesh}_o{order}.npz")
+    P, T, D = z["nodes"], z["elements"], z["D"]
+    x = np.random.default_rng(4).uniform(-1, 1, P.shape[0])
+    y = oracle.hypersingular_apply(P, T, x, line_idx=order, tri_idx=order)
+    assert np.linalg.norm(y - D @ x) <= 1e-13 * np.linalg.norm(D @ x)
+    assert np.abs(D @ np.ones(P.shape[0])).max() <= 1e-13 * np.abs(D).max()
+
+
 def test_kelvin_dense_matches_reference(oracle):
     g = gold("kelvin_ico1_o4.npz")
     V = oracle.kelvin_dense(g["nodes"], g["elements"], float(g["nu"]), 4)
