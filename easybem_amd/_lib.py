@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libeasybem_hip.so")
 TUNE_FILE = os.path.join(PKG, "tuning_mi355x.txt")
 
-KERNELS = {"V": 0, "K": 1, "kelvin": 2, "D": 3}
+KERNELS = {"V": 0, "K": 1, "kelvin": 2, "D": 3, "Kt": 4}
 EBH_OK = 0
 
 
@@ -398,7 +398,7 @@ class RequestResult(C.Structure):
 
 
 class Server:
-    """Continuous-batching Dirichlet solve server (ebh_server_*)."""
+    """Continuous-batching Dirichlet / Neumann solve server (ebh_server_*)."""
 
     def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
                  mixed=True, line_rule=4, tri_rule=4, max_iters=2000):
@@ -407,6 +407,9 @@ class Server:
         L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
         L.ebh_server_submit.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp,
                                         C.c_uint64, C.POINTER(C.c_int64)]
+        L.ebh_server_submit_bvp.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int,
+                                            vp, vp, C.c_uint64,
+                                            C.POINTER(C.c_int64)]
         L.ebh_server_wait.argtypes = [vp, C.c_int64, C.c_double,
                                       C.POINTER(RequestResult), vp]
         L.ebh_server_stats.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -419,15 +422,19 @@ class Server:
         self.h = h
         self.ne = {}
 
-    def submit(self, nodes, elements, x0, key=0):
+    def submit(self, nodes, elements, x0, key=0, problem="dirichlet"):
+        """problem: "dirichlet" (returns the P0 flux, ne entries) or
+        "neumann" (returns the P1 trace, np entries)."""
         nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         elements = np.ascontiguousarray(elements, dtype=np.int32)
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        kind = {"dirichlet": 0, "neumann": 1}[problem]
         rid = C.c_int64()
-        check(lib().ebh_server_submit(self.h, nodes.shape[0], _ptr(nodes),
-                                      elements.shape[0], _ptr(elements),
-                                      _ptr(x0), key, C.byref(rid)))
-        self.ne[rid.value] = elements.shape[0]
+        check(lib().ebh_server_submit_bvp(self.h, kind, nodes.shape[0],
+                                          _ptr(nodes), elements.shape[0],
+                                          _ptr(elements), _ptr(x0), key,
+                                          C.byref(rid)))
+        self.ne[rid.value] = nodes.shape[0] if kind else elements.shape[0]
         return rid.value
 
     def wait(self, rid, timeout=0.0, raise_on_error=False):

@@ -179,7 +179,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (!mesh || !desc || !out) { set_error("null argument"); return EBH_ERR_ARG; }
   *out = nullptr;
   const ebh_op_desc &d = *desc;
-  if (d.kernel < 0 || d.kernel > 3) { set_error("unknown kernel"); return EBH_ERR_ARG; }
+  if (d.kernel < 0 || d.kernel > 4) { set_error("unknown kernel"); return EBH_ERR_ARG; }
   if ((d.test_basis != 0 && d.test_basis != 1) ||
       (d.trial_basis != 0 && d.trial_basis != 1)) {
     set_error("basis must be EBH_P0 or EBH_P1");
@@ -192,6 +192,12 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (d.kernel == EBH_LAPLACE_HS &&
       (d.test_basis != EBH_P1 || d.trial_basis != EBH_P1)) {
     set_error("the hypersingular operator needs P1 test and trial spaces");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_LAPLACE_ADL &&
+      (d.test_basis != EBH_P1 || d.far_precision != EBH_FP64)) {
+    set_error("the adjoint double layer needs a P1 test space and the FP64 far "
+              "field ((y-x).n cancels to O(r^2) on smooth surfaces)");
     return EBH_ERR_UNSUPPORTED;
   }
   if (d.kernel == EBH_LAPLACE_HS ? !std::isfinite(d.stab) : d.stab != 0.0) {

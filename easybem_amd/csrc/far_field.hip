@@ -70,6 +70,7 @@ struct FarArgs {
                                       // 2(P1-C), c_q^2 |P_k-C|^2 per point;
                                       // others: C, then 2(x_q-C), |x_q-C|^2
   R rc[16];                           // median rules: c_q of point q
+  const double *tn;                   // adjoint double layer: unit normals [ne][3]
   const R *src;                       // [ne*nt][REC]
   const int *touch_ptr, *touch;
   const double *tw;                   // [nt][ne]
@@ -81,7 +82,8 @@ struct FarArgs {
 // Kelvin (x, y, z, s0, s1, s2, -, -); FP32 double layer keeps the source
 // point as a hi + lo float pair: (hi xyz, lo xyz, p xyz, -, -, -).
 template <typename R, int KIND> struct Rec {
-  static constexpr int n = KIND == 0 ? 4 : (KIND == 1 && sizeof(R) == 4 ? 12 : 8);
+  static constexpr int n =
+      KIND == 0 || KIND == 3 ? 4 : (KIND == 1 && sizeof(R) == 4 ? 12 : 8);
 };
 
 template <typename R, int KIND, int NT>
@@ -92,7 +94,13 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   // r^2 = |D|^2 + c G_k + c^2 |P_k - C|^2, G_k = 2 D.(P_k - C), and
   // G_0 + G_1 + G_2 = 0.  Per source point: D, |D|^2, G_0, G_1, G_2; per
   // interaction: one add and one FMA (none for the centroid) before rsqrt.
-  constexpr bool RAY = KIND == 0 && NT <= 7;
+  //
+  // Adjoint double layer (KIND 3): the same target frame, with the kernel
+  // (y - x).n_x / r^3 at target point x.  Every rule point lies in the
+  // target's plane, so (y - x).n_x = -(C - y).n_x is one value per source
+  // point and target triangle.
+  constexpr bool SLF = KIND == 0 || KIND == 3;  // single-layer target frame
+  constexpr bool RAY = SLF && NT <= 7;
   constexpr int C0 = NT % 3;  // 1 when the centroid leads the rule
   constexpr int REC = Rec<R, KIND>::n;
   constexpr int NC = KIND == 2 ? 3 : 1;
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
       X[q] = Y[q] = Z[q] = 0;
       E[q] = a.tsl[(9 + q) * (size_t)ne + tt];
     }
-  } else if constexpr (KIND == 0) {
+  } else if constexpr (SLF) {
     Cx = a.tsl[0 * (size_t)ne + tt];
     Cy = a.tsl[1 * (size_t)ne + tt];
     Cz = a.tsl[2 * (size_t)ne + tt];
@@ -150,6 +158,12 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
       Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
       E[q] = 0;
     }
+  }
+  R Nx = 0, Ny = 0, Nz = 0;
+  if constexpr (KIND == 3) {
+    Nx = (R)a.tn[3 * (size_t)tt];
+    Ny = (R)a.tn[3 * (size_t)tt + 1];
+    Nz = (R)a.tn[3 * (size_t)tt + 2];
   }
   R Ch[3] = {0, 0, 0}, Cl[3] = {0, 0, 0};
   if constexpr (COMP) {
@@ -196,8 +210,10 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
       for (int b = 0; b < NT; b++, rec += REC) {
         const R sx = rec[0], sy = rec[1], sz = rec[2];
         if constexpr (RAY) {
-          const R sg = rec[3] * keep;
           const R Dx = Cx - sx, Dy = Cy - sy, Dz = Cz - sz;
+          // single layer: sigma; adjoint double layer: sigma (y - x).n_x
+          const R sg = KIND == 3 ? -rec[3] * keep * fma(Dx, Nx, fma(Dy, Ny, Dz * Nz))
+                                 : rec[3] * keep;
           // an excluded (touching) triangle gets r^2 + 1 with strength 0
           const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
           R G[3];
@@ -207,18 +223,23 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 #pragma unroll
           for (int q = 0; q < NT; q++) {
             const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
-            acc[q] = fma(sg, rsq(r2), acc[q]);
+            const R ri = rsq(r2);
+            if constexpr (KIND == 3) acc[q] = fma(sg, ri * ri * ri, acc[q]);
+            else acc[q] = fma(sg, ri, acc[q]);
           }
-        } else if constexpr (KIND == 0) {
-          const R sg = rec[3] * keep;
+        } else if constexpr (SLF) {
           const R Dx = Cx - sx, Dy = Cy - sy, Dz = Cz - sz;
+          const R sg = KIND == 3 ? -rec[3] * keep * fma(Dx, Nx, fma(Dy, Ny, Dz * Nz))
+                                 : rec[3] * keep;
           // an excluded (touching) triangle gets |D|^2 + 1: its strength is
           // 0 and r^2 stays positive, also at a coincident point
           const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
 #pragma unroll
           for (int q = 0; q < NT; q++) {
             const R B = fma(Dx, X[q], fma(Dy, Y[q], fma(Dz, Z[q], E[q])));
-            acc[q] = fma(sg, rsq(A + B), acc[q]);
+            const R ri = rsq(A + B);
+            if constexpr (KIND == 3) acc[q] = fma(sg, ri * ri * ri, acc[q]);
+            else acc[q] = fma(sg, ri, acc[q]);
           }
         } else if constexpr (COMP) {
           const R px = rec[6] * keep, py = rec[7] * keep, pz = rec[8] * keep;
@@ -588,6 +609,7 @@ int run_far(ebh_op *op, hipStream_t s)
   a.touch_ptr = op->mesh->touch_ptr.p;
   a.touch = op->mesh->touch.p;
   a.tw = op->tw.p;
+  a.tn = op->normals.p;
   a.partial = op->partial.p;
   for (int q = 0; q < 16; q++) {
     const bool in = q < nt;
@@ -598,11 +620,14 @@ int run_far(ebh_op *op, hipStream_t s)
   }
   const int tb = (ne + op->block - 1) / op->block;
   dim3 grid((unsigned)(tb * op->chunks)), blk(op->block);
-  const int rec = op->d.kernel == 0 ? 4 : (op->d.kernel == 1 && sizeof(R) == 4 ? 12 : 8);
+  const int k = op->d.kernel;
+  const int rec = k == EBH_LAPLACE_SL || k == EBH_LAPLACE_ADL
+                      ? 4 : (k == EBH_LAPLACE_DL && sizeof(R) == 4 ? 12 : 8);
   const size_t lds = (size_t)op->tile * nt * rec * sizeof(R);
-  switch (op->d.kernel) {
+  switch (k) {
   case EBH_LAPLACE_SL: return launch_far<R, 0>(nt, grid, blk, lds, s, a);
   case EBH_LAPLACE_DL: return launch_far<R, 1>(nt, grid, blk, lds, s, a);
+  case EBH_LAPLACE_ADL: return launch_far<R, 3>(nt, grid, blk, lds, s, a);
   default: return launch_far<R, 2>(nt, grid, blk, lds, s, a);
   }
 }
@@ -621,6 +646,7 @@ int run_fold(ebh_op *op, const double *x, hipStream_t s)
   const int p1 = op->d.trial_basis;
   switch (op->d.kernel) {
   case EBH_LAPLACE_SL:
+  case EBH_LAPLACE_ADL:  // point strengths, as the single layer
     hipLaunchKernelGGL((k_fold<R, 0>), grid, blk, 0, s, ne, nt, op->sg64.p,
                        op->swt.p, op->normals.p, op->mesh->tri.p, p1, op->rule,
                        x, op->ncols, scale, src);
@@ -822,12 +848,15 @@ int op_build_far(ebh_op *op)
     }
     EBH_TRY(op->tq32.upload(t32));
     EBH_TRY(op->src32.alloc((size_t)ne * nt *
-                            (op->d.kernel == 0 ? 4 : op->d.kernel == 1 ? 12 : 8)));
+                            (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL
+                                 ? 4 : op->d.kernel == EBH_LAPLACE_DL ? 12 : 8)));
   } else {
     EBH_TRY(op->tq64.upload(tq));
-    EBH_TRY(op->src64.alloc((size_t)ne * nt * (op->d.kernel == 0 ? 4 : 8)));
+    EBH_TRY(op->src64.alloc((size_t)ne * nt *
+                            (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL
+                                 ? 4 : 8)));
   }
-  if (op->d.kernel == EBH_LAPLACE_SL) {
+  if (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL) {
     // single-layer target frame: centroid, then the median form (ray rules)
     // or the offsets of every point from the centroid
     const bool ray = op->ray && nt <= 7;

@@ -209,7 +209,10 @@ __global__ void k_near_assemble(int nnz, const int *__restrict__ cptr,
 int op_build_near(ebh_op *op)
 {
   const Topology &T = op->mesh->T;
-  const int kind = op->d.kernel;
+  // the adjoint double layer's near field is the transpose of the double
+  // layer's (test basis = the adjoint's trial basis, trial basis P1)
+  const bool adj = op->d.kernel == EBH_LAPLACE_ADL;
+  const int kind = adj ? EBH_LAPLACE_DL : op->d.kernel;
   const int nval = kind == EBH_KELVIN_SL ? 63 : 9;
   const size_t npairs = T.pairs.size();
   // pair info and per-class id lists
@@ -284,7 +287,8 @@ int op_build_near(ebh_op *op)
     EBH_HIP(hipStreamSynchronize(op->stream));
   }
   NearCSR csr;
-  build_near_csr(T, op->d.test_basis, op->d.trial_basis, csr);
+  build_near_csr(T, adj ? op->d.trial_basis : op->d.test_basis,
+                 adj ? EBH_P1 : op->d.trial_basis, csr);
   DevBuf<int> d_cptr;
   DevBuf<int64_t> d_contrib;
   EBH_TRY(op->near_ptr.upload(csr.row_ptr));
@@ -300,6 +304,26 @@ int op_build_near(ebh_op *op)
                      op->d.nu, op->near_val.p);
   EBH_HIP(hipGetLastError());
   EBH_HIP(hipStreamSynchronize(op->stream));
+  if (adj) {
+    // transpose on the host once: rows become the P1 test nodes
+    std::vector<double> val(nnz), tval(nnz);
+    EBH_HIP(hipMemcpy(val.data(), op->near_val.p, sizeof(double) * nnz,
+                      hipMemcpyDeviceToHost));
+    const int nr = csr.nrows, nc = T.np;
+    std::vector<int> tptr(nc + 1, 0), tcol(nnz);
+    for (int e = 0; e < nnz; e++) tptr[csr.col[e] + 1]++;
+    for (int c = 0; c < nc; c++) tptr[c + 1] += tptr[c];
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (int r = 0; r < nr; r++)
+      for (int e = csr.row_ptr[r]; e < csr.row_ptr[r + 1]; e++) {
+        const int d = fill[csr.col[e]]++;
+        tcol[d] = r;
+        tval[d] = val[e];
+      }
+    EBH_TRY(op->near_ptr.upload(tptr));
+    EBH_TRY(op->near_col.upload(tcol));
+    EBH_TRY(op->near_val.upload(tval));
+  }
   return EBH_OK;
 }
 

@@ -38,6 +38,7 @@ double ms_between(Clock::time_point a, Clock::time_point b)
 
 struct Request {
   int64_t id = 0;
+  int problem = EBH_DIRICHLET;
   int np = 0, ne = 0;
   std::vector<double> nodes;
   std::vector<int> elems;
@@ -51,23 +52,28 @@ struct Request {
   bool done = false;
 };
 
-// A resident mesh with the operators and solver of one request type.
+// A resident mesh with the operators and solver of one problem type:
+// Dirichlet: A = P0 single layer V, data operator the double layer K;
+// Neumann: A = stabilised hypersingular D + m m^T on P1, data operator the
+// adjoint double layer K'.
 struct Entry {
   uint64_t key = 0;
   ebh_mesh *mesh = nullptr;
-  ebh_op *V64 = nullptr, *V32 = nullptr, *K64 = nullptr;
+  ebh_op *A64 = nullptr, *A32 = nullptr, *K64 = nullptr;
   ebh_solver *solver = nullptr;
   size_t bytes = 0;
   uint64_t last_use = 0;
   ~Entry()
   {
     ebh_solver_destroy(solver);
-    ebh_op_destroy(V64);
-    ebh_op_destroy(V32);
+    ebh_op_destroy(A64);
+    ebh_op_destroy(A32);
     ebh_op_destroy(K64);
     ebh_mesh_destroy(mesh);
   }
 };
+
+using CacheKey = std::pair<uint64_t, int>;  // (mesh key, problem)
 
 // Device bytes of one request, estimated before anything is allocated:
 // near-field pairs (about 13 per triangle for V, with their FP64 blocks
@@ -92,7 +98,7 @@ struct ebh_server {
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;
   std::map<int64_t, std::shared_ptr<Request>> reqs;
-  std::multimap<uint64_t, std::unique_ptr<Entry>> idle;  // resident, not in use
+  std::multimap<CacheKey, std::unique_ptr<Entry>> idle;  // resident, not in use
   std::vector<std::thread> workers;
   bool stopping = false;
   size_t budget = 0, held = 0, peak = 0;
@@ -123,7 +129,7 @@ struct ebh_server {
           if (queue.empty()) return stopping;
           auto &h = queue.front();
           if (h->key) {
-            auto it = idle.find(h->key);
+            auto it = idle.find(CacheKey(h->key, h->problem));
             if (it != idle.end()) return true;
           }
           return evict_for(h->footprint) || (held == 0);
@@ -131,7 +137,7 @@ struct ebh_server {
         if (queue.empty()) return;
         r = queue.front();
         queue.pop_front();
-        auto it = r->key ? idle.find(r->key) : idle.end();
+        auto it = r->key ? idle.find(CacheKey(r->key, r->problem)) : idle.end();
         if (it != idle.end()) {
           entry = std::move(it->second);
           idle.erase(it);
@@ -163,7 +169,7 @@ struct ebh_server {
         if (entry && r->key && rc == EBH_OK) {
           entry->last_use = ++tick;
           entry->bytes = bytes;
-          idle.emplace(r->key, std::move(entry));
+          idle.emplace(CacheKey(r->key, r->problem), std::move(entry));
         } else {
           held -= bytes;
         }
@@ -179,37 +185,39 @@ struct ebh_server {
     e.reset(new Entry());
     e->key = r.key;
     EBH_TRY(ebh_mesh_create(r.np, r.nodes.data(), r.ne, r.elems.data(), &e->mesh));
+    const bool neu = r.problem == EBH_NEUMANN;
     ebh_op_desc d{};
-    d.kernel = EBH_LAPLACE_SL;
-    d.test_basis = EBH_P0;
-    d.trial_basis = EBH_P0;
+    d.kernel = neu ? EBH_LAPLACE_HS : EBH_LAPLACE_SL;
+    d.test_basis = d.trial_basis = neu ? EBH_P1 : EBH_P0;
     d.far_precision = EBH_FP64;
     d.line_rule = o.line_rule;
     d.tri_rule = o.tri_rule;
     d.max_rhs = 1;
-    EBH_TRY(ebh_op_create(e->mesh, &d, &e->V64));
+    // D + m m^T (laplace3dNeumannGPU.py:43-49): constants carry m.u
+    d.stab = neu ? 1.0 : 0.0;
+    EBH_TRY(ebh_op_create(e->mesh, &d, &e->A64));
     if (o.mixed) {
       d.far_precision = EBH_FP32;
-      EBH_TRY(ebh_op_create(e->mesh, &d, &e->V32));
+      EBH_TRY(ebh_op_create(e->mesh, &d, &e->A32));
     }
-    d.kernel = EBH_LAPLACE_DL;
-    d.trial_basis = EBH_P1;
+    d.stab = 0.0;
+    d.kernel = neu ? EBH_LAPLACE_ADL : EBH_LAPLACE_DL;
+    d.test_basis = neu ? EBH_P1 : EBH_P0;
+    d.trial_basis = neu ? EBH_P0 : EBH_P1;
     d.far_precision = EBH_FP64;
     EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
-    EBH_TRY(ebh_solver_create(e->V64, e->V32, o.restart, &e->solver));
+    EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
     // eager launches: stream capture must not overlap other threads'
     // allocations and copies, which the other workers make at any time
     EBH_TRY(ebh_solver_set(e->solver, 0, 0));
     return EBH_OK;
   }
 
-  int run(Request &r, std::unique_ptr<Entry> &e)
+  // Dirichlet data u = 1/|x - x0| at the nodes; b = (M/2 + K) u, the first
+  // Calderon row V t = (1/2 + K) u (laplace3dDirichletGPU.py:60-70).
+  int dirichlet_rhs(const Request &r, Entry &e, std::vector<double> &b)
   {
-    const auto t0 = Clock::now();
-    r.res.ne = r.ne;
-    if (!e) EBH_TRY(build(r, e));
-    // boundary data u = 1/|x - x0| and b = (1/2 M + K) u
-    std::vector<double> u(r.np), mu_(r.ne), ku(r.ne), b(r.ne);
+    std::vector<double> u(r.np), mu_(r.ne), ku(r.ne);
     for (int n = 0; n < r.np; n++) {
       const double dx = r.nodes[3 * n] - r.x0[0], dy = r.nodes[3 * n + 1] - r.x0[1],
                    dz = r.nodes[3 * n + 2] - r.x0[2];
@@ -219,12 +227,68 @@ struct ebh_server {
         return EBH_ERR_NONFINITE;
       }
     }
-    EBH_TRY(ebh_mass_apply(e->mesh, EBH_P0, u.data(), mu_.data()));
-    EBH_TRY(ebh_op_apply_host(e->K64, u.data(), ku.data(), 1));
+    EBH_TRY(ebh_mass_apply(e.mesh, EBH_P0, u.data(), mu_.data()));
+    EBH_TRY(ebh_op_apply_host(e.K64, u.data(), ku.data(), 1));
     for (int i = 0; i < r.ne; i++) b[i] = 0.5 * mu_[i] + ku[i];
+    return EBH_OK;
+  }
+
+  // Neumann data t = du/dn of u = 1/|x - x0|, the mean of -(x - x0).n/|x -
+  // x0|^3 over each triangle by the far-field rule; b = (M^T/2 - K'^T) t,
+  // the second Calderon row D u = (1/2 - K') t.
+  int neumann_rhs(const Request &r, Entry &e, std::vector<double> &b)
+  {
+    std::vector<double> rx, ry, rw;
+    if (!ebh::tri_rule(o.tri_rule, rx, ry, rw)) {
+      ebh::set_error("tri_rule index must be 0..6");
+      return EBH_ERR_ARG;
+    }
+    std::vector<double> t(r.ne), kt(r.np);
+    std::fill(b.begin(), b.end(), 0.0);
+    for (int i = 0; i < r.ne; i++) {
+      const double *P0 = &r.nodes[3 * (r.elems[3 * i] - 1)];
+      const double *P1 = &r.nodes[3 * (r.elems[3 * i + 1] - 1)];
+      const double *P2 = &r.nodes[3 * (r.elems[3 * i + 2] - 1)];
+      double u[3], v[3];
+      for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+      double nn[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                      u[0] * v[1] - u[1] * v[0]};
+      const double J = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+      double acc = 0.0;
+      for (size_t q = 0; q < rw.size(); q++) {
+        double d[3], r2 = 0.0, dn = 0.0;
+        for (int k = 0; k < 3; k++) {
+          d[k] = P0[k] + rx[q] * u[k] + ry[q] * v[k] - r.x0[k];
+          r2 += d[k] * d[k];
+          dn += d[k] * nn[k] / J;
+        }
+        acc += 2.0 * rw[q] * (-dn / (r2 * std::sqrt(r2)));
+      }
+      if (!std::isfinite(acc)) {
+        ebh::set_error("source point lies on the surface");
+        return EBH_ERR_NONFINITE;
+      }
+      t[i] = acc;
+      for (int k = 0; k < 3; k++) b[r.elems[3 * i + k] - 1] += 0.5 * J / 6.0 * acc;
+    }
+    EBH_TRY(ebh_op_apply_host(e.K64, t.data(), kt.data(), 1));
+    for (int n = 0; n < r.np; n++) b[n] -= kt[n];
+    return EBH_OK;
+  }
+
+  int run(Request &r, std::unique_ptr<Entry> &e)
+  {
+    const auto t0 = Clock::now();
+    r.res.ne = r.problem == EBH_NEUMANN ? r.np : r.ne;
+    if (!e) EBH_TRY(build(r, e));
+    const bool neu = r.problem == EBH_NEUMANN;
+    const int n = neu ? r.np : r.ne;
+    std::vector<double> b(n);
+    if (neu) EBH_TRY(neumann_rhs(r, *e, b));
+    else EBH_TRY(dirichlet_rhs(r, *e, b));
     ebh::DevBuf<double> db, dx;
     EBH_TRY(db.upload(b));
-    std::vector<double> zero(r.ne, 0.0);
+    std::vector<double> zero(n, 0.0);
     EBH_TRY(dx.upload(zero));
     const auto t1 = Clock::now();
     ebh_solve_opts so{};
@@ -236,8 +300,8 @@ struct ebh_server {
     so.max_outer = 10;
     ebh_solve_stats st{};
     const int rc = ebh_solver_solve(e->solver, db.p, dx.p, &so, &st);
-    r.t.resize(r.ne);
-    EBH_HIP(hipMemcpy(r.t.data(), dx.p, r.ne * sizeof(double), hipMemcpyDeviceToHost));
+    r.t.resize(n);
+    EBH_HIP(hipMemcpy(r.t.data(), dx.p, n * sizeof(double), hipMemcpyDeviceToHost));
     const auto t2 = Clock::now();
     r.res.iters = st.iters;
     r.res.relres = st.relres;
@@ -279,11 +343,21 @@ int ebh_server_submit(ebh_server *s, int np, const double *nodes, int ne,
                       const int *elems, const double *x0, uint64_t mesh_key,
                       int64_t *request_id)
 {
-  if (!s || !nodes || !elems || !x0 || !request_id || np < 3 || ne < 4) {
+  return ebh_server_submit_bvp(s, EBH_DIRICHLET, np, nodes, ne, elems, x0,
+                               mesh_key, request_id);
+}
+
+int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *nodes,
+                          int ne, const int *elems, const double *x0,
+                          uint64_t mesh_key, int64_t *request_id)
+{
+  if (!s || !nodes || !elems || !x0 || !request_id || np < 3 || ne < 4 ||
+      (problem != EBH_DIRICHLET && problem != EBH_NEUMANN)) {
     ebh::set_error("bad submit arguments");
     return EBH_ERR_ARG;
   }
   auto r = std::make_shared<Request>();
+  r->problem = problem;
   r->np = np;
   r->ne = ne;
   r->nodes.assign(nodes, nodes + (size_t)3 * np);

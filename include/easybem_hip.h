@@ -51,8 +51,13 @@ enum {
  * D = sum_c C_c^T V C_c (+ stab m m^T), as getLaplace3d's dense D
  * (src/BEM3dLaplace.cpp:652,769) and the stub applyGPULaplace3dHyperSing
  * (src/BEM3dLaplaceGPU.cu:1554-1555); test and trial basis must be P1. */
+/* EBH_LAPLACE_ADL: the adjoint double layer, kernel (y - x).n_x / (4 pi
+ * r^3), whose Galerkin matrix is the transpose of the double layer's
+ * (getLaplace3d's K, src/BEM3dLaplace.cpp:386-437,647); it gives the
+ * Neumann right-hand side (M^T/2 - K^T) t.  P1 test, P0 or P1 trial,
+ * FP64 far field only. */
 enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2,
-       EBH_LAPLACE_HS = 3 };
+       EBH_LAPLACE_HS = 3, EBH_LAPLACE_ADL = 4 };
 enum { EBH_P0 = 0, EBH_P1 = 1 };
 enum { EBH_FP64 = 0, EBH_FP32 = 1 };
 
@@ -66,7 +71,7 @@ typedef struct ebh_server ebh_server;
  * (src/quadrature.cpp:286-287,1047-1048); the reference's single `order`
  * argument corresponds to line_rule == tri_rule == order. */
 typedef struct {
-  int kernel;        /* EBH_LAPLACE_SL/_DL/_HS, EBH_KELVIN_SL             */
+  int kernel;        /* EBH_LAPLACE_SL/_DL/_HS/_ADL, EBH_KELVIN_SL        */
   int test_basis;    /* EBH_P0 (one DOF per triangle) or EBH_P1 (nodes)   */
   int trial_basis;   /* DL requires EBH_P1; Kelvin requires test==trial   */
   int far_precision; /* EBH_FP64 or EBH_FP32 (near field is always FP64) */
@@ -264,19 +269,30 @@ typedef struct {
   double setup_ms;     /* mesh / operators / right-hand side                 */
   double solve_ms;     /* the Krylov solve                                   */
   double total_ms;     /* submit -> done                                     */
-  int ne;              /* unknowns (triangles)                               */
+  int ne;              /* unknowns: triangles (Dirichlet) or nodes (Neumann) */
   size_t footprint;    /* estimated device bytes while admitted              */
   int cache_hit;       /* 1 when the mesh was already resident               */
 } ebh_request_result;
 
 int ebh_server_create(const ebh_server_opts *opts, ebh_server **out);
-/* Copies the inputs and returns at once with a request id. */
+/* Boundary value problems for u = 1/|x - x0| inside the mesh (x0 outside):
+ * EBH_DIRICHLET solves V t = (M/2 + K) u for the P0 flux t
+ * (python/examples/laplace3dDirichletGPU.py:60-83); EBH_NEUMANN solves
+ * (D + m m^T) u = (M^T/2 - K'^T) t for the P1 trace u with the stabilised
+ * hypersingular operator (python/examples/laplace3dNeumannGPU.py:36-66). */
+enum { EBH_DIRICHLET = 0, EBH_NEUMANN = 1 };
+/* Copies the inputs and returns at once with a request id (Dirichlet). */
 int ebh_server_submit(ebh_server *s, int np, const double *nodes, int ne,
                       const int *elems, const double *x0, uint64_t mesh_key,
                       int64_t *request_id);
-/* Blocks until the request finishes (timeout_s <= 0: no limit); t (ne
- * entries, may be NULL) receives the Neumann trace.  Each id can be waited
- * on once. */
+/* Same for either problem; resident meshes are cached per (mesh_key,
+ * problem). */
+int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *nodes,
+                          int ne, const int *elems, const double *x0,
+                          uint64_t mesh_key, int64_t *request_id);
+/* Blocks until the request finishes (timeout_s <= 0: no limit); t (may be
+ * NULL) receives the unknown trace: ne entries (Dirichlet: the flux) or np
+ * entries (Neumann: the trace).  Each id can be waited on once. */
 int ebh_server_wait(ebh_server *s, int64_t request_id, double timeout_s,
                     ebh_request_result *res, double *t);
 /* [0] submitted, [1] completed, [2] failed, [3] active, [4] queued,

@@ -89,3 +89,42 @@ def test_rejects_p0_and_nonzero_stab_elsewhere(L):
         L.Operator(m, "D", 0, 1, "fp64")
     with pytest.raises(L.EbhError):
         L.Operator(m, "V", 0, 0, "fp64", stab=1.0)
+
+
+# ------------------------------------------------ adjoint double layer
+@pytest.mark.parametrize("mesh", ["ico1", "torus8x4"])
+@pytest.mark.parametrize("order", [3, 4])
+def test_adjoint_double_layer_is_reference_k_transposed(L, mesh, order):
+    """P1-test, P0-trial adjoint double layer against K^T of the
+    reference's dense K (getLaplace3d)."""
+    z = np.load(os.path.join(GOLD, f"laplace_{mesh}_o{order}.npz"))
+    P, T, K = z["nodes"], z["elements"], z["K"]
+    t = np.random.default_rng(13).uniform(-1, 1, T.shape[0])
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "Kt", 1, 0, "fp64", line_rule=order, tri_rule=order)
+    err = rel(op(t), K.T @ t)
+    print("rel err", err)
+    assert err <= 1e-12
+
+
+def test_adjoint_double_layer_p1_trial(L, oracle):
+    """P1 x P1: the transpose of the oracle's P1 x P1 double layer."""
+    P, T = meshes.perturbed_icosphere(1, 2)[:2]
+    n = P.shape[0]
+    Kd = np.stack([oracle.apply("K", 1, 1, P, T, e, line_idx=4, tri_idx=4)
+                   for e in np.eye(n)], axis=1)
+    u = np.random.default_rng(6).uniform(-1, 1, n)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "Kt", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    err = rel(op(u), Kd.T @ u)
+    print("rel err", err)
+    assert err <= 1e-12
+
+
+def test_adjoint_double_layer_rejects_fp32_and_p0_test(L):
+    P, T = meshes.icosphere(1)
+    m = L.Mesh(P, T)
+    with pytest.raises(L.EbhError):
+        L.Operator(m, "Kt", 1, 0, "fp32")
+    with pytest.raises(L.EbhError):
+        L.Operator(m, "Kt", 0, 0, "fp64")

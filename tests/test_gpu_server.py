@@ -89,3 +89,84 @@ def test_resident_mesh_reused():
     assert r1["cache_hit"] == 0 and r2["cache_hit"] == 1 and r3["cache_hit"] == 1
     assert np.array_equal(t1, t3) and not np.array_equal(t1, t2)
     assert r3["setup_ms"] < r1["setup_ms"]
+
+
+def _neumann_data(P, T, x0, tri_rule=4):
+    """Triangle means of du/dn for u = 1/|x - x0| by the 7-point rule."""
+    from oracle import Oracle
+    pts, w = Oracle().tri_rule(tri_rule)
+    A, B, Cc = (P[T[:, k] - 1] for k in range(3))
+    n = np.cross(B - A, Cc - A)
+    J = np.linalg.norm(n, axis=1)
+    n /= J[:, None]
+    t = np.zeros(T.shape[0])
+    for (s1, s2), wq in zip(pts, w):
+        x = A + s1 * (B - A) + s2 * (Cc - A)
+        d = x - x0
+        r = np.linalg.norm(d, axis=1)
+        t += 2 * wq * (-(d * n).sum(1) / r ** 3)
+    Mt = np.zeros(P.shape[0])
+    np.add.at(Mt, T - 1, (J / 6 * t)[:, None])
+    return t, Mt
+
+
+def _weighted_zero_mean(P, T, v):
+    from oracle import lumped_areas
+    m = lumped_areas(P, T)
+    return v - (m @ v) / m.sum()
+
+
+def test_neumann_server_matches_direct_solve():
+    from easybem_amd import _lib as L
+    P, T, x0 = stream(1, seed=8, levels=(3,))[0]
+    srv = L.Server(max_slots=1)
+    res, u = srv.wait(srv.submit(P, T, x0, problem="neumann"))
+    srv.close()
+    assert res["status"] == 0 and res["relres"] <= 1e-8 and u.size == P.shape[0]
+    m = L.Mesh(P, T)
+    D64 = L.Operator(m, "D", 1, 1, "fp64", stab=1.0)
+    D32 = L.Operator(m, "D", 1, 1, "fp32", stab=1.0)
+    Kt = L.Operator(m, "Kt", 1, 0, "fp64")
+    t, Mt = _neumann_data(P, T, x0)
+    b = 0.5 * Mt - Kt(t)
+    u2, st = L.Solver(D64, D32).solve(b, tol=1e-8, mixed=True)
+    assert np.linalg.norm(u - u2) <= 1e-7 * np.linalg.norm(u2)
+
+
+def test_neumann_solution_converges_to_analytic_trace():
+    """Interior Neumann problem for u = 1/|x - x0|: the P1 trace matches u
+    up to a constant, with an error that shrinks under refinement."""
+    from easybem_amd import _lib as L
+    x0 = np.array([0.3, -1.9, 0.8])
+    srv = L.Server(max_slots=2)
+    errs = []
+    for lev in (2, 3, 4):
+        P, T = meshes.icosphere(lev)
+        res, uh = srv.wait(srv.submit(P, T, x0, problem="neumann"))
+        assert res["status"] == 0 and res["relres"] <= 1e-8, res
+        u = 1.0 / np.linalg.norm(P - x0, axis=1)
+        a, b = _weighted_zero_mean(P, T, uh), _weighted_zero_mean(P, T, u)
+        errs.append(np.linalg.norm(a - b) / np.linalg.norm(b))
+    srv.close()
+    print("relative trace errors", errs)
+    assert errs[2] < 0.5 * errs[1] < 0.25 * errs[0] and errs[2] < 1e-2
+
+
+def test_dirichlet_and_neumann_mix_identical_alone():
+    from easybem_amd import _lib as L
+    reqs = [(P, T, x0, ("dirichlet", "neumann")[k % 2])
+            for k, (P, T, x0) in enumerate(stream(6, seed=9, levels=(2, 3)))]
+    srv = L.Server(max_slots=1)
+    alone = []
+    for P, T, x0, pb in reqs:
+        res, v = srv.wait(srv.submit(P, T, x0, problem=pb))
+        assert res["status"] == 0, res
+        alone.append(v)
+    srv.close()
+    srv = L.Server(max_slots=3)
+    ids = [srv.submit(P, T, x0, problem=pb) for P, T, x0, pb in reqs]
+    for rid, v0 in zip(ids, alone):
+        res, v = srv.wait(rid)
+        assert res["status"] == 0 and res["relres"] <= 1e-8
+        assert np.array_equal(v, v0)
+    srv.close()
