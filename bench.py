@@ -130,41 +130,50 @@ def secondary(L, meshes, solve_batched=False):
     return out
 
 
+# single-stream service time (s) of one solve, by (problem, level): sets the
+# offered load of the serving stream
+SERVICE_S = {("dirichlet", 4): 0.08, ("dirichlet", 5): 0.45,
+             ("dirichlet", 6): 5.0, ("neumann", 4): 0.2, ("neumann", 5): 1.2,
+             ("neumann", 6): 12.0}
+
+
 def serving(L, meshes, n_requests=12, levels=(4, 5), rate=None, slots=4,
-            seed=20261203):
-    """Config 5: a stream of Dirichlet solves with Poisson arrivals.  Each
-    mesh is a seeded icosphere (level drawn from `levels`) with radius
-    1 + 0.1 sum_{l<=4} c_lm Y_lm and axes scaled by U(0.5, 1.5); the
-    boundary data comes from a point source outside the mesh.  Latency is
-    submit -> result."""
+            seed=20261203, neumann_share=1 / 3):
+    """Config 5: a stream of Dirichlet and Neumann solves with Poisson
+    arrivals.  Each mesh is a seeded icosphere (level drawn from `levels`)
+    with radius 1 + 0.1 sum_{l<=4} c_lm Y_lm and axes scaled by
+    U(0.5, 1.5); the boundary data comes from a point source outside the
+    mesh.  Latency is submit -> result."""
     rng = np.random.default_rng(seed)
     reqs = []
     for _ in range(n_requests):
         lev = int(rng.choice(levels))
+        pb = "neumann" if rng.random() < neumann_share else "dirichlet"
         P, T, x0 = meshes.perturbed_icosphere(lev, int(rng.integers(1 << 30)))
-        reqs.append((lev, P, T, x0))
+        reqs.append((lev, pb, P, T, x0))
     srv = L.Server(max_slots=slots, tol=1e-8, mixed=True)
     # warm the runtime (kernels loaded, streams created) on a small request
     P, T, x0 = meshes.perturbed_icosphere(2, 1)
     srv.wait(srv.submit(P, T, x0))
     if rate is None:
         # offered load about 0.7 of the single-stream service rate
-        rate = 0.7 / np.mean([{4: 0.08, 5: 0.45, 6: 5.0}[lv] for lv, *_ in reqs])
+        rate = 0.7 / np.mean([SERVICE_S[(pb, lv)] for lv, pb, *_ in reqs])
     gaps = rng.exponential(1.0 / rate, n_requests)
     t0 = time.perf_counter()
     ids, arrivals = [], []
     t_next = t0
-    for (lev, P, T, x0), gap in zip(reqs, gaps):
+    for (lev, pb, P, T, x0), gap in zip(reqs, gaps):
         t_next += gap
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
         arrivals.append(time.perf_counter() - t0)
-        ids.append((lev, srv.submit(P, T, x0)))
+        ids.append((lev, pb, srv.submit(P, T, x0, problem=pb)))
     res = []
-    for lev, rid in ids:
+    for lev, pb, rid in ids:
         r, _ = srv.wait(rid)
         r["level"] = lev
+        r["problem"] = pb
         res.append(r)
     wall = time.perf_counter() - t0
     st = srv.stats()
@@ -172,11 +181,16 @@ def serving(L, meshes, n_requests=12, levels=(4, 5), rate=None, slots=4,
     ok = [r for r in res if r["status"] == 0 and r["relres"] <= 1e-8]
     lat = np.array([r["total_ms"] for r in ok])
     by_level = {}
-    for lev in sorted(set(levels)):
-        v = np.array([r["total_ms"] for r in ok if r["level"] == lev])
-        if len(v):
-            by_level[f"L{lev}"] = {"n": int(len(v)), "p50_ms": float(np.percentile(v, 50)),
-                                   "max_ms": float(v.max())}
+    for pb in ("dirichlet", "neumann"):
+        for lev in sorted(set(levels)):
+            v = np.array([r["total_ms"] for r in ok
+                          if r["level"] == lev and r["problem"] == pb])
+            if len(v):
+                by_level[f"{pb} L{lev}"] = {
+                    "n": int(len(v)), "p50_ms": float(np.percentile(v, 50)),
+                    "max_ms": float(v.max()),
+                    "iters_max": int(max(r["iters"] for r in ok
+                                         if r["level"] == lev and r["problem"] == pb))}
     return {"requests": n_requests, "completed": len(ok), "levels": list(levels),
             "arrival_rate_per_s": rate, "slots": slots,
             "solves_per_s": len(ok) / wall,

@@ -328,6 +328,48 @@ class Solver:
             pass
 
 
+class Batch:
+    """Operators of one kind on several meshes applied with one persistent
+    far-field launch (ebh_batch_*)."""
+
+    def __init__(self, ops):
+        L = lib()
+        L.ebh_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int,
+                                       C.POINTER(C.c_void_p)]
+        L.ebh_batch_apply.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.c_void_p]
+        L.ebh_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.ebh_batch_destroy.argtypes = [C.c_void_p]
+        L.ebh_batch_destroy.restype = None
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[o.h.value for o in self.ops])
+        h = C.c_void_p()
+        check(L.ebh_batch_create(arr, len(self.ops), C.byref(h)))
+        self.h = h
+
+    def apply(self, xs, ys, stream=None):
+        n = len(self.ops)
+        xa = (C.c_void_p * n)(*[x.ptr for x in xs])
+        ya = (C.c_void_p * n)(*[y.ptr for y in ys])
+        check(lib().ebh_batch_apply(self.h, xa, ya, stream))
+
+    def info(self):
+        a = (C.c_int64 * 4)()
+        check(lib().ebh_batch_info(self.h, a))
+        return {"ops": a[0], "items": a[1], "workgroups": a[2], "lds": a[3]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class TuneResult(C.Structure):
     _fields_ = [("block", C.c_int), ("tile", C.c_int), ("chunks", C.c_int),
                 ("grid", C.c_int), ("mgrid", C.c_int), ("ms", C.c_double),

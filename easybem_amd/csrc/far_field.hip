@@ -86,8 +86,12 @@ template <typename R, int KIND> struct Rec {
       KIND == 0 || KIND == 3 ? 4 : (KIND == 1 && sizeof(R) == 4 ? 12 : 8);
 };
 
+// One workgroup's share of the far field: target triangles
+// tblk*blockDim .. +blockDim against source chunk s, partial sums to
+// a.partial[s].  Every thread of the workgroup calls it (it uses barriers).
 template <typename R, int KIND, int NT>
-__global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
+__device__ __forceinline__ void far_block(const FarArgs<R> &a, int s, int tblk,
+                                          R *tile)
 {
   // Single layer with a median-symmetric rule (the point set is the
   // centroid C and/or orbits C + c (P_k - C), k = 0, 1, 2): with D = C - y,
@@ -109,11 +113,8 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   // panels of a smooth surface, so x - y is formed as
   // (c_hi - y_hi) + (c_lo - y_lo) + (x - c) with c the target centroid.
   constexpr bool COMP = F32 && KIND == 1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  R *tile = reinterpret_cast<R *>(smem_raw);
   const int ne = a.ne;
-  const int s = blockIdx.x % a.chunks;
-  const int t = (blockIdx.x / a.chunks) * blockDim.x + threadIdx.x;
+  const int t = tblk * blockDim.x + threadIdx.x;
   const int tt = t < ne ? t : ne - 1;
   const int j0 = s * a.chunk;
   const int j1 = min(ne, j0 + a.chunk);
@@ -283,33 +284,79 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NT * NC; k++) acc64[k] += (double)acc[k];
   }
-  if (t >= ne) return;
-  const size_t stride = (size_t)ne;
-  double *out = a.partial + (size_t)s * a.nout * NC * stride + t;
-  if (a.nout == 1) {
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-      double v = 0.0;
-#pragma unroll
-      for (int q = 0; q < NT; q++) v = fma(a.tw[q * stride + t], acc64[q * NC + c], v);
-      out[c * stride] = v;
-    }
-  } else {
-    double wq[NT];
-#pragma unroll
-    for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + t];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
+  if (t < ne) {
+    const size_t stride = (size_t)ne;
+    double *out = a.partial + (size_t)s * a.nout * NC * stride + t;
+    if (a.nout == 1) {
 #pragma unroll
       for (int c = 0; c < NC; c++) {
         double v = 0.0;
 #pragma unroll
-        for (int q = 0; q < NT; q++)
-          v = fma(wq[q] * a.phi[q][k], acc64[q * NC + c], v);
-        out[(k * NC + c) * stride] = v;
+        for (int q = 0; q < NT; q++) v = fma(a.tw[q * stride + t], acc64[q * NC + c], v);
+        out[c * stride] = v;
       }
+    } else {
+      double wq[NT];
+#pragma unroll
+      for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + t];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          double v = 0.0;
+#pragma unroll
+          for (int q = 0; q < NT; q++)
+            v = fma(wq[q] * a.phi[q][k], acc64[q * NC + c], v);
+          out[(k * NC + c) * stride] = v;
+        }
+    }
   }
   (void)F32;
+}
+
+template <typename R, int KIND, int NT>
+__global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  far_block<R, KIND, NT>(a, blockIdx.x % a.chunks, blockIdx.x / a.chunks,
+                         reinterpret_cast<R *>(smem_raw));
+}
+
+// Persistent multi-mesh far field: one launch for several operators of one
+// kind.  Work items (operator k, target block of 256 triangles, source
+// chunk s) are split into 8 queues by s mod 8; workgroup w starts on queue
+// w mod 8 -- the XCD it was dispatched to, so the workgroups sharing a
+// chunk's source records share an L2 -- and moves on to the other queues
+// when its own is drained.  Each workgroup takes items from an atomic
+// counter until every queue is empty; there is no waiting between
+// workgroups.  off[x][k] holds the first item of operator k in queue x.
+template <typename R, int KIND, int NT>
+__global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict__ descs,
+                                                   const int *__restrict__ off,
+                                                   int nops, int *__restrict__ counters)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R *tile = reinterpret_cast<R *>(smem_raw);
+  __shared__ int sh_item;
+  const int home = blockIdx.x & 7;
+  for (int c = 0; c < 8; c++) {
+    const int x = (home + c) & 7;
+    const int *o = off + x * (nops + 1);
+    const int total = o[nops];
+    while (true) {
+      __syncthreads();  // sh_item and the LDS tile of the last item are free
+      if (threadIdx.x == 0) sh_item = atomicAdd(&counters[x], 1);
+      __syncthreads();
+      const int item = sh_item;
+      if (item >= total) break;
+      int k = 0;
+      while (item >= o[k + 1]) k++;
+      const FarArgs<R> &a = descs[k];
+      const int local = item - o[k];
+      const int cx = (a.chunks - x + 7) / 8;  // chunks s = x + 8 j of this queue
+      far_block<R, KIND, NT>(a, x + 8 * (local % cx), local / cx, tile);
+    }
+  }
 }
 
 // Multi-RHS single layer.  One lane per target quadrature point, RB
@@ -449,14 +496,15 @@ __global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
 
 // Source strengths.  One lane per source quadrature point (j, b).
 template <typename R, int KIND>
-__global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
-                       const double *__restrict__ swt,
-                       const double *__restrict__ normals,
-                       const int *__restrict__ tri, int trial_p1,
-                       RuleArgs rule, const double *__restrict__ x, int ncols,
-                       double scale, R *__restrict__ src)
+__device__ __forceinline__ void fold_point(int idx, int ne, int nt,
+                                           const double *__restrict__ sg,
+                                           const double *__restrict__ swt,
+                                           const double *__restrict__ normals,
+                                           const int *__restrict__ tri, int trial_p1,
+                                           const RuleArgs &rule,
+                                           const double *__restrict__ x, int ncols,
+                                           double scale, R *__restrict__ src)
 {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ne * nt) return;
   const int j = idx / nt, b = idx - j * nt;
   constexpr int NC = KIND == 2 ? 3 : 1;
@@ -501,29 +549,49 @@ __global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
   }
 }
 
-__global__ void k_reduce(int n, int chunks, const double *__restrict__ partial,
-                         double *__restrict__ out)
+template <typename R, int KIND>
+__global__ void k_fold(int ne, int nt, const double *__restrict__ sg,
+                       const double *__restrict__ swt,
+                       const double *__restrict__ normals,
+                       const int *__restrict__ tri, int trial_p1,
+                       RuleArgs rule, const double *__restrict__ x, int ncols,
+                       double scale, R *__restrict__ src)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  fold_point<R, KIND>(blockIdx.x * blockDim.x + threadIdx.x, ne, nt, sg, swt, normals,
+                      tri, trial_p1, rule, x, ncols, scale, src);
+}
+
+__device__ __forceinline__ void reduce_one(int k, int n, int chunks,
+                                           const double *__restrict__ partial,
+                                           double *__restrict__ out)
+{
   if (k >= n) return;
   double s = 0.0;
   for (int c = 0; c < chunks; c++) s += partial[(size_t)c * n + k];
   out[k] = s;
 }
 
-// y = near CSR * x + far; one lane per test DOF.
-// blockIdx.y selects the right-hand side (vectors back to back).
-__global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
-                         const int *__restrict__ ptr, const int *__restrict__ col,
-                         const double *__restrict__ val,
-                         const double *__restrict__ x,
-                         const double *__restrict__ far,
-                         const int *__restrict__ ntp, const int *__restrict__ nt,
-                         const int *__restrict__ ntk, double *__restrict__ y)
+__global__ void k_reduce(int n, int chunks, const double *__restrict__ partial,
+                         double *__restrict__ out)
 {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  reduce_one(blockIdx.x * blockDim.x + threadIdx.x, n, chunks, partial, out);
+}
+
+// One test DOF r of right-hand side rhs: near CSR row times x plus the far
+// sums gathered from the test triangles.
+__device__ __forceinline__ void finish_row(int r, int rhs, int nrows, int ncols, int nc,
+                                           int test_p1, int ne,
+                                           const int *__restrict__ ptr,
+                                           const int *__restrict__ col,
+                                           const double *__restrict__ val,
+                                           const double *__restrict__ x,
+                                           const double *__restrict__ far,
+                                           const int *__restrict__ ntp,
+                                           const int *__restrict__ nt,
+                                           const int *__restrict__ ntk,
+                                           double *__restrict__ y)
+{
   if (r >= nrows) return;
-  const int rhs = blockIdx.y;
   x += (size_t)rhs * ncols * nc;
   y += (size_t)rhs * nrows * nc;
   far += (size_t)rhs * (test_p1 ? 3 : 1) * nc * ne;
@@ -564,6 +632,20 @@ __global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
   y[2 * (size_t)nrows + r] = s2;
 }
 
+// y = near CSR * x + far; one lane per test DOF.
+// blockIdx.y selects the right-hand side (vectors back to back).
+__global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
+                         const int *__restrict__ ptr, const int *__restrict__ col,
+                         const double *__restrict__ val,
+                         const double *__restrict__ x,
+                         const double *__restrict__ far,
+                         const int *__restrict__ ntp, const int *__restrict__ nt,
+                         const int *__restrict__ ntk, double *__restrict__ y)
+{
+  finish_row(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nrows, ncols, nc,
+             test_p1, ne, ptr, col, val, x, far, ntp, nt, ntk, y);
+}
+
 template <typename R, int KIND>
 int launch_far(int nt, dim3 grid, dim3 blk, size_t lds, hipStream_t s,
                const FarArgs<R> &a)
@@ -585,9 +667,8 @@ int launch_far(int nt, dim3 grid, dim3 blk, size_t lds, hipStream_t s,
 }
 
 template <typename R>
-int run_far(ebh_op *op, hipStream_t s)
+void fill_far_args(const ebh_op *op, FarArgs<R> &a)
 {
-  FarArgs<R> a;
   const int ne = op->mesh->T.ne, nt = op->rule.nt;
   a.ne = ne;
   a.chunks = op->chunks;
@@ -618,12 +699,27 @@ int run_far(ebh_op *op, hipStream_t s)
     a.phi[q][1] = in ? op->rule.x[q] : 0.0;
     a.phi[q][2] = in ? op->rule.y[q] : 0.0;
   }
+}
+
+// LDS bytes of one source tile
+size_t far_lds(const ebh_op *op, int R_bytes)
+{
+  const int k = op->d.kernel;
+  const int rec = k == EBH_LAPLACE_SL || k == EBH_LAPLACE_ADL
+                      ? 4 : (k == EBH_LAPLACE_DL && R_bytes == 4 ? 12 : 8);
+  return (size_t)op->tile * op->rule.nt * rec * R_bytes;
+}
+
+template <typename R>
+int run_far(ebh_op *op, hipStream_t s)
+{
+  FarArgs<R> a;
+  fill_far_args(op, a);
+  const int ne = op->mesh->T.ne, nt = op->rule.nt;
   const int tb = (ne + op->block - 1) / op->block;
   dim3 grid((unsigned)(tb * op->chunks)), blk(op->block);
   const int k = op->d.kernel;
-  const int rec = k == EBH_LAPLACE_SL || k == EBH_LAPLACE_ADL
-                      ? 4 : (k == EBH_LAPLACE_DL && sizeof(R) == 4 ? 12 : 8);
-  const size_t lds = (size_t)op->tile * nt * rec * sizeof(R);
+  const size_t lds = far_lds(op, (int)sizeof(R));
   switch (k) {
   case EBH_LAPLACE_SL: return launch_far<R, 0>(nt, grid, blk, lds, s, a);
   case EBH_LAPLACE_DL: return launch_far<R, 1>(nt, grid, blk, lds, s, a);
@@ -926,6 +1022,24 @@ int op_build_far(ebh_op *op)
   return op_configure(op, block, 64, 0);
 }
 
+// chunk reduction of the far partial sums, then near-field SpMV plus the
+// far sums gathered to the test DOFs
+static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne;
+  const int nred = (op->d.test_basis ? 3 : 1) * op->nc * ne;
+  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
+                     op->chunks, op->partial.p, op->tri_out.p);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+                     op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
+                     op->near_ptr.p, op->near_col.p, op->near_val.p, x,
+                     op->tri_out.p, op->mesh->node_tri_ptr.p,
+                     op->mesh->node_tri.p, op->mesh->node_tri_k.p, y);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
@@ -940,16 +1054,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   EBH_TRY(f32 ? run_far<float>(op, s) : run_far<double>(op, s));
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
-  const int nred = (op->d.test_basis ? 3 : 1) * op->nc * ne;
-  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
-                     op->chunks, op->partial.p, op->tri_out.p);
-  EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
-                     op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
-                     op->near_ptr.p, op->near_col.p, op->near_val.p, x,
-                     op->tri_out.p, op->mesh->node_tri_ptr.p,
-                     op->mesh->node_tri.p, op->mesh->node_tri_k.p, y);
-  EBH_HIP(hipGetLastError());
+  EBH_TRY(op_finish(op, x, y, s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }
@@ -989,4 +1094,328 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
   return EBH_OK;
 }
 
+// ---------------------------------------------------- multi-mesh batches
+
+constexpr int kMaxBatch = 64;
+
+// Per-member data of the batched fold / reduce / finish kernels.
+struct MemberDesc {
+  int ne, nt, trial_p1, ncols, nrows, nc, test_p1, chunks, nred;
+  double scale;
+  const double *sg, *swt, *normals;
+  const int *tri;
+  void *src;
+  const double *partial;
+  double *tri_out;
+  const int *near_ptr, *near_col;
+  const double *near_val;
+  const int *ntp, *ntri, *ntk;
+};
+
+// Input and output vectors of one batch apply, by value.
+struct BatchPtrs {
+  const double *x[kMaxBatch];
+  double *y[kMaxBatch];
+};
+
+template <typename R, int KIND>
+__global__ void k_fold_multi(const MemberDesc *__restrict__ d, RuleArgs rule, BatchPtrs p)
+{
+  const MemberDesc &o = d[blockIdx.y];
+  fold_point<R, KIND>(blockIdx.x * blockDim.x + threadIdx.x, o.ne, o.nt, o.sg, o.swt,
+                      o.normals, o.tri, o.trial_p1, rule, p.x[blockIdx.y], o.ncols,
+                      o.scale, reinterpret_cast<R *>(o.src));
+}
+
+// chunk partial sums of every member
+__global__ void k_reduce_multi(const MemberDesc *__restrict__ d)
+{
+  const MemberDesc &o = d[blockIdx.y];
+  reduce_one(blockIdx.x * blockDim.x + threadIdx.x, o.nred, o.chunks, o.partial,
+             o.tri_out);
+}
+
+__global__ void k_finish_multi(const MemberDesc *__restrict__ d, BatchPtrs p)
+{
+  const MemberDesc &o = d[blockIdx.y];
+  finish_row(blockIdx.x * blockDim.x + threadIdx.x, 0, o.nrows, o.ncols, o.nc,
+             o.test_p1, o.ne, o.near_ptr, o.near_col, o.near_val, p.x[blockIdx.y],
+             o.tri_out, o.ntp, o.ntri, o.ntk, p.y[blockIdx.y]);
+}
+
+__global__ void k_zero_int(int n, int *p)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <typename R, int KIND>
+const void *multi_kernel(int nt)
+{
+  switch (nt) {
+  case 1: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 1>);
+  case 3: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 3>);
+  case 4: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 4>);
+  case 6: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 6>);
+  case 7: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 7>);
+  case 12: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 12>);
+  case 13: return reinterpret_cast<const void *>(&k_far_multi<R, KIND, 13>);
+  default: return nullptr;
+  }
+}
+
+template <typename R>
+const void *multi_kernel_for(int kernel, int nt)
+{
+  switch (kernel) {
+  case EBH_LAPLACE_SL: return multi_kernel<R, 0>(nt);
+  case EBH_LAPLACE_DL: return multi_kernel<R, 1>(nt);
+  case EBH_KELVIN_SL: return multi_kernel<R, 2>(nt);
+  case EBH_LAPLACE_ADL: return multi_kernel<R, 3>(nt);
+  default: return nullptr;
+  }
+}
+
 }  // namespace ebh
+
+// Operators of one kind on several meshes, applied with one persistent
+// far-field launch.  Target blocks are 256 triangles; each operator keeps
+// its own source chunks and LDS tile, so every result equals the
+// operator's own apply.
+struct ebh_batch {
+  std::vector<ebh_op *> ops;
+  int kernel = 0, prec = 0, nt = 0, nops = 0;
+  ebh::DevBuf<unsigned char> desc;  // FarArgs<R>[nops]
+  ebh::DevBuf<unsigned char> mdesc; // MemberDesc[nops]
+  int max_fold = 0, max_red = 0, max_rows = 0;
+  ebh::DevBuf<int> off;             // [8][nops + 1] first item per queue
+  ebh::DevBuf<int> counters;        // [8]
+  std::vector<int> gens;
+  size_t lds = 0;
+  int grid = 0, items = 0;
+  const void *kern = nullptr;
+};
+
+namespace ebh {
+namespace {
+
+// Members' source chunking for a batch: one chunk size for all, chosen so
+// that the launch holds about 8 work items per resident workgroup.
+int batch_shapes(ebh_batch *b, int grid)
+{
+  double work = 0.0;
+  for (const ebh_op *op : b->ops)
+    work += (double)((op->mesh->T.ne + 255) / 256) * op->mesh->T.ne;
+  int cs = (int)std::ceil(work / (8.0 * std::max(grid, 1)));
+  cs = std::max(64, (cs + 63) / 64 * 64);
+  for (ebh_op *op : b->ops) {
+    const int ne = op->mesh->T.ne;
+    EBH_TRY(op_configure(op, 256, 64, std::max(1, (ne + cs - 1) / cs)));
+  }
+  return EBH_OK;
+}
+
+template <typename R>
+int batch_prepare(ebh_batch *b)
+{
+  const int nops = b->nops;
+  std::vector<MemberDesc> md(nops);
+  b->max_fold = b->max_red = b->max_rows = 0;
+  for (int k = 0; k < nops; k++) {
+    const ebh_op *op = b->ops[k];
+    MemberDesc &m = md[k];
+    m.ne = op->mesh->T.ne;
+    m.nt = op->rule.nt;
+    m.trial_p1 = op->d.trial_basis;
+    m.ncols = op->ncols;
+    m.nrows = op->nrows;
+    m.nc = op->nc;
+    m.test_p1 = op->d.test_basis;
+    m.chunks = op->chunks;
+    m.nred = (op->d.test_basis ? 3 : 1) * op->nc * m.ne;
+    m.scale = op->d.kernel == EBH_KELVIN_SL
+                  ? (1.0 + op->d.nu) / (8.0 * M_PI * (1.0 - op->d.nu))
+                  : kInv4Pi;
+    m.sg = op->sg64.p;
+    m.swt = op->swt.p;
+    m.normals = op->normals.p;
+    m.tri = op->mesh->tri.p;
+    if constexpr (sizeof(R) == 4) m.src = op->src32.p; else m.src = op->src64.p;
+    m.partial = op->partial.p;
+    m.tri_out = op->tri_out.p;
+    m.near_ptr = op->near_ptr.p;
+    m.near_col = op->near_col.p;
+    m.near_val = op->near_val.p;
+    m.ntp = op->mesh->node_tri_ptr.p;
+    m.ntri = op->mesh->node_tri.p;
+    m.ntk = op->mesh->node_tri_k.p;
+    b->max_fold = std::max(b->max_fold, m.ne * m.nt);
+    b->max_red = std::max(b->max_red, m.nred);
+    b->max_rows = std::max(b->max_rows, m.nrows);
+  }
+  EBH_TRY(b->mdesc.upload(reinterpret_cast<const unsigned char *>(md.data()),
+                          sizeof(MemberDesc) * nops));
+  std::vector<FarArgs<R>> d(nops);
+  std::vector<int> off(8 * (size_t)(nops + 1), 0);
+  size_t lds = 0;
+  for (int k = 0; k < nops; k++) {
+    fill_far_args(b->ops[k], d[k]);
+    lds = std::max(lds, far_lds(b->ops[k], (int)sizeof(R)));
+  }
+  int items = 0;
+  for (int x = 0; x < 8; x++) {
+    int *o = &off[(size_t)x * (nops + 1)];
+    for (int k = 0; k < nops; k++) {
+      const ebh_op *op = b->ops[k];
+      const int tb = (op->mesh->T.ne + 255) / 256;
+      const int cx = std::max(0, (op->chunks - x + 7) / 8);
+      o[k + 1] = o[k] + tb * cx;
+    }
+    items += o[nops];
+  }
+  EBH_TRY(b->desc.upload(reinterpret_cast<const unsigned char *>(d.data()),
+                         sizeof(FarArgs<R>) * nops));
+  EBH_TRY(b->off.upload(off));
+  EBH_TRY(b->counters.alloc(8));
+  b->kern = multi_kernel_for<R>(b->kernel, b->nt);
+  if (!b->kern) {
+    set_error("unsupported kernel or rule size for a batch");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  int per_cu = 0, cus = 0;
+  EBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->kern, 256, lds));
+  EBH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                b->ops[0]->mesh->device));
+  b->grid = std::max(1, std::min(items, std::max(1, per_cu) * cus));
+  b->items = items;
+  b->lds = lds;
+  for (int k = 0; k < nops; k++) b->gens[k] = b->ops[k]->gen;
+  return EBH_OK;
+}
+
+template <typename R>
+int batch_apply(ebh_batch *b, const double *const *x, double *const *y, hipStream_t s)
+{
+  bool stale = b->grid == 0;
+  for (int k = 0; k < b->nops; k++) stale |= b->gens[k] != b->ops[k]->gen;
+  if (stale) EBH_TRY(batch_prepare<R>(b));
+  BatchPtrs p{};
+  for (int k = 0; k < b->nops; k++) {
+    p.x[k] = x[k];
+    p.y[k] = y[k];
+  }
+  const MemberDesc *md = reinterpret_cast<const MemberDesc *>(b->mdesc.p);
+  const dim3 fgrid((b->max_fold + 255) / 256, b->nops);
+  switch (b->kernel) {
+  case EBH_LAPLACE_DL:
+    hipLaunchKernelGGL((k_fold_multi<R, 1>), fgrid, dim3(256), 0, s, md,
+                       b->ops[0]->rule, p);
+    break;
+  case EBH_KELVIN_SL:
+    hipLaunchKernelGGL((k_fold_multi<R, 2>), fgrid, dim3(256), 0, s, md,
+                       b->ops[0]->rule, p);
+    break;
+  default:  // single layer and adjoint double layer: point strengths
+    hipLaunchKernelGGL((k_fold_multi<R, 0>), fgrid, dim3(256), 0, s, md,
+                       b->ops[0]->rule, p);
+    break;
+  }
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, 8, b->counters.p);
+  EBH_HIP(hipGetLastError());
+  const FarArgs<R> *descs = reinterpret_cast<const FarArgs<R> *>(b->desc.p);
+  const int *offp = b->off.p;
+  int nops = b->nops;
+  int *ctr = b->counters.p;
+  void *args[] = {(void *)&descs, (void *)&offp, (void *)&nops, (void *)&ctr};
+  EBH_HIP(hipLaunchKernel(b->kern, dim3(b->grid), dim3(256), args, b->lds, s));
+  hipLaunchKernelGGL(k_reduce_multi, dim3((b->max_red + 255) / 256, b->nops), dim3(256),
+                     0, s, md);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_multi, dim3((b->max_rows + 255) / 256, b->nops),
+                     dim3(256), 0, s, md, p);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+}  // namespace
+}  // namespace ebh
+
+using namespace ebh;
+
+extern "C" {
+
+int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out)
+{
+  if (!ops || !out || nops < 1 || nops > kMaxBatch) {
+    set_error("a batch holds 1.." + std::to_string(kMaxBatch) + " operators");
+    return EBH_ERR_ARG;
+  }
+  *out = nullptr;
+  for (int k = 0; k < nops; k++) {
+    if (!ops[k]) { set_error("null operator in batch"); return EBH_ERR_ARG; }
+    for (int j = 0; j < k; j++)
+      if (ops[j] == ops[k]) {
+        set_error("an operator appears twice in the batch");
+        return EBH_ERR_ARG;
+      }
+    if (ops[k]->inner) {
+      set_error("batch the hypersingular operator's wrapped single layer instead");
+      return EBH_ERR_UNSUPPORTED;
+    }
+    if (ops[k]->d.kernel != ops[0]->d.kernel ||
+        ops[k]->d.far_precision != ops[0]->d.far_precision ||
+        ops[k]->rule.nt != ops[0]->rule.nt ||
+        ops[k]->mesh->device != ops[0]->mesh->device) {
+      set_error("batched operators need one kernel, far-field precision, rule "
+                "size and device");
+      return EBH_ERR_ARG;
+    }
+  }
+  ebh_batch *b = new ebh_batch();
+  b->ops.assign(ops, ops + nops);
+  b->nops = nops;
+  b->kernel = ops[0]->d.kernel;
+  b->prec = ops[0]->d.far_precision;
+  b->nt = ops[0]->rule.nt;
+  b->gens.assign(nops, -1);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            ops[0]->mesh->device) != hipSuccess)
+    cus = 256;
+  int rc = batch_shapes(b, 4 * cus);
+  if (rc == EBH_OK)
+    rc = b->prec == EBH_FP32 ? batch_prepare<float>(b) : batch_prepare<double>(b);
+  if (rc != EBH_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return EBH_OK;
+}
+
+int ebh_batch_apply(ebh_batch *b, const double *const *x, double *const *y,
+                    void *stream)
+{
+  if (!b || !x || !y) { set_error("bad batch apply arguments"); return EBH_ERR_ARG; }
+  for (int k = 0; k < b->nops; k++)
+    if (!x[k] || !y[k]) { set_error("null vector in batch apply"); return EBH_ERR_ARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : b->ops[0]->stream;
+  return b->prec == EBH_FP32 ? batch_apply<float>(b, x, y, s)
+                             : batch_apply<double>(b, x, y, s);
+}
+
+int ebh_batch_info(const ebh_batch *b, int64_t *info)
+{
+  if (!b || !info) { set_error("null argument"); return EBH_ERR_ARG; }
+  info[0] = b->nops;
+  info[1] = b->items;
+  info[2] = b->grid;
+  info[3] = (int64_t)b->lds;
+  return EBH_OK;
+}
+
+void ebh_batch_destroy(ebh_batch *b) { delete b; }
+
+}  // extern "C"
+

@@ -143,6 +143,30 @@ int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
  * since the previous call (events on the op's stream), then resets. */
 int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
                    int64_t *launches);
+/* ---- multi-mesh batches ------------------------------------------------ */
+/* No counterpart in the reference, which keeps one mesh at a time
+ * (initGPU, src/BEM3dLaplaceGPU.cu:830-921).  Operators of one kernel,
+ * far-field precision and rule size on different meshes, applied together:
+ * per-operator fold kernels, ONE persistent far-field launch whose
+ * workgroups take (operator, 256-triangle target block, source chunk)
+ * items from 8 XCD-affine atomic queues until all are empty, then the
+ * per-operator reductions (one launch each for all members).  Creating a
+ * batch (at most 64 distinct operators) sets every member's launch shape:
+ * 256-triangle target blocks, 64-triangle tiles and one source-chunk size
+ * for all members, sized to about 8 items per resident workgroup; each
+ * y_k then equals ebh_op_apply(ops[k], x_k) bit for bit.  The operators
+ * must not be applied elsewhere while a batch apply runs; a launch shape
+ * changed later is picked up at the next apply. */
+typedef struct ebh_batch ebh_batch;
+int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out);
+/* x[k], y[k]: device vectors of ops[k]; stream: hipStream_t or NULL (the
+ * first operator's stream). */
+int ebh_batch_apply(ebh_batch *b, const double *const *x, double *const *y,
+                    void *stream);
+/* [0] operators, [1] work items, [2] workgroups, [3] LDS bytes */
+int ebh_batch_info(const ebh_batch *b, int64_t *info);
+void ebh_batch_destroy(ebh_batch *b);
+
 /* ---- launch-shape autotuner ----------------------------------------- */
 /* No counterpart in the reference, whose initGPU takes nblocks/nthreads by
  * hand (include/BEM3dLaplaceGPU.h:8-10).  Times whole applies with HIP
