@@ -127,7 +127,48 @@ def secondary(L, meshes, solve_batched=False):
                                      "iters_max": max(v["iters"] for v in st),
                                      "relres_max": max(v["relres"] for v in st)}
     out["cfg4_ico6_p0_V_64rhs"] = r4
+    out["multi_mesh"] = multi_mesh(L, meshes)
     return out
+
+
+def multi_mesh(L, meshes, reps=20):
+    """Many meshes of uneven size in one persistent far-field launch
+    (ebh_batch) against their own applies issued back to back (each
+    operator on its own stream, tuned launch shapes)."""
+    rng = np.random.default_rng(7)
+    res = {}
+    for name, levels in (("32 x icosphere L3", [3] * 32),
+                         ("16 x icosphere L2-L4", [2, 2, 3, 3, 3, 4, 4, 4] * 2)):
+        for prec in ("fp32", "fp64"):
+            ops, xs, ys, work = [], [], [], 0
+            for lev in levels:
+                P, T, _ = meshes.perturbed_icosphere(lev, int(rng.integers(1 << 30)))
+                op = L.Operator(L.Mesh(P, T), "V", 0, 0, prec)
+                ops.append(op)
+                xs.append(L.DeviceArray.from_host(rng.uniform(-1, 1, T.shape[0])))
+                ys.append(L.DeviceArray(T.shape[0]))
+                work += op.work()["far_interactions"]
+
+            def timed(fn):
+                for _ in range(3):
+                    fn()
+                L.check(L.lib().ebh_synchronize())
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                L.check(L.lib().ebh_synchronize())
+                return (time.perf_counter() - t0) / reps * 1e3
+
+            t_seq = timed(lambda: [op.apply(x, y) for op, x, y in zip(ops, xs, ys)])
+            b = L.Batch(ops)
+            t_b = timed(lambda: b.apply(xs, ys))
+            res[f"{name} {prec}"] = {
+                "separate_ms": t_seq, "batched_ms": t_b,
+                "speedup": t_seq / t_b,
+                "batched_far_tflops": 11 * work / t_b / 1e9,
+                "work_items": b.info()["items"]}
+            b.close()
+    return res
 
 
 # single-stream service time (s) of one solve, by (problem, level): sets the
