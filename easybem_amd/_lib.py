@@ -377,6 +377,24 @@ class TuneResult(C.Structure):
                 ("mms_default", C.c_double), ("tried", C.c_int)]
 
 
+def evaluate(mesh: Mesh, points, t, u, tri_rule=4, precision="fp64"):
+    """Representation formula at off-surface points (evaluateLaplace3d):
+    single layer of the P0 data t minus double layer of the P1 data u."""
+    L = lib()
+    L.ebh_evaluate_laplace_host.argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                            C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.empty(pts.shape[0])
+    check(L.ebh_evaluate_laplace_host(mesh.h, int(tri_rule),
+                                      0 if precision == "fp64" else 1,
+                                      pts.shape[0], _ptr(pts), _ptr(t), _ptr(u),
+                                      _ptr(out)))
+    return out
+
+
 def tune_save(path):
     lib().ebh_tune_save.argtypes = [C.c_char_p]
     check(lib().ebh_tune_save(os.fsencode(path)))

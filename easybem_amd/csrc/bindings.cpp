@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "BEM3dLaplaceGPU.h"
+#include "easybem_hip.h"
 
 namespace py = pybind11;
 
@@ -46,6 +47,19 @@ py::array_t<double> apply(const darr &x, size_t n_in, size_t n_out,
   return out;
 }
 
+// A device mesh for the calls that take nodes and elements directly.
+struct TempMesh {
+  ebh_mesh *m = nullptr;
+  TempMesh(const darr &nodes, const iarr &elements)
+  {
+    check_mesh(nodes, elements);
+    if (ebh_mesh_create((int)nodes.shape(0), nodes.data(), (int)elements.shape(0),
+                        elements.data(), &m) != EBH_OK)
+      throw std::runtime_error(ebh_last_error());
+  }
+  ~TempMesh() { ebh_mesh_destroy(m); }
+};
+
 void v_apply(const double *x, double *y, int ansatz) { applyGPULaplace3d1Layer(x, y, ansatz); }
 void k_apply(const double *x, double *y, int) { applyGPULaplace3d2Layer(x, y); }
 void d_apply(const double *x, double *y, int) { applyGPULaplace3dHyperSing(x, y); }
@@ -79,6 +93,51 @@ PYBIND11_MODULE(easybem, m)
       o(i, a) += J / 6.0; o(i, b) += J / 6.0; o(i, c) += J / 6.0;
     }
     return M;
+  });
+
+  // (C1, C2, C3), each (nElements, nNodes), like getLaplace3dCurl
+  // (src/BEM3dLaplace.cpp:489-558; python/bindings.cpp:62-86).
+  m.def("get_laplace3d_curl", [](darr nodes, iarr elements) {
+    check_mesh(nodes, elements);
+    const ssize_t np = nodes.shape(0), ne = elements.shape(0);
+    std::vector<double> cu((size_t)9 * ne);
+    if (ebh_surface_curls((int)np, nodes.data(), (int)ne, elements.data(), cu.data()) !=
+        EBH_OK)
+      throw std::invalid_argument(ebh_last_error());
+    auto T = elements.unchecked<2>();
+    py::array_t<double> C[3] = {py::array_t<double>({ne, np}), py::array_t<double>({ne, np}),
+                                py::array_t<double>({ne, np})};
+    for (int c = 0; c < 3; c++) {
+      auto o = C[c].mutable_unchecked<2>();
+      for (ssize_t i = 0; i < ne; i++)
+        for (ssize_t j = 0; j < np; j++) o(i, j) = 0.0;
+      for (ssize_t i = 0; i < ne; i++)
+        for (int k = 0; k < 3; k++) o(i, T(i, k) - 1) = cu[(3 * (size_t)i + c) * 3 + k];
+    }
+    return py::make_tuple(C[0], C[1], C[2]);
+  });
+
+  // Representation formula at points, evaluateLaplace3d
+  // (src/BEM3dLaplace.cpp:784-843; python/bindings.cpp:118-149): returns a
+  // 1-tuple holding an (nPoints, 1) array.
+  m.def("eval_laplace3d", [](darr nodes, iarr elements, darr points, darr neumann,
+                             darr dirichlet, int order) {
+    TempMesh tm(nodes, elements);
+    if (points.ndim() != 2 || points.shape(1) != 3)
+      throw std::invalid_argument("points must have shape (nPoints, 3)");
+    if (neumann.size() != elements.shape(0) || dirichlet.size() != nodes.shape(0))
+      throw std::invalid_argument("neumann needs nElements, dirichlet nNodes entries");
+    const ssize_t npts = points.shape(0);
+    py::array_t<double> out({npts, (ssize_t)1});
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = ebh_evaluate_laplace_host(tm.m, order, EBH_FP64, (int)npts, points.data(),
+                                     neumann.data(), dirichlet.data(),
+                                     out.mutable_data());
+    }
+    if (rc != EBH_OK) throw std::runtime_error(ebh_last_error());
+    return py::make_tuple(out);
   });
 
   m.def("init_gpu", [](darr nodes, iarr elements, int order, int nblocks, int nthreads) {

@@ -415,6 +415,43 @@ void ebh_op_destroy(ebh_op *op)
 // P0 x P1: each triangle adds J/6 to its three nodes
 // (src/BEM3dLaplace.cpp:464-477).  P1 x P1: the element matrix
 // (J/24) [[2,1,1],[1,2,1],[1,1,2]].
+int ebh_surface_curls(int np, const double *nodes, int ne, const int *elems,
+                      double *curl)
+{
+  if (!nodes || !elems || !curl || np < 3 || ne < 1) {
+    set_error("bad curl arguments");
+    return EBH_ERR_ARG;
+  }
+  Topology T;
+  T.np = np;
+  T.ne = ne;
+  T.nodes.assign(nodes, nodes + 3 * (size_t)np);
+  T.tri.resize(3 * (size_t)ne);
+  for (size_t k = 0; k < 3 * (size_t)ne; k++) {
+    if (elems[k] < 1 || elems[k] > np) {
+      set_error("element index out of range");
+      return EBH_ERR_MESH;
+    }
+    T.tri[k] = elems[k] - 1;
+  }
+  for (int i = 0; i < ne; i++) {
+    const double *a = &nodes[3 * T.tri[3 * i]], *b = &nodes[3 * T.tri[3 * i + 1]],
+                 *c = &nodes[3 * T.tri[3 * i + 2]];
+    const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2],
+                 n2 = u[0] * v[1] - u[1] * v[0];
+    if (!(n0 * n0 + n1 * n1 + n2 * n2 > 0.0)) {
+      set_error("degenerate triangle " + std::to_string(i));
+      return EBH_ERR_MESH;
+    }
+  }
+  std::vector<double> cu;
+  surface_curls(T, cu);
+  std::copy(cu.begin(), cu.end(), curl);
+  return EBH_OK;
+}
+
 int ebh_mass_apply(const ebh_mesh *m, int test_basis, const double *u, double *y)
 {
   if (!m || !u || !y) { set_error("null argument"); return EBH_ERR_ARG; }

@@ -143,6 +143,31 @@ int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
  * since the previous call (events on the op's stream), then resets. */
 int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
                    int64_t *launches);
+/* Surface curls of the P1 hat functions, getLaplace3dCurl
+ * (src/BEM3dLaplace.cpp:489-558): curl[(3i + c)*3 + k] is component c of
+ * the curl of triangle i's local hat function k.  elems are 1-based.
+ * Host only (no device work); rejects out-of-range and degenerate
+ * triangles. */
+int ebh_surface_curls(int np, const double *nodes, int ne, const int *elems,
+                      double *curl);
+
+/* ---- off-surface evaluation ------------------------------------------- */
+/* The representation formula at points off the surface, evaluateLaplace3d
+ * (src/BEM3dLaplace.cpp:784-843):
+ *   u(x) = sum_j J_j/(4 pi) sum_k w_k [t_j/|x - y_k|
+ *                                      - u(y_k) (x - y_k).n_j/|x - y_k|^3]
+ * with t P0 (ne entries), u P1 (np entries) and the triangle rule
+ * tri_rule (the reference's `order`).  points: npts x 3, result: npts.
+ * Device pointers on `stream` (returns after the stream has finished); the
+ * _host variant copies in and out.  No singular quadrature: the points
+ * must lie off the surface, as in the reference. */
+int ebh_evaluate_laplace(ebh_mesh *mesh, int tri_rule, int far_precision,
+                         int npts, const double *points, const double *t,
+                         const double *u, double *result, void *stream);
+int ebh_evaluate_laplace_host(ebh_mesh *mesh, int tri_rule, int far_precision,
+                              int npts, const double *points, const double *t,
+                              const double *u, double *result);
+
 /* ---- multi-mesh batches ------------------------------------------------ */
 /* No counterpart in the reference, which keeps one mesh at a time
  * (initGPU, src/BEM3dLaplaceGPU.cu:830-921).  Operators of one kernel,

@@ -70,6 +70,8 @@ class Oracle:
         L.orc_ss_full.restype = C.c_double
         L.orc_ss_table.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         L.orc_surface_curls.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp]
+        L.orc_evaluate.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int, _dp,
+                                   _dp, _dp, C.c_int, _dp, C.c_int]
         self.L = L
 
     # quadrature -------------------------------------------------------
@@ -163,6 +165,19 @@ class Oracle:
                 Cs[c][rows, elems[:, k] - 1] = cu[:, c, k]
         return tuple(Cs)
 
+    def evaluate(self, nodes, elems, points, t, u, tri_idx=4, nthreads=0):
+        """evaluateLaplace3d (src/BEM3dLaplace.cpp:784-843)."""
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        rc = self.L.orc_evaluate(np_, nodes, ne, elems, pts.shape[0], pts,
+                                 np.ascontiguousarray(t, dtype=np.float64),
+                                 np.ascontiguousarray(u, dtype=np.float64),
+                                 tri_idx, out, nthreads)
+        if rc:
+            raise RuntimeError(rc)
+        return out
+
     def hypersingular_apply(self, nodes, elems, x, line_idx=4, tri_idx=None,
                             stab=0.0):
         """y = sum_c C_c^T V C_c x (+ stab m (m.x)): the dense D of
@@ -199,6 +214,9 @@ class Reference:
         L.ebref_laplace3d_mass.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp]
         L.ebref_laplace3d_curl.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp,
                                            _dp, _dp]
+        L.ebref_evaluate_laplace3d.argtypes = [C.c_int, _dp, C.c_int, _ip,
+                                               C.c_int, _dp, _dp, _dp, C.c_int,
+                                               _dp]
         L.ebref_elasticity3d.argtypes = [C.c_int, _dp, C.c_int, _ip,
                                          C.c_double, C.c_int, _dp, C.c_void_p]
         L.ebref_line_rule.argtypes = [C.c_int, _dp, _dp]
@@ -224,6 +242,16 @@ class Reference:
         M = np.zeros(ne * np_)
         self.L.ebref_laplace3d_mass(np_, nodes, ne, elems, M)
         return M.reshape(np_, ne).T.copy()
+
+    def evaluate(self, nodes, elems, points, t, u, order):
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        self.L.ebref_evaluate_laplace3d(
+            np_, nodes, ne, elems, pts.shape[0], pts,
+            np.ascontiguousarray(t, dtype=np.float64),
+            np.ascontiguousarray(u, dtype=np.float64), order, out)
+        return out
 
     def curl(self, nodes, elems):
         """(C1, C2, C3), each [triangle, node], like getLaplace3dCurl."""

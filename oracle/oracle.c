@@ -887,6 +887,53 @@ int orc_surface_curls(int np, const double *nodes, int ne, const int *elems,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Representation formula, src/BEM3dLaplace.cpp:784-843: per triangle j and */
+/* point, sum_k w_k [t_j/r - u_h(y_k) (x - y_k).n_j/r^3], times J_j/(4 pi).  */
+/* ------------------------------------------------------------------------ */
+int orc_evaluate(int np, const double *nodes, int ne, const int *elems,
+                 int npts, const double *points, const double *t,
+                 const double *u, int tri_idx, double *result, int nthreads)
+{
+  (void)np;
+  double qp[2 * 16], qw[16];
+  const int nq = orc_tri_rule(tri_idx, qp, qw);
+  if (nq <= 0) return -1;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < npts; i++) {
+    const double *x = points + 3 * (size_t)i;
+    double acc = 0.0;
+    for (int j = 0; j < ne; j++) {
+      const int a = elems[3 * j] - 1, b = elems[3 * j + 1] - 1, c = elems[3 * j + 2] - 1;
+      double A[3], U[3], V[3], n[3];
+      for (int d = 0; d < 3; d++) {
+        A[d] = nodes[3 * a + d];
+        U[d] = nodes[3 * b + d] - A[d];
+        V[d] = nodes[3 * c + d] - A[d];
+      }
+      n[0] = U[1] * V[2] - U[2] * V[1];
+      n[1] = U[2] * V[0] - U[0] * V[2];
+      n[2] = U[0] * V[1] - U[1] * V[0];
+      const double J = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int d = 0; d < 3; d++) n[d] /= J;
+      double I = 0.0;
+      for (int k = 0; k < nq; k++) {
+        const double s1 = qp[2 * k], s2 = qp[2 * k + 1];
+        double r[3];
+        for (int d = 0; d < 3; d++) r[d] = x[d] - A[d] - U[d] * s1 - V[d] * s2;
+        const double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double uh = u[a] + (u[b] - u[a]) * s1 + (u[c] - u[a]) * s2;
+        I += qw[k] * t[j] / rn;
+        I -= qw[k] * uh * (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]) / (rn * rn * rn);
+      }
+      acc += J * I / 4.0 / ORC_PI;
+    }
+    result[i] = acc;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Dense P0 assembly restating getLaplace3d / getElasticity3d                */
 /* ------------------------------------------------------------------------ */
 int orc_laplace_dense(int np, const double *nodes, int ne, const int *elems,

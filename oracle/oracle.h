@@ -59,6 +59,12 @@ int orc_apply(int op, int test_basis, int trial_basis, int np,
 int orc_surface_curls(int np, const double *nodes, int ne, const int *elems,
                       double *curl);
 
+/* Representation formula at off-surface points restating evaluateLaplace3d
+ * (src/BEM3dLaplace.cpp:784-843): t P0 (ne), u P1 (np), result npts. */
+int orc_evaluate(int np, const double *nodes, int ne, const int *elems,
+                 int npts, const double *points, const double *t,
+                 const double *u, int tri_idx, double *result, int nthreads);
+
 /* One pair's local block in the pair's local vertex orders, for tests:
  * out[9] for op 0/1 (k test, l trial, out[3k+l]); out[63] for op 2
  * (out[7*(3k+l) + c], c = I, I11, I22, I33, I12, I13, I23).

@@ -43,6 +43,19 @@ int ebref_laplace3d_curl(int np, const double *nodes, int ne, const int *elems,
   return 0;
 }
 
+// evaluateLaplace3d at npts points (result npts).
+int ebref_evaluate_laplace3d(int np, const double *nodes, int ne, const int *elems,
+                             int npts, const double *points, const double *t,
+                             const double *u, int order, double *result)
+{
+  (void)np;
+  double *r = nullptr;
+  evaluateLaplace3d(nodes, ne, elems, npts, points, t, u, r, order);
+  std::memcpy(result, r, sizeof(double) * (size_t)npts);
+  delete[] r;
+  return 0;
+}
+
 int ebref_laplace3d_mass(int np, const double *nodes, int ne,
                          const int *elems, double *M)
 {

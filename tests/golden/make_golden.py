@@ -63,6 +63,18 @@ def main():
             np.savez_compressed(os.path.join(HERE, f"laplace_{name}_o{order}.npz"),
                                 nodes=P, elements=T, V=V, K=K, M=M, D=D,
                                 C1=C1, C2=C2, C3=C3)
+    # evaluateLaplace3d at interior, exterior and near-surface points
+    P, T = meshes.icosphere(2)
+    rng = np.random.default_rng(11)
+    t = rng.uniform(-1, 1, T.shape[0])
+    u = rng.uniform(-1, 1, P.shape[0])
+    pts = np.array([[0.0, 0.0, 0.0], [0.1, 0.2, 0.3], [0.5, -0.4, 0.2],
+                    [0.0, 0.0, 0.9], [2.0, 1.0, 0.5], [-0.7, 0.7, 0.0]])
+    ev = {"nodes": P, "elements": T, "points": pts, "t": t, "u": u}
+    for order in (3, 4):
+        ev[f"result_o{order}"] = ref.evaluate(P, T, pts, t, u, order)
+    np.savez_compressed(os.path.join(HERE, "evaluate_ico2.npz"), **ev)
+
     P, T = meshes.icosphere(1)
     Vk = ref.elasticity3d_V(P, T, 0.3, 4)
     np.savez_compressed(os.path.join(HERE, "kelvin_ico1_o4.npz"), nodes=P,

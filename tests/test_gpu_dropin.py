@@ -61,3 +61,15 @@ def test_mass_and_version(eb):
     M = eb.get_laplace3d_mass(z["nodes"], z["elements"].astype(np.int32))
     assert np.abs(M - z["M"]).max() <= 1e-15 * np.abs(z["M"]).max()
     assert eb.version() == "1.0"
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_eval_laplace3d_matches_reference(eb, order):
+    """eval_laplace3d against evaluateLaplace3d's values (golden)."""
+    z = np.load(os.path.join(GOLD, "evaluate_ico2.npz"))
+    out = eb.eval_laplace3d(z["nodes"], z["elements"].astype(np.int32),
+                            z["points"], z["t"], z["u"], order)
+    assert isinstance(out, tuple) and len(out) == 1
+    assert out[0].shape == (z["points"].shape[0], 1)
+    want = z[f"result_o{order}"]
+    assert np.abs(out[0][:, 0] - want).max() <= 1e-12 * np.abs(want).max()

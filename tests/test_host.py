@@ -129,3 +129,17 @@ def test_packaged_tuning_table_is_well_formed(L, tmp_path):
     assert len(out.read_text().split()) >= 6
     L.tune_clear()
     L.tune_load(str(saved))
+
+
+def test_dropin_module_curls_match_reference():
+    """easybem.get_laplace3d_curl (host only) against getLaplace3dCurl."""
+    sys.path.insert(0, os.path.join(ROOT, "easybem_amd"))
+    import easybem
+    z = np.load(os.path.join(ROOT, "tests", "golden", "laplace_torus8x4_o4.npz"))
+    Cs = easybem.get_laplace3d_curl(z["nodes"], z["elements"].astype(np.int32))
+    assert len(Cs) == 3
+    for c in range(3):
+        assert Cs[c].shape == z["C1"].shape
+        assert np.array_equal(Cs[c], z[f"C{c + 1}"])
+    with pytest.raises(ValueError):
+        easybem.get_laplace3d_curl(z["nodes"], z["elements"].astype(np.int32) + 1000)

@@ -103,6 +103,37 @@ def test_hypersingular_matches_reference_dense_d(oracle, mesh, order):
     assert np.abs(D @ np.ones(P.shape[0])).max() <= 1e-13 * np.abs(D).max()
 
 
+@pytest.mark.parametrize("order", [3, 4])
+def test_evaluate_matches_reference(oracle, order):
+    """orc_evaluate against evaluateLaplace3d at interior, exterior and
+    near-surface points."""
+    z = gold("evaluate_ico2.npz")
+    y = oracle.evaluate(z["nodes"], z["elements"], z["points"], z["t"], z["u"],
+                        tri_idx=order)
+    assert relmax(y, z[f"result_o{order}"]) <= 1e-14
+
+
+def test_evaluate_reproduces_harmonic_function(oracle):
+    """Green's representation: with the exact Cauchy data of a function
+    harmonic inside the sphere, the formula returns the function at interior
+    points, up to the discretisation error."""
+    from easybem_amd import meshes
+    x0 = np.array([0.4, 1.5, -1.2])
+    errs = []
+    for lev in (2, 3):
+        P, T = meshes.icosphere(lev)
+        u = 1.0 / np.linalg.norm(P - x0, axis=1)
+        c = meshes.centroids(P, T)
+        n = np.cross(P[T[:, 1] - 1] - P[T[:, 0] - 1], P[T[:, 2] - 1] - P[T[:, 0] - 1])
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        d = c - x0
+        t = -(d * n).sum(1) / np.linalg.norm(d, axis=1) ** 3
+        pts = np.array([[0.0, 0.0, 0.0], [0.2, -0.1, 0.3]])
+        exact = 1.0 / np.linalg.norm(pts - x0, axis=1)
+        errs.append(np.abs(oracle.evaluate(P, T, pts, t, u) - exact).max() / exact.max())
+    assert errs[1] < 0.5 * errs[0] and errs[1] < 1e-2
+
+
 def test_kelvin_dense_matches_reference(oracle):
     g = gold("kelvin_ico1_o4.npz")
     V = oracle.kelvin_dense(g["nodes"], g["elements"], float(g["nu"]), 4)
