@@ -395,6 +395,18 @@ def evaluate(mesh: Mesh, points, t, u, tri_rule=4, precision="fp64"):
     return out
 
 
+def assemble(mesh: Mesh, order=4):
+    """Dense (V, K, D, M) like getLaplace3d, assembled on the device."""
+    L = lib()
+    L.ebh_assemble_laplace.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+    ne, np_ = mesh.elements.shape[0], mesh.nodes.shape[0]
+    V, K = np.empty((ne, ne)), np.empty((ne, np_))
+    D, M = np.empty((np_, np_)), np.empty((ne, np_))
+    check(L.ebh_assemble_laplace(mesh.h, int(order), _ptr(V), _ptr(K), _ptr(D),
+                                 _ptr(M)))
+    return V, K, D, M
+
+
 def tune_save(path):
     lib().ebh_tune_save.argtypes = [C.c_char_p]
     check(lib().ebh_tune_save(os.fsencode(path)))

@@ -117,6 +117,25 @@ PYBIND11_MODULE(easybem, m)
     return py::make_tuple(C[0], C[1], C[2]);
   });
 
+  // (V, K, D, M) like getLaplace3d (src/BEM3dLaplace.cpp:561-781;
+  // python/bindings.cpp:88-116), assembled on the device.  V is returned as
+  // [test, trial]; the reference module returns its transpose
+  // (python/bindings.cpp:28-38 reads the row-major V column-major).
+  m.def("get_laplace3d", [](darr nodes, iarr elements, int order) {
+    TempMesh tm(nodes, elements);
+    const ssize_t np = nodes.shape(0), ne = elements.shape(0);
+    py::array_t<double> V({ne, ne}), K({ne, np}), D({np, np}), M({ne, np});
+    double *pv = V.mutable_data(), *pk = K.mutable_data(), *pd = D.mutable_data(),
+           *pm = M.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = ebh_assemble_laplace(tm.m, order, pv, pk, pd, pm);
+    }
+    if (rc != EBH_OK) throw std::runtime_error(ebh_last_error());
+    return py::make_tuple(V, K, D, M);
+  });
+
   // Representation formula at points, evaluateLaplace3d
   // (src/BEM3dLaplace.cpp:784-843; python/bindings.cpp:118-149): returns a
   // 1-tuple holding an (nPoints, 1) array.

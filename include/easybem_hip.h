@@ -151,6 +151,16 @@ int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
 int ebh_surface_curls(int np, const double *nodes, int ne, const int *elems,
                       double *curl);
 
+/* Dense Galerkin matrices, getLaplace3d (src/BEM3dLaplace.cpp:561-781), on
+ * the device at rule index `order` (line and triangle rule): V (ne x ne,
+ * row-major [test][trial]), K (ne x np, P0 test x P1 trial), D = sum_c
+ * C_c^T V C_c (np x np), M (ne x np).  Host outputs, each may be NULL.
+ * Entries equal the matrix-free operators' (same far quadrature data, same
+ * near-field values).  Memory: 8 (ne^2 + 2 ne np + np^2) bytes on the
+ * device. */
+int ebh_assemble_laplace(ebh_mesh *mesh, int order, double *V, double *K,
+                         double *D, double *M);
+
 /* ---- off-surface evaluation ------------------------------------------- */
 /* The representation formula at points off the surface, evaluateLaplace3d
  * (src/BEM3dLaplace.cpp:784-843):
