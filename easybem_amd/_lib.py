@@ -14,7 +14,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libeasybem_hip.so")
 TUNE_FILE = os.path.join(PKG, "tuning_mi355x.txt")
 
-KERNELS = {"V": 0, "K": 1, "kelvin": 2, "D": 3, "Kt": 4}
+KERNELS = {"V": 0, "K": 1, "kelvin": 2, "D": 3, "Kt": 4, "kelvin_K": 5,
+           "kelvin_D": 6}
 EBH_OK = 0
 
 

@@ -136,6 +136,24 @@ PYBIND11_MODULE(easybem, m)
     return py::make_tuple(V, K, D, M);
   });
 
+  // (V, K, D, M) like getElasticity3d (src/BEM3dElasticity.cpp:604-952;
+  // python/bindings.cpp:151-182), E = 1, component-blocked.
+  m.def("get_elasticity3d", [](darr nodes, iarr elements, double nu, int order) {
+    TempMesh tm(nodes, elements);
+    const ssize_t np = nodes.shape(0), ne = elements.shape(0);
+    py::array_t<double> V({3 * ne, 3 * ne}), K({3 * ne, 3 * np}), D({3 * np, 3 * np}),
+        M({3 * ne, 3 * np});
+    double *pv = V.mutable_data(), *pk = K.mutable_data(), *pd = D.mutable_data(),
+           *pm = M.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = ebh_assemble_elasticity(tm.m, nu, order, pv, pk, pd, pm);
+    }
+    if (rc != EBH_OK) throw std::runtime_error(ebh_last_error());
+    return py::make_tuple(V, K, D, M);
+  });
+
   // Representation formula at points, evaluateLaplace3d
   // (src/BEM3dLaplace.cpp:784-843; python/bindings.cpp:118-149): returns a
   // 1-tuple holding an (nPoints, 1) array.

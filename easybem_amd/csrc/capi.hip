@@ -16,7 +16,9 @@ using namespace ebh;
 
 size_t ebh_op::bytes() const
 {
-  return (inner ? inner->bytes() : 0) + curl.bytes() + mvec.bytes() + hsw.bytes() +
+  return (inner ? inner->bytes() : 0) + (inner_dl ? inner_dl->bytes() : 0) +
+         (inner_kel ? inner_kel->bytes() : 0) + kt1.bytes() + kt2.bytes() +
+         curl.bytes() + mvec.bytes() + hsw.bytes() +
          hsz.bytes() + hsdot.bytes() +
          near_ptr.bytes() + near_col.bytes() + near_val.bytes() + tq32.bytes() +
          sg32.bytes() + tq64.bytes() + sg64.bytes() + tw.bytes() + swt.bytes() +
@@ -179,7 +181,9 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (!mesh || !desc || !out) { set_error("null argument"); return EBH_ERR_ARG; }
   *out = nullptr;
   const ebh_op_desc &d = *desc;
-  if (d.kernel < 0 || d.kernel > 4) { set_error("unknown kernel"); return EBH_ERR_ARG; }
+  if (d.kernel < 0 || d.kernel > 6) { set_error("unknown kernel"); return EBH_ERR_ARG; }
+  const bool elastic = d.kernel == EBH_KELVIN_SL || d.kernel == EBH_KELVIN_DL ||
+                       d.kernel == EBH_KELVIN_HS;
   if ((d.test_basis != 0 && d.test_basis != 1) ||
       (d.trial_basis != 0 && d.trial_basis != 1)) {
     set_error("basis must be EBH_P0 or EBH_P1");
@@ -212,7 +216,15 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     set_error("far_precision must be EBH_FP64 or EBH_FP32");
     return EBH_ERR_ARG;
   }
-  if (d.kernel == EBH_KELVIN_SL && !(d.nu > -1.0 && d.nu < 0.5)) {
+  if (d.kernel == EBH_KELVIN_DL && (d.test_basis != EBH_P0 || d.trial_basis != EBH_P1)) {
+    set_error("the elasticity double layer has a P0 test and a P1 trial space");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_KELVIN_HS && (d.test_basis != EBH_P1 || d.trial_basis != EBH_P1)) {
+    set_error("the elasticity hypersingular operator needs P1 test and trial spaces");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (elastic && !(d.nu > -1.0 && d.nu < 0.5)) {
     set_error("Poisson ratio must lie in (-1, 0.5)");
     return EBH_ERR_ARG;
   }
@@ -240,7 +252,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     set_error("far-field rules of up to 7 points must be median-symmetric");
     return EBH_ERR_UNSUPPORTED;
   }
-  op->nc = d.kernel == EBH_KELVIN_SL ? 3 : 1;
+  op->nc = elastic ? 3 : 1;
   op->nrows = d.test_basis ? mesh->T.np : mesh->T.ne;
   op->ncols = d.trial_basis ? mesh->T.np : mesh->T.ne;
   int rc = EBH_OK;
@@ -258,7 +270,31 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
       set_error("hipEventCreate failed");
       return EBH_ERR_HIP;
     }
-  if (d.kernel == EBH_LAPLACE_HS) {
+  if (d.kernel == EBH_KELVIN_DL || d.kernel == EBH_KELVIN_HS) {
+    // wrapped operators: Laplace V on 3 (K) or 9 (D) P0 densities per RHS,
+    // Laplace K (P0 x P1) on the 3 components, Kelvin V on w
+    const int per = d.kernel == EBH_KELVIN_DL ? 3 : 9;
+    ebh_op_desc vd = d;
+    vd.kernel = EBH_LAPLACE_SL;
+    vd.test_basis = vd.trial_basis = EBH_P0;
+    vd.nu = 0.0;
+    vd.max_rhs = op->d.max_rhs > 1 ? per * op->d.max_rhs : 1;
+    ebh_op_desc kd = vd;
+    kd.kernel = EBH_KELVIN_SL;
+    kd.nu = d.nu;
+    kd.max_rhs = op->d.max_rhs;
+    ebh_op_desc dd = vd;
+    dd.kernel = EBH_LAPLACE_DL;
+    dd.trial_basis = EBH_P1;
+    dd.max_rhs = 1;
+    if ((rc = ebh_op_create(mesh, &vd, &op->inner)) ||
+        (rc = ebh_op_create(mesh, &kd, &op->inner_kel)) ||
+        (d.kernel == EBH_KELVIN_DL && (rc = ebh_op_create(mesh, &dd, &op->inner_dl))) ||
+        (rc = kel_build(op))) {
+      ebh_op_destroy(op);
+      return rc;
+    }
+  } else if (d.kernel == EBH_LAPLACE_HS) {
     // the wrapped P0 single layer takes three curl components per RHS
     ebh_op_desc vd = d;
     vd.kernel = EBH_LAPLACE_SL;
@@ -357,8 +393,13 @@ int ebh_op_work(const ebh_op *op, int64_t *work)
 {
   if (!op || !work) { set_error("null argument"); return EBH_ERR_ARG; }
   const int64_t nt = op->rule.nt;
-  // hypersingular: three single-layer applies per RHS
-  work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt * (op->inner ? 3 : 1);
+  // composite operators: far interactions summed over the wrapped applies
+  // (single-layer equivalents: hypersingular 3; elasticity K 3 V + 3 K +
+  // Kelvin; elasticity D 9 V + Kelvin)
+  const int64_t mult = op->d.kernel == EBH_LAPLACE_HS ? 3
+                       : op->d.kernel == EBH_KELVIN_DL ? 7
+                       : op->d.kernel == EBH_KELVIN_HS ? 10 : 1;
+  work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt * mult;
   work[1] = far_op(op)->near_nnz;
   work[2] = op->nrows;
   work[3] = op->ncols;
@@ -403,7 +444,9 @@ void ebh_op_destroy(ebh_op *op)
   if (!op) return;
   if (op->stream) (void)hipStreamSynchronize(op->stream);
   ebh_op_destroy(op->inner);
-  op->inner = nullptr;
+  ebh_op_destroy(op->inner_dl);
+  ebh_op_destroy(op->inner_kel);
+  op->inner = op->inner_dl = op->inner_kel = nullptr;
   for (auto &e : op->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : op->prof_ev) (void)hipEventDestroy(e);

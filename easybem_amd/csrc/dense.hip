@@ -142,6 +142,42 @@ __global__ void k_dense_zero(size_t n, double *p)
   if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_set_entry(double *x, size_t i, double v) { x[i] = v; }
+
+// column c of a row-major rows x cols matrix
+__global__ void k_put_column(int rows, int cols, int c, const double *__restrict__ y,
+                             double *__restrict__ A)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) A[(size_t)r * cols + c] = y[r];
+}
+
+// Dense matrix of an operator by applying it to the unit vectors (one
+// apply per column, all on the op's stream), copied to host row-major.
+int dense_by_columns(ebh_op *op, double *A)
+{
+  int rows = 0, cols = 0;
+  EBH_TRY(ebh_op_dims(op, &rows, &cols));
+  DevBuf<double> x, y, dA;
+  EBH_TRY(x.alloc(cols));
+  EBH_TRY(y.alloc(rows));
+  EBH_TRY(dA.alloc((size_t)rows * cols));
+  const hipStream_t s = op->stream;
+  hipLaunchKernelGGL(k_dense_zero, dim3((cols + 255) / 256), dim3(256), 0, s,
+                     (size_t)cols, x.p);
+  for (int c = 0; c < cols; c++) {
+    hipLaunchKernelGGL(k_set_entry, dim3(1), dim3(1), 0, s, x.p, (size_t)c, 1.0);
+    EBH_TRY(op_apply_one(op, x.p, y.p, s, false));
+    hipLaunchKernelGGL(k_put_column, dim3((rows + 255) / 256), dim3(256), 0, s, rows, cols,
+                       c, y.p, dA.p);
+    hipLaunchKernelGGL(k_set_entry, dim3(1), dim3(1), 0, s, x.p, (size_t)c, 0.0);
+  }
+  EBH_HIP(hipGetLastError());
+  EBH_HIP(hipStreamSynchronize(s));
+  EBH_HIP(hipMemcpy(A, dA.p, dA.bytes(), hipMemcpyDeviceToHost));
+  return EBH_OK;
+}
+
 DenseArgs dense_args(const ebh_op *v)
 {
   DenseArgs a;
@@ -257,6 +293,55 @@ int ebh_assemble_laplace(ebh_mesh *mesh, int order, double *V, double *K, double
   ebh_op_destroy(kop);
   ebh_op_destroy(vop);
   return rc;
+}
+
+// getElasticity3d (src/BEM3dElasticity.cpp:604-952): V (3ne x 3ne), K
+// (3ne x 3np), D (3np x 3np), M (3ne x 3np), component-blocked rows and
+// columns, row-major; each output may be NULL.  V, K and D are the
+// matrix-free operators' columns (one apply per column).
+int ebh_assemble_elasticity(ebh_mesh *mesh, double nu, int order, double *V, double *K,
+                            double *D, double *M)
+{
+  if (!mesh) { set_error("null mesh"); return EBH_ERR_ARG; }
+  const Topology &T = mesh->T;
+  const int ne = T.ne, np = T.np;
+  EBH_HIP(hipSetDevice(mesh->device));
+  if (M) {
+    std::fill(M, M + (size_t)9 * ne * np, 0.0);
+    for (int i = 0; i < ne; i++) {
+      const int *f = &T.tri[3 * i];
+      const double *a = &T.nodes[3 * f[0]], *b = &T.nodes[3 * f[1]], *c = &T.nodes[3 * f[2]];
+      const double p[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double q[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      const double n0 = p[1] * q[2] - p[2] * q[1], n1 = p[2] * q[0] - p[0] * q[2],
+                   n2 = p[0] * q[1] - p[1] * q[0];
+      const double J = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      for (int a3 = 0; a3 < 3; a3++)
+        for (int k = 0; k < 3; k++)
+          M[(size_t)(a3 * ne + i) * 3 * np + a3 * np + f[k]] += J / 6.0;
+    }
+  }
+  ebh_op_desc d{};
+  d.far_precision = EBH_FP64;
+  d.line_rule = d.tri_rule = order;
+  d.max_rhs = 1;
+  d.nu = nu;
+  const int kinds[3] = {EBH_KELVIN_SL, EBH_KELVIN_DL, EBH_KELVIN_HS};
+  const int tests[3] = {EBH_P0, EBH_P0, EBH_P1};
+  const int trials[3] = {EBH_P0, EBH_P1, EBH_P1};
+  double *outs[3] = {V, K, D};
+  for (int w = 0; w < 3; w++) {
+    if (!outs[w]) continue;
+    d.kernel = kinds[w];
+    d.test_basis = tests[w];
+    d.trial_basis = trials[w];
+    ebh_op *op = nullptr;
+    EBH_TRY(ebh_op_create(mesh, &d, &op));
+    const int rc = dense_by_columns(op, outs[w]);
+    ebh_op_destroy(op);
+    EBH_TRY(rc);
+  }
+  return EBH_OK;
 }
 
 }  // extern "C"

@@ -131,6 +131,11 @@ struct ebh_op {
   // [nrhs][3][ne] and the partial m.x sums
   ebh_op *inner = nullptr;
   ebh::DevBuf<double> curl, mvec, hsw, hsz, hsdot;
+  // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single
+  // layer (inner), Laplace double layer and Kelvin single layer they wrap,
+  // and two work buffers
+  ebh_op *inner_dl = nullptr, *inner_kel = nullptr;
+  ebh::DevBuf<double> kt1, kt2;
   // launch configuration
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   int gen = 0;  // bumped when the launch shape changes (captured graphs key on it)
@@ -154,6 +159,10 @@ int op_configure(ebh_op *op, int block, int tile, int chunks);
 int hs_build(ebh_op *op);
 int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
              bool timed);
+int kel_build(ebh_op *op);
+// applies an operator built from other operators (op->inner set)
+int composite_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
+                    bool timed);
 // the operator whose far-field kernels run (the wrapped single layer for
 // the hypersingular operator)
 inline ebh_op *far_op(ebh_op *op) { return op->inner ? op->inner : op; }

@@ -1043,7 +1043,7 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
-  if (op->inner) return hs_apply(op, x, y, 1, s, timed);
+  if (op->inner) return composite_apply(op, x, y, 1, s, timed);
   const int ne = op->mesh->T.ne;
   const bool f32 = op->d.far_precision == EBH_FP32;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
@@ -1062,7 +1062,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
 int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed)
 {
-  if (op->inner) return hs_apply(op, x, y, nrhs, s, timed);
+  if (op->inner) return composite_apply(op, x, y, nrhs, s, timed);
   if (nrhs == 1 || !op->rb) {
     const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
     for (int r = 0; r < nrhs; r++)
@@ -1360,7 +1360,7 @@ int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out)
         return EBH_ERR_ARG;
       }
     if (ops[k]->inner) {
-      set_error("batch the hypersingular operator's wrapped single layer instead");
+      set_error("batch the operators a composite operator wraps instead");
       return EBH_ERR_UNSUPPORTED;
     }
     if (ops[k]->d.kernel != ops[0]->d.kernel ||

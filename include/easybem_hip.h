@@ -56,8 +56,14 @@ enum {
  * (getLaplace3d's K, src/BEM3dLaplace.cpp:386-437,647); it gives the
  * Neumann right-hand side (M^T/2 - K^T) t.  P1 test, P0 or P1 trial,
  * FP64 far field only. */
+/* EBH_KELVIN_DL, EBH_KELVIN_HS: the elasticity double layer (P0 test x P1
+ * trial, 3 components each) and hypersingular operator (P1 x P1) of
+ * getElasticity3d (src/BEM3dElasticity.cpp:760-943), applied through the
+ * Laplace single and double layer and the Kelvin single layer (E = 1,
+ * Poisson ratio nu). */
 enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2,
-       EBH_LAPLACE_HS = 3, EBH_LAPLACE_ADL = 4 };
+       EBH_LAPLACE_HS = 3, EBH_LAPLACE_ADL = 4, EBH_KELVIN_DL = 5,
+       EBH_KELVIN_HS = 6 };
 enum { EBH_P0 = 0, EBH_P1 = 1 };
 enum { EBH_FP64 = 0, EBH_FP32 = 1 };
 
@@ -71,7 +77,7 @@ typedef struct ebh_server ebh_server;
  * (src/quadrature.cpp:286-287,1047-1048); the reference's single `order`
  * argument corresponds to line_rule == tri_rule == order. */
 typedef struct {
-  int kernel;        /* EBH_LAPLACE_SL/_DL/_HS/_ADL, EBH_KELVIN_SL        */
+  int kernel;        /* EBH_LAPLACE_SL/_DL/_HS/_ADL, EBH_KELVIN_SL/_DL/_HS */
   int test_basis;    /* EBH_P0 (one DOF per triangle) or EBH_P1 (nodes)   */
   int trial_basis;   /* DL requires EBH_P1; Kelvin requires test==trial   */
   int far_precision; /* EBH_FP64 or EBH_FP32 (near field is always FP64) */
@@ -160,6 +166,14 @@ int ebh_surface_curls(int np, const double *nodes, int ne, const int *elems,
  * device. */
 int ebh_assemble_laplace(ebh_mesh *mesh, int order, double *V, double *K,
                          double *D, double *M);
+
+/* Dense elasticity matrices, getElasticity3d (src/BEM3dElasticity.cpp:
+ * 604-952), E = 1: V (3ne x 3ne), K (3ne x 3np), D (3np x 3np), M
+ * (3ne x 3np), component-blocked (all x rows/columns, then y, then z),
+ * row-major host outputs, each may be NULL.  V, K and D are built column
+ * by column from the matrix-free operators (EBH_KELVIN_SL/_DL/_HS). */
+int ebh_assemble_elasticity(ebh_mesh *mesh, double nu, int order, double *V,
+                            double *K, double *D, double *M);
 
 /* ---- off-surface evaluation ------------------------------------------- */
 /* The representation formula at points off the surface, evaluateLaplace3d

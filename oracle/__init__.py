@@ -218,7 +218,8 @@ class Reference:
                                                C.c_int, _dp, _dp, _dp, C.c_int,
                                                _dp]
         L.ebref_elasticity3d.argtypes = [C.c_int, _dp, C.c_int, _ip,
-                                         C.c_double, C.c_int, _dp, C.c_void_p]
+                                         C.c_double, C.c_int, _dp, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
         L.ebref_line_rule.argtypes = [C.c_int, _dp, _dp]
         L.ebref_tri_rule.argtypes = [C.c_int, _dp, _dp]
         self.L = L
@@ -263,8 +264,21 @@ class Reference:
     def elasticity3d_V(self, nodes, elems, nu, order):
         nodes, elems, np_, ne = _mesh(nodes, elems)
         V = np.zeros(9 * ne * ne)
-        self.L.ebref_elasticity3d(np_, nodes, ne, elems, nu, order, V, None)
+        self.L.ebref_elasticity3d(np_, nodes, ne, elems, nu, order, V, None,
+                                  None, None)
         return V.reshape(3 * ne, 3 * ne).T.copy()  # column-major -> [row, col]
+
+    def elasticity3d(self, nodes, elems, nu, order):
+        """(V, K, D, M) like getElasticity3d, each [row, col]."""
+        nodes, elems, np_, ne = _mesh(nodes, elems)
+        V = np.zeros(9 * ne * ne); K = np.zeros(9 * ne * np_)
+        D = np.zeros(9 * np_ * np_); M = np.zeros(9 * ne * np_)
+        self.L.ebref_elasticity3d(np_, nodes, ne, elems, nu, order, V,
+                                  K.ctypes.data, D.ctypes.data, M.ctypes.data)
+        return (V.reshape(3 * ne, 3 * ne).T.copy(),
+                K.reshape(3 * np_, 3 * ne).T.copy(),
+                D.reshape(3 * np_, 3 * np_).T.copy(),
+                M.reshape(3 * np_, 3 * ne).T.copy())
 
     def line_rule(self, idx):
         p = np.zeros(128); w = np.zeros(128)

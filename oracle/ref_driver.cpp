@@ -66,14 +66,18 @@ int ebref_laplace3d_mass(int np, const double *nodes, int ne,
   return 0;
 }
 
-// V (3ne x 3ne) and K (3ne x 3np), both column-major as the reference writes.
+// V (3ne x 3ne), K (3ne x 3np), D (3np x 3np), M (3ne x 3np), all
+// column-major as the reference writes them; K, D, M may be NULL.
 int ebref_elasticity3d(int np, const double *nodes, int ne, const int *elems,
-                       double nu, int order, double *V, double *K)
+                       double nu, int order, double *V, double *K, double *D,
+                       double *M)
 {
   double *v = nullptr, *k = nullptr, *d = nullptr, *m = nullptr;
   getElasticity3d(np, nodes, ne, elems, nu, v, k, d, m, order);
   std::memcpy(V, v, sizeof(double) * (size_t)9 * ne * ne);
   if (K) std::memcpy(K, k, sizeof(double) * (size_t)9 * ne * np);
+  if (D) std::memcpy(D, d, sizeof(double) * (size_t)9 * np * np);
+  if (M) std::memcpy(M, m, sizeof(double) * (size_t)9 * ne * np);
   delete[] v; delete[] k; delete[] d; delete[] m;
   return 0;
 }

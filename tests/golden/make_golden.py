@@ -76,9 +76,13 @@ def main():
     np.savez_compressed(os.path.join(HERE, "evaluate_ico2.npz"), **ev)
 
     P, T = meshes.icosphere(1)
-    Vk = ref.elasticity3d_V(P, T, 0.3, 4)
+    Vk, Kk, Dk, Mk = ref.elasticity3d(P, T, 0.3, 4)
     np.savez_compressed(os.path.join(HERE, "kelvin_ico1_o4.npz"), nodes=P,
-                        elements=T, nu=0.3, V=Vk)
+                        elements=T, nu=0.3, V=Vk, K=Kk, D=Dk, M=Mk)
+    P, T = meshes.torus(1.0, 0.4, 8, 4)
+    Vk, Kk, Dk, Mk = ref.elasticity3d(P, T, 0.25, 3)
+    np.savez_compressed(os.path.join(HERE, "kelvin_torus8x4_o3.npz"), nodes=P,
+                        elements=T, nu=0.25, V=Vk, K=Kk, D=Dk, M=Mk)
 
     P, T = meshes.icosphere(3)
     t0 = time.perf_counter()

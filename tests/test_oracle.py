@@ -134,6 +134,44 @@ def test_evaluate_reproduces_harmonic_function(oracle):
     assert errs[1] < 0.5 * errs[0] and errs[1] < 1e-2
 
 
+@pytest.mark.parametrize("name", ["kelvin_ico1_o4", "kelvin_torus8x4_o3"])
+def test_elasticity_k_d_decompose_into_laplace_and_kelvin(oracle, name):
+    """getElasticity3d's K and D (src/BEM3dElasticity.cpp:760-943) equal
+    K = I(x)K_L + (V_E/(1+nu) - I(x)V) W and
+    D = mu [4 sum_m C_m^T V C_m (x) I - 2 C_b^T V C_a - C_a^T V C_b]
+        - 4 mu^2 W^T V_E W,
+    W the curl-cross map (w = sum_k c_k x u_k): the identities the device
+    elasticity operators apply.  V, K_L and the curls are the oracle's."""
+    z = gold(f"{name}.npz")
+    P, T, nu = z["nodes"], z["elements"], float(z["nu"])
+    order = int(name[-1])
+    ne, np_ = T.shape[0], P.shape[0]
+    V, K, _ = oracle.laplace_dense(P, T, order)
+    Cs = oracle.curl(P, T)
+    Ve = oracle.kelvin_dense(P, T, nu, order)
+    eps = np.zeros((3, 3, 3))
+    for m, n, b in ((0, 1, 2), (1, 2, 0), (2, 0, 1)):
+        eps[m, n, b], eps[m, b, n] = 1, -1
+    W = np.zeros((3 * ne, 3 * np_))
+    for m in range(3):
+        for n in range(3):
+            for b in range(3):
+                W[m * ne:(m + 1) * ne, b * np_:(b + 1) * np_] += eps[m, n, b] * Cs[n]
+    I3 = np.eye(3)
+    Kel = np.kron(I3, K) + (Ve / (1 + nu) - np.kron(I3, V)) @ W
+    assert relmax(Kel, z["K"]) <= 1e-13
+    mu = 0.5 / (1 + nu)
+    D = np.zeros((3 * np_, 3 * np_))
+    for a in range(3):
+        for b in range(3):
+            blk = -2 * Cs[b].T @ V @ Cs[a] - Cs[a].T @ V @ Cs[b]
+            if a == b:
+                blk += sum(4 * Cm.T @ V @ Cm for Cm in Cs)
+            D[a * np_:(a + 1) * np_, b * np_:(b + 1) * np_] = mu * blk
+    D -= 4 * mu * mu * W.T @ Ve @ W
+    assert relmax(D, z["D"]) <= 1e-13
+
+
 def test_kelvin_dense_matches_reference(oracle):
     g = gold("kelvin_ico1_o4.npz")
     V = oracle.kelvin_dense(g["nodes"], g["elements"], float(g["nu"]), 4)
