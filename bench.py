@@ -6,7 +6,10 @@ Dirichlet data u = 1/|x - (0,0,2)|, right-hand side b = (1/2 M + K) u,
 solved for the P1 Neumann trace with GMRES(50) by mixed-precision iterative
 refinement (FP32 far field inside, FP64 outer residual) to a relative FP64
 residual of 1e-8.  Near field: 5-point Gauss Sauter-Schwab; far field:
-7-point triangle rule.
+7-point triangle rule.  GMRES is right-preconditioned by the Calderon
+preconditioner (the hypersingular operator between inverse lumped masses,
+with a 1-point far rule); --precond none runs it without.  The line also
+reports the unpreconditioned solve rate.
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
 build runs on one GPU.
@@ -41,7 +44,8 @@ def log(*a):
 
 def cpu_baseline(P, T, n_matvec, seconds=10.0):
     """The oracle (FP64 C restatement, OpenMP) on sampled P1 rows of the same
-    operator, extrapolated to full matvecs x the FP64 solve's matvec count."""
+    operator, extrapolated to full matvecs x the FP64 solve's matvec count
+    (preconditioner applies counted by their far-field work)."""
     from oracle import Oracle
     o = Oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
@@ -62,7 +66,9 @@ def cpu_baseline(P, T, n_matvec, seconds=10.0):
             "cores": threads, "kind": "port",
             "sample": f"oracle P1 V apply on {nrows} random node rows "
                       f"({elapsed:.1f} s), extrapolated to a full matvec "
-                      f"({t_matvec:.1f} s) x {n_matvec} FP64 matvecs per solve"}
+                      f"({t_matvec:.1f} s) x {n_matvec:.1f} FP64 matvec-equivalents "
+                      "per solve (the FP64 GPU solve's operator and preconditioner "
+                      "applies)"}
 
 
 def secondary(L, meshes, solve_batched=False):
@@ -280,6 +286,10 @@ def main():
                     help="serving stream of N requests on icosphere levels 4-6 "
                          "(the default run serves 12 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--precond", choices=("calderon", "none"), default="calderon",
+                    help="right preconditioner of the GMRES solves (default: the "
+                         "Calderon preconditioner, hypersingular operator with a "
+                         "1-point far rule)")
     ap.add_argument("--retune", action="store_true",
                     help="run the launch-shape autotuner on the bench operators "
                          "instead of using the packaged table")
@@ -325,11 +335,23 @@ def main():
     dx = L.DeviceArray(b.size)
     zeros = np.zeros(b.size)
     solver = L.Solver(V64, V32, restart=50)
+    pc = None
+    if args.precond == "calderon":
+        # M_L^-1 (D + m m^T / (4|Gamma|)) M_L^-1: D through curls, 3-point
+        # Gauss near field, 1-point far rule; FP64 for the FP64 operator
+        P64 = L.precond_calderon(mesh, "fp64", line_rule=2, tri_rule=0)
+        P32 = L.precond_calderon(mesh, "fp32", line_rule=2, tri_rule=0)
+        solver.set_precond(P64, P32)
+        pc = ("Calderon, right: M_L^-1 (D + m m^T/(4|Gamma|)) M_L^-1, D the "
+              "hypersingular operator through curls with a 1-point far rule "
+              "and 3-point Gauss near rule (FP32 with the FP32 operator, FP64 "
+              "with the FP64 one); the residual is that of the unpreconditioned "
+              "system")
 
-    def solve(mixed=True):
+    def solve(mixed=True, s=None):
         L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))
-        return solver.solve_device(db, dx, tol=1e-8, mixed=mixed,
-                                   max_iters=3000, inner_tol=args.inner_tol)
+        return (s or solver).solve_device(db, dx, tol=1e-8, mixed=mixed,
+                                          max_iters=3000, inner_tol=args.inner_tol)
 
     for _ in range(args.warmup):
         solve()
@@ -361,6 +383,21 @@ def main():
     far_ms, far_launches = L.op_profile(V32, False)
     far_flop = FLOP_SL * w["far_interactions"]
 
+    unpc = {}
+    if pc is not None and not args.no_fp64_compare:
+        # the same solve without the preconditioner, for comparison
+        plain = L.Solver(V64, V32, restart=50)
+        solve(s=plain)
+        ts = time.perf_counter()
+        for _ in range(3):
+            stp = solve(s=plain)
+        tp = (time.perf_counter() - ts) / 3
+        unpc = {"unpreconditioned": {"solves_per_s": 1.0 / tp, "ms": tp * 1e3,
+                                     "iters": stp["iters"],
+                                     "matvec32": stp["matvec32"],
+                                     "matvec64": stp["matvec64"]}}
+        plain.close()
+
     out_fp64 = {}
     if not args.no_fp64_compare:
         solve(mixed=False)
@@ -372,6 +409,9 @@ def main():
         out_fp64 = {"fp64_solves_per_s": 1.0 / t64s,
                     "fp64_iters": st64["iters"],
                     "mixed_speedup_vs_fp64": solves_per_s * t64s}
+        # CPU-equivalent operator applies of the FP64 solve: a preconditioner
+        # apply is 3 single-layer applies with 1 of the 49 far interactions
+        st64["matvec64"] += st64.get("precond", 0) * 3.0 / 49.0
     else:
         st64 = {"matvec64": stats[-1]["iters"]}
 
@@ -395,6 +435,7 @@ def main():
         "config": {"workload": f"icosphere L{args.level} P1 Laplace Dirichlet "
                                "solve, u=1/|x-(0,0,2)|, b=(M/2+K)u, "
                                "GMRES(50) mixed IR to 1e-8",
+                   "preconditioner": pc,
                    "triangles": int(T.shape[0]), "vertices": int(P.shape[0]),
                    "near_rule": "5-point Gauss (line index 4)",
                    "far_rule": "7-point triangle (index 4)"},
@@ -403,7 +444,9 @@ def main():
         "iters": stats[-1]["iters"], "outer": stats[-1]["outer"],
         "matvec32_per_solve": stats[-1]["matvec32"],
         "matvec64_per_solve": stats[-1]["matvec64"],
+        "precond_applies_per_solve": stats[-1]["precond"],
         **out_fp64,
+        **unpc,
         "matvec_ms": {"fp32": t32["total"], "fp64": t64["total"]},
         "autotune": tune,
         "matvec_far_tflops": {"fp32": far_flop / (t32["far"] * 1e-3) / 1e12,

@@ -251,7 +251,8 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("matvec64", C.c_int),
                 ("matvec32", C.c_int), ("relres", C.c_double),
-                ("ms", C.c_double), ("converged", C.c_int)]
+                ("ms", C.c_double), ("converged", C.c_int),
+                ("precond", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -296,6 +297,14 @@ class Solver:
     def set(self, graphs=True, check_every=0):
         check(lib().ebh_solver_set(self.h, int(graphs), check_every))
 
+    def set_precond(self, p64=None, p32=None):
+        """Right preconditioners for the FP64 / FP32 operator (None: off)."""
+        L = lib()
+        L.ebh_solver_set_precond.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        check(L.ebh_solver_set_precond(self.h, p64.h if p64 else None,
+                                       p32.h if p32 else None))
+        self.p64, self.p32 = p64, p32
+
     def solve_device(self, b: DeviceArray, x: DeviceArray, tol=1e-8,
                      mixed=False, max_iters=2000, inner_tol=1e-5,
                      max_outer=10, raise_on_noconv=True, nb=1):
@@ -327,6 +336,27 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def precond_calderon(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0,
+                     max_rhs=1):
+    """Calderon preconditioner M_L^-1 (D + m m^T/(4|Gamma|)) M_L^-1 for the
+    P1 single layer (ebh_precond_calderon), as an Operator."""
+    L = lib()
+    L.ebh_precond_calderon.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    check(L.ebh_precond_calderon(mesh.h, 0 if precision == "fp64" else 1,
+                                 int(line_rule), int(tri_rule), int(max_rhs),
+                                 C.byref(h)))
+    op = Operator.__new__(Operator)
+    op.mesh = mesh
+    op.h = h
+    op.desc = None
+    r, c = C.c_int(), C.c_int()
+    check(L.ebh_op_dims(h, C.byref(r), C.byref(c)))
+    op.nrows, op.ncols = r.value, c.value
+    return op
 
 
 class Batch:

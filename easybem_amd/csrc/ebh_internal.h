@@ -131,6 +131,10 @@ struct ebh_op {
   // [nrhs][3][ne] and the partial m.x sums
   ebh_op *inner = nullptr;
   ebh::DevBuf<double> curl, mvec, hsw, hsz, hsdot;
+  // Calderon preconditioner form of the hypersingular operator:
+  // diag(dinv) (D + stab m m^T) diag(dinv), dinv = 1 / lumped node areas
+  bool hs_scale = false;
+  ebh::DevBuf<double> dinv;
   // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single
   // layer (inner), Laplace double layer and Kelvin single layer they wrap,
   // and two work buffers

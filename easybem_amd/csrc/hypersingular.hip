@@ -21,9 +21,11 @@ namespace {
 constexpr int kDotBlocks = 64;
 
 // w[(3r + c) ne + i] = sum_k curl[(3i + c) 3 + k] x[r np + tri[3i + k]]
+// (x scaled by dinv first when dinv is given)
 __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
                                                  const double *__restrict__ curl,
                                                  const int *__restrict__ tri,
+                                                 const double *__restrict__ dinv,
                                                  const double *__restrict__ x,
                                                  double *__restrict__ w)
 {
@@ -31,7 +33,13 @@ __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
   const int r = blockIdx.y;
   if (i >= ne) return;
   const double *xr = x + (size_t)r * np;
-  const double x0 = xr[tri[3 * i]], x1 = xr[tri[3 * i + 1]], x2 = xr[tri[3 * i + 2]];
+  const int n0 = tri[3 * i], n1 = tri[3 * i + 1], n2 = tri[3 * i + 2];
+  double x0 = xr[n0], x1 = xr[n1], x2 = xr[n2];
+  if (dinv) {
+    x0 *= dinv[n0];
+    x1 *= dinv[n1];
+    x2 *= dinv[n2];
+  }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const double *o = curl + (3 * (size_t)i + c) * 3;
@@ -41,6 +49,7 @@ __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
 
 // part[r kDotBlocks + b] = sum over block b's strided nodes of m_n x_n
 __global__ __launch_bounds__(256) void k_hs_dot(int np, const double *__restrict__ m,
+                                                const double *__restrict__ dinv,
                                                 const double *__restrict__ x,
                                                 double *__restrict__ part)
 {
@@ -49,7 +58,7 @@ __global__ __launch_bounds__(256) void k_hs_dot(int np, const double *__restrict
   const double *xr = x + (size_t)r * np;
   double s = 0.0;
   for (int n = blockIdx.x * 256 + threadIdx.x; n < np; n += kDotBlocks * 256)
-    s = fma(m[n], xr[n], s);
+    s = fma(m[n], dinv ? xr[n] * dinv[n] : xr[n], s);
   red[threadIdx.x] = s;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
@@ -66,7 +75,7 @@ __global__ __launch_bounds__(256) void k_hs_gather(
     const int *__restrict__ ptr, const int *__restrict__ ntri,
     const int *__restrict__ nk, const double *__restrict__ z,
     const double *__restrict__ m, const double *__restrict__ part,
-    double *__restrict__ y)
+    const double *__restrict__ dinv, double *__restrict__ y)
 {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
@@ -84,7 +93,7 @@ __global__ __launch_bounds__(256) void k_hs_gather(
     for (int b = 0; b < kDotBlocks; b++) s += part[r * kDotBlocks + b];
     acc = fma(stab * m[n], s, acc);
   }
-  y[(size_t)r * np + n] = acc;
+  y[(size_t)r * np + n] = dinv ? acc * dinv[n] : acc;
 }
 
 }  // namespace
@@ -107,6 +116,11 @@ int hs_build(ebh_op *op)
   }
   EBH_TRY(op->curl.upload(curl));
   EBH_TRY(op->mvec.upload(m));
+  if (op->hs_scale) {
+    std::vector<double> d(m.size());
+    for (size_t k = 0; k < m.size(); k++) d[k] = 1.0 / m[k];
+    EBH_TRY(op->dinv.upload(d));
+  }
   const size_t nb = (size_t)std::max(1, op->d.max_rhs);
   EBH_TRY(op->hsw.alloc(3 * (size_t)T.ne * nb));
   EBH_TRY(op->hsz.alloc(3 * (size_t)T.ne * nb));
@@ -124,12 +138,13 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
   }
   const int ne = op->mesh->T.ne, np = op->mesh->T.np;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
+  const double *dinv = op->hs_scale ? op->dinv.p : nullptr;
   hipLaunchKernelGGL(k_hs_curl, dim3((ne + 255) / 256, nrhs), dim3(256), 0, s, ne, np,
-                     op->curl.p, op->mesh->tri.p, x, op->hsw.p);
+                     op->curl.p, op->mesh->tri.p, dinv, x, op->hsw.p);
   EBH_HIP(hipGetLastError());
   if (op->d.stab != 0.0) {
     hipLaunchKernelGGL(k_hs_dot, dim3(kDotBlocks, nrhs), dim3(256), 0, s, np,
-                       op->mvec.p, x, op->hsdot.p);
+                       op->mvec.p, dinv, x, op->hsdot.p);
     EBH_HIP(hipGetLastError());
   }
   if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
@@ -138,10 +153,60 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
   hipLaunchKernelGGL(k_hs_gather, dim3((np + 255) / 256, nrhs), dim3(256), 0, s, ne,
                      np, op->d.stab, op->curl.p, op->mesh->node_tri_ptr.p,
                      op->mesh->node_tri.p, op->mesh->node_tri_k.p, op->hsz.p,
-                     op->mvec.p, op->hsdot.p, y);
+                     op->mvec.p, op->hsdot.p, dinv, y);
   EBH_HIP(hipGetLastError());
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }
 
 }  // namespace ebh
+
+using namespace ebh;
+
+extern "C" int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
+                                    int tri_rule, int max_rhs, ebh_op **out)
+{
+  if (!mesh || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  *out = nullptr;
+  double area = 0.0;
+  const Topology &T = mesh->T;
+  for (int i = 0; i < T.ne; i++) {
+    const double *P0 = &T.nodes[3 * T.tri[3 * i]];
+    const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
+    const double *P2 = &T.nodes[3 * T.tri[3 * i + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
+                 c2 = u[0] * v[1] - u[1] * v[0];
+    area += 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  }
+  ebh_op_desc d{};
+  d.kernel = EBH_LAPLACE_HS;
+  d.test_basis = d.trial_basis = EBH_P1;
+  d.far_precision = far_precision;
+  d.line_rule = line_rule;
+  d.tri_rule = tri_rule;
+  d.max_rhs = max_rhs;
+  // constants: V 1 ~ 1 on the unit sphere, so the rank-one term maps them
+  // to about 1/4 as D V does the other harmonics
+  d.stab = 0.25 / area;
+  ebh_op *op = nullptr;
+  EBH_TRY(ebh_op_create(mesh, &d, &op));
+  std::vector<double> m(T.np);
+  int rc = EBH_OK;
+  if (hipMemcpy(m.data(), op->mvec.p, sizeof(double) * T.np, hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    set_error("copy of the lumped areas failed");
+    rc = EBH_ERR_HIP;
+  }
+  for (double &v : m) v = 1.0 / v;
+  if (rc == EBH_OK) rc = op->dinv.upload(m);
+  if (rc != EBH_OK) {
+    ebh_op_destroy(op);
+    return rc;
+  }
+  op->hs_scale = true;
+  *out = op;
+  return EBH_OK;
+}
+

@@ -209,6 +209,10 @@ using namespace ebh;
 
 struct ebh_solver {
   ebh_op *a64 = nullptr, *a32 = nullptr;
+  // right preconditioners for the FP64 and FP32 operators (may be null)
+  ebh_op *p64 = nullptr, *p32 = nullptr;
+  DevBuf<double> Zt, Ut;             // nbmax x n: preconditioned vectors
+  int precond_applies = 0;
   int n = 0, m = 50, nbmax = 1;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -218,7 +222,8 @@ struct ebh_solver {
   DevBuf<int> active, jdone, allon;
   double *h_est = nullptr;           // pinned host mirrors
   int *h_active = nullptr, *h_jdone = nullptr;
-  std::map<std::tuple<const ebh_op *, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<const ebh_op *, const ebh_op *, int, int, int, int>, hipGraphExec_t>
+      graphs;
   bool use_graphs = true;
   int check = 4;
   int matvec64 = 0, matvec32 = 0;
@@ -259,12 +264,21 @@ struct ebh_solver {
   // second CGS coefficients live after the first in hc
   double *h2() { return hc.p + (size_t)nbmax * (m + 2); }
 
-  // One Arnoldi step j for nb systems (kernels only; graph-capturable).
+  ebh_op *precond_for(const ebh_op *A) const { return A == a64 ? p64 : p32; }
+
+  // One Arnoldi step j for nb systems (kernels only; graph-capturable):
+  // w = A P v_j (P the right preconditioner, if any), orthogonalised.
   int step_kernels(ebh_op *A, int j, int nb)
   {
     const size_t vs = vstride();
     double *vj = V.p + (size_t)j * vs, *w = V.p + (size_t)(j + 1) * vs;
-    EBH_TRY(op_apply_multi(A, vj, w, nb, stream, false));
+    ebh_op *P = precond_for(A);
+    if (P) {
+      EBH_TRY(op_apply_multi(P, vj, Zt.p, nb, stream, false));
+      EBH_TRY(op_apply_multi(A, Zt.p, w, nb, stream, false));
+    } else {
+      EBH_TRY(op_apply_multi(A, vj, w, nb, stream, false));
+    }
     const int ld = m + 2;
     const dim3 gv((n + 255) / 256, nb);
     // classical Gram-Schmidt, twice
@@ -290,10 +304,13 @@ struct ebh_solver {
   int step(ebh_op *A, int j, int nb)
   {
     // per-launch event profiling of the operator needs eager launches
+    ebh_op *P = precond_for(A);
+    if (P) precond_applies += nb;
     if (!use_graphs || A->prof_on) {
       EBH_TRY(step_kernels(A, j, nb));
     } else {
-      auto key = std::make_tuple((const ebh_op *)A, j, nb, far_op(A)->gen);
+      auto key = std::make_tuple((const ebh_op *)A, (const ebh_op *)P, j, nb,
+                                 far_op(A)->gen, P ? far_op(P)->gen : -1);
       auto it = graphs.find(key);
       if (it == graphs.end()) {
         hipGraph_t gr;
@@ -384,11 +401,22 @@ struct ebh_solver {
         iters[r] += h_jdone[r];
         all_met &= h_est[r] <= atol_h[r];
       }
-      // Z += V y with each system's own number of steps
+      // Z += P V y with each system's own number of steps
       hipLaunchKernelGGL(k_bbacksolve, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, m,
                          Rm.p, g.p, jdone.p, yv.p);
-      hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
-                         jdone.p, V.p, vstride(), yv.p, m, 1.0, Z, allon.p);
+      ebh_op *P = precond_for(A);
+      if (P) {
+        const size_t tot = (size_t)nb * n;
+        hipLaunchKernelGGL(k_fill, grid1(tot), dim3(256), 0, stream, tot, 0.0, Ut.p);
+        hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
+                           jdone.p, V.p, vstride(), yv.p, m, 1.0, Ut.p, allon.p);
+        EBH_TRY(op_apply_multi(P, Ut.p, Zt.p, nb, stream, false));
+        precond_applies += nb;
+        hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, stream, tot, 1.0, Zt.p, 1.0, Z);
+      } else {
+        hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
+                           jdone.p, V.p, vstride(), yv.p, m, 1.0, Z, allon.p);
+      }
       EBH_HIP(hipGetLastError());
       // the Arnoldi estimate met every tolerance: no restart residual needed
       // (callers that need the true residual compute it themselves)
@@ -426,7 +454,7 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
     set_error("batch size must be 1.." + std::to_string(s->nbmax));
     return EBH_ERR_ARG;
   }
-  s->matvec64 = s->matvec32 = 0;
+  s->matvec64 = s->matvec32 = s->precond_applies = 0;
   EBH_HIP(hipEventRecord(s->e0, s->stream));
   const size_t tot = (size_t)nb * s->n;
   std::vector<double> bn;
@@ -504,6 +532,7 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
       st[r].outer = outer;
       st[r].matvec64 = s->matvec64 / nb;
       st[r].matvec32 = s->matvec32 / nb;
+      st[r].precond = s->precond_applies / nb;
       st[r].relres = rel;
       st[r].ms = ms;
       st[r].converged = rel <= opt.tol;
@@ -592,6 +621,31 @@ int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart, int max_bat
 int ebh_solver_create(ebh_op *a64, ebh_op *a32, int restart, ebh_solver **out)
 {
   return ebh_solver_create_batched(a64, a32, restart, 1, out);
+}
+
+int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32)
+{
+  if (!s) { set_error("null solver"); return EBH_ERR_ARG; }
+  for (ebh_op *P : {p64, p32}) {
+    if (!P) continue;
+    if (P->nrows * P->nc != s->n || P->ncols * P->nc != s->n) {
+      set_error("preconditioner size does not match the system");
+      return EBH_ERR_ARG;
+    }
+    if (std::max(1, P->d.max_rhs) < s->nbmax) {
+      set_error("preconditioner max_rhs is below the solver's batch size");
+      return EBH_ERR_ARG;
+    }
+  }
+  EBH_HIP(hipStreamSynchronize(s->stream));
+  const size_t tot = (size_t)s->nbmax * s->n;
+  if (p64 || p32) {
+    EBH_TRY(s->Zt.alloc(tot));
+    EBH_TRY(s->Ut.alloc(tot));
+  }
+  s->p64 = p64;
+  s->p32 = p32;
+  return EBH_OK;
 }
 
 int ebh_solver_set(ebh_solver *s, int use_graphs, int check_every)

@@ -278,6 +278,7 @@ typedef struct {
   double relres;    /* final ||b - A64 x|| / ||b|| (true residual, FP64)  */
   double ms;        /* wall time of the solve on the stream (HIP events)  */
   int converged;
+  int precond;      /* right-preconditioner applies                       */
 } ebh_solve_stats;
 
 /* a64: FP64 operator (square: rows == cols).  a32: optional FP32-far
@@ -297,6 +298,21 @@ int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart,
  * invalidated), so multithreaded callers -- the solve server among them --
  * set use_graphs = 0. */
 int ebh_solver_set(ebh_solver *s, int use_graphs, int check_every);
+/* Right preconditioning (no counterpart in the reference): GMRES runs on
+ * A P y = b with x = P y, so the residual it minimises and reports stays
+ * ||b - A x||.  p64 goes with the FP64 operator, p32 with the FP32 one
+ * (the inner solves of the mixed refinement); either may be NULL.  The
+ * preconditioners are square operators of the system's size whose max_rhs
+ * covers the solver's batch. */
+int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32);
+/* Calderon preconditioner for the P1 single layer (no counterpart in the
+ * reference): P = M_L^-1 (D + m m^T / (4 |Gamma|)) M_L^-1 with D the
+ * hypersingular operator at the given rules and far-field precision, m the
+ * lumped node areas and M_L = diag(m).  Since D V = 1/4 - K^2 (Calderon),
+ * P V is well conditioned independently of the mesh size, and a coarse
+ * far rule (tri_rule 0, one point per triangle) suffices. */
+int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
+                         int tri_rule, int max_rhs, ebh_op **out);
 /* nb systems: b and x hold nb vectors of rows entries back to back; stats
  * (if non-NULL) receives nb entries. */
 int ebh_solver_solve_batched(ebh_solver *s, int nb, const double *b,
