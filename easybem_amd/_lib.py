@@ -359,6 +359,23 @@ def precond_calderon(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0,
     return op
 
 
+def precond_calderon_hs(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0):
+    """Dual Calderon preconditioner M_L^-1 V M_L^-1 for the P1
+    hypersingular operator (ebh_precond_calderon_hs), as an Operator."""
+    L = lib()
+    L.ebh_precond_calderon_hs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    check(L.ebh_precond_calderon_hs(mesh.h, 0 if precision == "fp64" else 1,
+                                    int(line_rule), int(tri_rule), C.byref(h)))
+    op = Operator.__new__(Operator)
+    op.mesh = mesh
+    op.h = h
+    op.desc = None
+    op.nrows = op.ncols = mesh.nodes.shape[0]
+    return op
+
+
 class Batch:
     """Operators of one kind on several meshes applied with one persistent
     far-field launch (ebh_batch_*)."""
@@ -486,7 +503,8 @@ class ServerOpts(C.Structure):
     _fields_ = [("max_slots", C.c_int), ("hbm_budget", C.c_size_t),
                 ("restart", C.c_int), ("tol", C.c_double), ("mixed", C.c_int),
                 ("line_rule", C.c_int), ("tri_rule", C.c_int),
-                ("max_iters", C.c_int)]
+                ("max_iters", C.c_int), ("dirichlet_p1", C.c_int),
+                ("no_precond", C.c_int)]
 
 
 class RequestResult(C.Structure):
@@ -504,7 +522,8 @@ class Server:
     """Continuous-batching Dirichlet / Neumann solve server (ebh_server_*)."""
 
     def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
-                 mixed=True, line_rule=4, tri_rule=4, max_iters=2000):
+                 mixed=True, line_rule=4, tri_rule=4, max_iters=2000,
+                 dirichlet_p1=False, precond=True):
         L = lib()
         vp = C.c_void_p
         L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -519,7 +538,9 @@ class Server:
         L.ebh_server_destroy.argtypes = [vp]
         L.ebh_server_destroy.restype = None
         o = ServerOpts(max_slots, hbm_budget, restart, tol, int(mixed),
-                       line_rule, tri_rule, max_iters)
+                       line_rule, tri_rule, max_iters, int(dirichlet_p1),
+                       int(not precond))
+        self.dirichlet_p1 = bool(dirichlet_p1)
         h = vp()
         check(L.ebh_server_create(C.byref(o), C.byref(h)))
         self.h = h
@@ -537,7 +558,8 @@ class Server:
                                           _ptr(nodes), elements.shape[0],
                                           _ptr(elements), _ptr(x0), key,
                                           C.byref(rid)))
-        self.ne[rid.value] = nodes.shape[0] if kind else elements.shape[0]
+        self.ne[rid.value] = (nodes.shape[0] if kind or self.dirichlet_p1
+                              else elements.shape[0])
         return rid.value
 
     def wait(self, rid, timeout=0.0, raise_on_error=False):

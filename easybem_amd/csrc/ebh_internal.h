@@ -135,6 +135,9 @@ struct ebh_op {
   // diag(dinv) (D + stab m m^T) diag(dinv), dinv = 1 / lumped node areas
   bool hs_scale = false;
   ebh::DevBuf<double> dinv;
+  // single layer applied between the same inverse lumped masses (the
+  // Calderon preconditioner of the hypersingular operator); uses hsw
+  bool pscale = false;
   // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single
   // layer (inner), Laplace double layer and Kelvin single layer they wrap,
   // and two work buffers

@@ -1040,10 +1040,23 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
   return EBH_OK;
 }
 
+__global__ void k_scale_vec(int n, const double *__restrict__ d, const double *x,
+                            double *y)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = d[i] * x[i];
+}
+
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
   if (op->inner) return composite_apply(op, x, y, 1, s, timed);
+  if (op->pscale) {
+    hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
+                       op->ncols, op->dinv.p, x, op->hsw.p);
+    EBH_HIP(hipGetLastError());
+    x = op->hsw.p;
+  }
   const int ne = op->mesh->T.ne;
   const bool f32 = op->d.far_precision == EBH_FP32;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
@@ -1055,6 +1068,11 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
   EBH_TRY(op_finish(op, x, y, s));
+  if (op->pscale) {
+    hipLaunchKernelGGL(k_scale_vec, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+                       op->nrows, op->dinv.p, y, y);
+    EBH_HIP(hipGetLastError());
+  }
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }
@@ -1063,7 +1081,7 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed)
 {
   if (op->inner) return composite_apply(op, x, y, nrhs, s, timed);
-  if (nrhs == 1 || !op->rb) {
+  if (nrhs == 1 || !op->rb || op->pscale) {
     const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
     for (int r = 0; r < nrhs; r++)
       EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, s, timed));
@@ -1359,7 +1377,7 @@ int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out)
         set_error("an operator appears twice in the batch");
         return EBH_ERR_ARG;
       }
-    if (ops[k]->inner) {
+    if (ops[k]->inner || ops[k]->pscale) {
       set_error("batch the operators a composite operator wraps instead");
       return EBH_ERR_UNSUPPORTED;
     }

@@ -210,3 +210,42 @@ extern "C" int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_
   return EBH_OK;
 }
 
+extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int line_rule,
+                                       int tri_rule, ebh_op **out)
+{
+  if (!mesh || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  *out = nullptr;
+  const Topology &T = mesh->T;
+  std::vector<double> m((size_t)T.np, 0.0);
+  for (int i = 0; i < T.ne; i++) {
+    const double *P0 = &T.nodes[3 * T.tri[3 * i]];
+    const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
+    const double *P2 = &T.nodes[3 * T.tri[3 * i + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
+                 c2 = u[0] * v[1] - u[1] * v[0];
+    const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    for (int k = 0; k < 3; k++) m[T.tri[3 * i + k]] += J / 6.0;
+  }
+  for (double &v : m) v = 1.0 / v;
+  ebh_op_desc d{};
+  d.kernel = EBH_LAPLACE_SL;
+  d.test_basis = d.trial_basis = EBH_P1;
+  d.far_precision = far_precision;
+  d.line_rule = line_rule;
+  d.tri_rule = tri_rule;
+  d.max_rhs = 1;
+  ebh_op *op = nullptr;
+  EBH_TRY(ebh_op_create(mesh, &d, &op));
+  int rc = op->dinv.upload(m);
+  if (rc == EBH_OK) rc = op->hsw.alloc(T.np);
+  if (rc != EBH_OK) {
+    ebh_op_destroy(op);
+    return rc;
+  }
+  op->pscale = true;
+  *out = op;
+  return EBH_OK;
+}
+

@@ -60,6 +60,7 @@ struct Entry {
   uint64_t key = 0;
   ebh_mesh *mesh = nullptr;
   ebh_op *A64 = nullptr, *A32 = nullptr, *K64 = nullptr;
+  ebh_op *P64 = nullptr, *P32 = nullptr;  // right preconditioners
   ebh_solver *solver = nullptr;
   size_t bytes = 0;
   uint64_t last_use = 0;
@@ -69,6 +70,8 @@ struct Entry {
     ebh_op_destroy(A64);
     ebh_op_destroy(A32);
     ebh_op_destroy(K64);
+    ebh_op_destroy(P64);
+    ebh_op_destroy(P32);
     ebh_mesh_destroy(mesh);
   }
 };
@@ -186,9 +189,10 @@ struct ebh_server {
     e->key = r.key;
     EBH_TRY(ebh_mesh_create(r.np, r.nodes.data(), r.ne, r.elems.data(), &e->mesh));
     const bool neu = r.problem == EBH_NEUMANN;
+    const bool p1 = neu || o.dirichlet_p1;
     ebh_op_desc d{};
     d.kernel = neu ? EBH_LAPLACE_HS : EBH_LAPLACE_SL;
-    d.test_basis = d.trial_basis = neu ? EBH_P1 : EBH_P0;
+    d.test_basis = d.trial_basis = p1 ? EBH_P1 : EBH_P0;
     d.far_precision = EBH_FP64;
     d.line_rule = o.line_rule;
     d.tri_rule = o.tri_rule;
@@ -202,11 +206,21 @@ struct ebh_server {
     }
     d.stab = 0.0;
     d.kernel = neu ? EBH_LAPLACE_ADL : EBH_LAPLACE_DL;
-    d.test_basis = neu ? EBH_P1 : EBH_P0;
+    d.test_basis = p1 ? EBH_P1 : EBH_P0;
     d.trial_basis = neu ? EBH_P0 : EBH_P1;
     d.far_precision = EBH_FP64;
     EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
     EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
+    if (p1 && !o.no_precond) {
+      // Calderon preconditioners with a 1-point far rule: D for V, V for D
+      auto make = [&](int prec, ebh_op **out) {
+        return neu ? ebh_precond_calderon_hs(e->mesh, prec, 2, 0, out)
+                   : ebh_precond_calderon(e->mesh, prec, 2, 0, 1, out);
+      };
+      EBH_TRY(make(EBH_FP64, &e->P64));
+      if (o.mixed) EBH_TRY(make(EBH_FP32, &e->P32));
+      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
+    }
     // eager launches: stream capture must not overlap other threads'
     // allocations and copies, which the other workers make at any time
     EBH_TRY(ebh_solver_set(e->solver, 0, 0));
@@ -217,7 +231,8 @@ struct ebh_server {
   // Calderon row V t = (1/2 + K) u (laplace3dDirichletGPU.py:60-70).
   int dirichlet_rhs(const Request &r, Entry &e, std::vector<double> &b)
   {
-    std::vector<double> u(r.np), mu_(r.ne), ku(r.ne);
+    const int nrow = o.dirichlet_p1 ? r.np : r.ne;
+    std::vector<double> u(r.np), mu_(nrow), ku(nrow);
     for (int n = 0; n < r.np; n++) {
       const double dx = r.nodes[3 * n] - r.x0[0], dy = r.nodes[3 * n + 1] - r.x0[1],
                    dz = r.nodes[3 * n + 2] - r.x0[2];
@@ -227,9 +242,9 @@ struct ebh_server {
         return EBH_ERR_NONFINITE;
       }
     }
-    EBH_TRY(ebh_mass_apply(e.mesh, EBH_P0, u.data(), mu_.data()));
+    EBH_TRY(ebh_mass_apply(e.mesh, o.dirichlet_p1 ? EBH_P1 : EBH_P0, u.data(), mu_.data()));
     EBH_TRY(ebh_op_apply_host(e.K64, u.data(), ku.data(), 1));
-    for (int i = 0; i < r.ne; i++) b[i] = 0.5 * mu_[i] + ku[i];
+    for (int i = 0; i < nrow; i++) b[i] = 0.5 * mu_[i] + ku[i];
     return EBH_OK;
   }
 
@@ -279,10 +294,10 @@ struct ebh_server {
   int run(Request &r, std::unique_ptr<Entry> &e)
   {
     const auto t0 = Clock::now();
-    r.res.ne = r.problem == EBH_NEUMANN ? r.np : r.ne;
+    r.res.ne = r.problem == EBH_NEUMANN || o.dirichlet_p1 ? r.np : r.ne;
     if (!e) EBH_TRY(build(r, e));
     const bool neu = r.problem == EBH_NEUMANN;
-    const int n = neu ? r.np : r.ne;
+    const int n = neu || o.dirichlet_p1 ? r.np : r.ne;
     std::vector<double> b(n);
     if (neu) EBH_TRY(neumann_rhs(r, *e, b));
     else EBH_TRY(dirichlet_rhs(r, *e, b));

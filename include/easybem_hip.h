@@ -313,6 +313,11 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32);
  * far rule (tri_rule 0, one point per triangle) suffices. */
 int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
                          int tri_rule, int max_rhs, ebh_op **out);
+/* The dual Calderon preconditioner, for the (stabilised) hypersingular
+ * operator on P1: P = M_L^-1 V M_L^-1 with V the P1 single layer at the
+ * given rules (no counterpart in the reference). */
+int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int line_rule,
+                            int tri_rule, ebh_op **out);
 /* nb systems: b and x hold nb vectors of rows entries back to back; stats
  * (if non-NULL) receives nb entries. */
 int ebh_solver_solve_batched(ebh_solver *s, int nb, const double *b,
@@ -348,6 +353,11 @@ typedef struct {
   int line_rule;       /* near-field rule index (default 4)                  */
   int tri_rule;        /* far-field rule index (default 4)                   */
   int max_iters;       /* per request (default 2000)                         */
+  int dirichlet_p1;    /* 1: Dirichlet flux in P1 (np entries) with the
+                          Calderon preconditioner; 0: P0 flux (ne entries),
+                          as the reference's GPU example                     */
+  int no_precond;      /* 1: no Calderon preconditioners (P1 Dirichlet and
+                          Neumann solves use them by default)                */
 } ebh_server_opts;
 
 typedef struct {
@@ -380,8 +390,9 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
                           int ne, const int *elems, const double *x0,
                           uint64_t mesh_key, int64_t *request_id);
 /* Blocks until the request finishes (timeout_s <= 0: no limit); t (may be
- * NULL) receives the unknown trace: ne entries (Dirichlet: the flux) or np
- * entries (Neumann: the trace).  Each id can be waited on once. */
+ * NULL) receives the unknown trace: the Dirichlet flux (ne entries, or np
+ * with dirichlet_p1) or the Neumann trace (np entries).  Each id can be
+ * waited on once. */
 int ebh_server_wait(ebh_server *s, int64_t request_id, double timeout_s,
                     ebh_request_result *res, double *t);
 /* [0] submitted, [1] completed, [2] failed, [3] active, [4] queued,

@@ -129,8 +129,50 @@ def test_neumann_server_matches_direct_solve():
     Kt = L.Operator(m, "Kt", 1, 0, "fp64")
     t, Mt = _neumann_data(P, T, x0)
     b = 0.5 * Mt - Kt(t)
-    u2, st = L.Solver(D64, D32).solve(b, tol=1e-8, mixed=True)
-    assert np.linalg.norm(u - u2) <= 1e-7 * np.linalg.norm(u2)
+    s = L.Solver(D64, D32)
+    s.set_precond(L.precond_calderon_hs(m, "fp64"), L.precond_calderon_hs(m, "fp32"))
+    u2, st = s.solve(b, tol=1e-8, mixed=True)
+    assert np.linalg.norm(u - u2) <= 1e-12 * np.linalg.norm(u2)
+    # without the preconditioner: the same solution to the solve tolerance
+    u3, st3 = L.Solver(D64, D32).solve(b, tol=1e-8, mixed=True)
+    assert np.linalg.norm(u - u3) <= 1e-6 * np.linalg.norm(u3)
+
+
+def test_dual_calderon_cuts_neumann_iterations():
+    """On a perturbed, anisotropically scaled sphere (level 4) the
+    preconditioned Neumann solve needs markedly fewer iterations."""
+    from easybem_amd import _lib as L
+    P, T, x0 = meshes.perturbed_icosphere(4, 77)
+    m = L.Mesh(P, T)
+    D64 = L.Operator(m, "D", 1, 1, "fp64", stab=1.0)
+    D32 = L.Operator(m, "D", 1, 1, "fp32", stab=1.0)
+    t, Mt = _neumann_data(P, T, x0)
+    b = 0.5 * Mt - L.Operator(m, "Kt", 1, 0, "fp64")(t)
+    _, st0 = L.Solver(D64, D32).solve(b, tol=1e-8, mixed=True)
+    s = L.Solver(D64, D32)
+    s.set_precond(L.precond_calderon_hs(m, "fp64"), L.precond_calderon_hs(m, "fp32"))
+    _, st1 = s.solve(b, tol=1e-8, mixed=True)
+    print("iterations", st0["iters"], st1["iters"])
+    assert st1["relres"] <= 1e-8 and st1["iters"] < 0.75 * st0["iters"]
+
+
+def test_dirichlet_p1_server_matches_preconditioned_direct_solve():
+    from easybem_amd import _lib as L
+    P, T, x0 = stream(1, seed=6, levels=(3,))[0]
+    srv = L.Server(max_slots=1, dirichlet_p1=True)
+    res, t = srv.wait(srv.submit(P, T, x0))
+    srv.close()
+    assert res["status"] == 0 and res["relres"] <= 1e-8 and t.size == P.shape[0]
+    m = L.Mesh(P, T)
+    V64 = L.Operator(m, "V", 1, 1, "fp64")
+    V32 = L.Operator(m, "V", 1, 1, "fp32")
+    K = L.Operator(m, "K", 1, 1, "fp64")
+    u = 1.0 / np.linalg.norm(P - x0, axis=1)
+    b = 0.5 * m.mass(u, 1) + K(u)
+    s = L.Solver(V64, V32)
+    s.set_precond(L.precond_calderon(m, "fp64"), L.precond_calderon(m, "fp32"))
+    t2, st = s.solve(b, tol=1e-8, mixed=True)
+    assert np.linalg.norm(t - t2) <= 1e-12 * np.linalg.norm(t2)
 
 
 def test_neumann_solution_converges_to_analytic_trace():
