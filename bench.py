@@ -288,7 +288,7 @@ def main():
                     help="also time configs 1, 3 and 4 (matvecs, batched solve)")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving stream of N requests on icosphere levels 4-6 "
-                         "(the default run serves 12 requests on levels 4-5)")
+                         "(the default run serves 24 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--precond", choices=("calderon", "none"), default="calderon",
                     help="right preconditioner of the GMRES solves (default: the "
