@@ -35,7 +35,7 @@ FLOP_SL = 11               # per far interaction, single layer, one RHS
 # scaled by the same counter on a 512 MiB device copy
 # (profiles/k_far_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
 # with the packaged launch shape.
-TRAFFIC_FAR32_L5_BYTES = 46.28e6
+TRAFFIC_FAR32_L5_BYTES = 190.19e6
 
 
 def log(*a):
