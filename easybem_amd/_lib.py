@@ -64,6 +64,9 @@ def lib():
         L.ebh_op_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.ebh_op_get_launch.argtypes = [vp, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ebh_op_set_schedule.argtypes = [vp, C.c_int, C.c_int]
+        L.ebh_op_get_schedule.argtypes = [vp, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
         L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
         L.ebh_op_work.argtypes = [vp, i64p]
@@ -208,6 +211,28 @@ class Operator:
         check(lib().ebh_op_get_launch(self.h, C.byref(b), C.byref(t),
                                       C.byref(c)))
         return b.value, t.value, c.value
+
+    SCHEDULES = {"chunked": 0, "symmetric": 1}
+
+    def set_schedule(self, schedule, split=0):
+        """Far-field schedule: "chunked" (every ordered triangle pair, the
+        shape of set_launch) or "symmetric" (each unordered pair of 64-triangle
+        groups once; V with equal test and trial bases only).  split: step
+        ranges per group pair of the symmetric schedule (0: default)."""
+        check(lib().ebh_op_set_schedule(self.h, self.SCHEDULES[schedule],
+                                        int(split)))
+
+    @property
+    def schedule(self):
+        v = C.c_int()
+        check(lib().ebh_op_get_schedule(self.h, C.byref(v), None))
+        return {0: "chunked", 1: "symmetric"}[v.value]
+
+    @property
+    def split(self):
+        v, k = C.c_int(), C.c_int()
+        check(lib().ebh_op_get_schedule(self.h, C.byref(v), C.byref(k)))
+        return k.value
 
     def time(self, x: DeviceArray, y: DeviceArray, reps=10, nrhs=1):
         ms = (C.c_double * 4)()
@@ -422,7 +447,8 @@ class TuneResult(C.Structure):
     _fields_ = [("block", C.c_int), ("tile", C.c_int), ("chunks", C.c_int),
                 ("grid", C.c_int), ("mgrid", C.c_int), ("ms", C.c_double),
                 ("ms_default", C.c_double), ("mms", C.c_double),
-                ("mms_default", C.c_double), ("tried", C.c_int)]
+                ("mms_default", C.c_double), ("tried", C.c_int),
+                ("schedule", C.c_int), ("split", C.c_int)]
 
 
 def evaluate(mesh: Mesh, points, t, u, tri_rule=4, precision="fp64"):

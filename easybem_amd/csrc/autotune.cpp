@@ -6,7 +6,9 @@
 // a tile) and the number of source chunks, expressed here as the total
 // workgroup count of the launch.  The tuner times whole applies with HIP
 // events on the operator's stream and walks the three parameters by
-// coordinate descent from the current shape.  Results go into a process-wide
+// coordinate descent from the current shape; operators that admit the
+// pair-symmetric schedule then also time it over its step split, and the
+// faster schedule wins.  Results go into a process-wide
 // table keyed by device, operator kind and mesh size class; operators created
 // later under the same key start from the tuned shape.
 #include <algorithm>
@@ -28,6 +30,7 @@ namespace {
 struct Entry {
   int block, tile, grid, mgrid;
   double ms;
+  int sched;  // 0: chunked; k >= 1: pair-symmetric with k step ranges
 };
 
 std::mutex g_mu;
@@ -80,8 +83,15 @@ int time_apply(ebh_op *op, const double *x, double *y, bool multi, int reps,
   return EBH_OK;
 }
 
+// workgroups of the far-field launch
 int grid_of(const ebh_op *op)
 {
+  if (op->sym) {
+    const int ng = op->chunks / op->ksplit;
+    int nblk = 0;
+    for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+    return nblk * op->ksplit;
+  }
   const int tb = (op->mesh->T.ne + op->block - 1) / op->block;
   return tb * op->chunks;
 }
@@ -100,6 +110,9 @@ void tune_apply(ebh_op *op)
   if (e.mgrid > 0) op->mgrid = e.mgrid;
   // a recorded shape was valid when it was tuned; on failure the operator
   // keeps its default shape
+  if (e.sched > 0 && sym_eligible(op)) {
+    if (op_configure(op, 0, 0, e.sched, true) == EBH_OK) return;
+  }
   if (op_configure(op, e.block, e.tile, chunks_for_grid(op, e.block, e.grid)) != EBH_OK)
     (void)op_configure(op, 0, 0, 0);
 }
@@ -145,10 +158,18 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
     return EBH_OK;
   };
 
-  int bb = op->block, bt = op->tile, bg = grid_of(op);
+  // the starting configuration, then the chunked search from its shape (or
+  // from the default chunked shape when it starts symmetric)
+  double ms0 = 0.0;
+  EBH_TRY(time_apply(op, x.p, y.p, false, reps, &ms0));
+  spent += ms0 * reps;
+  tried++;
+  const bool start_sym = op->sym;
+  const int start_ks = op->ksplit;
+  int bb = start_sym ? 256 : op->block, bt = start_sym ? 64 : op->tile;
+  int bg = start_sym ? 4096 : grid_of(op);
   double best = 0.0;
   EBH_TRY(eval(bb, bt, bg, &best));
-  const double ms0 = best;
   const int blocks[] = {64, 128, 256};
   const int tiles[] = {16, 32, 64};
   const int grids[] = {1024, 2048, 4096, 8192, 16384, 32768};
@@ -172,7 +193,26 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
       if (ms < best) { best = ms; bt = t; }
     }
   }
-  EBH_TRY(op_configure(op, bb, bt, chunks_for_grid(op, bb, bg)));
+  // pair-symmetric schedule over its step split (stops after two slower
+  // splits in a row)
+  int bs = 0;
+  if (start_sym && ms0 < best) { best = ms0; bs = start_ks; }
+  if (sym_eligible(op)) {
+    int worse = 0;
+    double sbest = 1e300;
+    for (int ks = 1; ks <= 32 && worse < 2 && spent < budget_ms; ks *= 2) {
+      // larger splits stop at the partial-sum memory bound
+      if (op_configure(op, 0, 0, ks, true) != EBH_OK) break;
+      double ms;
+      EBH_TRY(time_apply(op, x.p, y.p, false, reps, &ms));
+      spent += ms * reps;
+      tried++;
+      if (ms < sbest) { sbest = ms; worse = 0; if (ms < best) { best = ms; bs = ks; } }
+      else worse++;
+    }
+  }
+  if (bs) EBH_TRY(op_configure(op, 0, 0, bs, true));
+  else EBH_TRY(op_configure(op, bb, bt, chunks_for_grid(op, bb, bg)));
 
   // multi-RHS kernel: its workgroup count, with the single-RHS shape fixed
   int bm = 0;
@@ -187,7 +227,7 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
       if (spent >= budget_ms) break;
       if (g == bm) continue;
       op->mgrid = g;
-      EBH_TRY(op_configure(op, bb, bt, op->chunks));
+      EBH_TRY(op_configure(op, bb, bt, bs ? bs : op->chunks, bs > 0));
       double ms;
       EBH_TRY(time_apply(op, x.p, y.p, true, reps, &ms));
       spent += ms * reps;
@@ -195,22 +235,25 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
       if (ms < mbest) { mbest = ms; bm = g; }
     }
     op->mgrid = bm;
-    EBH_TRY(op_configure(op, bb, bt, op->chunks));
+    EBH_TRY(op_configure(op, bb, bt, bs ? bs : op->chunks, bs > 0));
   }
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_table[key_of(op)] = Entry{bb, bt, bg, bm, best};
+    g_table[key_of(op)] = Entry{bb, bt, bg, bm, best, bs};
   }
-  out->block = bb;
-  out->tile = bt;
+  // the shape in effect (the symmetric schedule fixes block and tile)
+  out->block = op->block;
+  out->tile = op->tile;
   out->chunks = op->chunks;
-  out->grid = bg;
+  out->grid = bs ? grid_of(op) : bg;
   out->mgrid = bm;
   out->ms = best;
   out->ms_default = ms0;
   out->mms = mbest;
   out->mms_default = mms0;
   out->tried = tried;
+  out->schedule = bs ? EBH_SCHED_SYMMETRIC : EBH_SCHED_CHUNKED;
+  out->split = bs;
   return EBH_OK;
 }
 
@@ -222,7 +265,8 @@ int ebh_tune_save(const char *path)
   std::lock_guard<std::mutex> lk(g_mu);
   for (const auto &kv : g_table)
     f << kv.first << ' ' << kv.second.block << ' ' << kv.second.tile << ' '
-      << kv.second.grid << ' ' << kv.second.mgrid << ' ' << kv.second.ms << '\n';
+      << kv.second.grid << ' ' << kv.second.mgrid << ' ' << kv.second.ms << ' '
+      << kv.second.sched << '\n';
   return f ? EBH_OK : EBH_ERR_ARG;
 }
 
@@ -240,9 +284,12 @@ int ebh_tune_load(const char *path)
     std::istringstream is(line);
     std::string key;
     Entry e;
-    if (!(is >> key >> e.block >> e.tile >> e.grid >> e.mgrid >> e.ms) ||
-        (e.block != 64 && e.block != 128 && e.block != 256) || e.tile < 1 ||
-        e.tile > 64 || e.grid < 1 || e.mgrid < 0) {
+    e.sched = 0;
+    const bool ok = static_cast<bool>(is >> key >> e.block >> e.tile >> e.grid >> e.mgrid >> e.ms);
+    if (ok && !(is >> e.sched)) e.sched = 0;  // six-column lines: chunked
+    if (!ok || (e.block != 64 && e.block != 128 && e.block != 256) || e.tile < 1 ||
+        e.tile > 64 || e.grid < 1 || e.mgrid < 0 || e.sched < 0 || e.sched > 64 ||
+        (e.sched & (e.sched - 1))) {
       set_error(std::string(path) + ": malformed line " + std::to_string(n));
       return EBH_ERR_ARG;
     }

@@ -353,6 +353,32 @@ int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks)
   return op_configure(far_op(op), block, tile, chunks);
 }
 
+int ebh_op_set_schedule(ebh_op *op, int schedule, int split)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (schedule != EBH_SCHED_CHUNKED && schedule != EBH_SCHED_SYMMETRIC) {
+    set_error("schedule must be EBH_SCHED_CHUNKED or EBH_SCHED_SYMMETRIC");
+    return EBH_ERR_ARG;
+  }
+  if (split < 0) {
+    set_error("split must be 0 (default) or a power of 2 up to 64");
+    return EBH_ERR_ARG;
+  }
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  ebh_op *f = far_op(op);
+  if (schedule == EBH_SCHED_SYMMETRIC) return op_configure(f, 0, 0, split, true);
+  return f->sym ? op_configure(f, 0, 0, 0) : EBH_OK;
+}
+
+int ebh_op_get_schedule(const ebh_op *op, int *schedule, int *split)
+{
+  if (!op || !schedule) { set_error("null argument"); return EBH_ERR_ARG; }
+  const ebh_op *f = far_op(op);
+  *schedule = f->sym ? EBH_SCHED_SYMMETRIC : EBH_SCHED_CHUNKED;
+  if (split) *split = f->sym ? f->ksplit : 0;
+  return EBH_OK;
+}
+
 int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }

@@ -145,6 +145,8 @@ struct ebh_op {
   ebh::DevBuf<double> kt1, kt2;
   // launch configuration
   int block = 256, tile = 64, chunks = 1, chunk = 0;
+  bool sym = false;  // pair-symmetric far-field schedule (k_far_sym)
+  int ksplit = 1;    // its step ranges per group pair
   int gen = 0;  // bumped when the launch shape changes (captured graphs key on it)
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -162,7 +164,11 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed);
 int op_build_near(ebh_op *op);
 int op_build_far(ebh_op *op);
-int op_configure(ebh_op *op, int block, int tile, int chunks);
+// sym: the pair-symmetric schedule (block, tile and chunks then follow from
+// the mesh); only for operators that sym_eligible accepts
+int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
+bool sym_eligible(const ebh_op *op);
+int sym_default_split(int ng);
 int hs_build(ebh_op *op);
 int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
              bool timed);

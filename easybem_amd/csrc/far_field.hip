@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "ebh_internal.h"
 
@@ -58,6 +59,7 @@ __device__ __forceinline__ double rsq(double x)
 template <typename R>
 struct FarArgs {
   int ne, chunks, chunk, tile, nout;  // nout: 1 (P0 test) or 3 (P1 test)
+  int ksplit;                         // pair-symmetric: step ranges per group pair
   R kappa;                            // Kelvin: 3 - 4 nu
   const R *tq;                        // [nt][3][ne] (FP32 double layer: offsets
                                       // from the triangle centroid)
@@ -356,6 +358,164 @@ __global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict_
   }
 }
 
+// Pair-symmetric single-layer far field, for V with equal test and trial
+// bases and a median rule.  G(x - y) = G(y - x), so one rsqrt serves the
+// point pair (a of triangle i, b of triangle j) in both directions: the lane
+// of target triangle i adds sigma_b / r to its point a, and sigma_a / r to
+// an accumulator of source point b, sigma_a being i's own folded strength.
+//
+// Triangles form groups of 64 (one wave).  A wave takes the group pair
+// (I, J), I < J, in 64 steps: at step k lane L pairs its target triangle
+// 64 I + L with source triangle 64 J + (L + k) mod 64, read from LDS, and
+// after every step the source accumulators move down one lane (ds_bpermute
+// from lane L + 1), so after 64 steps lane L holds the sums of source
+// triangle 64 J + L.  Pairs inside one group (I == J) take the target side
+// only.  Contributions to the targets of group I from the sources of group
+// J go to partial[J][.][t] (target side), those to the targets of group J
+// from group I to partial[I][.][t] (source side): each slot of the
+// [ng][nout][ne] buffer is written exactly once, and k_reduce_wide adds the ng
+// slots of a target in a fixed order.  A workgroup is 4 waves, target groups
+// 4M .. 4M+3, with source group J >= 4M staged in LDS; a wave with I > J has
+// no work (its pair is the source side of the (J, I) wave).  For small
+// meshes the 64 steps split into ksplit ranges, one workgroup each, with
+// ksplit partial slots per group (slot J ksplit + range).
+template <typename R, int NT>
+__global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
+{
+  struct alignas(4 * sizeof(R)) R4 { R v[4]; };
+  __shared__ R4 lds[64 * NT];
+  constexpr int C0 = NT % 3;
+  const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
+  int rem = (int)blockIdx.x / ks, M = 0;
+  const int rng = (int)blockIdx.x - rem * ks;
+  while (rem >= ng - 4 * M) { rem -= ng - 4 * M; M++; }
+  const int J = 4 * M + rem;
+  const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
+  {
+    // source group J: (x, y, z, sigma) of its 64 NT points; points past the
+    // mesh get strength 0, far away
+    const R4 *g = reinterpret_cast<const R4 *>(a.src);
+    const int p0 = J * 64 * NT, pend = ne * NT;
+    for (int p = threadIdx.x; p < 64 * NT; p += blockDim.x) {
+      R4 r;
+      if (p0 + p < pend) {
+        r = g[p0 + p];
+      } else {
+        r.v[0] = r.v[1] = r.v[2] = R(1e18);
+        r.v[3] = R(0);
+      }
+      lds[p] = r;
+    }
+  }
+  __syncthreads();
+  const int L = threadIdx.x & 63;
+  const int I = 4 * M + (int)(threadIdx.x >> 6);
+  if (I > J) return;  // wave-uniform; no barrier follows
+  const int t = I * 64 + L;
+  const int tt = t < ne ? t : ne - 1;
+  const R Cx = a.tsl[0 * (size_t)ne + tt], Cy = a.tsl[1 * (size_t)ne + tt],
+          Cz = a.tsl[2 * (size_t)ne + tt];
+  const R F0x = a.tsl[3 * (size_t)ne + tt], F0y = a.tsl[4 * (size_t)ne + tt],
+          F0z = a.tsl[5 * (size_t)ne + tt];
+  const R F1x = a.tsl[6 * (size_t)ne + tt], F1y = a.tsl[7 * (size_t)ne + tt],
+          F1z = a.tsl[8 * (size_t)ne + tt];
+  R E[NT], sq[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    E[q] = a.tsl[(9 + q) * (size_t)ne + tt];
+    sq[q] = t < ne ? a.src[4 * ((size_t)tt * NT + q) + 3] : R(0);
+  }
+  // touching triangles of group J (a lane past the mesh excludes them all)
+  uint64_t mask = ~0ull;
+  if (t < ne) {
+    mask = 0;
+    const int j0 = J * 64;
+    for (int c = a.touch_ptr[tt]; c < a.touch_ptr[tt + 1]; c++) {
+      const unsigned j = (unsigned)(a.touch[c] - j0);
+      if (j < 64u) mask |= 1ull << j;
+    }
+  }
+  R acc[NT], S[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) acc[q] = S[q] = R(0);
+
+  auto steps = [&](auto sym_tag) {
+    constexpr bool SYM = decltype(sym_tag)::value;
+    const int from = ((L + 1) & 63) << 2;  // ds_bpermute byte address
+    for (int k = k0; k < k1; k++) {
+      const int jl = (L + k) & 63;
+      // 0 for a touching source triangle (handled by the near field), else 1
+      const R keep = (R)(1u - (unsigned)((mask >> jl) & 1ull));
+      const R4 *rec = lds + jl * NT;
+#pragma unroll
+      for (int b = 0; b < NT; b++) {
+        const R4 r = rec[b];
+        const R Dx = Cx - r.v[0], Dy = Cy - r.v[1], Dz = Cz - r.v[2];
+        const R sg = r.v[3] * keep;
+        // an excluded triangle gets r^2 + 1: finite, with strength 0
+        const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
+        R G[3];
+        G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
+        G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
+        G[2] = -(G[0] + G[1]);
+        R sb = R(0);
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
+          const R ri = rsq(r2);
+          acc[q] = fma(sg, ri, acc[q]);
+          if constexpr (SYM) sb = fma(sq[q], ri, sb);
+        }
+        if constexpr (SYM) S[b] = fma(sb, keep, S[b]);
+      }
+      if constexpr (SYM) {
+#pragma unroll
+        for (int b = 0; b < NT; b++) {
+          if constexpr (sizeof(R) == 4) {
+            S[b] = __int_as_float(__builtin_amdgcn_ds_bpermute(from, __float_as_int(S[b])));
+          } else {
+            const long long v = __double_as_longlong(S[b]);
+            const int lo = __builtin_amdgcn_ds_bpermute(from, (int)v);
+            const int hi = __builtin_amdgcn_ds_bpermute(from, (int)(v >> 32));
+            S[b] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+          }
+        }
+      }
+    }
+  };
+  if (I < J) steps(std::true_type{});
+  else steps(std::false_type{});
+
+  // point sums of triangle tri to partial[c]
+  auto store = [&](const R *v, int tri, int c) {
+    const size_t stride = (size_t)ne;
+    double *out = a.partial + (size_t)c * a.nout * stride + tri;
+    double wq[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
+    if (a.nout == 1) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < NT; q++) s = fma(wq[q], (double)v[q], s);
+      out[0] = s;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], (double)v[q], s);
+        out[k * stride] = s;
+      }
+    }
+  };
+  if (t < ne) store(acc, t, J * ks + rng);
+  if (I < J) {
+    // after the steps k0 .. k1-1 lane L holds source (L + k1) mod 64
+    const int j = J * 64 + ((L + k1) & 63);
+    if (j < ne) store(S, j, I * ks + rng);
+  }
+}
+
 // Multi-RHS single layer.  One lane per target quadrature point, RB
 // right-hand sides per lane; a block holds TB = blockDim/NT whole triangles.
 // Source records are read with wave-uniform addresses (scalar loads: the
@@ -574,6 +734,28 @@ __global__ void k_reduce(int n, int chunks, const double *__restrict__ partial,
   reduce_one(blockIdx.x * blockDim.x + threadIdx.x, n, chunks, partial, out);
 }
 
+// Chunk reduction for many chunks (the pair-symmetric schedule): 32 outputs
+// per workgroup, 8 threads per output each summing every 8th chunk, then the
+// 8 sums in a fixed order.
+__global__ __launch_bounds__(256) void k_reduce_wide(int n, int chunks,
+                                                     const double *__restrict__ partial,
+                                                     double *__restrict__ out)
+{
+  __shared__ double red[8][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int k = blockIdx.x * 32 + tx;
+  double s = 0.0;
+  if (k < n)
+    for (int c = ty; c < chunks; c += 8) s += partial[(size_t)c * n + k];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && k < n) {
+    double v = red[0][tx];
+    for (int y = 1; y < 8; y++) v += red[y][tx];
+    out[k] = v;
+  }
+}
+
 // One test DOF r of right-hand side rhs: near CSR row times x plus the far
 // sums gathered from the test triangles.
 __device__ __forceinline__ void finish_row(int r, int rhs, int nrows, int ncols, int nc,
@@ -671,6 +853,7 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
   a.chunks = op->chunks;
   a.chunk = op->chunk;
   a.tile = op->tile;
+  a.ksplit = op->sym ? op->ksplit : 1;
   a.nout = op->d.test_basis ? 3 : 1;
   a.kappa = (R)(3.0 - 4.0 * op->d.nu);
   if constexpr (sizeof(R) == 4) {
@@ -708,10 +891,32 @@ size_t far_lds(const ebh_op *op, int R_bytes)
 }
 
 template <typename R>
+int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a)
+{
+  const int ng = op->chunks / op->ksplit;
+  int nblk = 0;
+  for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+  const dim3 grid((unsigned)(nblk * op->ksplit)), blk(256);
+  switch (op->rule.nt) {
+  case 1: hipLaunchKernelGGL((k_far_sym<R, 1>), grid, blk, 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_sym<R, 3>), grid, blk, 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_sym<R, 4>), grid, blk, 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_sym<R, 6>), grid, blk, 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_sym<R, 7>), grid, blk, 0, s, a); break;
+  default:
+    set_error("no pair-symmetric kernel for rule size " + std::to_string(op->rule.nt));
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+template <typename R>
 int run_far(ebh_op *op, hipStream_t s)
 {
   FarArgs<R> a;
   fill_far_args(op, a);
+  if (op->sym) return run_far_sym<R>(op, s, a);
   const int ne = op->mesh->T.ne, nt = op->rule.nt;
   const int tb = (ne + op->block - 1) / op->block;
   dim3 grid((unsigned)(tb * op->chunks)), blk(op->block);
@@ -851,9 +1056,62 @@ int chunks_for_grid(const ebh_op *op, int block, int grid)
   return std::min(chunks, ne);
 }
 
-int op_configure(ebh_op *op, int block, int tile, int chunks)
+// step ranges per group pair without a tuned value: about 640 workgroups
+// for small meshes, at most 8 ranges (measured on MI355X: more ranges cost
+// more in partial sums than they gain in occupancy)
+int sym_default_split(int ng)
+{
+  int nblk = 0;
+  for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+  int ks = 1;
+  while (ks < 8 && nblk * ks * 2 <= 640) ks *= 2;
+  return ks;
+}
+
+// The symmetric schedule's partial sums take (ne / 64) x nout x ne FP64
+// values per step range, quadratic in the mesh size; above this bound
+// (ne = 95,000 with P1 test functions) the chunked schedule is used.
+constexpr double kSymPartialMax = 4.0 * 1024 * 1024 * 1024;
+
+bool sym_eligible(const ebh_op *op)
+{
+  const double ne = op->mesh->T.ne;
+  const double bytes = std::ceil(ne / 64.0) * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+  return op->d.kernel == EBH_LAPLACE_SL && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
+         op->d.test_basis == op->d.trial_basis && bytes <= kSymPartialMax;
+}
+
+int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
 {
   const int ne = op->mesh->T.ne;
+  if (sym) {
+    if (!sym_eligible(op)) {
+      set_error("the pair-symmetric schedule needs a Laplace single layer with equal "
+                "test and trial bases, a median-symmetric far rule and partial sums "
+                "of at most 4 GiB");
+      return EBH_ERR_UNSUPPORTED;
+    }
+    // groups of 64 triangles; chunks (when given) is the number of step
+    // ranges per group pair, 1 .. 64, a power of 2
+    const int ng = (ne + 63) / 64;
+    int ks = chunks;
+    if (ks <= 0) ks = sym_default_split(ng);
+    if (ks > 64 || (ks & (ks - 1))) {
+      set_error("pair-symmetric step split must be a power of 2 up to 64");
+      return EBH_ERR_ARG;
+    }
+    const double bytes = (double)ng * ks * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+    if (bytes > kSymPartialMax) {
+      set_error("pair-symmetric split " + std::to_string(ks) +
+                " needs more than 4 GiB of partial sums on this mesh");
+      return EBH_ERR_ARG;
+    }
+    block = 256;
+    tile = 64;
+    op->ksplit = ks;
+    chunks = ng * ks;
+  }
+  op->sym = sym;
   if (block) {
     if (block != 64 && block != 128 && block != 256) {
       set_error("far-field block size must be 64, 128 or 256");
@@ -870,9 +1128,14 @@ int op_configure(ebh_op *op, int block, int tile, int chunks)
   }
   // default: ~4096 workgroups, 256 CUs x several resident blocks
   if (chunks <= 0) chunks = chunks_for_grid(op, op->block, 4096);
-  chunks = std::min(chunks, ne);
-  op->chunk = (ne + chunks - 1) / chunks;
-  op->chunks = (ne + op->chunk - 1) / op->chunk;
+  if (sym) {
+    op->chunk = 64;
+    op->chunks = chunks;
+  } else {
+    chunks = std::min(chunks, ne);
+    op->chunk = (ne + chunks - 1) / chunks;
+    op->chunks = (ne + op->chunk - 1) / op->chunk;
+  }
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
   size_t need = (size_t)op->chunks * nout * ne;
   if (op->rb) {
@@ -1015,6 +1278,9 @@ int op_build_far(ebh_op *op)
       EBH_TRY(op->msig64.alloc(npts * groups * op->rb));
     }
   }
+  // default: the pair-symmetric schedule where it applies (half the
+  // rsqrts), else chunked; a tuned table entry may replace either
+  if (sym_eligible(op)) return op_configure(op, 0, 0, 0, true);
   const int block = ne >= 16384 ? 256 : ne >= 4096 ? 128 : 64;
   return op_configure(op, block, 64, 0);
 }
@@ -1025,8 +1291,12 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
 {
   const int ne = op->mesh->T.ne;
   const int nred = (op->d.test_basis ? 3 : 1) * op->nc * ne;
-  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
-                     op->chunks, op->partial.p, op->tri_out.p);
+  if (op->sym)
+    hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
+                       op->chunks, op->partial.p, op->tri_out.p);
+  else
+    hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
+                       op->chunks, op->partial.p, op->tri_out.p);
   EBH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,

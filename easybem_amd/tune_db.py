@@ -4,9 +4,10 @@ standard workloads, on the GPU this runs on:
 
     python -m easybem_amd.tune_db [--out PATH] [--budget-ms MS]
 
-Workloads: icosphere levels 3-6 (single layer P0 and P1, double layer P1,
-both far-field precisions), the 256x128 torus with the Kelvin single layer,
-and the 64-RHS single layer on level 6.  Each line of the output keys a
+Workloads: icosphere levels 3-6 (single layer P0 and P1 with the 7-point
+and the 1-point far rule -- the latter is the Calderon preconditioners' --,
+double layer P1, both far-field precisions), the 256x128 torus with the
+Kelvin single layer, and the 64-RHS single layer on level 6.  Each line of the output keys a
 shape on device, operator kind, precision, rule size, test basis, RHS block
 and mesh size class; ``easybem_amd._lib`` loads the file when the library is
 first opened.
@@ -25,14 +26,14 @@ def workloads():
     for lev in (3, 4, 5, 6):
         P, T = meshes.icosphere(lev)
         yield f"icosphere L{lev}", P, T, [
-            ("V", 0, 0, p, 1) for p in ("fp32", "fp64")] + [
-            ("V", 1, 1, p, 1) for p in ("fp32", "fp64")] + [
-            ("K", 1, 1, "fp64", 1), ("K", 0, 1, "fp64", 1)]
+            ("V", b, b, p, 1, r) for r in (4, 0) for b in (0, 1)
+            for p in ("fp32", "fp64")] + [
+            ("K", 1, 1, "fp64", 1, 4), ("K", 0, 1, "fp64", 1, 4)]
     P, T = meshes.icosphere(6)
-    yield "icosphere L6, 64 RHS", P, T, [("V", 0, 0, p, 64)
+    yield "icosphere L6, 64 RHS", P, T, [("V", 0, 0, p, 64, 4)
                                          for p in ("fp32", "fp64")]
     P, T = meshes.torus(1.0, 0.4, 256, 128)
-    yield "torus 256x128", P, T, [("kelvin", 1, 1, p, 1)
+    yield "torus 256x128", P, T, [("kelvin", 1, 1, p, 1, 4)
                                   for p in ("fp32", "fp64")]
 
 
@@ -46,13 +47,17 @@ def main():
     L.tune_clear()
     for name, P, T, ops in workloads():
         m = L.Mesh(P, T)
-        for kern, test, trial, prec, nrhs in ops:
+        for kern, test, trial, prec, nrhs, rule in ops:
             t0 = time.time()
-            op = L.Operator(m, kern, test, trial, prec, line_rule=4,
-                            tri_rule=4, nu=0.3, max_rhs=nrhs)
+            op = L.Operator(m, kern, test, trial, prec,
+                            line_rule=4 if rule else 2, tri_rule=rule,
+                            nu=0.3, max_rhs=nrhs)
             r = op.autotune(budget_ms=args.budget_ms)
-            print(f"{name} {kern}{test}{trial} {prec} rhs={nrhs}: block "
-                  f"{r['block']} tile {r['tile']} workgroups {r['grid']} "
+            sched = (f"symmetric split {r['split']}" if r["schedule"] else
+                     f"block {r['block']} tile {r['tile']} workgroups "
+                     f"{r['grid']}")
+            print(f"{name} {kern}{test}{trial} {prec} rule {rule} "
+                  f"rhs={nrhs}: {sched} "
                   f"mgrid {r['mgrid']}  {r['ms_default']:.3f} -> "
                   f"{r['ms']:.3f} ms"
                   + (f", multi {r['mms_default']:.2f} -> {r['mms']:.2f} ms"

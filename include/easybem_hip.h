@@ -134,9 +134,25 @@ int ebh_op_apply(ebh_op *op, const double *x, double *y, int nrhs,
 /* Same on host memory (copies in and out, synchronous). */
 int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs);
 /* Far-field launch configuration: block threads, source tile (triangles),
- * source chunks.  Zero keeps the current value. */
+ * source chunks.  Zero keeps the current value.  Selects the chunked
+ * schedule. */
 int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks);
 int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks);
+/* Far-field schedule.  EBH_SCHED_CHUNKED: every target triangle against
+ * source chunks (the shape set by ebh_op_set_launch).  EBH_SCHED_SYMMETRIC:
+ * each unordered pair of 64-triangle groups once, one rsqrt per point pair
+ * serving both directions; only for EBH_LAPLACE_SL with equal test and trial
+ * bases and a median-symmetric far rule (triangle rule indices 0-4),
+ * EBH_ERR_UNSUPPORTED otherwise.  split: the symmetric schedule's step
+ * ranges per group pair (workgroups per pair; a power of 2 up to 64, 0 for
+ * a default from the mesh size), ignored for EBH_SCHED_CHUNKED.  Both
+ * schedules compute the same sums in a different order.  No counterpart in
+ * the reference (its far-field loop, src/BEM3dLaplaceGPU.cu:445-488, visits
+ * every ordered pair). */
+#define EBH_SCHED_CHUNKED 0
+#define EBH_SCHED_SYMMETRIC 1
+int ebh_op_set_schedule(ebh_op *op, int schedule, int split);
+int ebh_op_get_schedule(const ebh_op *op, int *schedule, int *split);
 /* Averages over the last `ebh_op_time` calls: milliseconds per apply of
  * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
  * measured with HIP events on the op's stream over `reps` applies of the
@@ -221,7 +237,8 @@ void ebh_batch_destroy(ebh_batch *b);
  * hand (include/BEM3dLaplaceGPU.h:8-10).  Times whole applies with HIP
  * events on the op's stream over far-field launch shapes -- workgroup size
  * 64/128/256 target triangles, LDS source tile 16/32/64 triangles, 1024 to
- * 32768 workgroups -- by coordinate descent from the current shape, keeps
+ * 32768 workgroups -- by coordinate descent from the current shape, then,
+ * where it applies, the symmetric schedule with 1 to 32 step ranges; keeps
  * the fastest and records it in a process-wide table under the operator's
  * key (device, kernel, precision, rule size, test basis, RHS block, mesh
  * size class in half-octaves); operators created later under the same key
@@ -235,10 +252,13 @@ typedef struct {
   double ms, ms_default;          /* apply ms: chosen / starting shape */
   double mms, mms_default;        /* multi-RHS apply ms: chosen / starting */
   int tried;                      /* shapes timed */
+  int schedule, split;            /* chosen EBH_SCHED_* and symmetric split */
 } ebh_tune_result;
 int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out);
-/* Tuning table as text, one line per key: "key block tile grid mgrid ms".
- * Loading merges into the table (later lines win). */
+/* Tuning table as text, one line per key: "key block tile grid mgrid ms
+ * sched", sched 0 for the chunked schedule or the split of the symmetric
+ * one (a missing last column reads as 0).  Loading merges into the table
+ * (later lines win). */
 int ebh_tune_save(const char *path);
 int ebh_tune_load(const char *path);
 int ebh_tune_clear(void);

@@ -122,9 +122,11 @@ def test_autotune_keeps_results_and_records_shape(L, prec, tune_table):
     assert r["block"] in (64, 128, 256) and 1 <= r["tile"] <= 64
     assert r["tried"] >= 1 and r["ms"] <= r["ms_default"]
     assert op.launch() == (r["block"], r["tile"], r["chunks"])
+    assert op.schedule == ("symmetric" if r["schedule"] else "chunked")
+    assert op.split == r["split"]
     assert rel(op(x), y0) < (1e-14 if prec == "fp64" else 1e-6)
     op2 = L.Operator(m, "V", 1, 1, prec)
-    assert op2.launch() == op.launch()
+    assert op2.launch() == op.launch() and op2.schedule == op.schedule
     L.tune_clear()
     op3 = L.Operator(m, "V", 1, 1, prec)
     assert op3.launch()[1] == 64

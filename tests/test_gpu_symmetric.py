@@ -1,0 +1,168 @@
+"""Pair-symmetric far-field schedule (k_far_sym) of the single layer.
+
+The symmetric schedule sums every unordered pair of 64-triangle groups once
+and uses each rsqrt for both directions, so it computes the same matrix-free
+apply in another order.  Bars: against the oracle as the chunked schedule
+(FP64 <= 1e-12, FP32 <= 1e-6) and against the chunked FP64 apply <= 1e-14.
+Meshes cover one partial group (20 triangles), group counts that are not a
+multiple of 4 (the workgroup's 4 target groups) and a partial last group.
+"""
+import numpy as np
+import pytest
+
+from easybem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def L():
+    from easybem_amd import _lib
+    return _lib
+
+
+MESHES = {
+    "ico0": lambda: meshes.icosphere(0),           # 20 triangles, 1 group
+    "blob1": lambda: meshes.perturbed_icosphere(1, 3)[:2],  # 80: 2 groups
+    "torus12x6": lambda: meshes.torus(1.0, 0.4, 12, 6),     # 144: 3 groups
+    "ico2": lambda: meshes.icosphere(2),           # 320: 5 groups
+    "torus40x20": lambda: meshes.torus(1.0, 0.4, 40, 20),   # 1600: 25 groups
+}
+
+
+def _x(basis, P, T, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, P.shape[0] if basis else T.shape[0])
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_symmetric_matches_oracle_and_chunked(L, oracle, mesh, basis, prec):
+    P, T = MESHES[mesh]()
+    print(f"case mesh={mesh} ne={T.shape[0]} basis={basis} {prec}", flush=True)
+    x = _x(basis, P, T, 5)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "V", basis, basis, prec, line_rule=4, tri_rule=4)
+    op.set_schedule("chunked")
+    y_chunk = op(x)
+    op.set_schedule("symmetric")
+    assert op.schedule == "symmetric"
+    y = op(x)
+    yr = oracle.apply("V", basis, basis, P, T, x, line_idx=4, tri_idx=4).ravel()
+    err = rel(y, yr)
+    print("rel err vs oracle", err, "vs chunked", rel(y, y_chunk))
+    assert err <= (1e-12 if prec == "fp64" else 1e-6)
+    if prec == "fp64":
+        assert rel(y, y_chunk) <= 1e-14
+    # a second apply gives the same bits (fixed summation order)
+    assert np.array_equal(op(x), y)
+
+
+@pytest.mark.parametrize("tri_rule", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_symmetric_every_median_rule(L, tri_rule, prec):
+    P, T = meshes.icosphere(2)
+    print(f"case ico2 P1 tri_rule={tri_rule} {prec}", flush=True)
+    x = _x(1, P, T, 7)
+    m = L.Mesh(P, T)
+    ref = L.Operator(m, "V", 1, 1, "fp64", line_rule=3, tri_rule=tri_rule)
+    ref.set_schedule("chunked")
+    y_ref = ref(x)
+    op = L.Operator(m, "V", 1, 1, prec, line_rule=3, tri_rule=tri_rule)
+    op.set_schedule("symmetric")
+    err = rel(op(x), y_ref)
+    print("rel err", err)
+    assert err <= (1e-14 if prec == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("split", [1, 2, 8, 64])
+def test_symmetric_step_split(L, split):
+    """Every split of the 64 steps into ranges gives the same apply."""
+    P, T = meshes.torus(1.0, 0.4, 40, 20)
+    m = L.Mesh(P, T)
+    x = _x(1, P, T, 4)
+    op = L.Operator(m, "V", 1, 1, "fp64")
+    op.set_schedule("chunked")
+    y0 = op(x)
+    op.set_schedule("symmetric", split)
+    assert op.split == split
+    assert rel(op(x), y0) <= 1e-14
+
+
+def test_symmetric_split_rejected(L):
+    P, T = meshes.icosphere(1)
+    op = L.Operator(L.Mesh(P, T), "V", 0, 0, "fp64")
+    for bad in [3, 128, -2]:
+        with pytest.raises(L.EbhError):
+            op.set_schedule("symmetric", bad)
+
+
+@pytest.mark.parametrize("kernel,test,trial,tri_rule", [
+    ("K", 0, 1, 4), ("V", 0, 1, 4), ("V", 1, 0, 4), ("kelvin", 0, 0, 4),
+    ("V", 0, 0, 5)])
+def test_symmetric_rejected_where_it_does_not_apply(L, kernel, test, trial,
+                                                    tri_rule):
+    P, T = meshes.icosphere(1)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, kernel, test, trial, "fp64", tri_rule=tri_rule)
+    with pytest.raises(L.EbhError) as e:
+        op.set_schedule("symmetric")
+    assert e.value.code == -4
+    assert op.schedule == "chunked"
+
+
+def test_set_launch_returns_to_chunked(L):
+    P, T = meshes.icosphere(2)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "V", 1, 1, "fp64")
+    x = _x(1, P, T, 3)
+    op.set_schedule("symmetric")
+    ks = op.split
+    assert ks >= 1 and op.launch() == (256, 64, 5 * ks)
+    y = op(x)
+    op.set_launch(128, 32, 7)
+    assert op.schedule == "chunked"
+    assert rel(op(x), y) <= 1e-14
+
+
+def test_symmetric_hypersingular_inner(L):
+    """The hypersingular operator applies its P0 single layer through the
+    schedule of that inner operator."""
+    P, T = meshes.icosphere(2)
+    m = L.Mesh(P, T)
+    D = L.Operator(m, "D", 1, 1, "fp64", line_rule=2, tri_rule=0)
+    x = _x(1, P, T, 9)
+    D.set_schedule("chunked")
+    y0 = D(x)
+    D.set_schedule("symmetric")
+    assert D.schedule == "symmetric"
+    assert rel(D(x), y0) <= 1e-14
+
+
+def test_symmetric_mixed_solve(L):
+    """Mixed-precision GMRES with both operators on the symmetric schedule
+    (captured graphs included) reaches the FP64 residual and the chunked
+    solution."""
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    u = 1.0 / np.linalg.norm(P - np.array([0.0, 0.0, 2.0]), axis=1)
+    K = L.Operator(m, "K", 1, 1, "fp64")
+    b = 0.5 * m.mass(u, 1) + K(u)
+    sols = {}
+    for sched in ["chunked", "symmetric"]:
+        V64 = L.Operator(m, "V", 1, 1, "fp64")
+        V32 = L.Operator(m, "V", 1, 1, "fp32")
+        V64.set_schedule(sched)
+        V32.set_schedule(sched)
+        t, st = L.Solver(V64, V32).solve(b, tol=1e-8, mixed=True)
+        print(sched, st)
+        assert st["converged"] and st["relres"] <= 1e-8
+        r = b - V64(t)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8
+        sols[sched] = t
+    assert rel(sols["symmetric"], sols["chunked"]) < 1e-6

@@ -99,17 +99,22 @@ def test_tuning_table_round_trip(L, tmp_path):
     L.tune_save(str(saved))
     L.tune_clear()
     src = tmp_path / "in.txt"
-    src.write_text("gfx950x256/k0/p1/nt7/test1/rb0/ne29 256 64 4096 4096 3.2\n"
+    src.write_text("gfx950x256/k0/p1/nt7/test1/rb0/ne29 256 64 4096 4096 3.2 2\n"
                    "gfx950x256/k0/p0/nt7/test1/rb0/ne29 128 32 8192 0 10.5\n")
     L.tune_load(str(src))
     out = tmp_path / "out.txt"
     L.tune_save(str(out))
     lines = sorted(out.read_text().split("\n"))
-    assert lines[1:] == sorted(src.read_text().split("\n"))[1:]
-    bad = tmp_path / "bad.txt"
-    bad.write_text("key 96 64 4096 0 1.0\n")
-    with pytest.raises(L.EbhError):
-        L.tune_load(str(bad))
+    # a six-column line (no schedule column) reads as the chunked schedule
+    assert lines[1:] == sorted(
+        ["gfx950x256/k0/p1/nt7/test1/rb0/ne29 256 64 4096 4096 3.2 2",
+         "gfx950x256/k0/p0/nt7/test1/rb0/ne29 128 32 8192 0 10.5 0"])
+    for text in ["key 96 64 4096 0 1.0\n", "key 256 64 4096 0 1.0 3\n",
+                 "key 256 64 4096 0 1.0 128\n"]:
+        bad = tmp_path / "bad.txt"
+        bad.write_text(text)
+        with pytest.raises(L.EbhError):
+            L.tune_load(str(bad))
     L.tune_clear()
     L.tune_save(str(out))
     assert out.read_text() == ""
