@@ -30,12 +30,19 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
 FLOP_SL = 11               # per far interaction, single layer, one RHS
-# HBM bytes per launch of k_far<float,0,7> on the level-5 icosphere: two
+# HBM bytes per launch of the FP32 single-layer far kernel on the level-5
+# icosphere with the packaged schedule (pair-symmetric, one step range): two
 # rocprofv3 passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --pmc-probe`, each
 # scaled by the same counter on a 512 MiB device copy
-# (profiles/k_far_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
-# with the packaged launch shape.
+# (profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
+# with that schedule.
 TRAFFIC_FAR32_L5_BYTES = 190.19e6
+TRAFFIC_SCHEDULE = ("symmetric", 1)
+# The pair-symmetric schedule evaluates each far point pair once for both
+# directions: 11 FLOP for the target side plus one FMA (2 FLOP) for the
+# source side, against 2 x 11 algorithmic FLOP (pairs inside one 64-triangle
+# group, 1/ng of the work, take one direction only).
+SYM_EXECUTED_PER_ALGORITHMIC = 13.0 / 22.0
 
 
 def log(*a):
@@ -332,8 +339,16 @@ def main():
                 "shapes_timed": r["tried"]}
     for name, op in (("fp32", V32), ("fp64", V64)):
         blk, tile, chunks = op.launch()
-        tune[name] = {"block": blk, "tile": tile, "chunks": chunks,
-                      "workgroups": chunks * -(-int(T.shape[0]) // blk)}
+        if op.schedule == "symmetric":
+            ks = op.split
+            ng = chunks // ks
+            wg = ks * sum(ng - 4 * m for m in range(-(-ng // 4)))
+            tune[name] = {"schedule": "symmetric", "split": ks,
+                          "groups": ng, "workgroups": wg}
+        else:
+            tune[name] = {"schedule": "chunked", "block": blk, "tile": tile,
+                          "chunks": chunks,
+                          "workgroups": chunks * -(-int(T.shape[0]) // blk)}
     b = 0.5 * mesh.mass(u, 1) + K64(u)
     db = L.DeviceArray.from_host(b)
     dx = L.DeviceArray(b.size)
@@ -420,6 +435,7 @@ def main():
         st64 = {"matvec64": stats[-1]["iters"]}
 
     avg_far_ms = far_ms / max(far_launches, 1)
+    sym32 = V32.schedule == "symmetric"
     achieved = far_flop / (avg_far_ms * 1e-3) / 1e12
     lat_sorted = np.sort(lat)
     res = {
@@ -455,13 +471,19 @@ def main():
         "autotune": tune,
         "matvec_far_tflops": {"fp32": far_flop / (t32["far"] * 1e-3) / 1e12,
                               "fp64": far_flop / (t64["far"] * 1e-3) / 1e12},
-        "roofline": {"kernel": "k_far<float, single layer, 7 points>",
+        "roofline": {"kernel": ("k_far_sym<float, 7 points>" if sym32 else
+                                "k_far<float, single layer, 7 points>"),
                      "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": (TRAFFIC_FAR32_L5_BYTES if args.level == 5
-                                 and not args.retune else None),
+                                 and (V32.schedule, V32.split) == TRAFFIC_SCHEDULE
+                                 else None),
                      "flop_per_launch": far_flop,
+                     **({"executed_flop_per_launch":
+                         far_flop * SYM_EXECUTED_PER_ALGORITHMIC,
+                         "executed_tflops":
+                         achieved * SYM_EXECUTED_PER_ALGORITHMIC} if sym32 else {}),
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
     }
     if not args.no_cpu_baseline:

@@ -14,6 +14,15 @@ const char *get_error() { return g_error.c_str(); }
 
 using namespace ebh;
 
+namespace {
+// the single layer inside a composite takes its densities three at a time
+int inner_three(ebh_op *v)
+{
+  v->nr_max = 3;
+  return op_configure(v, v->block, v->tile, v->sym ? v->ksplit : v->chunks, v->sym);
+}
+}  // namespace
+
 size_t ebh_op::bytes() const
 {
   return (inner ? inner->bytes() : 0) + (inner_dl ? inner_dl->bytes() : 0) +
@@ -24,7 +33,8 @@ size_t ebh_op::bytes() const
          sg32.bytes() + tq64.bytes() + sg64.bytes() + tw.bytes() + swt.bytes() +
          normals.bytes() + src32.bytes() + src64.bytes() + partial.bytes() +
          tri_out.bytes() + mgeom32.bytes() + msig32.bytes() + mgeom64.bytes() +
-         msig64.bytes();
+         msig64.bytes() + src3_32.bytes() + src3_64.bytes() + tsl32.bytes() +
+         tsl64.bytes() + tc32.bytes() + dinv.bytes();
 }
 
 extern "C" {
@@ -290,7 +300,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     if ((rc = ebh_op_create(mesh, &vd, &op->inner)) ||
         (rc = ebh_op_create(mesh, &kd, &op->inner_kel)) ||
         (d.kernel == EBH_KELVIN_DL && (rc = ebh_op_create(mesh, &dd, &op->inner_dl))) ||
-        (rc = kel_build(op))) {
+        (rc = kel_build(op)) || (rc = inner_three(op->inner))) {
       ebh_op_destroy(op);
       return rc;
     }
@@ -301,7 +311,8 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     vd.test_basis = vd.trial_basis = EBH_P0;
     vd.stab = 0.0;
     vd.max_rhs = op->d.max_rhs > 1 ? 3 * op->d.max_rhs : 1;
-    if ((rc = ebh_op_create(mesh, &vd, &op->inner)) || (rc = hs_build(op))) {
+    if ((rc = ebh_op_create(mesh, &vd, &op->inner)) || (rc = hs_build(op)) ||
+        (rc = inner_three(op->inner))) {
       ebh_op_destroy(op);
       return rc;
     }

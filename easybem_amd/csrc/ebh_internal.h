@@ -147,6 +147,11 @@ struct ebh_op {
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   bool sym = false;  // pair-symmetric far-field schedule (k_far_sym)
   int ksplit = 1;    // its step ranges per group pair
+  // right-hand sides per symmetric launch the buffers hold (1, or 3 for the
+  // single layer inside a composite operator, applied to 3 or 9 densities)
+  int nr_max = 1;
+  ebh::DevBuf<float> src3_32;
+  ebh::DevBuf<double> src3_64;
   int gen = 0;  // bumped when the launch shape changes (captured graphs key on it)
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -379,11 +379,13 @@ __global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict_
 // no work (its pair is the source side of the (J, I) wave).  For small
 // meshes the 64 steps split into ksplit ranges, one workgroup each, with
 // ksplit partial slots per group (slot J ksplit + range).
-template <typename R, int NT>
+template <typename R, int NT, int NR>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
-  struct alignas(4 * sizeof(R)) R4 { R v[4]; };
-  __shared__ R4 lds[64 * NT];
+  // record: x, y, z and NR strengths (NR = 3: the Kelvin fold's layout)
+  constexpr int RW = NR == 1 ? 4 : 8;
+  struct alignas(4 * sizeof(R)) RecT { R v[RW]; };
+  __shared__ RecT lds[64 * NT];
   constexpr int C0 = NT % 3;
   const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
   int rem = (int)blockIdx.x / ks, M = 0;
@@ -392,17 +394,17 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
   const int J = 4 * M + rem;
   const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
   {
-    // source group J: (x, y, z, sigma) of its 64 NT points; points past the
-    // mesh get strength 0, far away
-    const R4 *g = reinterpret_cast<const R4 *>(a.src);
+    // source group J: positions and strengths of its 64 NT points; points
+    // past the mesh get strength 0, far away
+    const RecT *g = reinterpret_cast<const RecT *>(a.src);
     const int p0 = J * 64 * NT, pend = ne * NT;
     for (int p = threadIdx.x; p < 64 * NT; p += blockDim.x) {
-      R4 r;
+      RecT r;
       if (p0 + p < pend) {
         r = g[p0 + p];
       } else {
-        r.v[0] = r.v[1] = r.v[2] = R(1e18);
-        r.v[3] = R(0);
+#pragma unroll
+        for (int k = 0; k < RW; k++) r.v[k] = k < 3 ? R(1e18) : R(0);
       }
       lds[p] = r;
     }
@@ -419,11 +421,13 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
           F0z = a.tsl[5 * (size_t)ne + tt];
   const R F1x = a.tsl[6 * (size_t)ne + tt], F1y = a.tsl[7 * (size_t)ne + tt],
           F1z = a.tsl[8 * (size_t)ne + tt];
-  R E[NT], sq[NT];
+  R E[NT], sq[NT][NR];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
     E[q] = a.tsl[(9 + q) * (size_t)ne + tt];
-    sq[q] = t < ne ? a.src[4 * ((size_t)tt * NT + q) + 3] : R(0);
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
   }
   // touching triangles of group J (a lane past the mesh excludes them all)
   uint64_t mask = ~0ull;
@@ -435,9 +439,11 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
       if (j < 64u) mask |= 1ull << j;
     }
   }
-  R acc[NT], S[NT];
+  R acc[NT][NR], S[NT][NR];
 #pragma unroll
-  for (int q = 0; q < NT; q++) acc[q] = S[q] = R(0);
+  for (int q = 0; q < NT; q++)
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[q][r] = S[q][r] = R(0);
 
   auto steps = [&](auto sym_tag) {
     constexpr bool SYM = decltype(sym_tag)::value;
@@ -446,65 +452,81 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
       const int jl = (L + k) & 63;
       // 0 for a touching source triangle (handled by the near field), else 1
       const R keep = (R)(1u - (unsigned)((mask >> jl) & 1ull));
-      const R4 *rec = lds + jl * NT;
+      const RecT *rec = lds + jl * NT;
 #pragma unroll
       for (int b = 0; b < NT; b++) {
-        const R4 r = rec[b];
-        const R Dx = Cx - r.v[0], Dy = Cy - r.v[1], Dz = Cz - r.v[2];
-        const R sg = r.v[3] * keep;
+        const RecT rb = rec[b];
+        const R Dx = Cx - rb.v[0], Dy = Cy - rb.v[1], Dz = Cz - rb.v[2];
+        R sg[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) sg[r] = rb.v[3 + r] * keep;
         // an excluded triangle gets r^2 + 1: finite, with strength 0
         const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
         R G[3];
         G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
         G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
         G[2] = -(G[0] + G[1]);
-        R sb = R(0);
+        R sb[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) sb[r] = R(0);
 #pragma unroll
         for (int q = 0; q < NT; q++) {
           const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
           const R ri = rsq(r2);
-          acc[q] = fma(sg, ri, acc[q]);
-          if constexpr (SYM) sb = fma(sq[q], ri, sb);
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            acc[q][r] = fma(sg[r], ri, acc[q][r]);
+            if constexpr (SYM) sb[r] = fma(sq[q][r], ri, sb[r]);
+          }
         }
-        if constexpr (SYM) S[b] = fma(sb, keep, S[b]);
+        if constexpr (SYM) {
+#pragma unroll
+          for (int r = 0; r < NR; r++) S[b][r] = fma(sb[r], keep, S[b][r]);
+        }
       }
       if constexpr (SYM) {
 #pragma unroll
-        for (int b = 0; b < NT; b++) {
-          if constexpr (sizeof(R) == 4) {
-            S[b] = __int_as_float(__builtin_amdgcn_ds_bpermute(from, __float_as_int(S[b])));
-          } else {
-            const long long v = __double_as_longlong(S[b]);
-            const int lo = __builtin_amdgcn_ds_bpermute(from, (int)v);
-            const int hi = __builtin_amdgcn_ds_bpermute(from, (int)(v >> 32));
-            S[b] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+        for (int b = 0; b < NT; b++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            if constexpr (sizeof(R) == 4) {
+              S[b][r] = __int_as_float(
+                  __builtin_amdgcn_ds_bpermute(from, __float_as_int(S[b][r])));
+            } else {
+              const long long v = __double_as_longlong(S[b][r]);
+              const int lo = __builtin_amdgcn_ds_bpermute(from, (int)v);
+              const int hi = __builtin_amdgcn_ds_bpermute(from, (int)(v >> 32));
+              S[b][r] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+            }
           }
-        }
       }
     }
   };
   if (I < J) steps(std::true_type{});
   else steps(std::false_type{});
 
-  // point sums of triangle tri to partial[c]
-  auto store = [&](const R *v, int tri, int c) {
+  // point sums of triangle tri to partial[c][r][.][tri]
+  auto store = [&](const R (*v)[NR], int tri, int c) {
     const size_t stride = (size_t)ne;
-    double *out = a.partial + (size_t)c * a.nout * stride + tri;
     double wq[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
-    if (a.nout == 1) {
-      double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < NT; q++) s = fma(wq[q], (double)v[q], s);
-      out[0] = s;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
+    for (int r = 0; r < NR; r++) {
+      double *out = a.partial + ((size_t)c * NR + r) * a.nout * stride + tri;
+      if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], (double)v[q], s);
-        out[k * stride] = s;
+        for (int q = 0; q < NT; q++) s = fma(wq[q], (double)v[q][r], s);
+        out[0] = s;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], (double)v[q][r], s);
+          out[k * stride] = s;
+        }
       }
     }
   };
@@ -890,25 +912,34 @@ size_t far_lds(const ebh_op *op, int R_bytes)
   return (size_t)op->tile * op->rule.nt * rec * R_bytes;
 }
 
-template <typename R>
-int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a)
+template <typename R, int NR>
+int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
 {
-  const int ng = op->chunks / op->ksplit;
-  int nblk = 0;
-  for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
-  const dim3 grid((unsigned)(nblk * op->ksplit)), blk(256);
-  switch (op->rule.nt) {
-  case 1: hipLaunchKernelGGL((k_far_sym<R, 1>), grid, blk, 0, s, a); break;
-  case 3: hipLaunchKernelGGL((k_far_sym<R, 3>), grid, blk, 0, s, a); break;
-  case 4: hipLaunchKernelGGL((k_far_sym<R, 4>), grid, blk, 0, s, a); break;
-  case 6: hipLaunchKernelGGL((k_far_sym<R, 6>), grid, blk, 0, s, a); break;
-  case 7: hipLaunchKernelGGL((k_far_sym<R, 7>), grid, blk, 0, s, a); break;
+  const dim3 blk(256);
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR>), grid, blk, 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR>), grid, blk, 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR>), grid, blk, 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR>), grid, blk, 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_sym<R, 7, NR>), grid, blk, 0, s, a); break;
   default:
-    set_error("no pair-symmetric kernel for rule size " + std::to_string(op->rule.nt));
+    set_error("no pair-symmetric kernel for rule size " + std::to_string(nt));
     return EBH_ERR_UNSUPPORTED;
   }
   EBH_HIP(hipGetLastError());
   return EBH_OK;
+}
+
+// nr: right-hand sides per launch (1, or 3 with the 8-wide source records)
+template <typename R>
+int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
+{
+  const int ng = op->chunks / op->ksplit;
+  int nblk = 0;
+  for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+  const dim3 grid((unsigned)(nblk * op->ksplit));
+  return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, a)
+                 : launch_sym<R, 1>(op->rule.nt, grid, s, a);
 }
 
 template <typename R>
@@ -1149,6 +1180,15 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     op->mchunks = (ne + op->mchunk - 1) / op->mchunk;
     need = std::max(need, (size_t)op->mchunks * op->d.max_rhs * nout * ne);
   }
+  if (op->sym && op->nr_max == 3) {
+    // three right-hand sides per symmetric launch: 8-wide source records,
+    // partial sums and chunk sums of 3 RHS
+    const size_t npts = (size_t)ne * op->rule.nt;
+    if (op->d.far_precision == EBH_FP32) EBH_TRY(op->src3_32.alloc(npts * 8));
+    else EBH_TRY(op->src3_64.alloc(npts * 8));
+    need = std::max(need, (size_t)op->chunks * 3 * nout * ne);
+    if (op->tri_out.n < 3 * nout * ne) EBH_TRY(op->tri_out.alloc(3 * nout * ne));
+  }
   EBH_TRY(op->partial.alloc(need));
   op->gen++;
   return EBH_OK;
@@ -1344,13 +1384,59 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   return EBH_OK;
 }
 
+// Three right-hand sides (back to back) in one pair-symmetric launch: the
+// three strengths share a source record (the Kelvin fold's layout) and each
+// rsqrt serves 3 x 2 accumulations.
+template <typename R>
+int run_three(ebh_op *op, const double *x, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne, nt = op->rule.nt;
+  const int n = ne * nt;
+  R *src;
+  if constexpr (sizeof(R) == 4) src = op->src3_32.p; else src = op->src3_64.p;
+  hipLaunchKernelGGL((k_fold<R, 2>), dim3((n + 255) / 256), dim3(256), 0, s, ne, nt,
+                     op->sg64.p, op->swt.p, op->normals.p, op->mesh->tri.p,
+                     op->d.trial_basis, op->rule, x, op->ncols, kInv4Pi, src);
+  EBH_HIP(hipGetLastError());
+  FarArgs<R> a;
+  fill_far_args(op, a);
+  a.src = src;
+  const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
+  EBH_TRY(run_far_sym<R>(op, s, a, 3));
+  if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
+  return EBH_OK;
+}
+
+static int op_apply_three(ebh_op *op, const double *x, double *y, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne;
+  EBH_TRY(op->d.far_precision == EBH_FP32 ? run_three<float>(op, x, s)
+                                          : run_three<double>(op, x, s));
+  const int nred = 3 * (op->d.test_basis ? 3 : 1) * ne;
+  hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
+                     op->chunks, op->partial.p, op->tri_out.p);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256, 3), dim3(256), 0, s,
+                     op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
+                     op->near_ptr.p, op->near_col.p, op->near_val.p, x,
+                     op->tri_out.p, op->mesh->node_tri_ptr.p, op->mesh->node_tri.p,
+                     op->mesh->node_tri_k.p, y);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed)
 {
   if (op->inner) return composite_apply(op, x, y, nrhs, s, timed);
   if (nrhs == 1 || !op->rb || op->pscale) {
     const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;
-    for (int r = 0; r < nrhs; r++)
+    int r = 0;
+    if (op->sym && op->nr_max == 3 && !op->pscale && !timed)
+      for (; r + 3 <= nrhs; r += 3)
+        EBH_TRY(op_apply_three(op, x + r * cx, y + r * cy, s));
+    for (; r < nrhs; r++)
       EBH_TRY(op_apply_one(op, x + r * cx, y + r * cy, s, timed));
     return EBH_OK;
   }
