@@ -36,7 +36,7 @@ FLOP_SL = 11               # per far interaction, single layer, one RHS
 # scaled by the same counter on a 512 MiB device copy
 # (profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
 # with that schedule.
-TRAFFIC_FAR32_L5_BYTES = 190.19e6
+TRAFFIC_FAR32_L5_BYTES = 203.59e6
 TRAFFIC_SCHEDULE = ("symmetric", 1)
 # The pair-symmetric schedule evaluates each far point pair once for both
 # directions: 11 FLOP for the target side plus one FMA (2 FLOP) for the
