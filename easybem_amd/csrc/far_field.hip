@@ -379,6 +379,29 @@ __global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict_
 // no work (its pair is the source side of the (J, I) wave).  For small
 // meshes the 64 steps split into ksplit ranges, one workgroup each, with
 // ksplit partial slots per group (slot J ksplit + range).
+// value of lane L + 1 (mod 64) in lane L: DPP wave_rol:1 (control 0x134),
+// every lane of the wave active (the rotation reads no out-of-wave lane, so
+// the "old" operand is never used; passing the value itself avoids a copy)
+__device__ __forceinline__ int lane_rol1(int v)
+{
+  return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float lane_rol1(float v)
+{
+  return __int_as_float(lane_rol1(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_rol1(double v)
+{
+  const long long u = __double_as_longlong(v);
+  const int lo = lane_rol1((int)u), hi = lane_rol1((int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Added to r^2 of an excluded (touching) pair in the symmetric kernel.
+template <typename R> struct Excl;
+template <> struct Excl<float> { static constexpr float v = 0x1p100f; };
+template <> struct Excl<double> { static constexpr double v = 1e200; };
+
 template <typename R, int NT, int NR>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
@@ -445,51 +468,57 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 #pragma unroll
     for (int r = 0; r < NR; r++) acc[q][r] = S[q][r] = R(0);
 
+  // One-point rules move the source records between lanes too (DPP
+  // wave_rol:1, lane L reads lane L + 1 -- VALU moves instead of LDS reads,
+  // which would otherwise bound a step of so little arithmetic); longer
+  // rules read them from LDS by index.  The accumulators move with DPP for
+  // one-point rules and with ds_bpermute otherwise.
+  constexpr bool ROT = NT == 1;
   auto steps = [&](auto sym_tag) {
     constexpr bool SYM = decltype(sym_tag)::value;
     const int from = ((L + 1) & 63) << 2;  // ds_bpermute byte address
+    RecT cur;
+    if constexpr (ROT) cur = lds[(L + k0) & 63];
     for (int k = k0; k < k1; k++) {
       const int jl = (L + k) & 63;
-      // 0 for a touching source triangle (handled by the near field), else 1
-      const R keep = (R)(1u - (unsigned)((mask >> jl) & 1ull));
+      // a touching source triangle (handled by the near field) gets r^2 +
+      // 2^100 (FP32) or + 1e200 (FP64): 1/r below 1e-15 resp. 1e-100 in
+      // both directions, so no strength needs masking
+      const R excl = ((mask >> jl) & 1ull) ? Excl<R>::v : R(0);
       const RecT *rec = lds + jl * NT;
 #pragma unroll
       for (int b = 0; b < NT; b++) {
-        const RecT rb = rec[b];
+        RecT rb;
+        if constexpr (ROT) rb = cur; else rb = rec[b];
         const R Dx = Cx - rb.v[0], Dy = Cy - rb.v[1], Dz = Cz - rb.v[2];
-        R sg[NR];
-#pragma unroll
-        for (int r = 0; r < NR; r++) sg[r] = rb.v[3 + r] * keep;
-        // an excluded triangle gets r^2 + 1: finite, with strength 0
-        const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, R(1) - keep)));
+        const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
         R G[3];
         G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
         G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
         G[2] = -(G[0] + G[1]);
-        R sb[NR];
-#pragma unroll
-        for (int r = 0; r < NR; r++) sb[r] = R(0);
 #pragma unroll
         for (int q = 0; q < NT; q++) {
           const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
           const R ri = rsq(r2);
 #pragma unroll
           for (int r = 0; r < NR; r++) {
-            acc[q][r] = fma(sg[r], ri, acc[q][r]);
-            if constexpr (SYM) sb[r] = fma(sq[q][r], ri, sb[r]);
+            acc[q][r] = fma(rb.v[3 + r], ri, acc[q][r]);
+            if constexpr (SYM) S[b][r] = fma(sq[q][r], ri, S[b][r]);
           }
         }
-        if constexpr (SYM) {
+      }
+      if constexpr (ROT) {
 #pragma unroll
-          for (int r = 0; r < NR; r++) S[b][r] = fma(sb[r], keep, S[b][r]);
-        }
+        for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
       }
       if constexpr (SYM) {
 #pragma unroll
         for (int b = 0; b < NT; b++)
 #pragma unroll
           for (int r = 0; r < NR; r++) {
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (ROT) {
+              S[b][r] = lane_rol1(S[b][r]);
+            } else if constexpr (sizeof(R) == 4) {
               S[b][r] = __int_as_float(
                   __builtin_amdgcn_ds_bpermute(from, __float_as_int(S[b][r])));
             } else {
