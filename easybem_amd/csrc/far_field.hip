@@ -416,6 +416,11 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
   while (rem >= ng - 4 * M) { rem -= ng - 4 * M; M++; }
   const int J = 4 * M + rem;
   const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
+  // FP64 works with doubled positions, so the distances it forms are 2r and
+  // rsqrt gives 1/(2r); a Newton step from that estimate needs one multiply
+  // fewer (see below).  FP32 uses the positions as they are.
+  constexpr bool F64 = sizeof(R) == 8;
+  const R ps = F64 ? R(2) : R(1);
   {
     // source group J: positions and strengths of its 64 NT points; points
     // past the mesh get strength 0, far away
@@ -425,6 +430,11 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
       RecT r;
       if (p0 + p < pend) {
         r = g[p0 + p];
+        if constexpr (F64) {
+          r.v[0] *= ps;
+          r.v[1] *= ps;
+          r.v[2] *= ps;
+        }
       } else {
 #pragma unroll
         for (int k = 0; k < RW; k++) r.v[k] = k < 3 ? R(1e18) : R(0);
@@ -438,16 +448,19 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
   if (I > J) return;  // wave-uniform; no barrier follows
   const int t = I * 64 + L;
   const int tt = t < ne ? t : ne - 1;
-  const R Cx = a.tsl[0 * (size_t)ne + tt], Cy = a.tsl[1 * (size_t)ne + tt],
-          Cz = a.tsl[2 * (size_t)ne + tt];
+  // FP64: D, G and c^2 |P_k - C|^2 scale by 2, 2 and 4 with the positions,
+  // so r2 below is (2r)^2 with the same instructions
+  const R Cx = ps * a.tsl[0 * (size_t)ne + tt], Cy = ps * a.tsl[1 * (size_t)ne + tt],
+          Cz = ps * a.tsl[2 * (size_t)ne + tt];
   const R F0x = a.tsl[3 * (size_t)ne + tt], F0y = a.tsl[4 * (size_t)ne + tt],
           F0z = a.tsl[5 * (size_t)ne + tt];
   const R F1x = a.tsl[6 * (size_t)ne + tt], F1y = a.tsl[7 * (size_t)ne + tt],
           F1z = a.tsl[8 * (size_t)ne + tt];
-  R E[NT], sq[NT][NR];
+  R E[NT], sq[NT][NR], rc[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
-    E[q] = a.tsl[(9 + q) * (size_t)ne + tt];
+    E[q] = ps * ps * a.tsl[(9 + q) * (size_t)ne + tt];
+    rc[q] = ps * a.rc[q];
 #pragma unroll
     for (int r = 0; r < NR; r++)
       sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
@@ -498,8 +511,16 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
         G[2] = -(G[0] + G[1]);
 #pragma unroll
         for (int q = 0; q < NT; q++) {
-          const R r2 = q < C0 ? A : fma(a.rc[q], G[(q - C0) % 3], A + E[q]);
-          const R ri = rsq(r2);
+          const R r2 = q < C0 ? A : fma(rc[q], G[(q - C0) % 3], A + E[q]);
+          R ri;
+          if constexpr (F64) {
+            // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for
+            // 1/r from 2h is h (3 - r2 h^2): 3 instructions after rsqrt
+            const R h = __builtin_amdgcn_rsq(r2);
+            ri = h * fma(-r2, h * h, R(3));
+          } else {
+            ri = rsq(r2);
+          }
 #pragma unroll
           for (int r = 0; r < NR; r++) {
             acc[q][r] = fma(rb.v[3 + r], ri, acc[q][r]);
