@@ -509,24 +509,31 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
         G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
         G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
         G[2] = -(G[0] + G[1]);
+        R r2[NT], ri[NT];
 #pragma unroll
-        for (int q = 0; q < NT; q++) {
-          const R r2 = q < C0 ? A : fma(rc[q], G[(q - C0) % 3], A + E[q]);
-          R ri;
-          if constexpr (F64) {
-            // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for
-            // 1/r from 2h is h (3 - r2 h^2): 3 instructions after rsqrt
-            const R h = __builtin_amdgcn_rsq(r2);
-            ri = h * fma(-r2, h * h, R(3));
-          } else {
-            ri = rsq(r2);
-          }
+        for (int q = 0; q < NT; q++)
+          r2[q] = q < C0 ? A : fma(rc[q], G[(q - C0) % 3], A + E[q]);
+        if constexpr (F64) {
+          // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for 1/r
+          // from 2h is h (3 - r2 h^2): 3 instructions after rsqrt.  All
+          // estimates first, so each result is not needed right after its
+          // instruction.
+          R h[NT];
+#pragma unroll
+          for (int q = 0; q < NT; q++) h[q] = __builtin_amdgcn_rsq(r2[q]);
+#pragma unroll
+          for (int q = 0; q < NT; q++) ri[q] = h[q] * fma(-r2[q], h[q] * h[q], R(3));
+        } else {
+#pragma unroll
+          for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++)
 #pragma unroll
           for (int r = 0; r < NR; r++) {
-            acc[q][r] = fma(rb.v[3 + r], ri, acc[q][r]);
-            if constexpr (SYM) S[b][r] = fma(sq[q][r], ri, S[b][r]);
+            acc[q][r] = fma(rb.v[3 + r], ri[q], acc[q][r]);
+            if constexpr (SYM) S[b][r] = fma(sq[q][r], ri[q], S[b][r]);
           }
-        }
       }
       if constexpr (ROT) {
 #pragma unroll
