@@ -402,17 +402,25 @@ template <typename R> struct Excl;
 template <> struct Excl<float> { static constexpr float v = 0x1p100f; };
 template <> struct Excl<double> { static constexpr double v = 1e200; };
 
+// source record of the symmetric kernel: x, y, z and NR strengths (NR = 3:
+// the Kelvin fold's layout)
+template <typename R, int NR>
+struct alignas(4 * sizeof(R)) SymRec {
+  R v[NR == 1 ? 4 : 8];
+};
+
+// Workgroup bid of the symmetric schedule (see above); lds holds 64 NT
+// records.  Every thread of the workgroup calls it (one barrier).
 template <typename R, int NT, int NR>
-__global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
+__device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
+                                          SymRec<R, NR> *__restrict__ lds)
 {
-  // record: x, y, z and NR strengths (NR = 3: the Kelvin fold's layout)
   constexpr int RW = NR == 1 ? 4 : 8;
-  struct alignas(4 * sizeof(R)) RecT { R v[RW]; };
-  __shared__ RecT lds[64 * NT];
+  using RecT = SymRec<R, NR>;
   constexpr int C0 = NT % 3;
   const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
-  int rem = (int)blockIdx.x / ks, M = 0;
-  const int rng = (int)blockIdx.x - rem * ks;
+  int rem = bid / ks, M = 0;
+  const int rng = bid - rem * ks;
   while (rem >= ng - 4 * M) { rem -= ng - 4 * M; M++; }
   const int J = 4 * M + rem;
   const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
@@ -445,7 +453,7 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
   __syncthreads();
   const int L = threadIdx.x & 63;
   const int I = 4 * M + (int)(threadIdx.x >> 6);
-  if (I > J) return;  // wave-uniform; no barrier follows
+  if (I > J) return;  // wave-uniform; no barrier follows in this function
   const int t = I * 64 + L;
   const int tt = t < ne ? t : ne - 1;
   // FP64: D, G and c^2 |P_k - C|^2 scale by 2, 2 and 4 with the positions,
@@ -592,6 +600,37 @@ __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
     // after the steps k0 .. k1-1 lane L holds source (L + k1) mod 64
     const int j = J * 64 + ((L + k1) & 63);
     if (j < ne) store(S, j, I * ks + rng);
+  }
+}
+
+template <typename R, int NT, int NR>
+__global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
+{
+  __shared__ SymRec<R, NR> lds[64 * NT];
+  sym_block<R, NT, NR>(a, (int)blockIdx.x, lds);
+}
+
+// Persistent multi-mesh form of the symmetric schedule: the workgroups of
+// several operators' launches are items of one queue (operator k owns items
+// off[k] .. off[k+1]-1, item off[k] + b being its workgroup b), taken from
+// an atomic counter until the queue is empty.
+template <typename R, int NT>
+__global__ __launch_bounds__(256) void k_far_sym_multi(const FarArgs<R> *__restrict__ descs,
+                                                       const int *__restrict__ off,
+                                                       int nops, int *__restrict__ counter)
+{
+  __shared__ SymRec<R, 1> lds[64 * NT];
+  __shared__ int sh_item;
+  const int total = off[nops];
+  while (true) {
+    __syncthreads();  // sh_item and the LDS records of the last item are free
+    if (threadIdx.x == 0) sh_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = sh_item;
+    if (item >= total) break;
+    int k = 0;
+    while (item >= off[k + 1]) k++;
+    sym_block<R, NT, 1>(descs[k], item - off[k], lds);
   }
 }
 
@@ -1529,6 +1568,7 @@ constexpr int kMaxBatch = 64;
 // Per-member data of the batched fold / reduce / finish kernels.
 struct MemberDesc {
   int ne, nt, trial_p1, ncols, nrows, nc, test_p1, chunks, nred;
+  int sym;  // pair-symmetric schedule (group partials, wide reduction)
   double scale;
   const double *sg, *swt, *normals;
   const int *tri;
@@ -1555,12 +1595,34 @@ __global__ void k_fold_multi(const MemberDesc *__restrict__ d, RuleArgs rule, Ba
                       o.scale, reinterpret_cast<R *>(o.src));
 }
 
-// chunk partial sums of every member
+// chunk partial sums of every chunked member
 __global__ void k_reduce_multi(const MemberDesc *__restrict__ d)
 {
   const MemberDesc &o = d[blockIdx.y];
+  if (o.sym) return;
   reduce_one(blockIdx.x * blockDim.x + threadIdx.x, o.nred, o.chunks, o.partial,
              o.tri_out);
+}
+
+// group partial sums of every symmetric member, in k_reduce_wide's order
+__global__ __launch_bounds__(256) void k_reduce_wide_multi(const MemberDesc *__restrict__ d)
+{
+  const MemberDesc &o = d[blockIdx.y];
+  if (!o.sym || (int)blockIdx.x * 32 >= o.nred) return;  // workgroup-uniform
+  __shared__ double red[8][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int k = blockIdx.x * 32 + tx;
+  const int n = o.nred;
+  double s = 0.0;
+  if (k < n)
+    for (int c = ty; c < o.chunks; c += 8) s += o.partial[(size_t)c * n + k];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && k < n) {
+    double v = red[0][tx];
+    for (int y = 1; y < 8; y++) v += red[y][tx];
+    o.tri_out[k] = v;
+  }
 }
 
 __global__ void k_finish_multi(const MemberDesc *__restrict__ d, BatchPtrs p)
@@ -1593,6 +1655,19 @@ const void *multi_kernel(int nt)
 }
 
 template <typename R>
+const void *multi_sym_kernel(int nt)
+{
+  switch (nt) {
+  case 1: return reinterpret_cast<const void *>(&k_far_sym_multi<R, 1>);
+  case 3: return reinterpret_cast<const void *>(&k_far_sym_multi<R, 3>);
+  case 4: return reinterpret_cast<const void *>(&k_far_sym_multi<R, 4>);
+  case 6: return reinterpret_cast<const void *>(&k_far_sym_multi<R, 6>);
+  case 7: return reinterpret_cast<const void *>(&k_far_sym_multi<R, 7>);
+  default: return nullptr;
+  }
+}
+
+template <typename R>
 const void *multi_kernel_for(int kernel, int nt)
 {
   switch (kernel) {
@@ -1607,9 +1682,10 @@ const void *multi_kernel_for(int kernel, int nt)
 }  // namespace ebh
 
 // Operators of one kind on several meshes, applied with one persistent
-// far-field launch.  Target blocks are 256 triangles; each operator keeps
-// its own source chunks and LDS tile, so every result equals the
-// operator's own apply.
+// far-field launch per schedule.  Chunked members: target blocks of 256
+// triangles, each operator with its own source chunks and LDS tile.
+// Symmetric members: each operator's workgroups as queue items.  Every
+// result equals the operator's own apply.
 struct ebh_batch {
   std::vector<ebh_op *> ops;
   int kernel = 0, prec = 0, nt = 0, nops = 0;
@@ -1617,27 +1693,31 @@ struct ebh_batch {
   ebh::DevBuf<unsigned char> mdesc; // MemberDesc[nops]
   int max_fold = 0, max_red = 0, max_rows = 0;
   ebh::DevBuf<int> off;             // [8][nops + 1] first item per queue
-  ebh::DevBuf<int> counters;        // [8]
+  ebh::DevBuf<int> soff;            // [nops + 1] first symmetric item
+  ebh::DevBuf<int> counters;        // [9]: 8 chunked queues, 1 symmetric
   std::vector<int> gens;
   size_t lds = 0;
-  int grid = 0, items = 0;
-  const void *kern = nullptr;
+  int grid = 0, items = 0, sgrid = 0, sitems = 0;
+  const void *kern = nullptr, *skern = nullptr;
 };
 
 namespace ebh {
 namespace {
 
-// Members' source chunking for a batch: one chunk size for all, chosen so
-// that the launch holds about 8 work items per resident workgroup.
+// Members' far-field schedules for a batch: the symmetric schedule with one
+// step range where it applies (the batch supplies the parallelism), else
+// one chunk size for all, chosen so that the launch holds about 8 work
+// items per resident workgroup.
 int batch_shapes(ebh_batch *b, int grid)
 {
   double work = 0.0;
   for (const ebh_op *op : b->ops)
-    work += (double)((op->mesh->T.ne + 255) / 256) * op->mesh->T.ne;
+    if (!sym_eligible(op)) work += (double)((op->mesh->T.ne + 255) / 256) * op->mesh->T.ne;
   int cs = (int)std::ceil(work / (8.0 * std::max(grid, 1)));
   cs = std::max(64, (cs + 63) / 64 * 64);
   for (ebh_op *op : b->ops) {
     const int ne = op->mesh->T.ne;
+    if (sym_eligible(op) && op_configure(op, 0, 0, 1, true) == EBH_OK) continue;
     EBH_TRY(op_configure(op, 256, 64, std::max(1, (ne + cs - 1) / cs)));
   }
   return EBH_OK;
@@ -1660,6 +1740,7 @@ int batch_prepare(ebh_batch *b)
     m.nc = op->nc;
     m.test_p1 = op->d.test_basis;
     m.chunks = op->chunks;
+    m.sym = op->sym ? 1 : 0;
     m.nred = (op->d.test_basis ? 3 : 1) * op->nc * m.ne;
     m.scale = op->d.kernel == EBH_KELVIN_SL
                   ? (1.0 + op->d.nu) / (8.0 * M_PI * (1.0 - op->d.nu))
@@ -1696,17 +1777,31 @@ int batch_prepare(ebh_batch *b)
     for (int k = 0; k < nops; k++) {
       const ebh_op *op = b->ops[k];
       const int tb = (op->mesh->T.ne + 255) / 256;
-      const int cx = std::max(0, (op->chunks - x + 7) / 8);
+      const int cx = op->sym ? 0 : std::max(0, (op->chunks - x + 7) / 8);
       o[k + 1] = o[k] + tb * cx;
     }
     items += o[nops];
   }
+  // symmetric members: their workgroups, operator by operator
+  std::vector<int> soff((size_t)nops + 1, 0);
+  for (int k = 0; k < nops; k++) {
+    const ebh_op *op = b->ops[k];
+    int nblk = 0;
+    if (op->sym) {
+      const int ng = op->chunks / op->ksplit;
+      for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+      nblk *= op->ksplit;
+    }
+    soff[k + 1] = soff[k] + nblk;
+  }
   EBH_TRY(b->desc.upload(reinterpret_cast<const unsigned char *>(d.data()),
                          sizeof(FarArgs<R>) * nops));
   EBH_TRY(b->off.upload(off));
-  EBH_TRY(b->counters.alloc(8));
+  EBH_TRY(b->soff.upload(soff));
+  EBH_TRY(b->counters.alloc(9));
   b->kern = multi_kernel_for<R>(b->kernel, b->nt);
-  if (!b->kern) {
+  b->skern = multi_sym_kernel<R>(b->nt);
+  if (!b->kern || (soff[nops] > 0 && !b->skern)) {
     set_error("unsupported kernel or rule size for a batch");
     return EBH_ERR_UNSUPPORTED;
   }
@@ -1716,6 +1811,13 @@ int batch_prepare(ebh_batch *b)
                                 b->ops[0]->mesh->device));
   b->grid = std::max(1, std::min(items, std::max(1, per_cu) * cus));
   b->items = items;
+  b->sitems = soff[nops];
+  b->sgrid = 0;
+  if (b->sitems > 0) {
+    int sper = 0;
+    EBH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sper, b->skern, 256, 0));
+    b->sgrid = std::max(1, std::min(b->sitems, std::max(1, sper) * cus));
+  }
   b->lds = lds;
   for (int k = 0; k < nops; k++) b->gens[k] = b->ops[k]->gen;
   return EBH_OK;
@@ -1749,17 +1851,28 @@ int batch_apply(ebh_batch *b, const double *const *x, double *const *y, hipStrea
     break;
   }
   EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, 8, b->counters.p);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, 9, b->counters.p);
   EBH_HIP(hipGetLastError());
   const FarArgs<R> *descs = reinterpret_cast<const FarArgs<R> *>(b->desc.p);
-  const int *offp = b->off.p;
   int nops = b->nops;
-  int *ctr = b->counters.p;
-  void *args[] = {(void *)&descs, (void *)&offp, (void *)&nops, (void *)&ctr};
-  EBH_HIP(hipLaunchKernel(b->kern, dim3(b->grid), dim3(256), args, b->lds, s));
-  hipLaunchKernelGGL(k_reduce_multi, dim3((b->max_red + 255) / 256, b->nops), dim3(256),
-                     0, s, md);
-  EBH_HIP(hipGetLastError());
+  if (b->items > 0) {
+    const int *offp = b->off.p;
+    int *ctr = b->counters.p;
+    void *args[] = {(void *)&descs, (void *)&offp, (void *)&nops, (void *)&ctr};
+    EBH_HIP(hipLaunchKernel(b->kern, dim3(b->grid), dim3(256), args, b->lds, s));
+    hipLaunchKernelGGL(k_reduce_multi, dim3((b->max_red + 255) / 256, b->nops),
+                       dim3(256), 0, s, md);
+    EBH_HIP(hipGetLastError());
+  }
+  if (b->sitems > 0) {
+    const int *soffp = b->soff.p;
+    int *sctr = b->counters.p + 8;
+    void *args[] = {(void *)&descs, (void *)&soffp, (void *)&nops, (void *)&sctr};
+    EBH_HIP(hipLaunchKernel(b->skern, dim3(b->sgrid), dim3(256), args, 0, s));
+    hipLaunchKernelGGL(k_reduce_wide_multi, dim3((b->max_red + 31) / 32, b->nops),
+                       dim3(256), 0, s, md);
+    EBH_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_finish_multi, dim3((b->max_rows + 255) / 256, b->nops),
                      dim3(256), 0, s, md, p);
   EBH_HIP(hipGetLastError());
@@ -1837,8 +1950,8 @@ int ebh_batch_info(const ebh_batch *b, int64_t *info)
 {
   if (!b || !info) { set_error("null argument"); return EBH_ERR_ARG; }
   info[0] = b->nops;
-  info[1] = b->items;
-  info[2] = b->grid;
+  info[1] = b->items + b->sitems;
+  info[2] = b->grid + b->sgrid;
   info[3] = (int64_t)b->lds;
   return EBH_OK;
 }

@@ -212,16 +212,18 @@ int ebh_evaluate_laplace_host(ebh_mesh *mesh, int tri_rule, int far_precision,
 /* No counterpart in the reference, which keeps one mesh at a time
  * (initGPU, src/BEM3dLaplaceGPU.cu:830-921).  Operators of one kernel,
  * far-field precision and rule size on different meshes, applied together:
- * per-operator fold kernels, ONE persistent far-field launch whose
- * workgroups take (operator, 256-triangle target block, source chunk)
- * items from 8 XCD-affine atomic queues until all are empty, then the
- * per-operator reductions (one launch each for all members).  Creating a
- * batch (at most 64 distinct operators) sets every member's launch shape:
- * 256-triangle target blocks, 64-triangle tiles and one source-chunk size
- * for all members, sized to about 8 items per resident workgroup; each
- * y_k then equals ebh_op_apply(ops[k], x_k) bit for bit.  The operators
- * must not be applied elsewhere while a batch apply runs; a launch shape
- * changed later is picked up at the next apply. */
+ * per-operator fold kernels, one persistent far-field launch per schedule,
+ * then the per-operator reductions (one launch each for all members).
+ * Symmetric members: their workgroups are items of one atomic queue.
+ * Chunked members: (operator, 256-triangle target block, source chunk)
+ * items from 8 XCD-affine atomic queues.  Workgroups take items until the
+ * queues are empty.  Creating a batch (at most 64 distinct operators) sets
+ * every member's schedule: the symmetric one with one step range where it
+ * applies, else 256-triangle target blocks, 64-triangle tiles and one
+ * source-chunk size for all chunked members, sized to about 8 items per
+ * resident workgroup; each y_k then equals ebh_op_apply(ops[k], x_k) bit
+ * for bit.  The operators must not be applied elsewhere while a batch apply
+ * runs; a schedule changed later is picked up at the next apply. */
 typedef struct ebh_batch ebh_batch;
 int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out);
 /* x[k], y[k]: device vectors of ops[k]; stream: hipStream_t or NULL (the
