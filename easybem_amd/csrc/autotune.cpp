@@ -161,6 +161,7 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
   // the starting configuration, then the chunked search from its shape (or
   // from the default chunked shape when it starts symmetric)
   double ms0 = 0.0;
+  EBH_TRY(time_apply(op, x.p, y.p, false, 1, &ms0));  // warm-up, not kept
   EBH_TRY(time_apply(op, x.p, y.p, false, reps, &ms0));
   spent += ms0 * reps;
   tried++;
@@ -194,7 +195,8 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
     }
   }
   // pair-symmetric schedule over its step split (stops after two slower
-  // splits in a row)
+  // splits in a row).  A larger split must win by 2%: each one adds a full
+  // set of partial sums to write and reduce.
   int bs = 0;
   if (start_sym && ms0 < best) { best = ms0; bs = start_ks; }
   if (sym_eligible(op)) {
@@ -207,8 +209,13 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
       EBH_TRY(time_apply(op, x.p, y.p, false, reps, &ms));
       spent += ms * reps;
       tried++;
-      if (ms < sbest) { sbest = ms; worse = 0; if (ms < best) { best = ms; bs = ks; } }
-      else worse++;
+      if (ms < 0.98 * sbest) {
+        sbest = ms;
+        worse = 0;
+        if (ms < best) { best = ms; bs = ks; }
+      } else {
+        worse++;
+      }
     }
   }
   if (bs) EBH_TRY(op_configure(op, 0, 0, bs, true));
