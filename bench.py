@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
+PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (datasheet)
 FLOP_SL = 11               # per far interaction, single layer, one RHS
 # HBM bytes per launch of the FP32 single-layer far kernel on the level-5
 # icosphere with the packaged schedule (pair-symmetric, one step range): two
@@ -43,6 +44,19 @@ TRAFFIC_SCHEDULE = ("symmetric", 1)
 # source side, against 2 x 11 algorithmic FLOP (pairs inside one 64-triangle
 # group, 1/ng of the work, take one direction only).
 SYM_EXECUTED_PER_ALGORITHMIC = 13.0 / 22.0
+
+
+def near_pass_bytes(w, ne):
+    """Algorithmic HBM bytes of one near-field pass (k_finish) of a P1 x P1
+    scalar operator: the near CSR (8 B value + 4 B column per entry, 4 B
+    row pointers), x and y once, and the far sums gathered from the 3 ne
+    (triangle, local node) entries of the node lists (4 + 4 B indices, 8 B
+    value) with their row pointers."""
+    rows, cols, nnz = w["rows"], w["cols"], w["near_nnz"]
+    csr = 12 * nnz + 4 * (rows + 1)
+    vectors = 8 * cols + 8 * rows
+    gather = 16 * 3 * ne + 4 * (rows + 1)
+    return csr + vectors + gather
 
 
 def log(*a):
@@ -395,12 +409,15 @@ def main():
     w = V32.work()
     t32 = V32.time(xv, yv, reps=5)
     t64 = V64.time(xv, yv, reps=3)
-    # far-kernel launch time with HIP events on the operator's stream
+    # far-kernel and near-field-pass launch times with HIP events on the
+    # operator's stream
     L.op_profile(V32, True, 64)
     for _ in range(10):
         V32.apply(xv, yv)
+    near_ms, near_launches = L.op_profile_near(V32)
     far_ms, far_launches = L.op_profile(V32, False)
     far_flop = FLOP_SL * w["far_interactions"]
+    near_bytes = near_pass_bytes(w, int(T.shape[0]))
 
     unpc = {}
     if pc is not None and not args.no_fp64_compare:
@@ -485,6 +502,15 @@ def main():
                          "executed_tflops":
                          achieved * SYM_EXECUTED_PER_ALGORITHMIC} if sym32 else {}),
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
+        "near_field": {"kernel": "k_finish (near CSR x + far gather, FP64)",
+                       "bound": "hbm",
+                       "achieved": near_bytes / (near_ms / near_launches * 1e-3) / 1e9,
+                       "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                       "frac": near_bytes / (near_ms / near_launches * 1e-3) / 1e9
+                       / PEAK_HBM_GBPS,
+                       "bytes_per_launch": near_bytes,
+                       "avg_launch_ms": near_ms / near_launches,
+                       "launches": near_launches},
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(P, T, st64.get("matvec64", 1))

@@ -507,6 +507,17 @@ def op_profile(op: Operator, enable: bool, max_launches=4096):
     return ms.value, n.value
 
 
+def op_profile_near(op: Operator):
+    """Near-field pass timing recorded since op_profile(op, True):
+    (total_ms, launches)."""
+    L = lib()
+    L.ebh_op_profile_near.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64)]
+    ms, n = C.c_double(), C.c_int64()
+    check(L.ebh_op_profile_near(op.h, C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
 def mesh_check(nodes, elements):
     """Host-only validation and classification (no GPU needed)."""
     L = lib()

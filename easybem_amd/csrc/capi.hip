@@ -465,14 +465,32 @@ int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
   }
   if (enable) {
     if (max_launches < 1) max_launches = 4096;
-    while ((int)op->prof_ev.size() < 2 * max_launches) {
-      hipEvent_t e;
-      EBH_HIP(hipEventCreate(&e));
-      op->prof_ev.push_back(e);
-    }
+    for (auto *v : {&op->prof_ev, &op->prof_nev})
+      while ((int)v->size() < 2 * max_launches) {
+        hipEvent_t e;
+        EBH_HIP(hipEventCreate(&e));
+        v->push_back(e);
+      }
   }
   op->prof_on = enable != 0;
   op->prof_count = 0;
+  op->prof_ncount = 0;
+  return EBH_OK;
+}
+
+int ebh_op_profile_near(ebh_op *op, double *near_ms, int64_t *launches)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  ebh_op *f = far_op(op);
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  double tot = 0.0;
+  for (int k = 0; k < f->prof_ncount; k++) {
+    float ms = 0.f;
+    EBH_HIP(hipEventElapsedTime(&ms, f->prof_nev[2 * k], f->prof_nev[2 * k + 1]));
+    tot += ms;
+  }
+  if (near_ms) *near_ms = tot;
+  if (launches) *launches = f->prof_ncount;
   return EBH_OK;
 }
 
@@ -487,6 +505,7 @@ void ebh_op_destroy(ebh_op *op)
   for (auto &e : op->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : op->prof_ev) (void)hipEventDestroy(e);
+  for (auto &e : op->prof_nev) (void)hipEventDestroy(e);
   if (op->stream) (void)hipStreamDestroy(op->stream);
   delete op;
 }

