@@ -19,7 +19,12 @@ namespace {
 int inner_three(ebh_op *v)
 {
   v->nr_max = 3;
-  return op_configure(v, v->block, v->tile, v->sym ? v->ksplit : v->chunks, v->sym);
+  if (!v->sym) return op_configure(v, v->block, v->tile, v->chunks);
+  // three right-hand sides per symmetric launch triple its partial sums:
+  // keep the split if they fit, else one step range, else chunked
+  if (op_configure(v, 0, 0, v->ksplit, true) == EBH_OK) return EBH_OK;
+  if (op_configure(v, 0, 0, 1, true) == EBH_OK) return EBH_OK;
+  return op_configure(v, 0, 0, 0);
 }
 }  // namespace
 

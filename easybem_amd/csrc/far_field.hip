@@ -1227,14 +1227,16 @@ int sym_default_split(int ng)
 }
 
 // The symmetric schedule's partial sums take (ne / 64) x nout x ne FP64
-// values per step range, quadratic in the mesh size; above this bound
-// (ne = 95,000 with P1 test functions) the chunked schedule is used.
+// values per step range and right-hand side of a launch, quadratic in the
+// mesh size; above this bound (ne = 95,000 with P1 test functions and one
+// right-hand side) the chunked schedule is used.
 constexpr double kSymPartialMax = 4.0 * 1024 * 1024 * 1024;
 
 bool sym_eligible(const ebh_op *op)
 {
   const double ne = op->mesh->T.ne;
-  const double bytes = std::ceil(ne / 64.0) * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+  const double bytes =
+      std::ceil(ne / 64.0) * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
   return op->d.kernel == EBH_LAPLACE_SL && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
          op->d.test_basis == op->d.trial_basis && bytes <= kSymPartialMax;
 }
@@ -1258,7 +1260,7 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
       set_error("pair-symmetric step split must be a power of 2 up to 64");
       return EBH_ERR_ARG;
     }
-    const double bytes = (double)ng * ks * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+    const double bytes = (double)ng * ks * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
     if (bytes > kSymPartialMax) {
       set_error("pair-symmetric split " + std::to_string(ks) +
                 " needs more than 4 GiB of partial sums on this mesh");

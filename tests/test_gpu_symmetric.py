@@ -188,3 +188,21 @@ def test_symmetric_three_rhs_in_composites(L, kernel, test, prec):
     err = rel(op(x), y0)
     print(kernel, prec, "rel err", err)
     assert err <= (1e-13 if prec == "fp64" else 2e-6)
+
+
+def test_symmetric_full_size_l5(L):
+    """The bench mesh (20480 triangles, 320 groups): symmetric against
+    chunked in FP64, and the FP32 symmetric apply against FP64."""
+    P, T = meshes.icosphere(5)
+    m = L.Mesh(P, T)
+    x = _x(1, P, T, 21)
+    op = L.Operator(m, "V", 1, 1, "fp64")
+    op.set_schedule("chunked")
+    y0 = op(x)
+    op.set_schedule("symmetric", 1)
+    y1 = op(x)
+    op32 = L.Operator(m, "V", 1, 1, "fp32")
+    op32.set_schedule("symmetric", 1)
+    e64, e32 = rel(y1, y0), rel(op32(x), y0)
+    print("fp64 symmetric vs chunked", e64, "fp32 symmetric vs fp64", e32)
+    assert e64 <= 1e-14 and e32 <= 1e-6
