@@ -25,6 +25,10 @@
 // Block b works on chunk b % chunks, so with chunks a multiple of 8 every
 // block of one source chunk runs on the same XCD (blocks are dealt to the 8
 // XCDs round-robin).
+//
+// The single layer with equal test and trial bases also has a pair-symmetric
+// schedule (k_far_sym, below): one rsqrt per point pair for both directions,
+// source triangles rotating through the lanes of a wave.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -371,14 +375,17 @@ __global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict_
 // from lane L + 1), so after 64 steps lane L holds the sums of source
 // triangle 64 J + L.  Pairs inside one group (I == J) take the target side
 // only.  Contributions to the targets of group I from the sources of group
-// J go to partial[J][.][t] (target side), those to the targets of group J
-// from group I to partial[I][.][t] (source side): each slot of the
-// [ng][nout][ne] buffer is written exactly once, and k_reduce_wide adds the ng
-// slots of a target in a fixed order.  A workgroup is 4 waves, target groups
+// J go to partial slot J (target side), those to the targets of group J
+// from group I to slot I (source side): each slot of a target is written
+// exactly once, and k_reduce_wide adds the ng slots of a target in a fixed
+// order.  A workgroup is 4 waves, target groups
 // 4M .. 4M+3, with source group J >= 4M staged in LDS; a wave with I > J has
 // no work (its pair is the source side of the (J, I) wave).  For small
 // meshes the 64 steps split into ksplit ranges, one workgroup each, with
-// ksplit partial slots per group (slot J ksplit + range).
+// ksplit partial slots per group (slot J ksplit + range); a launch with NR
+// right-hand sides keeps NR of them per slot, so the buffer is
+// [ng ksplit][NR][nout][ne].
+
 // value of lane L + 1 (mod 64) in lane L: DPP wave_rol:1 (control 0x134),
 // every lane of the wave active (the rotation reads no out-of-wave lane, so
 // the "old" operand is never used; passing the value itself avoids a copy)
