@@ -144,6 +144,11 @@ def secondary(L, meshes, solve_batched=False):
                     / tm["far"] / 1e9}
     if solve_batched:
         s = L.Solver(ops["fp64"], ops["fp32"], restart=50, batch=64)
+        # near-field sparse approximate inverse on 2 triangle rings
+        t0 = time.perf_counter()
+        S = L.precond_spai(ops["fp64"], rings=2, max_rhs=64)
+        r4["spai_build_s"] = time.perf_counter() - t0
+        s.set_precond(S, S)
         B = rng.uniform(-1, 1, 64 * ops["fp64"].ncols)
         db = L.DeviceArray.from_host(B)
         dx = L.DeviceArray.from_host(np.zeros_like(B))
@@ -261,9 +266,10 @@ def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
                     "iters_max": int(max(r["iters"] for r in ok
                                          if r["level"] == lev and r["problem"] == pb))}
     return {"requests": n_requests, "completed": len(ok), "levels": list(levels),
-            "formulation": "Dirichlet: P1 flux, V with the Calderon preconditioner; "
-                           "Neumann: P1 trace, D + m m^T with the dual Calderon "
-                           "preconditioner; mixed-precision refinement to 1e-8",
+            "formulation": "Dirichlet: P1 flux, V with its near-field sparse "
+                           "approximate inverse (3 node rings); Neumann: P1 trace, "
+                           "D + m m^T with the dual Calderon preconditioner; "
+                           "mixed-precision refinement to 1e-8",
             "arrival_rate_per_s": rate, "slots": slots,
             "solves_per_s": len(ok) / wall,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -311,10 +317,10 @@ def main():
                     help="serving stream of N requests on icosphere levels 4-6 "
                          "(the default run serves 24 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--precond", choices=("calderon", "none"), default="calderon",
+    ap.add_argument("--precond", choices=("spai", "calderon", "none"), default="spai",
                     help="right preconditioner of the GMRES solves (default: the "
-                         "Calderon preconditioner, hypersingular operator with a "
-                         "1-point far rule)")
+                         "near-field sparse approximate inverse of V; calderon: the "
+                         "hypersingular operator with a 1-point far rule)")
     ap.add_argument("--retune", action="store_true",
                     help="run the launch-shape autotuner on the bench operators "
                          "instead of using the packaged table")
@@ -369,7 +375,17 @@ def main():
     zeros = np.zeros(b.size)
     solver = L.Solver(V64, V32, restart=50)
     pc = None
-    if args.precond == "calderon":
+    if args.precond == "spai":
+        # rows of the inverse of V on 3-ring node patches (about 37 entries
+        # per row), applied in FP64 for both operators
+        S = L.precond_spai(V64, rings=3)
+        solver.set_precond(S, S)
+        pc = ("near-field sparse approximate inverse, right: row c of M is row c "
+              "of the inverse of V on the nodes within three edges of node c "
+              "(V's entries there exact: near-field Sauter-Schwab plus far "
+              "quadrature), applied in FP64; the residual is that of the "
+              "unpreconditioned system")
+    elif args.precond == "calderon":
         # M_L^-1 (D + m m^T / (4|Gamma|)) M_L^-1: D through curls, 3-point
         # Gauss near field, 1-point far rule; FP64 for the FP64 operator
         P64 = L.precond_calderon(mesh, "fp64", line_rule=2, tri_rule=0)
@@ -445,9 +461,11 @@ def main():
         out_fp64 = {"fp64_solves_per_s": 1.0 / t64s,
                     "fp64_iters": st64["iters"],
                     "mixed_speedup_vs_fp64": solves_per_s * t64s}
-        # CPU-equivalent operator applies of the FP64 solve: a preconditioner
-        # apply is 3 single-layer applies with 1 of the 49 far interactions
-        st64["matvec64"] += st64.get("precond", 0) * 3.0 / 49.0
+        # CPU-equivalent operator applies of the FP64 solve: a Calderon
+        # preconditioner apply is 3 single-layer applies with 1 of the 49 far
+        # interactions (the sparse inverse is negligible)
+        if args.precond == "calderon":
+            st64["matvec64"] += st64.get("precond", 0) * 3.0 / 49.0
     else:
         st64 = {"matvec64": stats[-1]["iters"]}
 

@@ -384,6 +384,26 @@ def precond_calderon(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0,
     return op
 
 
+def precond_spai(V: Operator, rings=None, max_rhs=1):
+    """Near-field sparse approximate inverse of the single layer V
+    (ebh_precond_spai) on patches of `rings` adjacency steps (default: 3
+    for P1, 2 for P0), as an Operator applying y = M x in FP64; use it as
+    both preconditioners of a solver."""
+    L = lib()
+    L.ebh_precond_spai.argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                   C.POINTER(C.c_void_p)]
+    if rings is None:
+        rings = 3 if V.nrows == V.mesh.nodes.shape[0] else 2
+    h = C.c_void_p()
+    check(L.ebh_precond_spai(V.h, int(rings), int(max_rhs), C.byref(h)))
+    op = Operator.__new__(Operator)
+    op.mesh = V.mesh
+    op.desc = None
+    op.h = h
+    op.nrows = op.ncols = V.nrows
+    return op
+
+
 def precond_calderon_hs(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0):
     """Dual Calderon preconditioner M_L^-1 V M_L^-1 for the P1
     hypersingular operator (ebh_precond_calderon_hs), as an Operator."""

@@ -30,7 +30,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
 LIB_SOURCES = ["host_geometry.cpp", "near_field.hip", "far_field.hip",
                "capi.hip", "solver.hip", "legacy_api.cpp", "server.cpp",
                "autotune.cpp", "hypersingular.hip", "evaluate.hip",
-               "dense.hip", "elasticity.hip"]
+               "dense.hip", "elasticity.hip", "spai.hip"]
 
 
 def _run(cmd):

@@ -126,6 +126,10 @@ extern "C" {
 int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
 {
   if (!op || !out) { set_error("null argument"); return EBH_ERR_ARG; }
+  if (op->d.kernel == EBH_SPARSE_INVERSE) {
+    set_error("a sparse-inverse operator has no far field to tune");
+    return EBH_ERR_UNSUPPORTED;
+  }
   if (op->inner) {
     EBH_HIP(hipStreamSynchronize(op->stream));
     return ebh_op_autotune(op->inner, budget_ms, out);

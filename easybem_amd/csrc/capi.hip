@@ -362,9 +362,19 @@ int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs)
   return EBH_OK;
 }
 
+namespace {
+bool no_far(const ebh_op *op)
+{
+  if (op->d.kernel != EBH_SPARSE_INVERSE) return false;
+  set_error("a sparse-inverse operator has no far field");
+  return true;
+}
+}  // namespace
+
 int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (no_far(op)) return EBH_ERR_UNSUPPORTED;
   EBH_HIP(hipStreamSynchronize(op->stream));
   return op_configure(far_op(op), block, tile, chunks);
 }
@@ -372,6 +382,7 @@ int ebh_op_set_launch(ebh_op *op, int block, int tile, int chunks)
 int ebh_op_set_schedule(ebh_op *op, int schedule, int split)
 {
   if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (no_far(op)) return EBH_ERR_UNSUPPORTED;
   if (schedule != EBH_SCHED_CHUNKED && schedule != EBH_SCHED_SYMMETRIC) {
     set_error("schedule must be EBH_SCHED_CHUNKED or EBH_SCHED_SYMMETRIC");
     return EBH_ERR_ARG;
@@ -440,7 +451,8 @@ int ebh_op_work(const ebh_op *op, int64_t *work)
   // Kelvin; elasticity D 9 V + Kelvin)
   const int64_t mult = op->d.kernel == EBH_LAPLACE_HS ? 3
                        : op->d.kernel == EBH_KELVIN_DL ? 7
-                       : op->d.kernel == EBH_KELVIN_HS ? 10 : 1;
+                       : op->d.kernel == EBH_KELVIN_HS ? 10
+                       : op->d.kernel == EBH_SPARSE_INVERSE ? 0 : 1;
   work[0] = op->mesh->T.n_class[kDisjoint] * nt * nt * mult;
   work[1] = far_op(op)->near_nnz;
   work[2] = op->nrows;

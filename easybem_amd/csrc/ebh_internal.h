@@ -175,6 +175,8 @@ int op_build_far(ebh_op *op);
 // the mesh); only for operators that sym_eligible accepts
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
+// y = M x for an EBH_SPARSE_INVERSE operator (nrhs vectors back to back)
+int spai_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s, bool timed);
 int sym_default_split(int ng);
 int hs_build(ebh_op *op);
 int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,

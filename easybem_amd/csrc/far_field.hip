@@ -1496,6 +1496,7 @@ __global__ void k_scale_vec(int n, const double *__restrict__ d, const double *x
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
+  if (op->d.kernel == EBH_SPARSE_INVERSE) return spai_apply(op, x, y, 1, s, timed);
   if (op->inner) return composite_apply(op, x, y, 1, s, timed);
   if (op->pscale) {
     hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
@@ -1568,6 +1569,7 @@ static int op_apply_three(ebh_op *op, const double *x, double *y, hipStream_t s)
 int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
                    hipStream_t s, bool timed)
 {
+  if (op->d.kernel == EBH_SPARSE_INVERSE) return spai_apply(op, x, y, nrhs, s, timed);
   if (op->inner) return composite_apply(op, x, y, nrhs, s, timed);
   if (nrhs == 1 || !op->rb || op->pscale) {
     const size_t cx = (size_t)op->ncols * op->nc, cy = (size_t)op->nrows * op->nc;

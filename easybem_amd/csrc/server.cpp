@@ -211,15 +211,16 @@ struct ebh_server {
     d.far_precision = EBH_FP64;
     EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
     EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
-    if (p1 && !o.no_precond) {
-      // Calderon preconditioners with a 1-point far rule: D for V, V for D
-      auto make = [&](int prec, ebh_op **out) {
-        return neu ? ebh_precond_calderon_hs(e->mesh, prec, 2, 0, out)
-                   : ebh_precond_calderon(e->mesh, prec, 2, 0, 1, out);
-      };
-      EBH_TRY(make(EBH_FP64, &e->P64));
-      if (o.mixed) EBH_TRY(make(EBH_FP32, &e->P32));
+    if (neu && !o.no_precond) {
+      // dual Calderon preconditioners with a 1-point far rule: V for D
+      EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP64, 2, 0, &e->P64));
+      if (o.mixed) EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP32, 2, 0, &e->P32));
       EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
+    } else if (!o.no_precond) {
+      // near-field sparse approximate inverse of V (3 node rings for P1, 2
+      // triangle rings for P0), one FP64 operator for both precisions
+      EBH_TRY(ebh_precond_spai(e->A64, p1 ? 3 : 2, 1, &e->P64));
+      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P64));
     }
     // eager launches: stream capture must not overlap other threads'
     // allocations and copies, which the other workers make at any time

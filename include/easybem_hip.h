@@ -61,9 +61,11 @@ enum {
  * getElasticity3d (src/BEM3dElasticity.cpp:760-943), applied through the
  * Laplace single and double layer and the Kelvin single layer (E = 1,
  * Poisson ratio nu). */
+/* EBH_SPARSE_INVERSE: the kind of the operators ebh_precond_spai returns
+ * (a sparse matrix applied on the device); ebh_op_create does not take it. */
 enum { EBH_LAPLACE_SL = 0, EBH_LAPLACE_DL = 1, EBH_KELVIN_SL = 2,
        EBH_LAPLACE_HS = 3, EBH_LAPLACE_ADL = 4, EBH_KELVIN_DL = 5,
-       EBH_KELVIN_HS = 6 };
+       EBH_KELVIN_HS = 6, EBH_SPARSE_INVERSE = 7 };
 enum { EBH_P0 = 0, EBH_P1 = 1 };
 enum { EBH_FP64 = 0, EBH_FP32 = 1 };
 
@@ -340,6 +342,18 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32);
  * far rule (tri_rule 0, one point per triangle) suffices. */
 int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
                          int tri_rule, int max_rhs, ebh_op **out);
+/* Near-field sparse approximate inverse of a single layer V (P0 x P0 or
+ * P1 x P1, either precision; no counterpart in the reference): row c of M
+ * is row c of the inverse of V restricted to the patch of DOFs within
+ * `rings` (1..4) adjacency steps of DOF c -- P1 nodes are adjacent when they
+ * share a triangle, P0 triangles when they share a vertex -- with V's
+ * entries on the patch computed exactly (near-field CSR plus far
+ * quadrature).  Returns a square EBH_SPARSE_INVERSE operator applying
+ * y = M x in FP64, for ebh_solver_set_precond (as both the FP64 and the
+ * FP32 preconditioner).  Build cost: one far-quadrature pass over the pairs
+ * within 2 rings steps on the device and one dense solve per DOF on the
+ * host. */
+int ebh_precond_spai(ebh_op *V, int rings, int max_rhs, ebh_op **out);
 /* The dual Calderon preconditioner, for the (stabilised) hypersingular
  * operator on P1: P = M_L^-1 V M_L^-1 with V the P1 single layer at the
  * given rules (no counterpart in the reference). */
@@ -380,11 +394,11 @@ typedef struct {
   int line_rule;       /* near-field rule index (default 4)                  */
   int tri_rule;        /* far-field rule index (default 4)                   */
   int max_iters;       /* per request (default 2000)                         */
-  int dirichlet_p1;    /* 1: Dirichlet flux in P1 (np entries) with the
-                          Calderon preconditioner; 0: P0 flux (ne entries),
-                          as the reference's GPU example                     */
-  int no_precond;      /* 1: no Calderon preconditioners (P1 Dirichlet and
-                          Neumann solves use them by default)                */
+  int dirichlet_p1;    /* 1: Dirichlet flux in P1 (np entries); 0: P0 flux
+                          (ne entries), as the reference's GPU example       */
+  int no_precond;      /* 1: no preconditioners (by default Dirichlet solves
+                          use the sparse approximate inverse of V, Neumann
+                          solves the dual Calderon preconditioner)           */
 } ebh_server_opts;
 
 typedef struct {
