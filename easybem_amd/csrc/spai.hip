@@ -16,6 +16,7 @@
 // matrix-vector product in FP64, 8 lanes per row.
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "ebh_internal.h"
@@ -130,14 +131,10 @@ inline long find_entry(const std::vector<int> &ptr, const std::vector<int> &col,
   return (it != last && *it == b) ? (long)(it - col.begin()) : -1;
 }
 
-// Solves A^T x = rhs in place (A m x m, row-major) by Gaussian elimination
-// with partial pivoting; false if a pivot vanishes.
-bool solve_transposed(int m, std::vector<double> &A, std::vector<double> &rhs)
+// Solves B x = rhs in place (B m x m, row-major, overwritten) by Gaussian
+// elimination with partial pivoting; false if a pivot vanishes.
+bool solve_inplace(int m, double *B, double *rhs)
 {
-  // work on B = A^T
-  std::vector<double> B((size_t)m * m);
-  for (int r = 0; r < m; r++)
-    for (int c = 0; c < m; c++) B[(size_t)r * m + c] = A[(size_t)c * m + r];
   for (int k = 0; k < m; k++) {
     int piv = k;
     for (int r = k + 1; r < m; r++)
@@ -291,35 +288,51 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
       }
   }
 
-  // rows of M: patch inverses
-  std::vector<int> mptr(n + 1, 0), mcol;
-  std::vector<double> mval;
-  std::vector<double> Aloc, rhs;
-  for (int c = 0; c < n; c++) {
-    const std::vector<int> &p = patch[c];
-    const int m = (int)p.size();
-    Aloc.assign((size_t)m * m, 0.0);
-    rhs.assign(m, 0.0);
-    for (int r = 0; r < m; r++) {
-      if (p[r] == c) rhs[r] = 1.0;
-      for (int s = 0; s < m; s++) {
-        const long k = find_entry(eptr, ecol, p[r], p[s]);
-        Aloc[(size_t)r * m + s] = k >= 0 ? ev[k] : 0.0;
+  // rows of M: patch inverses, rows split over host threads (each row is
+  // independent; the result does not depend on the split)
+  std::vector<int> mptr(n + 1, 0);
+  for (int c = 0; c < n; c++) mptr[c + 1] = mptr[c] + (int)patch[c].size();
+  std::vector<int> mcol(mptr[n]);
+  std::vector<double> mval(mptr[n]);
+  const int nthreads =
+      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<int> bad(nthreads, -1);
+  auto rows_of = [&](int tid) {
+    std::vector<double> B, rhs;
+    for (int c = tid; c < n; c += nthreads) {
+      const std::vector<int> &p = patch[c];
+      const int m = (int)p.size();
+      // B = transpose of V restricted to the patch
+      B.assign((size_t)m * m, 0.0);
+      rhs.assign(m, 0.0);
+      for (int r = 0; r < m; r++) {
+        if (p[r] == c) rhs[r] = 1.0;
+        for (int q = 0; q < m; q++) {
+          const long k = find_entry(eptr, ecol, p[r], p[q]);
+          B[(size_t)q * m + r] = k >= 0 ? ev[k] : 0.0;
+        }
       }
+      bool ok = solve_inplace(m, B.data(), rhs.data());
+      for (int r = 0; r < m && ok; r++) ok = std::isfinite(rhs[r]);
+      if (!ok) {
+        if (bad[tid] < 0) bad[tid] = c;
+        continue;
+      }
+      std::copy(p.begin(), p.end(), mcol.begin() + mptr[c]);
+      std::copy(rhs.begin(), rhs.end(), mval.begin() + mptr[c]);
     }
-    if (!solve_transposed(m, Aloc, rhs)) {
-      set_error("singular patch matrix at DOF " + std::to_string(c));
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(rows_of, t);
+    rows_of(0);
+    for (auto &th : pool) th.join();
+  }
+  for (int t = 0; t < nthreads; t++)
+    if (bad[t] >= 0) {
+      set_error("singular or non-finite patch inverse at DOF " + std::to_string(bad[t]));
       return EBH_ERR_NONFINITE;
     }
-    for (int r = 0; r < m; r++)
-      if (!std::isfinite(rhs[r])) {
-        set_error("non-finite patch inverse at DOF " + std::to_string(c));
-        return EBH_ERR_NONFINITE;
-      }
-    mcol.insert(mcol.end(), p.begin(), p.end());
-    mval.insert(mval.end(), rhs.begin(), rhs.end());
-    mptr[c + 1] = (int)mcol.size();
-  }
 
   ebh_op *op = new ebh_op();
   op->mesh = A->mesh;
