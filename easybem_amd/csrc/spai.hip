@@ -1,5 +1,6 @@
-// Near-field sparse approximate inverse of the single layer, a right
-// preconditioner for GMRES on V (no counterpart in the reference).
+// Near-field sparse approximate inverse of the single layer or of the
+// hypersingular operator, a right preconditioner for GMRES (no counterpart
+// in the reference).
 //
 // Row c of M is row c of the inverse of V restricted to a patch around DOF
 // c: the DOFs within `rings` adjacency steps of c, P1 nodes being adjacent
@@ -98,6 +99,88 @@ __global__ __launch_bounds__(256) void k_spai_far(int nent, const int *__restric
   eval[e] = s * kInv4PiSpai;
 }
 
+// Entries of the hypersingular operator D = sum_c C_c^T V0 C_c (V0 the P0
+// single layer, C_c the surface-curl components of the hat functions) for
+// the pattern entries (a, b) of P1 nodes:
+//   D_ab = sum_{i in supp a} sum_{j in supp b} (curl_i phi_a . curl_j phi_b) V0_ij,
+// V0_ij from V0's near-field CSR for touching triangles, else from the far
+// rule.  One thread per entry.
+__global__ __launch_bounds__(256) void k_spai_hs(int nent, const int *__restrict__ erow,
+                                                 const int *__restrict__ ecol, RuleArgs rule,
+                                                 const double *__restrict__ nodes,
+                                                 const int *__restrict__ tri,
+                                                 const int *__restrict__ ntp,
+                                                 const int *__restrict__ ntri,
+                                                 const int *__restrict__ ntk,
+                                                 const int *__restrict__ touch_ptr,
+                                                 const int *__restrict__ touch,
+                                                 const double *__restrict__ curl,
+                                                 const int *__restrict__ vptr,
+                                                 const int *__restrict__ vcol,
+                                                 const double *__restrict__ vval,
+                                                 double *__restrict__ eval)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int a = erow[e], b = ecol[e];
+  const int nt = rule.nt;
+  double s = 0.0;
+  for (int ia = ntp[a]; ia < ntp[a + 1]; ia++) {
+    const int i = ntri[ia], ki = ntk[ia];
+    double ci[3];
+    for (int c = 0; c < 3; c++) ci[c] = curl[(3 * (size_t)i + c) * 3 + ki];
+    double X[16][3], W[16];
+    {
+      const double *P0 = nodes + 3 * tri[3 * i], *P1 = nodes + 3 * tri[3 * i + 1],
+                   *P2 = nodes + 3 * tri[3 * i + 2];
+      const double u[3] = {P1[0] - P0[0], P1[1] - P0[1], P1[2] - P0[2]};
+      const double v[3] = {P2[0] - P0[0], P2[1] - P0[1], P2[2] - P0[2]};
+      const double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2],
+                   n2 = u[0] * v[1] - u[1] * v[0];
+      const double J = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      for (int q = 0; q < nt; q++) {
+        const double x1 = rule.x[q], x2 = rule.y[q];
+        for (int d = 0; d < 3; d++) X[q][d] = P0[d] + x1 * u[d] + x2 * v[d];
+        W[q] = rule.w[q] * J;
+      }
+    }
+    for (int ib = ntp[b]; ib < ntp[b + 1]; ib++) {
+      const int j = ntri[ib], kj = ntk[ib];
+      double dot = 0.0;
+      for (int c = 0; c < 3; c++) dot += ci[c] * curl[(3 * (size_t)j + c) * 3 + kj];
+      bool touching = false;
+      for (int c = touch_ptr[i]; c < touch_ptr[i + 1]; c++) touching |= touch[c] == j;
+      double vij = 0.0;
+      if (touching) {
+        for (int k = vptr[i]; k < vptr[i + 1]; k++)
+          if (vcol[k] == j) vij = vval[k];
+      } else {
+        const double *P0 = nodes + 3 * tri[3 * j], *P1 = nodes + 3 * tri[3 * j + 1],
+                     *P2 = nodes + 3 * tri[3 * j + 2];
+        const double u[3] = {P1[0] - P0[0], P1[1] - P0[1], P1[2] - P0[2]};
+        const double v[3] = {P2[0] - P0[0], P2[1] - P0[1], P2[2] - P0[2]};
+        const double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2],
+                     n2 = u[0] * v[1] - u[1] * v[0];
+        const double J = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        for (int p = 0; p < nt; p++) {
+          const double y1 = rule.x[p], y2 = rule.y[p];
+          const double Y[3] = {P0[0] + y1 * u[0] + y2 * v[0], P0[1] + y1 * u[1] + y2 * v[1],
+                               P0[2] + y1 * u[2] + y2 * v[2]};
+          double t = 0.0;
+          for (int q = 0; q < nt; q++) {
+            const double dx = X[q][0] - Y[0], dy = X[q][1] - Y[1], dz = X[q][2] - Y[2];
+            t += W[q] / sqrt(dx * dx + dy * dy + dz * dz);
+          }
+          vij += rule.w[p] * J * t;
+        }
+        vij *= kInv4PiSpai;
+      }
+      s += dot * vij;
+    }
+  }
+  eval[e] = s;
+}
+
 __device__ __forceinline__ double spai_row_sum(double v)
 {
 #pragma unroll
@@ -190,10 +273,13 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
     set_error("patch radius must be 1..4 adjacency steps");
     return EBH_ERR_ARG;
   }
-  if (A->inner || A->d.kernel != EBH_LAPLACE_SL || A->d.test_basis != A->d.trial_basis ||
-      A->nc != 1) {
+  // the single layer with equal bases, or the hypersingular operator
+  // D + stab m m^T (not its Calderon-scaled form)
+  const bool hs = A->d.kernel == EBH_LAPLACE_HS && A->inner && !A->hs_scale;
+  if (!hs && (A->inner || A->d.kernel != EBH_LAPLACE_SL ||
+              A->d.test_basis != A->d.trial_basis || A->nc != 1)) {
     set_error("the sparse approximate inverse needs a Laplace single layer with equal "
-              "test and trial bases");
+              "test and trial bases or the hypersingular operator");
     return EBH_ERR_UNSUPPORTED;
   }
   EBH_HIP(hipSetDevice(A->mesh->device));
@@ -254,9 +340,32 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
   }
   const int nent = (int)ecol.size();
 
-  // far parts on the device
+  // entries on the device: the hypersingular operator's whole, or the
+  // single layer's far part
   std::vector<double> ev(nent, 0.0);
-  {
+  if (hs) {
+    const ebh_op *V0 = A->inner;
+    DevBuf<int> drow, dcol;
+    DevBuf<double> dval;
+    EBH_TRY(drow.upload(erow));
+    EBH_TRY(dcol.upload(ecol));
+    EBH_TRY(dval.alloc(nent));
+    hipLaunchKernelGGL(k_spai_hs, dim3((nent + 255) / 256), dim3(256), 0, A->stream, nent,
+                       drow.p, dcol.p, V0->rule, A->mesh->nodes.p, A->mesh->tri.p,
+                       A->mesh->node_tri_ptr.p, A->mesh->node_tri.p, A->mesh->node_tri_k.p,
+                       A->mesh->touch_ptr.p, A->mesh->touch.p, A->curl.p, V0->near_ptr.p,
+                       V0->near_col.p, V0->near_val.p, dval.p);
+    EBH_HIP(hipGetLastError());
+    EBH_HIP(hipStreamSynchronize(A->stream));
+    EBH_HIP(hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost));
+    if (A->d.stab != 0.0) {
+      // + stab m_a m_b, m the lumped node areas
+      std::vector<double> m(n);
+      EBH_HIP(hipMemcpy(m.data(), A->mvec.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int a = 0; a < n; a++)
+        for (int k = eptr[a]; k < eptr[a + 1]; k++) ev[k] += A->d.stab * m[a] * m[ecol[k]];
+    }
+  } else {
     DevBuf<int> drow, dcol;
     DevBuf<double> dval;
     EBH_TRY(drow.upload(erow));
@@ -272,7 +381,7 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
     EBH_HIP(hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost));
   }
   // touching parts: V's near-field CSR
-  {
+  if (!hs) {
     std::vector<int> nptr(A->nrows + 1), ncol(A->near_nnz);
     std::vector<double> nval(A->near_nnz);
     EBH_HIP(hipMemcpy(nptr.data(), A->near_ptr.p, sizeof(int) * nptr.size(),

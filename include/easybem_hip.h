@@ -343,16 +343,22 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32);
 int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
                          int tri_rule, int max_rhs, ebh_op **out);
 /* Near-field sparse approximate inverse of a single layer V (P0 x P0 or
- * P1 x P1, either precision; no counterpart in the reference): row c of M
- * is row c of the inverse of V restricted to the patch of DOFs within
- * `rings` (1..4) adjacency steps of DOF c -- P1 nodes are adjacent when they
- * share a triangle, P0 triangles when they share a vertex -- with V's
- * entries on the patch computed exactly (near-field CSR plus far
- * quadrature).  Returns a square EBH_SPARSE_INVERSE operator applying
- * y = M x in FP64, for ebh_solver_set_precond (as both the FP64 and the
- * FP32 preconditioner).  Build cost: one far-quadrature pass over the pairs
- * within 2 rings steps on the device and one dense solve per DOF on the
- * host. */
+ * P1 x P1, either precision) or of the hypersingular operator D + stab m m^T
+ * (EBH_LAPLACE_HS without the Calderon scaling; its entries come from the
+ * curls and the wrapped P0 single layer); no counterpart in the reference.
+ * For V it beats the Calderon preconditioner; for D it helps on coarse
+ * meshes only (D's small eigenvalues belong to smooth modes, which a local
+ * inverse cannot reach), and ebh_precond_calderon_hs is the better choice
+ * from about 20000 triangles up.
+ * Row c of M is row c of the inverse of the operator restricted to the
+ * patch of DOFs within `rings` (1..4) adjacency steps of DOF c -- P1 nodes
+ * are adjacent when they share a triangle, P0 triangles when they share a
+ * vertex -- with the operator's entries on the patch computed exactly
+ * (near-field CSR plus far quadrature).  Returns a square
+ * EBH_SPARSE_INVERSE operator applying y = M x in FP64, for
+ * ebh_solver_set_precond (as both the FP64 and the FP32 preconditioner).
+ * Build cost: one quadrature pass over the pairs within 2 rings steps on
+ * the device and one dense solve per DOF on host threads. */
 int ebh_precond_spai(ebh_op *V, int rings, int max_rhs, ebh_op **out);
 /* The dual Calderon preconditioner, for the (stabilised) hypersingular
  * operator on P1: P = M_L^-1 V M_L^-1 with V the P1 single layer at the
