@@ -257,14 +257,15 @@ def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
     by_level = {}
     for pb in ("dirichlet", "neumann"):
         for lev in sorted(set(levels)):
-            v = np.array([r["total_ms"] for r in ok
-                          if r["level"] == lev and r["problem"] == pb])
-            if len(v):
+            sel = [r for r in ok if r["level"] == lev and r["problem"] == pb]
+            if sel:
+                v = np.array([r["total_ms"] for r in sel])
                 by_level[f"{pb} L{lev}"] = {
-                    "n": int(len(v)), "p50_ms": float(np.percentile(v, 50)),
+                    "n": len(sel), "p50_ms": float(np.percentile(v, 50)),
                     "max_ms": float(v.max()),
-                    "iters_max": int(max(r["iters"] for r in ok
-                                         if r["level"] == lev and r["problem"] == pb))}
+                    "setup_ms_p50": float(np.median([r["setup_ms"] for r in sel])),
+                    "solve_ms_p50": float(np.median([r["solve_ms"] for r in sel])),
+                    "iters_max": int(max(r["iters"] for r in sel))}
     return {"requests": n_requests, "completed": len(ok), "levels": list(levels),
             "formulation": "Dirichlet: P1 flux, V with its near-field sparse "
                            "approximate inverse (3 node rings); Neumann: P1 trace, "
