@@ -3,7 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "easybem_hip.h"
@@ -80,6 +84,10 @@ struct ebh_mesh {
   ebh::DevBuf<int> tri;                         // ne*3 (0-based)
   ebh::DevBuf<int> touch_ptr, touch;            // exclusion lists
   ebh::DevBuf<int> node_tri_ptr, node_tri, node_tri_k;
+  // near-field CSR patterns with their contribution lists, by (test basis,
+  // trial basis): every operator of the mesh with those bases shares one
+  std::mutex csr_mu;
+  std::map<std::pair<int, int>, std::shared_ptr<const ebh::NearCSR>> csr_cache;
   size_t bytes() const
   {
     return nodes.bytes() + tri.bytes() + touch_ptr.bytes() + touch.bytes() +

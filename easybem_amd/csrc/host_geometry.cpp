@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 namespace ebh {
 
@@ -466,24 +467,48 @@ void build_near_csr(const Topology &T, int test_basis, int trial_basis,
       }
     }
   }
-  out.row_ptr.assign(out.nrows + 1, 0);
-  out.contrib_ptr.clear();
-  out.contrib_ptr.push_back(0);
-  out.contrib.reserve(items.size());
-  for (int r = 0; r < out.nrows; r++) {
-    auto b = items.begin() + cnt[r], e = items.begin() + cnt[r + 1];
-    std::sort(b, e, [](const Item &x, const Item &y) {
-      return x.col != y.col ? x.col < y.col : x.c < y.c;
-    });
-    for (auto it = b; it != e;) {
-      auto jt = it;
-      while (jt != e && jt->col == it->col) { out.contrib.push_back(jt->c); ++jt; }
-      out.col.push_back(it->col);
-      out.contrib_ptr.push_back((int)out.contrib.size());
-      it = jt;
+  // sort every row by (column, contribution) on host threads (rows are
+  // independent), then group equal columns
+  const int nthreads =
+      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<int> ncol(out.nrows, 0);
+  auto sort_rows = [&](int tid) {
+    for (int r = tid; r < out.nrows; r += nthreads) {
+      auto b = items.begin() + cnt[r], e = items.begin() + cnt[r + 1];
+      std::sort(b, e, [](const Item &x, const Item &y) {
+        return x.col != y.col ? x.col < y.col : x.c < y.c;
+      });
+      int d = 0;
+      for (auto it = b; it != e; ++it)
+        if (it == b || it->col != (it - 1)->col) d++;
+      ncol[r] = d;
     }
-    out.row_ptr[r + 1] = (int)out.col.size();
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(sort_rows, t);
+    sort_rows(0);
+    for (auto &th : pool) th.join();
   }
+  out.row_ptr.assign(out.nrows + 1, 0);
+  for (int r = 0; r < out.nrows; r++) out.row_ptr[r + 1] = out.row_ptr[r] + ncol[r];
+  const int nnz = out.row_ptr[out.nrows];
+  out.col.resize(nnz);
+  out.contrib_ptr.resize(nnz + 1);
+  out.contrib.resize(items.size());
+  // the contribution list is the sorted items in order; entry starts are
+  // the first item of each column run
+  for (size_t i = 0; i < items.size(); i++) out.contrib[i] = items[i].c;
+  for (int r = 0; r < out.nrows; r++) {
+    int e = out.row_ptr[r];
+    for (int i = cnt[r]; i < cnt[r + 1]; i++)
+      if (i == cnt[r] || items[i].col != items[i - 1].col) {
+        out.col[e] = items[i].col;
+        out.contrib_ptr[e] = i;
+        e++;
+      }
+  }
+  out.contrib_ptr[nnz] = (int)items.size();
 }
 
 // The surface gradient of the hat functions solves the 2x2 metric system of

@@ -286,9 +286,22 @@ int op_build_near(ebh_op *op)
     EBH_HIP(hipGetLastError());
     EBH_HIP(hipStreamSynchronize(op->stream));
   }
-  NearCSR csr;
-  build_near_csr(T, adj ? op->d.trial_basis : op->d.test_basis,
-                 adj ? EBH_P1 : op->d.trial_basis, csr);
+  const std::pair<int, int> bases(adj ? op->d.trial_basis : op->d.test_basis,
+                                  adj ? EBH_P1 : op->d.trial_basis);
+  std::shared_ptr<const NearCSR> shared;
+  {
+    std::lock_guard<std::mutex> lk(op->mesh->csr_mu);
+    auto it = op->mesh->csr_cache.find(bases);
+    if (it != op->mesh->csr_cache.end()) {
+      shared = it->second;
+    } else {
+      auto built = std::make_shared<NearCSR>();
+      build_near_csr(T, bases.first, bases.second, *built);
+      shared = built;
+      op->mesh->csr_cache[bases] = shared;
+    }
+  }
+  const NearCSR &csr = *shared;
   DevBuf<int> d_cptr;
   DevBuf<int64_t> d_contrib;
   EBH_TRY(op->near_ptr.upload(csr.row_ptr));

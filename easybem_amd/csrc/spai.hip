@@ -28,6 +28,14 @@ namespace {
 constexpr double kInv4PiSpai = 0.07957747154594766788;
 constexpr int kSpaiLanes = 8;
 
+// 1/sqrt(x): the hardware FP64 estimate (relative error below 6e-8) and one
+// Newton step (below 5e-15)
+__device__ __forceinline__ double rsq64(double x)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
+
 // Far part of V_ab for the pattern entry e = (row[e], col[e]): the sum over
 // triangle pairs (i in supp a, j in supp b) that do not touch of
 // sum_q sum_p w_q J_i phi_a(x_q) w_p J_j phi_b(y_p) / (4 pi |x_q - y_p|).
@@ -90,7 +98,7 @@ __global__ __launch_bounds__(256) void k_spai_far(int nent, const int *__restric
         double t = 0.0;
         for (int q = 0; q < nt; q++) {
           const double dx = X[q][0] - Y[0], dy = X[q][1] - Y[1], dz = X[q][2] - Y[2];
-          t += W[q] / sqrt(dx * dx + dy * dy + dz * dz);
+          t += W[q] * rsq64(dx * dx + dy * dy + dz * dz);
         }
         s += wp * t;
       }
@@ -169,7 +177,7 @@ __global__ __launch_bounds__(256) void k_spai_hs(int nent, const int *__restrict
           double t = 0.0;
           for (int q = 0; q < nt; q++) {
             const double dx = X[q][0] - Y[0], dy = X[q][1] - Y[1], dz = X[q][2] - Y[2];
-            t += W[q] / sqrt(dx * dx + dy * dy + dz * dz);
+            t += W[q] * rsq64(dx * dx + dy * dy + dz * dz);
           }
           vij += rule.w[p] * J * t;
         }
