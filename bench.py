@@ -37,13 +37,8 @@ FLOP_SL = 11               # per far interaction, single layer, one RHS
 # scaled by the same counter on a 512 MiB device copy
 # (profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
 # with that schedule.
-TRAFFIC_FAR32_L5_BYTES = 203.59e6
+TRAFFIC_FAR32_L5_BYTES = 203.96e6
 TRAFFIC_SCHEDULE = ("symmetric", 1)
-# The pair-symmetric schedule evaluates each far point pair once for both
-# directions: 11 FLOP for the target side plus one FMA (2 FLOP) for the
-# source side, against 2 x 11 algorithmic FLOP (pairs inside one 64-triangle
-# group, 1/ng of the work, take one direction only).
-SYM_EXECUTED_PER_ALGORITHMIC = 13.0 / 22.0
 
 
 def near_pass_bytes(w, ne):
@@ -516,10 +511,6 @@ def main():
                                  and (V32.schedule, V32.split) == TRAFFIC_SCHEDULE
                                  else None),
                      "flop_per_launch": far_flop,
-                     **({"executed_flop_per_launch":
-                         far_flop * SYM_EXECUTED_PER_ALGORITHMIC,
-                         "executed_tflops":
-                         achieved * SYM_EXECUTED_PER_ALGORITHMIC} if sym32 else {}),
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
         "near_field": {"kernel": "k_finish (near CSR x + far gather, FP64)",
                        "bound": "hbm",

@@ -73,6 +73,9 @@ struct FarArgs {
                                       // 2(P1-C), c_q^2 |P_k-C|^2 per point;
                                       // others: C, then 2(x_q-C), |x_q-C|^2
   R rc[16];                           // median rules: c_q of point q
+  R kap[4];                           // symmetric kernel: -c c' / 2 for the
+                                      // median orbits (o, o'), [2 o + o'],
+                                      // FP64 times 4 (doubled positions)
   const double *tn;                   // adjoint double layer: unit normals [ne][3]
   const R *src;                       // [ne*nt][REC]
   const int *touch_ptr, *touch;
@@ -385,6 +388,18 @@ __global__ __launch_bounds__(256) void k_far_multi(const FarArgs<R> *__restrict_
 // ksplit partial slots per group (slot J ksplit + range); a launch with NR
 // right-hand sides keeps NR of them per slot, so the buffer is
 // [ng ksplit][NR][nout][ne].
+//
+// With a median rule every point of a triangle is its centroid or C + c_q
+// (P_k - C), k = 0, 1, 2, so for target point q (direction k) and source
+// point b (direction l) of triangles with centroids C, C' and D = C - C':
+//   r^2 = |D + c_q m_k - c_b m'_l|^2
+//       = A_b + B_q + kappa_qb W_kl,
+//   A_b = |D|^2 - c_b D.F'_l + E'_b,  B_q = c_q D.F_k + E_q,
+//   W_kl = F_k.F'_l,  kappa_qb = -c_q c_b / 2,
+// with m = P - C, F = 2m and E = c^2 |m|^2 (terms of a centroid point are 0).
+// Since m_0 + m_1 + m_2 = 0, the sums for k = 2 or l = 2 follow from those
+// for 0 and 1: per triangle pair the kernel forms D, |D|^2, 2 + 2 + 4 dot
+// products and a few sums; per point pair one add and one FMA.
 
 // value of lane L + 1 (mod 64) in lane L: DPP wave_rol:1 (control 0x134),
 // every lane of the wave active (the rotation reads no out-of-wave lane, so
@@ -409,22 +424,48 @@ template <typename R> struct Excl;
 template <> struct Excl<float> { static constexpr float v = 0x1p100f; };
 template <> struct Excl<double> { static constexpr double v = 1e200; };
 
-// source record of the symmetric kernel: x, y, z and NR strengths (NR = 3:
-// the Kelvin fold's layout)
+// source record of the one-point symmetric kernel: x, y, z and NR strengths
+// (NR = 3: the Kelvin fold's layout)
 template <typename R, int NR>
 struct alignas(4 * sizeof(R)) SymRec {
   R v[NR == 1 ? 4 : 8];
 };
 
-// Workgroup bid of the symmetric schedule (see above); lds holds 64 NT
-// records.  Every thread of the workgroup calls it (one barrier).
+// LDS layout of the symmetric kernel's source group (64 triangles).
+// One-point rules: one SymRec per triangle.  Longer rules: one record per
+// triangle of GW values, the source triangle's frame in the target frame's
+// form (centroid C', F'_0 = 2 (P'_0 - C'), F'_1 = 2 (P'_1 - C'), E'_b =
+// c_b^2 |P'_l - C'|^2 of its NT - C0 median points), then the NT x NR
+// strengths.  GW is a multiple of 4 with GW / 4 odd, so the 16-byte reads of
+// 16 lanes at rotating records spread over the LDS banks.
+template <int NT, int NR>
+struct SymLayout {
+  static constexpr int C0 = NT % 3;
+  static constexpr int NO = NT - C0;
+  static constexpr int USED = NT == 1 ? (NR == 1 ? 4 : 8) : 9 + NO + NT * NR;
+  static constexpr int Q4 = (USED + 3) / 4;
+  static constexpr int GW = NT == 1 ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
+  static constexpr int SIG = 9 + NO;  // offset of the strengths
+  static constexpr int LDS = 64 * GW;
+};
+
+// Workgroup bid of the symmetric schedule (see above); lds holds the source
+// group in the SymLayout form.  Every thread of the workgroup calls it (one
+// barrier).
 template <typename R, int NT, int NR>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
-                                          SymRec<R, NR> *__restrict__ lds)
+                                          R *__restrict__ lds)
 {
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
-  constexpr int C0 = NT % 3;
+  using Lay = SymLayout<NT, NR>;
+  constexpr int C0 = Lay::C0, NO = Lay::NO, GW = Lay::GW;
+  // One-point rules move the source records between lanes (DPP wave_rol:1,
+  // lane L reads lane L + 1 -- VALU moves instead of LDS reads, which would
+  // otherwise bound a step of so little arithmetic); longer rules read the
+  // source triangle's record from LDS by index.  The accumulators move with
+  // DPP for one-point rules and with ds_bpermute otherwise.
+  constexpr bool ROT = NT == 1;
   const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
   int rem = bid / ks, M = 0;
   const int rng = bid - rem * ks;
@@ -433,15 +474,17 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
   // FP64 works with doubled positions, so the distances it forms are 2r and
   // rsqrt gives 1/(2r); a Newton step from that estimate needs one multiply
-  // fewer (see below).  FP32 uses the positions as they are.
+  // fewer (see below).  FP32 uses the positions as they are.  Centroids and
+  // the radii c_q scale by 2, E and kappa by 4, F and W not at all.
   constexpr bool F64 = sizeof(R) == 8;
   const R ps = F64 ? R(2) : R(1);
-  {
-    // source group J: positions and strengths of its 64 NT points; points
-    // past the mesh get strength 0, far away
+  if constexpr (ROT) {
+    // source group J: positions and strengths of its 64 points; points past
+    // the mesh get strength 0, far away
     const RecT *g = reinterpret_cast<const RecT *>(a.src);
-    const int p0 = J * 64 * NT, pend = ne * NT;
-    for (int p = threadIdx.x; p < 64 * NT; p += blockDim.x) {
+    RecT *l = reinterpret_cast<RecT *>(lds);
+    const int p0 = J * 64, pend = ne;
+    for (int p = threadIdx.x; p < 64; p += blockDim.x) {
       RecT r;
       if (p0 + p < pend) {
         r = g[p0 + p];
@@ -454,7 +497,31 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
         for (int k = 0; k < RW; k++) r.v[k] = k < 3 ? R(1e18) : R(0);
       }
-      lds[p] = r;
+      l[p] = r;
+    }
+  } else {
+    // source group J: frames and strengths of its 64 triangles, one field of
+    // 64 triangles per pass (coalesced rows of the frame); triangles past
+    // the mesh get strength 0 and a centroid far away
+    const int j0 = J * 64;
+    for (int i = threadIdx.x; i < 64 * Lay::USED; i += blockDim.x) {
+      const int tri = i & 63, f = i >> 6, j = j0 + tri;
+      R v;
+      if (j < ne) {
+        if (f < 3) {
+          v = ps * a.tsl[f * (size_t)ne + j];
+        } else if (f < 9) {
+          v = a.tsl[f * (size_t)ne + j];
+        } else if (f < Lay::SIG) {
+          v = ps * ps * a.tsl[(f + C0) * (size_t)ne + j];
+        } else {
+          const int sr = f - Lay::SIG, b = sr / NR, r = sr - b * NR;
+          v = a.src[RW * ((size_t)j * NT + b) + 3 + r];
+        }
+      } else {
+        v = f < 3 ? R(1e18) : R(0);
+      }
+      lds[tri * GW + f] = v;
     }
   }
   __syncthreads();
@@ -463,8 +530,6 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   if (I > J) return;  // wave-uniform; no barrier follows in this function
   const int t = I * 64 + L;
   const int tt = t < ne ? t : ne - 1;
-  // FP64: D, G and c^2 |P_k - C|^2 scale by 2, 2 and 4 with the positions,
-  // so r2 below is (2r)^2 with the same instructions
   const R Cx = ps * a.tsl[0 * (size_t)ne + tt], Cy = ps * a.tsl[1 * (size_t)ne + tt],
           Cz = ps * a.tsl[2 * (size_t)ne + tt];
   const R F0x = a.tsl[3 * (size_t)ne + tt], F0y = a.tsl[4 * (size_t)ne + tt],
@@ -496,63 +561,92 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
     for (int r = 0; r < NR; r++) acc[q][r] = S[q][r] = R(0);
 
-  // One-point rules move the source records between lanes too (DPP
-  // wave_rol:1, lane L reads lane L + 1 -- VALU moves instead of LDS reads,
-  // which would otherwise bound a step of so little arithmetic); longer
-  // rules read them from LDS by index.  The accumulators move with DPP for
-  // one-point rules and with ds_bpermute otherwise.
-  constexpr bool ROT = NT == 1;
+  // 1/r from r2 (= r^2, FP64: (2r)^2) for the NT target points against one
+  // source point, then both directions' sums
+  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag) {
+    constexpr bool SYM = decltype(sym_tag)::value;
+    R ri[NT];
+    if constexpr (F64) {
+      // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for 1/r
+      // from 2h is h (3 - r2 h^2): 3 instructions after rsqrt.  All
+      // estimates first, so each result is not needed right after its
+      // instruction.
+      R h[NT];
+#pragma unroll
+      for (int q = 0; q < NT; q++) h[q] = __builtin_amdgcn_rsq(r2[q]);
+#pragma unroll
+      for (int q = 0; q < NT; q++) ri[q] = h[q] * fma(-r2[q], h[q] * h[q], R(3));
+    } else {
+#pragma unroll
+      for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        acc[q][r] = fma(sig[r], ri[q], acc[q][r]);
+        if constexpr (SYM) S[b][r] = fma(sq[q][r], ri[q], S[b][r]);
+      }
+  };
+
   auto steps = [&](auto sym_tag) {
     constexpr bool SYM = decltype(sym_tag)::value;
     const int from = ((L + 1) & 63) << 2;  // ds_bpermute byte address
     RecT cur;
-    if constexpr (ROT) cur = lds[(L + k0) & 63];
+    if constexpr (ROT) cur = reinterpret_cast<const RecT *>(lds)[(L + k0) & 63];
     for (int k = k0; k < k1; k++) {
       const int jl = (L + k) & 63;
       // a touching source triangle (handled by the near field) gets r^2 +
       // 2^100 (FP32) or + 1e200 (FP64): 1/r below 1e-15 resp. 1e-100 in
       // both directions, so no strength needs masking
       const R excl = ((mask >> jl) & 1ull) ? Excl<R>::v : R(0);
-      const RecT *rec = lds + jl * NT;
-#pragma unroll
-      for (int b = 0; b < NT; b++) {
-        RecT rb;
-        if constexpr (ROT) rb = cur; else rb = rec[b];
-        const R Dx = Cx - rb.v[0], Dy = Cy - rb.v[1], Dz = Cz - rb.v[2];
-        const R A = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
-        R G[3];
-        G[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
-        G[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
-        G[2] = -(G[0] + G[1]);
-        R r2[NT], ri[NT];
-#pragma unroll
-        for (int q = 0; q < NT; q++)
-          r2[q] = q < C0 ? A : fma(rc[q], G[(q - C0) % 3], A + E[q]);
-        if constexpr (F64) {
-          // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for 1/r
-          // from 2h is h (3 - r2 h^2): 3 instructions after rsqrt.  All
-          // estimates first, so each result is not needed right after its
-          // instruction.
-          R h[NT];
-#pragma unroll
-          for (int q = 0; q < NT; q++) h[q] = __builtin_amdgcn_rsq(r2[q]);
-#pragma unroll
-          for (int q = 0; q < NT; q++) ri[q] = h[q] * fma(-r2[q], h[q] * h[q], R(3));
-        } else {
-#pragma unroll
-          for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < NT; q++)
-#pragma unroll
-          for (int r = 0; r < NR; r++) {
-            acc[q][r] = fma(rb.v[3 + r], ri[q], acc[q][r]);
-            if constexpr (SYM) S[b][r] = fma(sq[q][r], ri[q], S[b][r]);
-          }
-      }
       if constexpr (ROT) {
+        const R Dx = Cx - cur.v[0], Dy = Cy - cur.v[1], Dz = Cz - cur.v[2];
+        R r2[1];
+        r2[0] = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
+        finish_point(r2, cur.v + 3, 0, sym_tag);
 #pragma unroll
         for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
+      } else {
+        const R *g = lds + jl * GW;
+        const R Dx = Cx - g[0], Dy = Cy - g[1], Dz = Cz - g[2];
+        const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
+        // D.F_k (target directions), D.F'_l (source directions), F_k.F'_l
+        R U[3], P[3], W[3][3];
+        U[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
+        U[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
+        U[2] = -(U[0] + U[1]);
+        P[0] = fma(Dx, g[3], fma(Dy, g[4], Dz * g[5]));
+        P[1] = fma(Dx, g[6], fma(Dy, g[7], Dz * g[8]));
+        P[2] = -(P[0] + P[1]);
+        if constexpr (NO > 0) {
+          W[0][0] = fma(F0x, g[3], fma(F0y, g[4], F0z * g[5]));
+          W[0][1] = fma(F0x, g[6], fma(F0y, g[7], F0z * g[8]));
+          W[1][0] = fma(F1x, g[3], fma(F1y, g[4], F1z * g[5]));
+          W[1][1] = fma(F1x, g[6], fma(F1y, g[7], F1z * g[8]));
+          W[0][2] = -(W[0][0] + W[0][1]);
+          W[1][2] = -(W[1][0] + W[1][1]);
+          W[2][0] = -(W[0][0] + W[1][0]);
+          W[2][1] = -(W[0][1] + W[1][1]);
+          W[2][2] = -(W[0][2] + W[1][2]);
+        }
+        R B[NT];
+#pragma unroll
+        for (int q = 0; q < NT; q++) B[q] = q < C0 ? R(0) : fma(rc[q], U[(q - C0) % 3], E[q]);
+#pragma unroll
+        for (int b = 0; b < NT; b++) {
+          const int lb = (b - C0) % 3, ob = (b - C0) / 3;
+          const R Ab = b < C0 ? A0 : fma(-rc[b], P[lb], A0 + g[9 + b - C0]);
+          R r2[NT];
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const int kq = (q - C0) % 3, oq = (q - C0) / 3;
+            r2[q] = q < C0 ? Ab
+                    : b < C0 ? Ab + B[q]
+                             : fma(a.kap[2 * oq + ob], W[kq][lb], Ab + B[q]);
+          }
+          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag);
+        }
       }
       if constexpr (SYM) {
 #pragma unroll
@@ -613,7 +707,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 template <typename R, int NT, int NR>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
-  __shared__ SymRec<R, NR> lds[64 * NT];
+  __shared__ __align__(16) R lds[SymLayout<NT, NR>::LDS];
   sym_block<R, NT, NR>(a, (int)blockIdx.x, lds);
 }
 
@@ -626,7 +720,7 @@ __global__ __launch_bounds__(256) void k_far_sym_multi(const FarArgs<R> *__restr
                                                        const int *__restrict__ off,
                                                        int nops, int *__restrict__ counter)
 {
-  __shared__ SymRec<R, 1> lds[64 * NT];
+  __shared__ __align__(16) R lds[SymLayout<NT, 1>::LDS];
   __shared__ int sh_item;
   const int total = off[nops];
   while (true) {
@@ -1035,6 +1129,15 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
     a.phi[q][1] = in ? op->rule.x[q] : 0.0;
     a.phi[q][2] = in ? op->rule.y[q] : 0.0;
   }
+  // kappa of the symmetric kernel for median orbits o, o' (nt <= 7: at most
+  // two orbits of three points after the centroid, if any)
+  const int c0 = nt % 3, norb = (nt - c0) / 3;
+  for (int o = 0; o < 2; o++)
+    for (int p = 0; p < 2; p++) {
+      const double co = o < norb ? op->ray_c[c0 + 3 * o] : 0.0;
+      const double cp = p < norb ? op->ray_c[c0 + 3 * p] : 0.0;
+      a.kap[2 * o + p] = (R)(-0.5 * co * cp * (sizeof(R) == 8 ? 4.0 : 1.0));
+    }
 }
 
 // LDS bytes of one source tile

@@ -158,6 +158,8 @@ bool ray_order(int nt, double *x, double *y, double *w, double *c)
       if (mem[k] < 0) return false;
       used[mem[k]] = true;
       order.push_back(mem[k]);
+      // one radius per orbit (the members' agree to rounding)
+      cc[mem[k]] = cc[q];
     }
   }
   std::vector<double> ox(nt), oy(nt), ow(nt);
