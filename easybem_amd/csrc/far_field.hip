@@ -432,40 +432,53 @@ struct alignas(4 * sizeof(R)) SymRec {
 };
 
 // LDS layout of the symmetric kernel's source group (64 triangles).
-// One-point rules: one SymRec per triangle.  Longer rules: one record per
-// triangle of GW values, the source triangle's frame in the target frame's
-// form (centroid C', F'_0 = 2 (P'_0 - C'), F'_1 = 2 (P'_1 - C'), E'_b =
-// c_b^2 |P'_l - C'|^2 of its NT - C0 median points), then the NT x NR
-// strengths.  GW is a multiple of 4 with GW / 4 odd, so the 16-byte reads of
-// 16 lanes at rotating records spread over the LDS banks.
-template <int NT, int NR>
+// One-point single-layer rules: one SymRec per triangle.  Otherwise one
+// record per triangle of GW values, the source triangle's frame in the
+// target frame's form (centroid C', F'_0 = 2 (P'_0 - C'), F'_1 = 2 (P'_1 -
+// C'), E'_b = c_b^2 |P'_l - C'|^2 of its NT - C0 median points), for the
+// double layer its unit normal n', then the NT x NR strengths.  GW is a
+// multiple of 4 with GW / 4 odd, so the 16-byte reads of 16 lanes at
+// rotating records spread over the LDS banks.
+template <int NT, int NR, bool DL = false>
 struct SymLayout {
+  static constexpr bool ROT = NT == 1 && !DL;
   static constexpr int C0 = NT % 3;
   static constexpr int NO = NT - C0;
-  static constexpr int USED = NT == 1 ? (NR == 1 ? 4 : 8) : 9 + NO + NT * NR;
+  static constexpr int NRM = 9 + NO;              // offset of n' (double layer)
+  static constexpr int SIG = NRM + (DL ? 3 : 0);  // offset of the strengths
+  static constexpr int USED = ROT ? (NR == 1 ? 4 : 8) : SIG + NT * NR;
   static constexpr int Q4 = (USED + 3) / 4;
-  static constexpr int GW = NT == 1 ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
-  static constexpr int SIG = 9 + NO;  // offset of the strengths
+  static constexpr int GW = ROT ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
   static constexpr int LDS = 64 * GW;
 };
 
 // Workgroup bid of the symmetric schedule (see above); lds holds the source
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
 // barrier).
-template <typename R, int NT, int NR>
+//
+// DL: the double layer (x - y).n_y / r^3 with the same point sets.  Rule
+// points lie in their triangle's plane, so for target point q of triangle i
+// and source point b of triangle j, (x_q - y_b).n_j = D.n_j + c_q m_k.n_j
+// does not depend on b, and (y_b - x_q).n_i = -D.n_i + c_b m'_l.n_i not on
+// q: each direction is one FMA per point pair (strength times r^-3) plus
+// one per point and step for that factor.  The strengths are the single
+// layer's (w J u_h / 4 pi per point).
+template <typename R, int NT, int NR, bool DL = false>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                                           R *__restrict__ lds)
 {
+  static_assert(!DL || NR == 1, "double layer: one right-hand side");
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
-  using Lay = SymLayout<NT, NR>;
+  using Lay = SymLayout<NT, NR, DL>;
   constexpr int C0 = Lay::C0, NO = Lay::NO, GW = Lay::GW;
-  // One-point rules move the source records between lanes (DPP wave_rol:1,
-  // lane L reads lane L + 1 -- VALU moves instead of LDS reads, which would
-  // otherwise bound a step of so little arithmetic); longer rules read the
-  // source triangle's record from LDS by index.  The accumulators move with
-  // DPP for one-point rules and with ds_bpermute otherwise.
-  constexpr bool ROT = NT == 1;
+  // One-point single-layer rules move the source records between lanes (DPP
+  // wave_rol:1, lane L reads lane L + 1 -- VALU moves instead of LDS reads,
+  // which would otherwise bound a step of so little arithmetic); otherwise
+  // the source triangle's record is read from LDS by index.  The
+  // accumulators move with DPP for one-point rules and with ds_bpermute
+  // otherwise.
+  constexpr bool ROT = Lay::ROT;
   const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
   int rem = bid / ks, M = 0;
   const int rng = bid - rem * ks;
@@ -512,8 +525,10 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
           v = ps * a.tsl[f * (size_t)ne + j];
         } else if (f < 9) {
           v = a.tsl[f * (size_t)ne + j];
-        } else if (f < Lay::SIG) {
+        } else if (f < Lay::NRM) {
           v = ps * ps * a.tsl[(f + C0) * (size_t)ne + j];
+        } else if (f < Lay::SIG) {
+          v = (R)a.tn[3 * (size_t)j + (f - Lay::NRM)];
         } else {
           const int sr = f - Lay::SIG, b = sr / NR, r = sr - b * NR;
           v = a.src[RW * ((size_t)j * NT + b) + 3 + r];
@@ -536,11 +551,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
           F0z = a.tsl[5 * (size_t)ne + tt];
   const R F1x = a.tsl[6 * (size_t)ne + tt], F1y = a.tsl[7 * (size_t)ne + tt],
           F1z = a.tsl[8 * (size_t)ne + tt];
-  R E[NT], sq[NT][NR], rc[NT];
+  R E[NT], sq[NT][NR], rc[NT], hc[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
     E[q] = ps * ps * a.tsl[(9 + q) * (size_t)ne + tt];
     rc[q] = ps * a.rc[q];
+    hc[q] = R(0.5) * a.rc[q];  // double layer: c_q m.n = (c_q / 2) F.n
 #pragma unroll
     for (int r = 0; r < NR; r++)
       sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
@@ -555,6 +571,14 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       if (j < 64u) mask |= 1ull << j;
     }
   }
+  // double layer: the target triangle's unit normal; D and c scale by ps
+  R Nix = 0, Niy = 0, Niz = 0;
+  if constexpr (DL) {
+    Nix = (R)a.tn[3 * (size_t)tt];
+    Niy = (R)a.tn[3 * (size_t)tt + 1];
+    Niz = (R)a.tn[3 * (size_t)tt + 2];
+  }
+  const R hps = R(1) / ps;
   R acc[NT][NR], S[NT][NR];
 #pragma unroll
   for (int q = 0; q < NT; q++)
@@ -562,8 +586,10 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     for (int r = 0; r < NR; r++) acc[q][r] = S[q][r] = R(0);
 
   // 1/r from r2 (= r^2, FP64: (2r)^2) for the NT target points against one
-  // source point, then both directions' sums
-  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag) {
+  // source point, then both directions' sums.  Double layer: it[q] gathers
+  // sigma_b / r^3 for target point q over the step's source points, and
+  // source point b's sum of sigma_q / r^3 is scaled by its factor sfb.
+  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag, R *it, R sfb) {
     constexpr bool SYM = decltype(sym_tag)::value;
     R ri[NT];
     if constexpr (F64) {
@@ -580,13 +606,24 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
       for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
     }
+    if constexpr (DL) {
+      R is = R(0);
 #pragma unroll
-    for (int q = 0; q < NT; q++)
-#pragma unroll
-      for (int r = 0; r < NR; r++) {
-        acc[q][r] = fma(sig[r], ri[q], acc[q][r]);
-        if constexpr (SYM) S[b][r] = fma(sq[q][r], ri[q], S[b][r]);
+      for (int q = 0; q < NT; q++) {
+        const R r3 = ri[q] * ri[q] * ri[q];
+        it[q] = fma(sig[0], r3, it[q]);
+        if constexpr (SYM) is = fma(sq[q][0], r3, is);
       }
+      if constexpr (SYM) S[b][0] = fma(sfb, is, S[b][0]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < NT; q++)
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          acc[q][r] = fma(sig[r], ri[q], acc[q][r]);
+          if constexpr (SYM) S[b][r] = fma(sq[q][r], ri[q], S[b][r]);
+        }
+    }
   };
 
   auto steps = [&](auto sym_tag) {
@@ -604,7 +641,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         const R Dx = Cx - cur.v[0], Dy = Cy - cur.v[1], Dz = Cz - cur.v[2];
         R r2[1];
         r2[0] = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
-        finish_point(r2, cur.v + 3, 0, sym_tag);
+        finish_point(r2, cur.v + 3, 0, sym_tag, nullptr, R(0));
 #pragma unroll
         for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
       } else {
@@ -633,6 +670,26 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         R B[NT];
 #pragma unroll
         for (int q = 0; q < NT; q++) B[q] = q < C0 ? R(0) : fma(rc[q], U[(q - C0) % 3], E[q]);
+        // double layer: tf[q] = (x_q - y).n_j and sf[b] = (y_b - x).n_i
+        R it[NT], tf[NT], sf[NT];
+        if constexpr (DL) {
+          const R *nj = g + Lay::NRM;
+          const R dt = hps * fma(Dx, nj[0], fma(Dy, nj[1], Dz * nj[2]));
+          const R ds = -hps * fma(Dx, Nix, fma(Dy, Niy, Dz * Niz));
+          R Gt[3], Gs[3];
+          Gt[0] = fma(F0x, nj[0], fma(F0y, nj[1], F0z * nj[2]));
+          Gt[1] = fma(F1x, nj[0], fma(F1y, nj[1], F1z * nj[2]));
+          Gt[2] = -(Gt[0] + Gt[1]);
+          Gs[0] = fma(g[3], Nix, fma(g[4], Niy, g[5] * Niz));
+          Gs[1] = fma(g[6], Nix, fma(g[7], Niy, g[8] * Niz));
+          Gs[2] = -(Gs[0] + Gs[1]);
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            tf[q] = q < C0 ? dt : fma(hc[q], Gt[(q - C0) % 3], dt);
+            sf[q] = q < C0 ? ds : fma(hc[q], Gs[(q - C0) % 3], ds);
+            it[q] = R(0);
+          }
+        }
 #pragma unroll
         for (int b = 0; b < NT; b++) {
           const int lb = (b - C0) % 3, ob = (b - C0) / 3;
@@ -645,7 +702,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                     : b < C0 ? Ab + B[q]
                              : fma(a.kap[2 * oq + ob], W[kq][lb], Ab + B[q]);
           }
-          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag);
+          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag, it, DL ? sf[b] : R(0));
+        }
+        if constexpr (DL) {
+#pragma unroll
+          for (int q = 0; q < NT; q++) acc[q][0] = fma(tf[q], it[q], acc[q][0]);
         }
       }
       if constexpr (SYM) {
@@ -704,11 +765,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   }
 }
 
-template <typename R, int NT, int NR>
+template <typename R, int NT, int NR, bool DL = false>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
-  __shared__ __align__(16) R lds[SymLayout<NT, NR>::LDS];
-  sym_block<R, NT, NR>(a, (int)blockIdx.x, lds);
+  __shared__ __align__(16) R lds[SymLayout<NT, NR, DL>::LDS];
+  sym_block<R, NT, NR, DL>(a, (int)blockIdx.x, lds);
 }
 
 // Persistent multi-mesh form of the symmetric schedule: the workgroups of
@@ -1149,16 +1210,16 @@ size_t far_lds(const ebh_op *op, int R_bytes)
   return (size_t)op->tile * op->rule.nt * rec * R_bytes;
 }
 
-template <typename R, int NR>
+template <typename R, int NR, bool DL = false>
 int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
 {
   const dim3 blk(256);
   switch (nt) {
-  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR>), grid, blk, 0, s, a); break;
-  case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR>), grid, blk, 0, s, a); break;
-  case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR>), grid, blk, 0, s, a); break;
-  case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR>), grid, blk, 0, s, a); break;
-  case 7: hipLaunchKernelGGL((k_far_sym<R, 7, NR>), grid, blk, 0, s, a); break;
+  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR, DL>), grid, blk, 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR, DL>), grid, blk, 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR, DL>), grid, blk, 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR, DL>), grid, blk, 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_sym<R, 7, NR, DL>), grid, blk, 0, s, a); break;
   default:
     set_error("no pair-symmetric kernel for rule size " + std::to_string(nt));
     return EBH_ERR_UNSUPPORTED;
@@ -1175,6 +1236,12 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
   int nblk = 0;
   for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
   const dim3 grid((unsigned)(nblk * op->ksplit));
+  if (op->d.kernel == EBH_LAPLACE_DL) {
+    // the symmetric double layer exists in FP64 only (sym_eligible)
+    if constexpr (sizeof(R) == 8) return launch_sym<R, 1, true>(op->rule.nt, grid, s, a);
+    set_error("no FP32 pair-symmetric double layer");
+    return EBH_ERR_UNSUPPORTED;
+  }
   return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, a)
                  : launch_sym<R, 1>(op->rule.nt, grid, s, a);
 }
@@ -1210,9 +1277,12 @@ int run_fold(ebh_op *op, const double *x, hipStream_t s)
                            : kInv4Pi;
   dim3 grid((n + 255) / 256), blk(256);
   const int p1 = op->d.trial_basis;
-  switch (op->d.kernel) {
+  // the adjoint double layer and the symmetric double layer take point
+  // strengths, as the single layer
+  const int kind = op->sym && op->d.kernel == EBH_LAPLACE_DL ? EBH_LAPLACE_SL : op->d.kernel;
+  switch (kind) {
   case EBH_LAPLACE_SL:
-  case EBH_LAPLACE_ADL:  // point strengths, as the single layer
+  case EBH_LAPLACE_ADL:
     hipLaunchKernelGGL((k_fold<R, 0>), grid, blk, 0, s, ne, nt, op->sg64.p,
                        op->swt.p, op->normals.p, op->mesh->tri.p, p1, op->rule,
                        x, op->ncols, scale, src);
@@ -1347,8 +1417,14 @@ bool sym_eligible(const ebh_op *op)
   const double ne = op->mesh->T.ne;
   const double bytes =
       std::ceil(ne / 64.0) * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
-  return op->d.kernel == EBH_LAPLACE_SL && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
-         op->d.test_basis == op->d.trial_basis && bytes <= kSymPartialMax;
+  // the single layer with equal bases, or the FP64 double layer (its
+  // point sums are weighted by the test functions and its strengths by the
+  // trial density, so the bases may differ)
+  const bool sl = op->d.kernel == EBH_LAPLACE_SL && op->d.test_basis == op->d.trial_basis;
+  const bool dl = op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 &&
+                  op->nr_max == 1;
+  return (sl || dl) && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
+         bytes <= kSymPartialMax;
 }
 
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
@@ -1357,8 +1433,8 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   if (sym) {
     if (!sym_eligible(op)) {
       set_error("the pair-symmetric schedule needs a Laplace single layer with equal "
-                "test and trial bases, a median-symmetric far rule and partial sums "
-                "of at most 4 GiB");
+                "test and trial bases or the FP64 double layer, a median-symmetric "
+                "far rule and partial sums of at most 4 GiB");
       return EBH_ERR_UNSUPPORTED;
     }
     // groups of 64 triangles; chunks (when given) is the number of step
@@ -1491,9 +1567,12 @@ int op_build_far(ebh_op *op)
                             (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL
                                  ? 4 : 8)));
   }
-  if (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL) {
+  if (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL ||
+      (op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 && op->ray &&
+       nt <= 7)) {
     // single-layer target frame: centroid, then the median form (ray rules)
-    // or the offsets of every point from the centroid
+    // or the offsets of every point from the centroid; the FP64 double layer
+    // uses it on the symmetric schedule
     const bool ray = op->ray && nt <= 7;
     std::vector<double> tsl((size_t)(ray ? 9 + nt : 3 + 4 * nt) * ne);
     for (int t = 0; t < ne; t++) {
@@ -1852,6 +1931,12 @@ struct ebh_batch {
 namespace ebh {
 namespace {
 
+// the persistent symmetric launch holds single-layer members
+bool batch_sym(const ebh_op *op)
+{
+  return op->d.kernel == EBH_LAPLACE_SL && sym_eligible(op);
+}
+
 // Members' far-field schedules for a batch: the symmetric schedule with one
 // step range where it applies (the batch supplies the parallelism), else
 // one chunk size for all, chosen so that the launch holds about 8 work
@@ -1860,12 +1945,12 @@ int batch_shapes(ebh_batch *b, int grid)
 {
   double work = 0.0;
   for (const ebh_op *op : b->ops)
-    if (!sym_eligible(op)) work += (double)((op->mesh->T.ne + 255) / 256) * op->mesh->T.ne;
+    if (!batch_sym(op)) work += (double)((op->mesh->T.ne + 255) / 256) * op->mesh->T.ne;
   int cs = (int)std::ceil(work / (8.0 * std::max(grid, 1)));
   cs = std::max(64, (cs + 63) / 64 * 64);
   for (ebh_op *op : b->ops) {
     const int ne = op->mesh->T.ne;
-    if (sym_eligible(op) && op_configure(op, 0, 0, 1, true) == EBH_OK) continue;
+    if (batch_sym(op) && op_configure(op, 0, 0, 1, true) == EBH_OK) continue;
     EBH_TRY(op_configure(op, 256, 64, std::max(1, (ne + cs - 1) / cs)));
   }
   return EBH_OK;

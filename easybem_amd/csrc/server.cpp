@@ -81,8 +81,11 @@ using CacheKey = std::pair<uint64_t, int>;  // (mesh key, problem)
 // Device bytes of one request, estimated before anything is allocated:
 // near-field pairs (about 13 per triangle for V, with their FP64 blocks
 // during set-up) and CSR, far-field quadrature data and partial sums, the
-// Krylov basis and work vectors.
-size_t estimate_bytes(int np, int ne, int restart, bool mixed)
+// Krylov basis and work vectors.  Operators on the pair-symmetric schedule
+// keep (ne / 64) x nout x ne FP64 partial sums (up to 4 GiB each): the
+// system operators (Neumann: the P0 single layer inside D) in each
+// precision, and the Dirichlet right-hand side's FP64 double layer.
+size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1)
 {
   const double pairs = 14.0 * ne;
   const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
@@ -90,7 +93,14 @@ size_t estimate_bytes(int np, int ne, int restart, bool mixed)
   const double ops = (mixed ? 3 : 2) * (near / 2 + far_one);
   const double krylov = (restart + 6.0) * ne * 8;
   const double mesh = np * 24.0 + ne * 100.0;
-  return (size_t)(1.25 * (near + ops + krylov + mesh)) + (16u << 20);
+  auto sym_partial = [&](int nout) {
+    const double b = std::ceil(ne / 64.0) * nout * (double)ne * 8.0;
+    return b <= 4.0 * 1024 * 1024 * 1024 ? b : 0.0;
+  };
+  const double partial = neumann ? (mixed ? 2 : 1) * sym_partial(1)
+                                 : (mixed ? 2 : 1) * sym_partial(p1 ? 3 : 1) +
+                                       sym_partial(p1 ? 3 : 1);
+  return (size_t)(1.25 * (near + ops + krylov + mesh) + partial) + (16u << 20);
 }
 
 }  // namespace
@@ -382,7 +392,9 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
   r->elems.assign(elems, elems + (size_t)3 * ne);
   for (int k = 0; k < 3; k++) r->x0[k] = x0[k];
   r->key = mesh_key;
-  r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0);
+  r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0,
+                                problem == EBH_NEUMANN,
+                                problem == EBH_NEUMANN || s->o.dirichlet_p1 != 0);
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

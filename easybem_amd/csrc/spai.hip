@@ -214,14 +214,6 @@ __global__ __launch_bounds__(256) void k_spai_apply(int n, const int *__restrict
   if (sub == 0) y[r] = s;
 }
 
-// entry (a, b) of a row-sorted pattern, or -1
-inline long find_entry(const std::vector<int> &ptr, const std::vector<int> &col, int a, int b)
-{
-  const auto first = col.begin() + ptr[a], last = col.begin() + ptr[a + 1];
-  const auto it = std::lower_bound(first, last, b);
-  return (it != last && *it == b) ? (long)(it - col.begin()) : -1;
-}
-
 // Solves B x = rhs in place (B m x m, row-major, overwritten) by Gaussian
 // elimination with partial pivoting; false if a pivot vanishes.
 bool solve_inplace(int m, double *B, double *rhs)
@@ -314,39 +306,65 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
       for (int c = T.touch_ptr[i]; c < T.touch_ptr[i + 1]; c++)
         if (T.touch[c] != i) adj[i].push_back(T.touch[c]);
   }
+  // host threads over rows (row c goes to thread c mod nthreads; every row
+  // is independent, so results do not depend on the split)
+  const int nthreads =
+      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto on_threads = [&](auto fn) {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(fn, t);
+    fn(0);
+    for (auto &th : pool) th.join();
+  };
   // DOFs within `depth` adjacency steps of c, sorted (breadth-first search
-  // with a visit stamp per DOF)
-  std::vector<int> stamp(n, -1), frontier, next;
-  auto ball = [&](int c, int depth, std::vector<int> &out) {
+  // with a visit stamp per DOF and thread)
+  struct Bfs {
+    std::vector<int> stamp, frontier, next;
+    int tick = 0;
+  };
+  std::vector<Bfs> bfs(nthreads);
+  auto ball = [&](Bfs &w, int c, int depth, std::vector<int> &out) {
+    const int mark = ++w.tick;
     out.assign(1, c);
-    stamp[c] = c;
-    frontier.assign(1, c);
+    w.stamp[c] = mark;
+    w.frontier.assign(1, c);
     for (int d = 0; d < depth; d++) {
-      next.clear();
-      for (int a : frontier)
+      w.next.clear();
+      for (int a : w.frontier)
         for (int b : adj[a])
-          if (stamp[b] != c) {
-            stamp[b] = c;
-            next.push_back(b);
+          if (w.stamp[b] != mark) {
+            w.stamp[b] = mark;
+            w.next.push_back(b);
             out.push_back(b);
           }
-      frontier.swap(next);
+      w.frontier.swap(w.next);
     }
     std::sort(out.begin(), out.end());
   };
   // patches, and the pattern of every pair that shares a patch: two DOFs
   // share a patch of radius k exactly when they are 2k steps apart at most
-  std::vector<std::vector<int>> patch(n);
-  std::vector<int> eptr(n + 1, 0), ecol, erow, row;
-  for (int c = 0; c < n; c++) ball(c, rings, patch[c]);
-  std::fill(stamp.begin(), stamp.end(), -1);
-  for (int a = 0; a < n; a++) {
-    ball(a, 2 * rings, row);
-    eptr[a + 1] = eptr[a] + (int)row.size();
-    ecol.insert(ecol.end(), row.begin(), row.end());
-    erow.insert(erow.end(), row.size(), a);
-  }
-  const int nent = (int)ecol.size();
+  std::vector<std::vector<int>> patch(n), prow(n);
+  on_threads([&](int tid) {
+    Bfs &w = bfs[tid];
+    w.stamp.assign(n, 0);
+    for (int c = tid; c < n; c += nthreads) {
+      ball(w, c, rings, patch[c]);
+      ball(w, c, 2 * rings, prow[c]);
+    }
+  });
+  std::vector<int> eptr(n + 1, 0);
+  for (int a = 0; a < n; a++) eptr[a + 1] = eptr[a] + (int)prow[a].size();
+  const int nent = eptr[n];
+  std::vector<int> ecol(nent), erow(nent);
+  on_threads([&](int tid) {
+    for (int a = tid; a < n; a += nthreads) {
+      std::copy(prow[a].begin(), prow[a].end(), ecol.begin() + eptr[a]);
+      std::fill(erow.begin() + eptr[a], erow.begin() + eptr[a + 1], a);
+      std::vector<int>().swap(prow[a]);
+    }
+  });
+  // per thread: position of a DOF in the row or patch at hand, else -1
+  std::vector<std::vector<int>> posv(nthreads);
 
   // entries on the device: the hypersingular operator's whole, or the
   // single layer's far part
@@ -398,11 +416,18 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
                       hipMemcpyDeviceToHost));
     EBH_HIP(hipMemcpy(nval.data(), A->near_val.p, sizeof(double) * nval.size(),
                       hipMemcpyDeviceToHost));
-    for (int a = 0; a < n; a++)
-      for (int e = nptr[a]; e < nptr[a + 1]; e++) {
-        const long k = find_entry(eptr, ecol, a, ncol[e]);
-        if (k >= 0) ev[k] += nval[e];
+    on_threads([&](int tid) {
+      std::vector<int> &pos = posv[tid];
+      pos.assign(n, -1);
+      for (int a = tid; a < n; a += nthreads) {
+        for (int k = eptr[a]; k < eptr[a + 1]; k++) pos[ecol[k]] = k;
+        for (int e = nptr[a]; e < nptr[a + 1]; e++) {
+          const int k = pos[ncol[e]];
+          if (k >= 0) ev[k] += nval[e];
+        }
+        for (int k = eptr[a]; k < eptr[a + 1]; k++) pos[ecol[k]] = -1;
       }
+    });
   }
 
   // rows of M: patch inverses, rows split over host threads (each row is
@@ -411,24 +436,27 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
   for (int c = 0; c < n; c++) mptr[c + 1] = mptr[c] + (int)patch[c].size();
   std::vector<int> mcol(mptr[n]);
   std::vector<double> mval(mptr[n]);
-  const int nthreads =
-      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<int> bad(nthreads, -1);
   auto rows_of = [&](int tid) {
     std::vector<double> B, rhs;
+    std::vector<int> &loc = posv[tid];
+    loc.assign(n, -1);
     for (int c = tid; c < n; c += nthreads) {
       const std::vector<int> &p = patch[c];
       const int m = (int)p.size();
-      // B = transpose of V restricted to the patch
+      // B = transpose of V restricted to the patch: B[q][r] = V[p_r][p_q]
+      // from row p_r of the pattern (which holds every DOF of the patch)
       B.assign((size_t)m * m, 0.0);
       rhs.assign(m, 0.0);
+      for (int r = 0; r < m; r++) loc[p[r]] = r;
       for (int r = 0; r < m; r++) {
         if (p[r] == c) rhs[r] = 1.0;
-        for (int q = 0; q < m; q++) {
-          const long k = find_entry(eptr, ecol, p[r], p[q]);
-          B[(size_t)q * m + r] = k >= 0 ? ev[k] : 0.0;
+        for (int k = eptr[p[r]]; k < eptr[p[r] + 1]; k++) {
+          const int q = loc[ecol[k]];
+          if (q >= 0) B[(size_t)q * m + r] = ev[k];
         }
       }
+      for (int r = 0; r < m; r++) loc[p[r]] = -1;
       bool ok = solve_inplace(m, B.data(), rhs.data());
       for (int r = 0; r < m && ok; r++) ok = std::isfinite(rhs[r]);
       if (!ok) {
@@ -439,12 +467,7 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
       std::copy(rhs.begin(), rhs.end(), mval.begin() + mptr[c]);
     }
   };
-  {
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; t++) pool.emplace_back(rows_of, t);
-    rows_of(0);
-    for (auto &th : pool) th.join();
-  }
+  on_threads(rows_of);
   for (int t = 0; t < nthreads; t++)
     if (bad[t] >= 0) {
       set_error("singular or non-finite patch inverse at DOF " + std::to_string(bad[t]));

@@ -1,9 +1,12 @@
-"""Pair-symmetric far-field schedule (k_far_sym) of the single layer.
+"""Pair-symmetric far-field schedule (k_far_sym) of the single layer and of
+the FP64 double layer.
 
 The symmetric schedule sums every unordered pair of 64-triangle groups once
 and uses each rsqrt for both directions, so it computes the same matrix-free
 apply in another order.  Bars: against the oracle as the chunked schedule
-(FP64 <= 1e-12, FP32 <= 1e-6) and against the chunked FP64 apply <= 1e-14.
+(FP64 <= 1e-12, FP32 <= 1e-6) and against the chunked FP64 apply <= 1e-14
+(single layer) or 1e-13 (double layer: r^-3 and the kernel's cancelling
+dot product).
 Meshes cover one partial group (20 triangles), group counts that are not a
 multiple of 4 (the workgroup's 4 target groups) and a partial last group.
 """
@@ -102,14 +105,14 @@ def test_symmetric_split_rejected(L):
             op.set_schedule("symmetric", bad)
 
 
-@pytest.mark.parametrize("kernel,test,trial,tri_rule", [
-    ("K", 0, 1, 4), ("V", 0, 1, 4), ("V", 1, 0, 4), ("kelvin", 0, 0, 4),
-    ("V", 0, 0, 5)])
+@pytest.mark.parametrize("kernel,test,trial,tri_rule,prec", [
+    ("K", 0, 1, 4, "fp32"), ("K", 1, 1, 5, "fp64"), ("V", 0, 1, 4, "fp64"),
+    ("V", 1, 0, 4, "fp64"), ("kelvin", 0, 0, 4, "fp64"), ("V", 0, 0, 5, "fp64")])
 def test_symmetric_rejected_where_it_does_not_apply(L, kernel, test, trial,
-                                                    tri_rule):
+                                                    tri_rule, prec):
     P, T = meshes.icosphere(1)
     m = L.Mesh(P, T)
-    op = L.Operator(m, kernel, test, trial, "fp64", tri_rule=tri_rule)
+    op = L.Operator(m, kernel, test, trial, prec, tri_rule=tri_rule)
     with pytest.raises(L.EbhError) as e:
         op.set_schedule("symmetric")
     assert e.value.code == -4
@@ -206,3 +209,56 @@ def test_symmetric_full_size_l5(L):
     e64, e32 = rel(y1, y0), rel(op32(x), y0)
     print("fp64 symmetric vs chunked", e64, "fp32 symmetric vs fp64", e32)
     assert e64 <= 1e-14 and e32 <= 1e-6
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("test", [0, 1])
+def test_symmetric_double_layer(L, oracle, mesh, test):
+    """FP64 double layer (P1 trial, P0 or P1 test) on the symmetric
+    schedule against the chunked schedule and the oracle."""
+    P, T = MESHES[mesh]()
+    print(f"case K mesh={mesh} ne={T.shape[0]} test={test}", flush=True)
+    x = _x(1, P, T, 8)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "K", test, 1, "fp64", line_rule=4, tri_rule=4)
+    op.set_schedule("chunked")
+    y0 = op(x)
+    op.set_schedule("symmetric")
+    assert op.schedule == "symmetric"
+    y = op(x)
+    yr = oracle.apply("K", test, 1, P, T, x, line_idx=4, tri_idx=4).ravel()
+    print("rel err vs chunked", rel(y, y0), "vs oracle", rel(y, yr))
+    assert rel(y, y0) <= 1e-13
+    assert rel(y, yr) <= 1e-12
+    assert np.array_equal(op(x), y)
+
+
+@pytest.mark.parametrize("tri_rule", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("split", [1, 4, 64])
+def test_symmetric_double_layer_rules_and_splits(L, tri_rule, split):
+    P, T = meshes.torus(1.0, 0.4, 40, 20)
+    m = L.Mesh(P, T)
+    x = _x(1, P, T, 10)
+    op = L.Operator(m, "K", 1, 1, "fp64", line_rule=3, tri_rule=tri_rule)
+    op.set_schedule("chunked")
+    y0 = op(x)
+    op.set_schedule("symmetric", split)
+    assert op.split == split
+    err = rel(op(x), y0)
+    print(f"K rule {tri_rule} split {split}: rel err {err}")
+    assert err <= 1e-13
+
+
+def test_symmetric_double_layer_full_size_l5(L):
+    """The bench's right-hand side operator (P1 x P1 K on 20480 triangles)
+    and its default schedule."""
+    P, T = meshes.icosphere(5)
+    m = L.Mesh(P, T)
+    u = 1.0 / np.linalg.norm(P - np.array([0.0, 0.0, 2.0]), axis=1)
+    op = L.Operator(m, "K", 1, 1, "fp64")
+    op.set_schedule("chunked")
+    y0 = op(u)
+    op.set_schedule("symmetric", 1)
+    err = rel(op(u), y0)
+    print("K L5 symmetric vs chunked", err)
+    assert err <= 1e-13
