@@ -446,71 +446,94 @@ void build_near_csr(const Topology &T, int test_basis, int trial_basis,
   out.nrows = test_basis ? T.np : T.ne;
   out.ncols = trial_basis ? T.np : T.ne;
   struct Item { int col; int64_t c; };
-  // bucket contributions by row, then sort within each row
-  std::vector<int> cnt(out.nrows + 1, 0);
-  const int nk = test_basis ? 3 : 1, nl = trial_basis ? 3 : 1;
-  for (const NearPair &P : T.pairs)
-    for (int k = 0; k < nk; k++) cnt[(test_basis ? P.oi[k] : P.i) + 1] += nl;
-  for (int r = 0; r < out.nrows; r++) cnt[r + 1] += cnt[r];
-  std::vector<Item> items(cnt[out.nrows]);
-  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-  for (size_t p = 0; p < T.pairs.size(); p++) {
-    const NearPair &P = T.pairs[p];
-    for (int k = 0; k < nk; k++) {
-      int row = test_basis ? P.oi[k] : P.i;
+  const int nl = trial_basis ? 3 : 1;
+  // The pairs of test triangle i are T.pairs[touch_ptr[i] .. touch_ptr[i+1]),
+  // so every row's contributions can be gathered on its own: row a of a P1
+  // test basis takes the pairs of the triangles around node a (local test
+  // node k with oi[k] == a), a P0 row those of its triangle.  Rows run on
+  // host threads (row r on thread r mod nthreads); a row's items are sorted
+  // by (column, contribution), which is unique per item, so the result does
+  // not depend on the gathering order or the split.
+  auto for_items = [&](int r, auto &&fn) {
+    auto pair_items = [&](int p, int k) {
+      const NearPair &P = T.pairs[p];
       for (int l = 0; l < nl; l++) {
-        int col = trial_basis ? P.oj[l] : P.j;
+        const int col = trial_basis ? P.oj[l] : P.j;
         int code;
         if (test_basis && trial_basis) code = 3 * k + l;
         else if (!test_basis && trial_basis) code = 9 + l;
         else if (test_basis && !trial_basis) code = 12 + k;
         else code = 15;
-        items[fill[row]++] = Item{col, (int64_t)p * 16 + code};
+        fn(col, (int64_t)p * 16 + code);
       }
-    }
-  }
-  // sort every row by (column, contribution) on host threads (rows are
-  // independent), then group equal columns
-  const int nthreads =
-      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<int> ncol(out.nrows, 0);
-  auto sort_rows = [&](int tid) {
-    for (int r = tid; r < out.nrows; r += nthreads) {
-      auto b = items.begin() + cnt[r], e = items.begin() + cnt[r + 1];
-      std::sort(b, e, [](const Item &x, const Item &y) {
-        return x.col != y.col ? x.col < y.col : x.c < y.c;
-      });
-      int d = 0;
-      for (auto it = b; it != e; ++it)
-        if (it == b || it->col != (it - 1)->col) d++;
-      ncol[r] = d;
+    };
+    if (test_basis) {
+      for (int q = T.node_tri_ptr[r]; q < T.node_tri_ptr[r + 1]; q++) {
+        const int i = T.node_tri[q];
+        for (int p = T.touch_ptr[i]; p < T.touch_ptr[i + 1]; p++) {
+          const NearPair &P = T.pairs[p];
+          const int k = P.oi[0] == r ? 0 : P.oi[1] == r ? 1 : 2;
+          pair_items(p, k);
+        }
+      }
+    } else {
+      for (int p = T.touch_ptr[r]; p < T.touch_ptr[r + 1]; p++) pair_items(p, 0);
     }
   };
-  {
+  const int nthreads =
+      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto on_threads = [&](auto fn) {
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; t++) pool.emplace_back(sort_rows, t);
-    sort_rows(0);
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(fn, t);
+    fn(0);
     for (auto &th : pool) th.join();
-  }
+  };
+  // pass 1: items and distinct columns per row (a visit stamp per column)
+  std::vector<int> nitem(out.nrows), ncol(out.nrows);
+  on_threads([&](int tid) {
+    std::vector<int> stamp(out.ncols, -1);
+    for (int r = tid; r < out.nrows; r += nthreads) {
+      int ni = 0, nc = 0;
+      for_items(r, [&](int col, int64_t) {
+        ni++;
+        if (stamp[col] != r) { stamp[col] = r; nc++; }
+      });
+      nitem[r] = ni;
+      ncol[r] = nc;
+    }
+  });
+  std::vector<int> cnt(out.nrows + 1, 0);
   out.row_ptr.assign(out.nrows + 1, 0);
-  for (int r = 0; r < out.nrows; r++) out.row_ptr[r + 1] = out.row_ptr[r] + ncol[r];
+  for (int r = 0; r < out.nrows; r++) {
+    cnt[r + 1] = cnt[r] + nitem[r];
+    out.row_ptr[r + 1] = out.row_ptr[r] + ncol[r];
+  }
   const int nnz = out.row_ptr[out.nrows];
   out.col.resize(nnz);
   out.contrib_ptr.resize(nnz + 1);
-  out.contrib.resize(items.size());
-  // the contribution list is the sorted items in order; entry starts are
-  // the first item of each column run
-  for (size_t i = 0; i < items.size(); i++) out.contrib[i] = items[i].c;
-  for (int r = 0; r < out.nrows; r++) {
-    int e = out.row_ptr[r];
-    for (int i = cnt[r]; i < cnt[r + 1]; i++)
-      if (i == cnt[r] || items[i].col != items[i - 1].col) {
-        out.col[e] = items[i].col;
-        out.contrib_ptr[e] = i;
-        e++;
+  out.contrib.resize(cnt[out.nrows]);
+  // pass 2: gather, sort and emit every row; entry starts are the first
+  // item of each column run
+  on_threads([&](int tid) {
+    std::vector<Item> items;
+    for (int r = tid; r < out.nrows; r += nthreads) {
+      items.clear();
+      for_items(r, [&](int col, int64_t c) { items.push_back(Item{col, c}); });
+      std::sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+        return x.col != y.col ? x.col < y.col : x.c < y.c;
+      });
+      int e = out.row_ptr[r];
+      for (size_t i = 0; i < items.size(); i++) {
+        out.contrib[cnt[r] + i] = items[i].c;
+        if (i == 0 || items[i].col != items[i - 1].col) {
+          out.col[e] = items[i].col;
+          out.contrib_ptr[e] = cnt[r] + (int)i;
+          e++;
+        }
       }
-  }
-  out.contrib_ptr[nnz] = (int)items.size();
+    }
+  });
+  out.contrib_ptr[nnz] = cnt[out.nrows];
 }
 
 // The surface gradient of the hat functions solves the 2x2 metric system of
