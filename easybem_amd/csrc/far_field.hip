@@ -456,17 +456,21 @@ struct SymLayout {
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
 // barrier).
 //
-// DL: the double layer (x - y).n_y / r^3 with the same point sets.  Rule
+// DK = 1: the double layer (x - y).n_y / r^3 with the same point sets.  Rule
 // points lie in their triangle's plane, so for target point q of triangle i
-// and source point b of triangle j, (x_q - y_b).n_j = D.n_j + c_q m_k.n_j
-// does not depend on b, and (y_b - x_q).n_i = -D.n_i + c_b m'_l.n_i not on
-// q: each direction is one FMA per point pair (strength times r^-3) plus
-// one per point and step for that factor.  The strengths are the single
-// layer's (w J u_h / 4 pi per point).
-template <typename R, int NT, int NR, bool DL = false>
+// and source point b of triangle j, tf_q = (x_q - y_b).n_j = D.n_j + c_q
+// m_k.n_j does not depend on b, and sf_b = (y_b - x_q).n_i = -D.n_i + c_b
+// m'_l.n_i not on q: each direction is one FMA per point pair (strength
+// times r^-3) plus one per point and step for that factor.  DK = 2: the
+// adjoint double layer (y - x).n_x / r^3, whose factors are the same two
+// with the roles swapped: target point q takes sigma_b sf_b / r^3, source
+// point b takes sigma_q tf_q / r^3, so the strengths are scaled once per
+// step.  The strengths are the single layer's (w J u_h / 4 pi per point).
+template <typename R, int NT, int NR, int DK = 0>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                                           R *__restrict__ lds)
 {
+  constexpr bool DL = DK != 0, ADJ = DK == 2;
   static_assert(!DL || NR == 1, "double layer: one right-hand side");
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
@@ -589,7 +593,8 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   // source point, then both directions' sums.  Double layer: it[q] gathers
   // sigma_b / r^3 for target point q over the step's source points, and
   // source point b's sum of sigma_q / r^3 is scaled by its factor sfb.
-  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag, R *it, R sfb) {
+  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag, R *it, R sfb,
+                          const R *sqs) {
     constexpr bool SYM = decltype(sym_tag)::value;
     R ri[NT];
     if constexpr (F64) {
@@ -606,7 +611,15 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
       for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
     }
-    if constexpr (DL) {
+    if constexpr (ADJ) {
+      const R sg = sig[0] * sfb;
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        const R r3 = ri[q] * ri[q] * ri[q];
+        acc[q][0] = fma(sg, r3, acc[q][0]);
+        if constexpr (SYM) S[b][0] = fma(sqs[q], r3, S[b][0]);
+      }
+    } else if constexpr (DL) {
       R is = R(0);
 #pragma unroll
       for (int q = 0; q < NT; q++) {
@@ -641,7 +654,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         const R Dx = Cx - cur.v[0], Dy = Cy - cur.v[1], Dz = Cz - cur.v[2];
         R r2[1];
         r2[0] = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
-        finish_point(r2, cur.v + 3, 0, sym_tag, nullptr, R(0));
+        finish_point(r2, cur.v + 3, 0, sym_tag, nullptr, R(0), nullptr);
 #pragma unroll
         for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
       } else {
@@ -670,8 +683,9 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         R B[NT];
 #pragma unroll
         for (int q = 0; q < NT; q++) B[q] = q < C0 ? R(0) : fma(rc[q], U[(q - C0) % 3], E[q]);
-        // double layer: tf[q] = (x_q - y).n_j and sf[b] = (y_b - x).n_i
-        R it[NT], tf[NT], sf[NT];
+        // double layer: tf[q] = (x_q - y).n_j and sf[b] = (y_b - x).n_i;
+        // adjoint: own strengths times tf
+        R it[NT], tf[NT], sf[NT], sqs[NT];
         if constexpr (DL) {
           const R *nj = g + Lay::NRM;
           const R dt = hps * fma(Dx, nj[0], fma(Dy, nj[1], Dz * nj[2]));
@@ -688,6 +702,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
             tf[q] = q < C0 ? dt : fma(hc[q], Gt[(q - C0) % 3], dt);
             sf[q] = q < C0 ? ds : fma(hc[q], Gs[(q - C0) % 3], ds);
             it[q] = R(0);
+            if constexpr (ADJ) sqs[q] = sq[q][0] * tf[q];
           }
         }
 #pragma unroll
@@ -702,9 +717,9 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                     : b < C0 ? Ab + B[q]
                              : fma(a.kap[2 * oq + ob], W[kq][lb], Ab + B[q]);
           }
-          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag, it, DL ? sf[b] : R(0));
+          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag, it, DL ? sf[b] : R(0), sqs);
         }
-        if constexpr (DL) {
+        if constexpr (DL && !ADJ) {
 #pragma unroll
           for (int q = 0; q < NT; q++) acc[q][0] = fma(tf[q], it[q], acc[q][0]);
         }
@@ -765,11 +780,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   }
 }
 
-template <typename R, int NT, int NR, bool DL = false>
+template <typename R, int NT, int NR, int DK = 0>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
-  __shared__ __align__(16) R lds[SymLayout<NT, NR, DL>::LDS];
-  sym_block<R, NT, NR, DL>(a, (int)blockIdx.x, lds);
+  __shared__ __align__(16) R lds[SymLayout<NT, NR, DK != 0>::LDS];
+  sym_block<R, NT, NR, DK>(a, (int)blockIdx.x, lds);
 }
 
 // Persistent multi-mesh form of the symmetric schedule: the workgroups of
@@ -1210,16 +1225,16 @@ size_t far_lds(const ebh_op *op, int R_bytes)
   return (size_t)op->tile * op->rule.nt * rec * R_bytes;
 }
 
-template <typename R, int NR, bool DL = false>
+template <typename R, int NR, int DK = 0>
 int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
 {
   const dim3 blk(256);
   switch (nt) {
-  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR, DL>), grid, blk, 0, s, a); break;
-  case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR, DL>), grid, blk, 0, s, a); break;
-  case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR, DL>), grid, blk, 0, s, a); break;
-  case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR, DL>), grid, blk, 0, s, a); break;
-  case 7: hipLaunchKernelGGL((k_far_sym<R, 7, NR, DL>), grid, blk, 0, s, a); break;
+  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR, DK>), grid, blk, 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR, DK>), grid, blk, 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR, DK>), grid, blk, 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR, DK>), grid, blk, 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_sym<R, 7, NR, DK>), grid, blk, 0, s, a); break;
   default:
     set_error("no pair-symmetric kernel for rule size " + std::to_string(nt));
     return EBH_ERR_UNSUPPORTED;
@@ -1236,9 +1251,11 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
   int nblk = 0;
   for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
   const dim3 grid((unsigned)(nblk * op->ksplit));
-  if (op->d.kernel == EBH_LAPLACE_DL) {
-    // the symmetric double layer exists in FP64 only (sym_eligible)
-    if constexpr (sizeof(R) == 8) return launch_sym<R, 1, true>(op->rule.nt, grid, s, a);
+  if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) {
+    // the symmetric double layers exist in FP64 only (sym_eligible)
+    if constexpr (sizeof(R) == 8)
+      return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, a)
+                                            : launch_sym<R, 1, 2>(op->rule.nt, grid, s, a);
     set_error("no FP32 pair-symmetric double layer");
     return EBH_ERR_UNSUPPORTED;
   }
@@ -1417,12 +1434,12 @@ bool sym_eligible(const ebh_op *op)
   const double ne = op->mesh->T.ne;
   const double bytes =
       std::ceil(ne / 64.0) * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
-  // the single layer with equal bases, or the FP64 double layer (its
-  // point sums are weighted by the test functions and its strengths by the
-  // trial density, so the bases may differ)
+  // the single layer with equal bases, or the FP64 double layer or its
+  // adjoint (their point sums are weighted by the test functions and their
+  // strengths by the trial density, so the bases may differ)
   const bool sl = op->d.kernel == EBH_LAPLACE_SL && op->d.test_basis == op->d.trial_basis;
-  const bool dl = op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 &&
-                  op->nr_max == 1;
+  const bool dl = (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) &&
+                  op->d.far_precision == EBH_FP64 && op->nr_max == 1;
   return (sl || dl) && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
          bytes <= kSymPartialMax;
 }
@@ -1433,7 +1450,7 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   if (sym) {
     if (!sym_eligible(op)) {
       set_error("the pair-symmetric schedule needs a Laplace single layer with equal "
-                "test and trial bases or the FP64 double layer, a median-symmetric "
+                "test and trial bases or an FP64 double layer, a median-symmetric "
                 "far rule and partial sums of at most 4 GiB");
       return EBH_ERR_UNSUPPORTED;
     }

@@ -6,7 +6,8 @@ standard workloads, on the GPU this runs on:
 
 Workloads: icosphere levels 3-6 (single layer P0 and P1 with the 7-point
 and the 1-point far rule -- the latter is the Calderon preconditioners' --,
-double layer P1, both far-field precisions), the 256x128 torus with the
+both far-field precisions; the FP64 double layer with P0 and P1 tests and
+its adjoint), the 256x128 torus with the
 Kelvin single layer, and the 64-RHS single layer on level 6.  Each line of the output keys a
 shape on device, operator kind, precision, rule size, test basis, RHS block
 and mesh size class; ``easybem_amd._lib`` loads the file when the library is
@@ -28,7 +29,8 @@ def workloads():
         yield f"icosphere L{lev}", P, T, [
             ("V", b, b, p, 1, r) for r in (4, 0) for b in (0, 1)
             for p in ("fp32", "fp64")] + [
-            ("K", 1, 1, "fp64", 1, 4), ("K", 0, 1, "fp64", 1, 4)]
+            ("K", 1, 1, "fp64", 1, 4), ("K", 0, 1, "fp64", 1, 4),
+            ("Kt", 1, 0, "fp64", 1, 4)]
     P, T = meshes.icosphere(6)
     yield "icosphere L6, 64 RHS", P, T, [("V", 0, 0, p, 64, 4)
                                          for p in ("fp32", "fp64")]

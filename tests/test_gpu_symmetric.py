@@ -1,5 +1,5 @@
 """Pair-symmetric far-field schedule (k_far_sym) of the single layer and of
-the FP64 double layer.
+the FP64 double layer and its adjoint.
 
 The symmetric schedule sums every unordered pair of 64-triangle groups once
 and uses each rsqrt for both directions, so it computes the same matrix-free
@@ -261,4 +261,39 @@ def test_symmetric_double_layer_full_size_l5(L):
     op.set_schedule("symmetric", 1)
     err = rel(op(u), y0)
     print("K L5 symmetric vs chunked", err)
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("trial", [0, 1])
+def test_symmetric_adjoint_double_layer(L, mesh, trial):
+    """FP64 adjoint double layer (P1 test, P0 or P1 trial) on the symmetric
+    schedule against the chunked schedule (the chunked adjoint is checked
+    against the reference's K^T in the hypersingular tests)."""
+    P, T = MESHES[mesh]()
+    print(f"case Kt mesh={mesh} ne={T.shape[0]} trial={trial}", flush=True)
+    x = _x(trial, P, T, 9)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "Kt", 1, trial, "fp64", line_rule=4, tri_rule=4)
+    op.set_schedule("chunked")
+    y0 = op(x)
+    op.set_schedule("symmetric")
+    assert op.schedule == "symmetric"
+    y = op(x)
+    print("rel err vs chunked", rel(y, y0))
+    assert rel(y, y0) <= 1e-13
+
+
+@pytest.mark.parametrize("split", [1, 8])
+def test_symmetric_adjoint_double_layer_l5(L, split):
+    """The Neumann right-hand side operator at 20480 triangles."""
+    P, T = meshes.icosphere(5)
+    m = L.Mesh(P, T)
+    t = np.random.default_rng(14).uniform(-1, 1, T.shape[0])
+    op = L.Operator(m, "Kt", 1, 0, "fp64")
+    op.set_schedule("chunked")
+    y0 = op(t)
+    op.set_schedule("symmetric", split)
+    err = rel(op(t), y0)
+    print("Kt L5 symmetric vs chunked", err)
     assert err <= 1e-13
