@@ -404,9 +404,10 @@ def precond_spai(V: Operator, rings=None, max_rhs=1):
     return op
 
 
-def precond_calderon_hs(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0):
-    """Dual Calderon preconditioner M_L^-1 V M_L^-1 for the P1
-    hypersingular operator (ebh_precond_calderon_hs), as an Operator."""
+def precond_calderon_hs(mesh: Mesh, precision="fp32", line_rule=3, tri_rule=1):
+    """Dual Calderon preconditioner Q V Q for the P1 hypersingular
+    operator, Q a 3-step Chebyshev approximation of the consistent P1 mass
+    inverse (ebh_precond_calderon_hs), as an Operator."""
     L = lib()
     L.ebh_precond_calderon_hs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_void_p)]

@@ -39,7 +39,9 @@ size_t ebh_op::bytes() const
          normals.bytes() + src32.bytes() + src64.bytes() + partial.bytes() +
          tri_out.bytes() + mgeom32.bytes() + msig32.bytes() + mgeom64.bytes() +
          msig64.bytes() + src3_32.bytes() + src3_64.bytes() + tsl32.bytes() +
-         tsl64.bytes() + tc32.bytes() + dinv.bytes();
+         tsl64.bytes() + tc32.bytes() + dinv.bytes() + mass_ptr.bytes() +
+         mass_col.bytes() + mass_val.bytes() + cheb_r.bytes() + cheb_d.bytes() +
+         cheb_t.bytes();
 }
 
 extern "C" {

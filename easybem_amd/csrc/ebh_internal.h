@@ -143,9 +143,14 @@ struct ebh_op {
   // diag(dinv) (D + stab m m^T) diag(dinv), dinv = 1 / lumped node areas
   bool hs_scale = false;
   ebh::DevBuf<double> dinv;
-  // single layer applied between the same inverse lumped masses (the
-  // Calderon preconditioner of the hypersingular operator); uses hsw
+  // single layer applied between two inverse masses (the Calderon
+  // preconditioner of the hypersingular operator); uses hsw.  pcheb > 0:
+  // the consistent P1 mass inverse by pcheb Chebyshev steps (CSR of the
+  // mass matrix, work vectors of np), else diag(dinv)
   bool pscale = false;
+  int pcheb = 0;
+  ebh::DevBuf<int> mass_ptr, mass_col;
+  ebh::DevBuf<double> mass_val, cheb_r, cheb_d, cheb_t;
   // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single
   // layer (inner), Laplace double layer and Kelvin single layer they wrap,
   // and two work buffers
@@ -185,6 +190,12 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
 // y = M x for an EBH_SPARSE_INVERSE operator (nrhs vectors back to back)
 int spai_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s, bool timed);
+// y = A x for an FP64 CSR matrix of n rows (nrhs vectors back to back)
+int csr_spmv(int n, const int *ptr, const int *col, const double *val, const double *x,
+             double *y, int nrhs, hipStream_t s);
+// z = Q v with Q the Chebyshev approximation of the P1 mass inverse of a
+// hypersingular Calderon preconditioner (z may alias v)
+int mass_inv_apply(ebh_op *op, const double *v, double *z, hipStream_t s);
 int sym_default_split(int ng);
 int hs_build(ebh_op *op);
 int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,

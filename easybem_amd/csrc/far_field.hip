@@ -1698,9 +1698,13 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   if (op->d.kernel == EBH_SPARSE_INVERSE) return spai_apply(op, x, y, 1, s, timed);
   if (op->inner) return composite_apply(op, x, y, 1, s, timed);
   if (op->pscale) {
-    hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
-                       op->ncols, op->dinv.p, x, op->hsw.p);
-    EBH_HIP(hipGetLastError());
+    if (op->pcheb) {
+      EBH_TRY(mass_inv_apply(op, x, op->hsw.p, s));
+    } else {
+      hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
+                         op->ncols, op->dinv.p, x, op->hsw.p);
+      EBH_HIP(hipGetLastError());
+    }
     x = op->hsw.p;
   }
   const int ne = op->mesh->T.ne;
@@ -1715,9 +1719,13 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
   EBH_TRY(op_finish(op, x, y, s));
   if (op->pscale) {
-    hipLaunchKernelGGL(k_scale_vec, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
-                       op->nrows, op->dinv.p, y, y);
-    EBH_HIP(hipGetLastError());
+    if (op->pcheb) {
+      EBH_TRY(mass_inv_apply(op, y, y, s));
+    } else {
+      hipLaunchKernelGGL(k_scale_vec, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
+                         op->nrows, op->dinv.p, y, y);
+      EBH_HIP(hipGetLastError());
+    }
   }
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;

@@ -96,7 +96,65 @@ __global__ __launch_bounds__(256) void k_hs_gather(
   y[(size_t)r * np + n] = dinv ? acc * dinv[n] : acc;
 }
 
+// Chebyshev iteration for M z = v preconditioned by the lumped mass M_L
+// (Saad, Iterative Methods for Sparse Linear Systems, alg. 12.1): for P1
+// elements the eigenvalues of M_L^-1 M lie in [1/4, 1] on any triangle
+// mesh (each element's are 1, 1/4, 1/4).  A fixed number of steps is a
+// fixed polynomial in M_L^-1 M times M_L^-1, a linear operator.
+constexpr double kChebLo = 0.25, kChebHi = 1.0;
+
+// r = v, d = M_L^-1 v / theta, z = d
+__global__ void k_cheb_init(int n, const double *v, const double *__restrict__ dinv,
+                            double inv_theta, double *__restrict__ r,
+                            double *__restrict__ d, double *z)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double vi = v[i];
+  const double di = dinv[i] * vi * inv_theta;
+  r[i] = vi;
+  d[i] = di;
+  z[i] = di;  // z may alias v: read above, by the same thread
+}
+
+// r -= t (t = M d), d = a d + b M_L^-1 r, z += d
+__global__ void k_cheb_step(int n, const double *__restrict__ t,
+                            const double *__restrict__ dinv, double a, double b,
+                            double *__restrict__ r, double *__restrict__ d,
+                            double *__restrict__ z)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ri = r[i] - t[i];
+  const double di = fma(a, d[i], b * dinv[i] * ri);
+  r[i] = ri;
+  d[i] = di;
+  z[i] += di;
+}
+
 }  // namespace
+
+int mass_inv_apply(ebh_op *op, const double *v, double *z, hipStream_t s)
+{
+  const int n = op->nrows;
+  const dim3 grid((n + 255) / 256), blk(256);
+  const double th = 0.5 * (kChebHi + kChebLo), de = 0.5 * (kChebHi - kChebLo);
+  const double sig = th / de;
+  double rho = 1.0 / sig;
+  hipLaunchKernelGGL(k_cheb_init, grid, blk, 0, s, n, v, op->dinv.p, 1.0 / th,
+                     op->cheb_r.p, op->cheb_d.p, z);
+  EBH_HIP(hipGetLastError());
+  for (int k = 1; k < op->pcheb; k++) {
+    EBH_TRY(csr_spmv(n, op->mass_ptr.p, op->mass_col.p, op->mass_val.p, op->cheb_d.p,
+                     op->cheb_t.p, 1, s));
+    const double rn = 1.0 / (2.0 * sig - rho);
+    hipLaunchKernelGGL(k_cheb_step, grid, blk, 0, s, n, op->cheb_t.p, op->dinv.p,
+                       rn * rho, 2.0 * rn / de, op->cheb_r.p, op->cheb_d.p, z);
+    EBH_HIP(hipGetLastError());
+    rho = rn;
+  }
+  return EBH_OK;
+}
 
 int hs_build(ebh_op *op)
 {
@@ -217,6 +275,9 @@ extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int li
   *out = nullptr;
   const Topology &T = mesh->T;
   std::vector<double> m((size_t)T.np, 0.0);
+  struct Trip { int r, c; double v; };
+  std::vector<Trip> trip;
+  trip.reserve((size_t)9 * T.ne);
   for (int i = 0; i < T.ne; i++) {
     const double *P0 = &T.nodes[3 * T.tri[3 * i]];
     const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
@@ -227,8 +288,29 @@ extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int li
                  c2 = u[0] * v[1] - u[1] * v[0];
     const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
     for (int k = 0; k < 3; k++) m[T.tri[3 * i + k]] += J / 6.0;
+    // consistent P1 mass: |T|/12 (1 + delta_ab) = J/24 (1 + delta_ab)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        trip.push_back({T.tri[3 * i + a], T.tri[3 * i + b], (a == b ? 2.0 : 1.0) * J / 24.0});
   }
   for (double &v : m) v = 1.0 / v;
+  // CSR of the mass matrix: triplets sorted by (row, column), equal ones
+  // summed in triangle order
+  std::stable_sort(trip.begin(), trip.end(), [](const Trip &x, const Trip &y) {
+    return x.r != y.r ? x.r < y.r : x.c < y.c;
+  });
+  std::vector<int> mptr(T.np + 1, 0), mcol;
+  std::vector<double> mval;
+  for (size_t k = 0; k < trip.size(); k++) {
+    if (k > 0 && trip[k].r == trip[k - 1].r && trip[k].c == trip[k - 1].c) {
+      mval.back() += trip[k].v;
+      continue;
+    }
+    mcol.push_back(trip[k].c);
+    mval.push_back(trip[k].v);
+    mptr[trip[k].r + 1]++;
+  }
+  for (int r = 0; r < T.np; r++) mptr[r + 1] += mptr[r];
   ebh_op_desc d{};
   d.kernel = EBH_LAPLACE_SL;
   d.test_basis = d.trial_basis = EBH_P1;
@@ -240,11 +322,18 @@ extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int li
   EBH_TRY(ebh_op_create(mesh, &d, &op));
   int rc = op->dinv.upload(m);
   if (rc == EBH_OK) rc = op->hsw.alloc(T.np);
+  if (rc == EBH_OK) rc = op->mass_ptr.upload(mptr);
+  if (rc == EBH_OK) rc = op->mass_col.upload(mcol);
+  if (rc == EBH_OK) rc = op->mass_val.upload(mval);
+  if (rc == EBH_OK) rc = op->cheb_r.alloc(T.np);
+  if (rc == EBH_OK) rc = op->cheb_d.alloc(T.np);
+  if (rc == EBH_OK) rc = op->cheb_t.alloc(T.np);
   if (rc != EBH_OK) {
     ebh_op_destroy(op);
     return rc;
   }
   op->pscale = true;
+  op->pcheb = 3;
   *out = op;
   return EBH_OK;
 }

@@ -222,11 +222,12 @@ struct ebh_server {
     EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
     EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
     if (neu && !o.no_precond) {
-      // dual Calderon preconditioners with a 1-point far rule (V for D):
-      // D's small eigenvalues are its smooth modes, which a local inverse
-      // cannot reach (its sparse inverse needs more iterations from L5 up)
-      EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP64, 2, 0, &e->P64));
-      if (o.mixed) EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP32, 2, 0, &e->P32));
+      // dual Calderon preconditioners Q V Q (Q the Chebyshev inverse of the
+      // P1 mass) with a 3-point far and 4-point Gauss near rule: D's small
+      // eigenvalues are its smooth modes, which a local inverse cannot reach
+      // (its sparse inverse needs more iterations from L5 up)
+      EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP64, 3, 1, &e->P64));
+      if (o.mixed) EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP32, 3, 1, &e->P32));
       EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
     } else if (!o.no_precond) {
       // near-field sparse approximate inverse of V (3 node rings for P1, 2

@@ -128,3 +128,73 @@ def test_adjoint_double_layer_rejects_fp32_and_p0_test(L):
         L.Operator(m, "Kt", 1, 0, "fp32")
     with pytest.raises(L.EbhError):
         L.Operator(m, "Kt", 0, 0, "fp64")
+
+
+# ------------------------------------- dual Calderon preconditioner of D
+def _p1_mass(P, T):
+    n = P.shape[0]
+    M = np.zeros((n, n))
+    for tri in T - 1:
+        p = P[tri]
+        area = 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+        for a in range(3):
+            for c in range(3):
+                M[tri[a], tri[c]] += area / 12 * (2 if a == c else 1)
+    return M, M.sum(axis=1)
+
+
+def _cheb_mass_inverse(M, ML, v, steps=3, lo=0.25, hi=1.0):
+    """Chebyshev iteration for M z = v preconditioned by diag(ML) (Saad,
+    alg. 12.1); the eigenvalues of diag(ML)^-1 M lie in [1/4, 1]."""
+    th, de = (hi + lo) / 2, (hi - lo) / 2
+    sig = th / de
+    rho = 1 / sig
+    r = v.copy()
+    d = v / ML / th
+    z = d.copy()
+    for _ in range(steps - 1):
+        r = r - M @ d
+        rn = 1 / (2 * sig - rho)
+        d = rn * rho * d + 2 * rn / de * (r / ML)
+        z = z + d
+        rho = rn
+    return z
+
+
+def test_calderon_hs_is_chebyshev_mass_sandwich(L):
+    """P = Q V Q with Q three Chebyshev steps for the consistent P1 mass,
+    against the same polynomial in numpy with the dense mass matrix."""
+    P, T, _ = meshes.perturbed_icosphere(2, 5)
+    m = L.Mesh(P, T)
+    Pc = L.precond_calderon_hs(m, "fp64")
+    V = L.Operator(m, "V", 1, 1, "fp64", line_rule=3, tri_rule=1)
+    M, ML = _p1_mass(P, T)
+    ev = np.linalg.eigvals(M / ML[:, None]).real
+    assert ev.min() >= 0.25 - 1e-12 and ev.max() <= 1 + 1e-12
+    x = np.random.default_rng(15).uniform(-1, 1, P.shape[0])
+    y = Pc(x)
+    yr = _cheb_mass_inverse(M, ML, V(_cheb_mass_inverse(M, ML, x)))
+    err = rel(y, yr)
+    print("Q V Q vs numpy", err)
+    assert err <= 1e-13
+    # Q is close to M^-1: three steps leave at most 2 / T_3(5/3) = 0.15
+    # of the error in the M_L-weighted norm
+    q = _cheb_mass_inverse(M, ML, x)
+    z = np.linalg.solve(M, x)
+    assert rel(q, z) <= 0.3
+
+
+def test_calderon_hs_neumann_iterations(L):
+    """FP64 GMRES(50) on D + m m^T of an L4 serving mesh: the preconditioned
+    solve converges in few iterations."""
+    P, T, x0 = meshes.perturbed_icosphere(4, 3)
+    m = L.Mesh(P, T)
+    D64 = L.Operator(m, "D", 1, 1, "fp64", stab=1.0)
+    u = 1.0 / np.linalg.norm(P - x0, axis=1)
+    b = D64(u)
+    s = L.Solver(D64)
+    s.set_precond(L.precond_calderon_hs(m, "fp64"))
+    x, st = s.solve(b, tol=1e-8)
+    print("iterations", st["iters"])
+    assert st["converged"] and st["iters"] <= 10
+    assert rel(D64(x), b) <= 1e-8

@@ -162,7 +162,7 @@ def test_dual_calderon_cuts_neumann_iterations():
     s.set_precond(L.precond_calderon_hs(m, "fp64"), L.precond_calderon_hs(m, "fp32"))
     _, st1 = s.solve(b, tol=1e-8, mixed=True)
     print("iterations", st0["iters"], st1["iters"])
-    assert st1["relres"] <= 1e-8 and st1["iters"] < 0.75 * st0["iters"]
+    assert st1["relres"] <= 1e-8 and st1["iters"] <= 0.4 * st0["iters"]
 
 
 def test_dirichlet_p1_server_matches_preconditioned_direct_solve():
