@@ -262,10 +262,11 @@ def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
                     "solve_ms_p50": float(np.median([r["solve_ms"] for r in sel])),
                     "iters_max": int(max(r["iters"] for r in sel))}
     return {"requests": n_requests, "completed": len(ok), "levels": list(levels),
-            "formulation": "Dirichlet: P1 flux, V with its near-field sparse "
-                           "approximate inverse (3 node rings); Neumann: P1 trace, "
-                           "D + m m^T with the dual Calderon preconditioner; "
-                           "mixed-precision refinement to 1e-8",
+            "formulation": "Dirichlet: P1 flux, V with the Calderon "
+                           "preconditioner Q D Q; Neumann: P1 trace, D + m m^T "
+                           "with the dual Calderon preconditioner Q V Q (Q a "
+                           "Chebyshev inverse of the P1 mass); mixed-precision "
+                           "refinement to 1e-8",
             "arrival_rate_per_s": rate, "slots": slots,
             "solves_per_s": len(ok) / wall,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -313,10 +314,11 @@ def main():
                     help="serving stream of N requests on icosphere levels 4-6 "
                          "(the default run serves 24 requests on levels 4-5)")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--precond", choices=("spai", "calderon", "none"), default="spai",
+    ap.add_argument("--precond", choices=("calderon", "spai", "none"), default="calderon",
                     help="right preconditioner of the GMRES solves (default: the "
-                         "near-field sparse approximate inverse of V; calderon: the "
-                         "hypersingular operator with a 1-point far rule)")
+                         "hypersingular operator with a 1-point far rule between "
+                         "Chebyshev inverses of the P1 mass; spai: the near-field "
+                         "sparse approximate inverse of V)")
     ap.add_argument("--retune", action="store_true",
                     help="run the launch-shape autotuner on the bench operators "
                          "instead of using the packaged table")
@@ -382,16 +384,18 @@ def main():
               "quadrature), applied in FP64; the residual is that of the "
               "unpreconditioned system")
     elif args.precond == "calderon":
-        # M_L^-1 (D + m m^T / (4|Gamma|)) M_L^-1: D through curls, 3-point
-        # Gauss near field, 1-point far rule; FP64 for the FP64 operator
+        # Q (D + m m^T / (4|Gamma|)) Q: D through curls, 3-point Gauss near
+        # field, 1-point far rule; Q five Chebyshev steps for the P1 mass
+        # inverse; FP64 for the FP64 operator
         P64 = L.precond_calderon(mesh, "fp64", line_rule=2, tri_rule=0)
         P32 = L.precond_calderon(mesh, "fp32", line_rule=2, tri_rule=0)
         solver.set_precond(P64, P32)
-        pc = ("Calderon, right: M_L^-1 (D + m m^T/(4|Gamma|)) M_L^-1, D the "
-              "hypersingular operator through curls with a 1-point far rule "
-              "and 3-point Gauss near rule (FP32 with the FP32 operator, FP64 "
-              "with the FP64 one); the residual is that of the unpreconditioned "
-              "system")
+        pc = ("Calderon, right: Q (D + m m^T/(4|Gamma|)) Q, D the hypersingular "
+              "operator through curls with a 1-point far rule and 3-point Gauss "
+              "near rule, Q five Chebyshev steps for the consistent P1 mass "
+              "inverse (lumped-mass preconditioned; FP32 D with the FP32 "
+              "operator, FP64 with the FP64 one); the residual is that of the "
+              "unpreconditioned system")
 
     def solve(mixed=True, s=None):
         L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))

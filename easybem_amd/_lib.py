@@ -365,8 +365,9 @@ class Solver:
 
 def precond_calderon(mesh: Mesh, precision="fp32", line_rule=2, tri_rule=0,
                      max_rhs=1):
-    """Calderon preconditioner M_L^-1 (D + m m^T/(4|Gamma|)) M_L^-1 for the
-    P1 single layer (ebh_precond_calderon), as an Operator."""
+    """Calderon preconditioner Q (D + m m^T/(4|Gamma|)) Q for the P1 single
+    layer, Q a 5-step Chebyshev approximation of the consistent P1 mass
+    inverse (ebh_precond_calderon), as an Operator."""
     L = lib()
     L.ebh_precond_calderon.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,
                                        C.c_int, C.POINTER(C.c_void_p)]

@@ -41,7 +41,7 @@ size_t ebh_op::bytes() const
          msig64.bytes() + src3_32.bytes() + src3_64.bytes() + tsl32.bytes() +
          tsl64.bytes() + tc32.bytes() + dinv.bytes() + mass_ptr.bytes() +
          mass_col.bytes() + mass_val.bytes() + cheb_r.bytes() + cheb_d.bytes() +
-         cheb_t.bytes();
+         cheb_t.bytes() + chebx.bytes();
 }
 
 extern "C" {

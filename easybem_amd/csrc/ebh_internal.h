@@ -150,7 +150,7 @@ struct ebh_op {
   bool pscale = false;
   int pcheb = 0;
   ebh::DevBuf<int> mass_ptr, mass_col;
-  ebh::DevBuf<double> mass_val, cheb_r, cheb_d, cheb_t;
+  ebh::DevBuf<double> mass_val, cheb_r, cheb_d, cheb_t, chebx;
   // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single
   // layer (inner), Laplace double layer and Kelvin single layer they wrap,
   // and two work buffers
@@ -195,7 +195,7 @@ int csr_spmv(int n, const int *ptr, const int *col, const double *val, const dou
              double *y, int nrhs, hipStream_t s);
 // z = Q v with Q the Chebyshev approximation of the P1 mass inverse of a
 // hypersingular Calderon preconditioner (z may alias v)
-int mass_inv_apply(ebh_op *op, const double *v, double *z, hipStream_t s);
+int mass_inv_apply(ebh_op *op, const double *v, double *z, int nrhs, hipStream_t s);
 int sym_default_split(int ng);
 int hs_build(ebh_op *op);
 int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,

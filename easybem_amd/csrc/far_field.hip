@@ -1699,7 +1699,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   if (op->inner) return composite_apply(op, x, y, 1, s, timed);
   if (op->pscale) {
     if (op->pcheb) {
-      EBH_TRY(mass_inv_apply(op, x, op->hsw.p, s));
+      EBH_TRY(mass_inv_apply(op, x, op->hsw.p, 1, s));
     } else {
       hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
                          op->ncols, op->dinv.p, x, op->hsw.p);
@@ -1720,7 +1720,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   EBH_TRY(op_finish(op, x, y, s));
   if (op->pscale) {
     if (op->pcheb) {
-      EBH_TRY(mass_inv_apply(op, y, y, s));
+      EBH_TRY(mass_inv_apply(op, y, y, 1, s));
     } else {
       hipLaunchKernelGGL(k_scale_vec, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
                          op->nrows, op->dinv.p, y, y);

@@ -102,19 +102,24 @@ __global__ __launch_bounds__(256) void k_hs_gather(
 // mesh (each element's are 1, 1/4, 1/4).  A fixed number of steps is a
 // fixed polynomial in M_L^-1 M times M_L^-1, a linear operator.
 constexpr double kChebLo = 0.25, kChebHi = 1.0;
+// Chebyshev steps of the two Calderon preconditioners (odd counts suit
+// the single layer's: FP64 GMRES(50) on the L5 icosphere takes 11, 6, 5,
+// 4, 5 iterations with 2..6 steps, 4 with the exact mass inverse)
+constexpr int kCalderonVSteps = 5, kCalderonHsSteps = 3;
 
-// r = v, d = M_L^-1 v / theta, z = d
+// r = v, d = M_L^-1 v / theta, z = d; blockIdx.y: right-hand side
 __global__ void k_cheb_init(int n, const double *v, const double *__restrict__ dinv,
                             double inv_theta, double *__restrict__ r,
                             double *__restrict__ d, double *z)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double vi = v[i];
+  const size_t k = (size_t)blockIdx.y * n + i;
+  const double vi = v[k];
   const double di = dinv[i] * vi * inv_theta;
-  r[i] = vi;
-  d[i] = di;
-  z[i] = di;  // z may alias v: read above, by the same thread
+  r[k] = vi;
+  d[k] = di;
+  z[k] = di;  // z may alias v: read above, by the same thread
 }
 
 // r -= t (t = M d), d = a d + b M_L^-1 r, z += d
@@ -125,19 +130,20 @@ __global__ void k_cheb_step(int n, const double *__restrict__ t,
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double ri = r[i] - t[i];
-  const double di = fma(a, d[i], b * dinv[i] * ri);
-  r[i] = ri;
-  d[i] = di;
-  z[i] += di;
+  const size_t k = (size_t)blockIdx.y * n + i;
+  const double ri = r[k] - t[k];
+  const double di = fma(a, d[k], b * dinv[i] * ri);
+  r[k] = ri;
+  d[k] = di;
+  z[k] += di;
 }
 
 }  // namespace
 
-int mass_inv_apply(ebh_op *op, const double *v, double *z, hipStream_t s)
+int mass_inv_apply(ebh_op *op, const double *v, double *z, int nrhs, hipStream_t s)
 {
-  const int n = op->nrows;
-  const dim3 grid((n + 255) / 256), blk(256);
+  const int n = op->mesh->T.np;
+  const dim3 grid((n + 255) / 256, nrhs), blk(256);
   const double th = 0.5 * (kChebHi + kChebLo), de = 0.5 * (kChebHi - kChebLo);
   const double sig = th / de;
   double rho = 1.0 / sig;
@@ -146,13 +152,67 @@ int mass_inv_apply(ebh_op *op, const double *v, double *z, hipStream_t s)
   EBH_HIP(hipGetLastError());
   for (int k = 1; k < op->pcheb; k++) {
     EBH_TRY(csr_spmv(n, op->mass_ptr.p, op->mass_col.p, op->mass_val.p, op->cheb_d.p,
-                     op->cheb_t.p, 1, s));
+                     op->cheb_t.p, nrhs, s));
     const double rn = 1.0 / (2.0 * sig - rho);
     hipLaunchKernelGGL(k_cheb_step, grid, blk, 0, s, n, op->cheb_t.p, op->dinv.p,
                        rn * rho, 2.0 * rn / de, op->cheb_r.p, op->cheb_d.p, z);
     EBH_HIP(hipGetLastError());
     rho = rn;
   }
+  return EBH_OK;
+}
+
+// Consistent P1 mass matrix of the operator's mesh as CSR, the inverse
+// lumped masses, and Chebyshev work vectors for nb right-hand sides; the
+// mass inverse then takes `steps` Chebyshev steps (mass_inv_apply)
+int mass_setup(ebh_op *op, int steps, int nb)
+{
+  const Topology &T = op->mesh->T;
+  struct Trip { int r, c; double v; };
+  std::vector<Trip> trip;
+  trip.reserve((size_t)9 * T.ne);
+  std::vector<double> m((size_t)T.np, 0.0);
+  for (int i = 0; i < T.ne; i++) {
+    const double *P0 = &T.nodes[3 * T.tri[3 * i]];
+    const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
+    const double *P2 = &T.nodes[3 * T.tri[3 * i + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
+    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
+                 c2 = u[0] * v[1] - u[1] * v[0];
+    const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    for (int k = 0; k < 3; k++) m[T.tri[3 * i + k]] += J / 6.0;
+    // consistent P1 mass: |T|/12 (1 + delta_ab) = J/24 (1 + delta_ab)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        trip.push_back({T.tri[3 * i + a], T.tri[3 * i + b], (a == b ? 2.0 : 1.0) * J / 24.0});
+  }
+  for (double &v : m) v = 1.0 / v;
+  // triplets sorted by (row, column), equal ones summed in triangle order
+  std::stable_sort(trip.begin(), trip.end(), [](const Trip &x, const Trip &y) {
+    return x.r != y.r ? x.r < y.r : x.c < y.c;
+  });
+  std::vector<int> mptr(T.np + 1, 0), mcol;
+  std::vector<double> mval;
+  for (size_t k = 0; k < trip.size(); k++) {
+    if (k > 0 && trip[k].r == trip[k - 1].r && trip[k].c == trip[k - 1].c) {
+      mval.back() += trip[k].v;
+      continue;
+    }
+    mcol.push_back(trip[k].c);
+    mval.push_back(trip[k].v);
+    mptr[trip[k].r + 1]++;
+  }
+  for (int r = 0; r < T.np; r++) mptr[r + 1] += mptr[r];
+  const size_t nv = (size_t)T.np * std::max(1, nb);
+  EBH_TRY(op->dinv.upload(m));
+  EBH_TRY(op->mass_ptr.upload(mptr));
+  EBH_TRY(op->mass_col.upload(mcol));
+  EBH_TRY(op->mass_val.upload(mval));
+  EBH_TRY(op->cheb_r.alloc(nv));
+  EBH_TRY(op->cheb_d.alloc(nv));
+  EBH_TRY(op->cheb_t.alloc(nv));
+  op->pcheb = steps;
   return EBH_OK;
 }
 
@@ -196,7 +256,12 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
   }
   const int ne = op->mesh->T.ne, np = op->mesh->T.np;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
-  const double *dinv = op->hs_scale ? op->dinv.p : nullptr;
+  const double *dinv = op->hs_scale && !op->pcheb ? op->dinv.p : nullptr;
+  if (op->hs_scale && op->pcheb) {
+    // Calderon form with the Chebyshev mass inverse Q on both sides
+    EBH_TRY(mass_inv_apply(op, x, op->chebx.p, nrhs, s));
+    x = op->chebx.p;
+  }
   hipLaunchKernelGGL(k_hs_curl, dim3((ne + 255) / 256, nrhs), dim3(256), 0, s, ne, np,
                      op->curl.p, op->mesh->tri.p, dinv, x, op->hsw.p);
   EBH_HIP(hipGetLastError());
@@ -213,6 +278,7 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
                      op->mesh->node_tri.p, op->mesh->node_tri_k.p, op->hsz.p,
                      op->mvec.p, op->hsdot.p, dinv, y);
   EBH_HIP(hipGetLastError());
+  if (op->hs_scale && op->pcheb) EBH_TRY(mass_inv_apply(op, y, y, nrhs, s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }
@@ -259,6 +325,8 @@ extern "C" int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_
   }
   for (double &v : m) v = 1.0 / v;
   if (rc == EBH_OK) rc = op->dinv.upload(m);
+  if (rc == EBH_OK) rc = mass_setup(op, kCalderonVSteps, max_rhs);
+  if (rc == EBH_OK) rc = op->chebx.alloc((size_t)T.np * std::max(1, max_rhs));
   if (rc != EBH_OK) {
     ebh_op_destroy(op);
     return rc;
@@ -274,43 +342,6 @@ extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int li
   if (!mesh || !out) { set_error("null argument"); return EBH_ERR_ARG; }
   *out = nullptr;
   const Topology &T = mesh->T;
-  std::vector<double> m((size_t)T.np, 0.0);
-  struct Trip { int r, c; double v; };
-  std::vector<Trip> trip;
-  trip.reserve((size_t)9 * T.ne);
-  for (int i = 0; i < T.ne; i++) {
-    const double *P0 = &T.nodes[3 * T.tri[3 * i]];
-    const double *P1 = &T.nodes[3 * T.tri[3 * i + 1]];
-    const double *P2 = &T.nodes[3 * T.tri[3 * i + 2]];
-    double u[3], v[3];
-    for (int k = 0; k < 3; k++) { u[k] = P1[k] - P0[k]; v[k] = P2[k] - P0[k]; }
-    const double c0 = u[1] * v[2] - u[2] * v[1], c1 = u[2] * v[0] - u[0] * v[2],
-                 c2 = u[0] * v[1] - u[1] * v[0];
-    const double J = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-    for (int k = 0; k < 3; k++) m[T.tri[3 * i + k]] += J / 6.0;
-    // consistent P1 mass: |T|/12 (1 + delta_ab) = J/24 (1 + delta_ab)
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++)
-        trip.push_back({T.tri[3 * i + a], T.tri[3 * i + b], (a == b ? 2.0 : 1.0) * J / 24.0});
-  }
-  for (double &v : m) v = 1.0 / v;
-  // CSR of the mass matrix: triplets sorted by (row, column), equal ones
-  // summed in triangle order
-  std::stable_sort(trip.begin(), trip.end(), [](const Trip &x, const Trip &y) {
-    return x.r != y.r ? x.r < y.r : x.c < y.c;
-  });
-  std::vector<int> mptr(T.np + 1, 0), mcol;
-  std::vector<double> mval;
-  for (size_t k = 0; k < trip.size(); k++) {
-    if (k > 0 && trip[k].r == trip[k - 1].r && trip[k].c == trip[k - 1].c) {
-      mval.back() += trip[k].v;
-      continue;
-    }
-    mcol.push_back(trip[k].c);
-    mval.push_back(trip[k].v);
-    mptr[trip[k].r + 1]++;
-  }
-  for (int r = 0; r < T.np; r++) mptr[r + 1] += mptr[r];
   ebh_op_desc d{};
   d.kernel = EBH_LAPLACE_SL;
   d.test_basis = d.trial_basis = EBH_P1;
@@ -320,20 +351,13 @@ extern "C" int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int li
   d.max_rhs = 1;
   ebh_op *op = nullptr;
   EBH_TRY(ebh_op_create(mesh, &d, &op));
-  int rc = op->dinv.upload(m);
-  if (rc == EBH_OK) rc = op->hsw.alloc(T.np);
-  if (rc == EBH_OK) rc = op->mass_ptr.upload(mptr);
-  if (rc == EBH_OK) rc = op->mass_col.upload(mcol);
-  if (rc == EBH_OK) rc = op->mass_val.upload(mval);
-  if (rc == EBH_OK) rc = op->cheb_r.alloc(T.np);
-  if (rc == EBH_OK) rc = op->cheb_d.alloc(T.np);
-  if (rc == EBH_OK) rc = op->cheb_t.alloc(T.np);
+  int rc = op->hsw.alloc(T.np);
+  if (rc == EBH_OK) rc = mass_setup(op, kCalderonHsSteps, 1);
   if (rc != EBH_OK) {
     ebh_op_destroy(op);
     return rc;
   }
   op->pscale = true;
-  op->pcheb = 3;
   *out = op;
   return EBH_OK;
 }

@@ -229,10 +229,16 @@ struct ebh_server {
       EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP64, 3, 1, &e->P64));
       if (o.mixed) EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP32, 3, 1, &e->P32));
       EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
+    } else if (!o.no_precond && p1) {
+      // P1 Dirichlet: the Calderon preconditioner Q (D + m m^T/(4|Gamma|)) Q
+      // (Q the Chebyshev inverse of the P1 mass) with a 1-point far rule
+      EBH_TRY(ebh_precond_calderon(e->mesh, EBH_FP64, 2, 0, 1, &e->P64));
+      if (o.mixed) EBH_TRY(ebh_precond_calderon(e->mesh, EBH_FP32, 2, 0, 1, &e->P32));
+      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
     } else if (!o.no_precond) {
-      // near-field sparse approximate inverse of V (3 node rings for P1, 2
-      // triangle rings for P0), one FP64 operator for both precisions
-      EBH_TRY(ebh_precond_spai(e->A64, p1 ? 3 : 2, 1, &e->P64));
+      // P0 Dirichlet: near-field sparse approximate inverse of V (2 triangle
+      // rings), one FP64 operator for both precisions
+      EBH_TRY(ebh_precond_spai(e->A64, 2, 1, &e->P64));
       EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P64));
     }
     // eager launches: stream capture must not overlap other threads'

@@ -198,3 +198,25 @@ def test_calderon_hs_neumann_iterations(L):
     print("iterations", st["iters"])
     assert st["converged"] and st["iters"] <= 10
     assert rel(D64(x), b) <= 1e-8
+
+
+def test_calderon_v_is_chebyshev_mass_sandwich(L):
+    """The single layer's Calderon preconditioner P = Q (D + m m^T / (4
+    |Gamma|)) Q, Q five Chebyshev steps for the consistent P1 mass, against
+    numpy with the dense mass matrix; two right-hand sides at once."""
+    P, T, _ = meshes.perturbed_icosphere(2, 6)
+    m = L.Mesh(P, T)
+    Pc = L.precond_calderon(m, "fp64", line_rule=2, tri_rule=0, max_rhs=2)
+    M, ML = _p1_mass(P, T)
+    area = sum(0.5 * np.linalg.norm(np.cross(P[t[1]] - P[t[0]], P[t[2]] - P[t[0]]))
+               for t in T - 1)
+    D = L.Operator(m, "D", 1, 1, "fp64", line_rule=2, tri_rule=0)
+    rng = np.random.default_rng(16)
+    X = rng.uniform(-1, 1, (2, P.shape[0]))
+    Y = Pc(X.ravel()).reshape(2, -1)
+    for x, y in zip(X, Y):
+        q = _cheb_mass_inverse(M, ML, x, steps=5)
+        yr = _cheb_mass_inverse(M, ML, D(q) + 0.25 / area * ML * (ML @ q), steps=5)
+        err = rel(y, yr)
+        print("Q (D + stab m m^T) Q vs numpy", err)
+        assert err <= 1e-13

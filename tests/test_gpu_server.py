@@ -178,9 +178,9 @@ def test_dirichlet_p1_server_matches_preconditioned_direct_solve():
     K = L.Operator(m, "K", 1, 1, "fp64")
     u = 1.0 / np.linalg.norm(P - x0, axis=1)
     b = 0.5 * m.mass(u, 1) + K(u)
-    S = L.precond_spai(V64, rings=3)
     s = L.Solver(V64, V32)
-    s.set_precond(S, S)
+    s.set_precond(L.precond_calderon(m, "fp64", line_rule=2, tri_rule=0),
+                  L.precond_calderon(m, "fp32", line_rule=2, tri_rule=0))
     t2, st = s.solve(b, tol=1e-8, mixed=True)
     # right-hand sides formed by the server and here differ in the last bits
     err = np.linalg.norm(t - t2) / np.linalg.norm(t2)
