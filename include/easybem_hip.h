@@ -411,8 +411,9 @@ typedef struct {
   int max_iters;       /* per request (default 2000)                         */
   int dirichlet_p1;    /* 1: Dirichlet flux in P1 (np entries); 0: P0 flux
                           (ne entries), as the reference's GPU example       */
-  int no_precond;      /* 1: no preconditioners (by default Dirichlet solves
-                          use the sparse approximate inverse of V, Neumann
+  int no_precond;      /* 1: no preconditioners (by default P1 Dirichlet
+                          solves use the Calderon preconditioner of V, P0
+                          ones its sparse approximate inverse, Neumann
                           solves the dual Calderon preconditioner)           */
 } ebh_server_opts;
 
