@@ -140,15 +140,17 @@ struct ebh_op {
   ebh_op *inner = nullptr;
   ebh::DevBuf<double> curl, mvec, hsw, hsz, hsdot;
   // Calderon preconditioner form of the hypersingular operator:
-  // diag(dinv) (D + stab m m^T) diag(dinv), dinv = 1 / lumped node areas
+  // Q (D + stab m m^T) Q, Q the Chebyshev mass inverse below (input copy in
+  // chebx)
   bool hs_scale = false;
-  ebh::DevBuf<double> dinv;
-  // single layer applied between two inverse masses (the Calderon
-  // preconditioner of the hypersingular operator); uses hsw.  pcheb > 0:
-  // the consistent P1 mass inverse by pcheb Chebyshev steps (CSR of the
-  // mass matrix, work vectors of np), else diag(dinv)
+  // single layer applied between two Chebyshev mass inverses (the Calderon
+  // preconditioner of the hypersingular operator); uses hsw
   bool pscale = false;
+  // Chebyshev mass inverse of those two forms: pcheb steps, the inverse
+  // lumped node areas dinv, the CSR of the consistent P1 mass and work
+  // vectors [max_rhs][np]
   int pcheb = 0;
+  ebh::DevBuf<double> dinv;
   ebh::DevBuf<int> mass_ptr, mass_col;
   ebh::DevBuf<double> mass_val, cheb_r, cheb_d, cheb_t, chebx;
   // elasticity K and D (EBH_KELVIN_DL / EBH_KELVIN_HS): the Laplace single

@@ -1685,26 +1685,13 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
   return EBH_OK;
 }
 
-__global__ void k_scale_vec(int n, const double *__restrict__ d, const double *x,
-                            double *y)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = d[i] * x[i];
-}
-
 int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
                  bool timed)
 {
   if (op->d.kernel == EBH_SPARSE_INVERSE) return spai_apply(op, x, y, 1, s, timed);
   if (op->inner) return composite_apply(op, x, y, 1, s, timed);
   if (op->pscale) {
-    if (op->pcheb) {
-      EBH_TRY(mass_inv_apply(op, x, op->hsw.p, 1, s));
-    } else {
-      hipLaunchKernelGGL(k_scale_vec, dim3((op->ncols + 255) / 256), dim3(256), 0, s,
-                         op->ncols, op->dinv.p, x, op->hsw.p);
-      EBH_HIP(hipGetLastError());
-    }
+    EBH_TRY(mass_inv_apply(op, x, op->hsw.p, 1, s));
     x = op->hsw.p;
   }
   const int ne = op->mesh->T.ne;
@@ -1718,15 +1705,7 @@ int op_apply_one(ebh_op *op, const double *x, double *y, hipStream_t s,
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
   EBH_TRY(op_finish(op, x, y, s));
-  if (op->pscale) {
-    if (op->pcheb) {
-      EBH_TRY(mass_inv_apply(op, y, y, 1, s));
-    } else {
-      hipLaunchKernelGGL(k_scale_vec, dim3((op->nrows + 255) / 256), dim3(256), 0, s,
-                         op->nrows, op->dinv.p, y, y);
-      EBH_HIP(hipGetLastError());
-    }
-  }
+  if (op->pscale) EBH_TRY(mass_inv_apply(op, y, y, 1, s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }

@@ -21,11 +21,9 @@ namespace {
 constexpr int kDotBlocks = 64;
 
 // w[(3r + c) ne + i] = sum_k curl[(3i + c) 3 + k] x[r np + tri[3i + k]]
-// (x scaled by dinv first when dinv is given)
 __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
                                                  const double *__restrict__ curl,
                                                  const int *__restrict__ tri,
-                                                 const double *__restrict__ dinv,
                                                  const double *__restrict__ x,
                                                  double *__restrict__ w)
 {
@@ -34,12 +32,7 @@ __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
   if (i >= ne) return;
   const double *xr = x + (size_t)r * np;
   const int n0 = tri[3 * i], n1 = tri[3 * i + 1], n2 = tri[3 * i + 2];
-  double x0 = xr[n0], x1 = xr[n1], x2 = xr[n2];
-  if (dinv) {
-    x0 *= dinv[n0];
-    x1 *= dinv[n1];
-    x2 *= dinv[n2];
-  }
+  const double x0 = xr[n0], x1 = xr[n1], x2 = xr[n2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const double *o = curl + (3 * (size_t)i + c) * 3;
@@ -49,7 +42,6 @@ __global__ __launch_bounds__(256) void k_hs_curl(int ne, int np,
 
 // part[r kDotBlocks + b] = sum over block b's strided nodes of m_n x_n
 __global__ __launch_bounds__(256) void k_hs_dot(int np, const double *__restrict__ m,
-                                                const double *__restrict__ dinv,
                                                 const double *__restrict__ x,
                                                 double *__restrict__ part)
 {
@@ -58,7 +50,7 @@ __global__ __launch_bounds__(256) void k_hs_dot(int np, const double *__restrict
   const double *xr = x + (size_t)r * np;
   double s = 0.0;
   for (int n = blockIdx.x * 256 + threadIdx.x; n < np; n += kDotBlocks * 256)
-    s = fma(m[n], dinv ? xr[n] * dinv[n] : xr[n], s);
+    s = fma(m[n], xr[n], s);
   red[threadIdx.x] = s;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
@@ -75,7 +67,7 @@ __global__ __launch_bounds__(256) void k_hs_gather(
     const int *__restrict__ ptr, const int *__restrict__ ntri,
     const int *__restrict__ nk, const double *__restrict__ z,
     const double *__restrict__ m, const double *__restrict__ part,
-    const double *__restrict__ dinv, double *__restrict__ y)
+    double *__restrict__ y)
 {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
@@ -93,7 +85,7 @@ __global__ __launch_bounds__(256) void k_hs_gather(
     for (int b = 0; b < kDotBlocks; b++) s += part[r * kDotBlocks + b];
     acc = fma(stab * m[n], s, acc);
   }
-  y[(size_t)r * np + n] = dinv ? acc * dinv[n] : acc;
+  y[(size_t)r * np + n] = acc;
 }
 
 // Chebyshev iteration for M z = v preconditioned by the lumped mass M_L
@@ -234,11 +226,6 @@ int hs_build(ebh_op *op)
   }
   EBH_TRY(op->curl.upload(curl));
   EBH_TRY(op->mvec.upload(m));
-  if (op->hs_scale) {
-    std::vector<double> d(m.size());
-    for (size_t k = 0; k < m.size(); k++) d[k] = 1.0 / m[k];
-    EBH_TRY(op->dinv.upload(d));
-  }
   const size_t nb = (size_t)std::max(1, op->d.max_rhs);
   EBH_TRY(op->hsw.alloc(3 * (size_t)T.ne * nb));
   EBH_TRY(op->hsz.alloc(3 * (size_t)T.ne * nb));
@@ -256,18 +243,17 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
   }
   const int ne = op->mesh->T.ne, np = op->mesh->T.np;
   if (timed) EBH_HIP(hipEventRecord(op->ev[0], s));
-  const double *dinv = op->hs_scale && !op->pcheb ? op->dinv.p : nullptr;
-  if (op->hs_scale && op->pcheb) {
+  if (op->hs_scale) {
     // Calderon form with the Chebyshev mass inverse Q on both sides
     EBH_TRY(mass_inv_apply(op, x, op->chebx.p, nrhs, s));
     x = op->chebx.p;
   }
   hipLaunchKernelGGL(k_hs_curl, dim3((ne + 255) / 256, nrhs), dim3(256), 0, s, ne, np,
-                     op->curl.p, op->mesh->tri.p, dinv, x, op->hsw.p);
+                     op->curl.p, op->mesh->tri.p, x, op->hsw.p);
   EBH_HIP(hipGetLastError());
   if (op->d.stab != 0.0) {
     hipLaunchKernelGGL(k_hs_dot, dim3(kDotBlocks, nrhs), dim3(256), 0, s, np,
-                       op->mvec.p, dinv, x, op->hsdot.p);
+                       op->mvec.p, x, op->hsdot.p);
     EBH_HIP(hipGetLastError());
   }
   if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
@@ -276,9 +262,9 @@ int hs_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s,
   hipLaunchKernelGGL(k_hs_gather, dim3((np + 255) / 256, nrhs), dim3(256), 0, s, ne,
                      np, op->d.stab, op->curl.p, op->mesh->node_tri_ptr.p,
                      op->mesh->node_tri.p, op->mesh->node_tri_k.p, op->hsz.p,
-                     op->mvec.p, op->hsdot.p, dinv, y);
+                     op->mvec.p, op->hsdot.p, y);
   EBH_HIP(hipGetLastError());
-  if (op->hs_scale && op->pcheb) EBH_TRY(mass_inv_apply(op, y, y, nrhs, s));
+  if (op->hs_scale) EBH_TRY(mass_inv_apply(op, y, y, nrhs, s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[3], s));
   return EBH_OK;
 }
@@ -316,16 +302,7 @@ extern "C" int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_
   d.stab = 0.25 / area;
   ebh_op *op = nullptr;
   EBH_TRY(ebh_op_create(mesh, &d, &op));
-  std::vector<double> m(T.np);
-  int rc = EBH_OK;
-  if (hipMemcpy(m.data(), op->mvec.p, sizeof(double) * T.np, hipMemcpyDeviceToHost) !=
-      hipSuccess) {
-    set_error("copy of the lumped areas failed");
-    rc = EBH_ERR_HIP;
-  }
-  for (double &v : m) v = 1.0 / v;
-  if (rc == EBH_OK) rc = op->dinv.upload(m);
-  if (rc == EBH_OK) rc = mass_setup(op, kCalderonVSteps, max_rhs);
+  int rc = mass_setup(op, kCalderonVSteps, max_rhs);
   if (rc == EBH_OK) rc = op->chebx.alloc((size_t)T.np * std::max(1, max_rhs));
   if (rc != EBH_OK) {
     ebh_op_destroy(op);
