@@ -19,10 +19,12 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -209,32 +211,70 @@ struct ebh_server {
     d.max_rhs = 1;
     // D + m m^T (laplace3dNeumannGPU.py:43-49): constants carry m.u
     d.stab = neu ? 1.0 : 0.0;
-    EBH_TRY(ebh_op_create(e->mesh, &d, &e->A64));
-    if (o.mixed) {
-      d.far_precision = EBH_FP32;
-      EBH_TRY(ebh_op_create(e->mesh, &d, &e->A32));
-    }
-    d.stab = 0.0;
-    d.kernel = neu ? EBH_LAPLACE_ADL : EBH_LAPLACE_DL;
-    d.test_basis = p1 ? EBH_P1 : EBH_P0;
-    d.trial_basis = neu ? EBH_P0 : EBH_P1;
-    d.far_precision = EBH_FP64;
-    EBH_TRY(ebh_op_create(e->mesh, &d, &e->K64));
-    EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
-    if (neu && !o.no_precond) {
+    ebh_op_desc d32 = d;
+    d32.far_precision = EBH_FP32;
+    ebh_op_desc dk = d;
+    dk.stab = 0.0;
+    dk.kernel = neu ? EBH_LAPLACE_ADL : EBH_LAPLACE_DL;
+    dk.test_basis = p1 ? EBH_P1 : EBH_P0;
+    dk.trial_basis = neu ? EBH_P0 : EBH_P1;
+    // The operators and the Calderon preconditioners depend on the mesh
+    // only: they are built on host threads at once (each operator has its
+    // own stream; the mesh's near-field pattern cache is locked).  Errors
+    // come back with their messages, which are per thread.
+    ebh_mesh *mesh = e->mesh;
+    using Result = std::pair<int, std::string>;
+    std::vector<std::future<Result>> jobs;
+    auto spawn = [&](auto fn) {
+      auto job = [mesh, fn]() {
+        int rc = hipSetDevice(mesh->device) == hipSuccess ? EBH_OK : EBH_ERR_HIP;
+        if (rc == EBH_OK) rc = fn();
+        return Result(rc, rc == EBH_OK ? std::string() : std::string(ebh_last_error()));
+      };
+      try {
+        jobs.push_back(std::async(std::launch::async, job));
+      } catch (const std::system_error &) {
+        // no thread to be had: build it here
+        std::promise<Result> done;
+        done.set_value(job());
+        jobs.push_back(done.get_future());
+      }
+    };
+    Entry *en = e.get();
+    if (o.mixed) spawn([=]() { return ebh_op_create(mesh, &d32, &en->A32); });
+    spawn([=]() { return ebh_op_create(mesh, &dk, &en->K64); });
+    const int cal = o.no_precond ? 0 : neu ? 2 : p1 ? 1 : 0;
+    if (cal == 2) {
       // dual Calderon preconditioners Q V Q (Q the Chebyshev inverse of the
       // P1 mass) with a 3-point far and 4-point Gauss near rule: D's small
       // eigenvalues are its smooth modes, which a local inverse cannot reach
       // (its sparse inverse needs more iterations from L5 up)
-      EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP64, 3, 1, &e->P64));
-      if (o.mixed) EBH_TRY(ebh_precond_calderon_hs(e->mesh, EBH_FP32, 3, 1, &e->P32));
-      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
-    } else if (!o.no_precond && p1) {
+      spawn([=]() { return ebh_precond_calderon_hs(mesh, EBH_FP64, 3, 1, &en->P64); });
+      if (o.mixed)
+        spawn([=]() { return ebh_precond_calderon_hs(mesh, EBH_FP32, 3, 1, &en->P32); });
+    } else if (cal == 1) {
       // P1 Dirichlet: the Calderon preconditioner Q (D + m m^T/(4|Gamma|)) Q
       // (Q the Chebyshev inverse of the P1 mass) with a 1-point far rule
-      EBH_TRY(ebh_precond_calderon(e->mesh, EBH_FP64, 2, 0, 1, &e->P64));
-      if (o.mixed) EBH_TRY(ebh_precond_calderon(e->mesh, EBH_FP32, 2, 0, 1, &e->P32));
-      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
+      spawn([=]() { return ebh_precond_calderon(mesh, EBH_FP64, 2, 0, 1, &en->P64); });
+      if (o.mixed)
+        spawn([=]() { return ebh_precond_calderon(mesh, EBH_FP32, 2, 0, 1, &en->P32); });
+    }
+    int rc = ebh_op_create(mesh, &d, &e->A64);
+    std::string msg = rc == EBH_OK ? std::string() : std::string(ebh_last_error());
+    for (auto &j : jobs) {
+      const Result r2 = j.get();  // every job ends before the entry can go
+      if (rc == EBH_OK && r2.first != EBH_OK) {
+        rc = r2.first;
+        msg = r2.second;
+      }
+    }
+    if (rc != EBH_OK) {
+      ebh::set_error(msg);
+      return rc;
+    }
+    EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
+    if (cal) {
+      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, o.mixed ? e->P32 : nullptr));
     } else if (!o.no_precond) {
       // P0 Dirichlet: near-field sparse approximate inverse of V (2 triangle
       // rings), one FP64 operator for both precisions
