@@ -407,7 +407,7 @@ def precond_spai(V: Operator, rings=None, max_rhs=1):
 
 def precond_calderon_hs(mesh: Mesh, precision="fp32", line_rule=3, tri_rule=1):
     """Dual Calderon preconditioner Q V Q for the P1 hypersingular
-    operator, Q a 3-step Chebyshev approximation of the consistent P1 mass
+    operator, Q a 5-step Chebyshev approximation of the consistent P1 mass
     inverse (ebh_precond_calderon_hs), as an Operator."""
     L = lib()
     L.ebh_precond_calderon_hs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int,

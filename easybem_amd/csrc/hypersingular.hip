@@ -94,10 +94,11 @@ __global__ __launch_bounds__(256) void k_hs_gather(
 // mesh (each element's are 1, 1/4, 1/4).  A fixed number of steps is a
 // fixed polynomial in M_L^-1 M times M_L^-1, a linear operator.
 constexpr double kChebLo = 0.25, kChebHi = 1.0;
-// Chebyshev steps of the two Calderon preconditioners (odd counts suit
-// the single layer's: FP64 GMRES(50) on the L5 icosphere takes 11, 6, 5,
-// 4, 5 iterations with 2..6 steps, 4 with the exact mass inverse)
-constexpr int kCalderonVSteps = 5, kCalderonHsSteps = 3;
+// Chebyshev steps of the two Calderon preconditioners.  FP64 GMRES(50)
+// on V of the L5 icosphere takes 11, 6, 5, 4, 5 iterations with 2..6
+// steps, 4 with the exact mass inverse; on D + m m^T of the L4 and L5
+// serving meshes 7 with 3 steps and 6 with 5.
+constexpr int kCalderonVSteps = 5, kCalderonHsSteps = 5;
 
 // r = v, d = M_L^-1 v / theta, z = d; blockIdx.y: right-hand side
 __global__ void k_cheb_init(int n, const double *v, const double *__restrict__ dinv,

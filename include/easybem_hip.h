@@ -366,12 +366,13 @@ int ebh_precond_calderon(ebh_mesh *mesh, int far_precision, int line_rule,
 int ebh_precond_spai(ebh_op *V, int rings, int max_rhs, ebh_op **out);
 /* The dual Calderon preconditioner, for the (stabilised) hypersingular
  * operator on P1: P = Q V Q with V the P1 single layer at the given rules
- * and Q three Chebyshev steps for the consistent P1 mass matrix M,
+ * and Q five Chebyshev steps for the consistent P1 mass matrix M,
  * preconditioned by the lumped mass M_L (whose ratio to M has eigenvalues
  * in [1/4, 1]): a fixed polynomial, so P is linear.  Rules 3 and 1 (4-point
- * Gauss near field, 3-point far rule) keep GMRES(50) on D + m m^T at 7
- * iterations on the L4 and L5 serving meshes, where M_L^-1 V M_L^-1 with a
- * 1-point far rule took 31 and 33 (no counterpart in the reference). */
+ * Gauss near field, 3-point far rule) keep FP64 GMRES(50) on D + m m^T at
+ * 6 iterations on the L4 and L5 serving meshes, where M_L^-1 V M_L^-1
+ * with a 1-point far rule took 31 and 33 (no counterpart in the
+ * reference). */
 int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int line_rule,
                             int tri_rule, ebh_op **out);
 /* nb systems: b and x hold nb vectors of rows entries back to back; stats
