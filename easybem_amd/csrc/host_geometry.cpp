@@ -3,6 +3,8 @@
 #include "host_geometry.h"
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -330,35 +332,16 @@ std::string build_topology(int np, const double *nodes, int ne,
                 v[1] * v[1] + v[2] * v[2];
     if (!(J > 1e-12 * L2)) return "element " + std::to_string(i) + " is degenerate";
   }
-  // closed, consistently oriented 2-manifold: each undirected edge appears
-  // exactly twice, once in each direction
-  {
-    std::vector<std::pair<int64_t, int>> e;
-    e.reserve((size_t)3 * ne);
-    for (int i = 0; i < ne; i++)
-      for (int k = 0; k < 3; k++) {
-        int a = T.tri[3 * i + k], b = T.tri[3 * i + (k + 1) % 3];
-        int64_t key = (int64_t)std::min(a, b) * np + std::max(a, b);
-        e.push_back({key, a < b ? 1 : -1});
-      }
-    std::sort(e.begin(), e.end());
-    for (size_t k = 0; k < e.size();) {
-      size_t m = k;
-      int sum = 0;
-      while (m < e.size() && e[m].first == e[k].first) { sum += e[m].second; m++; }
-      if (m - k != 2)
-        return "edge (" + std::to_string(e[k].first / np + 1) + "," +
-               std::to_string(e[k].first % np + 1) + ") is shared by " +
-               std::to_string(m - k) +
-               " triangles: the mesh must be a closed 2-manifold";
-      if (sum != 0)
-        return "edge (" + std::to_string(e[k].first / np + 1) + "," +
-               std::to_string(e[k].first % np + 1) +
-               ") has the same direction in both triangles: inconsistent "
-               "orientation";
-      k = m;
-    }
-  }
+  const int nthreads =
+      (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  // runs fn(tid) on host threads; item r of a loop goes to thread r mod
+  // nthreads, and every item below is independent of the others
+  auto on_threads = [&](auto fn) {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(fn, t);
+    fn(0);
+    for (auto &th : pool) th.join();
+  };
   // node -> triangles
   T.node_tri_ptr.assign(np + 1, 0);
   for (int i = 0; i < 3 * ne; i++) T.node_tri_ptr[T.tri[i] + 1]++;
@@ -375,15 +358,63 @@ std::string build_topology(int np, const double *nodes, int ne,
         fill[n]++;
       }
   }
+  // closed, consistently oriented 2-manifold: each undirected edge appears
+  // exactly twice, once in each direction.  Edge {a, b}, a < b, is checked
+  // at node a: around a, a -> b is the edge to the next vertex of a
+  // triangle and b -> a the edge from the previous one.  The failing edge
+  // of smallest (a, b) is reported.
+  {
+    struct Bad { int64_t key = INT64_MAX; int count = 0; bool orient = false; };
+    std::vector<Bad> bad(nthreads);
+    on_threads([&](int tid) {
+      std::vector<int> nb, out, in;
+      for (int a = tid; a < np; a += nthreads) {
+        nb.clear();
+        out.clear();
+        in.clear();
+        auto slot = [&](int b) {
+          for (size_t q = 0; q < nb.size(); q++)
+            if (nb[q] == b) return (int)q;
+          nb.push_back(b);
+          out.push_back(0);
+          in.push_back(0);
+          return (int)nb.size() - 1;
+        };
+        for (int p = T.node_tri_ptr[a]; p < T.node_tri_ptr[a + 1]; p++) {
+          const int t = T.node_tri[p], k = T.node_tri_k[p];
+          out[slot(T.tri[3 * t + (k + 1) % 3])]++;
+          in[slot(T.tri[3 * t + (k + 2) % 3])]++;
+        }
+        for (size_t q = 0; q < nb.size(); q++) {
+          if (nb[q] < a) continue;
+          const int count = out[q] + in[q];
+          const bool fail = count != 2 || out[q] != in[q];
+          const int64_t key = (int64_t)a * np + nb[q];
+          if (fail && key < bad[tid].key) bad[tid] = Bad{key, count, count == 2};
+        }
+      }
+    });
+    Bad worst;
+    for (const Bad &b : bad)
+      if (b.key < worst.key) worst = b;
+    if (worst.key != INT64_MAX) {
+      const std::string edge = "edge (" + std::to_string(worst.key / np + 1) + "," +
+                               std::to_string(worst.key % np + 1) + ")";
+      if (!worst.orient)
+        return edge + " is shared by " + std::to_string(worst.count) +
+               " triangles: the mesh must be a closed 2-manifold";
+      return edge + " has the same direction in both triangles: inconsistent "
+                    "orientation";
+    }
+  }
   for (int n = 0; n < np; n++)
     if (T.node_tri_ptr[n + 1] == T.node_tri_ptr[n])
       return "node " + std::to_string(n + 1) + " belongs to no triangle";
-  // touching lists and classified pairs
+  // touching lists and classified pairs, in (i, j) order: the touching
+  // triangles of i (its vertices' triangles, sorted, self included) are
+  // counted, then filled at their offsets
   T.touch_ptr.assign(ne + 1, 0);
-  T.touch.clear();
-  T.pairs.clear();
-  std::vector<int> cand;
-  for (int i = 0; i < ne; i++) {
+  auto touching = [&](int i, std::vector<int> &cand) {
     cand.clear();
     for (int k = 0; k < 3; k++) {
       int n = T.tri[3 * i + k];
@@ -392,49 +423,79 @@ std::string build_topology(int np, const double *nodes, int ne,
     }
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-    T.max_touch = std::max<int>(T.max_touch, (int)cand.size());
-    const int *fi = &T.tri[3 * i];
-    for (int j : cand) {
-      T.touch.push_back(j);
-      const int *fj = &T.tri[3 * j];
-      NearPair P;
-      P.i = i;
-      P.j = j;
-      if (i == j) {
-        P.cls = kCoincident;
-        for (int k = 0; k < 3; k++) P.oi[k] = P.oj[k] = fi[k];
-      } else {
-        // shared nodes scanned in j's order (src/BEM3dLaplace.cpp:692-709)
-        int ci[3], cj[3], cnt = 0;
-        for (int k = 0; k < 3; k++)
-          for (int l = 0; l < 3; l++)
-            if (fj[k] == fi[l]) { ci[cnt] = l; cj[cnt] = k; cnt++; break; }
-        if (cnt == 1) {
-          P.cls = kVertex;
-          for (int k = 0; k < 3; k++) {
-            P.oi[k] = fi[(ci[0] + k) % 3];
-            P.oj[k] = fj[(cj[0] + k) % 3];
-          }
-        } else if (cnt == 2) {
-          // orient so the shared edge runs cj0 -> cj0+1 in j
-          if (cj[0] == 0 && cj[1] == 2) {
-            std::swap(cj[0], cj[1]);
-            std::swap(ci[0], ci[1]);
-          }
-          int remi = 3 - ci[0] - ci[1], remj = 3 - cj[0] - cj[1];
-          P.cls = kEdge;
-          P.oi[0] = fi[ci[0]]; P.oi[1] = fi[ci[1]]; P.oi[2] = fi[remi];
-          P.oj[0] = fj[cj[0]]; P.oj[1] = fj[cj[1]]; P.oj[2] = fj[remj];
-        } else {
-          return "triangles " + std::to_string(i) + " and " +
-                 std::to_string(j) + " share all three nodes";
-        }
-      }
-      T.n_class[P.cls]++;
-      T.pairs.push_back(P);
+  };
+  on_threads([&](int tid) {
+    std::vector<int> cand;
+    for (int i = tid; i < ne; i += nthreads) {
+      touching(i, cand);
+      T.touch_ptr[i + 1] = (int)cand.size();
     }
-    T.touch_ptr[i + 1] = (int)T.touch.size();
+  });
+  for (int i = 0; i < ne; i++) {
+    T.max_touch = std::max(T.max_touch, T.touch_ptr[i + 1]);
+    T.touch_ptr[i + 1] += T.touch_ptr[i];
   }
+  T.touch.resize(T.touch_ptr[ne]);
+  T.pairs.resize(T.touch_ptr[ne]);
+  std::vector<std::array<int64_t, 4>> ncls(nthreads, std::array<int64_t, 4>{0, 0, 0, 0});
+  std::vector<int> dup(nthreads, INT_MAX), dupj(nthreads, 0);
+  on_threads([&](int tid) {
+    std::vector<int> cand;
+    for (int i = tid; i < ne; i += nthreads) {
+      touching(i, cand);
+      const int *fi = &T.tri[3 * i];
+      int e = T.touch_ptr[i];
+      for (int j : cand) {
+        T.touch[e] = j;
+        const int *fj = &T.tri[3 * j];
+        NearPair P;
+        P.i = i;
+        P.j = j;
+        if (i == j) {
+          P.cls = kCoincident;
+          for (int k = 0; k < 3; k++) P.oi[k] = P.oj[k] = fi[k];
+        } else {
+          // shared nodes scanned in j's order (src/BEM3dLaplace.cpp:692-709)
+          int ci[3], cj[3], cnt = 0;
+          for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+              if (fj[k] == fi[l]) { ci[cnt] = l; cj[cnt] = k; cnt++; break; }
+          if (cnt == 1) {
+            P.cls = kVertex;
+            for (int k = 0; k < 3; k++) {
+              P.oi[k] = fi[(ci[0] + k) % 3];
+              P.oj[k] = fj[(cj[0] + k) % 3];
+            }
+          } else if (cnt == 2) {
+            // orient so the shared edge runs cj0 -> cj0+1 in j
+            if (cj[0] == 0 && cj[1] == 2) {
+              std::swap(cj[0], cj[1]);
+              std::swap(ci[0], ci[1]);
+            }
+            int remi = 3 - ci[0] - ci[1], remj = 3 - cj[0] - cj[1];
+            P.cls = kEdge;
+            P.oi[0] = fi[ci[0]]; P.oi[1] = fi[ci[1]]; P.oi[2] = fi[remi];
+            P.oj[0] = fj[cj[0]]; P.oj[1] = fj[cj[1]]; P.oj[2] = fj[remj];
+          } else {
+            // reported for the smallest i (then j) after the loop
+            if (i < dup[tid]) { dup[tid] = i; dupj[tid] = j; }
+            P.cls = kCoincident;
+          }
+        }
+        ncls[tid][P.cls]++;
+        T.pairs[e] = P;
+        e++;
+      }
+    }
+  });
+  int di = INT_MAX, dj = 0;
+  for (int t = 0; t < nthreads; t++)
+    if (dup[t] < di) { di = dup[t]; dj = dupj[t]; }
+  if (di != INT_MAX)
+    return "triangles " + std::to_string(di) + " and " + std::to_string(dj) +
+           " share all three nodes";
+  for (int t = 0; t < nthreads; t++)
+    for (int c = 0; c < 4; c++) T.n_class[c] += ncls[t][c];
   T.n_class[kDisjoint] = (int64_t)ne * ne - (int64_t)T.pairs.size();
   return "";
 }
