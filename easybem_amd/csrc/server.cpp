@@ -195,7 +195,9 @@ struct ebh_server {
     }
   }
 
-  int build(Request &r, std::unique_ptr<Entry> &e)
+  // b: when given, the right-hand side is formed as soon as its K (or K')
+  // exists, while the other operators are still being built
+  int build(Request &r, std::unique_ptr<Entry> &e, std::vector<double> *b)
   {
     e.reset(new Entry());
     e->key = r.key;
@@ -242,7 +244,13 @@ struct ebh_server {
     };
     Entry *en = e.get();
     if (o.mixed) spawn([=]() { return ebh_op_create(mesh, &d32, &en->A32); });
-    spawn([=]() { return ebh_op_create(mesh, &dk, &en->K64); });
+    const Request *rq = &r;
+    spawn([=]() {
+      int rc = ebh_op_create(mesh, &dk, &en->K64);
+      if (rc == EBH_OK && b)
+        rc = neu ? neumann_rhs(*rq, *en, *b) : dirichlet_rhs(*rq, *en, *b);
+      return rc;
+    });
     const int cal = o.no_precond ? 0 : neu ? 2 : p1 ? 1 : 0;
     if (cal == 2) {
       // dual Calderon preconditioners Q V Q (Q the Chebyshev inverse of the
@@ -355,12 +363,17 @@ struct ebh_server {
   {
     const auto t0 = Clock::now();
     r.res.ne = r.problem == EBH_NEUMANN || o.dirichlet_p1 ? r.np : r.ne;
-    if (!e) EBH_TRY(build(r, e));
     const bool neu = r.problem == EBH_NEUMANN;
     const int n = neu || o.dirichlet_p1 ? r.np : r.ne;
     std::vector<double> b(n);
-    if (neu) EBH_TRY(neumann_rhs(r, *e, b));
-    else EBH_TRY(dirichlet_rhs(r, *e, b));
+    if (!e) {
+      // a new mesh: operators, preconditioners and right-hand side
+      EBH_TRY(build(r, e, &b));
+    } else if (neu) {
+      EBH_TRY(neumann_rhs(r, *e, b));
+    } else {
+      EBH_TRY(dirichlet_rhs(r, *e, b));
+    }
     ebh::DevBuf<double> db, dx;
     EBH_TRY(db.upload(b));
     std::vector<double> zero(n, 0.0);
