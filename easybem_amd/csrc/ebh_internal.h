@@ -148,7 +148,8 @@ struct ebh_op {
   bool pscale = false;
   // Chebyshev mass inverse of those two forms: pcheb steps, the inverse
   // lumped node areas dinv, the CSR of the consistent P1 mass and work
-  // vectors [max_rhs][np]
+  // vectors [max_rhs][np] (residual; cheb_d and cheb_t the search
+  // direction, alternately; chebx the hypersingular form's input copy)
   int pcheb = 0;
   ebh::DevBuf<double> dinv;
   ebh::DevBuf<int> mass_ptr, mass_col;
@@ -192,9 +193,6 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
 // y = M x for an EBH_SPARSE_INVERSE operator (nrhs vectors back to back)
 int spai_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s, bool timed);
-// y = A x for an FP64 CSR matrix of n rows (nrhs vectors back to back)
-int csr_spmv(int n, const int *ptr, const int *col, const double *val, const double *x,
-             double *y, int nrhs, hipStream_t s);
 // z = Q v with Q the Chebyshev approximation of the P1 mass inverse of a
 // hypersingular Calderon preconditioner (z may alias v)
 int mass_inv_apply(ebh_op *op, const double *v, double *z, int nrhs, hipStream_t s);

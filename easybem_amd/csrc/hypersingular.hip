@@ -115,19 +115,26 @@ __global__ void k_cheb_init(int n, const double *v, const double *__restrict__ d
   z[k] = di;  // z may alias v: read above, by the same thread
 }
 
-// r -= t (t = M d), d = a d + b M_L^-1 r, z += d
-__global__ void k_cheb_step(int n, const double *__restrict__ t,
+// One step, row i of the mass matrix per thread: r -= M d, dn = a d +
+// b M_L^-1 r, z += dn (d and dn distinct: other rows still read d)
+__global__ void k_cheb_step(int n, const int *__restrict__ ptr,
+                            const int *__restrict__ col,
+                            const double *__restrict__ val,
                             const double *__restrict__ dinv, double a, double b,
-                            double *__restrict__ r, double *__restrict__ d,
-                            double *__restrict__ z)
+                            double *__restrict__ r, const double *__restrict__ d,
+                            double *__restrict__ dn, double *__restrict__ z)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const size_t k = (size_t)blockIdx.y * n + i;
-  const double ri = r[k] - t[k];
-  const double di = fma(a, d[k], b * dinv[i] * ri);
+  const size_t off = (size_t)blockIdx.y * n;
+  const double *dr = d + off;
+  double t = 0.0;
+  for (int e = ptr[i]; e < ptr[i + 1]; e++) t = fma(val[e], dr[col[e]], t);
+  const size_t k = off + i;
+  const double ri = r[k] - t;
+  const double di = fma(a, dr[i], b * dinv[i] * ri);
   r[k] = ri;
-  d[k] = di;
+  dn[k] = di;
   z[k] += di;
 }
 
@@ -140,16 +147,18 @@ int mass_inv_apply(ebh_op *op, const double *v, double *z, int nrhs, hipStream_t
   const double th = 0.5 * (kChebHi + kChebLo), de = 0.5 * (kChebHi - kChebLo);
   const double sig = th / de;
   double rho = 1.0 / sig;
+  // the search direction alternates between two buffers
+  double *d = op->cheb_d.p, *dn = op->cheb_t.p;
   hipLaunchKernelGGL(k_cheb_init, grid, blk, 0, s, n, v, op->dinv.p, 1.0 / th,
-                     op->cheb_r.p, op->cheb_d.p, z);
+                     op->cheb_r.p, d, z);
   EBH_HIP(hipGetLastError());
   for (int k = 1; k < op->pcheb; k++) {
-    EBH_TRY(csr_spmv(n, op->mass_ptr.p, op->mass_col.p, op->mass_val.p, op->cheb_d.p,
-                     op->cheb_t.p, nrhs, s));
     const double rn = 1.0 / (2.0 * sig - rho);
-    hipLaunchKernelGGL(k_cheb_step, grid, blk, 0, s, n, op->cheb_t.p, op->dinv.p,
-                       rn * rho, 2.0 * rn / de, op->cheb_r.p, op->cheb_d.p, z);
+    hipLaunchKernelGGL(k_cheb_step, grid, blk, 0, s, n, op->mass_ptr.p, op->mass_col.p,
+                       op->mass_val.p, op->dinv.p, rn * rho, 2.0 * rn / de, op->cheb_r.p,
+                       d, dn, z);
     EBH_HIP(hipGetLastError());
+    std::swap(d, dn);
     rho = rn;
   }
   return EBH_OK;

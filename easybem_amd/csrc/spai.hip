@@ -245,15 +245,6 @@ bool solve_inplace(int m, double *B, double *rhs)
 
 }  // namespace
 
-int csr_spmv(int n, const int *ptr, const int *col, const double *val, const double *x,
-             double *y, int nrhs, hipStream_t s)
-{
-  const dim3 grid((unsigned)(((size_t)n * kSpaiLanes + 255) / 256), (unsigned)nrhs);
-  hipLaunchKernelGGL(k_spai_apply, grid, dim3(256), 0, s, n, ptr, col, val, x, y);
-  EBH_HIP(hipGetLastError());
-  return EBH_OK;
-}
-
 int spai_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s, bool timed)
 {
   const int n = op->nrows;
