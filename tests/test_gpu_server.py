@@ -225,3 +225,25 @@ def test_dirichlet_and_neumann_mix_identical_alone():
         assert res["status"] == 0 and res["relres"] <= 1e-8
         assert np.array_equal(v, v0)
     srv.close()
+
+
+@pytest.mark.parametrize("p1", [True, False])
+def test_setup_error_on_a_worker_thread_is_reported(p1):
+    """A Dirichlet source point on a mesh node makes the nodal data, and so
+    the right-hand side, non-finite.  The error arises in the set-up job
+    that forms it (a host thread of its own); the request fails with that
+    message, the server goes on, and the same mesh then solves with a
+    valid point."""
+    from easybem_amd import _lib as L
+    P, T, x0 = stream(1, seed=12, levels=(3,))[0]
+    srv = L.Server(max_slots=2, dirichlet_p1=p1)
+    bad = srv.submit(P, T, P[7].copy(), key=77)
+    res, _ = srv.wait(bad)
+    assert res["status"] != 0 and "on the surface" in res["error"]
+    good = srv.submit(P, T, x0, key=77)
+    res2, sol = srv.wait(good)
+    st = srv.stats()
+    srv.close()
+    assert res2["status"] == 0 and res2["relres"] <= 1e-8
+    assert np.all(np.isfinite(sol))
+    assert st["failed"] == 1 and st["completed"] == 1
