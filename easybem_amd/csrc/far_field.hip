@@ -823,6 +823,7 @@ struct FarMArgs {
   const R *tq;                  // [nt][3][ne]
   const R *geom;                // [ne*nt][4]
   const R *sig;                 // [groups][ne*nt][RB]
+  const double *sigd;           // [groups][ne][RB] (k_far_mrhs_tri)
   const int *touch_ptr, *touch;
   const double *tw;             // [nt][ne]
   double *partial;              // [chunks][nrhs][nout][ne]
@@ -909,14 +910,98 @@ __global__ __launch_bounds__(256) void k_far_mrhs(FarMArgs<R> a)
   }
 }
 
-template <typename R, int RB>
+// Multi-RHS single layer with piecewise-constant test and trial functions.
+// A source triangle's strength is one value per RHS over all its points, and
+// a target triangle's output is its points' weighted sum, so each triangle
+// pair first sums G = sum_q w_q sum_b w_b / |x_q - y_b| over its NT x NT point
+// pairs (in R) and then spends RB FP64 FMAs on G * sigma_j: RB FMAs per
+// triangle pair instead of per point pair.  One lane per target triangle;
+// source records as in k_far_mrhs (wave-uniform addresses, scalar loads),
+// with the source weight in the fourth slot of each point and FP64 strengths
+// per triangle.  FP64 and non-packed FP32 FMAs issue at one rate, so the FP64
+// sums cost no more than FP32 ones and need no FP32 tile in registers.
+template <typename R, int NT, int RB>
+__global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
+{
+  const int ne = a.ne;
+  int b = blockIdx.x;
+  const int s = b % a.chunks;
+  b /= a.chunks;
+  const int g = b % a.groups;
+  const int tb = b / a.groups;
+  const int t = tb * 256 + (int)threadIdx.x;
+  const int tt = t < ne ? t : ne - 1;
+  const int j0 = s * a.chunk, j1 = min(ne, j0 + a.chunk);
+  R X[NT], Y[NT], Z[NT], W[NT];
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
+    Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
+    Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+    W[q] = (R)a.tw[(size_t)q * ne + tt];
+  }
+  double acc64[RB];
+#pragma unroll
+  for (int r = 0; r < RB; r++) acc64[r] = 0.0;
+  int cur = a.touch_ptr[tt];
+  const int cend = a.touch_ptr[tt + 1];
+  while (cur < cend && a.touch[cur] < j0) cur++;
+  int nextx = cur < cend ? a.touch[cur] : INT_MAX;
+  const R eps = Eps<R>::v;
+  const R *__restrict__ geom = a.geom;
+  const double *__restrict__ sig = a.sigd + (size_t)g * ne * RB;
+  for (int js = j0; js < j1; js += 64) {
+    const int nt = min(64, j1 - js);
+    uint64_t mask = 0;
+    while (nextx < js + nt) {
+      mask |= 1ull << (nextx - js);
+      ++cur;
+      nextx = cur < cend ? a.touch[cur] : INT_MAX;
+    }
+    // G of the pair (t, js + jj), 0 for an excluded (touching) pair
+    auto pair_sum = [&](int jj) -> R {
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      const R *gp = geom + (size_t)(js + jj) * NT * 4;
+      R G = R(0);
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        R Gq = R(0);
+#pragma unroll
+        for (int bq = 0; bq < NT; bq++) {
+          const R dx = X[q] - gp[4 * bq], dy = Y[q] - gp[4 * bq + 1],
+                  dz = Z[q] - gp[4 * bq + 2];
+          const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+          Gq = fma(gp[4 * bq + 3], rsq(r2), Gq);
+        }
+        G = fma(W[q], Gq, G);
+      }
+      return G * keep;
+    };
+    for (int jj = 0; jj < nt; jj++) {
+      const double G = (double)pair_sum(jj);
+      const double *sp = sig + (size_t)(js + jj) * RB;
+#pragma unroll
+      for (int r = 0; r < RB; r++) acc64[r] = fma(sp[r], G, acc64[r]);
+    }
+  }
+  if (t < ne) {
+    const int nrhs_here = min(RB, a.nrhs - g * RB);
+    for (int r = 0; r < nrhs_here; r++)
+      a.partial[((size_t)s * a.nrhs + g * RB + r) * ne + t] = acc64[r];
+  }
+}
+
+// Source records of the multi-RHS kernels.  Per point strengths
+// w_b * scale * x(y_b) (k_far_mrhs), or with TRI (P0 trial, k_far_mrhs_tri)
+// the weight w_b in the point record and scale * x_j once per triangle.
+template <typename R, int RB, bool TRI>
 __global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
                             const double *__restrict__ sg,
                             const double *__restrict__ swt,
                             const int *__restrict__ tri, int trial_p1,
                             RuleArgs rule, const double *__restrict__ x,
                             int ncols, double scale, R *__restrict__ geom,
-                            R *__restrict__ sig)
+                            R *__restrict__ sig, double *__restrict__ sigd)
 {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int npts = ne * nt;
@@ -928,7 +1013,16 @@ __global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
     o[0] = (R)sg[3 * p];
     o[1] = (R)sg[3 * p + 1];
     o[2] = (R)sg[3 * p + 2];
-    o[3] = (R)0;
+    o[3] = TRI ? (R)swt[p] : (R)0;
+  }
+  if constexpr (TRI) {
+    if (b != 0) return;
+    double *o = sigd + ((size_t)g * ne + j) * RB;
+    for (int r = 0; r < RB; r++) {
+      const int rr = g * RB + r;
+      o[r] = rr < nrhs ? scale * x[(size_t)rr * ncols + j] : 0.0;
+    }
+    return;
   }
   const double f = swt[p] * scale;
   const double e1 = rule.x[b], e2 = rule.y[b];
@@ -1319,6 +1413,31 @@ int run_fold(ebh_op *op, const double *x, hipStream_t s)
   return EBH_OK;
 }
 
+// the triangle-pair multi-RHS kernel serves P0 test and trial functions with
+// rules of at most 7 points (its target points live in registers)
+bool mrhs_tri(const ebh_op *op)
+{
+  return !op->d.test_basis && !op->d.trial_basis && op->rule.nt <= 7;
+}
+
+template <typename R, int RB>
+int launch_mrhs_tri(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
+{
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((k_far_mrhs_tri<R, 1, RB>), grid, dim3(256), 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_mrhs_tri<R, 3, RB>), grid, dim3(256), 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_mrhs_tri<R, 4, RB>), grid, dim3(256), 0, s, a); break;
+  case 6: hipLaunchKernelGGL((k_far_mrhs_tri<R, 6, RB>), grid, dim3(256), 0, s, a); break;
+  case 7: hipLaunchKernelGGL((k_far_mrhs_tri<R, 7, RB>), grid, dim3(256), 0, s, a); break;
+  default:
+    set_error("unsupported far-field rule size " + std::to_string(nt) +
+              " for the P0 multi-RHS kernel");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 template <typename R, int RB>
 int launch_mrhs_nt(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
 {
@@ -1348,23 +1467,19 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   else { geom = op->mgeom64.p; sig = op->msig64.p; }
   const int n = ne * nt * groups;
   const int p1 = op->d.trial_basis;
+  const bool tri = mrhs_tri(op);
+  const dim3 fg((n + 255) / 256);
+#define EBH_FOLD(RB_, TRI_)                                                               \
+  hipLaunchKernelGGL((k_fold_mrhs<R, RB_, TRI_>), fg, dim3(256), 0, s, ne, nt, nrhs,    \
+                     groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1, op->rule, x,   \
+                     op->ncols, kInv4Pi, geom, sig, op->msig64.p)
   switch (rb) {
-  case 8:
-    hipLaunchKernelGGL((k_fold_mrhs<R, 8>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
-                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
-                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
-    break;
-  case 16:
-    hipLaunchKernelGGL((k_fold_mrhs<R, 16>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
-                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
-                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
-    break;
-  default:
-    hipLaunchKernelGGL((k_fold_mrhs<R, 32>), dim3((n + 255) / 256), dim3(256), 0, s, ne,
-                       nt, nrhs, groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1,
-                       op->rule, x, op->ncols, kInv4Pi, geom, sig);
-    break;
+  case 64: EBH_FOLD(64, true); break;
+  case 8: if (tri) EBH_FOLD(8, true); else EBH_FOLD(8, false); break;
+  case 16: if (tri) EBH_FOLD(16, true); else EBH_FOLD(16, false); break;
+  default: if (tri) EBH_FOLD(32, true); else EBH_FOLD(32, false); break;
   }
+#undef EBH_FOLD
   EBH_HIP(hipGetLastError());
   FarMArgs<R> a;
   a.ne = ne;
@@ -1376,6 +1491,7 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   a.tq = sizeof(R) == 4 ? (const R *)op->tq32.p : (const R *)op->tq64.p;
   a.geom = geom;
   a.sig = sig;
+  a.sigd = op->msig64.p;
   a.touch_ptr = op->mesh->touch_ptr.p;
   a.touch = op->mesh->touch.p;
   a.tw = op->tw.p;
@@ -1386,15 +1502,25 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
     a.phi[q][1] = in ? op->rule.x[q] : 0.0;
     a.phi[q][2] = in ? op->rule.y[q] : 0.0;
   }
-  const int tbn = (ne + 256 / nt - 1) / (256 / nt);
+  const int tpb = mrhs_tri(op) ? 256 : 256 / nt;
+  const int tbn = (ne + tpb - 1) / tpb;
   dim3 grid((unsigned)(tbn * groups * op->mchunks));
   const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
   int rc;
-  switch (rb) {
-  case 8: rc = launch_mrhs_nt<R, 8>(nt, grid, s, a); break;
-  case 16: rc = launch_mrhs_nt<R, 16>(nt, grid, s, a); break;
-  default: rc = launch_mrhs_nt<R, 32>(nt, grid, s, a); break;
+  if (mrhs_tri(op)) {
+    switch (rb) {
+    case 64: rc = launch_mrhs_tri<R, 64>(nt, grid, s, a); break;
+    case 8: rc = launch_mrhs_tri<R, 8>(nt, grid, s, a); break;
+    case 16: rc = launch_mrhs_tri<R, 16>(nt, grid, s, a); break;
+    default: rc = launch_mrhs_tri<R, 32>(nt, grid, s, a); break;
+    }
+  } else {
+    switch (rb) {
+    case 8: rc = launch_mrhs_nt<R, 8>(nt, grid, s, a); break;
+    case 16: rc = launch_mrhs_nt<R, 16>(nt, grid, s, a); break;
+    default: rc = launch_mrhs_nt<R, 32>(nt, grid, s, a); break;
+    }
   }
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
   return rc;
@@ -1503,7 +1629,9 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   size_t need = (size_t)op->chunks * nout * ne;
   if (op->rb) {
     // multi-RHS launch: one lane per target point, 256/nt triangles a block
-    const int tbn = (ne + 256 / op->rule.nt - 1) / (256 / op->rule.nt);
+    // (P0 test and trial: one lane per target triangle, 256 a block)
+    const int tpb = mrhs_tri(op) ? 256 : 256 / op->rule.nt;
+    const int tbn = (ne + tpb - 1) / tpb;
     const int groups = (op->d.max_rhs + op->rb - 1) / op->rb;
     int mc = std::max(1, (op->mgrid + tbn * groups - 1) / (tbn * groups));
     if (mc >= 8) mc = (mc + 7) / 8 * 8;
@@ -1642,12 +1770,18 @@ int op_build_far(ebh_op *op)
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
   EBH_TRY(op->tri_out.alloc(nout * ne * (size_t)op->d.max_rhs));
   if (op->d.kernel == EBH_LAPLACE_SL && op->d.max_rhs > 1) {
-    op->rb = op->d.max_rhs >= 32 ? 32 : op->d.max_rhs >= 16 ? 16 : 8;
+    // 64 RHS a group for the triangle-pair kernel (each pair's sum once)
+    op->rb = op->d.max_rhs >= 64 && mrhs_tri(op) ? 64
+             : op->d.max_rhs >= 32                ? 32
+             : op->d.max_rhs >= 16                ? 16
+                                                  : 8;
     const size_t groups = (op->d.max_rhs + op->rb - 1) / op->rb;
     const size_t npts = (size_t)ne * nt;
     if (op->d.far_precision == EBH_FP32) {
       EBH_TRY(op->mgeom32.alloc(npts * 4));
-      EBH_TRY(op->msig32.alloc(npts * groups * op->rb));
+      // FP64 strengths per triangle for the triangle-pair kernel
+      if (mrhs_tri(op)) EBH_TRY(op->msig64.alloc((size_t)ne * groups * op->rb));
+      else EBH_TRY(op->msig32.alloc(npts * groups * op->rb));
     } else {
       EBH_TRY(op->mgeom64.alloc(npts * 4));
       EBH_TRY(op->msig64.alloc(npts * groups * op->rb));

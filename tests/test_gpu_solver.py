@@ -95,6 +95,24 @@ def test_multi_rhs_apply_matches_single(basis, prec, tol):
             assert np.linalg.norm(Y[r] - ref) / np.linalg.norm(ref) <= tol
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+@pytest.mark.parametrize("rules", [(4, 4), (2, 1)])
+def test_multi_rhs_p0_rules(prec, tol, rules):
+    """The P0 multi-RHS kernel sums each triangle pair's point pairs before
+    the right-hand sides; other rule sizes against the single-RHS apply."""
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    kw = dict(line_rule=rules[0], tri_rule=rules[1])
+    one = L.Operator(m, "V", 0, 0, prec, **kw)
+    many = L.Operator(m, "V", 0, 0, prec, max_rhs=40, **kw)
+    X = np.random.default_rng(4).uniform(-1, 1, (37, one.ncols))
+    Y = many(X.ravel()).reshape(X.shape)
+    for r in range(X.shape[0]):
+        ref = one(X[r])
+        assert np.linalg.norm(Y[r] - ref) / np.linalg.norm(ref) <= tol
+
+
 def test_batched_solve_matches_individual_solves():
     from easybem_amd import _lib as L
     P, T = meshes.icosphere(3)
