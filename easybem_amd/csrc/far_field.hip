@@ -824,6 +824,9 @@ struct FarMArgs {
   const R *geom;                // [ne*nt][4]
   const R *sig;                 // [groups][ne*nt][RB]
   const double *sigd;           // [groups][ne][RB] (k_far_mrhs_tri)
+  const R *tsl;                 // median rules: target frame, as FarArgs::tsl
+  R rc[16];                     // median rules: c_q of point q
+  R kap[4];                     // median rules: -c c' / 2 for orbits (o, o')
   const int *touch_ptr, *touch;
   const double *tw;             // [nt][ne]
   double *partial;              // [chunks][nrhs][nout][ne]
@@ -920,7 +923,16 @@ __global__ __launch_bounds__(256) void k_far_mrhs(FarMArgs<R> a)
 // with the source weight in the fourth slot of each point and FP64 strengths
 // per triangle.  FP64 and non-packed FP32 FMAs issue at one rate, so the FP64
 // sums cost no more than FP32 ones and need no FP32 tile in registers.
-template <typename R, int NT, int RB>
+//
+// MED (rules of 6 or 7 points on the medians, point q = C + c_q (P_k - C)):
+// r^2 of each point pair from triangle-pair terms as in the pair-symmetric
+// kernel, |D|^2 + E_q + E'_b + c_q D.F_k - c'_b D.F'_l - (c_q c'_b / 2) F_k.F'_l
+// with D = C - C' and F = 2 (P - C): one add and one FMA per point pair.
+// The source record of triangle j is then C', F'_0, F'_1, E'_b, w_b
+// ([ne][MedRec]) instead of its points.
+template <int NT> struct MedRec { static constexpr int W = (9 + 2 * NT + 3) / 4 * 4; };
+
+template <typename R, int NT, int RB, bool MED>
 __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
 {
   const int ne = a.ne;
@@ -933,12 +945,30 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
   const int tt = t < ne ? t : ne - 1;
   const int j0 = s * a.chunk, j1 = min(ne, j0 + a.chunk);
   R X[NT], Y[NT], Z[NT], W[NT];
+  R Cx = 0, Cy = 0, Cz = 0, F0x = 0, F0y = 0, F0z = 0, F1x = 0, F1y = 0, F1z = 0;
 #pragma unroll
   for (int q = 0; q < NT; q++) {
-    X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
-    Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
-    Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+    if constexpr (MED) {
+      X[q] = a.tsl[(9 + q) * (size_t)ne + tt];  // E_q
+      Y[q] = a.rc[q];
+      Z[q] = R(0);
+    } else {
+      X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
+      Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
+      Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+    }
     W[q] = (R)a.tw[(size_t)q * ne + tt];
+  }
+  if constexpr (MED) {
+    Cx = a.tsl[0 * (size_t)ne + tt];
+    Cy = a.tsl[1 * (size_t)ne + tt];
+    Cz = a.tsl[2 * (size_t)ne + tt];
+    F0x = a.tsl[3 * (size_t)ne + tt];
+    F0y = a.tsl[4 * (size_t)ne + tt];
+    F0z = a.tsl[5 * (size_t)ne + tt];
+    F1x = a.tsl[6 * (size_t)ne + tt];
+    F1y = a.tsl[7 * (size_t)ne + tt];
+    F1z = a.tsl[8 * (size_t)ne + tt];
   }
   double acc64[RB];
 #pragma unroll
@@ -961,6 +991,54 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
     // G of the pair (t, js + jj), 0 for an excluded (touching) pair
     auto pair_sum = [&](int jj) -> R {
       const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      if constexpr (MED) {
+        constexpr int C0 = NT % 3;
+        const R *gp = geom + (size_t)(js + jj) * MedRec<NT>::W;
+        const R Dx = Cx - gp[0], Dy = Cy - gp[1], Dz = Cz - gp[2];
+        // a touching pair's expanded r^2 can round below zero at coincident
+        // points: lift it far away before the rsqrt (and multiply by 0 after)
+        const R excl = keep == R(0) ? R(1e30) : R(0);
+        const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, eps + excl)));
+        R U[3], P[3], Wk[3][3];
+        U[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
+        U[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
+        U[2] = -(U[0] + U[1]);
+        P[0] = fma(Dx, gp[3], fma(Dy, gp[4], Dz * gp[5]));
+        P[1] = fma(Dx, gp[6], fma(Dy, gp[7], Dz * gp[8]));
+        P[2] = -(P[0] + P[1]);
+        Wk[0][0] = fma(F0x, gp[3], fma(F0y, gp[4], F0z * gp[5]));
+        Wk[0][1] = fma(F0x, gp[6], fma(F0y, gp[7], F0z * gp[8]));
+        Wk[1][0] = fma(F1x, gp[3], fma(F1y, gp[4], F1z * gp[5]));
+        Wk[1][1] = fma(F1x, gp[6], fma(F1y, gp[7], F1z * gp[8]));
+        Wk[0][2] = -(Wk[0][0] + Wk[0][1]);
+        Wk[1][2] = -(Wk[1][0] + Wk[1][1]);
+#pragma unroll
+        for (int l = 0; l < 3; l++) Wk[2][l] = -(Wk[0][l] + Wk[1][l]);
+        R B[NT], Gq[NT];
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          B[q] = q < C0 ? R(0) : fma(Y[q], U[(q - C0) % 3], X[q]);
+          Gq[q] = R(0);
+        }
+#pragma unroll
+        for (int b = 0; b < NT; b++) {
+          const int lb = (b - C0) % 3, ob = (b - C0) / 3;
+          const R Ab = b < C0 ? A0 : fma(-a.rc[b], P[lb], A0 + gp[9 + b]);
+          const R wb = gp[9 + NT + b];
+#pragma unroll
+          for (int q = 0; q < NT; q++) {
+            const int kq = (q - C0) % 3, oq = (q - C0) / 3;
+            const R r2 = q < C0   ? Ab
+                         : b < C0 ? Ab + B[q]
+                                  : fma(a.kap[2 * oq + ob], Wk[kq][lb], Ab + B[q]);
+            Gq[q] = fma(wb, rsq(r2), Gq[q]);
+          }
+        }
+        R G = R(0);
+#pragma unroll
+        for (int q = 0; q < NT; q++) G = fma(W[q], Gq[q], G);
+        return G * keep;
+      }
       const R *gp = geom + (size_t)(js + jj) * NT * 4;
       R G = R(0);
 #pragma unroll
@@ -1001,14 +1079,24 @@ __global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
                             const int *__restrict__ tri, int trial_p1,
                             RuleArgs rule, const double *__restrict__ x,
                             int ncols, double scale, R *__restrict__ geom,
-                            R *__restrict__ sig, double *__restrict__ sigd)
+                            R *__restrict__ sig, double *__restrict__ sigd,
+                            const R *__restrict__ tsl, int med)
 {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int npts = ne * nt;
   if (idx >= npts * groups) return;
   const int g = idx / npts, p = idx - g * npts;
   const int j = p / nt, b = p - j * nt;
-  if (g == 0) {
+  if (TRI && med) {
+    // median record of source triangle j (k_far_mrhs_tri, MED)
+    if (g == 0) {
+      R *o = geom + (size_t)j * med;
+      if (b == 0)
+        for (int f = 0; f < 9; f++) o[f] = tsl[(size_t)f * ne + j];
+      o[9 + b] = tsl[(size_t)(9 + b) * ne + j];
+      o[9 + nt + b] = (R)swt[p];
+    }
+  } else if (g == 0) {
     R *o = geom + (size_t)p * 4;
     o[0] = (R)sg[3 * p];
     o[1] = (R)sg[3 * p + 1];
@@ -1420,15 +1508,40 @@ bool mrhs_tri(const ebh_op *op)
   return !op->d.test_basis && !op->d.trial_basis && op->rule.nt <= 7;
 }
 
+// record width of the median form of k_far_mrhs_tri (FP32, rules of 6 or 7
+// points on the medians; the record fits the 4 values per point of mgeom),
+// else 0.  FP64 keeps the point form: there the refined rsqrt dominates and
+// the median form's registers cost occupancy (measured 15% slower at L6).
+int mrhs_med(const ebh_op *op)
+{
+  const int nt = op->rule.nt;
+  if (!mrhs_tri(op) || !op->ray || nt < 6 || op->d.far_precision != EBH_FP32) return 0;
+  return nt == 6 ? MedRec<6>::W : MedRec<7>::W;
+}
+
 template <typename R, int RB>
-int launch_mrhs_tri(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
+int launch_mrhs_tri(int nt, bool med, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
 {
   switch (nt) {
-  case 1: hipLaunchKernelGGL((k_far_mrhs_tri<R, 1, RB>), grid, dim3(256), 0, s, a); break;
-  case 3: hipLaunchKernelGGL((k_far_mrhs_tri<R, 3, RB>), grid, dim3(256), 0, s, a); break;
-  case 4: hipLaunchKernelGGL((k_far_mrhs_tri<R, 4, RB>), grid, dim3(256), 0, s, a); break;
-  case 6: hipLaunchKernelGGL((k_far_mrhs_tri<R, 6, RB>), grid, dim3(256), 0, s, a); break;
-  case 7: hipLaunchKernelGGL((k_far_mrhs_tri<R, 7, RB>), grid, dim3(256), 0, s, a); break;
+  case 1: hipLaunchKernelGGL((k_far_mrhs_tri<R, 1, RB, false>), grid, dim3(256), 0, s, a); break;
+  case 3: hipLaunchKernelGGL((k_far_mrhs_tri<R, 3, RB, false>), grid, dim3(256), 0, s, a); break;
+  case 4: hipLaunchKernelGGL((k_far_mrhs_tri<R, 4, RB, false>), grid, dim3(256), 0, s, a); break;
+  case 6:
+    if constexpr (sizeof(R) == 4)
+      if (med) {
+        hipLaunchKernelGGL((k_far_mrhs_tri<R, 6, RB, true>), grid, dim3(256), 0, s, a);
+        break;
+      }
+    hipLaunchKernelGGL((k_far_mrhs_tri<R, 6, RB, false>), grid, dim3(256), 0, s, a);
+    break;
+  case 7:
+    if constexpr (sizeof(R) == 4)
+      if (med) {
+        hipLaunchKernelGGL((k_far_mrhs_tri<R, 7, RB, true>), grid, dim3(256), 0, s, a);
+        break;
+      }
+    hipLaunchKernelGGL((k_far_mrhs_tri<R, 7, RB, false>), grid, dim3(256), 0, s, a);
+    break;
   default:
     set_error("unsupported far-field rule size " + std::to_string(nt) +
               " for the P0 multi-RHS kernel");
@@ -1468,11 +1581,13 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   const int n = ne * nt * groups;
   const int p1 = op->d.trial_basis;
   const bool tri = mrhs_tri(op);
+  const int med = mrhs_med(op);
+  const R *tsl = sizeof(R) == 4 ? (const R *)op->tsl32.p : (const R *)op->tsl64.p;
   const dim3 fg((n + 255) / 256);
 #define EBH_FOLD(RB_, TRI_)                                                               \
   hipLaunchKernelGGL((k_fold_mrhs<R, RB_, TRI_>), fg, dim3(256), 0, s, ne, nt, nrhs,    \
                      groups, op->sg64.p, op->swt.p, op->mesh->tri.p, p1, op->rule, x,   \
-                     op->ncols, kInv4Pi, geom, sig, op->msig64.p)
+                     op->ncols, kInv4Pi, geom, sig, op->msig64.p, tsl, med)
   switch (rb) {
   case 64: EBH_FOLD(64, true); break;
   case 8: if (tri) EBH_FOLD(8, true); else EBH_FOLD(8, false); break;
@@ -1492,6 +1607,17 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   a.geom = geom;
   a.sig = sig;
   a.sigd = op->msig64.p;
+  a.tsl = tsl;
+  {
+    const int c0 = nt % 3, norb = (nt - c0) / 3;
+    for (int q = 0; q < 16; q++) a.rc[q] = (R)op->ray_c[q];
+    for (int o = 0; o < 2; o++)
+      for (int p = 0; p < 2; p++) {
+        const double co = o < norb ? op->ray_c[c0 + 3 * o] : 0.0;
+        const double cp = p < norb ? op->ray_c[c0 + 3 * p] : 0.0;
+        a.kap[2 * o + p] = (R)(-0.5 * co * cp);
+      }
+  }
   a.touch_ptr = op->mesh->touch_ptr.p;
   a.touch = op->mesh->touch.p;
   a.tw = op->tw.p;
@@ -1510,10 +1636,10 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   int rc;
   if (mrhs_tri(op)) {
     switch (rb) {
-    case 64: rc = launch_mrhs_tri<R, 64>(nt, grid, s, a); break;
-    case 8: rc = launch_mrhs_tri<R, 8>(nt, grid, s, a); break;
-    case 16: rc = launch_mrhs_tri<R, 16>(nt, grid, s, a); break;
-    default: rc = launch_mrhs_tri<R, 32>(nt, grid, s, a); break;
+    case 64: rc = launch_mrhs_tri<R, 64>(nt, med > 0, grid, s, a); break;
+    case 8: rc = launch_mrhs_tri<R, 8>(nt, med > 0, grid, s, a); break;
+    case 16: rc = launch_mrhs_tri<R, 16>(nt, med > 0, grid, s, a); break;
+    default: rc = launch_mrhs_tri<R, 32>(nt, med > 0, grid, s, a); break;
     }
   } else {
     switch (rb) {

@@ -96,10 +96,12 @@ def test_multi_rhs_apply_matches_single(basis, prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
-@pytest.mark.parametrize("rules", [(4, 4), (2, 1)])
+@pytest.mark.parametrize("rules", [(4, 4), (4, 3), (2, 1)])
 def test_multi_rhs_p0_rules(prec, tol, rules):
     """The P0 multi-RHS kernel sums each triangle pair's point pairs before
-    the right-hand sides; other rule sizes against the single-RHS apply."""
+    the right-hand sides (FP32 with 6- and 7-point median rules: r^2 from
+    triangle-pair terms); rule sizes 7, 6 and 3 against the single-RHS
+    apply."""
     from easybem_amd import _lib as L
     P, T = meshes.icosphere(3)
     m = L.Mesh(P, T)
