@@ -193,6 +193,9 @@ class Operator:
 
     def apply_host(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size == 0 or x.size % self.ncols:
+            raise ValueError(f"input of {x.size} entries is not a whole number "
+                             f"of {self.ncols}-entry vectors")
         nrhs = x.size // self.ncols
         y = np.empty(nrhs * self.nrows)
         check(lib().ebh_op_apply_host(self.h, _ptr(x), _ptr(y), nrhs))
@@ -201,6 +204,10 @@ class Operator:
     __call__ = apply_host
 
     def apply(self, x: DeviceArray, y: DeviceArray, nrhs=1, stream=None):
+        if nrhs < 1 or x.n < nrhs * self.ncols or y.n < nrhs * self.nrows:
+            raise ValueError(f"apply of {nrhs} vectors needs x of "
+                             f"{nrhs * self.ncols} and y of {nrhs * self.nrows} "
+                             f"entries (got {x.n} and {y.n})")
         check(lib().ebh_op_apply(self.h, x.ptr, y.ptr, nrhs, stream))
 
     def set_launch(self, block=0, tile=0, chunks=0):
@@ -333,6 +340,10 @@ class Solver:
     def solve_device(self, b: DeviceArray, x: DeviceArray, tol=1e-8,
                      mixed=False, max_iters=2000, inner_tol=1e-5,
                      max_outer=10, raise_on_noconv=True, nb=1):
+        if nb < 1 or nb > self.batch or b.n < nb * self.n or x.n < nb * self.n:
+            raise ValueError(f"solve of {nb} systems (batch {self.batch}) needs "
+                             f"b and x of {nb * self.n} entries (got {b.n} and "
+                             f"{x.n})")
         o = SolveOpts(self.restart, max_iters, tol, int(mixed), inner_tol,
                       max_outer)
         st = (SolveStats * nb)()
@@ -345,6 +356,9 @@ class Solver:
 
     def solve(self, b, x0=None, **kw):
         b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.size == 0 or b.size % self.n:
+            raise ValueError(f"right-hand side of {b.size} entries is not a "
+                             f"whole number of {self.n}-entry vectors")
         nb = b.size // self.n
         db = DeviceArray.from_host(b)
         dx = DeviceArray.from_host(np.zeros_like(b) if x0 is None else x0)
@@ -571,7 +585,7 @@ class RequestResult(C.Structure):
                 ("queue_ms", C.c_double), ("setup_ms", C.c_double),
                 ("solve_ms", C.c_double), ("total_ms", C.c_double),
                 ("ne", C.c_int), ("footprint", C.c_size_t),
-                ("cache_hit", C.c_int)]
+                ("cache_hit", C.c_int), ("device_peak", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -623,8 +637,10 @@ class Server:
 
     def wait(self, rid, timeout=0.0, raise_on_error=False):
         res = RequestResult()
-        t = np.empty(self.ne.pop(rid))
+        t = np.empty(self.ne[rid])
         rc = lib().ebh_server_wait(self.h, rid, timeout, C.byref(res), _ptr(t))
+        if rc != -7:  # EBH_ERR_STATE: timed out, the request is still pending
+            self.ne.pop(rid)
         if rc != EBH_OK and raise_on_error:
             check(rc)
         d = res.as_dict()
@@ -632,10 +648,11 @@ class Server:
         return d, (t if rc == EBH_OK else None)
 
     def stats(self):
-        a = (C.c_int64 * 8)()
+        a = (C.c_int64 * 10)()
         check(lib().ebh_server_stats(self.h, a))
         keys = ["submitted", "completed", "failed", "active", "queued",
-                "bytes_held", "peak_bytes", "cache_hits"]
+                "bytes_held", "peak_bytes", "cache_hits", "device_bytes",
+                "device_peak"]
         return dict(zip(keys, list(a)))
 
     def close(self):

@@ -98,6 +98,20 @@ int grid_of(const ebh_op *op)
 
 }  // namespace
 
+int tune_max_split(int ne)
+{
+  const int cls = (int)std::lround(2.0 * std::log2((double)std::max(ne, 1)));
+  const std::string tail = "/ne" + std::to_string(cls);
+  int best = 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (const auto &kv : g_table) {
+    const std::string &k = kv.first;
+    if (k.size() >= tail.size() && k.compare(k.size() - tail.size(), tail.size(), tail) == 0)
+      best = std::max(best, kv.second.sched);
+  }
+  return best;
+}
+
 void tune_apply(ebh_op *op)
 {
   Entry e;

@@ -10,6 +10,11 @@ namespace ebh {
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 const char *get_error() { return g_error.c_str(); }
+std::shared_ptr<MemAccount> &current_account()
+{
+  static thread_local std::shared_ptr<MemAccount> a;
+  return a;
+}
 }  // namespace ebh
 
 using namespace ebh;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -34,11 +35,43 @@ const char *get_error();
     if (rc_ != EBH_OK) return rc_;                                        \
   } while (0)
 
+// Device-byte accounting.  Every library allocation (DevBuf) is charged to
+// the account current on the allocating thread and to that account's
+// parents, and returned to the same accounts when it is freed, on whatever
+// thread.  The solve server gives each request an account whose parent is
+// the server's, so it can compare measured bytes with its estimates.
+struct MemAccount {
+  std::atomic<int64_t> live{0}, peak{0};
+  std::shared_ptr<MemAccount> parent;
+  void add(int64_t b)
+  {
+    for (MemAccount *a = this; a; a = a->parent.get()) {
+      const int64_t v = a->live.fetch_add(b) + b;
+      int64_t p = a->peak.load();
+      while (v > p && !a->peak.compare_exchange_weak(p, v)) {}
+    }
+  }
+};
+// the calling thread's current account (empty: allocations are not charged)
+std::shared_ptr<MemAccount> &current_account();
+// makes `a` the thread's current account until the scope ends
+struct AccountScope {
+  std::shared_ptr<MemAccount> prev;
+  explicit AccountScope(std::shared_ptr<MemAccount> a) : prev(current_account())
+  {
+    current_account() = std::move(a);
+  }
+  ~AccountScope() { current_account() = std::move(prev); }
+  AccountScope(const AccountScope &) = delete;
+  AccountScope &operator=(const AccountScope &) = delete;
+};
+
 // Owning device buffer (hipMalloc / hipFree).
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  std::shared_ptr<MemAccount> acct;  // charged with the bytes while allocated
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -46,6 +79,8 @@ struct DevBuf {
   void release()
   {
     if (p) (void)hipFree(p);
+    if (acct) acct->add(-(int64_t)(n * sizeof(T)));
+    acct.reset();
     p = nullptr;
     n = 0;
   }
@@ -61,6 +96,8 @@ struct DevBuf {
       return EBH_ERR_NOMEM;
     }
     n = count;
+    acct = current_account();
+    if (acct) acct->add((int64_t)(n * sizeof(T)));
     return EBH_OK;
   }
   int upload(const T *h, size_t count)
@@ -212,4 +249,7 @@ inline const ebh_op *far_op(const ebh_op *op) { return op->inner ? op->inner : o
 int chunks_for_grid(const ebh_op *op, int block, int grid);
 // applies a tuned launch shape recorded for this operator's key, if any
 void tune_apply(ebh_op *op);
+// the largest pair-symmetric step split the tuning table records for meshes
+// of ne triangles' size class (0 if none)
+int tune_max_split(int ne);
 }  // namespace ebh

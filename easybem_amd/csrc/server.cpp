@@ -38,15 +38,32 @@ double ms_between(Clock::time_point a, Clock::time_point b)
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// FNV-1a over the bytes of a mesh: a cached mesh is reused only for a
+// request whose mesh is the same
+uint64_t mesh_hash(const std::vector<double> &nodes, const std::vector<int> &elems)
+{
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t k = 0; k < n; k++) h = (h ^ c[k]) * 1099511628211ull;
+  };
+  mix(nodes.data(), nodes.size() * sizeof(double));
+  mix(elems.data(), elems.size() * sizeof(int));
+  return h;
+}
+
 struct Request {
   int64_t id = 0;
   int problem = EBH_DIRICHLET;
   int np = 0, ne = 0;
   std::vector<double> nodes;
   std::vector<int> elems;
+  uint64_t hash = 0;
   double x0[3] = {0, 0, 0};
   uint64_t key = 0;
   size_t footprint = 0;
+  // device bytes the library allocates for this request (set-up and solve)
+  std::shared_ptr<ebh::MemAccount> acct;
   Clock::time_point t_submit, t_admit, t_done;
   ebh_request_result res{};
   std::vector<double> t;
@@ -60,6 +77,8 @@ struct Request {
 // adjoint double layer K'.
 struct Entry {
   uint64_t key = 0;
+  int np = 0, ne = 0;
+  uint64_t hash = 0;  // mesh_hash of the mesh it was built for
   ebh_mesh *mesh = nullptr;
   ebh_op *A64 = nullptr, *A32 = nullptr, *K64 = nullptr;
   ebh_op *P64 = nullptr, *P32 = nullptr;  // right preconditioners
@@ -80,28 +99,54 @@ struct Entry {
 
 using CacheKey = std::pair<uint64_t, int>;  // (mesh key, problem)
 
-// Device bytes of one request, estimated before anything is allocated:
-// near-field pairs (about 13 per triangle for V, with their FP64 blocks
-// during set-up) and CSR, far-field quadrature data and partial sums, the
-// Krylov basis and work vectors.  Operators on the pair-symmetric schedule
-// keep (ne / 64) x nout x ne FP64 partial sums (up to 4 GiB each): the
-// system operators (Neumann: the P0 single layer inside D) in each
-// precision, and the Dirichlet right-hand side's FP64 double layer.
-size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1)
+// Far-field partial sums of one operator, nsum FP64 values per target
+// triangle and source slot (outputs per triangle times right-hand sides per
+// launch).  The pair-symmetric schedule keeps ceil(ne/64) x split slots while
+// that fits in 4 GiB (one step range if only that fits), else the chunked
+// schedule keeps about 4096 workgroups' chunks; the split is the larger of
+// the default and any tuned split for this size class, as op_configure
+// applies them.
+double partial_bytes(int ne, int nsum)
+{
+  const double lim = 4.0 * 1024 * 1024 * 1024;
+  const int ng = (ne + 63) / 64;
+  const int ks = std::max(ebh::sym_default_split(ng), ebh::tune_max_split(ne));
+  const double one = (double)ng * nsum * ne * 8.0;
+  if (one * ks <= lim) return one * ks;
+  if (one <= lim) return one;
+  const int tb = (ne + 255) / 256;
+  int chunks = std::max(8, (4096 + tb - 1) / tb);
+  chunks = (chunks + 7) / 8 * 8;
+  return (double)chunks * nsum * ne * 8.0;
+}
+
+// Device bytes of one request, estimated before anything is allocated: the
+// far-field partial sums of every operator the request builds (above), plus
+// the rest -- near-field pairs (about 14 per triangle, with their FP64
+// blocks during set-up) and CSR, far-field quadrature data, the Krylov basis
+// and work vectors, the mesh -- by a bound that measured requests stay
+// under.  Operators with partial sums:
+//   P0 Dirichlet: V (FP64, FP32) and the FP64 double layer, 1 value each;
+//   P1 Dirichlet: V (FP64, FP32) and the double layer with 3 values, the
+//     Calderon preconditioners (FP64, FP32) with a P0 V on 3 densities;
+//   Neumann: D (FP64, FP32) with a P0 V on 3 densities, the adjoint double
+//     layer with 3 values, the dual Calderon preconditioners (FP64, FP32)
+//     with a P1 V.
+size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1,
+                      bool precond)
 {
   const double pairs = 14.0 * ne;
   const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
   const double far_one = ne * 7.0 * (3 * 12 + 64 + 16) + 4096.0 * 256 * 8 * 2;
-  const double ops = (mixed ? 3 : 2) * (near / 2 + far_one);
+  const int nprec = precond && (neumann || p1) ? (mixed ? 2 : 1) : 0;
+  const int nops = (mixed ? 3 : 2) + nprec;
+  const double ops = nops * (near / 2 + far_one);
   const double krylov = (restart + 6.0) * ne * 8;
   const double mesh = np * 24.0 + ne * 100.0;
-  auto sym_partial = [&](int nout) {
-    const double b = std::ceil(ne / 64.0) * nout * (double)ne * 8.0;
-    return b <= 4.0 * 1024 * 1024 * 1024 ? b : 0.0;
-  };
-  const double partial = neumann ? (mixed ? 2 : 1) * sym_partial(1)
-                                 : (mixed ? 2 : 1) * sym_partial(p1 ? 3 : 1) +
-                                       sym_partial(p1 ? 3 : 1);
+  const int nsys = mixed ? 2 : 1;
+  const double partial = neumann || p1
+                             ? (nsys + 1 + nprec) * partial_bytes(ne, 3)
+                             : (nsys + 1) * partial_bytes(ne, 1);
   return (size_t)(1.25 * (near + ops + krylov + mesh) + partial) + (16u << 20);
 }
 
@@ -119,15 +164,27 @@ struct ebh_server {
   size_t budget = 0, held = 0, peak = 0;
   int64_t next_id = 1, n_sub = 0, n_done = 0, n_fail = 0, n_active = 0, n_hits = 0;
   uint64_t tick = 0;
+  // measured device bytes of all requests and resident meshes
+  std::shared_ptr<ebh::MemAccount> acct = std::make_shared<ebh::MemAccount>();
+  // evicted meshes, destroyed by a worker after it releases mu (their
+  // hipFree and stream destruction must not block submit, wait or stats)
+  std::vector<std::unique_ptr<Entry>> trash;
 
-  // must hold mu
+  // Must hold mu.  Evicts least recently used idle meshes until `need` more
+  // bytes fit, but only when evicting them all would let it fit: otherwise
+  // the cache stays as it is.
   bool evict_for(size_t need)
   {
+    if (held + need <= budget) return true;
+    size_t idle_bytes = 0;
+    for (auto &kv : idle) idle_bytes += kv.second->bytes;
+    if (held - idle_bytes + need > budget) return false;
     while (held + need > budget && !idle.empty()) {
       auto lru = idle.begin();
       for (auto it = idle.begin(); it != idle.end(); ++it)
         if (it->second->last_use < lru->second->last_use) lru = it;
       held -= lru->second->bytes;
+      trash.push_back(std::move(lru->second));
       idle.erase(lru);
     }
     return held + need <= budget;
@@ -138,6 +195,7 @@ struct ebh_server {
     while (true) {
       std::shared_ptr<Request> r;
       std::unique_ptr<Entry> entry;
+      std::vector<std::unique_ptr<Entry>> dead;
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] {
@@ -165,10 +223,23 @@ struct ebh_server {
         peak = std::max(peak, held);
         n_active++;
         r->t_admit = Clock::now();
+        dead.swap(trash);
       }
+      dead.clear();
       const bool hit = entry != nullptr;
-      int rc = run(*r, entry);
+      bool keep = false;  // a cached mesh that does not match stays cached
+      int rc;
+      {
+        ebh::AccountScope scope(r->acct);
+        rc = run(*r, entry, keep);
+      }
       if (rc != EBH_OK) r->err = ebh_last_error();
+      // bytes stay accounted while the mesh stays resident
+      const size_t bytes = hit ? entry->bytes : r->footprint;
+      const bool cache = entry && r->key && (rc == EBH_OK || keep);
+      // a mesh that is not kept is freed before the request reports done,
+      // outside the lock
+      if (!cache) entry.reset();
       {
         std::lock_guard<std::mutex> lk(mu);
         r->t_done = Clock::now();
@@ -176,12 +247,11 @@ struct ebh_server {
         r->res.queue_ms = ms_between(r->t_submit, r->t_admit);
         r->res.total_ms = ms_between(r->t_submit, r->t_done);
         r->res.footprint = r->footprint;
+        r->res.device_peak = (size_t)std::max<int64_t>(0, r->acct->peak.load());
         r->done = true;
         n_active--;
         (rc == EBH_OK ? n_done : n_fail)++;
-        // bytes stay accounted while the mesh stays resident
-        const size_t bytes = hit ? entry->bytes : r->footprint;
-        if (entry && r->key && rc == EBH_OK) {
+        if (cache) {
           entry->last_use = ++tick;
           entry->bytes = bytes;
           idle.emplace(CacheKey(r->key, r->problem), std::move(entry));
@@ -189,7 +259,6 @@ struct ebh_server {
           held -= bytes;
         }
       }
-      entry.reset();
       cv_done.notify_all();
       cv_work.notify_all();
     }
@@ -201,6 +270,9 @@ struct ebh_server {
   {
     e.reset(new Entry());
     e->key = r.key;
+    e->np = r.np;
+    e->ne = r.ne;
+    e->hash = r.hash;
     EBH_TRY(ebh_mesh_create(r.np, r.nodes.data(), r.ne, r.elems.data(), &e->mesh));
     const bool neu = r.problem == EBH_NEUMANN;
     const bool p1 = neu || o.dirichlet_p1;
@@ -227,8 +299,10 @@ struct ebh_server {
     ebh_mesh *mesh = e->mesh;
     using Result = std::pair<int, std::string>;
     std::vector<std::future<Result>> jobs;
+    std::shared_ptr<ebh::MemAccount> acct = r.acct;
     auto spawn = [&](auto fn) {
-      auto job = [mesh, fn]() {
+      auto job = [mesh, fn, acct]() {
+        ebh::AccountScope scope(acct);
         int rc = hipSetDevice(mesh->device) == hipSuccess ? EBH_OK : EBH_ERR_HIP;
         if (rc == EBH_OK) rc = fn();
         return Result(rc, rc == EBH_OK ? std::string() : std::string(ebh_last_error()));
@@ -359,9 +433,17 @@ struct ebh_server {
     return EBH_OK;
   }
 
-  int run(Request &r, std::unique_ptr<Entry> &e)
+  int run(Request &r, std::unique_ptr<Entry> &e, bool &keep)
   {
     const auto t0 = Clock::now();
+    if (e && (e->np != r.np || e->ne != r.ne || e->hash != r.hash)) {
+      // the key names another mesh: fail this request, keep the cached one
+      keep = true;
+      r.res.cache_hit = 0;
+      ebh::set_error("mesh_key " + std::to_string(r.key) +
+                     " is resident for a different mesh");
+      return EBH_ERR_ARG;
+    }
     r.res.ne = r.problem == EBH_NEUMANN || o.dirichlet_p1 ? r.np : r.ne;
     const bool neu = r.problem == EBH_NEUMANN;
     const int n = neu || o.dirichlet_p1 ? r.np : r.ne;
@@ -450,11 +532,15 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
   r->ne = ne;
   r->nodes.assign(nodes, nodes + (size_t)3 * np);
   r->elems.assign(elems, elems + (size_t)3 * ne);
+  if (mesh_key) r->hash = mesh_hash(r->nodes, r->elems);
   for (int k = 0; k < 3; k++) r->x0[k] = x0[k];
   r->key = mesh_key;
+  r->acct = std::make_shared<ebh::MemAccount>();
+  r->acct->parent = s->acct;
   r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0,
                                 problem == EBH_NEUMANN,
-                                problem == EBH_NEUMANN || s->o.dirichlet_p1 != 0);
+                                problem == EBH_NEUMANN || s->o.dirichlet_p1 != 0,
+                                s->o.no_precond == 0);
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -510,6 +596,8 @@ int ebh_server_stats(ebh_server *s, int64_t *st)
   st[5] = (int64_t)s->held;
   st[6] = (int64_t)s->peak;
   st[7] = s->n_hits;
+  st[8] = s->acct->live.load();
+  st[9] = s->acct->peak.load();
   return EBH_OK;
 }
 
@@ -523,6 +611,7 @@ void ebh_server_destroy(ebh_server *s)
   s->cv_work.notify_all();
   for (auto &w : s->workers) w.join();
   s->idle.clear();
+  s->trash.clear();
   delete s;
 }
 

@@ -400,7 +400,13 @@ void ebh_solver_destroy(ebh_solver *s);
  * admitted as soon as a slot is free and its estimated device footprint
  * (mesh, near-field CSR, far-field workspace, Krylov basis) fits in the
  * remaining budget.  Meshes submitted with the same non-zero mesh_key stay
- * resident (with their near-field blocks) and are reused. */
+ * resident (with their near-field blocks) and are reused; a request whose
+ * key is resident for a different mesh (other sizes, nodes or elements)
+ * fails with EBH_ERR_ARG and leaves the resident mesh in place.  Idle
+ * resident meshes are evicted, least recently used first, only when that
+ * lets the head of the queue in.  Every device allocation the library makes
+ * for a request is counted (ebh_request_result.device_peak, and the
+ * server's totals in ebh_server_stats). */
 typedef struct {
   int max_slots;       /* concurrent requests on the device (default 4)     */
   size_t hbm_budget;   /* bytes the server may hold; 0 = 80% of free HBM     */
@@ -429,6 +435,10 @@ typedef struct {
   int ne;              /* unknowns: triangles (Dirichlet) or nodes (Neumann) */
   size_t footprint;    /* estimated device bytes while admitted              */
   int cache_hit;       /* 1 when the mesh was already resident               */
+  size_t device_peak;  /* measured: peak device bytes the library held for
+                          this request (its set-up and solve allocations; a
+                          resident mesh it reuses is counted by the request
+                          that built it)                                     */
 } ebh_request_result;
 
 int ebh_server_create(const ebh_server_opts *opts, ebh_server **out);
@@ -453,8 +463,10 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
  * waited on once. */
 int ebh_server_wait(ebh_server *s, int64_t request_id, double timeout_s,
                     ebh_request_result *res, double *t);
-/* [0] submitted, [1] completed, [2] failed, [3] active, [4] queued,
- * [5] bytes admitted now, [6] peak bytes admitted, [7] cache hits. */
+/* stats[0..9]: [0] submitted, [1] completed, [2] failed, [3] active,
+ * [4] queued, [5] estimated bytes admitted now, [6] peak estimated bytes
+ * admitted, [7] cache hits, [8] measured device bytes held now (requests
+ * and resident meshes), [9] measured peak device bytes. */
 int ebh_server_stats(ebh_server *s, int64_t *stats);
 /* Finishes queued work, then stops the workers. */
 void ebh_server_destroy(ebh_server *s);

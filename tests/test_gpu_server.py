@@ -247,3 +247,45 @@ def test_setup_error_on_a_worker_thread_is_reported(p1):
     assert res2["status"] == 0 and res2["relres"] <= 1e-8
     assert np.all(np.isfinite(sol))
     assert st["failed"] == 1 and st["completed"] == 1
+
+
+def test_key_resident_for_another_mesh_fails_that_request_only():
+    """A mesh_key already resident for a different mesh: the request fails
+    with EBH_ERR_ARG before any apply, the resident mesh stays cached, and
+    a later request with the original mesh still hits it."""
+    from easybem_amd import _lib as L
+    (P, T, x0), (P2, T2, x2) = stream(2, seed=13, levels=(3, 2))
+    srv = L.Server(max_slots=1)
+    r1, t1 = srv.wait(srv.submit(P, T, x0, key=99))
+    r2, t2 = srv.wait(srv.submit(P2, T2, x2, key=99))
+    # same sizes, moved nodes: also another mesh
+    r3, _ = srv.wait(srv.submit(P * 1.01, T, x0, key=99))
+    r4, t4 = srv.wait(srv.submit(P, T, x0, key=99))
+    st = srv.stats()
+    srv.close()
+    assert r1["status"] == 0 and r4["status"] == 0
+    for r in (r2, r3):
+        assert r["status"] == -1 and "different mesh" in r["error"]
+    assert t2 is None and r4["cache_hit"] == 1 and np.array_equal(t1, t4)
+    assert st["failed"] == 2 and st["completed"] == 2
+
+
+def test_measured_device_bytes_within_estimate_and_budget():
+    """Every device allocation made for a request is counted: the measured
+    per-request peak stays within the request's admission estimate, and
+    the server's measured peak within its budget."""
+    from easybem_amd import _lib as L
+    reqs = stream(6, seed=14, levels=(3, 4))
+    budget = 600 << 20
+    srv = L.Server(max_slots=3, hbm_budget=budget)
+    ids = [srv.submit(P, T, x0, problem=("dirichlet", "neumann")[k % 2])
+           for k, (P, T, x0) in enumerate(reqs)]
+    for rid in ids:
+        res, _ = srv.wait(rid)
+        print(f"measured {res['device_peak']} of {res['footprint']} estimated")
+        assert res["status"] == 0
+        assert 0 < res["device_peak"] <= res["footprint"]
+    st = srv.stats()
+    srv.close()
+    print(st)
+    assert 0 < st["device_peak"] <= budget and st["device_bytes"] == 0
