@@ -140,7 +140,11 @@ int ebh_mesh_create(int np, const double *nodes, int ne, const int *elems,
   }
   m->device = dev;
   const Topology &T = m->T;
-  int rc = EBH_OK;
+  int rc = eval_scratch_resolve(dev);
+  if (rc != EBH_OK) {
+    delete m;
+    return rc;
+  }
   if ((rc = m->nodes.upload(T.nodes)) || (rc = m->tri.upload(T.tri)) ||
       (rc = m->touch_ptr.upload(T.touch_ptr)) || (rc = m->touch.upload(T.touch)) ||
       (rc = m->node_tri_ptr.upload(T.node_tri_ptr)) ||

@@ -249,6 +249,9 @@ inline const ebh_op *far_op(const ebh_op *op) { return op->inner ? op->inner : o
 int chunks_for_grid(const ebh_op *op, int block, int grid);
 // applies a tuned launch shape recorded for this operator's key, if any
 void tune_apply(ebh_op *op);
+// looks up the device address of ebh_evaluate_laplace_noalloc's workspace
+// on device dev (which must be current); once per device
+int eval_scratch_resolve(int dev);
 // the largest pair-symmetric step split the tuning table records for meshes
 // of ne triangles' size class (0 if none)
 int tune_max_split(int ne);

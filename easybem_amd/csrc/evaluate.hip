@@ -13,7 +13,10 @@
 // them in a fixed order.  No singular quadrature: the points must lie off
 // the surface, as in the reference.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "ebh_internal.h"
 
@@ -136,44 +139,72 @@ __global__ void k_eval_reduce(int npts, int chunks, const double *__restrict__ p
   out[p] = s;
 }
 
+// Source chunks of one evaluation: about 4096 workgroups, so that few
+// points split the sources into more chunks; at most max_chunks.
+void eval_chunks(int npts, int nsrc, size_t max_chunks, int *chunks_out, int *chunk_out)
+{
+  const int tb = (npts + 255) / 256;
+  size_t want = (size_t)std::max(1, std::min((4096 + tb - 1) / tb, (nsrc + 63) / 64));
+  int chunks = (int)std::max<size_t>(1, std::min(want, max_chunks));
+  const int chunk = (nsrc + chunks - 1) / chunks;
+  *chunks_out = (nsrc + chunk - 1) / chunk;
+  *chunk_out = chunk;
+}
+
+// The three launches of one evaluation on stream s with the source records
+// rec (nsrc records) and the partial sums part (chunks x npts).
+template <typename R>
+int eval_launch(const ebh_mesh *m, const RuleArgs &rule, int npts, const double *pts,
+                const double *t, const double *u, double *out, hipStream_t s, R *rec,
+                double *part, int chunks, int chunk)
+{
+  const int ne = m->T.ne, nt = rule.nt, nsrc = ne * nt;
+  hipLaunchKernelGGL((k_eval_fold<R>), dim3((nsrc + 255) / 256), dim3(256), 0, s, ne,
+                     nt, rule, m->nodes.p, m->tri.p, t, u, rec);
+  EBH_HIP(hipGetLastError());
+  const int tb = (npts + 255) / 256;
+  hipLaunchKernelGGL((k_eval<R>), dim3(tb * chunks), dim3(256), 0, s, npts, nsrc,
+                     chunks, chunk, pts, rec, part);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_eval_reduce, dim3((npts + 255) / 256), dim3(256), 0, s, npts,
+                     chunks, part, out);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 template <typename R>
 int eval_run(const ebh_mesh *m, const RuleArgs &rule, int npts, const double *pts,
              const double *t, const double *u, double *out, hipStream_t s)
 {
-  const int ne = m->T.ne, nt = rule.nt, nsrc = ne * nt;
+  const int nsrc = m->T.ne * rule.nt;
   DevBuf<R> rec;
   DevBuf<double> part;
   EBH_TRY(rec.alloc((size_t)nsrc * ERec<R>::n));
-  hipLaunchKernelGGL((k_eval_fold<R>), dim3((nsrc + 255) / 256), dim3(256), 0, s, ne,
-                     nt, rule, m->nodes.p, m->tri.p, t, u, rec.p);
-  EBH_HIP(hipGetLastError());
-  // about 4096 workgroups: few points split the sources into more chunks
-  const int tb = (npts + 255) / 256;
-  int chunks = std::max(1, std::min((4096 + tb - 1) / tb, (nsrc + 63) / 64));
-  const int chunk = (nsrc + chunks - 1) / chunks;
-  chunks = (nsrc + chunk - 1) / chunk;
+  int chunks, chunk;
+  eval_chunks(npts, nsrc, SIZE_MAX, &chunks, &chunk);
   EBH_TRY(part.alloc((size_t)chunks * npts));
-  hipLaunchKernelGGL((k_eval<R>), dim3(tb * chunks), dim3(256), 0, s, npts, nsrc,
-                     chunks, chunk, pts, rec.p, part.p);
-  EBH_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_eval_reduce, dim3((npts + 255) / 256), dim3(256), 0, s, npts,
-                     chunks, part.p, out);
-  EBH_HIP(hipGetLastError());
+  EBH_TRY(eval_launch<R>(m, rule, npts, pts, t, u, out, s, rec.p, part.p, chunks, chunk));
   // the scratch buffers are freed on return
   EBH_HIP(hipStreamSynchronize(s));
   return EBH_OK;
 }
 
-}  // namespace
-}  // namespace ebh
+// Workspace of ebh_evaluate_laplace_noalloc: the source records, then the
+// partial sums, in one file-scope device array that each GPU holds until
+// the process ends.  Host code never uses the array's name as an address:
+// its device address is looked up once per device (hipGetSymbolAddress,
+// with that device current, from ebh_mesh_create) and kept in g_scratch.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
-using namespace ebh;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
 
-extern "C" {
-
-int ebh_evaluate_laplace(ebh_mesh *mesh, int tri_rule_idx, int far_precision,
-                         int npts, const double *points, const double *t,
-                         const double *u, double *result, void *stream)
+// validated rule arguments of an evaluation call
+int eval_args(const ebh_mesh *mesh, int tri_rule_idx, int far_precision, int npts,
+              const double *points, const double *t, const double *u,
+              const double *result, RuleArgs &rule)
 {
   if (!mesh || !points || !t || !u || !result || npts < 1) {
     set_error("bad evaluation arguments");
@@ -188,18 +219,100 @@ int ebh_evaluate_laplace(ebh_mesh *mesh, int tri_rule_idx, int far_precision,
     set_error("tri_rule index must be 0..6");
     return EBH_ERR_UNSUPPORTED;
   }
-  RuleArgs rule{};
+  rule = RuleArgs{};
   rule.nt = (int)rw.size();
   for (int q = 0; q < rule.nt; q++) {
     rule.x[q] = rx[q];
     rule.y[q] = ry[q];
     rule.w[q] = rw[q];
   }
+  return EBH_OK;
+}
+
+template <typename R>
+int eval_noalloc(const ebh_mesh *m, const RuleArgs &rule, int npts, const double *pts,
+                 const double *t, const double *u, double *out, hipStream_t s,
+                 unsigned char *base)
+{
+  const size_t nsrc = (size_t)m->T.ne * rule.nt;
+  // records first; the partial sums start at the next 256-byte boundary
+  const size_t rec_bytes = (nsrc * ERec<R>::n * sizeof(R) + 255) / 256 * 256;
+  const size_t row = (size_t)npts * sizeof(double);
+  if (rec_bytes + row > kScratchBytes) {
+    set_error("evaluation needs " + std::to_string(rec_bytes + row) +
+              " bytes of workspace (source records of " + std::to_string(nsrc) +
+              " points and one partial sum per point); the allocation-free "
+              "entry point holds " + std::to_string(kScratchBytes));
+    return EBH_ERR_ARG;
+  }
+  int chunks, chunk;
+  eval_chunks(npts, (int)nsrc, (kScratchBytes - rec_bytes) / row, &chunks, &chunk);
+  return eval_launch<R>(m, rule, npts, pts, t, u, out, s, reinterpret_cast<R *>(base),
+                        reinterpret_cast<double *>(base + rec_bytes), chunks, chunk);
+}
+
+}  // namespace
+
+// Device address of the workspace array on device dev (the current device).
+int eval_scratch_resolve(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index " + std::to_string(dev) + " out of range");
+    return EBH_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return EBH_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress of the evaluation workspace failed: ") +
+              hipGetErrorString(e));
+    return EBH_ERR_HIP;
+  }
+  g_scratch[dev] = ptr;
+  return EBH_OK;
+}
+
+}  // namespace ebh
+
+using namespace ebh;
+
+extern "C" {
+
+int ebh_evaluate_laplace(ebh_mesh *mesh, int tri_rule_idx, int far_precision,
+                         int npts, const double *points, const double *t,
+                         const double *u, double *result, void *stream)
+{
+  RuleArgs rule;
+  EBH_TRY(eval_args(mesh, tri_rule_idx, far_precision, npts, points, t, u, result, rule));
   EBH_HIP(hipSetDevice(mesh->device));
   hipStream_t s = (hipStream_t)stream;
   return far_precision == EBH_FP32
              ? eval_run<float>(mesh, rule, npts, points, t, u, result, s)
              : eval_run<double>(mesh, rule, npts, points, t, u, result, s);
+}
+
+int ebh_evaluate_laplace_noalloc(ebh_mesh *mesh, int tri_rule_idx, int far_precision,
+                                 int npts, const double *points, const double *t,
+                                 const double *u, double *result, void *stream)
+{
+  RuleArgs rule;
+  EBH_TRY(eval_args(mesh, tri_rule_idx, far_precision, npts, points, t, u, result, rule));
+  EBH_HIP(hipSetDevice(mesh->device));
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (mesh->device >= 0 && mesh->device < kMaxDevices) base = g_scratch[mesh->device];
+  }
+  if (!base) {
+    set_error("evaluation workspace not set up on this device (ebh_mesh_create sets it up)");
+    return EBH_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  auto *b = static_cast<unsigned char *>(base);
+  return far_precision == EBH_FP32
+             ? eval_noalloc<float>(mesh, rule, npts, points, t, u, result, s, b)
+             : eval_noalloc<double>(mesh, rule, npts, points, t, u, result, s, b);
 }
 
 int ebh_evaluate_laplace_host(ebh_mesh *mesh, int tri_rule_idx, int far_precision,

@@ -214,6 +214,23 @@ int ebh_evaluate_laplace(ebh_mesh *mesh, int tri_rule, int far_precision,
 int ebh_evaluate_laplace_host(ebh_mesh *mesh, int tri_rule, int far_precision,
                               int npts, const double *points, const double *t,
                               const double *u, double *result);
+/* Allocation-free variant of ebh_evaluate_laplace.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels, with the source records and partial sums
+ * in a 64 MiB workspace that the library holds on each GPU until the
+ * process ends (set up by ebh_mesh_create); the call allocates nothing and
+ * does not synchronise (the result is ready when `stream` is).  Limit: the
+ * source records (48 B per far-rule point in FP32, 64 B in FP64; ne x nt
+ * points) plus 8 B per evaluation point must fit in 64 MiB, else
+ * EBH_ERR_ARG; within that, fewer source chunks are used where the
+ * partial sums would not fit (the sum order then differs from
+ * ebh_evaluate_laplace's).  Every call on one GPU shares that GPU's
+ * workspace, so calls that overlap on one GPU race: issue them on one
+ * stream or synchronise between them. */
+int ebh_evaluate_laplace_noalloc(ebh_mesh *mesh, int tri_rule, int far_precision,
+                                 int npts, const double *points, const double *t,
+                                 const double *u, double *result, void *stream);
 
 /* ---- multi-mesh batches ------------------------------------------------ */
 /* No counterpart in the reference, which keeps one mesh at a time
