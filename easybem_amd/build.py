@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-fgpu-flush-denormals-to-zero"]
 
 LIB_SOURCES = ["host_geometry.cpp", "near_field.hip", "far_field.hip",
-               "capi.hip", "solver.hip", "legacy_api.cpp", "server.cpp",
+               "capi.hip", "solver.hip", "legacy_api.cpp", "legacy_dense.cpp", "server.cpp",
                "autotune.cpp", "hypersingular.hip", "evaluate.hip",
                "dense.hip", "elasticity.hip", "spai.hip"]
 
@@ -82,9 +82,25 @@ def build_module() -> str:
     return out
 
 
+def build_cpp_check() -> str:
+    """tests/cpp/dropin_check: a C++ caller of the drop-in dense surface
+    (include/BEM3dLaplace.h, include/BEM3dElasticity.h) linked against the
+    library, run by the tests."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
+    if not os.path.exists(src):
+        return ""
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_check")
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src,
+          "-o", out + ".tmp", "-L" + PKG, "-leasybem_hip",
+          "-Wl,-rpath,$ORIGIN/../../easybem_amd"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(jobs: int = 8) -> None:
     build_lib(jobs)
     build_module()
+    build_cpp_check()
 
 
 if __name__ == "__main__":

@@ -258,7 +258,7 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (nq <= 0) { set_error("line_rule index must be 0..18"); return EBH_ERR_ARG; }
   std::vector<double> rx, ry, rw;
   if (!tri_rule(d.tri_rule, rx, ry, rw)) {
-    set_error("tri_rule index must be 0..6");
+    set_error("tri_rule index must be 0..19");
     return EBH_ERR_UNSUPPORTED;
   }
   ebh_op *op = new ebh_op();

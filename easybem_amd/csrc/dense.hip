@@ -43,7 +43,7 @@ struct DenseArgs {
   const double *normals; // [ne][3]
   const int *touch_ptr, *touch;
   const int *ntp, *ntri, *ntk;  // node -> (triangle, local index)
-  double phi[16][3];     // trial shape functions at the rule points
+  double phi[kMaxTriPoints][3];  // trial shape functions at the rule points
 };
 
 __global__ __launch_bounds__(256) void k_dense_v(DenseArgs a, double *__restrict__ V)
@@ -195,7 +195,7 @@ DenseArgs dense_args(const ebh_op *v)
   a.ntp = v->mesh->node_tri_ptr.p;
   a.ntri = v->mesh->node_tri.p;
   a.ntk = v->mesh->node_tri_k.p;
-  for (int q = 0; q < 16; q++) {
+  for (int q = 0; q < kMaxTriPoints; q++) {
     const bool in = q < a.nt;
     a.phi[q][0] = in ? 1.0 - v->rule.x[q] - v->rule.y[q] : 0.0;
     a.phi[q][1] = in ? v->rule.x[q] : 0.0;

@@ -135,8 +135,8 @@ struct ebh_mesh {
 // Far-field rule data passed by value to kernels.
 struct RuleArgs {
   int nt;                 // points per triangle
-  double x[16], y[16];    // barycentric-parameter coordinates of the points
-  double w[16];           // weights (sum 1/2)
+  double x[ebh::kMaxTriPoints], y[ebh::kMaxTriPoints];  // parameter coordinates
+  double w[ebh::kMaxTriPoints];                         // weights (sum 1/2)
 };
 
 struct ebh_op {
@@ -146,7 +146,7 @@ struct ebh_op {
   RuleArgs rule{};
   // rule in centroid-median order (ray_order): point q = C + ray_c[q] (P_k - C)
   bool ray = false;
-  double ray_c[16] = {};
+  double ray_c[ebh::kMaxTriPoints] = {};
   int nc = 1;                    // components (3 for Kelvin)
   int nrows = 0, ncols = 0;      // test / trial DOFs (per component)
   int nout = 1;                  // far outputs per target triangle (x nc)
@@ -161,6 +161,8 @@ struct ebh_op {
   ebh::DevBuf<double> tsl64;
   ebh::DevBuf<double> tq64, sg64;
   ebh::DevBuf<double> tw, swt;       // w_q J per target / source point
+  ebh::DevBuf<double> rphi;          // rules without a register kernel: the
+                                     // test shape functions [nt][3] at the points
   ebh::DevBuf<double> normals;       // ne*3 unit normals
   // per-apply workspace
   ebh::DevBuf<float> src32;
@@ -245,6 +247,8 @@ int composite_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_
 // the hypersingular operator)
 inline ebh_op *far_op(ebh_op *op) { return op->inner ? op->inner : op; }
 inline const ebh_op *far_op(const ebh_op *op) { return op->inner ? op->inner : op; }
+// rule sizes with a register-resident far kernel (1, 3, 4, 6, 7, 12, 13)
+bool reg_rule(int nt);
 // source chunks that give about `grid` workgroups at this block size
 int chunks_for_grid(const ebh_op *op, int block, int grid);
 // applies a tuned launch shape recorded for this operator's key, if any

@@ -216,7 +216,7 @@ int eval_args(const ebh_mesh *mesh, int tri_rule_idx, int far_precision, int npt
   }
   std::vector<double> rx, ry, rw;
   if (!tri_rule(tri_rule_idx, rx, ry, rw)) {
-    set_error("tri_rule index must be 0..6");
+    set_error("tri_rule index must be 0..19");
     return EBH_ERR_UNSUPPORTED;
   }
   rule = RuleArgs{};

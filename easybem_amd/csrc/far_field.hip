@@ -60,6 +60,17 @@ __device__ __forceinline__ double rsq(double x)
   return fma(0.5 * y, e, y);
 }
 
+}  // namespace
+
+// rule sizes whose points a lane keeps in registers (k_far, k_far_multi and
+// the multi-RHS kernels); other rules (16 to 79 points) run k_far_pt
+bool reg_rule(int nt)
+{
+  return nt == 1 || nt == 3 || nt == 4 || nt == 6 || nt == 7 || nt == 12 || nt == 13;
+}
+
+namespace {
+
 template <typename R>
 struct FarArgs {
   int ne, chunks, chunk, tile, nout;  // nout: 1 (P0 test) or 3 (P1 test)
@@ -326,6 +337,147 @@ __global__ __launch_bounds__(256) void k_far(FarArgs<R> a)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   far_block<R, KIND, NT>(a, blockIdx.x % a.chunks, blockIdx.x / a.chunks,
                          reinterpret_cast<R *>(smem_raw));
+}
+
+// Rules whose points do not fit a lane's registers (triangle rule indices
+// 7..19, 16 to 79 points; the reference's `order` 7 and above): one lane per
+// target quadrature point, a workgroup of 256 lanes holds TB = 256 / nt whole
+// target triangles, and source triangles pass through LDS in tiles of TS
+// triangles (all lanes read the same record: a broadcast).  Per interaction
+// the point forms of far_block: x - y, r^2, rsqrt, the kernel's factor; an
+// excluded (touching) source triangle has strength 0 and r^2 + eps.  The NT
+// lanes of a triangle then reduce their weighted point sums through LDS in a
+// fixed order into partial[chunk][out][triangle], as far_block writes them.
+template <typename R>
+struct FarPtArgs {
+  int ne, nt, chunks, chunk, ts, nout;  // ts: source triangles per LDS tile
+  R kappa;                              // Kelvin: 3 - 4 nu
+  const R *tq;                          // [nt][3][ne] (FP32 double layer:
+                                        // offsets from the centroid)
+  const R *tc;                          // FP32 double layer: centroid hi/lo [6][ne]
+  const double *tn;                     // adjoint double layer: normals [ne][3]
+  const R *src;                         // [ne*nt][REC]
+  const int *touch_ptr, *touch;
+  const double *tw;                     // [nt][ne]
+  const double *phi;                    // [nt][3] test shape functions
+  double *partial;                      // [chunks][nout*nc][ne]
+};
+
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_far_pt(FarPtArgs<R> a)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int REC = Rec<R, KIND>::n;
+  constexpr int NC = KIND == 2 ? 3 : 1;
+  constexpr bool COMP = sizeof(R) == 4 && KIND == 1;
+  R *tile = reinterpret_cast<R *>(smem_raw);
+  double *red = reinterpret_cast<double *>(smem_raw);  // reused after the tiles
+  const int ne = a.ne, nt = a.nt, TB = 256 / nt;
+  const int s = blockIdx.x % a.chunks, tb = blockIdx.x / a.chunks;
+  const int lt = threadIdx.x / nt, q = threadIdx.x - lt * nt;
+  const bool lane_ok = lt < TB;
+  const int t = tb * TB + lt;
+  const int tt = (lane_ok && t < ne) ? t : ne - 1;
+  const int qq = lane_ok ? q : 0;
+  const int j0 = s * a.chunk, j1 = min(ne, j0 + a.chunk);
+  const R X = a.tq[(3 * qq + 0) * (size_t)ne + tt];
+  const R Y = a.tq[(3 * qq + 1) * (size_t)ne + tt];
+  const R Z = a.tq[(3 * qq + 2) * (size_t)ne + tt];
+  R Ch[3] = {0, 0, 0}, Cl[3] = {0, 0, 0};
+  if constexpr (COMP) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Ch[k] = a.tc[k * (size_t)ne + tt];
+      Cl[k] = a.tc[(3 + k) * (size_t)ne + tt];
+    }
+  }
+  R Nx = 0, Ny = 0, Nz = 0;
+  if constexpr (KIND == 3) {
+    Nx = (R)a.tn[3 * (size_t)tt];
+    Ny = (R)a.tn[3 * (size_t)tt + 1];
+    Nz = (R)a.tn[3 * (size_t)tt + 2];
+  }
+  double acc64[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) acc64[c] = 0.0;
+  int cur = a.touch_ptr[tt];
+  const int cend = a.touch_ptr[tt + 1];
+  while (cur < cend && a.touch[cur] < j0) cur++;
+  int nextx = cur < cend ? a.touch[cur] : INT_MAX;
+  const R eps = Eps<R>::v;
+  for (int js = j0; js < j1; js += a.ts) {
+    const int nts = min(a.ts, j1 - js);
+    __syncthreads();
+    {
+      const uint4 *g = reinterpret_cast<const uint4 *>(a.src + (size_t)js * nt * REC);
+      uint4 *l = reinterpret_cast<uint4 *>(tile);
+      const int nv = nts * nt * REC * (int)sizeof(R) / 16;
+      for (int k = threadIdx.x; k < nv; k += blockDim.x) l[k] = g[k];
+    }
+    __syncthreads();
+    uint64_t mask = 0;
+    while (nextx < js + nts) {
+      mask |= 1ull << (nextx - js);
+      ++cur;
+      nextx = cur < cend ? a.touch[cur] : INT_MAX;
+    }
+    R acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = R(0);
+    for (int jj = 0; jj < nts; jj++) {
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      const R *rec = tile + jj * nt * REC;
+      for (int b = 0; b < nt; b++, rec += REC) {
+        R dx, dy, dz;
+        if constexpr (COMP) {
+          dx = ((Ch[0] - rec[0]) + (Cl[0] - rec[3])) + X;
+          dy = ((Ch[1] - rec[1]) + (Cl[1] - rec[4])) + Y;
+          dz = ((Ch[2] - rec[2]) + (Cl[2] - rec[5])) + Z;
+        } else {
+          dx = X - rec[0];
+          dy = Y - rec[1];
+          dz = Z - rec[2];
+        }
+        const R ri = rsq(fma(dx, dx, fma(dy, dy, fma(dz, dz, eps))));
+        if constexpr (KIND == 0) {
+          acc[0] = fma(rec[3] * keep, ri, acc[0]);
+        } else if constexpr (KIND == 3) {
+          // (y - x).n_x sigma / r^3
+          const R dn = fma(dx, Nx, fma(dy, Ny, dz * Nz));
+          acc[0] = fma(-rec[3] * keep * dn, ri * ri * ri, acc[0]);
+        } else if constexpr (KIND == 1) {
+          const R *p = rec + (COMP ? 6 : 3);
+          const R dp = fma(dx, p[0], fma(dy, p[1], dz * p[2])) * keep;
+          acc[0] = fma(dp, ri * ri * ri, acc[0]);
+        } else {
+          const R gx = rec[3] * keep, gy = rec[4] * keep, gz = rec[5] * keep;
+          const R ds = fma(dx, gx, fma(dy, gy, dz * gz)) * (ri * ri);
+          acc[0] = fma(ri, fma(dx, ds, a.kappa * gx), acc[0]);
+          acc[1] = fma(ri, fma(dy, ds, a.kappa * gy), acc[1]);
+          acc[2] = fma(ri, fma(dz, ds, a.kappa * gz), acc[2]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc64[c] += (double)acc[c];
+  }
+  // weighted point sums of each triangle, one output at a time
+  const double wq = lane_ok ? a.tw[(size_t)qq * ne + tt] : 0.0;
+  for (int k = 0; k < a.nout; k++)
+    for (int c = 0; c < NC; c++) {
+      __syncthreads();
+      const double f = a.nout == 1 ? 1.0 : a.phi[3 * qq + k];
+      red[threadIdx.x] = lane_ok ? wq * f * acc64[c] : 0.0;
+      __syncthreads();
+      if ((int)threadIdx.x < TB) {
+        const int t2 = tb * TB + (int)threadIdx.x;
+        if (t2 < ne) {
+          double v = 0.0;
+          for (int q2 = 0; q2 < nt; q2++) v += red[threadIdx.x * nt + q2];
+          a.partial[((size_t)s * a.nout * NC + k * NC + c) * ne + t2] = v;
+        }
+      }
+    }
 }
 
 // Persistent multi-mesh far field: one launch for several operators of one
@@ -1445,9 +1597,64 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
                  : launch_sym<R, 1>(op->rule.nt, grid, s, a);
 }
 
+// source triangles per LDS tile of k_far_pt (at most 64, the mask width, and
+// 32 KiB of records)
+int pt_tile(const ebh_op *op, int R_bytes)
+{
+  const int k = op->d.kernel;
+  const int rec = k == EBH_LAPLACE_SL || k == EBH_LAPLACE_ADL
+                      ? 4 : (k == EBH_LAPLACE_DL && R_bytes == 4 ? 12 : 8);
+  const int per = op->rule.nt * rec * R_bytes;
+  return std::max(1, std::min(64, 32768 / per));
+}
+
+template <typename R>
+int run_far_pt(ebh_op *op, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne, nt = op->rule.nt;
+  FarPtArgs<R> a;
+  a.ne = ne;
+  a.nt = nt;
+  a.chunks = op->chunks;
+  a.chunk = op->chunk;
+  a.ts = pt_tile(op, (int)sizeof(R));
+  a.nout = op->d.test_basis ? 3 : 1;
+  a.kappa = (R)(3.0 - 4.0 * op->d.nu);
+  if constexpr (sizeof(R) == 4) {
+    a.tq = op->tq32.p;
+    a.tc = op->tc32.p;
+    a.src = op->src32.p;
+  } else {
+    a.tq = op->tq64.p;
+    a.tc = nullptr;
+    a.src = op->src64.p;
+  }
+  a.tn = op->normals.p;
+  a.touch_ptr = op->mesh->touch_ptr.p;
+  a.touch = op->mesh->touch.p;
+  a.tw = op->tw.p;
+  a.phi = op->rphi.p;
+  a.partial = op->partial.p;
+  const int k = op->d.kernel;
+  const int rec = k == EBH_LAPLACE_SL || k == EBH_LAPLACE_ADL
+                      ? 4 : (k == EBH_LAPLACE_DL && sizeof(R) == 4 ? 12 : 8);
+  const size_t lds = std::max((size_t)a.ts * nt * rec * sizeof(R), (size_t)256 * 8);
+  const int tbn = (ne + 256 / nt - 1) / (256 / nt);
+  const dim3 grid((unsigned)(tbn * op->chunks)), blk(256);
+  switch (k) {
+  case EBH_LAPLACE_SL: hipLaunchKernelGGL((k_far_pt<R, 0>), grid, blk, lds, s, a); break;
+  case EBH_LAPLACE_DL: hipLaunchKernelGGL((k_far_pt<R, 1>), grid, blk, lds, s, a); break;
+  case EBH_LAPLACE_ADL: hipLaunchKernelGGL((k_far_pt<R, 3>), grid, blk, lds, s, a); break;
+  default: hipLaunchKernelGGL((k_far_pt<R, 2>), grid, blk, lds, s, a); break;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 template <typename R>
 int run_far(ebh_op *op, hipStream_t s)
 {
+  if (!reg_rule(op->rule.nt)) return run_far_pt<R>(op, s);
   FarArgs<R> a;
   fill_far_args(op, a);
   if (op->sym) return run_far_sym<R>(op, s, a);
@@ -1657,7 +1864,9 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
 int chunks_for_grid(const ebh_op *op, int block, int grid)
 {
   const int ne = op->mesh->T.ne;
-  const int tb = (ne + block - 1) / block;
+  // k_far_pt holds 256 / nt target triangles a workgroup
+  const int tpb = reg_rule(op->rule.nt) ? block : 256 / op->rule.nt;
+  const int tb = (ne + tpb - 1) / tpb;
   int chunks = std::max(1, (grid + tb - 1) / tb);
   if (chunks >= 8) chunks = (chunks + 7) / 8 * 8;
   return std::min(chunks, ne);
@@ -1838,9 +2047,10 @@ int op_build_far(ebh_op *op)
                             (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL
                                  ? 4 : 8)));
   }
-  if (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL ||
-      (op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 && op->ray &&
-       nt <= 7)) {
+  if (reg_rule(nt) &&
+      (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL ||
+       (op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 && op->ray &&
+        nt <= 7))) {
     // single-layer target frame: centroid, then the median form (ray rules)
     // or the offsets of every point from the centroid; the FP64 double layer
     // uses it on the symmetric schedule
@@ -1895,7 +2105,19 @@ int op_build_far(ebh_op *op)
   EBH_TRY(op->normals.upload(nrm));
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
   EBH_TRY(op->tri_out.alloc(nout * ne * (size_t)op->d.max_rhs));
-  if (op->d.kernel == EBH_LAPLACE_SL && op->d.max_rhs > 1) {
+  if (!reg_rule(nt)) {
+    // k_far_pt reads the test shape functions at the points from the device
+    std::vector<double> phi(3 * (size_t)nt);
+    for (int q = 0; q < nt; q++) {
+      phi[3 * q] = 1.0 - op->rule.x[q] - op->rule.y[q];
+      phi[3 * q + 1] = op->rule.x[q];
+      phi[3 * q + 2] = op->rule.y[q];
+    }
+    EBH_TRY(op->rphi.upload(phi));
+  }
+  // multi-RHS kernels for the rules a lane holds in registers; other rules
+  // apply their right-hand sides one at a time
+  if (op->d.kernel == EBH_LAPLACE_SL && op->d.max_rhs > 1 && reg_rule(nt)) {
     // 64 RHS a group for the triangle-pair kernel (each pair's sum once)
     op->rb = op->d.max_rhs >= 64 && mrhs_tri(op) ? 64
              : op->d.max_rhs >= 32                ? 32
@@ -2399,6 +2621,11 @@ int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out)
       }
     if (ops[k]->inner || ops[k]->pscale) {
       set_error("batch the operators a composite operator wraps instead");
+      return EBH_ERR_UNSUPPORTED;
+    }
+    if (!reg_rule(ops[k]->rule.nt)) {
+      set_error("batches take far rules of at most 13 points (triangle rule "
+                "indices 0..6)");
       return EBH_ERR_UNSUPPORTED;
     }
     if (ops[k]->d.kernel != ops[0]->d.kernel ||

@@ -16,7 +16,8 @@ int line_rule_size(int idx);
 bool gauss01(int n, std::vector<double> &p, std::vector<double> &w);
 bool line_rule(int idx, std::vector<double> &p, std::vector<double> &w);
 // Symmetric triangle rules, index d-1 of polynomial degree d, weights summing
-// to 1/2 (src/quadrature.cpp:1047-1048).  Indices 0..6 are supported.
+// to 1/2 (src/quadrature.cpp:1047-1048), indices 0..19 (1 to 79 points).
+constexpr int kMaxTriPoints = 80;
 bool tri_rule(int idx, std::vector<double> &x, std::vector<double> &y,
               std::vector<double> &w);
 // Reorders a rule whose points are the centroid or lie on the medians

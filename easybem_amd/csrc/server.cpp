@@ -397,7 +397,7 @@ struct ebh_server {
   {
     std::vector<double> rx, ry, rw;
     if (!ebh::tri_rule(o.tri_rule, rx, ry, rw)) {
-      ebh::set_error("tri_rule index must be 0..6");
+      ebh::set_error("tri_rule index must be 0..19");
       return EBH_ERR_ARG;
     }
     std::vector<double> t(r.ne), kt(r.np);

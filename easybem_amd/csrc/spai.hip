@@ -282,6 +282,12 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
               "test and trial bases or the hypersingular operator");
     return EBH_ERR_UNSUPPORTED;
   }
+  // the patch quadrature keeps a triangle's rule points in registers
+  if (far_op(A)->rule.nt > 16) {
+    set_error("the sparse approximate inverse takes far rules of at most 16 points "
+              "(triangle rule index 7)");
+    return EBH_ERR_UNSUPPORTED;
+  }
   EBH_HIP(hipSetDevice(A->mesh->device));
   EBH_HIP(hipStreamSynchronize(A->stream));
   const Topology &T = A->mesh->T;

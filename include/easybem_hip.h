@@ -84,7 +84,8 @@ typedef struct {
   int trial_basis;   /* DL requires EBH_P1; Kelvin requires test==trial   */
   int far_precision; /* EBH_FP64 or EBH_FP32 (near field is always FP64) */
   int line_rule;     /* near-field Gauss index, 0..18                     */
-  int tri_rule;      /* far-field triangle rule index, 0..6               */
+  int tri_rule;      /* far-field triangle rule index, 0..19 (1 to 79
+                        points; 16+ points: one lane per target point)   */
   double nu;         /* Poisson ratio for Kelvin (E = 1)                  */
   int max_rhs;       /* largest nrhs passed to ebh_op_apply (>= 1)        */
   double stab;       /* EBH_LAPLACE_HS: adds stab m (m.x), m_n the lumped

@@ -28,7 +28,7 @@
 #include "oracle.h"
 
 #define ORC_PI 3.14159265358979323846
-#define MAXQ 64
+#define MAXQ 80
 
 /* ------------------------------------------------------------------------ */
 /* Quadrature                                                               */
@@ -87,7 +87,8 @@ int orc_line_rule(int idx, double *pts, double *wts)
  * summing to 1/2, indexed like the reference table
  * (src/quadrature.cpp:1047-1048): index d-1 has polynomial degree d.
  * 1, 3, 4 points: Strang & Fix; 6, 12, 13 points: Dunavant (1985);
- * 7 points: Radon's degree-5 rule in closed form. */
+ * 7 points: Radon's degree-5 rule in closed form; 16..79 points: the
+ * Dunavant table below. */
 static int push3(double *p, double *w, int n, double a, double wt)
 { /* orbit (a, a), (a, 1-2a), (1-2a, a) */
   double b = 1.0 - 2.0 * a;
@@ -105,6 +106,146 @@ static int push6(double *p, double *w, int n, double a, double b, double wt)
   }
   return n;
 }
+
+/* Dunavant (1985) degree 8..20, reference indices 7..19 (16..79 points),
+ * weights halved: centroid weight (0: none), then kDunN3 orbits {a, w}
+ * (push3) and kDunN6 orbits {a, b, w} (push6), back to back.  The degree-20
+ * orbit a = 0.45513668195028 is Dunavant's; the reference's table has
+ * 0.45513668695028 there (a transcription error, 5e-9 off the orbit). */
+static const int kDunN3[13] = {3, 4, 2, 5, 5, 6, 6, 6, 7, 8, 9, 8, 10};
+static const int kDunN6[13] = {1, 1, 3, 2, 3, 3, 4, 5, 5, 6, 7, 8, 8};
+static const double kDunCen[13] = {
+    0.072157803838893503, 0.048567898141399501, 0.045408995191376998,
+    0, 0, 0.026260461700401,
+    0, 0, 0.023437848713821,
+    0.0167185996454015, 0.0154049699688235, 0.016453165694459499,
+    0.016528527770811999,
+};
+static const double kDunOrb[] = {
+    /* index 7 */
+    0.45929258829272301, 0.047545817133642497, 0.17056930775175999,
+    0.051608685267358997, 0.050547228317031012, 0.016229248811599001,
+    0.263112829634638, 0.72849239295540402, 0.0136151570872175,
+    /* index 8 */
+    0.48968251919873801, 0.015667350113569501, 0.43708959149293702,
+    0.038913770502387, 0.18820353561903302, 0.039823869463604999,
+    0.044729513394453024, 0.012788837829349, 0.22196298916076601,
+    0.74119859878449801, 0.021641769688644501,
+    /* index 9 */
+    0.485577633383657, 0.018362978878233498, 0.10948157548503701, 0.022660529717764,
+    0.30793983876412101, 0.55035294182099903, 0.036378958422710002,
+    0.246672560639903, 0.72832390459741103, 0.0141636212655285,
+    0.066803251012200027, 0.92365593358749998, 0.0047108334818665,
+    /* index 10 */
+    0.534611048270758, 0.0004635031644805, 0.39896930296585498,
+    0.038574767457406497, 0.203309900431282, 0.029661488690387001,
+    0.11935091228258099, 0.018092270251709001, 0.032364948111275993,
+    0.0068298655013389999, 0.35662064826129303, 0.59320121342821297,
+    0.026168555981102001, 0.17148898030404203, 0.807489003159792,
+    0.0103538298195705,
+    /* index 11 */
+    0.48821738977380502, 0.0128655332202275, 0.43972439229445998,
+    0.021846272269019001, 0.27121038501211603, 0.031429112108942503,
+    0.12757614554158597, 0.0173980564653545, 0.021317350453209982,
+    0.0030831305257795001, 0.27571326968551402, 0.60894323577978804,
+    0.020185778883190501, 0.28132558098993998, 0.69583608678780307,
+    0.0111783866011515, 0.11625191590759698, 0.85801403354407302,
+    0.0086581155543294999,
+    /* index 12 */
+    0.49504818493970498, 0.0056400726046649998, 0.46871663510957401,
+    0.015711759181227001, 0.41452133680127701, 0.023536251252097001,
+    0.22939957204283101, 0.023681793268177501, 0.11442449519633002,
+    0.015583764522897, 0.024811391363459001, 0.0039878857325369998,
+    0.26879499705876098, 0.63635117456166002, 0.018424201364366,
+    0.29173006673428803, 0.69016915998690498, 0.0087007316519110006,
+    0.12635738549166903, 0.851409537834241, 0.0077608934195224997,
+    /* index 13 */
+    0.48896391036217901, 0.0109417906847145, 0.417644719340454,
+    0.016394176772062501, 0.27347752830883898, 0.025887052253646001,
+    0.177205532412543, 0.021081294368496501, 0.061799883090872976,
+    0.0072168498348885004, 0.019390961248700989, 0.0024617018012000002,
+    0.17226668782135601, 0.77060855477499601, 0.012332876606282,
+    0.33686145979634496, 0.57022229084668297, 0.019285755393530502,
+    0.29837288213625801, 0.68698016780808802, 0.007218154056767, 0.118974497696957,
+    0.87975717137017106, 0.0025051144192504999,
+    /* index 14 */
+    0.50697291685824297, 0.00095843782142449995, 0.43140635428302299,
+    0.022124513635572499, 0.27769364484714398, 0.025593274359425999,
+    0.12646489104125402, 0.011843867935343999, 0.070808385974685972,
+    0.0066448878450105003, 0.018965170241073026, 0.002374458304096,
+    0.26131137114008701, 0.60495446689329102, 0.019275036299796499,
+    0.38804676709026897, 0.57558655551281401, 0.013607907160312,
+    0.28571222004991603, 0.72446266307665497, 0.0010910386833985,
+    0.21559966407228398, 0.74755646605183801, 0.010752659923865499,
+    0.10357561657638598, 0.88396457409241602, 0.0038369713155245,
+    /* index 15 */
+    0.497380541948438, 0.0032029392892924998, 0.41346943854935203,
+    0.020855148369693498, 0.470458599066991, 0.013445742125032, 0.240553749969521,
+    0.021066261380824999, 0.147965794222573, 0.015000133421386499,
+    0.075465187657474009, 0.0071000494625119998, 0.016596402623024975,
+    0.0017912311756365, 0.296555596579887, 0.59986871117486096,
+    0.016386573730313499, 0.337723063403079, 0.64219352494150495,
+    0.0076491531242204999, 0.204748281642812, 0.79959272097132694,
+    0.0011931220964195, 0.189358492130623, 0.768699721401368, 0.0095423963779494997,
+    0.085283615682657021, 0.90039906408666104, 0.0034250272732710002,
+    /* index 16 */
+    0.49717054055677401, 0.0025467077202535001, 0.48217632262462501,
+    0.0073354322638189999, 0.450239969020782, 0.012175439176836,
+    0.40026623937739703, 0.015553775434484499, 0.25214126797095299,
+    0.015628555609310001, 0.162047004658461, 0.0124078271698325,
+    0.075875882260745986, 0.0070280365352785, 0.015654726967821997,
+    0.0015973380868894999, 0.33431986736365804, 0.655493203809423,
+    0.0040598276594965002, 0.29222153779694399, 0.57233759053202005,
+    0.0134028711415815, 0.31957488542319001, 0.62600119028622803,
+    0.0092299966054109998, 0.19070422419229199, 0.79642721497407098,
+    0.0042384342671640002, 0.18048321164874598, 0.75235100593772897,
+    0.0091463983850124993, 0.080711313679564023, 0.90462550409560794,
+    0.0033328160020825001,
+    /* index 17 */
+    0.49334480863092101, 0.0045362183397019999, 0.46921059424195699,
+    0.009380658469797, 0.43628139588700598, 0.0097205489927384998,
+    0.39484617067341599, 0.013876974305405, 0.24979456880315698,
+    0.016128112675728501, 0.16143219374384299, 0.012537016308461,
+    0.076598227485370984, 0.0076359639859159998, 0.024252439353450006,
+    0.0033969610114815002, 0.04314636721696502, -0.00111154936496,
+    0.35891149494094399, 0.63265796885663605, 0.0031659570382029999,
+    0.29440247675195697, 0.57441097151085496, 0.013628769024569,
+    0.32501780164181404, 0.62477904679251206, 0.0088383928247324998,
+    0.18473755966604599, 0.74893317652303704, 0.0091897423190350006,
+    0.21879680001332102, 0.76920700542044296, 0.0040523664040959997,
+    0.10117959713640801, 0.88396230227346706, 0.0038170645353625002,
+    0.020874755282585977, 1.0143472600053629, 2.3093830397e-05,
+    /* index 18 */
+    0.48960998707300601, 0.0051653659456359999, 0.45453689269789299,
+    0.011193623631508, 0.40141668064943103, 0.015133062934733999,
+    0.25555165440309802, 0.015245483901099, 0.17707794215213002,
+    0.012079606370820501, 0.11006105322795201, 0.0080254017934004995,
+    0.05552862425184002, 0.004042290130892, 0.012621863777229014,
+    0.0010396810137425, 0.39575478735694303, 0.60063379479464496,
+    0.0019424384524905, 0.30792998388043602, 0.55760326158878404,
+    0.012787080306011001, 0.26456694840652001, 0.72098702581736496,
+    0.0044404517866689997, 0.35853935220595101, 0.59452706895587104,
+    0.0080622733808655006, 0.15780740596859499, 0.83933147368083905,
+    0.0012459709087455, 0.075050596975910988, 0.70108797892617303,
+    0.0091214200594755008, 0.14242160111338298, 0.82293132406985703,
+    0.0051292818680994997, 0.065494628082937989, 0.92434425262078401,
+    0.001899964427651,
+    /* index 19 */
+    0.50095046435220003, 0.00043350959283149998, 0.48821295793472852,
+    0.0058300263582240002, 0.45513668195028251, 0.0114384681782105,
+    0.40199625931828953, 0.015224491336968999, 0.25589290975942147,
+    0.0153124458626775, 0.17648825599510598, 0.0121840288384, 0.10417085533675852,
+    0.0079987160160120002, 0.053068963840930017, 0.0038491509078010001,
+    0.041618715196028999, -0.00031603024874399997, 0.011581921406822016,
+    0.00087556715059649995, 0.34485577022900099, 0.60640264610616001,
+    0.0082329195947880001, 0.37784326959485398, 0.61584261445654098,
+    0.0024195167702425, 0.306635479062357, 0.55904800039029501, 0.012902453267325,
+    0.24941936277474203, 0.73660674326286601, 0.0042355455272205002,
+    0.21277572480280199, 0.711675142287434, 0.0091774570531399993,
+    0.14696543605323897, 0.86140271715498695, 0.00035220233895399999,
+    0.13772697882892299, 0.83558695791236293, 0.0050563424637309999,
+    0.059696109149007026, 0.92975617155685297, 0.001786954692975,
+};
 
 int orc_tri_rule(int idx, double *p, double *w)
 {
@@ -138,7 +279,21 @@ int orc_tri_rule(int idx, double *p, double *w)
     return push6(p, w, n, 0.048690315425316, 0.312865496004874,
                  0.077113760890257 / 2);
   default:
-    return -1;
+    break;
+  }
+  if (idx < 7 || idx > 19) return -1;
+  {
+    const double *o = kDunOrb;
+    int r, k;
+    for (r = 7; r < idx; r++) o += 2 * kDunN3[r - 7] + 3 * kDunN6[r - 7];
+    if (kDunCen[idx - 7] != 0.0) {
+      p[0] = p[1] = 1.0 / 3.0;
+      w[0] = kDunCen[idx - 7];
+      n = 1;
+    }
+    for (k = 0; k < kDunN3[idx - 7]; k++, o += 2) n = push3(p, w, n, o[0], o[1]);
+    for (k = 0; k < kDunN6[idx - 7]; k++, o += 3) n = push6(p, w, n, o[0], o[1], o[2]);
+    return n;
   }
 }
 
@@ -895,7 +1050,7 @@ int orc_evaluate(int np, const double *nodes, int ne, const int *elems,
                  const double *u, int tri_idx, double *result, int nthreads)
 {
   (void)np;
-  double qp[2 * 16], qw[16];
+  double qp[2 * MAXQ], qw[MAXQ];
   const int nq = orc_tri_rule(tri_idx, qp, qw);
   if (nq <= 0) return -1;
   if (nthreads > 0) omp_set_num_threads(nthreads);

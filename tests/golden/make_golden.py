@@ -3,11 +3,18 @@ CPU operator (oracle/_ref/libeasybem_ref.so, built by oracle/Makefile from
 the reference's own sources).  Run in a checkout where the reference sources
 were present at build time:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [all|rules|orders|example]
 
 Fixtures (inputs and reference outputs only):
   quadrature.npz        the reference's Gauss-Legendre (indices 0..18) and
-                        triangle (0..6) tables
+                        triangle (0..19) tables
+  laplace_<mesh>_o<k>.npz  for k = 7, 8, 9 as well: the higher orders the
+                        reference's example callers pass
+  example_shaft.npz     python/examples/laplace3dDirichletGPU.py's first two
+                        passes (the shaft mesh scaled by 1/100 at order 9,
+                        refined once at order 10): u = 1/|x - (2,2,2)|,
+                        b = (M/2 + K) u, the dense solution t of V t = b, and
+                        V x for a seeded x
   laplace_<mesh>_o<k>.npz  dense V[test,trial], K[test,node], M[test,node]
                         of getLaplace3d at order k
   kelvin_ico1_o4.npz    dense Kelvin V (nu = 0.3) of getElasticity3d
@@ -43,17 +50,61 @@ def touching_mask(T):
     return (inc.astype(np.int32) @ inc.T.astype(np.int32)) > 0
 
 
-def main():
-    ref = Reference()
-    out = {}
+def rules(ref):
     q = {}
     for i in range(19):
         p, w = ref.line_rule(i)
         q[f"line{i}_p"], q[f"line{i}_w"] = p, w
-    for i in range(7):
+    for i in range(20):
         p, w = ref.tri_rule(i)
         q[f"tri{i}_p"], q[f"tri{i}_w"] = p, w
     np.savez_compressed(os.path.join(HERE, "quadrature.npz"), **q)
+
+
+def dense_orders(ref, orders):
+    cases = {"ico1": meshes.icosphere(1), "torus8x4": meshes.torus(1.0, 0.4, 8, 4)}
+    for name, (P, T) in cases.items():
+        for order in orders:
+            V, K, D, M = ref.laplace3d(P, T, order)
+            C1, C2, C3 = ref.curl(P, T)
+            np.savez_compressed(os.path.join(HERE, f"laplace_{name}_o{order}.npz"),
+                                nodes=P, elements=T, V=V, K=K, M=M, D=D,
+                                C1=C1, C2=C2, C3=C3)
+
+
+def example(ref):
+    """laplace3dDirichletGPU.py:25-81, passes i = 0, 1 (order 9 + i)."""
+    from easybem_amd import mesh_formats as mf
+    P, T, I = mf.load3dSurfaceFromNetgen(
+        os.path.join(HERE, "meshes", "shaft-surface_verycoarse.vol"))
+    P = P / 100
+    xstar = np.array([2.0, 2.0, 2.0])
+    out = {}
+    for i in range(2):
+        if i:
+            P, T, I = mf.refineMesh(P, T, I)
+        order = 9 + i
+        u = 1.0 / np.linalg.norm(P - xstar, axis=1)
+        V, K, _, M = ref.laplace3d(P, T, order)
+        b = (0.5 * M + K) @ u
+        x = np.random.default_rng(100 + i).uniform(-1, 1, T.shape[0])
+        out.update({f"nodes{i}": P, f"elements{i}": T, f"order{i}": order,
+                    f"u{i}": u, f"b{i}": b, f"t{i}": np.linalg.solve(V, b),
+                    f"x{i}": x, f"Vx{i}": V @ x})
+        print("example pass", i, T.shape[0], "triangles")
+    np.savez_compressed(os.path.join(HERE, "example_shaft.npz"), **out)
+
+
+def main(part="all"):
+    ref = Reference()
+    if part in ("all", "rules"):
+        rules(ref)
+    if part in ("all", "orders"):
+        dense_orders(ref, (7, 8, 9))
+    if part in ("all", "example"):
+        example(ref)
+    if part != "all":
+        return
 
     cases = {"ico1": meshes.icosphere(1), "torus8x4": meshes.torus(1.0, 0.4, 8, 4)}
     for name, (P, T) in cases.items():
@@ -115,4 +166,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else "all")

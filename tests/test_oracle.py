@@ -37,29 +37,58 @@ def test_line_rules_match_reference_tables(oracle):
         assert np.abs(w - q[f"line{i}_w"]).max() < 1e-15
 
 
-def test_triangle_rules_match_reference_tables(oracle):
+def _match_points(A, B):
+    """For each row of B (x, y, w) the nearest row of A by position: the
+    largest position and weight differences over B (the two tables list an
+    orbit's points in different orders)."""
+    dp = dw = 0.0
+    for x, y, w in B:
+        d = np.hypot(A[:, 0] - x, A[:, 1] - y)
+        k = d.argmin()
+        dp, dw = max(dp, d[k]), max(dw, abs(A[k, 2] - w))
+    return dp, dw
+
+
+@pytest.mark.parametrize("i", range(20))
+def test_triangle_rules_match_reference_tables(oracle, i):
+    """The same point sets and weights as the reference's triangle table
+    (src/quadrature.cpp:1047-1048), except one 3-point orbit of the
+    degree-20 rule (index 19): the reference lists a = 0.45513668695028
+    where Dunavant's orbit has 0.45513668195028 (its points then miss the
+    orbit by 5e-9, and its moments are off; see the next test)."""
     q = gold("quadrature.npz")
-    for i in range(7):
-        p, w = oracle.tri_rule(i)
-        A = np.c_[p, w]
-        B = np.c_[q[f"tri{i}_p"], q[f"tri{i}_w"]]
-        A = A[np.lexsort(A.T[::-1])]
-        B = B[np.lexsort(B.T[::-1])]
-        assert A.shape == B.shape
-        assert np.abs(A - B).max() < 2e-15
+    p, w = oracle.tri_rule(i)
+    A = np.c_[p, w]
+    B = np.c_[q[f"tri{i}_p"], q[f"tri{i}_w"]]
+    assert A.shape == B.shape
+    if i == 19:
+        bad = np.abs(B[:, :2] - 0.4551366869502825).min(1) < 1e-15
+        assert bad.sum() == 3
+        dp, dw = _match_points(A, B[bad])
+        assert 4e-9 < dp < 8e-9 and dw < 1e-17
+        B = B[~bad]
+    dp, dw = _match_points(A, B)
+    assert dp < 3e-15 and dw < 3e-16
 
 
-@pytest.mark.parametrize("idx", range(7))
+@pytest.mark.parametrize("idx", range(20))
 def test_triangle_rule_exact_on_monomials(oracle, idx):
     """Index d-1 integrates x^a y^b exactly for a+b <= d:
-    int_T x^a y^b = a! b! / (a+b+2)!."""
+    int_T x^a y^b = a! b! / (a+b+2)!  (the reference's own degree-20 table
+    fails this by about 1e-9 at its mistyped orbit)."""
     from math import factorial
     p, w = oracle.tri_rule(idx)
+    q = gold("quadrature.npz")
+    pr, wr = q[f"tri{idx}_p"], q[f"tri{idx}_w"]
+    worst_ref = 0.0
     for a in range(idx + 2):
         for b in range(idx + 2 - a):
             ex = factorial(a) * factorial(b) / factorial(a + b + 2)
             got = np.sum(w * p[:, 0] ** a * p[:, 1] ** b)
             assert abs(got - ex) < 1e-14, (idx, a, b)
+            worst_ref = max(worst_ref, abs(np.sum(wr * pr[:, 0] ** a * pr[:, 1] ** b) - ex))
+    if idx == 19:
+        assert worst_ref > 1e-10
 
 
 @pytest.mark.parametrize("n", [1, 2, 5, 16, 64])
@@ -71,7 +100,7 @@ def test_gauss_legendre_exact(oracle, n):
 
 # ----------------------------------------------------- reference operator
 @pytest.mark.parametrize("mesh", ["ico1", "torus8x4"])
-@pytest.mark.parametrize("order", [3, 4])
+@pytest.mark.parametrize("order", [3, 4, 7, 8, 9])
 def test_laplace_dense_matches_reference(oracle, mesh, order):
     g = gold(f"laplace_{mesh}_o{order}.npz")
     V, K, M = oracle.laplace_dense(g["nodes"], g["elements"], order)
@@ -90,7 +119,7 @@ def test_surface_curls_match_reference(oracle, mesh):
 
 
 @pytest.mark.parametrize("mesh", ["ico1", "torus8x4"])
-@pytest.mark.parametrize("order", [3, 4])
+@pytest.mark.parametrize("order", [3, 4, 7, 9])
 def test_hypersingular_matches_reference_dense_d(oracle, mesh, order):
     """sum_c C_c^T V C_c through three oracle P0 applies equals the
     reference's dense D; D annihilates constants (surface curls of a
