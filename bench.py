@@ -9,7 +9,9 @@ residual of 1e-8.  Near field: 5-point Gauss Sauter-Schwab; far field:
 7-point triangle rule.  GMRES is right-preconditioned by the Calderon
 preconditioner (the hypersingular operator between inverse lumped masses,
 with a 1-point far rule); --precond none runs it without.  The line also
-reports the unpreconditioned solve rate.
+reports the unpreconditioned solve rate, configs 1, 3 and 4 (matvecs and
+the batched 64-RHS solves), a serving stream on icosphere levels 4-6 and the
+CPU baseline (--no-secondary, --no-serve, --no-cpu-baseline skip them).
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
 build runs on one GPU.
@@ -58,33 +60,83 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P, T, n_matvec, seconds=10.0):
-    """The oracle (FP64 C restatement, OpenMP) on sampled P1 rows of the same
-    operator, extrapolated to full matvecs x the FP64 solve's matvec count
-    (preconditioner applies counted by their far-field work)."""
-    from oracle import Oracle
-    o = Oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    rng = np.random.default_rng(7)
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _sampled_matvec(o, P, T, threads, seconds, seed):
+    """Seconds of one full P1 V matvec (config 2's operator) extrapolated
+    from the oracle on random node rows, sized to about `seconds` of work."""
+    rng = np.random.default_rng(seed)
     x = rng.uniform(-1, 1, P.shape[0])
-    nrows, elapsed = 16, 0.0
+    nrows = 8
     while True:
         rows = rng.choice(P.shape[0], size=nrows, replace=False)
         t0 = time.perf_counter()
-        o.apply("V", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4,
-                nthreads=threads)
+        o.apply("V", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4, nthreads=threads)
         elapsed = time.perf_counter() - t0
         if elapsed > seconds / 4 or nrows >= P.shape[0]:
             break
         nrows = min(P.shape[0], int(nrows * max(2.0, seconds / 4 / max(elapsed, 1e-3))))
-    t_matvec = elapsed / nrows * P.shape[0]
-    return {"value": 1.0 / (t_matvec * n_matvec), "unit": "solves/s",
+    return elapsed / nrows * P.shape[0], nrows, elapsed
+
+
+def cpu_baseline(P, T, n_matvec, seconds=10.0):
+    """The oracle (FP64 C restatement of the reference CPU operator) on the
+    host cores, 1 thread (as the single-threaded reference) and all cores
+    (OpenMP over targets).  The headline value is config 2's solve rate at
+    all cores: a full P1 V matvec extrapolated from sampled node rows, times
+    the FP64 GPU solve's matvec count (preconditioner applies counted by
+    their far-field work).  Config 1: full matvecs at both thread counts and
+    a dense solve (oracle assembly at order 4 + LAPACK)."""
+    from oracle import Oracle
+    from easybem_amd import meshes
+    o = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    t_all, rows_all, el_all = _sampled_matvec(o, P, T, threads, seconds, 7)
+    t_one, rows_one, el_one = _sampled_matvec(o, P, T, 1, seconds / 2, 8)
+    # config 1: icosphere L3, P0 V, 4-point Gauss near / 7-point far
+    P1, T1 = meshes.icosphere(3)
+    x1 = np.random.default_rng(20261203).uniform(-1, 1, T1.shape[0])
+    c1 = {}
+    for name, th in (("1_thread", 1), ("all_cores", threads)):
+        t0 = time.perf_counter()
+        o.apply("V", 0, 0, P1, T1, x1, line_idx=3, tri_idx=4, nthreads=th)
+        c1[f"matvec_s_{name}"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    V, _, _ = o.laplace_dense(P1, T1, 4)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    np.linalg.solve(V, x1)
+    c1["dense_solve_s"] = {"assembly_all_cores": t_asm,
+                           "lapack_solve": time.perf_counter() - t0}
+    return {"value": 1.0 / (t_all * n_matvec), "unit": "solves/s",
             "cores": threads, "kind": "port",
-            "sample": f"oracle P1 V apply on {nrows} random node rows "
-                      f"({elapsed:.1f} s), extrapolated to a full matvec "
-                      f"({t_matvec:.1f} s) x {n_matvec:.1f} FP64 matvec-equivalents "
-                      "per solve (the FP64 GPU solve's operator and preconditioner "
-                      "applies)"}
+            "value_1_thread": 1.0 / (t_one * n_matvec),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"config 2 (icosphere L5 P1 V): oracle apply on {rows_all} "
+                      f"random node rows on {threads} threads ({el_all:.1f} s) and "
+                      f"{rows_one} rows on 1 thread ({el_one:.1f} s), extrapolated "
+                      f"to full matvecs ({t_all:.1f} s / {t_one:.0f} s) x "
+                      f"{n_matvec:.1f} FP64 matvec-equivalents per solve (the FP64 "
+                      "GPU solve's operator and preconditioner applies); config 1 "
+                      "full matvecs and dense solve measured whole",
+            "config1_ico3_p0_V": c1}
 
 
 def secondary(L, meshes, solve_batched=False):
@@ -106,8 +158,11 @@ def secondary(L, meshes, solve_batched=False):
         tm = op.time(x, y, reps=20)
         Vt = y.to_host()
         A = meshes.areas(P, T)
+        tf = 11 * op.work()["far_interactions"] / tm["far"] / 1e9
         r1[prec] = {"matvec_ms": tm["total"], "far_ms": tm["far"],
-                    "far_tflops": 11 * op.work()["far_interactions"] / tm["far"] / 1e9,
+                    "far_tflops": tf,
+                    "far_frac": tf / (PEAK_FP32_TFLOPS if prec == "fp32"
+                                      else PEAK_FP64_TFLOPS),
                     "rayleigh_Y20": float(t @ Vt / np.sum(A * t * t))}
     out["cfg1_ico3_p0_V"] = r1
     # config 3: torus 256x128, Kelvin P1, FP32 far field
@@ -119,8 +174,10 @@ def secondary(L, meshes, solve_batched=False):
         x = L.DeviceArray.from_host(rng.standard_normal(op.ncols))
         y = L.DeviceArray(op.nrows)
         tm = op.time(x, y, reps=2)
-        r3[prec] = {"matvec_ms": tm["total"],
-                    "far_tflops": 30 * op.work()["far_interactions"] / tm["far"] / 1e9}
+        tf = 30 * op.work()["far_interactions"] / tm["far"] / 1e9
+        r3[prec] = {"matvec_ms": tm["total"], "far_tflops": tf,
+                    "far_frac": tf / (PEAK_FP32_TFLOPS if prec == "fp32"
+                                      else PEAK_FP64_TFLOPS)}
     out["cfg3_torus_p1_kelvin"] = r3
     # config 4: icosphere L6, P0 V, 64 right-hand sides in one launch
     P, T = meshes.icosphere(6)
@@ -157,6 +214,18 @@ def secondary(L, meshes, solve_batched=False):
         r4["batched_mixed_solve"] = {"seconds": dt, "solves_per_s": 64 / dt,
                                      "iters_max": max(v["iters"] for v in st),
                                      "relres_max": max(v["relres"] for v in st)}
+        # the same 64 systems in plain FP64 (FP64 64-RHS applies inside)
+        s64 = L.Solver(ops["fp64"], None, restart=50, batch=64)
+        s64.set_precond(S, S)
+        dx = L.DeviceArray.from_host(np.zeros_like(B))
+        t0 = time.perf_counter()
+        st = s64.solve_device(db, dx, tol=1e-8, mixed=False, nb=64, max_iters=3000)
+        dt64 = time.perf_counter() - t0
+        r4["batched_fp64_solve"] = {"seconds": dt64, "solves_per_s": 64 / dt64,
+                                    "iters_max": max(v["iters"] for v in st),
+                                    "relres_max": max(v["relres"] for v in st)}
+        r4["mixed_speedup_vs_fp64"] = dt64 / dt
+        s64.close()
     out["cfg4_ico6_p0_V_64rhs"] = r4
     out["multi_mesh"] = multi_mesh(L, meshes)
     return out
@@ -204,9 +273,9 @@ def multi_mesh(L, meshes, reps=20):
 
 # single-stream service time (s) of one solve, by (problem, level): sets the
 # offered load of the serving stream
-SERVICE_S = {("dirichlet", 4): 0.03, ("dirichlet", 5): 0.12,
-             ("dirichlet", 6): 1.2, ("neumann", 4): 0.05, ("neumann", 5): 0.4,
-             ("neumann", 6): 4.0}
+SERVICE_S = {("dirichlet", 4): 0.03, ("dirichlet", 5): 0.1,
+             ("dirichlet", 6): 1.0, ("neumann", 4): 0.05, ("neumann", 5): 0.2,
+             ("neumann", 6): 1.6}
 
 
 def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
@@ -311,12 +380,15 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
-    ap.add_argument("--secondary", action="store_true",
-                    help="also time configs 1, 3 and 4 (matvecs, batched solve)")
-    ap.add_argument("--serve", type=int, default=0,
-                    help="serving stream of N requests on icosphere levels 4-6 "
-                         "(the default run serves 24 requests on levels 4-5)")
+    ap.add_argument("--inner-tol", type=float, default=1e-9,
+                    help="tolerance of the FP32 inner solves (the default, below "
+                         "the target, lets one refinement pass reach it: the FP64 "
+                         "residual is then applied once, to certify the result)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip configs 1, 3 and 4 (matvecs, batched solves), "
+                         "which the default run times")
+    ap.add_argument("--serve", type=int, default=24,
+                    help="serving stream of N requests on icosphere levels 4-6")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--precond", choices=("calderon", "spai", "none"), default="calderon",
                     help="right preconditioner of the GMRES solves (default: the "
@@ -390,16 +462,16 @@ def main():
     elif args.precond == "calderon":
         # Q (D + m m^T / (4|Gamma|)) Q: D through curls, 3-point Gauss near
         # field, 1-point far rule; Q five Chebyshev steps for the P1 mass
-        # inverse; FP64 for the FP64 operator
+        # inverse.  FP64 also inside the FP32 inner solves: an FP32 one is
+        # linear only to its rounding, and the inner residual then stalls
+        # near 1.6e-8, so one refinement pass could not reach 1e-8
         P64 = L.precond_calderon(mesh, "fp64", line_rule=2, tri_rule=0)
-        P32 = L.precond_calderon(mesh, "fp32", line_rule=2, tri_rule=0)
-        solver.set_precond(P64, P32)
+        solver.set_precond(P64, P64)
         pc = ("Calderon, right: Q (D + m m^T/(4|Gamma|)) Q, D the hypersingular "
               "operator through curls with a 1-point far rule and 3-point Gauss "
               "near rule, Q five Chebyshev steps for the consistent P1 mass "
-              "inverse (lumped-mass preconditioned; FP32 D with the FP32 "
-              "operator, FP64 with the FP64 one); the residual is that of the "
-              "unpreconditioned system")
+              "inverse (lumped-mass preconditioned), in FP64 for the FP32 inner "
+              "solves too; the residual is that of the unpreconditioned system")
 
     def solve(mixed=True, s=None):
         L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))
@@ -531,13 +603,14 @@ def main():
                        "launches": near_launches},
     }
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(P, T, st64.get("matvec64", 1))
-    if args.secondary:
+    if not args.no_secondary:
+        log("secondary configurations")
         res["secondary"] = secondary(L, meshes, solve_batched=True)
-    if args.serve:
+    if not args.no_serve and args.serve > 0:
+        log("serving")
         res["serving"] = serving(L, meshes, n_requests=args.serve, levels=(4, 5, 6))
-    elif not args.no_serve:
-        res["serving"] = serving(L, meshes)
     print(json.dumps(res))
 
 

@@ -591,18 +591,24 @@ struct alignas(4 * sizeof(R)) SymRec {
 // double layer its unit normal n', then the NT x NR strengths.  GW is a
 // multiple of 4 with GW / 4 odd, so the 16-byte reads of 16 lanes at
 // rotating records spread over the LDS banks.
-template <int NT, int NR, bool DL = false>
+// HL (the FP32 single layer): the record also holds the rounding remainder
+// of the source centroid, C' - (float)C', at offset LO.
+template <int NT, int NR, bool DL = false, bool HL = false>
 struct SymLayout {
   static constexpr bool ROT = NT == 1 && !DL;
   static constexpr int C0 = NT % 3;
   static constexpr int NO = NT - C0;
   static constexpr int NRM = 9 + NO;              // offset of n' (double layer)
   static constexpr int SIG = NRM + (DL ? 3 : 0);  // offset of the strengths
-  static constexpr int USED = ROT ? (NR == 1 ? 4 : 8) : SIG + NT * NR;
+  static constexpr int LO = SIG + NT * NR;        // centroid remainder (HL)
+  static constexpr int USED = ROT ? (NR == 1 ? 4 : 8) : LO + (HL ? 3 : 0);
   static constexpr int Q4 = (USED + 3) / 4;
   static constexpr int GW = ROT ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
   static constexpr int LDS = 64 * GW;
 };
+// the layout of sym_block<R, NT, NR, DK>
+template <typename R, int NT, int NR, int DK>
+using SymLay = SymLayout<NT, NR, DK != 0, sizeof(R) == 4 && DK == 0 && NT != 1>;
 
 // Workgroup bid of the symmetric schedule (see above); lds holds the source
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
@@ -626,8 +632,19 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   static_assert(!DL || NR == 1, "double layer: one right-hand side");
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
-  using Lay = SymLayout<NT, NR, DL>;
+  using Lay = SymLay<R, NT, NR, DK>;
   constexpr int C0 = Lay::C0, NO = Lay::NO, GW = Lay::GW;
+  // FP32 single layer: the centroid difference D = C - C' is formed from
+  // hi and lo parts, (C_hi - C'_hi) + (C_lo - C'_lo), so that it carries no
+  // rounding of the positions (which would be systematic for each triangle
+  // pair), and with one right-hand side every step's point sums (FP32 over
+  // the step's 7 source points) go to FP64 accumulators.  Measured at L5 P1:
+  // FP32-vs-FP64 apply error 6e-8 -> 8e-9 (random density) and 7e-9 (the
+  // solution of the bench's solve); flushing every 1, 4 or 8 steps costs the
+  // same time, and an FP32 Newton step on rsqrt costs 60% and helps nothing.
+  constexpr bool HL = sizeof(R) == 4 && !DL && !Lay::ROT;
+  constexpr bool FL = HL && NR == 1;
+  constexpr int KF = 1;
   // One-point single-layer rules move the source records between lanes (DPP
   // wave_rol:1, lane L reads lane L + 1 -- VALU moves instead of LDS reads,
   // which would otherwise bound a step of so little arithmetic); otherwise
@@ -685,9 +702,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
           v = ps * ps * a.tsl[(f + C0) * (size_t)ne + j];
         } else if (f < Lay::SIG) {
           v = (R)a.tn[3 * (size_t)j + (f - Lay::NRM)];
-        } else {
+        } else if (f < Lay::LO) {
           const int sr = f - Lay::SIG, b = sr / NR, r = sr - b * NR;
           v = a.src[RW * ((size_t)j * NT + b) + 3 + r];
+        } else {
+          v = a.tsl[(9 + NT + f - Lay::LO) * (size_t)ne + j];
         }
       } else {
         v = f < 3 ? R(1e18) : R(0);
@@ -707,6 +726,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
           F0z = a.tsl[5 * (size_t)ne + tt];
   const R F1x = a.tsl[6 * (size_t)ne + tt], F1y = a.tsl[7 * (size_t)ne + tt],
           F1z = a.tsl[8 * (size_t)ne + tt];
+  R Clx = 0, Cly = 0, Clz = 0;
+  if constexpr (HL) {
+    Clx = a.tsl[(9 + NT + 0) * (size_t)ne + tt];
+    Cly = a.tsl[(9 + NT + 1) * (size_t)ne + tt];
+    Clz = a.tsl[(9 + NT + 2) * (size_t)ne + tt];
+  }
   R E[NT], sq[NT][NR], rc[NT], hc[NT];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
@@ -736,10 +761,37 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   }
   const R hps = R(1) / ps;
   R acc[NT][NR], S[NT][NR];
+  // FP64 sums of the flushed steps (FL); the source side's stay aligned
+  // with S: after a flush at step k, lane L holds source (L + k) mod 64 in both
+  double acc64[NT][NR], S64[NT][NR];
 #pragma unroll
   for (int q = 0; q < NT; q++)
 #pragma unroll
-    for (int r = 0; r < NR; r++) acc[q][r] = S[q][r] = R(0);
+    for (int r = 0; r < NR; r++) {
+      acc[q][r] = S[q][r] = R(0);
+      acc64[q][r] = S64[q][r] = 0.0;
+    }
+  // FL: moves the FP32 sums into the FP64 ones; the source side's FP64 sums
+  // first move down `shift` lanes (the steps since the last flush)
+  auto flush = [&](int shift, auto sym_tag) {
+    constexpr bool SYM = decltype(sym_tag)::value;
+    const int from = ((L + shift) & 63) << 2;
+#pragma unroll
+    for (int q = 0; q < NT; q++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        acc64[q][r] += (double)acc[q][r];
+        acc[q][r] = R(0);
+        if constexpr (SYM) {
+          const long long v = __double_as_longlong(S64[q][r]);
+          const int lo = __builtin_amdgcn_ds_bpermute(from, (int)v);
+          const int hi = __builtin_amdgcn_ds_bpermute(from, (int)(v >> 32));
+          S64[q][r] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo) +
+                      (double)S[q][r];
+          S[q][r] = R(0);
+        }
+      }
+  };
 
   // 1/r from r2 (= r^2, FP64: (2r)^2) for the NT target points against one
   // source point, then both directions' sums.  Double layer: it[q] gathers
@@ -811,7 +863,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
       } else {
         const R *g = lds + jl * GW;
-        const R Dx = Cx - g[0], Dy = Cy - g[1], Dz = Cz - g[2];
+        R Dx = Cx - g[0], Dy = Cy - g[1], Dz = Cz - g[2];
+        if constexpr (HL) {
+          Dx += Clx - g[Lay::LO];
+          Dy += Cly - g[Lay::LO + 1];
+          Dz += Clz - g[Lay::LO + 2];
+        }
         const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
         // D.F_k (target directions), D.F'_l (source directions), F_k.F'_l
         R U[3], P[3], W[3][3];
@@ -894,13 +951,26 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
             }
           }
       }
+      if constexpr (FL)
+        if ((k + 1 - k0) % KF == 0) flush(KF, sym_tag);
     }
+    if constexpr (FL)
+      if ((k1 - k0) % KF) flush((k1 - k0) % KF, sym_tag);
   };
   if (I < J) steps(std::true_type{});
   else steps(std::false_type{});
+  if constexpr (!FL) {
+#pragma unroll
+    for (int q = 0; q < NT; q++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        acc64[q][r] = (double)acc[q][r];
+        S64[q][r] = (double)S[q][r];
+      }
+  }
 
   // point sums of triangle tri to partial[c][r][.][tri]
-  auto store = [&](const R (*v)[NR], int tri, int c) {
+  auto store = [&](const double (*v)[NR], int tri, int c) {
     const size_t stride = (size_t)ne;
     double wq[NT];
 #pragma unroll
@@ -911,31 +981,31 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < NT; q++) s = fma(wq[q], (double)v[q][r], s);
+        for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
         out[0] = s;
       } else {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           double s = 0.0;
 #pragma unroll
-          for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], (double)v[q][r], s);
+          for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], v[q][r], s);
           out[k * stride] = s;
         }
       }
     }
   };
-  if (t < ne) store(acc, t, J * ks + rng);
+  if (t < ne) store(acc64, t, J * ks + rng);
   if (I < J) {
     // after the steps k0 .. k1-1 lane L holds source (L + k1) mod 64
     const int j = J * 64 + ((L + k1) & 63);
-    if (j < ne) store(S, j, I * ks + rng);
+    if (j < ne) store(S64, j, I * ks + rng);
   }
 }
 
 template <typename R, int NT, int NR, int DK = 0>
 __global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
 {
-  __shared__ __align__(16) R lds[SymLayout<NT, NR, DK != 0>::LDS];
+  __shared__ __align__(16) R lds[SymLay<R, NT, NR, DK>::LDS];
   sym_block<R, NT, NR, DK>(a, (int)blockIdx.x, lds);
 }
 
@@ -948,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_far_sym_multi(const FarArgs<R> *__restr
                                                        const int *__restrict__ off,
                                                        int nops, int *__restrict__ counter)
 {
-  __shared__ __align__(16) R lds[SymLayout<NT, 1>::LDS];
+  __shared__ __align__(16) R lds[SymLay<R, NT, 1, 0>::LDS];
   __shared__ int sh_item;
   const int total = off[nops];
   while (true) {
@@ -2055,7 +2125,11 @@ int op_build_far(ebh_op *op)
     // or the offsets of every point from the centroid; the FP64 double layer
     // uses it on the symmetric schedule
     const bool ray = op->ray && nt <= 7;
-    std::vector<double> tsl((size_t)(ray ? 9 + nt : 3 + 4 * nt) * ne);
+    // FP32 median rules: three more rows, the rounding remainders of the
+    // centroid, C - (float)C (the symmetric kernel's hi/lo differences)
+    const bool f32 = op->d.far_precision == EBH_FP32;
+    const size_t rows = ray ? 9 + nt + (f32 ? 3 : 0) : 3 + 4 * nt;
+    std::vector<double> tsl(rows * ne);
     for (int t = 0; t < ne; t++) {
       double c[3];
       for (int k = 0; k < 3; k++)
@@ -2080,6 +2154,9 @@ int op_build_far(ebh_op *op)
           const double cq = op->ray_c[q];
           tsl[(size_t)(9 + q) * ne + t] = q < c0 ? 0.0 : cq * cq * e2[(q - c0) % 3];
         }
+        if (f32)
+          for (int k = 0; k < 3; k++)
+            tsl[(size_t)(9 + nt + k) * ne + t] = c[k] - (double)(float)c[k];
         continue;
       }
       for (int q = 0; q < nt; q++) {

@@ -501,9 +501,12 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
       for (int r = 0; r < nb; r++) {
         const double rel = bn[r] > 0 ? rn[r] / bn[r] : 0.0;
         all &= rel <= opt.tol;
-        // systems already at the target get a tolerance their residual meets
+        // systems already at the target get a tolerance their residual meets;
+        // an inner solve need not go below a tenth of the remaining
+        // reduction (inner_tol below 0.1 tol makes the first pass solve to
+        // it, so that an accurate FP32 operator needs one FP64 residual)
         const double itol =
-            rel <= opt.tol ? 2.0 : std::max(opt.inner_tol, 0.5 * opt.tol / rel);
+            rel <= opt.tol ? 2.0 : std::max(opt.inner_tol, 0.1 * opt.tol / rel);
         atol[r] = itol * rn[r];
       }
       if (all || outer >= opt.max_outer || total >= opt.max_iters) break;
