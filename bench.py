@@ -37,9 +37,13 @@ FLOP_SL = 11               # per far interaction, single layer, one RHS
 # icosphere with the packaged schedule (pair-symmetric, one step range): two
 # rocprofv3 passes (FETCH_SIZE, WRITE_SIZE) over `bench.py --pmc-probe`, each
 # scaled by the same counter on a 512 MiB device copy
-# (profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md).  Only valid at level 5
-# with that schedule.
-TRAFFIC_FAR32_L5_BYTES = 203.96e6
+# (profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md).  A counter cannot be
+# read inside this run, so the line reports that profile's figure, with its
+# source, and only at level 5 with that schedule.
+TRAFFIC_FAR32_L5_BYTES = 206.1e6
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over "
+                  "bench.py --pmc-probe, scaled by a 512 MiB copy: "
+                  "profiles/k_far_sym_fp32_sl7_icosphere5_traffic.md")
 TRAFFIC_SCHEDULE = ("symmetric", 1)
 
 
@@ -352,7 +356,8 @@ def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
 def pmc_probe(L, meshes, level):
     """Fixed workload for hardware-counter passes: a plain device copy of
     512 MiB (the calibration of FETCH_SIZE / WRITE_SIZE) three times, then
-    three FP32-far P1 single-layer applies at the bench mesh."""
+    three FP32-far and three FP64 P1 single-layer applies at the bench
+    mesh."""
     import torch
     n = 64 << 20
     src = torch.empty(n, dtype=torch.float64, device="cuda").uniform_()
@@ -367,6 +372,10 @@ def pmc_probe(L, meshes, level):
     y = L.DeviceArray(V32.nrows)
     for _ in range(3):
         V32.apply(x, y)
+    L.check(L.lib().ebh_synchronize())
+    V64 = L.Operator(mesh, "V", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    for _ in range(3):
+        V64.apply(x, y)
     L.check(L.lib().ebh_synchronize())
     print(json.dumps({"probe": "pmc", "copy_bytes_each_way": n * 8,
                       "far_interactions": V32.work()["far_interactions"]}))
@@ -590,6 +599,7 @@ def main():
                      "traffic": (TRAFFIC_FAR32_L5_BYTES if args.level == 5
                                  and (V32.schedule, V32.split) == TRAFFIC_SCHEDULE
                                  else None),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "flop_per_launch": far_flop,
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
         "near_field": {"kernel": "k_finish (near CSR x + far gather, FP64)",
