@@ -329,6 +329,13 @@ class Solver:
     def set(self, graphs=True, check_every=0):
         check(lib().ebh_solver_set(self.h, int(graphs), check_every))
 
+    def set_block(self, block=True):
+        """Batched solves in one shared block Krylov space (block GMRES)
+        instead of independent Krylov spaces in lockstep."""
+        L = lib()
+        L.ebh_solver_set_block.argtypes = [C.c_void_p, C.c_int]
+        check(L.ebh_solver_set_block(self.h, int(block)))
+
     def set_precond(self, p64=None, p32=None):
         """Right preconditioners for the FP64 / FP32 operator (None: off)."""
         L = lib()

@@ -59,7 +59,7 @@ def build_lib(jobs: int = 8) -> str:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
-          "-lpthread"])
+          "-lpthread", "-lrocblas"])
     os.replace(tmp, LIB)
     return LIB
 

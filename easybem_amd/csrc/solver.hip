@@ -16,6 +16,8 @@
 //     while the others continue in lockstep.
 // Reductions are deterministic: fixed per-block partial sums, reduced in a
 // fixed order by one wave per output.
+#include <rocblas/rocblas.h>
+
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -200,6 +202,63 @@ __global__ void k_brestart(int nb, int m, const double *__restrict__ beta2,
   scale[r] = b > 0.0 ? 1.0 / b : 0.0;
 }
 
+// ---- block GMRES: column Gram-Schmidt inside one block of p vectors ------
+// partial[(k - k0) * kDotBlocks + b] = sum over block b's rows of W_k . W_c,
+// k = k0 .. k0 + gridDim.y - 1 (columns of n entries back to back).
+__global__ __launch_bounds__(kThreads) void k_coldots(int n, int c, int k0,
+                                                      const double *__restrict__ W,
+                                                      double *__restrict__ partial)
+{
+  __shared__ double red[kThreads / 64];
+  const int k = k0 + (int)blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double *a = W + (size_t)k * n, *b = W + (size_t)c * n;
+  double s = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    s = fma(a[i], b[i], s);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; q++) t += red[q];
+    partial[(size_t)blockIdx.y * kDotBlocks + blockIdx.x] = t;
+  }
+}
+
+// W_c -= sum_{k < c} coef[k] W_k
+__global__ void k_colupdate(int n, int c, const double *__restrict__ coef,
+                            double *__restrict__ W)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < c; k++) s = fma(coef[k], W[(size_t)k * n + i], s);
+  W[(size_t)c * n + i] -= s;
+}
+
+// S[k + c p] = coef[k] + coef2[k] (k < c), the CGS2 coefficients of column c
+__global__ void k_colcoef(int c, int p, const double *__restrict__ coef,
+                          const double *__restrict__ coef2, double *__restrict__ S)
+{
+  const int k = threadIdx.x;
+  if (k < p) S[k + (size_t)c * p] = k < c ? coef[k] + coef2[k] : 0.0;
+}
+
+// Normalises column c: S[c + c p] = ||W_c||, W_c /= ||W_c||; a column whose
+// norm fell below 1e-12 of its norm before the orthogonalisation (norm0sq)
+// depends on the others and is set to zero with S[c + c p] = 0.
+__global__ void k_colnormalize(int n, int c, int p, const double *__restrict__ nsq,
+                               const double *__restrict__ norm0sq, double *__restrict__ S,
+                               double *__restrict__ W)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double nrm = sqrt(fmax(nsq[0], 0.0));
+  const bool keep = nrm > 1e-12 * sqrt(fmax(norm0sq[0], 0.0)) && nrm > 0.0;
+  if (i == 0) S[c + (size_t)c * p] = keep ? nrm : 0.0;
+  if (i < n) W[(size_t)c * n + i] = keep ? W[(size_t)c * n + i] / nrm : 0.0;
+}
+
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
@@ -227,10 +286,25 @@ struct ebh_solver {
   bool use_graphs = true;
   int check = 4;
   int matvec64 = 0, matvec32 = 0;
+  // block GMRES (batched solves with `block` set): one block Krylov space of
+  // nb columns per restart cycle, packed densely in V (column c at V + c n);
+  // Hb, H2b: block Gram-Schmidt coefficients ((m+1) nb x nb); Sb: the factor
+  // of the new block (nb x nb); Yb: the cycle's coefficients; cw: column
+  // Gram-Schmidt scratch; blas: rocBLAS handle on `stream` with workspace
+  bool block = false;
+  rocblas_handle blas = nullptr;
+  DevBuf<double> Hb, H2b, Sb, Yb, cw;
+  DevBuf<unsigned char> blas_ws;
+  // host side of the block least-squares problem: R (the block Hessenberg
+  // matrix reduced by Householder reflectors, column-major, leading
+  // dimension (m+1) nb), the reflectors (one of nb+1 entries per column,
+  // starting at the column's own row) and the reduced right-hand sides
+  std::vector<double> hR, hV, hbeta, hG, hH, hS;
 
   ~ebh_solver()
   {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (blas) (void)rocblas_destroy_handle(blas);
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     if (h_est) (void)hipHostFree(h_est);
     if (h_active) (void)hipHostFree(h_active);
@@ -363,6 +437,298 @@ struct ebh_solver {
     return bdots(nb, 1, Rv.p, 0, Rv.p, allon.p, beta2.p, 1, 0);
   }
 
+  // ---------------------------------------------------------- block GMRES
+  int blas_setup(int p)
+  {
+    const size_t m1 = (size_t)m + 1;
+    EBH_TRY(Hb.alloc(m1 * p * p));
+    EBH_TRY(H2b.alloc(m1 * p * p));
+    EBH_TRY(Sb.alloc((size_t)p * p));
+    EBH_TRY(Yb.alloc((size_t)m * p * p));
+    EBH_TRY(cw.alloc(2 * (size_t)p + 2));
+    if (!blas) {
+      if (rocblas_create_handle(&blas) != rocblas_status_success) {
+        blas = nullptr;
+        set_error("rocblas_create_handle failed");
+        return EBH_ERR_HIP;
+      }
+      // a workspace of its own, so that no call allocates (graph capture)
+      EBH_TRY(blas_ws.alloc(size_t(32) << 20));
+      if (rocblas_set_workspace(blas, blas_ws.p, blas_ws.bytes()) !=
+              rocblas_status_success ||
+          rocblas_set_pointer_mode(blas, rocblas_pointer_mode_host) !=
+              rocblas_status_success) {
+        set_error("rocBLAS set-up failed");
+        return EBH_ERR_HIP;
+      }
+    }
+    if (rocblas_set_stream(blas, stream) != rocblas_status_success) {
+      set_error("rocblas_set_stream failed");
+      return EBH_ERR_HIP;
+    }
+    return EBH_OK;
+  }
+
+  int dgemm(rocblas_operation ta, int rows, int cols, int kk, double alpha, const double *A,
+            int lda, const double *B, int ldb, double beta, double *C, int ldc)
+  {
+    if (rocblas_dgemm(blas, ta, rocblas_operation_none, rows, cols, kk, &alpha, A, lda, B,
+                      ldb, &beta, C, ldc) != rocblas_status_success) {
+      set_error("rocblas_dgemm failed");
+      return EBH_ERR_HIP;
+    }
+    return EBH_OK;
+  }
+
+  // dots of columns k0 .. k0+cnt-1 of W with column c into out[0 .. cnt-1]
+  int coldots(const double *W, int c, int k0, int cnt, double *out)
+  {
+    hipLaunchKernelGGL(k_coldots, dim3(kDotBlocks, cnt), dim3(kThreads), 0, stream, n, c,
+                       k0, W, partial.p);
+    hipLaunchKernelGGL(k_bdots_finish, dim3((cnt + 3) / 4), dim3(256), 0, stream, cnt, 1,
+                       partial.p, out, 0, 0);
+    EBH_HIP(hipGetLastError());
+    return EBH_OK;
+  }
+
+  // W = Q S for the p columns at W: Q in place (orthonormal columns, a
+  // dependent column set to zero), S (p x p, column-major, upper triangular)
+  // on the device.  Classical Gram-Schmidt twice per column, deterministic
+  // reductions; kernels only.
+  int block_qr(double *W, int p, double *S)
+  {
+    double *coef = cw.p, *coef2 = cw.p + p, *n0 = cw.p + 2 * p, *nsq = n0 + 1;
+    for (int c = 0; c < p; c++) {
+      EBH_TRY(coldots(W, c, c, 1, n0));
+      if (c > 0) {
+        EBH_TRY(coldots(W, c, 0, c, coef));
+        hipLaunchKernelGGL(k_colupdate, grid1(n), dim3(256), 0, stream, n, c, coef, W);
+        EBH_TRY(coldots(W, c, 0, c, coef2));
+        hipLaunchKernelGGL(k_colupdate, grid1(n), dim3(256), 0, stream, n, c, coef2, W);
+      }
+      hipLaunchKernelGGL(k_colcoef, dim3(1), dim3(std::max(p, 64)), 0, stream, c, p, coef,
+                         coef2, S);
+      EBH_TRY(coldots(W, c, c, 1, nsq));
+      hipLaunchKernelGGL(k_colnormalize, grid1(n), dim3(256), 0, stream, n, c, p, nsq, n0,
+                         S, W);
+      EBH_HIP(hipGetLastError());
+    }
+    return EBH_OK;
+  }
+
+  // Block step j (kernels only, graph-capturable): W = A P V_j, block
+  // Gram-Schmidt twice against V_0 .. V_j into Hb, then W = V_{j+1} S.
+  int block_step_kernels(ebh_op *A, int j, int p)
+  {
+    const size_t bs = (size_t)p * n;
+    double *vj = V.p + (size_t)j * bs, *w = V.p + (size_t)(j + 1) * bs;
+    ebh_op *P = precond_for(A);
+    if (P) {
+      EBH_TRY(op_apply_multi(P, vj, Zt.p, p, stream, false));
+      EBH_TRY(op_apply_multi(A, Zt.p, w, p, stream, false));
+    } else {
+      EBH_TRY(op_apply_multi(A, vj, w, p, stream, false));
+    }
+    const int K = (j + 1) * p;
+    EBH_TRY(dgemm(rocblas_operation_transpose, K, p, n, 1.0, V.p, n, w, n, 0.0, Hb.p, K));
+    EBH_TRY(dgemm(rocblas_operation_none, n, p, K, -1.0, V.p, n, Hb.p, K, 1.0, w, n));
+    EBH_TRY(dgemm(rocblas_operation_transpose, K, p, n, 1.0, V.p, n, w, n, 0.0, H2b.p, K));
+    EBH_TRY(dgemm(rocblas_operation_none, n, p, K, -1.0, V.p, n, H2b.p, K, 1.0, w, n));
+    hipLaunchKernelGGL(k_axpby, grid1((size_t)K * p), dim3(256), 0, stream, (size_t)K * p,
+                       1.0, H2b.p, 1.0, Hb.p);
+    EBH_HIP(hipGetLastError());
+    return block_qr(w, p, Sb.p);
+  }
+
+  int block_step(ebh_op *A, int j, int p)
+  {
+    ebh_op *P = precond_for(A);
+    if (P) precond_applies += p;
+    if (!use_graphs || A->prof_on) {
+      EBH_TRY(block_step_kernels(A, j, p));
+    } else {
+      // block steps are keyed by -p in the batch slot
+      auto key = std::make_tuple((const ebh_op *)A, (const ebh_op *)P, j, -p,
+                                 far_op(A)->gen, P ? far_op(P)->gen : -1);
+      auto it = graphs.find(key);
+      if (it == graphs.end()) {
+        hipGraph_t gr;
+        EBH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int rc = block_step_kernels(A, j, p);
+        const hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc != EBH_OK) return rc;
+        if (e != hipSuccess) {
+          set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+          return EBH_ERR_HIP;
+        }
+        hipGraphExec_t ex;
+        const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e2 != hipSuccess) {
+          set_error(std::string("graph instantiate failed: ") + hipGetErrorString(e2));
+          return EBH_ERR_HIP;
+        }
+        it = graphs.emplace(key, ex).first;
+      }
+      EBH_HIP(hipGraphLaunch(it->second, stream));
+    }
+    if (A == a64) matvec64 += p; else matvec32 += p;
+    return EBH_OK;
+  }
+
+  // Householder reflector of x[0..len-1] (x := alpha e_1): v in place of x
+  // with v[0] = x[0] - alpha, returns beta = 2 / v.v (0 for x = 0)
+  static double house(double *x, int len, double *alpha)
+  {
+    double ss = 0.0;
+    for (int i = 1; i < len; i++) ss += x[i] * x[i];
+    const double nrm = std::sqrt(x[0] * x[0] + ss);
+    if (nrm == 0.0) { *alpha = 0.0; return 0.0; }
+    *alpha = x[0] >= 0.0 ? -nrm : nrm;
+    x[0] -= *alpha;
+    return 2.0 / (x[0] * x[0] + ss);
+  }
+  static void apply_house(const double *v, double beta, int len, double *y)
+  {
+    if (beta == 0.0) return;
+    double d = 0.0;
+    for (int i = 0; i < len; i++) d += v[i] * y[i];
+    d *= beta;
+    for (int i = 0; i < len; i++) y[i] -= d * v[i];
+  }
+
+  // Host half of block step j: the new block column [Hb; Sb] of the block
+  // Hessenberg matrix, reduced by the earlier reflectors and p new ones
+  // (column g's reflector acts on rows g .. g + p); the same reflectors act
+  // on the right-hand sides hG.  est[r] = || rows (j+1)p .. (j+2)p-1 of
+  // column r of hG ||, the residual of system r after this step.
+  int block_lsq(int j, int p, std::vector<double> &est)
+  {
+    const int K = (j + 1) * p;
+    const size_t LD = (size_t)(m + 1) * p;
+    hH.resize((size_t)K * p);
+    hS.resize((size_t)p * p);
+    EBH_HIP(hipMemcpyAsync(hH.data(), Hb.p, hH.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, stream));
+    EBH_HIP(hipMemcpyAsync(hS.data(), Sb.p, hS.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, stream));
+    EBH_HIP(hipStreamSynchronize(stream));
+    for (int c = 0; c < p; c++) {
+      double *col = &hR[(size_t)(j * p + c) * LD];
+      std::fill(col, col + LD, 0.0);
+      for (int i = 0; i < K; i++) col[i] = hH[(size_t)c * K + i];
+      for (int i = 0; i <= c; i++) col[K + i] = hS[(size_t)c * p + i];
+      for (int g = 0; g < j * p; g++)
+        apply_house(&hV[(size_t)g * (p + 1)], hbeta[g], p + 1, col + g);
+    }
+    for (int c = 0; c < p; c++) {
+      const int g = j * p + c;
+      double *col = &hR[(size_t)g * LD];
+      double *v = &hV[(size_t)g * (p + 1)];
+      for (int i = 0; i <= p; i++) v[i] = col[g + i];
+      double alpha;
+      hbeta[g] = house(v, p + 1, &alpha);
+      col[g] = alpha;
+      for (int i = 1; i <= p; i++) col[g + i] = 0.0;
+      for (int c2 = c + 1; c2 < p; c2++)
+        apply_house(v, hbeta[g], p + 1, &hR[(size_t)(j * p + c2) * LD + g]);
+      for (int r = 0; r < p; r++) apply_house(v, hbeta[g], p + 1, &hG[(size_t)r * LD + g]);
+    }
+    est.resize(p);
+    for (int r = 0; r < p; r++) {
+      double t = 0.0;
+      for (int i = K; i < K + p; i++) t += hG[(size_t)r * LD + i] * hG[(size_t)r * LD + i];
+      est[r] = std::sqrt(t);
+    }
+    return EBH_OK;
+  }
+
+  // Block GMRES(m) on A Z = B for the p = nb columns of B: restart cycles of
+  // up to m block steps in one shared block Krylov space (each step one
+  // p-column apply of A P), until every column's residual estimate meets
+  // its tolerance; iters receives the block steps (the same for every
+  // system).
+  int block_gmres(ebh_op *A, const double *B, double *Z, bool z_zero, int p,
+                  const std::vector<double> &atol_h, int maxit, std::vector<int> &iters)
+  {
+    EBH_TRY(blas_setup(p));
+    const size_t LD = (size_t)(m + 1) * p;
+    hR.assign(LD * m * p, 0.0);
+    hV.assign((size_t)m * p * (p + 1), 0.0);
+    hbeta.assign((size_t)m * p, 0.0);
+    hG.assign(LD * p, 0.0);
+    iters.assign(p, 0);
+    const size_t bs = (size_t)p * n;
+    int total = 0;
+    std::vector<double> est(p), y;
+    while (true) {
+      EBH_TRY(residual(A, B, Z, z_zero, p));
+      z_zero = false;
+      hipLaunchKernelGGL(k_copy, grid1(bs), dim3(256), 0, stream, bs, Rv.p, V.p);
+      EBH_TRY(block_qr(V.p, p, Sb.p));
+      hS.resize((size_t)p * p);
+      EBH_HIP(hipMemcpyAsync(hS.data(), Sb.p, hS.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream));
+      EBH_HIP(hipStreamSynchronize(stream));
+      std::fill(hG.begin(), hG.end(), 0.0);
+      bool met = true;
+      for (int r = 0; r < p; r++) {
+        double t = 0.0;
+        for (int i = 0; i <= r; i++) {
+          hG[(size_t)r * LD + i] = hS[(size_t)r * p + i];
+          t += hS[(size_t)r * p + i] * hS[(size_t)r * p + i];
+        }
+        met &= std::sqrt(t) <= atol_h[r];
+      }
+      if (met || total >= maxit) break;
+      int J = 0;
+      for (int j = 0; j < m && total < maxit; j++) {
+        EBH_TRY(block_step(A, j, p));
+        EBH_TRY(block_lsq(j, p, est));
+        total++;
+        J = j + 1;
+        bool all = true;
+        for (int r = 0; r < p; r++) all &= est[r] <= atol_h[r];
+        if (all) break;
+      }
+      for (int r = 0; r < p; r++) iters[r] += J;
+      // Y = R^-1 G over the J p columns (a zero pivot, from a dependent
+      // column, gives a zero coefficient)
+      const int Kc = J * p;
+      y.assign((size_t)Kc * p, 0.0);
+      double rmax = 0.0;
+      for (int g = 0; g < Kc; g++) rmax = std::max(rmax, std::fabs(hR[(size_t)g * LD + g]));
+      for (int r = 0; r < p; r++) {
+        double *yy = &y[(size_t)r * Kc];
+        const double *gg = &hG[(size_t)r * LD];
+        for (int k = Kc - 1; k >= 0; k--) {
+          double t = gg[k];
+          for (int l = k + 1; l < Kc; l++) t -= hR[(size_t)l * LD + k] * yy[l];
+          const double d = hR[(size_t)k * LD + k];
+          yy[k] = std::fabs(d) > 1e-14 * rmax ? t / d : 0.0;
+        }
+      }
+      EBH_HIP(hipMemcpyAsync(Yb.p, y.data(), y.size() * sizeof(double),
+                             hipMemcpyHostToDevice, stream));
+      // Z += P V Y
+      ebh_op *P = precond_for(A);
+      if (P) {
+        EBH_TRY(dgemm(rocblas_operation_none, n, p, Kc, 1.0, V.p, n, Yb.p, Kc, 0.0, Ut.p, n));
+        EBH_TRY(op_apply_multi(P, Ut.p, Zt.p, p, stream, false));
+        precond_applies += p;
+        hipLaunchKernelGGL(k_axpby, grid1(bs), dim3(256), 0, stream, bs, 1.0, Zt.p, 1.0, Z);
+        EBH_HIP(hipGetLastError());
+      } else {
+        EBH_TRY(dgemm(rocblas_operation_none, n, p, Kc, 1.0, V.p, n, Yb.p, Kc, 1.0, Z, n));
+      }
+      EBH_HIP(hipStreamSynchronize(stream));
+      bool all = true;
+      for (int r = 0; r < p; r++) all &= est[r] <= atol_h[r];
+      if (all || total >= maxit) break;
+    }
+    return EBH_OK;
+  }
+
   // GMRES(m) on A Z = B for nb systems with absolute tolerances atol_h;
   // iters receives the Arnoldi steps per system.
   int gmres(ebh_op *A, const double *B, double *Z, bool z_zero, int nb,
@@ -474,7 +840,9 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
   if (!opt.mixed) {
     std::vector<double> atol(nb);
     for (int r = 0; r < nb; r++) atol[r] = opt.tol * bn[r];
-    EBH_TRY(s->gmres(s->a64, b, x, x_zero, nb, atol, opt.max_iters, iters));
+    EBH_TRY(s->block && nb > 1
+                ? s->block_gmres(s->a64, b, x, x_zero, nb, atol, opt.max_iters, iters)
+                : s->gmres(s->a64, b, x, x_zero, nb, atol, opt.max_iters, iters));
     // true FP64 residual of the returned iterates
     EBH_TRY(s->apply(s->a64, x, s->D.p, nb));
     hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, s->stream, tot, 1.0, b, -1.0,
@@ -511,7 +879,11 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
       }
       if (all || outer >= opt.max_outer || total >= opt.max_iters) break;
       hipLaunchKernelGGL(k_fill, grid1(tot), dim3(256), 0, s->stream, tot, 0.0, s->X2.p);
-      EBH_TRY(s->gmres(s->a32, s->D.p, s->X2.p, true, nb, atol, opt.max_iters - total, it));
+      EBH_TRY(s->block && nb > 1
+                  ? s->block_gmres(s->a32, s->D.p, s->X2.p, true, nb, atol,
+                                   opt.max_iters - total, it)
+                  : s->gmres(s->a32, s->D.p, s->X2.p, true, nb, atol, opt.max_iters - total,
+                             it));
       int mx = 0;
       for (int r = 0; r < nb; r++) { iters[r] += it[r]; mx = std::max(mx, it[r]); }
       total += mx;
@@ -648,6 +1020,18 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32)
   }
   s->p64 = p64;
   s->p32 = p32;
+  return EBH_OK;
+}
+
+int ebh_solver_set_block(ebh_solver *s, int block)
+{
+  if (!s) { set_error("null solver"); return EBH_ERR_ARG; }
+  if (block && s->nbmax < 2) {
+    set_error("block GMRES needs a batched solver (max_batch >= 2)");
+    return EBH_ERR_ARG;
+  }
+  EBH_HIP(hipStreamSynchronize(s->stream));
+  s->block = block != 0;
   return EBH_OK;
 }
 

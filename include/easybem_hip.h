@@ -393,6 +393,17 @@ int ebh_precond_spai(ebh_op *V, int rings, int max_rhs, ebh_op **out);
  * reference). */
 int ebh_precond_calderon_hs(ebh_mesh *mesh, int far_precision, int line_rule,
                             int tri_rule, ebh_op **out);
+/* Block GMRES for batched solves (no counterpart in the reference): with
+ * block set, a batched solve of nb >= 2 systems builds one block Krylov
+ * space of nb columns per restart cycle -- each step one nb-column apply of
+ * A P, block classical Gram-Schmidt twice (rocBLAS GEMMs), a column
+ * Gram-Schmidt QR of the new block and the block Hessenberg least-squares
+ * problem reduced by Householder reflectors on the host -- instead of nb
+ * independent Krylov spaces in lockstep; the restart length counts block
+ * steps.  Mixed-precision refinement uses it for its inner solves.  Each
+ * step's device work is captured as a hipGraph when graphs are on.  The
+ * solver needs max_batch >= 2. */
+int ebh_solver_set_block(ebh_solver *s, int block);
 /* nb systems: b and x hold nb vectors of rows entries back to back; stats
  * (if non-NULL) receives nb entries. */
 int ebh_solver_solve_batched(ebh_solver *s, int nb, const double *b,
