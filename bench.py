@@ -375,6 +375,31 @@ def serving(L, meshes, n_requests=24, levels=(4, 5), rate=None, slots=4,
             "max_relres": max(r["relres"] for r in ok) if ok else None}
 
 
+def resident_graphs(L, meshes, level=4, n=12):
+    """Repeated Dirichlet requests on one resident mesh (the same mesh_key,
+    moving point sources), one at a time, with the workers' Arnoldi steps
+    replayed from captured hipGraphs or launched eagerly: median solve and
+    total latency of the cache hits."""
+    P, T, x0 = meshes.perturbed_icosphere(level, 4242)
+    out = {}
+    for graphs in (True, False):
+        srv = L.Server(max_slots=1, tol=1e-8, mixed=True, dirichlet_p1=True,
+                       graphs=graphs)
+        res = []
+        for k in range(n):
+            r, _ = srv.wait(srv.submit(P, T, x0 * (1.0 + 0.01 * k), key=7))
+            res.append(r)
+        srv.close()
+        hits = [r for r in res if r["cache_hit"]]
+        out["graphs" if graphs else "eager"] = {
+            "solve_ms_p50": float(np.median([r["solve_ms"] for r in hits])),
+            "total_ms_p50": float(np.median([r["total_ms"] for r in hits])),
+            "graph_steps": int(hits[-1]["graph_steps"]),
+            "iters": int(hits[-1]["iters"])}
+    out["level"] = level
+    return out
+
+
 def pmc_probe(L, meshes, level):
     """Fixed workload for hardware-counter passes: a plain device copy of
     512 MiB (the calibration of FETCH_SIZE / WRITE_SIZE) three times, then
@@ -643,6 +668,7 @@ def main():
     if not args.no_serve and args.serve > 0:
         log("serving")
         res["serving"] = serving(L, meshes, n_requests=args.serve, levels=(4, 5, 6))
+        res["serving"]["resident_mesh_graphs"] = resident_graphs(L, meshes)
     print(json.dumps(res))
 
 

@@ -284,7 +284,7 @@ class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer", C.c_int), ("matvec64", C.c_int),
                 ("matvec32", C.c_int), ("relres", C.c_double),
                 ("ms", C.c_double), ("converged", C.c_int),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("graph_steps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -584,7 +584,7 @@ class ServerOpts(C.Structure):
                 ("restart", C.c_int), ("tol", C.c_double), ("mixed", C.c_int),
                 ("line_rule", C.c_int), ("tri_rule", C.c_int),
                 ("max_iters", C.c_int), ("dirichlet_p1", C.c_int),
-                ("no_precond", C.c_int)]
+                ("no_precond", C.c_int), ("no_graphs", C.c_int)]
 
 
 class RequestResult(C.Structure):
@@ -592,7 +592,8 @@ class RequestResult(C.Structure):
                 ("queue_ms", C.c_double), ("setup_ms", C.c_double),
                 ("solve_ms", C.c_double), ("total_ms", C.c_double),
                 ("ne", C.c_int), ("footprint", C.c_size_t),
-                ("cache_hit", C.c_int), ("device_peak", C.c_size_t)]
+                ("cache_hit", C.c_int), ("device_peak", C.c_size_t),
+                ("graph_steps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -603,7 +604,7 @@ class Server:
 
     def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
                  mixed=True, line_rule=4, tri_rule=4, max_iters=2000,
-                 dirichlet_p1=False, precond=True):
+                 dirichlet_p1=False, precond=True, graphs=True):
         L = lib()
         vp = C.c_void_p
         L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -619,7 +620,7 @@ class Server:
         L.ebh_server_destroy.restype = None
         o = ServerOpts(max_slots, hbm_budget, restart, tol, int(mixed),
                        line_rule, tri_rule, max_iters, int(dirichlet_p1),
-                       int(not precond))
+                       int(not precond), int(not graphs))
         self.dirichlet_p1 = bool(dirichlet_p1)
         h = vp()
         check(L.ebh_server_create(C.byref(o), C.byref(h)))

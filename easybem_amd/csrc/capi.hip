@@ -10,6 +10,11 @@ namespace ebh {
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 const char *get_error() { return g_error.c_str(); }
+std::shared_mutex &capture_gate()
+{
+  static std::shared_mutex g;
+  return g;
+}
 std::shared_ptr<MemAccount> &current_account()
 {
   static thread_local std::shared_ptr<MemAccount> a;
@@ -64,7 +69,7 @@ int ebh_malloc(void **ptr, size_t bytes)
 {
   if (!ptr) { set_error("null pointer"); return EBH_ERR_ARG; }
   *ptr = nullptr;
-  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+  if (gated([&] { return hipMalloc(ptr, bytes ? bytes : 1); }) != hipSuccess) {
     *ptr = nullptr;
     set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
     return EBH_ERR_NOMEM;
@@ -73,33 +78,33 @@ int ebh_malloc(void **ptr, size_t bytes)
 }
 int ebh_free(void *ptr)
 {
-  if (ptr) EBH_HIP(hipFree(ptr));
+  if (ptr) EBH_HIP(gated([&] { return hipFree(ptr); }));
   return EBH_OK;
 }
 // Operator and solver streams are non-blocking, so these helpers first wait
 // for all device work: they are for callers that do not manage streams.
 int ebh_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-  EBH_HIP(hipDeviceSynchronize());
-  EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  EBH_HIP(gated([&] { return hipDeviceSynchronize(); }));
+  EBH_HIP(gated([&] { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); }));
   return EBH_OK;
 }
 int ebh_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-  EBH_HIP(hipDeviceSynchronize());
-  EBH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  EBH_HIP(gated([&] { return hipDeviceSynchronize(); }));
+  EBH_HIP(gated([&] { return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost); }));
   return EBH_OK;
 }
 int ebh_memset(void *dst, int value, size_t bytes)
 {
-  EBH_HIP(hipDeviceSynchronize());
-  EBH_HIP(hipMemset(dst, value, bytes));
-  EBH_HIP(hipDeviceSynchronize());
+  EBH_HIP(gated([&] { return hipDeviceSynchronize(); }));
+  EBH_HIP(gated([&] { return hipMemset(dst, value, bytes); }));
+  EBH_HIP(gated([&] { return hipDeviceSynchronize(); }));
   return EBH_OK;
 }
 int ebh_synchronize(void)
 {
-  EBH_HIP(hipDeviceSynchronize());
+  EBH_HIP(gated([&] { return hipDeviceSynchronize(); }));
   return EBH_OK;
 }
 int ebh_device_info(int *n_devices, int *n_cu, size_t *free_bytes,
@@ -285,7 +290,8 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   // non-blocking: no implicit ordering with the legacy default stream, so
   // other host threads' default-stream calls never touch this stream (or a
   // graph being captured on it)
-  if (hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (gated([&] { return hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking); }) !=
+      hipSuccess) {
     delete op;
     set_error("hipStreamCreate failed");
     return EBH_ERR_HIP;
@@ -369,7 +375,7 @@ int ebh_op_apply_host(ebh_op *op, const double *x, double *y, int nrhs)
   EBH_TRY(dy.alloc(cy));
   EBH_TRY(ebh_op_apply(op, dx.p, dy.p, nrhs, nullptr));
   EBH_HIP(hipStreamSynchronize(op->stream));
-  EBH_HIP(hipMemcpy(y, dy.p, cy * sizeof(double), hipMemcpyDeviceToHost));
+  EBH_HIP(gated([&] { return hipMemcpy(y, dy.p, cy * sizeof(double), hipMemcpyDeviceToHost); }));
   return EBH_OK;
 }
 
@@ -534,7 +540,7 @@ void ebh_op_destroy(ebh_op *op)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : op->prof_ev) (void)hipEventDestroy(e);
   for (auto &e : op->prof_nev) (void)hipEventDestroy(e);
-  if (op->stream) (void)hipStreamDestroy(op->stream);
+  if (op->stream) (void)gated([&] { return hipStreamDestroy(op->stream); });
   delete op;
 }
 

@@ -174,7 +174,7 @@ int dense_by_columns(ebh_op *op, double *A)
   }
   EBH_HIP(hipGetLastError());
   EBH_HIP(hipStreamSynchronize(s));
-  EBH_HIP(hipMemcpy(A, dA.p, dA.bytes(), hipMemcpyDeviceToHost));
+  EBH_HIP(gated([&] { return hipMemcpy(A, dA.p, dA.bytes(), hipMemcpyDeviceToHost); }));
   return EBH_OK;
 }
 
@@ -284,9 +284,9 @@ int ebh_assemble_laplace(ebh_mesh *mesh, int order, double *V, double *K, double
       EBH_HIP(hipGetLastError());
     }
     EBH_HIP(hipStreamSynchronize(s));
-    if (V) EBH_HIP(hipMemcpy(V, dV.p, dV.bytes(), hipMemcpyDeviceToHost));
-    if (K) EBH_HIP(hipMemcpy(K, dK.p, dK.bytes(), hipMemcpyDeviceToHost));
-    if (D) EBH_HIP(hipMemcpy(D, dD.p, dD.bytes(), hipMemcpyDeviceToHost));
+    if (V) EBH_HIP(gated([&] { return hipMemcpy(V, dV.p, dV.bytes(), hipMemcpyDeviceToHost); }));
+    if (K) EBH_HIP(gated([&] { return hipMemcpy(K, dK.p, dK.bytes(), hipMemcpyDeviceToHost); }));
+    if (D) EBH_HIP(gated([&] { return hipMemcpy(D, dD.p, dD.bytes(), hipMemcpyDeviceToHost); }));
     return EBH_OK;
   };
   rc = run();

@@ -7,6 +7,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -66,6 +67,21 @@ struct AccountScope {
   AccountScope &operator=(const AccountScope &) = delete;
 };
 
+// Capture gate.  A stream capture (a solver recording an Arnoldi step as a
+// hipGraph) holds it exclusively; the library's device allocations, frees
+// and uploads hold it shared, so that no thread allocates while another
+// captures (the solve server runs several solvers on host threads).
+std::shared_mutex &capture_gate();
+// Runs a call on the legacy stream (a synchronous copy or memset, a device
+// synchronisation, a host or device allocation) under the capture gate: the
+// runtime refuses such calls while another thread captures.
+template <typename F>
+hipError_t gated(F call)
+{
+  std::shared_lock<std::shared_mutex> g(capture_gate());
+  return call();
+}
+
 // Owning device buffer (hipMalloc / hipFree).
 template <typename T>
 struct DevBuf {
@@ -78,7 +94,10 @@ struct DevBuf {
   ~DevBuf() { release(); }
   void release()
   {
-    if (p) (void)hipFree(p);
+    if (p) {
+      std::shared_lock<std::shared_mutex> g(capture_gate());
+      (void)hipFree(p);
+    }
     if (acct) acct->add(-(int64_t)(n * sizeof(T)));
     acct.reset();
     p = nullptr;
@@ -89,7 +108,12 @@ struct DevBuf {
     if (count == n && p) return EBH_OK;
     release();
     if (count == 0) return EBH_OK;
-    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+    hipError_t e;
+    {
+      std::shared_lock<std::shared_mutex> g(capture_gate());
+      e = hipMalloc(&p, count * sizeof(T));
+    }
+    if (e != hipSuccess) {
       p = nullptr;
       set_error("device allocation of " + std::to_string(count * sizeof(T)) +
                 " bytes failed");
@@ -103,7 +127,10 @@ struct DevBuf {
   int upload(const T *h, size_t count)
   {
     EBH_TRY(alloc(count));
-    if (count) EBH_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    if (count) {
+      std::shared_lock<std::shared_mutex> g(capture_gate());
+      EBH_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
     return EBH_OK;
   }
   int upload(const std::vector<T> &v) { return upload(v.data(), v.size()); }

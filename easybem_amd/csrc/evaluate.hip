@@ -333,7 +333,7 @@ int ebh_evaluate_laplace_host(ebh_mesh *mesh, int tri_rule_idx, int far_precisio
   EBH_TRY(dr.alloc(npts));
   EBH_TRY(ebh_evaluate_laplace(mesh, tri_rule_idx, far_precision, npts, dp.p, dt.p,
                                du.p, dr.p, nullptr));
-  EBH_HIP(hipMemcpy(result, dr.p, sizeof(double) * npts, hipMemcpyDeviceToHost));
+  EBH_HIP(gated([&] { return hipMemcpy(result, dr.p, sizeof(double) * npts, hipMemcpyDeviceToHost); }));
   return EBH_OK;
 }
 

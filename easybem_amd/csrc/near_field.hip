@@ -320,8 +320,8 @@ int op_build_near(ebh_op *op)
   if (adj) {
     // transpose on the host once: rows become the P1 test nodes
     std::vector<double> val(nnz), tval(nnz);
-    EBH_HIP(hipMemcpy(val.data(), op->near_val.p, sizeof(double) * nnz,
-                      hipMemcpyDeviceToHost));
+    EBH_HIP(gated([&] { return hipMemcpy(val.data(), op->near_val.p, sizeof(double) * nnz,
+                      hipMemcpyDeviceToHost); }));
     const int nr = csr.nrows, nc = T.np;
     std::vector<int> tptr(nc + 1, 0), tcol(nnz);
     for (int e = 0; e < nnz; e++) tptr[csr.col[e] + 1]++;

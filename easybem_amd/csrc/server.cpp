@@ -363,9 +363,10 @@ struct ebh_server {
       EBH_TRY(ebh_precond_spai(e->A64, 2, 1, &e->P64));
       EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P64));
     }
-    // eager launches: stream capture must not overlap other threads'
-    // allocations and copies, which the other workers make at any time
-    EBH_TRY(ebh_solver_set(e->solver, 0, 0));
+    // captured Arnoldi steps, replayed by every later solve on this mesh
+    // (the capture gate keeps the other workers' allocations out of a
+    // capture)
+    EBH_TRY(ebh_solver_set(e->solver, o.no_graphs ? 0 : 1, 0));
     return EBH_OK;
   }
 
@@ -471,9 +472,10 @@ struct ebh_server {
     ebh_solve_stats st{};
     const int rc = ebh_solver_solve(e->solver, db.p, dx.p, &so, &st);
     r.t.resize(n);
-    EBH_HIP(hipMemcpy(r.t.data(), dx.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    EBH_HIP(ebh::gated([&] { return hipMemcpy(r.t.data(), dx.p, n * sizeof(double), hipMemcpyDeviceToHost); }));
     const auto t2 = Clock::now();
     r.res.iters = st.iters;
+    r.res.graph_steps = st.graph_steps;
     r.res.relres = st.relres;
     r.res.setup_ms = ms_between(t0, t1);
     r.res.solve_ms = ms_between(t1, t2);

@@ -284,6 +284,8 @@ struct ebh_solver {
   std::map<std::tuple<const ebh_op *, const ebh_op *, int, int, int, int>, hipGraphExec_t>
       graphs;
   bool use_graphs = true;
+  int capture_failures = 0;  // captures the runtime invalidated (then eager)
+  int graph_steps = 0;       // steps of the current solve launched as graphs
   int check = 4;
   int matvec64 = 0, matvec32 = 0;
   // block GMRES (batched solves with `block` set): one block Krylov space of
@@ -306,9 +308,9 @@ struct ebh_solver {
     if (stream) (void)hipStreamSynchronize(stream);
     if (blas) (void)rocblas_destroy_handle(blas);
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    if (h_est) (void)hipHostFree(h_est);
-    if (h_active) (void)hipHostFree(h_active);
-    if (h_jdone) (void)hipHostFree(h_jdone);
+    if (h_est) (void)gated([&] { return hipHostFree(h_est); });
+    if (h_active) (void)gated([&] { return hipHostFree(h_active); });
+    if (h_jdone) (void)gated([&] { return hipHostFree(h_jdone); });
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
   }
@@ -375,6 +377,52 @@ struct ebh_solver {
     return EBH_OK;
   }
 
+  using GraphKey = std::tuple<const ebh_op *, const ebh_op *, int, int, int, int>;
+
+  // Replays the graph recorded under key, capturing it from `kernels` the
+  // first time.  The capture holds the capture gate exclusively, so no
+  // library allocation on another host thread overlaps it.  A capture that
+  // fails (the runtime invalidated it) runs the kernels eagerly instead,
+  // and this solver launches eagerly from then on: the result is the same.
+  template <typename F>
+  int replay(const GraphKey &key, F kernels)
+  {
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      hipGraph_t gr = nullptr;
+      hipError_t e;
+      int rc;
+      {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
+        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+          (void)hipGetLastError();
+          use_graphs = false;
+          return kernels();
+        }
+        rc = kernels();
+        e = hipStreamEndCapture(stream, &gr);
+      }
+      if (rc != EBH_OK) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      hipGraphExec_t ex = nullptr;
+      if (e == hipSuccess) e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      if (gr) (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        capture_failures++;
+        use_graphs = false;
+        return kernels();
+      }
+      it = graphs.emplace(key, ex).first;
+    }
+    EBH_HIP(hipGraphLaunch(it->second, stream));
+    graph_steps++;
+    return EBH_OK;
+  }
+
   int step(ebh_op *A, int j, int nb)
   {
     // per-launch event profiling of the operator needs eager launches
@@ -385,27 +433,7 @@ struct ebh_solver {
     } else {
       auto key = std::make_tuple((const ebh_op *)A, (const ebh_op *)P, j, nb,
                                  far_op(A)->gen, P ? far_op(P)->gen : -1);
-      auto it = graphs.find(key);
-      if (it == graphs.end()) {
-        hipGraph_t gr;
-        EBH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        const int rc = step_kernels(A, j, nb);
-        const hipError_t e = hipStreamEndCapture(stream, &gr);
-        if (rc != EBH_OK) return rc;
-        if (e != hipSuccess) {
-          set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
-          return EBH_ERR_HIP;
-        }
-        hipGraphExec_t ex;
-        const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e2 != hipSuccess) {
-          set_error(std::string("graph instantiate failed: ") + hipGetErrorString(e2));
-          return EBH_ERR_HIP;
-        }
-        it = graphs.emplace(key, ex).first;
-      }
-      EBH_HIP(hipGraphLaunch(it->second, stream));
+      EBH_TRY(replay(key, [&]() { return step_kernels(A, j, nb); }));
     }
     if (A == a64) matvec64 += nb; else matvec32 += nb;
     return EBH_OK;
@@ -550,27 +578,7 @@ struct ebh_solver {
       // block steps are keyed by -p in the batch slot
       auto key = std::make_tuple((const ebh_op *)A, (const ebh_op *)P, j, -p,
                                  far_op(A)->gen, P ? far_op(P)->gen : -1);
-      auto it = graphs.find(key);
-      if (it == graphs.end()) {
-        hipGraph_t gr;
-        EBH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        const int rc = block_step_kernels(A, j, p);
-        const hipError_t e = hipStreamEndCapture(stream, &gr);
-        if (rc != EBH_OK) return rc;
-        if (e != hipSuccess) {
-          set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
-          return EBH_ERR_HIP;
-        }
-        hipGraphExec_t ex;
-        const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        if (e2 != hipSuccess) {
-          set_error(std::string("graph instantiate failed: ") + hipGetErrorString(e2));
-          return EBH_ERR_HIP;
-        }
-        it = graphs.emplace(key, ex).first;
-      }
-      EBH_HIP(hipGraphLaunch(it->second, stream));
+      EBH_TRY(replay(key, [&]() { return block_step_kernels(A, j, p); }));
     }
     if (A == a64) matvec64 += p; else matvec32 += p;
     return EBH_OK;
@@ -820,7 +828,7 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
     set_error("batch size must be 1.." + std::to_string(s->nbmax));
     return EBH_ERR_ARG;
   }
-  s->matvec64 = s->matvec32 = s->precond_applies = 0;
+  s->matvec64 = s->matvec32 = s->precond_applies = s->graph_steps = 0;
   EBH_HIP(hipEventRecord(s->e0, s->stream));
   const size_t tot = (size_t)nb * s->n;
   std::vector<double> bn;
@@ -911,6 +919,7 @@ int solve_impl(ebh_solver *s, int nb, const double *b, double *x,
       st[r].relres = rel;
       st[r].ms = ms;
       st[r].converged = rel <= opt.tol;
+      st[r].graph_steps = s->graph_steps;
     }
   }
   if (!conv) {
@@ -981,9 +990,12 @@ int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart, int max_bat
     set_error("solver initialisation failed");
     return EBH_ERR_HIP;
   }
-  if (hipHostMalloc((void **)&s->h_est, nb * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&s->h_active, nb * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&s->h_jdone, nb * sizeof(int)) != hipSuccess ||
+  if (gated([&] { return hipHostMalloc((void **)&s->h_est, nb * sizeof(double)); }) !=
+          hipSuccess ||
+      gated([&] { return hipHostMalloc((void **)&s->h_active, nb * sizeof(int)); }) !=
+          hipSuccess ||
+      gated([&] { return hipHostMalloc((void **)&s->h_jdone, nb * sizeof(int)); }) !=
+          hipSuccess ||
       hipEventCreate(&s->e0) != hipSuccess || hipEventCreate(&s->e1) != hipSuccess) {
     delete s;
     set_error("solver host/event allocation failed");

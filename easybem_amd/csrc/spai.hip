@@ -389,11 +389,11 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
                        V0->near_col.p, V0->near_val.p, dval.p);
     EBH_HIP(hipGetLastError());
     EBH_HIP(hipStreamSynchronize(A->stream));
-    EBH_HIP(hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost));
+    EBH_HIP(gated([&] { return hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost); }));
     if (A->d.stab != 0.0) {
       // + stab m_a m_b, m the lumped node areas
       std::vector<double> m(n);
-      EBH_HIP(hipMemcpy(m.data(), A->mvec.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+      EBH_HIP(gated([&] { return hipMemcpy(m.data(), A->mvec.p, sizeof(double) * n, hipMemcpyDeviceToHost); }));
       for (int a = 0; a < n; a++)
         for (int k = eptr[a]; k < eptr[a + 1]; k++) ev[k] += A->d.stab * m[a] * m[ecol[k]];
     }
@@ -410,18 +410,18 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
                        dval.p);
     EBH_HIP(hipGetLastError());
     EBH_HIP(hipStreamSynchronize(A->stream));
-    EBH_HIP(hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost));
+    EBH_HIP(gated([&] { return hipMemcpy(ev.data(), dval.p, sizeof(double) * nent, hipMemcpyDeviceToHost); }));
   }
   // touching parts: V's near-field CSR
   if (!hs) {
     std::vector<int> nptr(A->nrows + 1), ncol(A->near_nnz);
     std::vector<double> nval(A->near_nnz);
-    EBH_HIP(hipMemcpy(nptr.data(), A->near_ptr.p, sizeof(int) * nptr.size(),
-                      hipMemcpyDeviceToHost));
-    EBH_HIP(hipMemcpy(ncol.data(), A->near_col.p, sizeof(int) * ncol.size(),
-                      hipMemcpyDeviceToHost));
-    EBH_HIP(hipMemcpy(nval.data(), A->near_val.p, sizeof(double) * nval.size(),
-                      hipMemcpyDeviceToHost));
+    EBH_HIP(gated([&] { return hipMemcpy(nptr.data(), A->near_ptr.p, sizeof(int) * nptr.size(),
+                      hipMemcpyDeviceToHost); }));
+    EBH_HIP(gated([&] { return hipMemcpy(ncol.data(), A->near_col.p, sizeof(int) * ncol.size(),
+                      hipMemcpyDeviceToHost); }));
+    EBH_HIP(gated([&] { return hipMemcpy(nval.data(), A->near_val.p, sizeof(double) * nval.size(),
+                      hipMemcpyDeviceToHost); }));
     on_threads([&](int tid) {
       std::vector<int> &pos = posv[tid];
       pos.assign(n, -1);
@@ -491,7 +491,8 @@ extern "C" int ebh_precond_spai(ebh_op *A, int rings, int max_rhs, ebh_op **out)
   op->nrows = op->ncols = n;
   op->near_nnz = (int64_t)mcol.size();
   int rc = EBH_OK;
-  if (hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (gated([&] { return hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking); }) !=
+      hipSuccess) {
     delete op;
     set_error("hipStreamCreate failed");
     return EBH_ERR_HIP;

@@ -326,6 +326,7 @@ typedef struct {
   double ms;        /* wall time of the solve on the stream (HIP events)  */
   int converged;
   int precond;      /* right-preconditioner applies                       */
+  int graph_steps;  /* Arnoldi (or block) steps replayed from a hipGraph  */
 } ebh_solve_stats;
 
 /* a64: FP64 operator (square: rows == cols).  a32: optional FP32-far
@@ -340,10 +341,11 @@ int ebh_solver_create_batched(ebh_op *a64, ebh_op *a32, int restart,
 /* use_graphs: capture each Arnoldi step as a hipGraph and replay it
  * (default 1); check_every: read convergence flags every k steps (default:
  * every step for meshes of 4096+ triangles, else every 4th).  A solver runs
- * on its FP64 operator's stream.  Stream capture must not overlap HIP
- * allocation or copy calls made by other host threads (the capture is
- * invalidated), so multithreaded callers -- the solve server among them --
- * set use_graphs = 0. */
+ * on its FP64 operator's stream.  A capture holds a process-wide gate that
+ * the library's own device allocations, frees and uploads also take, so
+ * solvers on several host threads (the solve server's workers) capture
+ * safely; a capture the runtime invalidates anyway (another component's
+ * allocation) falls back to eager launches for that solver. */
 int ebh_solver_set(ebh_solver *s, int use_graphs, int check_every);
 /* Right preconditioning (no counterpart in the reference): GMRES runs on
  * A P y = b with x = P y, so the residual it minimises and reports stays
@@ -451,6 +453,9 @@ typedef struct {
                           solves use the Calderon preconditioner of V, P0
                           ones its sparse approximate inverse, Neumann
                           solves the dual Calderon preconditioner)           */
+  int no_graphs;       /* 1: launch the Arnoldi steps eagerly; 0 (default):
+                          capture each step of a resident mesh's solver as a
+                          hipGraph once and replay it in later solves        */
 } ebh_server_opts;
 
 typedef struct {
@@ -468,6 +473,7 @@ typedef struct {
                           this request (its set-up and solve allocations; a
                           resident mesh it reuses is counted by the request
                           that built it)                                     */
+  int graph_steps;     /* solver steps replayed from captured hipGraphs      */
 } ebh_request_result;
 
 int ebh_server_create(const ebh_server_opts *opts, ebh_server **out);

@@ -289,3 +289,35 @@ def test_measured_device_bytes_within_estimate_and_budget():
     srv.close()
     print(st)
     assert 0 < st["device_peak"] <= budget and st["device_bytes"] == 0
+
+
+def test_served_solves_replay_graphs_with_identical_results():
+    """Workers capture a resident mesh's Arnoldi steps as hipGraphs (the
+    capture gate keeps other workers' allocations out of a capture) and
+    replay them in later solves on that mesh: the results equal those of a
+    server that launches eagerly, bit for bit, also with other requests
+    running at the same time."""
+    from easybem_amd import _lib as L
+    reqs = stream(6, seed=15, levels=(3, 4))
+    out = {}
+    for graphs in (False, True):
+        srv = L.Server(max_slots=3, graphs=graphs)
+        ids = []
+        for rep in range(2):  # the second pass hits the resident meshes
+            for k, (P, T, x0) in enumerate(reqs):
+                pb = ("dirichlet", "neumann")[k % 2]
+                ids.append(srv.submit(P, T, x0 + 0.1 * rep, key=100 + k, problem=pb))
+        res = [srv.wait(r) for r in ids]
+        srv.close()
+        for r, _ in res:
+            assert r["status"] == 0 and r["relres"] <= 1e-8, r
+        assert sum(r["cache_hit"] for r, _ in res) == len(reqs)
+        # the resident meshes' second solves replay the captured steps
+        hits = [r for r, _ in res if r["cache_hit"]]
+        if graphs:
+            assert all(r["graph_steps"] > 0 for r in hits)
+        else:
+            assert all(r["graph_steps"] == 0 for r, _ in res)
+        out[graphs] = [t for _, t in res]
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(a, b)
