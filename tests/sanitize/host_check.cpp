@@ -1,0 +1,159 @@
+// Host-side code under the sanitizers: the topology and near-field pattern
+// builders (easybem_amd/csrc/host_geometry.cpp, threaded) and the CPU
+// oracle (oracle/oracle.c), on valid and invalid meshes.  Built and run by
+// tests/test_sanitize.py with -fsanitize=address,undefined and with
+// -fsanitize=thread; any sanitizer report fails the run, and the checks
+// below (pair counts by closed form, rejections) end it with status 1.
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <algorithm>
+#include <vector>
+
+#include "host_geometry.h"
+#ifdef WITH_ORACLE
+#include "oracle.h"
+#endif
+
+namespace {
+
+int failures = 0;
+void expect(bool ok, const char *what)
+{
+  if (!ok) {
+    std::fprintf(stderr, "check failed: %s\n", what);
+    failures++;
+  }
+}
+
+// torus R = 1, r = 0.4 on an nu x nv grid, two triangles per quad, 1-based
+void torus(int nu, int nv, std::vector<double> &P, std::vector<int> &T)
+{
+  P.clear();
+  T.clear();
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j < nv; j++) {
+      const double a = 2 * M_PI * i / nu, b = 2 * M_PI * j / nv;
+      P.push_back((1 + 0.4 * std::cos(b)) * std::cos(a));
+      P.push_back((1 + 0.4 * std::cos(b)) * std::sin(a));
+      P.push_back(0.4 * std::sin(b));
+    }
+  auto id = [&](int i, int j) { return ((i + nu) % nu) * nv + (j + nv) % nv + 1; };
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j < nv; j++) {
+      const int a = id(i, j), b = id(i + 1, j), c = id(i + 1, j + 1), d = id(i, j + 1);
+      T.insert(T.end(), {a, b, c, a, c, d});
+    }
+}
+
+void check_mesh(const std::vector<double> &P, const std::vector<int> &T, bool torus_grid)
+{
+  const int np = (int)P.size() / 3, ne = (int)T.size() / 3;
+  ebh::Topology top;
+  const std::string err = ebh::build_topology(np, P.data(), ne, T.data(), top);
+  expect(err.empty(), "valid mesh accepted");
+  if (!err.empty()) return;
+  expect(top.n_class[ebh::kCoincident] == ne, "coincident pairs = ne");
+  expect(top.n_class[ebh::kEdge] == 3 * (int64_t)ne, "edge pairs = 3 ne");
+  if (torus_grid) expect(top.n_class[ebh::kVertex] == 18 * (int64_t)np, "vertex pairs = 18 np");
+  const int64_t total = top.n_class[0] + top.n_class[1] + top.n_class[2] + top.n_class[3];
+  expect(total == (int64_t)ne * ne, "pair classes cover all pairs");
+  for (int tb = 0; tb < 2; tb++)
+    for (int trb = 0; trb < 2; trb++) {
+      ebh::NearCSR c;
+      ebh::build_near_csr(top, tb, trb, c);
+      expect(c.row_ptr.size() == (size_t)(tb ? np : ne) + 1, "CSR rows");
+      expect(c.contrib_ptr.size() == c.col.size() + 1, "CSR contributions");
+    }
+  std::vector<double> curl;
+  ebh::surface_curls(top, curl);
+  expect(curl.size() == 9 * (size_t)ne, "curl size");
+#ifdef WITH_ORACLE
+  std::vector<double> V((size_t)ne * ne), K((size_t)ne * np), M((size_t)ne * np);
+  expect(orc_laplace_dense(np, P.data(), ne, T.data(), 3, 4, V.data(), K.data(), M.data()) ==
+             0, "oracle dense Laplace");
+  std::vector<double> Ve(9 * (size_t)ne * ne);
+  expect(orc_kelvin_dense(np, P.data(), ne, T.data(), 0.3, 3, 3, Ve.data()) == 0,
+         "oracle dense Kelvin");
+  // x holds 3 components of the larger DOF space (triangles for P0)
+  std::vector<double> x(3 * (size_t)std::max(np, ne), 0.5), y(3 * 8);
+  const int rows[8] = {0, 1, 2, 3, 5, 8, 13, 21};
+  for (int op = 0; op < 3; op++)
+    for (int tb = 0; tb < 2; tb++)
+      for (int trb = 0; trb < 2; trb++) {
+        if ((op == 1 && trb != 1) || (op == 2 && tb != trb)) continue;
+        expect(orc_apply(op, tb, trb, np, P.data(), ne, T.data(), 3, 9, 0.3, x.data(), 8,
+                         rows, y.data(), 2) == 0, "oracle apply");
+      }
+  std::vector<double> oc(9 * (size_t)ne), tt(ne, 1.0), uu(np, 1.0), pts = {0.0, 0.0, 3.0},
+      res(1);
+  expect(orc_surface_curls(np, P.data(), ne, T.data(), oc.data()) == 0, "oracle curls");
+  expect(orc_evaluate(np, P.data(), ne, T.data(), 1, pts.data(), tt.data(), uu.data(), 19,
+                      res.data(), 2) == 0, "oracle evaluate");
+#endif
+}
+
+}  // namespace
+
+int main()
+{
+  std::vector<double> P;
+  std::vector<int> T;
+  // valid meshes: torus grids (the threaded builders split their work
+  // across host threads from a few hundred triangles up)
+  torus(8, 4, P, T);
+  check_mesh(P, T, true);
+  torus(48, 24, P, T);
+  check_mesh(P, T, true);
+  // invalid meshes are rejected with a message
+  ebh::Topology top;
+  {
+    std::vector<int> open(T.begin(), T.end() - 3);
+    expect(!ebh::build_topology((int)P.size() / 3, P.data(), (int)open.size() / 3,
+                                open.data(), top).empty(), "open mesh rejected");
+  }
+  {
+    std::vector<int> bad = T;
+    bad[4] = (int)P.size() / 3 + 7;
+    expect(!ebh::build_topology((int)P.size() / 3, P.data(), (int)bad.size() / 3,
+                                bad.data(), top).empty(), "index out of range rejected");
+  }
+  {
+    std::vector<int> flip = T;
+    std::swap(flip[1], flip[2]);
+    expect(!ebh::build_topology((int)P.size() / 3, P.data(), (int)flip.size() / 3,
+                                flip.data(), top).empty(), "inconsistent orientation rejected");
+  }
+  {
+    std::vector<double> Q = P;
+    const int a = T[0] - 1, b = T[1] - 1;
+    for (int k = 0; k < 3; k++) Q[3 * b + k] = Q[3 * a + k];
+    expect(!ebh::build_topology((int)Q.size() / 3, Q.data(), (int)T.size() / 3, T.data(),
+                                top).empty(), "degenerate triangle rejected");
+  }
+  // quadrature and Sauter-Schwab tables at every index
+  for (int i = 0; i < 20; i++) {
+    std::vector<double> x, y, w;
+    expect(ebh::tri_rule(i, x, y, w), "triangle rule");
+    double c[ebh::kMaxTriPoints];
+    std::vector<double> x2 = x, y2 = y, w2 = w;
+    (void)ebh::ray_order((int)w.size(), x2.data(), y2.data(), w2.data(), c);
+#ifdef WITH_ORACLE
+    double p[2 * 80], ww[80];
+    expect(orc_tri_rule(i, p, ww) == (int)w.size(), "oracle triangle rule");
+#endif
+  }
+  for (int i = 0; i < 19; i++) {
+    std::vector<double> p, w;
+    expect(ebh::line_rule(i, p, w), "line rule");
+  }
+  for (int cls = 1; cls <= 3; cls++)
+    for (int kd = 1; kd <= 2; kd++)
+      for (int nq = 1; nq <= 6; nq++) {
+        std::vector<double> W;
+        ebh::ss_table(cls, kd, nq, W);
+        expect(!W.empty(), "Sauter-Schwab table");
+      }
+  std::printf("host checks: %d failures\n", failures);
+  return failures ? 1 : 0;
+}
