@@ -47,17 +47,25 @@ TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over "
 TRAFFIC_SCHEDULE = ("symmetric", 1)
 
 
-def near_pass_bytes(w, ne):
-    """Algorithmic HBM bytes of one near-field pass (k_finish) of a P1 x P1
-    scalar operator: the near CSR (8 B value + 4 B column per entry, 4 B
-    row pointers), x and y once, and the far sums gathered from the 3 ne
-    (triangle, local node) entries of the node lists (4 + 4 B indices, 8 B
-    value) with their row pointers."""
+def near_pass_bytes(w, ne, partial_rows, fused=False):
+    """Algorithmic HBM bytes of the pass after the far kernel of a P1 x P1
+    scalar operator on the pair-symmetric schedule: every partial row's
+    3 ne sums (8 B each) read once, the near CSR (8 B value + 4 B column per
+    entry, 4 B row pointers), x and y once.  Separate passes (the default):
+    k_reduce_wide also writes the 3 ne sums, which k_finish gathers through
+    the node lists (4 + 4 B indices, 8 B value) with their row pointers.
+    Fused (k_reduce_finish): the sums are read at the nodes' incidence
+    positions, with the node row pointers and one 4 B row block bound per
+    workgroup (about 5 rows a block)."""
     rows, cols, nnz = w["rows"], w["cols"], w["near_nnz"]
     csr = 12 * nnz + 4 * (rows + 1)
     vectors = 8 * cols + 8 * rows
-    gather = 16 * 3 * ne + 4 * (rows + 1)
-    return csr + vectors + gather
+    partials = 8 * 3 * ne * partial_rows
+    if fused:
+        far = partials + 4 * (rows + 1) + 4 * (rows // 5 + 1)
+    else:
+        far = partials + 8 * 3 * ne + 16 * 3 * ne + 4 * (rows + 1)
+    return csr + vectors + far
 
 
 def log(*a):
@@ -565,7 +573,19 @@ def main():
     near_ms, near_launches = L.op_profile_near(V32)
     far_ms, far_launches = L.op_profile(V32, False)
     far_flop = FLOP_SL * w["far_interactions"]
-    near_bytes = near_pass_bytes(w, int(T.shape[0]))
+    # the pass after the far kernel (partial reduction and near field),
+    # with HIP events around it on the operator's stream: the default
+    # separate passes, then the fused pass on the same operator
+    near = {}
+    if V32.schedule == "symmetric":
+        prow = V32.launch()[2]
+        for fz in (False, True):
+            V32.set_reduction(fz)
+            tn = V32.time(xv, yv, reps=10)["near"]
+            b_ = near_pass_bytes(w, int(T.shape[0]), prow, fz)
+            near[fz] = {"ms_per_apply": tn, "bytes_per_apply": b_,
+                        "achieved": b_ / (tn * 1e-3) / 1e9}
+        V32.set_reduction(False)
 
     unpc = {}
     if pc is not None and not args.no_fp64_compare:
@@ -649,15 +669,19 @@ def main():
                      "traffic_source": TRAFFIC_SOURCE,
                      "flop_per_launch": far_flop,
                      "avg_launch_ms": avg_far_ms, "launches": far_launches},
-        "near_field": {"kernel": "k_finish (near CSR x + far gather, FP64)",
-                       "bound": "hbm",
-                       "achieved": near_bytes / (near_ms / near_launches * 1e-3) / 1e9,
-                       "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                       "frac": near_bytes / (near_ms / near_launches * 1e-3) / 1e9
-                       / PEAK_HBM_GBPS,
-                       "bytes_per_launch": near_bytes,
-                       "avg_launch_ms": near_ms / near_launches,
-                       "launches": near_launches},
+        "near_field": ({
+            "kernel": "k_reduce_wide + k_finish (partial sums reduced, then near CSR x "
+                      "plus the gathered sums; FP64)",
+            "bound": "hbm", "achieved": near[False]["achieved"], "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s", "frac": near[False]["achieved"] / PEAK_HBM_GBPS,
+            "bytes_per_apply": near[False]["bytes_per_apply"],
+            "ms_per_apply": near[False]["ms_per_apply"],
+            "k_finish_ms": near_ms / max(near_launches, 1),
+            "fused": {"kernel": "k_reduce_finish (one launch)",
+                      "ms_per_apply": near[True]["ms_per_apply"],
+                      "bytes_per_apply": near[True]["bytes_per_apply"],
+                      "frac": near[True]["achieved"] / PEAK_HBM_GBPS}}
+            if near else None),
     }
     if not args.no_cpu_baseline:
         log("CPU baseline")

@@ -67,6 +67,8 @@ def lib():
         L.ebh_op_set_schedule.argtypes = [vp, C.c_int, C.c_int]
         L.ebh_op_get_schedule.argtypes = [vp, C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]
+        L.ebh_op_set_reduction.argtypes = [vp, C.c_int]
+        L.ebh_op_get_reduction.argtypes = [vp, C.POINTER(C.c_int)]
         L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
         L.ebh_op_work.argtypes = [vp, i64p]
@@ -240,6 +242,19 @@ class Operator:
         v, k = C.c_int(), C.c_int()
         check(lib().ebh_op_get_schedule(self.h, C.byref(v), C.byref(k)))
         return k.value
+
+    def set_reduction(self, fused=False):
+        """Symmetric schedule: reduce the group partial sums in a kernel of
+        their own before the near-field pass (the default) or inside it
+        (fused); y is bitwise the same."""
+        check(lib().ebh_op_set_reduction(self.h, int(bool(fused))))
+
+    @property
+    def fused(self):
+        """Whether the fused reduction and near-field pass runs."""
+        v = C.c_int()
+        check(lib().ebh_op_get_reduction(self.h, C.byref(v)))
+        return bool(v.value)
 
     def time(self, x: DeviceArray, y: DeviceArray, reps=10, nrhs=1):
         ms = (C.c_double * 4)()

@@ -154,10 +154,14 @@ int ebh_mesh_create(int np, const double *nodes, int ne, const int *elems,
       (rc = m->touch_ptr.upload(T.touch_ptr)) || (rc = m->touch.upload(T.touch)) ||
       (rc = m->node_tri_ptr.upload(T.node_tri_ptr)) ||
       (rc = m->node_tri.upload(T.node_tri)) ||
-      (rc = m->node_tri_k.upload(T.node_tri_k))) {
+      (rc = m->node_tri_k.upload(T.node_tri_k)) || (rc = m->tri_pos.upload(T.tri_pos)) ||
+      (rc = m->fin_blk.upload(T.fin_blk)) || (rc = m->fin_pos.upload(T.fin_pos))) {
     delete m;
     return rc;
   }
+  m->nfin = (int)T.fin_blk.size() - 1;
+  m->nfpos = T.fin_pos.back();
+  m->fin_ok = T.fin_max <= kFinMax;
   *out = m;
   return EBH_OK;
 }
@@ -420,6 +424,23 @@ int ebh_op_get_schedule(const ebh_op *op, int *schedule, int *split)
   const ebh_op *f = far_op(op);
   *schedule = f->sym ? EBH_SCHED_SYMMETRIC : EBH_SCHED_CHUNKED;
   if (split) *split = f->sym ? f->ksplit : 0;
+  return EBH_OK;
+}
+
+int ebh_op_set_reduction(ebh_op *op, int fused)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  ebh_op *f = far_op(op);
+  f->fused_req = fused != 0;
+  f->gen++;
+  if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_get_reduction(const ebh_op *op, int *fused)
+{
+  if (!op || !fused) { set_error("null argument"); return EBH_ERR_ARG; }
+  *fused = fused_pass(far_op(op)) ? 1 : 0;
   return EBH_OK;
 }
 

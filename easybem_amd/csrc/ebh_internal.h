@@ -148,6 +148,13 @@ struct ebh_mesh {
   ebh::DevBuf<int> tri;                         // ne*3 (0-based)
   ebh::DevBuf<int> touch_ptr, touch;            // exclusion lists
   ebh::DevBuf<int> node_tri_ptr, node_tri, node_tri_k;
+  // P1 test rows of the fused far reduction and near-field pass: padded
+  // incidence positions of (t, k), node row blocks and their first padded
+  // positions (Topology::tri_pos, fin_blk, fin_pos); nfpos the padded row
+  // length; fin_ok: every block's far sums fit the pass's LDS stage
+  ebh::DevBuf<int> tri_pos, fin_blk, fin_pos;
+  int nfin = 0, nfpos = 0;
+  bool fin_ok = false;
   // near-field CSR patterns with their contribution lists, by (test basis,
   // trial basis): every operator of the mesh with those bases shares one
   std::mutex csr_mu;
@@ -155,7 +162,8 @@ struct ebh_mesh {
   size_t bytes() const
   {
     return nodes.bytes() + tri.bytes() + touch_ptr.bytes() + touch.bytes() +
-           node_tri_ptr.bytes() + node_tri.bytes() + node_tri_k.bytes();
+           node_tri_ptr.bytes() + node_tri.bytes() + node_tri_k.bytes() + tri_pos.bytes() +
+           fin_blk.bytes() + fin_pos.bytes();
   }
 };
 
@@ -229,6 +237,9 @@ struct ebh_op {
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   bool sym = false;  // pair-symmetric far-field schedule (k_far_sym)
   int ksplit = 1;    // its step ranges per group pair
+  // its partial sums reduced inside the near-field pass (k_reduce_finish)
+  // instead of by k_reduce_wide before k_finish (ebh_op_set_reduction)
+  bool fused_req = false;
   // right-hand sides per symmetric launch the buffers hold (1, or 3 for the
   // single layer inside a composite operator, applied to 3 or 9 densities)
   int nr_max = 1;
@@ -274,6 +285,9 @@ int composite_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_
 // the hypersingular operator)
 inline ebh_op *far_op(ebh_op *op) { return op->inner ? op->inner : op; }
 inline const ebh_op *far_op(const ebh_op *op) { return op->inner ? op->inner : op; }
+// the pair-symmetric schedule's partial sums reduced inside the near-field
+// pass (k_reduce_finish)
+bool fused_pass(const ebh_op *op);
 // rule sizes with a register-resident far kernel (1, 3, 4, 6, 7, 12, 13)
 bool reg_rule(int nt);
 // source chunks that give about `grid` workgroups at this block size

@@ -69,6 +69,14 @@ bool reg_rule(int nt)
   return nt == 1 || nt == 3 || nt == 4 || nt == 6 || nt == 7 || nt == 12 || nt == 13;
 }
 
+// The pair-symmetric schedule's partial sums are reduced inside the
+// near-field pass (k_reduce_finish) when that is requested and every node
+// row block's sums fit that pass's LDS stage.
+bool fused_pass(const ebh_op *op)
+{
+  return op->sym && op->nc == 1 && op->mesh->fin_ok && op->fused_req;
+}
+
 namespace {
 
 template <typename R>
@@ -92,6 +100,14 @@ struct FarArgs {
   const int *touch_ptr, *touch;
   const double *tw;                   // [nt][ne]
   double *partial;                    // [chunks][nout*nc][ne]
+  const int *pos;                     // pair-symmetric, P1 test, fused pass:
+                                      // the sums of (t, k) go to padded
+                                      // incidence position pos[k ne + t] of
+                                      // a partial row, so that a node row
+                                      // block's sums are contiguous
+  int prow;                           // pair-symmetric: values per partial
+                                      // row and right-hand side (nout ne, or
+                                      // the padded length with pos)
   double phi[16][3];                  // test shape functions at the points
 };
 
@@ -977,19 +993,20 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      double *out = a.partial + ((size_t)c * NR + r) * a.nout * stride + tri;
+      double *out = a.partial + ((size_t)c * NR + r) * a.prow;
       if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
-        out[0] = s;
+        out[tri] = s;
       } else {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], v[q][r], s);
-          out[k * stride] = s;
+          const int at = k * ne + tri;
+          out[a.pos ? a.pos[at] : at] = s;
         }
       }
     }
@@ -1002,8 +1019,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   }
 }
 
+// at least 2 waves per SIMD: the FP64 7-point single layer sits at the
+// 256-register edge, and one register more would halve its occupancy
 template <typename R, int NT, int NR, int DK = 0>
-__global__ __launch_bounds__(256) void k_far_sym(FarArgs<R> a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_far_sym(
+    FarArgs<R> a)
 {
   __shared__ __align__(16) R lds[SymLay<R, NT, NR, DK>::LDS];
   sym_block<R, NT, NR, DK>(a, (int)blockIdx.x, lds);
@@ -1555,6 +1575,126 @@ inline unsigned finish_blocks(int nrows)
   return (unsigned)(((size_t)nrows * kRowLanes + 255) / 256);
 }
 
+static_assert(kFinRows * kRowLanes == 256, "one fused-pass row block per workgroup");
+
+// The pair-symmetric schedule's partial sums and the near-field pass in one
+// launch, for right-hand side plane rhs of nplanes (prow values per row and
+// plane).  Workgroup b owns test DOF rows r0 .. r1-1: with P1 test
+// functions node row block b of the mesh, whose incidences' far sums the far
+// kernel stored from padded position fpos[b] (a multiple of kFinPos) of
+// every partial row; with P0 test functions the kFinRows triangles from
+// position r0 = b kFinRows.  Stage 1 sums each position over the partial
+// rows in k_reduce_wide's order (8 slices of every 8th row, then the slices
+// in order) into LDS, kFinPos aligned positions a pass; stage 2 is
+// finish_row with those sums, so the result is bitwise that of
+// k_reduce_wide + k_finish.
+__device__ __forceinline__ void reduce_finish(int b, int rhs, int nplanes, int prow,
+                                              int nrows, int ncols, int test_p1, int chunks,
+                                              const int *__restrict__ blk,
+                                              const int *__restrict__ fpos,
+                                              const int *__restrict__ ptr,
+                                              const int *__restrict__ col,
+                                              const double *__restrict__ val,
+                                              const double *__restrict__ x,
+                                              const double *__restrict__ partial,
+                                              const int *__restrict__ ntp,
+                                              double *__restrict__ y,
+                                              double (*red)[kFinPos + 1], double *farv)
+{
+  const int r0 = test_p1 ? blk[b] : b * kFinRows;
+  const int r1 = test_p1 ? blk[b + 1] : min(r0 + kFinRows, nrows);
+  const int p0 = test_p1 ? ntp[r0] : r0, p1 = test_p1 ? ntp[r1] : r1;
+  const int base = test_p1 ? fpos[b] : r0;  // padded position of p0
+  const size_t stride = (size_t)nplanes * prow;  // one partial row
+  const double *src = partial + (size_t)rhs * prow + (base - p0);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int q0 = p0; q0 < p1; q0 += kFinPos) {  // workgroup-uniform
+    const int p = q0 + tx;
+    double s = 0.0;
+    if (p < p1) {
+      const double *v = src + p;
+      int c = ty;
+      // 8 loads in flight, added in row order
+      for (; c + 56 < chunks; c += 64) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = v[(size_t)(c + 8 * u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += t[u];
+      }
+      for (; c < chunks; c += 8) s += v[(size_t)c * stride];
+    }
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && p < p1) {
+      double w = red[0][tx];
+#pragma unroll
+      for (int k = 1; k < 8; k++) w += red[k][tx];
+      farv[p - p0] = w;
+    }
+    __syncthreads();
+  }
+  const int r = r0 + (int)threadIdx.x / kRowLanes, sub = (int)threadIdx.x % kRowLanes;
+  if (r >= r1) return;  // uniform over the row's lanes
+  x += (size_t)rhs * ncols;
+  y += (size_t)rhs * nrows;
+  double s = 0.0;
+  for (int e = ptr[r] + sub; e < ptr[r + 1]; e += kRowLanes) s = fma(val[e], x[col[e]], s);
+  if (test_p1) {
+    for (int p = ntp[r] + sub; p < ntp[r + 1]; p += kRowLanes) s += farv[p - p0];
+  } else if (sub == 0) {
+    s += farv[r - p0];
+  }
+  s = row_sum(s);
+  if (sub == 0) y[r] = s;
+}
+
+// y = near CSR * x + the summed far partials (one component); blockIdx.y
+// selects the right-hand side plane of nplanes
+__global__ __launch_bounds__(256) void k_reduce_finish(int nplanes, int prow, int nrows,
+                                                       int ncols, int test_p1, int chunks,
+                                                       const int *__restrict__ blk,
+                                                       const int *__restrict__ fpos,
+                                                       const int *__restrict__ ptr,
+                                                       const int *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ partial,
+                                                       const int *__restrict__ ntp,
+                                                       double *__restrict__ y)
+{
+  __shared__ double red[8][kFinPos + 1];
+  __shared__ double farv[kFinMax];
+  reduce_finish((int)blockIdx.x, (int)blockIdx.y, nplanes, prow, nrows, ncols, test_p1,
+                chunks, blk, fpos, ptr, col, val, x, partial, ntp, y, red, farv);
+}
+
+// values per partial row and right-hand side of the fused pass
+inline int fused_prow(const ebh_op *op)
+{
+  return op->d.test_basis ? op->mesh->nfpos : op->mesh->T.ne;
+}
+
+// workgroups of the fused pass
+inline unsigned fused_blocks(const ebh_op *op)
+{
+  return op->d.test_basis ? (unsigned)op->mesh->nfin
+                          : (unsigned)((op->nrows + kFinRows - 1) / kFinRows);
+}
+
+// the fused pass of an operator (nplanes right-hand sides back to back)
+int launch_reduce_finish(const ebh_op *op, const double *x, double *y, int nplanes,
+                         hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reduce_finish, dim3(fused_blocks(op), nplanes), dim3(256), 0, s,
+                     nplanes, fused_prow(op), op->nrows, op->ncols, op->d.test_basis,
+                     op->chunks, op->mesh->fin_blk.p, op->mesh->fin_pos.p, op->near_ptr.p,
+                     op->near_col.p, op->near_val.p, x, op->partial.p,
+                     op->mesh->node_tri_ptr.p, y);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
 template <typename R, int KIND>
 int launch_far(int nt, dim3 grid, dim3 blk, size_t lds, hipStream_t s,
                const FarArgs<R> &a)
@@ -1602,6 +1742,9 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
   a.tw = op->tw.p;
   a.tn = op->normals.p;
   a.partial = op->partial.p;
+  const bool fpos = fused_pass(op) && op->d.test_basis;
+  a.pos = fpos ? op->mesh->tri_pos.p : nullptr;
+  a.prow = fpos ? op->mesh->nfpos : a.nout * ne;
   for (int q = 0; q < 16; q++) {
     const bool in = q < nt;
     a.rc[q] = (R)op->ray_c[q];
@@ -1960,11 +2103,18 @@ int sym_default_split(int ng)
 // right-hand side) the chunked schedule is used.
 constexpr double kSymPartialMax = 4.0 * 1024 * 1024 * 1024;
 
+// values per partial row and right-hand side of the symmetric schedule (the
+// fused pass's padded row with P1 test functions)
+static double sym_row(const ebh_op *op)
+{
+  const double ne = op->mesh->T.ne;
+  return op->d.test_basis ? std::max(3.0 * ne, (double)op->mesh->nfpos) : ne;
+}
+
 bool sym_eligible(const ebh_op *op)
 {
   const double ne = op->mesh->T.ne;
-  const double bytes =
-      std::ceil(ne / 64.0) * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+  const double bytes = std::ceil(ne / 64.0) * op->nr_max * sym_row(op) * 8.0;
   // the single layer with equal bases, or the FP64 double layer or its
   // adjoint (their point sums are weighted by the test functions and their
   // strengths by the trial density, so the bases may differ)
@@ -1994,7 +2144,7 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
       set_error("pair-symmetric step split must be a power of 2 up to 64");
       return EBH_ERR_ARG;
     }
-    const double bytes = (double)ng * ks * op->nr_max * (op->d.test_basis ? 3 : 1) * ne * 8.0;
+    const double bytes = (double)ng * ks * op->nr_max * sym_row(op) * 8.0;
     if (bytes > kSymPartialMax) {
       set_error("pair-symmetric split " + std::to_string(ks) +
                 " needs more than 4 GiB of partial sums on this mesh");
@@ -2031,7 +2181,11 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     op->chunks = (ne + op->chunk - 1) / op->chunk;
   }
   const size_t nout = (size_t)(op->d.test_basis ? 3 : 1) * op->nc;
-  size_t need = (size_t)op->chunks * nout * ne;
+  // values per partial row and right-hand side: the fused pass's padded
+  // row with P1 test functions (fused_prow), nout ne otherwise
+  size_t prow = nout * ne;
+  if (sym && op->d.test_basis) prow = std::max(prow, (size_t)op->mesh->nfpos);
+  size_t need = (size_t)op->chunks * prow;
   if (op->rb) {
     // multi-RHS launch: one lane per target point, 256/nt triangles a block
     // (P0 test and trial: one lane per target triangle, 256 a block)
@@ -2051,7 +2205,7 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     const size_t npts = (size_t)ne * op->rule.nt;
     if (op->d.far_precision == EBH_FP32) EBH_TRY(op->src3_32.alloc(npts * 8));
     else EBH_TRY(op->src3_64.alloc(npts * 8));
-    need = std::max(need, (size_t)op->chunks * 3 * nout * ne);
+    need = std::max(need, (size_t)op->chunks * 3 * prow);
     if (op->tri_out.n < 3 * nout * ne) EBH_TRY(op->tri_out.alloc(3 * nout * ne));
   }
   EBH_TRY(op->partial.alloc(need));
@@ -2225,6 +2379,13 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
 {
   const int ne = op->mesh->T.ne;
   const int nred = (op->d.test_basis ? 3 : 1) * op->nc * ne;
+  if (fused_pass(op)) {
+    const bool nprof = op->prof_on && 2 * (op->prof_ncount + 1) <= (int)op->prof_nev.size();
+    if (nprof) EBH_HIP(hipEventRecord(op->prof_nev[2 * op->prof_ncount], s));
+    EBH_TRY(launch_reduce_finish(op, x, y, 1, s));
+    if (nprof) EBH_HIP(hipEventRecord(op->prof_nev[2 * op->prof_ncount++ + 1], s));
+    return EBH_OK;
+  }
   if (op->sym)
     hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
                        op->chunks, op->partial.p, op->tri_out.p);
@@ -2298,6 +2459,7 @@ static int op_apply_three(ebh_op *op, const double *x, double *y, hipStream_t s)
   const int ne = op->mesh->T.ne;
   EBH_TRY(op->d.far_precision == EBH_FP32 ? run_three<float>(op, x, s)
                                           : run_three<double>(op, x, s));
+  if (fused_pass(op)) return launch_reduce_finish(op, x, y, 3, s);
   const int nred = 3 * (op->d.test_basis ? 3 : 1) * ne;
   hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
                      op->chunks, op->partial.p, op->tri_out.p);
@@ -2359,6 +2521,9 @@ constexpr int kMaxBatch = 64;
 struct MemberDesc {
   int ne, nt, trial_p1, ncols, nrows, nc, test_p1, chunks, nred;
   int sym;  // pair-symmetric schedule (group partials, wide reduction)
+  int fused, nfin, prow;  // the fused pass (fused_pass), its workgroups and
+                          // partial row length
+  const int *fblk, *fpos;  // its node row blocks and their padded positions
   double scale;
   const double *sg, *swt, *normals;
   const int *tri;
@@ -2398,7 +2563,7 @@ __global__ void k_reduce_multi(const MemberDesc *__restrict__ d)
 __global__ __launch_bounds__(256) void k_reduce_wide_multi(const MemberDesc *__restrict__ d)
 {
   const MemberDesc &o = d[blockIdx.y];
-  if (!o.sym || (int)blockIdx.x * 32 >= o.nred) return;  // workgroup-uniform
+  if (!o.sym || o.fused || (int)blockIdx.x * 32 >= o.nred) return;  // workgroup-uniform
   __shared__ double red[8][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int k = blockIdx.x * 32 + tx;
@@ -2418,9 +2583,23 @@ __global__ __launch_bounds__(256) void k_reduce_wide_multi(const MemberDesc *__r
 __global__ void k_finish_multi(const MemberDesc *__restrict__ d, BatchPtrs p)
 {
   const MemberDesc &o = d[blockIdx.y];
+  if (o.fused) return;  // workgroup-uniform
   finish_row(blockIdx.x * blockDim.x + threadIdx.x, 0, o.nrows, o.ncols, o.nc,
              o.test_p1, o.ne, o.near_ptr, o.near_col, o.near_val, p.x[blockIdx.y],
              o.tri_out, o.ntp, o.ntri, o.ntk, p.y[blockIdx.y]);
+}
+
+// the fused pass of every member that has it
+__global__ __launch_bounds__(256) void k_reduce_finish_multi(const MemberDesc *__restrict__ d,
+                                                             BatchPtrs p)
+{
+  const MemberDesc &o = d[blockIdx.y];
+  if (!o.fused || (int)blockIdx.x >= o.nfin) return;  // workgroup-uniform
+  __shared__ double red[8][kFinPos + 1];
+  __shared__ double farv[kFinMax];
+  reduce_finish((int)blockIdx.x, 0, 1, o.prow, o.nrows, o.ncols, o.test_p1, o.chunks, o.fblk,
+                o.fpos, o.near_ptr, o.near_col, o.near_val, p.x[blockIdx.y], o.partial,
+                o.ntp, p.y[blockIdx.y], red, farv);
 }
 
 __global__ void k_zero_int(int n, int *p)
@@ -2481,7 +2660,7 @@ struct ebh_batch {
   int kernel = 0, prec = 0, nt = 0, nops = 0;
   ebh::DevBuf<unsigned char> desc;  // FarArgs<R>[nops]
   ebh::DevBuf<unsigned char> mdesc; // MemberDesc[nops]
-  int max_fold = 0, max_red = 0, max_rows = 0;
+  int max_fold = 0, max_red = 0, max_rows = 0, max_fin = 0;
   ebh::DevBuf<int> off;             // [8][nops + 1] first item per queue
   ebh::DevBuf<int> soff;            // [nops + 1] first symmetric item
   ebh::DevBuf<int> counters;        // [9]: 8 chunked queues, 1 symmetric
@@ -2524,7 +2703,7 @@ int batch_prepare(ebh_batch *b)
 {
   const int nops = b->nops;
   std::vector<MemberDesc> md(nops);
-  b->max_fold = b->max_red = b->max_rows = 0;
+  b->max_fold = b->max_red = b->max_rows = b->max_fin = 0;
   for (int k = 0; k < nops; k++) {
     const ebh_op *op = b->ops[k];
     MemberDesc &m = md[k];
@@ -2537,6 +2716,11 @@ int batch_prepare(ebh_batch *b)
     m.test_p1 = op->d.test_basis;
     m.chunks = op->chunks;
     m.sym = op->sym ? 1 : 0;
+    m.fused = fused_pass(op) ? 1 : 0;
+    m.nfin = m.fused ? (int)fused_blocks(op) : 0;
+    m.fblk = op->mesh->fin_blk.p;
+    m.fpos = op->mesh->fin_pos.p;
+    m.prow = m.fused ? fused_prow(op) : 0;
     m.nred = (op->d.test_basis ? 3 : 1) * op->nc * m.ne;
     m.scale = op->d.kernel == EBH_KELVIN_SL
                   ? (1.0 + op->d.nu) / (8.0 * M_PI * (1.0 - op->d.nu))
@@ -2557,6 +2741,7 @@ int batch_prepare(ebh_batch *b)
     b->max_fold = std::max(b->max_fold, m.ne * m.nt);
     b->max_red = std::max(b->max_red, m.nred);
     b->max_rows = std::max(b->max_rows, m.nrows);
+    b->max_fin = std::max(b->max_fin, m.nfin);
   }
   EBH_TRY(b->mdesc.upload(reinterpret_cast<const unsigned char *>(md.data()),
                           sizeof(MemberDesc) * nops));
@@ -2672,6 +2857,11 @@ int batch_apply(ebh_batch *b, const double *const *x, double *const *y, hipStrea
   hipLaunchKernelGGL(k_finish_multi, dim3(finish_blocks(b->max_rows), b->nops),
                      dim3(256), 0, s, md, p);
   EBH_HIP(hipGetLastError());
+  if (b->max_fin > 0) {
+    hipLaunchKernelGGL(k_reduce_finish_multi, dim3(b->max_fin, b->nops), dim3(256), 0, s,
+                       md, p);
+    EBH_HIP(hipGetLastError());
+  }
   return EBH_OK;
 }
 

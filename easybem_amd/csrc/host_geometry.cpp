@@ -453,6 +453,28 @@ void ss_table(int cls, int kdeg, int nq, std::vector<double> &W)
 }
 
 // ---------------------------------------------------------------- topology
+int fin_blocks(const std::vector<int> &ntp, std::vector<int> &blk, std::vector<int> &base)
+{
+  const int np = (int)ntp.size() - 1;
+  blk.assign(1, 0);
+  base.assign(1, 0);
+  int most = 0;
+  for (int r = 0; r < np;) {
+    const int r0 = r;
+    int n = 0;
+    while (r < np && r - r0 < kFinRows) {
+      const int v = ntp[r + 1] - ntp[r];
+      if (r > r0 && n + v > kFinPos) break;
+      n += v;
+      r++;
+    }
+    blk.push_back(r);
+    base.push_back(base.back() + (n + kFinPos - 1) / kFinPos * kFinPos);
+    most = std::max(most, n);
+  }
+  return most;
+}
+
 std::string build_topology(int np, const double *nodes, int ne,
                            const int *elems, Topology &T)
 {
@@ -511,6 +533,13 @@ std::string build_topology(int np, const double *nodes, int ne,
         T.node_tri_k[fill[n]] = k;
         fill[n]++;
       }
+  }
+  T.fin_max = fin_blocks(T.node_tri_ptr, T.fin_blk, T.fin_pos);
+  T.tri_pos.resize((size_t)3 * ne);
+  for (size_t b = 0; b + 1 < T.fin_blk.size(); b++) {
+    const int q0 = T.node_tri_ptr[T.fin_blk[b]];
+    for (int p = q0; p < T.node_tri_ptr[T.fin_blk[b + 1]]; p++)
+      T.tri_pos[(size_t)T.node_tri_k[p] * ne + T.node_tri[p]] = T.fin_pos[b] + (p - q0);
   }
   // closed, consistently oriented 2-manifold: each undirected edge appears
   // exactly twice, once in each direction.  Edge {a, b}, a < b, is checked

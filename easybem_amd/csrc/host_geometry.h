@@ -47,18 +47,36 @@ struct NearPair {
   int oj[3];      // same for j
 };
 
+// row blocks of the fused far reduction and near-field pass
+constexpr int kFinPos = 32, kFinRows = 32, kFinMax = 1024;
+
 struct Topology {
   int np = 0, ne = 0;
   std::vector<double> nodes;            // np*3
   std::vector<int> tri;                 // ne*3, 0-based
   std::vector<int> node_tri_ptr, node_tri;    // node -> triangles (CSR)
   std::vector<int> node_tri_k;                // local index of node in tri
+  // Blocks of node rows for the fused far reduction and near-field pass
+  // (fin_blocks): block b is rows fin_blk[b] .. fin_blk[b+1]-1; its
+  // incidences ntp[fin_blk[b]] .. ntp[fin_blk[b+1]]-1 sit at the padded
+  // positions fin_pos[b] .. of a partial row, fin_pos[b] a multiple of
+  // kFinPos; tri_pos[k * ne + t] is the padded position of (t, k), and
+  // fin_pos.back() the padded row length
+  std::vector<int> fin_blk, fin_pos, tri_pos;
+  int fin_max = 0;
   std::vector<int> touch_ptr, touch;          // triangle -> touching triangles
                                               // (sorted ascending, incl. self)
   std::vector<NearPair> pairs;                // all touching pairs, (i, j) order
   int64_t n_class[4] = {0, 0, 0, 0};
   int max_touch = 0;
 };
+
+// Row blocks of the fused pass over node rows with incidence row pointers
+// ntp (np + 1 entries): at most kFinRows rows and, unless one row alone has
+// more, kFinPos incidences per block, greedily; blk gets the nblk + 1 row
+// bounds, base the nblk + 1 padded first positions (multiples of kFinPos).
+// Returns the most incidences of a block.
+int fin_blocks(const std::vector<int> &ntp, std::vector<int> &blk, std::vector<int> &base);
 
 // Validates the mesh (index range, non-degenerate triangles, every edge
 // shared by exactly two triangles with opposite orientation) and builds the

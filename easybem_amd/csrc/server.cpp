@@ -99,25 +99,38 @@ struct Entry {
 
 using CacheKey = std::pair<uint64_t, int>;  // (mesh key, problem)
 
-// Far-field partial sums of one operator, nsum FP64 values per target
-// triangle and source slot (outputs per triangle times right-hand sides per
-// launch).  The pair-symmetric schedule keeps ceil(ne/64) x split slots while
-// that fits in 4 GiB (one step range if only that fits), else the chunked
-// schedule keeps about 4096 workgroups' chunks; the split is the larger of
-// the default and any tuned split for this size class, as op_configure
-// applies them.
-double partial_bytes(int ne, int nsum)
+// Far-field partial sums of one operator, row FP64 values per source slot
+// (outputs per triangle times triangles times right-hand sides per launch;
+// with P1 test functions the fused pass's padded row).  The pair-symmetric
+// schedule keeps ceil(ne/64) x split slots while that fits in 4 GiB (one
+// step range if only that fits), else the chunked schedule keeps about 4096
+// workgroups' chunks; the split is the larger of the default and any tuned
+// split for this size class, as op_configure applies them.
+double partial_bytes(int ne, double row)
 {
   const double lim = 4.0 * 1024 * 1024 * 1024;
   const int ng = (ne + 63) / 64;
   const int ks = std::max(ebh::sym_default_split(ng), ebh::tune_max_split(ne));
-  const double one = (double)ng * nsum * ne * 8.0;
+  const double one = (double)ng * row * 8.0;
   if (one * ks <= lim) return one * ks;
   if (one <= lim) return one;
   const int tb = (ne + 255) / 256;
   int chunks = std::max(8, (4096 + tb - 1) / tb);
   chunks = (chunks + 7) / 8 * 8;
-  return (double)chunks * nsum * ne * 8.0;
+  return (double)chunks * row * 8.0;
+}
+
+// The fused pass's padded partial row length for P1 test functions on a
+// mesh (1-based elements; indices out of range are skipped here and
+// rejected when the mesh is built).
+double padded_p1_row(int np, int ne, const int *elems)
+{
+  std::vector<int> ntp((size_t)np + 1, 0), blk, base;
+  for (size_t i = 0; i < (size_t)3 * ne; i++)
+    if (elems[i] >= 1 && elems[i] <= np) ntp[elems[i]]++;
+  for (int n = 0; n < np; n++) ntp[n + 1] += ntp[n];
+  ebh::fin_blocks(ntp, blk, base);
+  return std::max(3.0 * ne, (double)base.back());
 }
 
 // Device bytes of one request, estimated before anything is allocated: the
@@ -133,7 +146,7 @@ double partial_bytes(int ne, int nsum)
 //     layer with 3 values, the dual Calderon preconditioners (FP64, FP32)
 //     with a P1 V.
 size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1,
-                      bool precond)
+                      bool precond, double p1_row)
 {
   const double pairs = 14.0 * ne;
   const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
@@ -145,8 +158,8 @@ size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, boo
   const double mesh = np * 24.0 + ne * 100.0;
   const int nsys = mixed ? 2 : 1;
   const double partial = neumann || p1
-                             ? (nsys + 1 + nprec) * partial_bytes(ne, 3)
-                             : (nsys + 1) * partial_bytes(ne, 1);
+                             ? (nsys + 1 + nprec) * partial_bytes(ne, p1_row)
+                             : (nsys + 1) * partial_bytes(ne, ne);
   return (size_t)(1.25 * (near + ops + krylov + mesh) + partial) + (16u << 20);
 }
 
@@ -542,7 +555,7 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
   r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0,
                                 problem == EBH_NEUMANN,
                                 problem == EBH_NEUMANN || s->o.dirichlet_p1 != 0,
-                                s->o.no_precond == 0);
+                                s->o.no_precond == 0, padded_p1_row(np, ne, elems));
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

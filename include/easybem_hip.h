@@ -156,6 +156,18 @@ int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks);
 #define EBH_SCHED_SYMMETRIC 1
 int ebh_op_set_schedule(ebh_op *op, int schedule, int split);
 int ebh_op_get_schedule(const ebh_op *op, int *schedule, int *split);
+/* How the symmetric schedule's group partial sums reach y.  fused = 0 (the
+ * default): a reduction kernel, then the near-field pass gathering the sums;
+ * fused = 1: one near-field pass sums each test DOF's partials (which the
+ * far kernel then stores at the DOF's node-incidence positions) and adds
+ * the near CSR row.  Both give bitwise the same y; on MI355X the separate
+ * passes measure as fast or faster (the fused pass saves a launch, but its
+ * scattered stores cost the far kernel 0.1-1%).  get returns whether the
+ * fused pass runs (requested, symmetric schedule, one component, every node
+ * row block within the pass's LDS stage).  No counterpart in the
+ * reference. */
+int ebh_op_set_reduction(ebh_op *op, int fused);
+int ebh_op_get_reduction(const ebh_op *op, int *fused);
 /* Averages over the last `ebh_op_time` calls: milliseconds per apply of
  * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
  * measured with HIP events on the op's stream over `reps` applies of the

@@ -58,6 +58,29 @@ void check_mesh(const std::vector<double> &P, const std::vector<int> &T, bool to
   if (torus_grid) expect(top.n_class[ebh::kVertex] == 18 * (int64_t)np, "vertex pairs = 18 np");
   const int64_t total = top.n_class[0] + top.n_class[1] + top.n_class[2] + top.n_class[3];
   expect(total == (int64_t)ne * ne, "pair classes cover all pairs");
+  // fused-pass row blocks: aligned padded bases, every (t, k) at its own
+  // padded position inside its node's block, in incidence order
+  const size_t nb = top.fin_blk.size() - 1;
+  expect(top.fin_blk.front() == 0 && top.fin_blk.back() == np, "row blocks cover the rows");
+  expect(top.fin_pos.size() == nb + 1, "one padded base per block");
+  std::vector<int> owner((size_t)top.fin_pos.back(), -1);
+  for (size_t b = 0; b < nb; b++) {
+    const int r0 = top.fin_blk[b], r1 = top.fin_blk[b + 1];
+    const int q0 = top.node_tri_ptr[r0], n = top.node_tri_ptr[r1] - q0;
+    expect(r1 > r0 && r1 - r0 <= ebh::kFinRows, "rows per block");
+    expect(n <= ebh::kFinPos || r1 - r0 == 1, "incidences per block");
+    expect(top.fin_pos[b] % ebh::kFinPos == 0 && top.fin_pos[b + 1] - top.fin_pos[b] >= n,
+           "aligned padded block");
+    for (int p = q0; p < q0 + n; p++) {
+      const int t = top.node_tri[p], k = top.node_tri_k[p];
+      const int q = top.tri_pos[(size_t)k * ne + t];
+      expect(q == top.fin_pos[b] + (p - q0), "padded position of (t, k)");
+      if (q >= 0 && q < (int)owner.size()) owner[q] = p;
+    }
+  }
+  int placed = 0;
+  for (int v : owner) placed += v >= 0;
+  expect(placed == 3 * ne, "every (t, k) placed once");
   for (int tb = 0; tb < 2; tb++)
     for (int trb = 0; trb < 2; trb++) {
       ebh::NearCSR c;
