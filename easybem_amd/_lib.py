@@ -68,6 +68,8 @@ def lib():
         L.ebh_op_get_schedule.argtypes = [vp, C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]
         L.ebh_op_set_reduction.argtypes = [vp, C.c_int]
+        L.ebh_op_set_partial_limit.argtypes = [vp, C.c_int64]
+        L.ebh_op_get_partial.argtypes = [vp, i64p, C.POINTER(C.c_int), i64p]
         L.ebh_op_get_reduction.argtypes = [vp, C.POINTER(C.c_int)]
         L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
@@ -248,6 +250,19 @@ class Operator:
         their own before the near-field pass (the default) or inside it
         (fused); y is bitwise the same."""
         check(lib().ebh_op_set_reduction(self.h, int(bool(fused))))
+
+    def set_partial_limit(self, nbytes=0):
+        """Bound on the symmetric schedule's partial sums in bytes (0: the
+        default 4 GiB); above it the schedule runs in passes."""
+        check(lib().ebh_op_set_partial_limit(self.h, int(nbytes)))
+
+    @property
+    def partial(self):
+        """(limit, passes, partial buffer bytes) of the far schedule."""
+        lim, nb = C.c_int64(), C.c_int64()
+        npass = C.c_int()
+        check(lib().ebh_op_get_partial(self.h, C.byref(lim), C.byref(npass), C.byref(nb)))
+        return {"limit": lim.value, "passes": npass.value, "bytes": nb.value}
 
     @property
     def fused(self):
@@ -599,7 +614,8 @@ class ServerOpts(C.Structure):
                 ("restart", C.c_int), ("tol", C.c_double), ("mixed", C.c_int),
                 ("line_rule", C.c_int), ("tri_rule", C.c_int),
                 ("max_iters", C.c_int), ("dirichlet_p1", C.c_int),
-                ("no_precond", C.c_int), ("no_graphs", C.c_int)]
+                ("no_precond", C.c_int), ("no_graphs", C.c_int),
+                ("partial_limit", C.c_size_t)]
 
 
 class RequestResult(C.Structure):
@@ -619,7 +635,7 @@ class Server:
 
     def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
                  mixed=True, line_rule=4, tri_rule=4, max_iters=2000,
-                 dirichlet_p1=False, precond=True, graphs=True):
+                 dirichlet_p1=False, precond=True, graphs=True, partial_limit=0):
         L = lib()
         vp = C.c_void_p
         L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -635,7 +651,7 @@ class Server:
         L.ebh_server_destroy.restype = None
         o = ServerOpts(max_slots, hbm_budget, restart, tol, int(mixed),
                        line_rule, tri_rule, max_iters, int(dirichlet_p1),
-                       int(not precond), int(not graphs))
+                       int(not precond), int(not graphs), int(partial_limit))
         self.dirichlet_p1 = bool(dirichlet_p1)
         h = vp()
         check(L.ebh_server_create(C.byref(o), C.byref(h)))

@@ -15,6 +15,12 @@ std::shared_mutex &capture_gate()
   static std::shared_mutex g;
   return g;
 }
+double &thread_partial_limit()
+{
+  static thread_local double bytes = 4.0 * 1024 * 1024 * 1024;
+  return bytes;
+}
+
 std::shared_ptr<MemAccount> &current_account()
 {
   static thread_local std::shared_ptr<MemAccount> a;
@@ -434,6 +440,26 @@ int ebh_op_set_reduction(ebh_op *op, int fused)
   f->fused_req = fused != 0;
   f->gen++;
   if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_set_partial_limit(ebh_op *op, int64_t bytes)
+{
+  if (!op || bytes < 0) { set_error("null op or negative limit"); return EBH_ERR_ARG; }
+  ebh_op *f = far_op(op);
+  f->partial_limit = bytes ? (double)bytes : 4.0 * 1024 * 1024 * 1024;
+  if (f->sym) EBH_TRY(op_configure(f, 0, 0, f->ksplit, true));
+  if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_get_partial(const ebh_op *op, int64_t *limit, int *passes, int64_t *bytes)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  const ebh_op *f = far_op(op);
+  if (limit) *limit = (int64_t)f->partial_limit;
+  if (passes) *passes = f->sym ? std::max(1, (int)f->pass_m.size() - 1) : 0;
+  if (bytes) *bytes = (int64_t)f->partial.bytes();
   return EBH_OK;
 }
 

@@ -67,6 +67,21 @@ struct AccountScope {
   AccountScope &operator=(const AccountScope &) = delete;
 };
 
+// Bound on the pair-symmetric schedule's partial sums (bytes) that
+// operators created on the calling thread start with (ebh_op::partial_limit;
+// 4 GiB unless a PartialLimitScope is open)
+double &thread_partial_limit();
+struct PartialLimitScope {
+  double prev;
+  explicit PartialLimitScope(double bytes) : prev(thread_partial_limit())
+  {
+    thread_partial_limit() = bytes;
+  }
+  ~PartialLimitScope() { thread_partial_limit() = prev; }
+  PartialLimitScope(const PartialLimitScope &) = delete;
+  PartialLimitScope &operator=(const PartialLimitScope &) = delete;
+};
+
 // Capture gate.  A stream capture (a solver recording an Arnoldi step as a
 // hipGraph) holds it exclusively; the library's device allocations, frees
 // and uploads hold it shared, so that no thread allocates while another
@@ -240,6 +255,14 @@ struct ebh_op {
   // its partial sums reduced inside the near-field pass (k_reduce_finish)
   // instead of by k_reduce_wide before k_finish (ebh_op_set_reduction)
   bool fused_req = false;
+  // its partial sums per launch at most this many bytes (ebh_op_set_partial_limit):
+  // above it the schedule runs in passes of block rows pass_m[p] ..
+  // pass_m[p+1]-1, each followed by a reduction into 8 slice sums (pass_tsz
+  // and pass_ssz: the largest target- and source-side buffers, values per
+  // right-hand side)
+  double partial_limit = ebh::thread_partial_limit();
+  std::vector<int> pass_m;
+  size_t pass_tsz = 0, pass_ssz = 0;
   // right-hand sides per symmetric launch the buffers hold (1, or 3 for the
   // single layer inside a composite operator, applied to 3 or 9 densities)
   int nr_max = 1;

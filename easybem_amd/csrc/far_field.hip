@@ -74,7 +74,7 @@ bool reg_rule(int nt)
 // row block's sums fit that pass's LDS stage.
 bool fused_pass(const ebh_op *op)
 {
-  return op->sym && op->nc == 1 && op->mesh->fin_ok && op->fused_req;
+  return op->sym && op->nc == 1 && op->mesh->fin_ok && op->fused_req && op->pass_m.size() <= 2;
 }
 
 namespace {
@@ -108,6 +108,14 @@ struct FarArgs {
   int prow;                           // pair-symmetric: values per partial
                                       // row and right-hand side (nout ne, or
                                       // the padded length with pos)
+  // Pair-symmetric schedule in passes of block rows (bounded partial sums):
+  // the launch covers block rows m0 .. (its grid); target-side sums of its
+  // triangles t0 .. t0+tcols-1 go to partial, row c - c0, and source-side
+  // sums of triangles t0 .. t0+scols-1 to spart (sprow values per row and
+  // right-hand side).  One pass: m0 = t0 = c0 = 0, tcols = scols = ne,
+  // spart = partial, sprow = prow.
+  int m0, t0, c0, tcols, scols, sprow;
+  double *spart;
   double phi[16][3];                  // test shape functions at the points
 };
 
@@ -669,7 +677,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   // otherwise.
   constexpr bool ROT = Lay::ROT;
   const int ks = a.ksplit, ng = a.chunks / ks, ne = a.ne;
-  int rem = bid / ks, M = 0;
+  int rem = bid / ks, M = a.m0;
   const int rng = bid - rem * ks;
   while (rem >= ng - 4 * M) { rem -= ng - 4 * M; M++; }
   const int J = 4 * M + rem;
@@ -985,37 +993,40 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       }
   }
 
-  // point sums of triangle tri to partial[c][r][.][tri]
-  auto store = [&](const double (*v)[NR], int tri, int c) {
+  // point sums of triangle tri to partial row c: the target side's
+  // (partial[c - c0][r][.][tri - t0]) or the source side's (spart[...])
+  auto store = [&](const double (*v)[NR], int tri, int c, bool src_side) {
     const size_t stride = (size_t)ne;
+    double *base = src_side ? a.spart : a.partial;
+    const int rowv = src_side ? a.sprow : a.prow, cols = src_side ? a.scols : a.tcols;
+    const int tl = tri - a.t0;
     double wq[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      double *out = a.partial + ((size_t)c * NR + r) * a.prow;
+      double *out = base + ((size_t)(c - a.c0) * NR + r) * rowv;
       if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
-        out[tri] = s;
+        out[tl] = s;
       } else {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], v[q][r], s);
-          const int at = k * ne + tri;
-          out[a.pos ? a.pos[at] : at] = s;
+          out[a.pos ? a.pos[k * ne + tri] : k * cols + tl] = s;
         }
       }
     }
   };
-  if (t < ne) store(acc64, t, J * ks + rng);
+  if (t < ne) store(acc64, t, J * ks + rng, false);
   if (I < J) {
     // after the steps k0 .. k1-1 lane L holds source (L + k1) mod 64
     const int j = J * 64 + ((L + k1) & 63);
-    if (j < ne) store(S64, j, I * ks + rng);
+    if (j < ne) store(S64, j, I * ks + rng, true);
   }
 }
 
@@ -1475,6 +1486,65 @@ __global__ __launch_bounds__(256) void k_reduce_wide(int n, int chunks,
   }
 }
 
+// One pass of the pair-symmetric schedule in passes (block rows m0 ..
+// m1-1, partial rows c0 .. of the pass): for every entry e = (plane r,
+// output k, triangle t) with t >= t0, slice ty (of 8) adds the pass's
+// partial rows c = ty mod 8 in increasing order to acc[ty][e] -- the
+// source-side rows c0 .. c1-1 (t beyond the pass's triangles) or c0 ..
+// group(t) ks - 1 then the target-side rows group(t) ks .. ng ks - 1 (t of
+// the pass's triangles t0 .. t1-1).  Over all passes every entry sees rows
+// 0 .. ng ks - 1 in increasing order per slice, as k_reduce_wide sums them,
+// so the last pass, which writes the slice totals (in slice order) to out
+// for every entry, gives k_reduce_wide's sums bitwise.
+struct PassArgs {
+  int ne, nplanes, nout, ks, ng, c0, c1, t0, t1, tcols, scols, first, last;
+  const double *tb, *sb;
+  double *acc, *out;
+};
+
+__global__ __launch_bounds__(256) void k_pass_reduce(PassArgs p)
+{
+  __shared__ double red[8][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int lo = p.last ? 0 : p.t0;  // entries of triangles lo .. ne-1
+  const long w = p.ne - lo;
+  const long n = (long)p.nplanes * p.nout * w;
+  const size_t nall = (size_t)p.nplanes * p.nout * p.ne;
+  const long i = (long)blockIdx.x * 32 + tx;
+  double s = 0.0;
+  size_t e = 0;
+  if (i < n) {
+    const int rk = (int)(i / w), t = lo + (int)(i - rk * w);
+    e = (size_t)rk * p.ne + t;
+    if (!p.first) s = p.acc[(size_t)ty * nall + e];
+    if (t >= p.t0) {
+      const int tl = t - p.t0;
+      const bool own = t < p.t1;
+      const size_t srow = (size_t)p.nplanes * p.nout * p.scols;
+      const double *S = p.sb + (size_t)rk * p.scols + tl;
+      const int send = own ? (t >> 6) * p.ks : p.c1;
+      int c = p.c0 + (((ty - p.c0) % 8) + 8) % 8;
+      for (; c < send; c += 8) s += S[(size_t)(c - p.c0) * srow];
+      if (own) {
+        const size_t trow = (size_t)p.nplanes * p.nout * p.tcols;
+        const double *Tq = p.tb + (size_t)rk * p.tcols + tl;
+        for (; c < p.ng * p.ks; c += 8) s += Tq[(size_t)(c - p.c0) * trow];
+      }
+    }
+  }
+  if (p.last) {  // kernel-uniform
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && i < n) {
+      double v = red[0][tx];
+      for (int y = 1; y < 8; y++) v += red[y][tx];
+      p.out[e] = v;
+    }
+  } else if (i < n) {
+    p.acc[(size_t)ty * nall + e] = s;
+  }
+}
+
 // Lanes per test DOF in the near-field pass: every lane of a row takes
 // every 8th CSR entry and every 8th (triangle, local node) entry, and a
 // fixed butterfly adds the 8 partial sums, so the loads of a wave cover 8
@@ -1745,6 +1815,10 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
   const bool fpos = fused_pass(op) && op->d.test_basis;
   a.pos = fpos ? op->mesh->tri_pos.p : nullptr;
   a.prow = fpos ? op->mesh->nfpos : a.nout * ne;
+  a.m0 = a.t0 = a.c0 = 0;
+  a.tcols = a.scols = ne;
+  a.spart = a.partial;
+  a.sprow = a.prow;
   for (int q = 0; q < 16; q++) {
     const bool in = q < nt;
     a.rc[q] = (R)op->ray_c[q];
@@ -1790,24 +1864,72 @@ int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
   return EBH_OK;
 }
 
-// nr: right-hand sides per launch (1, or 3 with the 8-wide source records)
+// nr: right-hand sides per launch (1, or 3 with the 8-wide source records).
+// In passes (op->pass_m) each pass's launch is followed by its reduction,
+// the last writing the group sums to tri_out.
 template <typename R>
 int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
 {
-  const int ng = op->chunks / op->ksplit;
-  int nblk = 0;
-  for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
-  const dim3 grid((unsigned)(nblk * op->ksplit));
-  if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) {
-    // the symmetric double layers exist in FP64 only (sym_eligible)
-    if constexpr (sizeof(R) == 8)
-      return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, a)
-                                            : launch_sym<R, 1, 2>(op->rule.nt, grid, s, a);
-    set_error("no FP32 pair-symmetric double layer");
-    return EBH_ERR_UNSUPPORTED;
+  const int ks = op->ksplit, ng = op->chunks / ks;
+  auto launch = [&](dim3 grid, const FarArgs<R> &b) -> int {
+    if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) {
+      // the symmetric double layers exist in FP64 only (sym_eligible)
+      if constexpr (sizeof(R) == 8)
+        return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, b)
+                                              : launch_sym<R, 1, 2>(op->rule.nt, grid, s, b);
+      set_error("no FP32 pair-symmetric double layer");
+      return EBH_ERR_UNSUPPORTED;
+    }
+    return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, b)
+                   : launch_sym<R, 1>(op->rule.nt, grid, s, b);
+  };
+  if (op->pass_m.size() <= 2) {
+    int nblk = 0;
+    for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
+    return launch(dim3((unsigned)(nblk * ks)), a);
   }
-  return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, a)
-                 : launch_sym<R, 1>(op->rule.nt, grid, s, a);
+  const int ne = op->mesh->T.ne, nout = a.nout, npass = (int)op->pass_m.size() - 1;
+  double *tb = op->partial.p, *sb = tb + op->pass_tsz * op->nr_max,
+         *acc = sb + op->pass_ssz * op->nr_max;
+  for (int p = 0; p < npass; p++) {
+    const int m0 = op->pass_m[p], m1 = op->pass_m[p + 1];
+    FarArgs<R> b = a;
+    b.m0 = m0;
+    b.t0 = 256 * m0;
+    b.c0 = 4 * m0 * ks;
+    b.tcols = std::min(256 * (m1 - m0), ne - b.t0);
+    b.scols = ne - b.t0;
+    b.partial = tb;
+    b.spart = sb;
+    b.prow = nout * b.tcols;
+    b.sprow = nout * b.scols;
+    b.pos = nullptr;
+    int nblk = 0;
+    for (int M = m0; M < m1; M++) nblk += ng - 4 * M;
+    EBH_TRY(launch(dim3((unsigned)(nblk * ks)), b));
+    PassArgs q{};
+    q.ne = ne;
+    q.nplanes = nr;
+    q.nout = nout;
+    q.ks = ks;
+    q.ng = ng;
+    q.c0 = b.c0;
+    q.c1 = std::min(4 * m1, ng) * ks;
+    q.t0 = b.t0;
+    q.t1 = std::min(256 * m1, ne);
+    q.tcols = b.tcols;
+    q.scols = b.scols;
+    q.first = p == 0;
+    q.last = p == npass - 1;
+    q.tb = tb;
+    q.sb = sb;
+    q.acc = acc;
+    q.out = op->tri_out.p;
+    const long n = (long)nr * nout * (q.last ? ne : ne - b.t0);
+    hipLaunchKernelGGL(k_pass_reduce, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, q);
+    EBH_HIP(hipGetLastError());
+  }
+  return EBH_OK;
 }
 
 // source triangles per LDS tile of k_far_pt (at most 64, the mask width, and
@@ -2098,31 +2220,53 @@ int sym_default_split(int ng)
 }
 
 // The symmetric schedule's partial sums take (ne / 64) x nout x ne FP64
-// values per step range and right-hand side of a launch, quadratic in the
-// mesh size; above this bound (ne = 95,000 with P1 test functions and one
-// right-hand side) the chunked schedule is used.
-constexpr double kSymPartialMax = 4.0 * 1024 * 1024 * 1024;
-
-// values per partial row and right-hand side of the symmetric schedule (the
-// fused pass's padded row with P1 test functions)
-static double sym_row(const ebh_op *op)
-{
-  const double ne = op->mesh->T.ne;
-  return op->d.test_basis ? std::max(3.0 * ne, (double)op->mesh->nfpos) : ne;
-}
+// values per step range and right-hand side of a launch in one pass,
+// quadratic in the mesh size; above the operator's limit (ebh_op's
+// partial_limit, 4 GiB by default) it runs in passes of block rows, whose
+// buffers stay within the limit, plus 8 slice sums per output.
 
 bool sym_eligible(const ebh_op *op)
 {
-  const double ne = op->mesh->T.ne;
-  const double bytes = std::ceil(ne / 64.0) * op->nr_max * sym_row(op) * 8.0;
   // the single layer with equal bases, or the FP64 double layer or its
   // adjoint (their point sums are weighted by the test functions and their
   // strengths by the trial density, so the bases may differ)
   const bool sl = op->d.kernel == EBH_LAPLACE_SL && op->d.test_basis == op->d.trial_basis;
   const bool dl = (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) &&
                   op->d.far_precision == EBH_FP64 && op->nr_max == 1;
-  return (sl || dl) && op->ray && op->rule.nt <= 7 && op->nc == 1 &&
-         bytes <= kSymPartialMax;
+  return (sl || dl) && op->ray && op->rule.nt <= 7 && op->nc == 1;
+}
+
+// Passes of the pair-symmetric schedule for partial sums above the
+// operator's limit: block rows (4 target groups each) are taken greedily
+// while a pass's target-side buffer (its triangles, partial rows of its
+// groups and later) and source-side buffer (its groups' rows, triangles
+// from its first on) stay within the limit; one block row at least.
+// pass_tsz / pass_ssz: the largest buffers, values per right-hand side.
+static void plan_passes(ebh_op *op, size_t nout)
+{
+  const size_t ne = op->mesh->T.ne, ks = op->ksplit;
+  const size_t ng = op->chunks / ks, nrow = (ng + 3) / 4;
+  auto sizes = [&](size_t m0, size_t m1, size_t &tsz, size_t &ssz) {
+    const size_t t0 = 256 * m0;
+    const size_t tc = std::min(256 * (m1 - m0), ne - t0), sc = ne - t0;
+    tsz = (ng - 4 * m0) * ks * nout * tc;
+    ssz = (std::min(4 * m1, ng) - 4 * m0) * ks * nout * sc;
+  };
+  const double per = 8.0 * op->nr_max;
+  op->pass_m.assign(1, 0);
+  for (size_t m0 = 0; m0 < nrow;) {
+    size_t m1 = m0 + 1, t, sv;
+    while (m1 < nrow) {
+      sizes(m0, m1 + 1, t, sv);
+      if ((double)(t + sv) * per > op->partial_limit) break;
+      m1++;
+    }
+    sizes(m0, m1, t, sv);
+    op->pass_tsz = std::max(op->pass_tsz, t);
+    op->pass_ssz = std::max(op->pass_ssz, sv);
+    op->pass_m.push_back((int)m1);
+    m0 = m1;
+  }
 }
 
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
@@ -2131,8 +2275,8 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   if (sym) {
     if (!sym_eligible(op)) {
       set_error("the pair-symmetric schedule needs a Laplace single layer with equal "
-                "test and trial bases or an FP64 double layer, a median-symmetric "
-                "far rule and partial sums of at most 4 GiB");
+                "test and trial bases or an FP64 double layer and a median-symmetric "
+                "far rule");
       return EBH_ERR_UNSUPPORTED;
     }
     // groups of 64 triangles; chunks (when given) is the number of step
@@ -2142,12 +2286,6 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     if (ks <= 0) ks = sym_default_split(ng);
     if (ks > 64 || (ks & (ks - 1))) {
       set_error("pair-symmetric step split must be a power of 2 up to 64");
-      return EBH_ERR_ARG;
-    }
-    const double bytes = (double)ng * ks * op->nr_max * sym_row(op) * 8.0;
-    if (bytes > kSymPartialMax) {
-      set_error("pair-symmetric split " + std::to_string(ks) +
-                " needs more than 4 GiB of partial sums on this mesh");
       return EBH_ERR_ARG;
     }
     block = 256;
@@ -2185,7 +2323,14 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   // row with P1 test functions (fused_prow), nout ne otherwise
   size_t prow = nout * ne;
   if (sym && op->d.test_basis) prow = std::max(prow, (size_t)op->mesh->nfpos);
-  size_t need = (size_t)op->chunks * prow;
+  size_t need = (size_t)op->chunks * prow * (sym ? op->nr_max : 1);
+  op->pass_m.clear();
+  op->pass_tsz = op->pass_ssz = 0;
+  if (sym && (double)need * 8.0 > op->partial_limit) {
+    // passes: the largest pass buffers, then the 8 slice sums
+    plan_passes(op, nout);
+    need = (op->pass_tsz + op->pass_ssz) * op->nr_max + (size_t)8 * op->nr_max * nout * ne;
+  }
   if (op->rb) {
     // multi-RHS launch: one lane per target point, 256/nt triangles a block
     // (P0 test and trial: one lane per target triangle, 256 a block)
@@ -2205,7 +2350,6 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     const size_t npts = (size_t)ne * op->rule.nt;
     if (op->d.far_precision == EBH_FP32) EBH_TRY(op->src3_32.alloc(npts * 8));
     else EBH_TRY(op->src3_64.alloc(npts * 8));
-    need = std::max(need, (size_t)op->chunks * 3 * prow);
     if (op->tri_out.n < 3 * nout * ne) EBH_TRY(op->tri_out.alloc(3 * nout * ne));
   }
   EBH_TRY(op->partial.alloc(need));
@@ -2386,7 +2530,9 @@ static int op_finish(ebh_op *op, const double *x, double *y, hipStream_t s)
     if (nprof) EBH_HIP(hipEventRecord(op->prof_nev[2 * op->prof_ncount++ + 1], s));
     return EBH_OK;
   }
-  if (op->sym)
+  if (op->sym && op->pass_m.size() > 2) {
+    // the passes' last reduction wrote tri_out
+  } else if (op->sym)
     hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
                        op->chunks, op->partial.p, op->tri_out.p);
   else
@@ -2461,9 +2607,11 @@ static int op_apply_three(ebh_op *op, const double *x, double *y, hipStream_t s)
                                           : run_three<double>(op, x, s));
   if (fused_pass(op)) return launch_reduce_finish(op, x, y, 3, s);
   const int nred = 3 * (op->d.test_basis ? 3 : 1) * ne;
-  hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
-                     op->chunks, op->partial.p, op->tri_out.p);
-  EBH_HIP(hipGetLastError());
+  if (op->pass_m.size() <= 2) {
+    hipLaunchKernelGGL(k_reduce_wide, dim3((nred + 31) / 32), dim3(256), 0, s, nred,
+                       op->chunks, op->partial.p, op->tri_out.p);
+    EBH_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_finish, dim3(finish_blocks(op->nrows), 3), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,
@@ -2692,7 +2840,9 @@ int batch_shapes(ebh_batch *b, int grid)
   cs = std::max(64, (cs + 63) / 64 * 64);
   for (ebh_op *op : b->ops) {
     const int ne = op->mesh->T.ne;
-    if (batch_sym(op) && op_configure(op, 0, 0, 1, true) == EBH_OK) continue;
+    // (the persistent launch runs one pass per member)
+    if (batch_sym(op) && op_configure(op, 0, 0, 1, true) == EBH_OK && op->pass_m.size() <= 2)
+      continue;
     EBH_TRY(op_configure(op, 256, 64, std::max(1, (ne + cs - 1) / cs)));
   }
   return EBH_OK;

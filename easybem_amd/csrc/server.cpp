@@ -102,22 +102,21 @@ using CacheKey = std::pair<uint64_t, int>;  // (mesh key, problem)
 // Far-field partial sums of one operator, row FP64 values per source slot
 // (outputs per triangle times triangles times right-hand sides per launch;
 // with P1 test functions the fused pass's padded row).  The pair-symmetric
-// schedule keeps ceil(ne/64) x split slots while that fits in 4 GiB (one
-// step range if only that fits), else the chunked schedule keeps about 4096
-// workgroups' chunks; the split is the larger of the default and any tuned
-// split for this size class, as op_configure applies them.
-double partial_bytes(int ne, double row)
+// schedule keeps ceil(ne/64) x split slots in one pass while that fits in
+// the limit, else it runs in passes (op_configure); the split is the larger
+// of the default and any tuned split for this size class, as op_configure
+// applies them.
+double partial_bytes(int ne, double row, double limit)
 {
-  const double lim = 4.0 * 1024 * 1024 * 1024;
   const int ng = (ne + 63) / 64;
   const int ks = std::max(ebh::sym_default_split(ng), ebh::tune_max_split(ne));
-  const double one = (double)ng * row * 8.0;
-  if (one * ks <= lim) return one * ks;
-  if (one <= lim) return one;
-  const int tb = (ne + 255) / 256;
-  int chunks = std::max(8, (4096 + tb - 1) / tb);
-  chunks = (chunks + 7) / 8 * 8;
-  return (double)chunks * row * 8.0;
+  const double one = (double)ng * ks * row * 8.0;
+  if (one <= limit) return one;
+  // passes: buffers within the limit (one block row at least: its 256
+  // triangles' rows and its 4 groups' rows of every triangle), then 8 slice
+  // sums per output
+  const double block_row = (ng * 256.0 + 4.0 * ne) * ks * (row / ne) * 8.0;
+  return std::max(limit, block_row) + 8.0 * row * 8.0;
 }
 
 // The fused pass's padded partial row length for P1 test functions on a
@@ -146,7 +145,7 @@ double padded_p1_row(int np, int ne, const int *elems)
 //     layer with 3 values, the dual Calderon preconditioners (FP64, FP32)
 //     with a P1 V.
 size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1,
-                      bool precond, double p1_row)
+                      bool precond, double p1_row, double limit)
 {
   const double pairs = 14.0 * ne;
   const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
@@ -158,8 +157,8 @@ size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, boo
   const double mesh = np * 24.0 + ne * 100.0;
   const int nsys = mixed ? 2 : 1;
   const double partial = neumann || p1
-                             ? (nsys + 1 + nprec) * partial_bytes(ne, p1_row)
-                             : (nsys + 1) * partial_bytes(ne, ne);
+                             ? (nsys + 1 + nprec) * partial_bytes(ne, p1_row, limit)
+                             : (nsys + 1) * partial_bytes(ne, ne, limit);
   return (size_t)(1.25 * (near + ops + krylov + mesh) + partial) + (16u << 20);
 }
 
@@ -313,9 +312,12 @@ struct ebh_server {
     using Result = std::pair<int, std::string>;
     std::vector<std::future<Result>> jobs;
     std::shared_ptr<ebh::MemAccount> acct = r.acct;
+    const double plim = (double)o.partial_limit;
+    ebh::PartialLimitScope limit_here(plim);
     auto spawn = [&](auto fn) {
-      auto job = [mesh, fn, acct]() {
+      auto job = [mesh, fn, acct, plim]() {
         ebh::AccountScope scope(acct);
+        ebh::PartialLimitScope limit(plim);
         int rc = hipSetDevice(mesh->device) == hipSuccess ? EBH_OK : EBH_ERR_HIP;
         if (rc == EBH_OK) rc = fn();
         return Result(rc, rc == EBH_OK ? std::string() : std::string(ebh_last_error()));
@@ -512,6 +514,7 @@ int ebh_server_create(const ebh_server_opts *opts, ebh_server **out)
   if (!opts) o.mixed = 1;
   if (o.line_rule <= 0) o.line_rule = 4;
   if (o.tri_rule <= 0) o.tri_rule = 4;
+  if (o.partial_limit == 0) o.partial_limit = (size_t)512 << 20;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
     delete s;
@@ -555,7 +558,8 @@ int ebh_server_submit_bvp(ebh_server *s, int problem, int np, const double *node
   r->footprint = estimate_bytes(np, ne, s->o.restart, s->o.mixed != 0,
                                 problem == EBH_NEUMANN,
                                 problem == EBH_NEUMANN || s->o.dirichlet_p1 != 0,
-                                s->o.no_precond == 0, padded_p1_row(np, ne, elems));
+                                s->o.no_precond == 0, padded_p1_row(np, ne, elems),
+                                (double)s->o.partial_limit);
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -168,6 +168,18 @@ int ebh_op_get_schedule(const ebh_op *op, int *schedule, int *split);
  * reference. */
 int ebh_op_set_reduction(ebh_op *op, int fused);
 int ebh_op_get_reduction(const ebh_op *op, int *fused);
+/* Bound on the symmetric schedule's partial sums (bytes; 0 restores the
+ * default, 4 GiB).  In one pass they take ceil(ne/64) x split x (outputs
+ * per triangle) x ne FP64 values per right-hand side, quadratic in the
+ * mesh; above the bound the schedule runs in passes of block rows (4
+ * groups of 64 target triangles each), whose partial buffers stay within
+ * it (one block row at least), each pass followed by a reduction into 8
+ * slice sums per output (O(ne) memory).  y is bitwise that of one pass.
+ * get: the bound, the passes of the current schedule (0 for the chunked
+ * schedule) and the bytes of the partial buffer.  No counterpart in the
+ * reference. */
+int ebh_op_set_partial_limit(ebh_op *op, int64_t bytes);
+int ebh_op_get_partial(const ebh_op *op, int64_t *limit, int *passes, int64_t *bytes);
 /* Averages over the last `ebh_op_time` calls: milliseconds per apply of
  * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
  * measured with HIP events on the op's stream over `reps` applies of the
@@ -468,6 +480,9 @@ typedef struct {
   int no_graphs;       /* 1: launch the Arnoldi steps eagerly; 0 (default):
                           capture each step of a resident mesh's solver as a
                           hipGraph once and replay it in later solves        */
+  size_t partial_limit; /* far-field partial sums per operator, bytes
+                          (ebh_op_set_partial_limit; 0 = 512 MiB): larger
+                          meshes run the symmetric schedule in passes       */
 } ebh_server_opts;
 
 typedef struct {
