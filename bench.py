@@ -189,7 +189,8 @@ def secondary(L, meshes, solve_batched=False):
         tf = 30 * op.work()["far_interactions"] / tm["far"] / 1e9
         r3[prec] = {"matvec_ms": tm["total"], "far_tflops": tf,
                     "far_frac": tf / (PEAK_FP32_TFLOPS if prec == "fp32"
-                                      else PEAK_FP64_TFLOPS)}
+                                      else PEAK_FP64_TFLOPS),
+                    "schedule": op.schedule, "partial_passes": op.partial["passes"]}
     out["cfg3_torus_p1_kelvin"] = r3
     # config 4: icosphere L6, P0 V, 64 right-hand sides in one launch
     P, T = meshes.icosphere(6)

@@ -630,9 +630,11 @@ struct SymLayout {
   static constexpr int GW = ROT ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
   static constexpr int LDS = 64 * GW;
 };
-// the layout of sym_block<R, NT, NR, DK>
+// the layout of sym_block<R, NT, NR, DK> (DK 1, 2: double layers with the
+// normal; 0 and 3 (Kelvin) in FP32: the centroid remainder)
 template <typename R, int NT, int NR, int DK>
-using SymLay = SymLayout<NT, NR, DK != 0, sizeof(R) == 4 && DK == 0 && NT != 1>;
+using SymLay = SymLayout<NT, NR, DK == 1 || DK == 2,
+                         sizeof(R) == 4 && (DK == 0 || DK == 3) && NT != 1>;
 
 // Workgroup bid of the symmetric schedule (see above); lds holds the source
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
@@ -652,8 +654,9 @@ template <typename R, int NT, int NR, int DK = 0>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                                           R *__restrict__ lds)
 {
-  constexpr bool DL = DK != 0, ADJ = DK == 2;
+  constexpr bool DL = DK == 1 || DK == 2, ADJ = DK == 2, KEL = DK == 3;
   static_assert(!DL || NR == 1, "double layer: one right-hand side");
+  static_assert(!KEL || NR == 3, "Kelvin: three components");
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
   using Lay = SymLay<R, NT, NR, DK>;
@@ -784,6 +787,22 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     Niz = (R)a.tn[3 * (size_t)tt + 2];
   }
   const R hps = R(1) / ps;
+  // Kelvin: target point q = C + o_q with o_q = (c_q / 2) F_k (doubled in
+  // FP64, as C), and kappa times its strengths (the source side's
+  // kappa-diagonal term)
+  R oq[KEL ? NT : 1][3], ksq[KEL ? NT : 1][3];
+  if constexpr (KEL) {
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      const int k = q < C0 ? 0 : (q - C0) % 3;
+      const R h = q < C0 ? R(0) : R(0.5) * rc[q];
+      oq[q][0] = h * (k == 0 ? F0x : k == 1 ? F1x : -(F0x + F1x));
+      oq[q][1] = h * (k == 0 ? F0y : k == 1 ? F1y : -(F0y + F1y));
+      oq[q][2] = h * (k == 0 ? F0z : k == 1 ? F1z : -(F0z + F1z));
+#pragma unroll
+      for (int r = 0; r < 3; r++) ksq[q][r] = a.kappa * sq[q][r];
+    }
+  }
   R acc[NT][NR], S[NT][NR];
   // FP64 sums of the flushed steps (FL); the source side's stay aligned
   // with S: after a flush at step k, lane L holds source (L + k) mod 64 in both
@@ -867,6 +886,35 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     }
   };
 
+  // Kelvin, both directions of target point q and source point b with
+  // difference d (doubled in FP64) and r2 = |d|^2 (+ the exclusion): the
+  // kernel kappa I / r + d d^T / r^3 is even in d, so the source side takes
+  // the target's strengths through the same 1/r and d; each side is
+  // ri (kappa sigma + d (d.sigma) / r^2), the chunked kernel's form
+  auto kelvin_pair = [&](R dx, R dy, R dz, R r2, const R *sb, const R *kb, int q, int b,
+                         auto sym_tag) {
+    constexpr bool SYM = decltype(sym_tag)::value;
+    R ri;
+    if constexpr (F64) {
+      const R h = __builtin_amdgcn_rsq(r2);
+      ri = h * fma(-r2, h * h, R(3));
+    } else {
+      ri = rsq(r2);
+    }
+    // 1/r^2 divided by the position scale squared
+    const R ri2 = F64 ? R(0.25) * (ri * ri) : ri * ri;
+    const R ds = fma(dx, sb[0], fma(dy, sb[1], dz * sb[2])) * ri2;
+    acc[q][0] = fma(ri, fma(dx, ds, kb[0]), acc[q][0]);
+    acc[q][1] = fma(ri, fma(dy, ds, kb[1]), acc[q][1]);
+    acc[q][2] = fma(ri, fma(dz, ds, kb[2]), acc[q][2]);
+    if constexpr (SYM) {
+      const R us = fma(dx, sq[q][0], fma(dy, sq[q][1], dz * sq[q][2])) * ri2;
+      S[b][0] = fma(ri, fma(dx, us, ksq[q][0]), S[b][0]);
+      S[b][1] = fma(ri, fma(dy, us, ksq[q][1]), S[b][1]);
+      S[b][2] = fma(ri, fma(dz, us, ksq[q][2]), S[b][2]);
+    }
+  };
+
   auto steps = [&](auto sym_tag) {
     constexpr bool SYM = decltype(sym_tag)::value;
     const int from = ((L + 1) & 63) << 2;  // ds_bpermute byte address
@@ -882,7 +930,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         const R Dx = Cx - cur.v[0], Dy = Cy - cur.v[1], Dz = Cz - cur.v[2];
         R r2[1];
         r2[0] = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
-        finish_point(r2, cur.v + 3, 0, sym_tag, nullptr, R(0), nullptr);
+        if constexpr (KEL) {
+          const R kb[3] = {a.kappa * cur.v[3], a.kappa * cur.v[4], a.kappa * cur.v[5]};
+          kelvin_pair(Dx, Dy, Dz, r2[0], cur.v + 3, kb, 0, 0, sym_tag);
+        } else {
+          finish_point(r2, cur.v + 3, 0, sym_tag, nullptr, R(0), nullptr);
+        }
 #pragma unroll
         for (int c = 0; c < 3 + NR; c++) cur.v[c] = lane_rol1(cur.v[c]);
       } else {
@@ -893,6 +946,28 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
           Dy += Cly - g[Lay::LO + 1];
           Dz += Clz - g[Lay::LO + 2];
         }
+        if constexpr (KEL) {
+          // d = D - o'_b + o_q with o'_b = (c_b / 2) F'_l of the source
+#pragma unroll
+          for (int b = 0; b < NT; b++) {
+            R Dbx = Dx, Dby = Dy, Dbz = Dz;
+            if (b >= C0) {
+              const int lb = (b - C0) % 3;
+              const R hb = R(-0.5) * rc[b];
+              Dbx = fma(hb, lb == 0 ? g[3] : lb == 1 ? g[6] : -(g[3] + g[6]), Dx);
+              Dby = fma(hb, lb == 0 ? g[4] : lb == 1 ? g[7] : -(g[4] + g[7]), Dy);
+              Dbz = fma(hb, lb == 0 ? g[5] : lb == 1 ? g[8] : -(g[5] + g[8]), Dz);
+            }
+            const R *sb = g + Lay::SIG + b * NR;
+            const R kb[3] = {a.kappa * sb[0], a.kappa * sb[1], a.kappa * sb[2]};
+#pragma unroll
+            for (int q = 0; q < NT; q++) {
+              const R dx = Dbx + oq[q][0], dy = Dby + oq[q][1], dz = Dbz + oq[q][2];
+              kelvin_pair(dx, dy, dz, fma(dx, dx, fma(dy, dy, fma(dz, dz, excl))), sb, kb,
+                          q, b, sym_tag);
+            }
+          }
+        } else {
         const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, excl)));
         // D.F_k (target directions), D.F'_l (source directions), F_k.F'_l
         R U[3], P[3], W[3][3];
@@ -956,6 +1031,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
           for (int q = 0; q < NT; q++) acc[q][0] = fma(tf[q], it[q], acc[q][0]);
         }
+        }  // not KEL
       }
       if constexpr (SYM) {
 #pragma unroll
@@ -1003,21 +1079,24 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     double wq[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
+    double *row = base + (size_t)(c - a.c0) * NR * rowv;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      double *out = base + ((size_t)(c - a.c0) * NR + r) * rowv;
       if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
-        out[tl] = s;
+        row[(size_t)r * rowv + tl] = s;
       } else {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], v[q][r], s);
-          out[a.pos ? a.pos[k * ne + tri] : k * cols + tl] = s;
+          // planes: right-hand side r, then output k; Kelvin: output k, then
+          // component r (the chunked kernel's order)
+          if constexpr (KEL) row[((size_t)k * NR + r) * cols + tl] = s;
+          else row[(size_t)r * rowv + (a.pos ? a.pos[k * ne + tri] : k * cols + tl)] = s;
         }
       }
     }
@@ -1038,6 +1117,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 {
   __shared__ __align__(16) R lds[SymLay<R, NT, NR, DK>::LDS];
   sym_block<R, NT, NR, DK>(a, (int)blockIdx.x, lds);
+}
+
+// The Kelvin single layer (three components) on the symmetric schedule.
+// FP64 holds 7 x 3 target sums, strengths, kappa-scaled strengths and point
+// offsets plus the source side's sums in registers; with 6 or 7 points that
+// is more than 256 per lane (one wave per SIMD, spills at 7), so FP64 takes
+// it for rules of at most 4 points (sym_eligible) and the chunked kernel
+// otherwise.
+template <typename R, int NT>
+__global__ __launch_bounds__(256) void k_far_sym_kel(FarArgs<R> a)
+{
+  __shared__ __align__(16) R lds[SymLay<R, NT, 3, 3>::LDS];
+  sym_block<R, NT, 3, 3>(a, (int)blockIdx.x, lds);
 }
 
 // Persistent multi-mesh form of the symmetric schedule: the workgroups of
@@ -1871,7 +1963,33 @@ template <typename R>
 int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
 {
   const int ks = op->ksplit, ng = op->chunks / ks;
+  if (op->d.kernel == EBH_KELVIN_SL) nr = 3;  // the three components
   auto launch = [&](dim3 grid, const FarArgs<R> &b) -> int {
+    if (op->d.kernel == EBH_KELVIN_SL) {
+      switch (op->rule.nt) {
+      case 1: hipLaunchKernelGGL((k_far_sym_kel<R, 1>), grid, dim3(256), 0, s, b); break;
+      case 3: hipLaunchKernelGGL((k_far_sym_kel<R, 3>), grid, dim3(256), 0, s, b); break;
+      case 4: hipLaunchKernelGGL((k_far_sym_kel<R, 4>), grid, dim3(256), 0, s, b); break;
+      case 6:
+        if constexpr (sizeof(R) == 4) {
+          hipLaunchKernelGGL((k_far_sym_kel<R, 6>), grid, dim3(256), 0, s, b);
+          break;
+        }
+        [[fallthrough]];
+      case 7:
+        if constexpr (sizeof(R) == 4) {
+          hipLaunchKernelGGL((k_far_sym_kel<R, 7>), grid, dim3(256), 0, s, b);
+          break;
+        }
+        [[fallthrough]];
+      default:
+        set_error("no pair-symmetric Kelvin kernel for rule size " +
+                  std::to_string(op->rule.nt));
+        return EBH_ERR_UNSUPPORTED;
+      }
+      EBH_HIP(hipGetLastError());
+      return EBH_OK;
+    }
     if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) {
       // the symmetric double layers exist in FP64 only (sym_eligible)
       if constexpr (sizeof(R) == 8)
@@ -2233,7 +2351,12 @@ bool sym_eligible(const ebh_op *op)
   const bool sl = op->d.kernel == EBH_LAPLACE_SL && op->d.test_basis == op->d.trial_basis;
   const bool dl = (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) &&
                   op->d.far_precision == EBH_FP64 && op->nr_max == 1;
-  return (sl || dl) && op->ray && op->rule.nt <= 7 && op->nc == 1;
+  // the Kelvin single layer with equal bases (its three components are the
+  // launch's three right-hand-side planes)
+  const bool kel = op->d.kernel == EBH_KELVIN_SL && op->d.test_basis == op->d.trial_basis &&
+                   op->nr_max == 1 &&
+                   (op->d.far_precision == EBH_FP32 || op->rule.nt <= 4);
+  return (sl || dl || kel) && op->ray && op->rule.nt <= 7;
 }
 
 // Passes of the pair-symmetric schedule for partial sums above the
@@ -2274,9 +2397,9 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   const int ne = op->mesh->T.ne;
   if (sym) {
     if (!sym_eligible(op)) {
-      set_error("the pair-symmetric schedule needs a Laplace single layer with equal "
-                "test and trial bases or an FP64 double layer and a median-symmetric "
-                "far rule");
+      set_error("the pair-symmetric schedule needs a Laplace or Kelvin single layer with "
+                "equal test and trial bases or an FP64 Laplace double layer, and a "
+                "median-symmetric far rule");
       return EBH_ERR_UNSUPPORTED;
     }
     // groups of 64 triangles; chunks (when given) is the number of step
@@ -2417,11 +2540,11 @@ int op_build_far(ebh_op *op)
   }
   if (reg_rule(nt) &&
       (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL ||
-       (op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64 && op->ray &&
-        nt <= 7))) {
+       ((op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64) ||
+        op->d.kernel == EBH_KELVIN_SL) && op->ray && nt <= 7)) {
     // single-layer target frame: centroid, then the median form (ray rules)
     // or the offsets of every point from the centroid; the FP64 double layer
-    // uses it on the symmetric schedule
+    // and the Kelvin single layer use it on the symmetric schedule
     const bool ray = op->ray && nt <= 7;
     // FP32 median rules: three more rows, the rounding remainders of the
     // centroid, C - (float)C (the symmetric kernel's hi/lo differences)

@@ -2,7 +2,10 @@
 the library's autotuner (``ebh_op_autotune``) on the operators of the
 standard workloads, on the GPU this runs on:
 
-    python -m easybem_amd.tune_db [--out PATH] [--budget-ms MS]
+    python -m easybem_amd.tune_db [--out PATH] [--budget-ms MS] [--only NAME]
+
+--only retunes the workloads whose name contains NAME and keeps the other
+entries of the packaged table.
 
 Workloads: icosphere levels 3-6 (single layer P0 and P1 with the 7-point
 and the 1-point far rule -- the latter is the Calderon preconditioners' --,
@@ -43,11 +46,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=L.TUNE_FILE)
     ap.add_argument("--budget-ms", type=float, default=4000.0)
+    ap.add_argument("--only", default="", help="retune only these workloads")
     args = ap.parse_args()
-    os.environ["EBH_TUNE_FILE"] = ""  # start from default shapes
+    if not args.only:
+        os.environ["EBH_TUNE_FILE"] = ""  # start from default shapes
     L.lib()
-    L.tune_clear()
+    if not args.only:
+        L.tune_clear()
     for name, P, T, ops in workloads():
+        if args.only not in name:
+            continue
         m = L.Mesh(P, T)
         for kern, test, trial, prec, nrhs, rule in ops:
             t0 = time.time()

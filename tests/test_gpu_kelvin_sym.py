@@ -1,0 +1,106 @@
+"""The Kelvin single layer on the pair-symmetric schedule.
+
+The Kelvin kernel kappa I / r + d d^T / r^3 (src/BEM3dElasticity.cpp:289-348
+and 838-846 for its blocks) is even in d = x - y, so one rsqrt and one d per
+point pair serve both directions.  The symmetric kernel
+(k_far_sym_kel) is the default for equal bases in FP32 with every median
+rule and in FP64 with rules of at most 4 points.  Checked against the
+chunked kernel (FP64 <= 1e-13, FP32 <= 2e-6) and against oracle rows
+(FP64 <= 1e-12, FP32 <= 1e-6) for P0 and P1, with passes (bitwise equal to
+one pass), and on the benchmark's 256 x 128 torus with P1 functions.
+"""
+import numpy as np
+import pytest
+
+from easybem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.3
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from easybem_amd import _lib
+    return _lib
+
+
+MESHES = {
+    "torus12x6": lambda: meshes.torus(1.0, 0.4, 12, 6),
+    "ico2": lambda: meshes.icosphere(2),
+    "blob3": lambda: meshes.perturbed_icosphere(3, 7)[:2],
+}
+CASES = [("fp64", r) for r in (0, 1, 2)] + [("fp32", r) for r in (0, 1, 2, 3, 4)]
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("prec,rule", CASES)
+def test_kelvin_symmetric_matches_chunked_and_oracle(L, oracle, mesh, basis, prec, rule):
+    P, T = MESHES[mesh]()
+    m = L.Mesh(P, T)
+    print(f"case {mesh} basis {basis} {prec} rule {rule}", flush=True)
+    op = L.Operator(m, "kelvin", basis, basis, prec, line_rule=4, tri_rule=rule, nu=NU)
+    assert op.schedule == "symmetric"
+    n = P.shape[0] if basis else T.shape[0]
+    x = np.random.default_rng(rule + 10 * basis).uniform(-1, 1, 3 * n)
+    y = op(x)
+    op.set_schedule("chunked")
+    yc = op(x)
+    e_c = rel(y, yc)
+    rows = np.sort(np.random.default_rng(2).choice(n, 16, replace=False)).astype(np.int32)
+    yo = np.asarray(oracle.apply("kelvin", basis, basis, P, T, x, rows=rows, line_idx=4,
+                                 tri_idx=rule, nu=NU))
+    yr = np.stack([y[c * n + rows] for c in range(3)])
+    e_o = rel(yr, yo)
+    print(f"  symmetric vs chunked {e_c:.2e}, vs oracle rows {e_o:.2e}")
+    assert e_c <= (1e-13 if prec == "fp64" else 2e-6)
+    assert e_o <= (1e-12 if prec == "fp64" else 1e-6)
+
+
+def test_kelvin_fp64_large_rules_stay_chunked(L):
+    P, T = meshes.icosphere(2)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "kelvin", 1, 1, "fp64", line_rule=4, tri_rule=4, nu=NU)
+    assert op.schedule == "chunked"
+    with pytest.raises(L.EbhError):
+        op.set_schedule("symmetric")
+
+
+@pytest.mark.parametrize("prec,rule", [("fp32", 4), ("fp64", 2)])
+def test_kelvin_passes_bitwise_equal_one_pass(L, prec, rule):
+    P, T = meshes.torus(1.0, 0.4, 40, 20)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "kelvin", 1, 1, prec, line_rule=4, tri_rule=rule, nu=NU)
+    x = np.random.default_rng(5).uniform(-1, 1, 3 * P.shape[0])
+    y1 = op(x)
+    op.set_partial_limit(2_000_000)
+    assert op.partial["passes"] > 1
+    yp = op(x)
+    assert np.array_equal(y1, yp)
+
+
+def test_kelvin_config3_torus(L, oracle):
+    """The benchmark's Kelvin workload: 256 x 128 torus (65536 triangles),
+    P1, FP32 7-point far rule: symmetric against chunked and oracle rows."""
+    P, T = meshes.torus(1.0, 0.4, 256, 128)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "kelvin", 1, 1, "fp32", line_rule=4, tri_rule=4, nu=NU)
+    assert op.schedule == "symmetric"
+    n = P.shape[0]
+    x = np.random.default_rng(3).uniform(-1, 1, 3 * n)
+    print(f"case config 3 torus, partial {op.partial}", flush=True)
+    y = op(x)
+    op.set_schedule("chunked")
+    yc = op(x)
+    rows = np.arange(0, n, 4099, dtype=np.int32)
+    yo = np.asarray(oracle.apply("kelvin", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4,
+                                 nu=NU))
+    yr = np.stack([y[c * n + rows] for c in range(3)])
+    e_c, e_o = rel(y, yc), rel(yr, yo)
+    print(f"config 3: symmetric vs chunked {e_c:.2e}, vs oracle rows {e_o:.2e}")
+    assert e_c <= 2e-6 and e_o <= 1e-6
