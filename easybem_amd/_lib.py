@@ -71,6 +71,8 @@ def lib():
         L.ebh_op_set_partial_limit.argtypes = [vp, C.c_int64]
         L.ebh_op_get_partial.argtypes = [vp, i64p, C.POINTER(C.c_int), i64p]
         L.ebh_op_get_reduction.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ebh_op_set_mrhs_schedule.argtypes = [vp, C.c_int]
+        L.ebh_op_get_mrhs_schedule.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
         L.ebh_op_work.argtypes = [vp, i64p]
@@ -263,6 +265,22 @@ class Operator:
         npass = C.c_int()
         check(lib().ebh_op_get_partial(self.h, C.byref(lim), C.byref(npass), C.byref(nb)))
         return {"limit": lim.value, "passes": npass.value, "bytes": nb.value}
+
+    MRHS_SCHEDULES = {"auto": 0, "target": 1, "pairs": 2}
+
+    def set_mrhs_schedule(self, schedule="auto"):
+        """Multi-RHS schedule of a P0 single layer: "target" (every ordered
+        triangle pair), "pairs" (each unordered pair's point sums once, in
+        64 x 64 tiles applied in both directions) or "auto" (pairs where it
+        applies)."""
+        check(lib().ebh_op_set_mrhs_schedule(self.h, self.MRHS_SCHEDULES[schedule]))
+
+    @property
+    def mrhs_schedule(self):
+        """(schedule in effect, its passes) of multi-RHS applies."""
+        v, k = C.c_int(), C.c_int()
+        check(lib().ebh_op_get_mrhs_schedule(self.h, C.byref(v), C.byref(k)))
+        return {1: "target", 2: "pairs"}[v.value], k.value
 
     @property
     def fused(self):

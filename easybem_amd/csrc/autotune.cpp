@@ -249,7 +249,8 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
     spent += mbest * reps;
     tried++;
     for (int g : grids) {
-      if (spent >= budget_ms) break;
+      // (the pair-symmetric multi-RHS schedule has no workgroup count to tune)
+      if (spent >= budget_ms || op->mts) break;
       if (g == bm) continue;
       op->mgrid = g;
       EBH_TRY(op_configure(op, bb, bt, bs ? bs : op->chunks, bs > 0));

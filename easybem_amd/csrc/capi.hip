@@ -57,7 +57,8 @@ size_t ebh_op::bytes() const
          msig64.bytes() + src3_32.bytes() + src3_64.bytes() + tsl32.bytes() +
          tsl64.bytes() + tc32.bytes() + dinv.bytes() + mass_ptr.bytes() +
          mass_col.bytes() + mass_val.bytes() + cheb_r.bytes() + cheb_d.bytes() +
-         cheb_t.bytes() + chebx.bytes();
+         cheb_t.bytes() + chebx.bytes() + mt_rowoff.bytes() + mt_item.bytes() +
+         mt_kptr.bytes() + mt_tile32.bytes() + mt_tile64.bytes() + mt_part.bytes();
 }
 
 extern "C" {
@@ -449,7 +450,38 @@ int ebh_op_set_partial_limit(ebh_op *op, int64_t bytes)
   ebh_op *f = far_op(op);
   f->partial_limit = bytes ? (double)bytes : 4.0 * 1024 * 1024 * 1024;
   if (f->sym) EBH_TRY(op_configure(f, 0, 0, f->ksplit, true));
+  else if (f->mts) EBH_TRY(op_configure(f, f->block, f->tile, f->chunks));
   if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_set_mrhs_schedule(ebh_op *op, int schedule)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (schedule != EBH_MRHS_AUTO && schedule != EBH_MRHS_TARGET && schedule != EBH_MRHS_PAIRS) {
+    set_error("schedule must be EBH_MRHS_AUTO, EBH_MRHS_TARGET or EBH_MRHS_PAIRS");
+    return EBH_ERR_ARG;
+  }
+  if (no_far(op)) return EBH_ERR_UNSUPPORTED;
+  ebh_op *f = far_op(op);
+  if (schedule == EBH_MRHS_PAIRS && !mts_eligible(f)) {
+    set_error("the pair-symmetric multi-RHS schedule needs a P0 single layer with "
+              "max_rhs > 1 and a far rule of at most 7 points");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  f->mrhs_sched = schedule;
+  if (f->rb) EBH_TRY(op_configure(f, f->block, f->tile, f->sym ? f->ksplit : f->chunks, f->sym));
+  if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_get_mrhs_schedule(const ebh_op *op, int *schedule, int *passes)
+{
+  if (!op || !schedule) { set_error("null argument"); return EBH_ERR_ARG; }
+  const ebh_op *f = far_op(op);
+  *schedule = f->mts ? EBH_MRHS_PAIRS : EBH_MRHS_TARGET;
+  if (passes) *passes = f->mts ? (int)f->mt_rows.size() - 1 : 0;
   return EBH_OK;
 }
 

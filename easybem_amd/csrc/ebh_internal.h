@@ -223,6 +223,19 @@ struct ebh_op {
   int mgrid = 4096;  // multi-RHS launch: target workgroup count
   ebh::DevBuf<float> mgeom32, msig32;
   ebh::DevBuf<double> mgeom64, msig64;
+  // pair-symmetric multi-RHS P0 schedule (k_mrhs_tiles, k_mrhs_tile_mfma):
+  // requested (EBH_MRHS_*) and in effect; tile rows mt_rows[p] ..
+  // mt_rows[p+1]-1 form pass p, whose items are mt_ioff[p] .. mt_ioff[p+1]-1
+  // of mt_item, with per-group item pointers at mt_koff[p] of mt_kptr;
+  // the first tile of pass p is mt_t0[p]; mt_tiles / mt_items: the largest
+  // pass's tiles and items
+  int mrhs_sched = 0;
+  bool mts = false;
+  std::vector<int> mt_rows, mt_t0, mt_ioff, mt_koff;
+  size_t mt_tiles = 0, mt_items = 0;
+  ebh::DevBuf<int> mt_rowoff, mt_item, mt_kptr;
+  ebh::DevBuf<float> mt_tile32;
+  ebh::DevBuf<double> mt_tile64, mt_part;
   // hypersingular (EBH_LAPLACE_HS): the wrapped P0 single layer, the curls
   // [ne][3][3], lumped node areas, curl densities / single-layer outputs
   // [nrhs][3][ne] and the partial m.x sums
@@ -291,6 +304,9 @@ int op_build_far(ebh_op *op);
 // the mesh); only for operators that sym_eligible accepts
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
+// the pair-symmetric multi-RHS schedule applies (a P0 single layer with
+// max_rhs > 1 and a rule of at most 7 points)
+bool mts_eligible(const ebh_op *op);
 // y = M x for an EBH_SPARSE_INVERSE operator (nrhs vectors back to back)
 int spai_apply(ebh_op *op, const double *x, double *y, int nrhs, hipStream_t s, bool timed);
 // z = Q v with Q the Chebyshev approximation of the P1 mass inverse of a

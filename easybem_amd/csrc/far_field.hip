@@ -1277,6 +1277,112 @@ __global__ __launch_bounds__(256) void k_far_mrhs(FarMArgs<R> a)
 // ([ne][MedRec]) instead of its points.
 template <int NT> struct MedRec { static constexpr int W = (9 + 2 * NT + 3) / 4 * 4; };
 
+// A target triangle of the triangle-pair kernels: its points (MED: its
+// frame and the E_q, c_q of its points) and weights in registers.  sum(a, j,
+// keep) is G of the pair with source triangle j, 0 when keep is 0.
+template <typename R, int NT, bool MED>
+struct TriTarget {
+  R X[NT], Y[NT], Z[NT], W[NT];
+  R Cx = 0, Cy = 0, Cz = 0, F0x = 0, F0y = 0, F0z = 0, F1x = 0, F1y = 0, F1z = 0;
+
+  __device__ __forceinline__ void load(const FarMArgs<R> &a, int tt)
+  {
+    const int ne = a.ne;
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      if constexpr (MED) {
+        X[q] = a.tsl[(9 + q) * (size_t)ne + tt];  // E_q
+        Y[q] = a.rc[q];
+        Z[q] = R(0);
+      } else {
+        X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
+        Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
+        Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
+      }
+      W[q] = (R)a.tw[(size_t)q * ne + tt];
+    }
+    if constexpr (MED) {
+      Cx = a.tsl[0 * (size_t)ne + tt];
+      Cy = a.tsl[1 * (size_t)ne + tt];
+      Cz = a.tsl[2 * (size_t)ne + tt];
+      F0x = a.tsl[3 * (size_t)ne + tt];
+      F0y = a.tsl[4 * (size_t)ne + tt];
+      F0z = a.tsl[5 * (size_t)ne + tt];
+      F1x = a.tsl[6 * (size_t)ne + tt];
+      F1y = a.tsl[7 * (size_t)ne + tt];
+      F1z = a.tsl[8 * (size_t)ne + tt];
+    }
+  }
+
+  __device__ __forceinline__ R sum(const FarMArgs<R> &a, int j, R keep) const
+  {
+    const R eps = Eps<R>::v;
+    if constexpr (MED) {
+      constexpr int C0 = NT % 3;
+      const R *gp = a.geom + (size_t)j * MedRec<NT>::W;
+      const R Dx = Cx - gp[0], Dy = Cy - gp[1], Dz = Cz - gp[2];
+      // a touching pair's expanded r^2 can round below zero at coincident
+      // points: lift it far away before the rsqrt (and multiply by 0 after)
+      const R excl = keep == R(0) ? R(1e30) : R(0);
+      const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, eps + excl)));
+      R U[3], P[3], Wk[3][3];
+      U[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
+      U[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
+      U[2] = -(U[0] + U[1]);
+      P[0] = fma(Dx, gp[3], fma(Dy, gp[4], Dz * gp[5]));
+      P[1] = fma(Dx, gp[6], fma(Dy, gp[7], Dz * gp[8]));
+      P[2] = -(P[0] + P[1]);
+      Wk[0][0] = fma(F0x, gp[3], fma(F0y, gp[4], F0z * gp[5]));
+      Wk[0][1] = fma(F0x, gp[6], fma(F0y, gp[7], F0z * gp[8]));
+      Wk[1][0] = fma(F1x, gp[3], fma(F1y, gp[4], F1z * gp[5]));
+      Wk[1][1] = fma(F1x, gp[6], fma(F1y, gp[7], F1z * gp[8]));
+      Wk[0][2] = -(Wk[0][0] + Wk[0][1]);
+      Wk[1][2] = -(Wk[1][0] + Wk[1][1]);
+#pragma unroll
+      for (int l = 0; l < 3; l++) Wk[2][l] = -(Wk[0][l] + Wk[1][l]);
+      R B[NT], Gq[NT];
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        B[q] = q < C0 ? R(0) : fma(Y[q], U[(q - C0) % 3], X[q]);
+        Gq[q] = R(0);
+      }
+#pragma unroll
+      for (int b = 0; b < NT; b++) {
+        const int lb = (b - C0) % 3, ob = (b - C0) / 3;
+        const R Ab = b < C0 ? A0 : fma(-a.rc[b], P[lb], A0 + gp[9 + b]);
+        const R wb = gp[9 + NT + b];
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          const int kq = (q - C0) % 3, oq = (q - C0) / 3;
+          const R r2 = q < C0   ? Ab
+                       : b < C0 ? Ab + B[q]
+                                : fma(a.kap[2 * oq + ob], Wk[kq][lb], Ab + B[q]);
+          Gq[q] = fma(wb, rsq(r2), Gq[q]);
+        }
+      }
+      R G = R(0);
+#pragma unroll
+      for (int q = 0; q < NT; q++) G = fma(W[q], Gq[q], G);
+      return G * keep;
+    }
+    const R *gp = a.geom + (size_t)j * NT * 4;
+    R G = R(0);
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      R Gq = R(0);
+#pragma unroll
+      for (int bq = 0; bq < NT; bq++) {
+        const R dx = X[q] - gp[4 * bq], dy = Y[q] - gp[4 * bq + 1],
+                dz = Z[q] - gp[4 * bq + 2];
+        const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
+        Gq = fma(gp[4 * bq + 3], rsq(r2), Gq);
+      }
+      G = fma(W[q], Gq, G);
+    }
+    return G * keep;
+  }
+};
+
 template <typename R, int NT, int RB, bool MED>
 __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
 {
@@ -1289,32 +1395,8 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
   const int t = tb * 256 + (int)threadIdx.x;
   const int tt = t < ne ? t : ne - 1;
   const int j0 = s * a.chunk, j1 = min(ne, j0 + a.chunk);
-  R X[NT], Y[NT], Z[NT], W[NT];
-  R Cx = 0, Cy = 0, Cz = 0, F0x = 0, F0y = 0, F0z = 0, F1x = 0, F1y = 0, F1z = 0;
-#pragma unroll
-  for (int q = 0; q < NT; q++) {
-    if constexpr (MED) {
-      X[q] = a.tsl[(9 + q) * (size_t)ne + tt];  // E_q
-      Y[q] = a.rc[q];
-      Z[q] = R(0);
-    } else {
-      X[q] = a.tq[(3 * q + 0) * (size_t)ne + tt];
-      Y[q] = a.tq[(3 * q + 1) * (size_t)ne + tt];
-      Z[q] = a.tq[(3 * q + 2) * (size_t)ne + tt];
-    }
-    W[q] = (R)a.tw[(size_t)q * ne + tt];
-  }
-  if constexpr (MED) {
-    Cx = a.tsl[0 * (size_t)ne + tt];
-    Cy = a.tsl[1 * (size_t)ne + tt];
-    Cz = a.tsl[2 * (size_t)ne + tt];
-    F0x = a.tsl[3 * (size_t)ne + tt];
-    F0y = a.tsl[4 * (size_t)ne + tt];
-    F0z = a.tsl[5 * (size_t)ne + tt];
-    F1x = a.tsl[6 * (size_t)ne + tt];
-    F1y = a.tsl[7 * (size_t)ne + tt];
-    F1z = a.tsl[8 * (size_t)ne + tt];
-  }
+  TriTarget<R, NT, MED> tgt;
+  tgt.load(a, tt);
   double acc64[RB];
 #pragma unroll
   for (int r = 0; r < RB; r++) acc64[r] = 0.0;
@@ -1322,8 +1404,6 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
   const int cend = a.touch_ptr[tt + 1];
   while (cur < cend && a.touch[cur] < j0) cur++;
   int nextx = cur < cend ? a.touch[cur] : INT_MAX;
-  const R eps = Eps<R>::v;
-  const R *__restrict__ geom = a.geom;
   const double *__restrict__ sig = a.sigd + (size_t)g * ne * RB;
   for (int js = j0; js < j1; js += 64) {
     const int nt = min(64, j1 - js);
@@ -1333,75 +1413,10 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
       ++cur;
       nextx = cur < cend ? a.touch[cur] : INT_MAX;
     }
-    // G of the pair (t, js + jj), 0 for an excluded (touching) pair
-    auto pair_sum = [&](int jj) -> R {
-      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
-      if constexpr (MED) {
-        constexpr int C0 = NT % 3;
-        const R *gp = geom + (size_t)(js + jj) * MedRec<NT>::W;
-        const R Dx = Cx - gp[0], Dy = Cy - gp[1], Dz = Cz - gp[2];
-        // a touching pair's expanded r^2 can round below zero at coincident
-        // points: lift it far away before the rsqrt (and multiply by 0 after)
-        const R excl = keep == R(0) ? R(1e30) : R(0);
-        const R A0 = fma(Dx, Dx, fma(Dy, Dy, fma(Dz, Dz, eps + excl)));
-        R U[3], P[3], Wk[3][3];
-        U[0] = fma(Dx, F0x, fma(Dy, F0y, Dz * F0z));
-        U[1] = fma(Dx, F1x, fma(Dy, F1y, Dz * F1z));
-        U[2] = -(U[0] + U[1]);
-        P[0] = fma(Dx, gp[3], fma(Dy, gp[4], Dz * gp[5]));
-        P[1] = fma(Dx, gp[6], fma(Dy, gp[7], Dz * gp[8]));
-        P[2] = -(P[0] + P[1]);
-        Wk[0][0] = fma(F0x, gp[3], fma(F0y, gp[4], F0z * gp[5]));
-        Wk[0][1] = fma(F0x, gp[6], fma(F0y, gp[7], F0z * gp[8]));
-        Wk[1][0] = fma(F1x, gp[3], fma(F1y, gp[4], F1z * gp[5]));
-        Wk[1][1] = fma(F1x, gp[6], fma(F1y, gp[7], F1z * gp[8]));
-        Wk[0][2] = -(Wk[0][0] + Wk[0][1]);
-        Wk[1][2] = -(Wk[1][0] + Wk[1][1]);
-#pragma unroll
-        for (int l = 0; l < 3; l++) Wk[2][l] = -(Wk[0][l] + Wk[1][l]);
-        R B[NT], Gq[NT];
-#pragma unroll
-        for (int q = 0; q < NT; q++) {
-          B[q] = q < C0 ? R(0) : fma(Y[q], U[(q - C0) % 3], X[q]);
-          Gq[q] = R(0);
-        }
-#pragma unroll
-        for (int b = 0; b < NT; b++) {
-          const int lb = (b - C0) % 3, ob = (b - C0) / 3;
-          const R Ab = b < C0 ? A0 : fma(-a.rc[b], P[lb], A0 + gp[9 + b]);
-          const R wb = gp[9 + NT + b];
-#pragma unroll
-          for (int q = 0; q < NT; q++) {
-            const int kq = (q - C0) % 3, oq = (q - C0) / 3;
-            const R r2 = q < C0   ? Ab
-                         : b < C0 ? Ab + B[q]
-                                  : fma(a.kap[2 * oq + ob], Wk[kq][lb], Ab + B[q]);
-            Gq[q] = fma(wb, rsq(r2), Gq[q]);
-          }
-        }
-        R G = R(0);
-#pragma unroll
-        for (int q = 0; q < NT; q++) G = fma(W[q], Gq[q], G);
-        return G * keep;
-      }
-      const R *gp = geom + (size_t)(js + jj) * NT * 4;
-      R G = R(0);
-#pragma unroll
-      for (int q = 0; q < NT; q++) {
-        R Gq = R(0);
-#pragma unroll
-        for (int bq = 0; bq < NT; bq++) {
-          const R dx = X[q] - gp[4 * bq], dy = Y[q] - gp[4 * bq + 1],
-                  dz = Z[q] - gp[4 * bq + 2];
-          const R r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps)));
-          Gq = fma(gp[4 * bq + 3], rsq(r2), Gq);
-        }
-        G = fma(W[q], Gq, G);
-      }
-      return G * keep;
-    };
     for (int jj = 0; jj < nt; jj++) {
-      const double G = (double)pair_sum(jj);
+      // G of the pair (t, js + jj), 0 for an excluded (touching) pair
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      const double G = (double)tgt.sum(a, js + jj, keep);
       const double *sp = sig + (size_t)(js + jj) * RB;
 #pragma unroll
       for (int r = 0; r < RB; r++) acc64[r] = fma(sp[r], G, acc64[r]);
@@ -1412,6 +1427,171 @@ __global__ __launch_bounds__(256) void k_far_mrhs_tri(FarMArgs<R> a)
     for (int r = 0; r < nrhs_here; r++)
       a.partial[((size_t)s * a.nrhs + g * RB + r) * ne + t] = acc64[r];
   }
+}
+
+// Pair-symmetric multi-RHS schedule for P0 test and trial functions.  G of
+// a triangle pair (k_far_mrhs_tri's triangle-pair sum; the kernel and the
+// point sets are symmetric) serves both orders of the pair, so it is formed
+// once per unordered pair of 64-triangle groups and applied in both
+// directions by a second kernel (k_mrhs_tile_mfma): half the point-pair
+// work of k_far_mrhs_tri, and the RB FP64 products per ordered pair on the
+// matrix cores as 64 x 64 x RB tile products.
+//
+// Tiles of a pass: the group pairs (I, J), I0 <= I < I1, J >= I, row by
+// row (tile (I, J) has global index rowoff[I] + J - I); tile (I, J) holds
+// G(64 I + i, 64 J + j) at [j][i], 0 for touching pairs and for triangles
+// past the mesh.  Items of a pass (K, kind, M0, M1), each one wave and at
+// most a fixed number of tiles, give output group K the contributions of
+// the pass's tiles: kind 1, tiles (M, K) transposed for M0 <= M < M1 (M <
+// K); kind 0, tiles (K, M) for M0 <= M < M1 (M >= K, the rows of the
+// pass).  The items of K are listed in increasing M, and
+// k_mrhs_tile_reduce adds them in that order to K's sums of the earlier
+// passes: a fixed summation order, no atomics.  The buffers hold one
+// pass's tiles and item sums, bounded by the operator's partial limit.
+struct MtArgs {
+  int ng, I0, I1, t0, ntiles, items, groups, first;
+  const int *rowoff;  // [ng + 1] first tile of each row
+  const int4 *item;   // (K, kind, M0, M1) of the pass's items
+  const int *kptr;    // [ng - I0 + 1]: items of output group I0 + k
+  double *part;       // [items][groups][RB][64] item sums
+};
+
+// One wave a tile: lane i pairs its target triangle 64 I + i with the 64
+// source triangles of group J in order (wave-uniform source records).
+template <typename R, int NT, bool MED>
+__global__ __launch_bounds__(256) void k_mrhs_tiles(FarMArgs<R> a, MtArgs m,
+                                                    R *__restrict__ tiles)
+{
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (w >= m.ntiles) return;  // wave-uniform; no barrier in this kernel
+  const int L = threadIdx.x & 63;
+  const int tg = m.t0 + w;
+  int lo = m.I0, hi = m.I1 - 1;  // the row of tile tg
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.rowoff[mid] <= tg) lo = mid;
+    else hi = mid - 1;
+  }
+  const int I = lo, J = I + (tg - m.rowoff[I]);
+  const int ne = a.ne, t = I * 64 + L, tt = t < ne ? t : ne - 1;
+  TriTarget<R, NT, MED> tgt;
+  tgt.load(a, tt);
+  const int j0 = J * 64, nj = min(64, ne - j0);
+  // touching triangles of group J (a lane past the mesh excludes them all)
+  uint64_t mask = ~0ull;
+  if (t < ne) {
+    mask = 0;
+    for (int c = a.touch_ptr[tt]; c < a.touch_ptr[tt + 1]; c++) {
+      const unsigned j = (unsigned)(a.touch[c] - j0);
+      if (j < 64u) mask |= 1ull << j;
+    }
+  }
+  R *__restrict__ out = tiles + (size_t)w * 4096 + L;
+  for (int jj = 0; jj < 64; jj++) {
+    R G = R(0);
+    if (jj < nj) {
+      const R keep = (R)(1u - (unsigned)((mask >> jj) & 1ull));
+      G = tgt.sum(a, j0 + jj, keep);
+    }
+    out[jj * 64] = G;
+  }
+}
+
+// One wave an item, on the matrix cores: per tile the item's 64 outputs
+// take Out(64 x RB) += G(64 x 64) X(64 x RB) in FP64 MFMA 16x16x4 blocks:
+// four 16-output blocks (ib) times RB / 16 right-hand-side blocks (rb), 4
+// sources a step; A = G, lane i + 16 k holding output i, source k of a 16
+// x 4 block; B = X, lane r + 16 k holding source k, right-hand side r;
+// D: output i, right-hand side r at lane r + 16 (i % 4), item i / 4
+// (measured on gfx950).  The operands of 4 steps (16 sources) are loaded
+// together, then their 64 (RB = 64) MFMAs run.  RB = 8 fills half of each
+// B block, the other lanes 0.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename R, int RB, bool ROW>
+__device__ __forceinline__ void mfma_item(const FarMArgs<R> &a, const MtArgs &m,
+                                          const R *__restrict__ tiles, int w, int4 it)
+{
+  constexpr int NRB = RB >= 16 ? RB / 16 : 1, SB = 4;  // steps a block
+  const int g = blockIdx.y, l = threadIdx.x & 63, ne = a.ne;
+  const int li = l & 15, lk = l >> 4, K = it.x;
+  const double *__restrict__ sig = a.sigd + (size_t)g * ne * RB;
+  v4d acc[4][NRB];
+#pragma unroll
+  for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+    for (int rb = 0; rb < NRB; rb++) acc[ib][rb] = v4d{0.0, 0.0, 0.0, 0.0};
+  // block b: tile M = M0 + b / 4, sources 16 (b % 4) .. + 15 of it; row
+  // items read tile (K, M) at [source][output], column items tile (M, K),
+  // M < K, at [output][source]
+  const int nblk = (it.w - it.z) * 4;
+  for (int b = 0; b < nblk; b++) {
+    const int M = it.z + (b >> 2), s0 = (b & 3) * 16 + lk;
+    const R *T = ROW ? tiles + (size_t)(m.rowoff[K] + (M - K) - m.t0) * 4096 + li
+                     : tiles + (size_t)(m.rowoff[M] + (K - M) - m.t0) * 4096 + li * 64;
+    R ga[SB][4];
+    double xb[SB][NRB];
+#pragma unroll
+    for (int st = 0; st < SB; st++) {
+      const int src = s0 + 4 * st;
+#pragma unroll
+      for (int ib = 0; ib < 4; ib++) ga[st][ib] = ROW ? T[src * 64 + ib * 16] : T[ib * 16 * 64 + src];
+      // a source past the mesh (last group) has G = 0; its index is clamped
+      const double *xp = sig + (size_t)min(M * 64 + src, ne - 1) * RB;
+#pragma unroll
+      for (int rb = 0; rb < NRB; rb++)
+        xb[st][rb] = (RB >= 16 || li < RB) ? xp[rb * 16 + li] : 0.0;
+    }
+#pragma unroll
+    for (int st = 0; st < SB; st++)
+#pragma unroll
+      for (int ib = 0; ib < 4; ib++) {
+        const double av = (double)ga[st][ib];
+#pragma unroll
+        for (int rb = 0; rb < NRB; rb++)
+          acc[ib][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xb[st][rb], acc[ib][rb], 0, 0, 0);
+      }
+  }
+  const int nrhs_here = min(RB, a.nrhs - g * RB);
+  double *__restrict__ out = m.part + ((size_t)w * m.groups + g) * RB * 64;
+#pragma unroll
+  for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+    for (int rb = 0; rb < NRB; rb++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rb * 16 + li;
+        if (r < nrhs_here) out[r * 64 + ib * 16 + 4 * v + lk] = acc[ib][rb][v];
+      }
+}
+
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void k_mrhs_tile_mfma(FarMArgs<R> a, MtArgs m,
+                                                        const R *__restrict__ tiles)
+{
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (w >= m.items) return;  // wave-uniform; no barrier in this kernel
+  const int4 it = m.item[w];
+  if (it.y == 0) mfma_item<R, RB, true>(a, m, tiles, w, it);
+  else mfma_item<R, RB, false>(a, m, tiles, w, it);
+}
+
+// Output group K = I0 + blockIdx.x / (RB / 4), right-hand side r of group
+// blockIdx.y (4 a workgroup, one wave each), triangle lane: the sums of the
+// earlier passes (none in the first) plus the pass's items of K in order.
+template <int RB>
+__global__ __launch_bounds__(256) void k_mrhs_tile_reduce(MtArgs m, int ne, int nrhs,
+                                                          double *__restrict__ out)
+{
+  const int g = blockIdx.y, L = threadIdx.x & 63;
+  const int kr = blockIdx.x / (RB / 4);
+  const int r = (blockIdx.x - kr * (RB / 4)) * 4 + (int)(threadIdx.x >> 6);
+  const int t = (m.I0 + kr) * 64 + L, rr = g * RB + r;
+  if (t >= ne || rr >= nrhs) return;
+  double s = m.first ? 0.0 : out[(size_t)rr * ne + t];
+  for (int i = m.kptr[kr]; i < m.kptr[kr + 1]; i++)
+    s += m.part[(((size_t)i * m.groups + g) * RB + r) * 64 + L];
+  out[(size_t)rr * ne + t] = s;
 }
 
 // Source records of the multi-RHS kernels.  Per point strengths
@@ -2211,6 +2391,87 @@ int launch_mrhs_tri(int nt, bool med, dim3 grid, hipStream_t s, const FarMArgs<R
   return EBH_OK;
 }
 
+// kernels of the pair-symmetric multi-RHS schedule
+template <typename R>
+int launch_mt_tiles(int nt, bool med, dim3 grid, hipStream_t s, const FarMArgs<R> &a,
+                    const MtArgs &m, R *tiles)
+{
+  switch (nt) {
+  case 1: hipLaunchKernelGGL((k_mrhs_tiles<R, 1, false>), grid, dim3(256), 0, s, a, m, tiles); break;
+  case 3: hipLaunchKernelGGL((k_mrhs_tiles<R, 3, false>), grid, dim3(256), 0, s, a, m, tiles); break;
+  case 4: hipLaunchKernelGGL((k_mrhs_tiles<R, 4, false>), grid, dim3(256), 0, s, a, m, tiles); break;
+  case 6:
+    if constexpr (sizeof(R) == 4)
+      if (med) {
+        hipLaunchKernelGGL((k_mrhs_tiles<R, 6, true>), grid, dim3(256), 0, s, a, m, tiles);
+        break;
+      }
+    hipLaunchKernelGGL((k_mrhs_tiles<R, 6, false>), grid, dim3(256), 0, s, a, m, tiles);
+    break;
+  case 7:
+    if constexpr (sizeof(R) == 4)
+      if (med) {
+        hipLaunchKernelGGL((k_mrhs_tiles<R, 7, true>), grid, dim3(256), 0, s, a, m, tiles);
+        break;
+      }
+    hipLaunchKernelGGL((k_mrhs_tiles<R, 7, false>), grid, dim3(256), 0, s, a, m, tiles);
+    break;
+  default:
+    set_error("unsupported far-field rule size " + std::to_string(nt) +
+              " for the pair-symmetric multi-RHS schedule");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+template <typename R, int RB>
+int launch_mt_apply(dim3 ga, dim3 gr, hipStream_t s, const FarMArgs<R> &a, const MtArgs &m,
+                    const R *tiles, double *out)
+{
+  hipLaunchKernelGGL((k_mrhs_tile_mfma<R, RB>), ga, dim3(256), 0, s, a, m, tiles);
+  EBH_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_mrhs_tile_reduce<RB>), gr, dim3(256), 0, s, m, a.ne, a.nrhs, out);
+  EBH_HIP(hipGetLastError());
+  return EBH_OK;
+}
+
+// The passes of the pair-symmetric multi-RHS schedule: per pass the tiles,
+// the items and their reduction into tri_out ([nrhs][ne] far sums).
+template <typename R>
+int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
+{
+  const int ne = a.ne, ng = (ne + 63) / 64, rb = op->rb;
+  R *tiles;
+  if constexpr (sizeof(R) == 4) tiles = op->mt_tile32.p;
+  else tiles = op->mt_tile64.p;
+  const int np = (int)op->mt_rows.size() - 1;
+  for (int p = 0; p < np; p++) {
+    MtArgs m;
+    m.ng = ng;
+    m.I0 = op->mt_rows[p];
+    m.I1 = op->mt_rows[p + 1];
+    m.rowoff = op->mt_rowoff.p;
+    m.t0 = op->mt_t0[p];
+    m.ntiles = op->mt_t0[p + 1] - m.t0;
+    m.items = op->mt_ioff[p + 1] - op->mt_ioff[p];
+    m.groups = a.groups;
+    m.first = p == 0;
+    m.item = reinterpret_cast<const int4 *>(op->mt_item.p) + op->mt_ioff[p];
+    m.kptr = op->mt_kptr.p + op->mt_koff[p];
+    m.part = op->mt_part.p;
+    EBH_TRY(launch_mt_tiles<R>(op->rule.nt, med, dim3((m.ntiles + 3) / 4), s, a, m, tiles));
+    const dim3 ga((m.items + 3) / 4, a.groups), gr((ng - m.I0) * (rb / 4), a.groups);
+    switch (rb) {
+    case 64: EBH_TRY((launch_mt_apply<R, 64>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
+    case 8: EBH_TRY((launch_mt_apply<R, 8>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
+    case 16: EBH_TRY((launch_mt_apply<R, 16>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
+    default: EBH_TRY((launch_mt_apply<R, 32>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
+    }
+  }
+  return EBH_OK;
+}
+
 template <typename R, int RB>
 int launch_mrhs_nt(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
 {
@@ -2294,7 +2555,9 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
   int rc;
-  if (mrhs_tri(op)) {
+  if (op->mts) {
+    rc = run_mrhs_tiles<R>(op, a, med > 0, s);
+  } else if (mrhs_tri(op)) {
     switch (rb) {
     case 64: rc = launch_mrhs_tri<R, 64>(nt, med > 0, grid, s, a); break;
     case 8: rc = launch_mrhs_tri<R, 8>(nt, med > 0, grid, s, a); break;
@@ -2392,6 +2655,81 @@ static void plan_passes(ebh_op *op, size_t nout)
   }
 }
 
+// The pair-symmetric multi-RHS schedule: items of at most kMtSeg tiles (a
+// wave's share: kMtSeg x 4096 triangle pairs of RB FMAs each)
+constexpr int kMtSeg = 32;
+
+bool mts_eligible(const ebh_op *op)
+{
+  // tile indices (ng (ng + 1) / 2) fit an int
+  return op->rb && mrhs_tri(op) && (op->mesh->T.ne + 63) / 64 <= 65000;
+}
+
+// Passes of the pair-symmetric multi-RHS schedule: blocks of kMtSeg tile
+// rows are taken greedily while a pass's tiles and item sums stay within
+// the operator's partial limit (one block at least).  Items as described at
+// k_mrhs_tiles, column items on the same kMtSeg-row grid as the passes, and
+// each output's items in increasing source group: every output sees the
+// same sequence of additions whatever the passes, so y does not depend on
+// the limit.
+static int plan_mrhs_tiles(ebh_op *op)
+{
+  const int ne = op->mesh->T.ne, ng = (ne + 63) / 64;
+  const bool f32 = op->d.far_precision == EBH_FP32;
+  const double tb = f32 ? 4.0 : 8.0;
+  const size_t groups = (op->d.max_rhs + op->rb - 1) / op->rb;
+  const double ib = 8.0 * (double)(groups * op->rb * 64);
+  std::vector<int> rowoff(ng + 1, 0);
+  for (int I = 0; I < ng; I++) rowoff[I + 1] = rowoff[I] + (ng - I);
+  auto seg = [](int n) { return (n + kMtSeg - 1) / kMtSeg; };
+  auto bytes = [&](int I0, int I1) {
+    size_t n = 0;
+    for (int K = I0; K < I1; K++) n += seg(ng - K);
+    for (int K = I0 + 1; K < ng; K++) n += seg(std::min(I1, K) - I0);
+    return (double)(rowoff[I1] - rowoff[I0]) * 4096.0 * tb + (double)n * ib;
+  };
+  std::vector<int> item, kptr;
+  op->mt_rows.assign(1, 0);
+  op->mt_t0.assign(1, 0);
+  op->mt_ioff.assign(1, 0);
+  op->mt_koff.assign(1, 0);
+  op->mt_tiles = op->mt_items = 0;
+  for (int I0 = 0; I0 < ng;) {
+    int I1 = std::min(ng, I0 + kMtSeg);
+    while (I1 < ng && bytes(I0, std::min(ng, I1 + kMtSeg)) <= op->partial_limit)
+      I1 = std::min(ng, I1 + kMtSeg);
+    const int n0 = (int)(item.size() / 4);
+    for (int K = I0; K < ng; K++) {
+      kptr.push_back((int)(item.size() / 4) - n0);
+      const int c1 = std::min(I1, K);
+      for (int M0 = I0; M0 < c1; M0 += kMtSeg)
+        item.insert(item.end(), {K, 1, M0, std::min(c1, M0 + kMtSeg)});
+      if (K < I1)
+        for (int M0 = K; M0 < ng; M0 += kMtSeg)
+          item.insert(item.end(), {K, 0, M0, std::min(ng, M0 + kMtSeg)});
+    }
+    kptr.push_back((int)(item.size() / 4) - n0);
+    op->mt_rows.push_back(I1);
+    op->mt_t0.push_back(rowoff[I1]);
+    op->mt_ioff.push_back((int)(item.size() / 4));
+    op->mt_koff.push_back((int)kptr.size());
+    op->mt_tiles = std::max(op->mt_tiles, (size_t)(rowoff[I1] - rowoff[I0]));
+    op->mt_items = std::max(op->mt_items, item.size() / 4 - (size_t)n0);
+    I0 = I1;
+  }
+  EBH_TRY(op->mt_rowoff.upload(rowoff));
+  EBH_TRY(op->mt_item.upload(item));
+  EBH_TRY(op->mt_kptr.upload(kptr));
+  if (f32) {
+    op->mt_tile64.release();
+    EBH_TRY(op->mt_tile32.alloc(op->mt_tiles * 4096));
+  } else {
+    op->mt_tile32.release();
+    EBH_TRY(op->mt_tile64.alloc(op->mt_tiles * 4096));
+  }
+  return op->mt_part.alloc(op->mt_items * groups * op->rb * 64);
+}
+
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
 {
   const int ne = op->mesh->T.ne;
@@ -2465,7 +2803,15 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     mc = std::min(mc, ne);
     op->mchunk = (ne + mc - 1) / mc;
     op->mchunks = (ne + op->mchunk - 1) / op->mchunk;
-    need = std::max(need, (size_t)op->mchunks * op->d.max_rhs * nout * ne);
+    op->mts = op->mrhs_sched != EBH_MRHS_TARGET && mts_eligible(op);
+    if (op->mts) {
+      EBH_TRY(plan_mrhs_tiles(op));
+    } else {
+      op->mt_tile32.release();
+      op->mt_tile64.release();
+      op->mt_part.release();
+      need = std::max(need, (size_t)op->mchunks * op->d.max_rhs * nout * ne);
+    }
   }
   if (op->sym && op->nr_max == 3) {
     // three right-hand sides per symmetric launch: 8-wide source records,
@@ -2770,10 +3116,13 @@ int op_apply_multi(ebh_op *op, const double *x, double *y, int nrhs,
   if (timed) EBH_HIP(hipEventRecord(op->ev[1], s));
   EBH_TRY(f32 ? run_mrhs<float>(op, x, nrhs, s) : run_mrhs<double>(op, x, nrhs, s));
   if (timed) EBH_HIP(hipEventRecord(op->ev[2], s));
-  const int nred = (op->d.test_basis ? 3 : 1) * ne * nrhs;
-  hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
-                     op->mchunks, op->partial.p, op->tri_out.p);
-  EBH_HIP(hipGetLastError());
+  if (!op->mts) {
+    // (the pair-symmetric schedule's passes wrote the far sums to tri_out)
+    const int nred = (op->d.test_basis ? 3 : 1) * ne * nrhs;
+    hipLaunchKernelGGL(k_reduce, dim3((nred + 255) / 256), dim3(256), 0, s, nred,
+                       op->mchunks, op->partial.p, op->tri_out.p);
+    EBH_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_finish, dim3(finish_blocks(op->nrows), nrhs), dim3(256), 0, s,
                      op->nrows, op->ncols, op->nc, op->d.test_basis, ne,
                      op->near_ptr.p, op->near_col.p, op->near_val.p, x,

@@ -180,6 +180,24 @@ int ebh_op_get_reduction(const ebh_op *op, int *fused);
  * reference. */
 int ebh_op_set_partial_limit(ebh_op *op, int64_t bytes);
 int ebh_op_get_partial(const ebh_op *op, int64_t *limit, int *passes, int64_t *bytes);
+/* Schedule of a multi-RHS apply (max_rhs > 1) of a P0 single layer with a
+ * far rule of at most 7 points.  EBH_MRHS_TARGET: each target triangle
+ * against every source triangle, every triangle pair's point sums formed in
+ * both orders.  EBH_MRHS_PAIRS: the point sums of each unordered triangle
+ * pair formed once, into 64 x 64 tiles of group pairs, then applied to the
+ * right-hand sides in both directions as FP64 tile products on the matrix
+ * cores; in passes of blocks of 32 tile rows whose tiles and item sums stay
+ * within the partial limit (ebh_op_set_partial_limit; one block at least),
+ * y bitwise the same for every limit.  Both schedules sum in a fixed order
+ * (bitwise repeatable) and agree to rounding.
+ * EBH_MRHS_AUTO (the default) takes EBH_MRHS_PAIRS where it applies.  get:
+ * the schedule in effect (EBH_MRHS_TARGET for operators without it) and its
+ * passes (0 for EBH_MRHS_TARGET).  No counterpart in the reference. */
+#define EBH_MRHS_AUTO 0
+#define EBH_MRHS_TARGET 1
+#define EBH_MRHS_PAIRS 2
+int ebh_op_set_mrhs_schedule(ebh_op *op, int schedule);
+int ebh_op_get_mrhs_schedule(const ebh_op *op, int *schedule, int *passes);
 /* Averages over the last `ebh_op_time` calls: milliseconds per apply of
  * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
  * measured with HIP events on the op's stream over `reps` applies of the
