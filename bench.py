@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
+PEAK_FP64_MFMA_TFLOPS = 78.6  # MI355X dense FP64 matrix (datasheet)
 PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (datasheet)
 FLOP_SL = 11               # per far interaction, single layer, one RHS
 # HBM bytes per launch of the FP32 single-layer far kernel on the level-5
@@ -204,13 +205,37 @@ def secondary(L, meshes, solve_batched=False):
         x = L.DeviceArray.from_host(B)
         y = L.DeviceArray(op.nrows * 64)
         tm = op.time(x, y, reps=1, nrhs=64)
-        # P0 test and trial: each triangle pair sums its 7 x 7 point pairs
-        # (rule 4) once, 9 FLOP for r^2 and 2 for the weight per point
-        # pair, 2 per target point, then 2 per RHS per triangle pair
-        pts = op.work()["far_interactions"]
-        flop = 11 * pts + pts / 49 * (2 * 7 + 2 * 64)
-        r4[prec] = {"block_matvec_ms": tm["total"],
-                    "far_tflops": flop / tm["far"] / 1e9}
+        # the tile launches alone (far-kernel profiling records them)
+        L.op_profile(op, True, 64)
+        op.apply(x, y, nrhs=64)
+        tiles_ms, tiles_n = L.op_profile(op, False)
+        sched, passes = op.mrhs_schedule
+        pts = op.work()["far_interactions"]   # ordered far point pairs
+        tri_pairs = pts / 49                  # ordered far triangle pairs
+        # FLOP the pair-symmetric schedule executes: each unordered
+        # triangle pair's 7 x 7 point pairs once (11 FLOP each: r^2 and the
+        # source weight) plus 2 per target point for the target weights
+        # (k_mrhs_tiles, FP32 or FP64 VALU), then 2 FLOP per right-hand side
+        # per ordered pair (k_mrhs_tile_mfma, FP64 matrix cores)
+        flop_tiles = tri_pairs / 2 * (49 * 11 + 2 * 7)
+        flop_mfma = tri_pairs * 2 * 64
+        mfma_ms = tm["far"] - tiles_ms
+        valu_peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_FP64_TFLOPS
+        r4[prec] = {
+            "block_matvec_ms": tm["total"], "schedule": sched, "passes": passes,
+            "far_ms": tm["far"],
+            "k_mrhs_tiles": {"ms": tiles_ms, "launches": tiles_n, "bound": "valu",
+                             "tflops": flop_tiles / tiles_ms / 1e9,
+                             "frac": flop_tiles / tiles_ms / 1e9 / valu_peak},
+            "k_mrhs_tile_mfma_and_reduce": {
+                "ms": mfma_ms, "bound": "mfma",
+                "tflops": flop_mfma / mfma_ms / 1e9,
+                "frac": flop_mfma / mfma_ms / 1e9 / PEAK_FP64_MFMA_TFLOPS},
+            # BASELINE's convention (9 + 2 nrhs FLOP per ordered point pair)
+            # divided by the far time: above the peaks because the P0
+            # densities factor per triangle pair and each unordered pair's
+            # point sums are formed once
+            "far_tflops_convention": pts * (9 + 2 * 64) / tm["far"] / 1e9}
     if solve_batched:
         s = L.Solver(ops["fp64"], ops["fp32"], restart=50, batch=64)
         # near-field sparse approximate inverse on 2 triangle rings
@@ -261,6 +286,33 @@ def secondary(L, meshes, solve_batched=False):
         r4["block_mixed_speedup_vs_block_fp64"] = (r4["block_fp64_solve"]["seconds"] /
                                                    r4["block_mixed_solve"]["seconds"])
         r4["block_mixed_speedup_vs_lockstep_fp64"] = dt64 / r4["block_mixed_solve"]["seconds"]
+        # the plain FP64 path: the same systems one at a time, a single-RHS
+        # FP64 operator (pair-symmetric single-RHS schedule) with the sparse
+        # inverse built for one right-hand side; one untimed solve, then
+        # `nseq` of the 64 systems timed
+        n = ops["fp64"].ncols
+        V1 = L.Operator(m, "V", 0, 0, "fp64", line_rule=4, tri_rule=4)
+        S1 = L.precond_spai(V1, rings=2)
+        s1 = L.Solver(V1, None, restart=50)
+        s1.set_precond(S1, S1)
+        nseq = 4
+        its = []
+        for r in range(nseq + 1):
+            b1 = L.DeviceArray.from_host(B[r * n:(r + 1) * n])
+            x1 = L.DeviceArray.from_host(np.zeros(n))
+            if r == 1:
+                t0 = time.perf_counter()
+            st1 = s1.solve_device(b1, x1, tol=1e-8, mixed=False, max_iters=3000)
+            if r >= 1:
+                its.append(st1[0]["iters"] if isinstance(st1, list) else st1["iters"])
+        dts = (time.perf_counter() - t0) / nseq
+        s1.close()
+        r4["sequential_fp64_solve"] = {
+            "seconds_per_solve": dts, "solves_per_s": 1.0 / dts, "systems_timed": nseq,
+            "iters_max": max(its),
+            "note": "plain FP64 path: one system at a time, single-RHS FP64 operator"}
+        r4["block_mixed_speedup_vs_sequential_fp64"] = (
+            r4["block_mixed_solve"]["solves_per_s"] * dts)
     out["cfg4_ico6_p0_V_64rhs"] = r4
     out["multi_mesh"] = multi_mesh(L, meshes)
     return out

@@ -2348,14 +2348,16 @@ bool mrhs_tri(const ebh_op *op)
   return !op->d.test_basis && !op->d.trial_basis && op->rule.nt <= 7;
 }
 
-// record width of the median form of k_far_mrhs_tri (FP32, rules of 6 or 7
-// points on the medians; the record fits the 4 values per point of mgeom),
-// else 0.  FP64 keeps the point form: there the refined rsqrt dominates and
-// the median form's registers cost occupancy (measured 15% slower at L6).
+// record width of the median form of the triangle-pair sums (rules of 6 or
+// 7 points on the medians; the record fits the 4 values per point of
+// mgeom), else 0.  FP64 k_far_mrhs_tri keeps the point form: there the
+// median form's registers, beside its 64 accumulators, cost occupancy
+// (measured 15% slower at L6); k_mrhs_tiles holds no accumulators.
 int mrhs_med(const ebh_op *op)
 {
   const int nt = op->rule.nt;
-  if (!mrhs_tri(op) || !op->ray || nt < 6 || op->d.far_precision != EBH_FP32) return 0;
+  if (!mrhs_tri(op) || !op->ray || nt < 6) return 0;
+  if (op->d.far_precision != EBH_FP32 && !op->mts) return 0;
   return nt == 6 ? MedRec<6>::W : MedRec<7>::W;
 }
 
@@ -2401,20 +2403,12 @@ int launch_mt_tiles(int nt, bool med, dim3 grid, hipStream_t s, const FarMArgs<R
   case 3: hipLaunchKernelGGL((k_mrhs_tiles<R, 3, false>), grid, dim3(256), 0, s, a, m, tiles); break;
   case 4: hipLaunchKernelGGL((k_mrhs_tiles<R, 4, false>), grid, dim3(256), 0, s, a, m, tiles); break;
   case 6:
-    if constexpr (sizeof(R) == 4)
-      if (med) {
-        hipLaunchKernelGGL((k_mrhs_tiles<R, 6, true>), grid, dim3(256), 0, s, a, m, tiles);
-        break;
-      }
-    hipLaunchKernelGGL((k_mrhs_tiles<R, 6, false>), grid, dim3(256), 0, s, a, m, tiles);
+    if (med) hipLaunchKernelGGL((k_mrhs_tiles<R, 6, true>), grid, dim3(256), 0, s, a, m, tiles);
+    else hipLaunchKernelGGL((k_mrhs_tiles<R, 6, false>), grid, dim3(256), 0, s, a, m, tiles);
     break;
   case 7:
-    if constexpr (sizeof(R) == 4)
-      if (med) {
-        hipLaunchKernelGGL((k_mrhs_tiles<R, 7, true>), grid, dim3(256), 0, s, a, m, tiles);
-        break;
-      }
-    hipLaunchKernelGGL((k_mrhs_tiles<R, 7, false>), grid, dim3(256), 0, s, a, m, tiles);
+    if (med) hipLaunchKernelGGL((k_mrhs_tiles<R, 7, true>), grid, dim3(256), 0, s, a, m, tiles);
+    else hipLaunchKernelGGL((k_mrhs_tiles<R, 7, false>), grid, dim3(256), 0, s, a, m, tiles);
     break;
   default:
     set_error("unsupported far-field rule size " + std::to_string(nt) +
@@ -2460,7 +2454,12 @@ int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
     m.item = reinterpret_cast<const int4 *>(op->mt_item.p) + op->mt_ioff[p];
     m.kptr = op->mt_kptr.p + op->mt_koff[p];
     m.part = op->mt_part.p;
+    // far-kernel profiling records the tile launches (the VALU-bound kernel);
+    // the matrix-core products and reductions follow outside the pairs
+    const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
+    if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
     EBH_TRY(launch_mt_tiles<R>(op->rule.nt, med, dim3((m.ntiles + 3) / 4), s, a, m, tiles));
+    if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
     const dim3 ga((m.items + 3) / 4, a.groups), gr((ng - m.I0) * (rb / 4), a.groups);
     switch (rb) {
     case 64: EBH_TRY((launch_mt_apply<R, 64>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
@@ -2552,12 +2551,11 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   const int tpb = mrhs_tri(op) ? 256 : 256 / nt;
   const int tbn = (ne + tpb - 1) / tpb;
   dim3 grid((unsigned)(tbn * groups * op->mchunks));
+  if (op->mts) return run_mrhs_tiles<R>(op, a, med > 0, s);
   const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
   if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
   int rc;
-  if (op->mts) {
-    rc = run_mrhs_tiles<R>(op, a, med > 0, s);
-  } else if (mrhs_tri(op)) {
+  if (mrhs_tri(op)) {
     switch (rb) {
     case 64: rc = launch_mrhs_tri<R, 64>(nt, med > 0, grid, s, a); break;
     case 8: rc = launch_mrhs_tri<R, 8>(nt, med > 0, grid, s, a); break;
