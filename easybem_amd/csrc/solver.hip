@@ -261,6 +261,99 @@ __global__ void k_colnormalize(int n, int c, int p, const double *__restrict__ n
 
 inline dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// H = V^T W, the block Gram-Schmidt projection: V (n x K) and W (n x p),
+// column-major with leading dimension n, H (K x p) column-major.  The
+// reduction runs over the long dimension n, so it is split: wave (ib, c,
+// jb), one 64-thread workgroup, forms the 64 x 64 block of columns 64 ib ..
+// of V against columns 64 jb .. of W over rows c ch .. c ch + ch - 1, in
+// FP64 MFMA 16x16x4 blocks (A = V^T: lane i + 16 k holds row k, column i of
+// a 4 x 16 slice of V; B = W: lane j + 16 k holds row k, column j; D:
+// column i, column j at lane j + 16 (i % 4), item i / 4), into part[c];
+// k_gemm_tn_sum adds the chunks in order.  Rows and columns past the
+// matrices read as 0.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_gemm_tn(int n, int K, int p, int ch,
+                                                const double *__restrict__ V,
+                                                const double *__restrict__ W,
+                                                double *__restrict__ part)
+{
+  const int ib = blockIdx.x, c = blockIdx.y, jb = blockIdx.z, l = threadIdx.x;
+  const int li = l & 15, lk = l >> 4;
+  const int r0 = c * ch, r1 = min(n, r0 + ch);
+  const double *va[4], *wb[4];
+  bool vok[4], wok[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int ci = ib * 64 + q * 16 + li, cj = jb * 64 + q * 16 + li;
+    vok[q] = ci < K;
+    wok[q] = cj < p;
+    va[q] = V + (size_t)(vok[q] ? ci : 0) * n;
+    wb[q] = W + (size_t)(wok[q] ? cj : 0) * n;
+  }
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  // 2 steps (8 rows) a block: their operands first, then 32 MFMAs
+  for (int r = r0; r < r1; r += 8) {
+    double av[2][4], bv[2][4];
+#pragma unroll
+    for (int st = 0; st < 2; st++) {
+      const int row = r + 4 * st + lk;
+      const bool rok = row < r1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        av[st][q] = rok && vok[q] ? va[q][row] : 0.0;
+        bv[st][q] = rok && wok[q] ? wb[q][row] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < 2; st++)
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st][a], bv[st][b], acc[a][b],
+                                                           0, 0, 0);
+  }
+  // part[c] is K x p column-major, like H
+  double *o = part + (size_t)c * K * p;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = ib * 64 + a * 16 + 4 * v + lk, j = jb * 64 + b * 16 + li;
+        if (i < K && j < p) o[i + (size_t)j * K] = acc[a][b][v];
+      }
+}
+
+// H = sum of the chunks' parts in chunk order
+__global__ void k_gemm_tn_sum(int kp, int nch, const double *__restrict__ part,
+                              double *__restrict__ H)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kp) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; c++) s += part[(size_t)c * kp + e];
+  H[e] = s;
+}
+
+// chunks of the split reduction for K x p: about 2048 waves, chunks of at
+// least 256 rows (a multiple of 16)
+inline void gemm_tn_shape(int n, int K, int p, int *nch, int *ch)
+{
+  const int nib = ((K + 63) / 64) * ((p + 63) / 64);
+  int c = std::max(1, std::min(2048 / nib, (n + 255) / 256));
+  int h = (n + c - 1) / c;
+  h = (h + 15) / 16 * 16;
+  *ch = h;
+  *nch = (n + h - 1) / h;
+}
+
 }  // namespace
 }  // namespace ebh
 
@@ -297,6 +390,8 @@ struct ebh_solver {
   rocblas_handle blas = nullptr;
   DevBuf<double> Hb, H2b, Sb, Yb, cw;
   DevBuf<unsigned char> blas_ws;
+  // chunk parts of the projections V^T W (k_gemm_tn), sized for K = (m+1) nb
+  DevBuf<double> tn_part;
   // host side of the block least-squares problem: R (the block Hessenberg
   // matrix reduced by Householder reflectors, column-major, leading
   // dimension (m+1) nb), the reflectors (one of nb+1 entries per column,
@@ -474,6 +569,16 @@ struct ebh_solver {
     EBH_TRY(Sb.alloc((size_t)p * p));
     EBH_TRY(Yb.alloc((size_t)m * p * p));
     EBH_TRY(cw.alloc(2 * (size_t)p + 2));
+    {
+      size_t most = 0;
+      for (int j = 0; j <= m; j++) {
+        int nch, ch;
+        const int K = (j + 1) * p;
+        gemm_tn_shape(n, K, p, &nch, &ch);
+        most = std::max(most, (size_t)nch * K * p);
+      }
+      EBH_TRY(tn_part.alloc(most));
+    }
     if (!blas) {
       if (rocblas_create_handle(&blas) != rocblas_status_success) {
         blas = nullptr;
@@ -494,6 +599,19 @@ struct ebh_solver {
       set_error("rocblas_set_stream failed");
       return EBH_ERR_HIP;
     }
+    return EBH_OK;
+  }
+
+  // H = V^T W for the first K columns of V and the p columns of W
+  int gemm_tn(int K, int p, const double *W, double *H)
+  {
+    int nch, ch;
+    gemm_tn_shape(n, K, p, &nch, &ch);
+    hipLaunchKernelGGL(k_gemm_tn, dim3((K + 63) / 64, nch, (p + 63) / 64), dim3(64), 0,
+                       stream, n, K, p, ch, V.p, W, tn_part.p);
+    hipLaunchKernelGGL(k_gemm_tn_sum, grid1((size_t)K * p), dim3(256), 0, stream, K * p, nch,
+                       tn_part.p, H);
+    EBH_HIP(hipGetLastError());
     return EBH_OK;
   }
 
@@ -558,9 +676,9 @@ struct ebh_solver {
       EBH_TRY(op_apply_multi(A, vj, w, p, stream, false));
     }
     const int K = (j + 1) * p;
-    EBH_TRY(dgemm(rocblas_operation_transpose, K, p, n, 1.0, V.p, n, w, n, 0.0, Hb.p, K));
+    EBH_TRY(gemm_tn(K, p, w, Hb.p));
     EBH_TRY(dgemm(rocblas_operation_none, n, p, K, -1.0, V.p, n, Hb.p, K, 1.0, w, n));
-    EBH_TRY(dgemm(rocblas_operation_transpose, K, p, n, 1.0, V.p, n, w, n, 0.0, H2b.p, K));
+    EBH_TRY(gemm_tn(K, p, w, H2b.p));
     EBH_TRY(dgemm(rocblas_operation_none, n, p, K, -1.0, V.p, n, H2b.p, K, 1.0, w, n));
     hipLaunchKernelGGL(k_axpby, grid1((size_t)K * p), dim3(256), 0, stream, (size_t)K * p,
                        1.0, H2b.p, 1.0, Hb.p);

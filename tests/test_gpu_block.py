@@ -41,6 +41,27 @@ def test_block_fp64_matches_lockstep_solves():
     assert max(s["iters"] for s in st1) <= max(s["iters"] for s in st0)
 
 
+@pytest.mark.parametrize("nb", [5, 70])
+def test_block_odd_and_wide_blocks(nb):
+    """Block widths that are not a multiple of 16 (5) and wider than the
+    projection kernel's 64 columns (70): the split projection V^T W covers
+    partial and several column blocks."""
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(2)
+    m = L.Mesh(P, T)
+    V = L.Operator(m, "V", 0, 0, "fp64", max_rhs=nb)
+    one = L.Operator(m, "V", 0, 0, "fp64")
+    B = np.random.default_rng(nb).uniform(-1, 1, (nb, T.shape[0]))
+    blk = L.Solver(V, None, batch=nb)
+    blk.set_block(True)
+    print(f"case block width {nb}", flush=True)
+    X1, st1 = blk.solve(B.ravel(), tol=1e-10)
+    X1 = X1.reshape(nb, -1)
+    print("block steps", max(s["iters"] for s in st1))
+    assert all(s["converged"] for s in st1)
+    assert _recomputed(one, X1, B) <= 1e-10
+
+
 def test_block_graph_replays_equal_eager():
     from easybem_amd import _lib as L
     P, T = meshes.icosphere(3)
