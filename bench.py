@@ -489,6 +489,32 @@ def pmc_probe(L, meshes, level):
                       "far_interactions": V32.work()["far_interactions"]}))
 
 
+def mrhs_compare(L, meshes, level=6, reps=2):
+    """Config 4's 64-RHS block matvec (icosphere `level`, P0 V, 7-point far
+    rule) under both multi-RHS schedules, FP32 and FP64: per-apply times
+    (HIP events, ebh_op_time) and the pair schedule's largest deviation from
+    the target schedule, one JSON line each."""
+    P, T = meshes.icosphere(level)
+    m = L.Mesh(P, T)
+    n = T.shape[0]
+    X = np.random.default_rng(1).uniform(-1, 1, 64 * n)
+    for prec in ("fp32", "fp64"):
+        ys = {}
+        for sched in ("target", "pairs"):
+            op = L.Operator(m, "V", 0, 0, prec, line_rule=4, tri_rule=4, max_rhs=64)
+            op.set_mrhs_schedule(sched)
+            x = L.DeviceArray.from_host(X)
+            y = L.DeviceArray(64 * n)
+            tm = op.time(x, y, reps=reps, nrhs=64)
+            ys[sched] = y.to_host().reshape(64, n)
+            print(json.dumps({"precision": prec, "schedule": sched,
+                              "passes": op.mrhs_schedule[1], "ms": tm}), flush=True)
+            del op
+        a, b = ys["target"], ys["pairs"]
+        e = max(np.linalg.norm(a[r] - b[r]) / np.linalg.norm(a[r]) for r in range(64))
+        print(json.dumps({"precision": prec, "pairs_vs_target_max_rel": e}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -518,6 +544,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only the fixed copy + apply workload profiled "
                          "with hardware counters (no JSON bench line)")
+    ap.add_argument("--mrhs-compare", action="store_true",
+                    help="run only config 4's 64-RHS block matvec under both "
+                         "multi-RHS schedules (no JSON bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -529,6 +558,9 @@ def main():
     if args.pmc_probe:
         # same launch shapes as the bench line (the packaged table)
         pmc_probe(L, meshes, args.level)
+        return
+    if args.mrhs_compare:
+        mrhs_compare(L, meshes)
         return
     P, T = meshes.icosphere(args.level)
     x0 = np.array([0.0, 0.0, 2.0])
