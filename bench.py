@@ -92,21 +92,26 @@ def cpu_model():
     return "unknown"
 
 
-def _sampled_matvec(o, P, T, threads, seconds, seed):
-    """Seconds of one full P1 V matvec (config 2's operator) extrapolated
-    from the oracle on random node rows, sized to about `seconds` of work."""
+def _sampled_matvec(o, P, T, threads, seconds, seed, op="V", basis=1, nu=0.3):
+    """Seconds of one full matvec extrapolated from the oracle on random
+    test rows (node rows for P1, triangle rows for P0; the far rule 7 points,
+    the near rule 5-point Gauss), sized to about `seconds` of work: config 2's
+    P1 V by default, config 3's Kelvin P1 V, config 4's P0 V."""
     rng = np.random.default_rng(seed)
-    x = rng.uniform(-1, 1, P.shape[0])
+    ncols = (P.shape[0] if basis else T.shape[0]) * (3 if op == "kelvin" else 1)
+    ntest = P.shape[0] if basis else T.shape[0]
+    x = rng.uniform(-1, 1, ncols)
     nrows = 8
     while True:
-        rows = rng.choice(P.shape[0], size=nrows, replace=False)
+        rows = rng.choice(ntest, size=nrows, replace=False)
         t0 = time.perf_counter()
-        o.apply("V", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4, nthreads=threads)
+        o.apply(op, basis, basis, P, T, x, rows=rows, line_idx=4, tri_idx=4, nu=nu,
+                nthreads=threads)
         elapsed = time.perf_counter() - t0
-        if elapsed > seconds / 4 or nrows >= P.shape[0]:
+        if elapsed > seconds / 4 or nrows >= ntest:
             break
-        nrows = min(P.shape[0], int(nrows * max(2.0, seconds / 4 / max(elapsed, 1e-3))))
-    return elapsed / nrows * P.shape[0], nrows, elapsed
+        nrows = min(ntest, int(nrows * max(2.0, seconds / 4 / max(elapsed, 1e-3))))
+    return elapsed / nrows * ntest, nrows, elapsed
 
 
 def cpu_baseline(P, T, n_matvec, seconds=10.0):
@@ -138,6 +143,19 @@ def cpu_baseline(P, T, n_matvec, seconds=10.0):
     np.linalg.solve(V, x1)
     c1["dense_solve_s"] = {"assembly_all_cores": t_asm,
                            "lapack_solve": time.perf_counter() - t0}
+    # configs 3 and 4: sampled rows on all cores, extrapolated
+    P3, T3 = meshes.torus(1.0, 0.4, 256, 128)
+    t3, r3, e3 = _sampled_matvec(o, P3, T3, threads, 4.0, 9, op="kelvin", basis=1)
+    P4, T4 = meshes.icosphere(6)
+    t4, r4, e4 = _sampled_matvec(o, P4, T4, threads, 4.0, 10, op="V", basis=0)
+    c34 = {"config3_torus_p1_kelvin": {
+               "matvec_s_all_cores": t3, "extrapolated": True,
+               "sample": f"{r3} random node rows ({e3:.1f} s)"},
+           "config4_ico6_p0_V": {
+               "matvec_s_all_cores": t4, "extrapolated": True,
+               "block_matvec_64rhs_s": 64 * t4,
+               "sample": f"{r4} random triangle rows ({e4:.1f} s), one right-hand "
+                         "side per apply as the reference's operator, times 64"}}
     return {"value": 1.0 / (t_all * n_matvec), "unit": "solves/s",
             "cores": threads, "kind": "port",
             "value_1_thread": 1.0 / (t_one * n_matvec),
@@ -148,8 +166,9 @@ def cpu_baseline(P, T, n_matvec, seconds=10.0):
                       f"to full matvecs ({t_all:.1f} s / {t_one:.0f} s) x "
                       f"{n_matvec:.1f} FP64 matvec-equivalents per solve (the FP64 "
                       "GPU solve's operator and preconditioner applies); config 1 "
-                      "full matvecs and dense solve measured whole",
-            "config1_ico3_p0_V": c1}
+                      "full matvecs and dense solve measured whole; configs 3 "
+                      "and 4 row-sampled on all cores, extrapolated",
+            "config1_ico3_p0_V": c1, **c34}
 
 
 def secondary(L, meshes, solve_batched=False):
