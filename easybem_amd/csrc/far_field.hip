@@ -2663,61 +2663,27 @@ bool mts_eligible(const ebh_op *op)
   return op->rb && mrhs_tri(op) && (op->mesh->T.ne + 63) / 64 <= 65000;
 }
 
-// Passes of the pair-symmetric multi-RHS schedule: blocks of kMtSeg tile
-// rows are taken greedily while a pass's tiles and item sums stay within
-// the operator's partial limit (one block at least).  Items as described at
-// k_mrhs_tiles, column items on the same kMtSeg-row grid as the passes, and
-// each output's items in increasing source group: every output sees the
-// same sequence of additions whatever the passes, so y does not depend on
-// the limit.
+// Passes of the pair-symmetric multi-RHS schedule (mt_plan): blocks of
+// kMtSeg tile rows within the operator's partial limit, column items on the
+// same grid, each output's items in increasing source group, so every
+// output sees the same sequence of additions whatever the passes and y does
+// not depend on the limit.
 static int plan_mrhs_tiles(ebh_op *op)
 {
-  const int ne = op->mesh->T.ne, ng = (ne + 63) / 64;
   const bool f32 = op->d.far_precision == EBH_FP32;
-  const double tb = f32 ? 4.0 : 8.0;
   const size_t groups = (op->d.max_rhs + op->rb - 1) / op->rb;
-  const double ib = 8.0 * (double)(groups * op->rb * 64);
-  std::vector<int> rowoff(ng + 1, 0);
-  for (int I = 0; I < ng; I++) rowoff[I + 1] = rowoff[I] + (ng - I);
-  auto seg = [](int n) { return (n + kMtSeg - 1) / kMtSeg; };
-  auto bytes = [&](int I0, int I1) {
-    size_t n = 0;
-    for (int K = I0; K < I1; K++) n += seg(ng - K);
-    for (int K = I0 + 1; K < ng; K++) n += seg(std::min(I1, K) - I0);
-    return (double)(rowoff[I1] - rowoff[I0]) * 4096.0 * tb + (double)n * ib;
-  };
-  std::vector<int> item, kptr;
-  op->mt_rows.assign(1, 0);
-  op->mt_t0.assign(1, 0);
-  op->mt_ioff.assign(1, 0);
-  op->mt_koff.assign(1, 0);
-  op->mt_tiles = op->mt_items = 0;
-  for (int I0 = 0; I0 < ng;) {
-    int I1 = std::min(ng, I0 + kMtSeg);
-    while (I1 < ng && bytes(I0, std::min(ng, I1 + kMtSeg)) <= op->partial_limit)
-      I1 = std::min(ng, I1 + kMtSeg);
-    const int n0 = (int)(item.size() / 4);
-    for (int K = I0; K < ng; K++) {
-      kptr.push_back((int)(item.size() / 4) - n0);
-      const int c1 = std::min(I1, K);
-      for (int M0 = I0; M0 < c1; M0 += kMtSeg)
-        item.insert(item.end(), {K, 1, M0, std::min(c1, M0 + kMtSeg)});
-      if (K < I1)
-        for (int M0 = K; M0 < ng; M0 += kMtSeg)
-          item.insert(item.end(), {K, 0, M0, std::min(ng, M0 + kMtSeg)});
-    }
-    kptr.push_back((int)(item.size() / 4) - n0);
-    op->mt_rows.push_back(I1);
-    op->mt_t0.push_back(rowoff[I1]);
-    op->mt_ioff.push_back((int)(item.size() / 4));
-    op->mt_koff.push_back((int)kptr.size());
-    op->mt_tiles = std::max(op->mt_tiles, (size_t)(rowoff[I1] - rowoff[I0]));
-    op->mt_items = std::max(op->mt_items, item.size() / 4 - (size_t)n0);
-    I0 = I1;
-  }
-  EBH_TRY(op->mt_rowoff.upload(rowoff));
-  EBH_TRY(op->mt_item.upload(item));
-  EBH_TRY(op->mt_kptr.upload(kptr));
+  MtPlan p;
+  mt_plan(op->mesh->T.ne, 4096.0 * (f32 ? 4 : 8), 8.0 * (double)(groups * op->rb * 64),
+          op->partial_limit, kMtSeg, p);
+  op->mt_rows = p.rows;
+  op->mt_t0 = p.t0;
+  op->mt_ioff = p.ioff;
+  op->mt_koff = p.koff;
+  op->mt_tiles = p.max_tiles;
+  op->mt_items = p.max_items;
+  EBH_TRY(op->mt_rowoff.upload(p.rowoff));
+  EBH_TRY(op->mt_item.upload(p.item));
+  EBH_TRY(op->mt_kptr.upload(p.kptr));
   if (f32) {
     op->mt_tile64.release();
     EBH_TRY(op->mt_tile32.alloc(op->mt_tiles * 4096));

@@ -814,4 +814,47 @@ void surface_curls(const Topology &T, std::vector<double> &curl)
   }
 }
 
+void mt_plan(int ne, double tile_bytes, double item_bytes, double limit, int seg,
+             MtPlan &p)
+{
+  const int ng = (ne + 63) / 64;
+  p = MtPlan();
+  p.ng = ng;
+  p.rowoff.assign(ng + 1, 0);
+  for (int I = 0; I < ng; I++) p.rowoff[I + 1] = p.rowoff[I] + (ng - I);
+  auto nseg = [seg](int n) { return (n + seg - 1) / seg; };
+  auto bytes = [&](int I0, int I1) {
+    size_t n = 0;
+    for (int K = I0; K < I1; K++) n += nseg(ng - K);
+    for (int K = I0 + 1; K < ng; K++) n += nseg(std::min(I1, K) - I0);
+    return (double)(p.rowoff[I1] - p.rowoff[I0]) * tile_bytes + (double)n * item_bytes;
+  };
+  p.rows.assign(1, 0);
+  p.t0.assign(1, 0);
+  p.ioff.assign(1, 0);
+  p.koff.assign(1, 0);
+  for (int I0 = 0; I0 < ng;) {
+    int I1 = std::min(ng, I0 + seg);
+    while (I1 < ng && bytes(I0, std::min(ng, I1 + seg)) <= limit) I1 = std::min(ng, I1 + seg);
+    const int n0 = (int)(p.item.size() / 4);
+    for (int K = I0; K < ng; K++) {
+      p.kptr.push_back((int)(p.item.size() / 4) - n0);
+      const int c1 = std::min(I1, K);
+      for (int M0 = I0; M0 < c1; M0 += seg)
+        p.item.insert(p.item.end(), {K, 1, M0, std::min(c1, M0 + seg)});
+      if (K < I1)
+        for (int M0 = K; M0 < ng; M0 += seg)
+          p.item.insert(p.item.end(), {K, 0, M0, std::min(ng, M0 + seg)});
+    }
+    p.kptr.push_back((int)(p.item.size() / 4) - n0);
+    p.rows.push_back(I1);
+    p.t0.push_back(p.rowoff[I1]);
+    p.ioff.push_back((int)(p.item.size() / 4));
+    p.koff.push_back((int)p.kptr.size());
+    p.max_tiles = std::max(p.max_tiles, (size_t)(p.rowoff[I1] - p.rowoff[I0]));
+    p.max_items = std::max(p.max_items, p.item.size() / 4 - (size_t)n0);
+    I0 = I1;
+  }
+}
+
 }  // namespace ebh

@@ -103,4 +103,27 @@ struct NearCSR {
 void build_near_csr(const Topology &T, int test_basis, int trial_basis,
                     NearCSR &out);
 
+// Pass plan of the pair-symmetric multi-RHS schedule (far_field.hip,
+// k_mrhs_tiles / k_mrhs_tile_mfma) for ne triangles in groups of 64 (ng
+// groups).  Tiles are the group pairs (I, J), J >= I, row by row: tile (I,
+// J) has index rowoff[I] + J - I.  Pass p covers tile rows rows[p] ..
+// rows[p+1]-1 (blocks of `seg` rows, taken greedily while the pass's tiles
+// at tile_bytes and items at item_bytes stay within `limit`, one block at
+// least), its tiles from t0[p] on.  Its items (K, kind, M0, M1), 4 ints
+// each, are item[4 ioff[p]] .. item[4 ioff[p+1]] - 1: for each output group
+// K >= rows[p] the column items (kind 1: tiles (M, K), rows[p] <= M <
+// min(rows[p+1], K)) in segments of at most `seg` on the `seg`-row grid,
+// then, for K in the pass, the row items (kind 0: tiles (K, M), K <= M < ng)
+// in segments of at most `seg`; kptr[koff[p] + K - rows[p]] is K's first
+// item relative to ioff[p] (ng - rows[p] + 1 entries).  So every output's
+// items run through M = 0 .. ng-1 in the same segments, in the same order,
+// whatever the passes.  max_tiles / max_items: the largest pass's counts.
+struct MtPlan {
+  int ng = 0;
+  std::vector<int> rowoff, rows, t0, ioff, koff, item, kptr;
+  size_t max_tiles = 0, max_items = 0;
+};
+void mt_plan(int ne, double tile_bytes, double item_bytes, double limit, int seg,
+             MtPlan &p);
+
 }  // namespace ebh

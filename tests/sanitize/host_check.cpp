@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <string>
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "host_geometry.h"
@@ -118,10 +119,85 @@ void check_mesh(const std::vector<double> &P, const std::vector<int> &T, bool to
 
 }  // namespace
 
+// Pass plan of the pair-symmetric multi-RHS schedule (ebh::mt_plan): every
+// tile in exactly one pass, passes on the seg-row grid, each pass within the
+// limit unless it is one block, and every output's item segments the same,
+// in the same order, as in the one-pass plan (so y does not depend on the
+// limit).
+void check_mt_plan(int ne, double tile_bytes, double item_bytes, double limit, int seg)
+{
+  ebh::MtPlan p, one;
+  ebh::mt_plan(ne, tile_bytes, item_bytes, limit, seg, p);
+  ebh::mt_plan(ne, tile_bytes, item_bytes, 1e300, seg, one);
+  const int ng = (ne + 63) / 64, np_ = (int)p.rows.size() - 1;
+  expect(p.ng == ng && one.rows.size() == 2, "one pass without a limit");
+  expect(p.rows.front() == 0 && p.rows.back() == ng, "passes cover the tile rows");
+  expect(p.rowoff.back() == ng * (ng + 1) / 2, "tile count");
+  // per output group, the (M0, M1) segments in the order they are summed
+  std::vector<std::vector<std::pair<int, int>>> segs(ng), segs1(ng);
+  auto collect = [&](const ebh::MtPlan &q, std::vector<std::vector<std::pair<int, int>>> &out,
+                     bool check) {
+    for (size_t pp = 0; pp + 1 < q.rows.size(); pp++) {
+      const int I0 = q.rows[pp], I1 = q.rows[pp + 1];
+      if (check) {
+        expect(I1 > I0, "pass not empty");
+        if (pp + 2 < q.rows.size()) expect(I1 % seg == 0, "pass ends on the block grid");
+        expect(q.t0[pp] == q.rowoff[I0] && q.t0[pp + 1] == q.rowoff[I1], "pass tiles");
+        size_t nt = (size_t)(q.rowoff[I1] - q.rowoff[I0]);
+        size_t ni = (size_t)(q.ioff[pp + 1] - q.ioff[pp]);
+        const double b = nt * tile_bytes + ni * item_bytes;
+        expect(b <= limit || I1 - I0 <= seg, "pass within the limit or one block");
+        expect(nt <= q.max_tiles && ni <= q.max_items, "largest pass counts");
+        expect(q.koff[pp + 1] - q.koff[pp] == ng - I0 + 1, "item pointers per output");
+      }
+      const int *kp = &q.kptr[q.koff[pp]];
+      for (int K = I0; K < ng; K++) {
+        int lastM = -1;
+        for (int i = kp[K - I0]; i < kp[K - I0 + 1]; i++) {
+          const int *it = &q.item[4 * (size_t)(q.ioff[pp] + i)];
+          if (check) {
+            expect(it[0] == K, "item output group");
+            expect(it[3] > it[2] && it[3] - it[2] <= seg, "item length");
+            expect(it[2] > lastM, "items in increasing M");
+            if (it[1] == 1) {
+              expect(it[2] >= I0 && it[3] <= std::min(I1, K), "column item inside the pass");
+              expect(it[2] % seg == 0, "column item on the block grid");
+            } else {
+              expect(K < I1 && it[2] >= K && it[3] <= ng, "row item of a pass row");
+            }
+          }
+          lastM = it[3] - 1;
+          out[K].push_back({it[2], it[3]});
+        }
+      }
+    }
+  };
+  collect(p, segs, true);
+  collect(one, segs1, false);
+  for (int K = 0; K < ng; K++) {
+    int next = 0;
+    for (auto &sg : segs[K]) {
+      expect(sg.first == next, "output's items cover every source group once, in order");
+      next = sg.second;
+    }
+    expect(next == ng, "output's items reach the last group");
+    expect(segs[K] == segs1[K], "the same segments as one pass");
+  }
+}
+
 int main()
 {
   std::vector<double> P;
   std::vector<int> T;
+  // multi-RHS pass plans: icosphere L3 and L6 and a ragged mesh, limits
+  // from one block per pass to one pass; FP32 and FP64 tiles, 64 RHS
+  check_mt_plan(1280, 16384.0, 32768.0, 1.0, 32);
+  check_mt_plan(81920, 16384.0, 32768.0, 4.0 * (1 << 30), 32);
+  check_mt_plan(81920, 32768.0, 32768.0, 4.0 * (1 << 30), 32);
+  check_mt_plan(81920, 16384.0, 32768.0, 256.0 * (1 << 20), 32);
+  check_mt_plan(20480, 32768.0, 4096.0, 1 << 20, 32);
+  check_mt_plan(20000, 16384.0, 32768.0, 64.0 * (1 << 20), 32);
+  check_mt_plan(520, 16384.0, 2048.0, 1e300, 32);
   // valid meshes: torus grids (the threaded builders split their work
   // across host threads from a few hundred triangles up)
   torus(8, 4, P, T);

@@ -1,5 +1,7 @@
 """The host-side C++ and C under the sanitizers (CPU only): the threaded
-topology and near-field pattern builders (host_geometry.cpp) and the CPU
+topology and near-field pattern builders and the multi-RHS pass planner
+(host_geometry.cpp: every tile in one pass, each output's summation order
+the same for every partial limit) and the CPU
 oracle (oracle.c) built with AddressSanitizer + UndefinedBehaviorSanitizer,
 and the builders with ThreadSanitizer, running tests/sanitize/host_check.cpp
 on valid and invalid meshes.  A sanitizer report or a failed check fails the
