@@ -18,6 +18,10 @@ def rel(a, b):
 
 
 def mesh_case(name):
+    if name == "ico0":
+        return meshes.icosphere(0)          # 20 triangles: one partial group
+    if name == "ico1":
+        return meshes.icosphere(1)          # 80 triangles: a group and 16
     if name == "ico3":
         return meshes.icosphere(3)          # 1280 triangles: 20 whole groups
     if name == "torus":
@@ -36,7 +40,7 @@ def pair_ops(L, m, prec, max_rhs, **kw):
     return tgt, prs
 
 
-@pytest.mark.parametrize("name", ["ico3", "torus", "blob"])
+@pytest.mark.parametrize("name", ["ico0", "ico1", "ico3", "torus", "blob"])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-14), ("fp32", 5e-7)])
 def test_pairs_match_target_schedule(name, prec, tol):
     from easybem_amd import _lib as L
