@@ -257,9 +257,11 @@ def secondary(L, meshes, solve_batched=False):
             "far_tflops_convention": pts * (9 + 2 * 64) / tm["far"] / 1e9}
     if solve_batched:
         s = L.Solver(ops["fp64"], ops["fp32"], restart=50, batch=64)
-        # near-field sparse approximate inverse on 2 triangle rings
+        # near-field sparse approximate inverse on 3 triangle rings (at L6
+        # with 64 RHS: 10 block steps against 12 with 2 rings and 9 with 4;
+        # built in about 0.35 s against 0.19 and 1.15 s)
         t0 = time.perf_counter()
-        S = L.precond_spai(ops["fp64"], rings=2, max_rhs=64)
+        S = L.precond_spai(ops["fp64"], rings=3, max_rhs=64)
         r4["spai_build_s"] = time.perf_counter() - t0
         s.set_precond(S, S)
         B = rng.uniform(-1, 1, 64 * ops["fp64"].ncols)
@@ -311,7 +313,7 @@ def secondary(L, meshes, solve_batched=False):
         # `nseq` of the 64 systems timed
         n = ops["fp64"].ncols
         V1 = L.Operator(m, "V", 0, 0, "fp64", line_rule=4, tri_rule=4)
-        S1 = L.precond_spai(V1, rings=2)
+        S1 = L.precond_spai(V1, rings=3)
         s1 = L.Solver(V1, None, restart=50)
         s1.set_precond(S1, S1)
         nseq = 4
