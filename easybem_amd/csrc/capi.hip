@@ -58,7 +58,8 @@ size_t ebh_op::bytes() const
          tsl64.bytes() + tc32.bytes() + dinv.bytes() + mass_ptr.bytes() +
          mass_col.bytes() + mass_val.bytes() + cheb_r.bytes() + cheb_d.bytes() +
          cheb_t.bytes() + chebx.bytes() + mt_rowoff.bytes() + mt_item.bytes() +
-         mt_kptr.bytes() + mt_tile32.bytes() + mt_tile64.bytes() + mt_part.bytes();
+         mt_kptr.bytes() + mt_order.bytes() + mt_tile32.bytes() + mt_tile64.bytes() +
+         mt_part.bytes();
 }
 
 extern "C" {

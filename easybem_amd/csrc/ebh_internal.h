@@ -233,7 +233,7 @@ struct ebh_op {
   bool mts = false;
   std::vector<int> mt_rows, mt_t0, mt_ioff, mt_koff;
   size_t mt_tiles = 0, mt_items = 0;
-  ebh::DevBuf<int> mt_rowoff, mt_item, mt_kptr;
+  ebh::DevBuf<int> mt_rowoff, mt_item, mt_kptr, mt_order;
   ebh::DevBuf<float> mt_tile32;
   ebh::DevBuf<double> mt_tile64, mt_part;
   // hypersingular (EBH_LAPLACE_HS): the wrapped P0 single layer, the curls

@@ -1452,6 +1452,7 @@ struct MtArgs {
   int ng, I0, I1, t0, ntiles, items, groups, first;
   const int *rowoff;  // [ng + 1] first tile of each row
   const int4 *item;   // (K, kind, M0, M1) of the pass's items
+  const int *order;   // the items in launch order (longest first)
   const int *kptr;    // [ng - I0 + 1]: items of output group I0 + k
   double *part;       // [items][groups][RB][64] item sums
 };
@@ -1569,8 +1570,9 @@ template <typename R, int RB>
 __global__ __launch_bounds__(256) void k_mrhs_tile_mfma(FarMArgs<R> a, MtArgs m,
                                                         const R *__restrict__ tiles)
 {
-  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  if (w >= m.items) return;  // wave-uniform; no barrier in this kernel
+  const int wl = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (wl >= m.items) return;  // wave-uniform; no barrier in this kernel
+  const int w = m.order[wl];
   const int4 it = m.item[w];
   if (it.y == 0) mfma_item<R, RB, true>(a, m, tiles, w, it);
   else mfma_item<R, RB, false>(a, m, tiles, w, it);
@@ -2452,6 +2454,7 @@ int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
     m.groups = a.groups;
     m.first = p == 0;
     m.item = reinterpret_cast<const int4 *>(op->mt_item.p) + op->mt_ioff[p];
+    m.order = op->mt_order.p + op->mt_ioff[p];
     m.kptr = op->mt_kptr.p + op->mt_koff[p];
     m.part = op->mt_part.p;
     // far-kernel profiling records the tile launches (the VALU-bound kernel);
@@ -2684,6 +2687,7 @@ static int plan_mrhs_tiles(ebh_op *op)
   EBH_TRY(op->mt_rowoff.upload(p.rowoff));
   EBH_TRY(op->mt_item.upload(p.item));
   EBH_TRY(op->mt_kptr.upload(p.kptr));
+  EBH_TRY(op->mt_order.upload(p.order));
   if (f32) {
     op->mt_tile64.release();
     EBH_TRY(op->mt_tile32.alloc(op->mt_tiles * 4096));

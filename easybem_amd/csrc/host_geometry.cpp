@@ -847,6 +847,18 @@ void mt_plan(int ne, double tile_bytes, double item_bytes, double limit, int seg
           p.item.insert(p.item.end(), {K, 0, M0, std::min(ng, M0 + seg)});
     }
     p.kptr.push_back((int)(p.item.size() / 4) - n0);
+    {
+      // launch order: the longest items first, so that the short ones fill
+      // the last round of waves
+      const int ni = (int)(p.item.size() / 4) - n0;
+      std::vector<int> ord(ni);
+      for (int i = 0; i < ni; i++) ord[i] = i;
+      const int *it = p.item.data() + 4 * (size_t)n0;
+      std::stable_sort(ord.begin(), ord.end(), [it](int x, int y) {
+        return it[4 * x + 3] - it[4 * x + 2] > it[4 * y + 3] - it[4 * y + 2];
+      });
+      p.order.insert(p.order.end(), ord.begin(), ord.end());
+    }
     p.rows.push_back(I1);
     p.t0.push_back(p.rowoff[I1]);
     p.ioff.push_back((int)(p.item.size() / 4));

@@ -117,10 +117,12 @@ void build_near_csr(const Topology &T, int test_basis, int trial_basis,
 // in segments of at most `seg`; kptr[koff[p] + K - rows[p]] is K's first
 // item relative to ioff[p] (ng - rows[p] + 1 entries).  So every output's
 // items run through M = 0 .. ng-1 in the same segments, in the same order,
-// whatever the passes.  max_tiles / max_items: the largest pass's counts.
+// whatever the passes.  order[ioff[p] ..]: the pass's items (indices
+// relative to ioff[p]) in launch order, longer items first, ties in list
+// order.  max_tiles / max_items: the largest pass's counts.
 struct MtPlan {
   int ng = 0;
-  std::vector<int> rowoff, rows, t0, ioff, koff, item, kptr;
+  std::vector<int> rowoff, rows, t0, ioff, koff, item, kptr, order;
   size_t max_tiles = 0, max_items = 0;
 };
 void mt_plan(int ne, double tile_bytes, double item_bytes, double limit, int seg,

@@ -149,6 +149,19 @@ void check_mt_plan(int ne, double tile_bytes, double item_bytes, double limit, i
         expect(b <= limit || I1 - I0 <= seg, "pass within the limit or one block");
         expect(nt <= q.max_tiles && ni <= q.max_items, "largest pass counts");
         expect(q.koff[pp + 1] - q.koff[pp] == ng - I0 + 1, "item pointers per output");
+        // the launch order is a permutation of the pass's items, longest first
+        std::vector<char> seen(ni, 0);
+        int prev = 1 << 30;
+        for (size_t i = 0; i < ni; i++) {
+          const int o = q.order[q.ioff[pp] + i];
+          const bool ok = o >= 0 && (size_t)o < ni && !seen[o];
+          expect(ok, "launch order is a permutation");
+          if (!ok) break;
+          seen[o] = 1;
+          const int *it = &q.item[4 * (size_t)(q.ioff[pp] + o)];
+          expect(it[3] - it[2] <= prev, "longest items first");
+          prev = it[3] - it[2];
+        }
       }
       const int *kp = &q.kptr[q.koff[pp]];
       for (int K = I0; K < ng; K++) {
