@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (datasheet)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64 (datasheet)
 PEAK_FP64_MFMA_TFLOPS = 78.6  # MI355X dense FP64 matrix (datasheet)
+# device bytes the config-4 operators may keep their tiles in (the FP64
+# operator's 26.9 GB at icosphere L6 fit; the card holds 288 GB)
+RESIDENT_TILE_LIMIT = 48 << 30
 PEAK_HBM_GBPS = 8000.0     # MI355X HBM3E (datasheet)
 FLOP_SL = 11               # per far interaction, single layer, one RHS
 # HBM bytes per launch of the FP32 single-layer far kernel on the level-5
@@ -334,6 +337,41 @@ def secondary(L, meshes, solve_batched=False):
             "note": "plain FP64 path: one system at a time, single-RHS FP64 operator"}
         r4["block_mixed_speedup_vs_sequential_fp64"] = (
             r4["block_mixed_solve"]["solves_per_s"] * dts)
+        # resident tiles: each operator's tiles formed once and kept in HBM
+        # (a stored far field, 8 or 16 bytes per unordered triangle pair);
+        # every apply is the FP64 matrix-core products and their reduction
+        res = {}
+        B = rng.uniform(-1, 1, 64 * ops["fp64"].ncols)
+        x = L.DeviceArray.from_host(B)
+        y = L.DeviceArray(ops["fp64"].nrows * 64)
+        for prec in ("fp32", "fp64"):
+            op = ops[prec]
+            t0 = time.perf_counter()
+            op.set_resident_tiles(RESIDENT_TILE_LIMIT)
+            build_s = time.perf_counter() - t0
+            ok, nbytes = op.resident_tiles
+            tm = op.time(x, y, reps=3, nrhs=64)
+            tri_pairs = op.work()["far_interactions"] / 49
+            res[prec] = {"resident": ok, "tile_bytes": nbytes, "build_s": build_s,
+                         "block_matvec_ms": tm["total"], "products_ms": tm["far"],
+                         "products_mfma_frac": tri_pairs * 2 * 64 / tm["far"] / 1e9
+                         / PEAK_FP64_MFMA_TFLOPS}
+        sb = L.Solver(ops["fp64"], None, restart=50, batch=64)
+        sb.set_precond(S, S)
+        sb.set_block(True)
+        for k in range(2):
+            dx = L.DeviceArray.from_host(np.zeros_like(B))
+            t0 = time.perf_counter()
+            st = sb.solve_device(db, dx, tol=1e-8, mixed=False, nb=64, max_iters=3000)
+            dtr = time.perf_counter() - t0
+        sb.close()
+        res["block_fp64_solve"] = {
+            "seconds": dtr, "solves_per_s": 64 / dtr,
+            "block_steps": max(v["iters"] for v in st),
+            "relres_max": max(v["relres"] for v in st),
+            "note": "second solve on the solver; tiles formed before it"}
+        res["speedup_vs_matrix_free_block_mixed"] = r4["block_mixed_solve"]["seconds"] / dtr
+        r4["resident_tiles"] = res
     out["cfg4_ico6_p0_V_64rhs"] = r4
     out["multi_mesh"] = multi_mesh(L, meshes)
     return out
@@ -512,18 +550,21 @@ def pmc_probe(L, meshes, level):
 
 def mrhs_compare(L, meshes, level=6, reps=2):
     """Config 4's 64-RHS block matvec (icosphere `level`, P0 V, 7-point far
-    rule) under both multi-RHS schedules, FP32 and FP64: per-apply times
-    (HIP events, ebh_op_time) and the pair schedule's largest deviation from
-    the target schedule, one JSON line each."""
+    rule) under both multi-RHS schedules and with the pair schedule's tiles
+    resident, FP32 and FP64: per-apply times (HIP events, ebh_op_time) and
+    the pair schedule's largest deviation from the target schedule, one JSON
+    line each."""
     P, T = meshes.icosphere(level)
     m = L.Mesh(P, T)
     n = T.shape[0]
     X = np.random.default_rng(1).uniform(-1, 1, 64 * n)
     for prec in ("fp32", "fp64"):
         ys = {}
-        for sched in ("target", "pairs"):
+        for sched in ("target", "pairs", "resident"):
             op = L.Operator(m, "V", 0, 0, prec, line_rule=4, tri_rule=4, max_rhs=64)
-            op.set_mrhs_schedule(sched)
+            op.set_mrhs_schedule("target" if sched == "target" else "pairs")
+            if sched == "resident":
+                op.set_resident_tiles(RESIDENT_TILE_LIMIT)
             x = L.DeviceArray.from_host(X)
             y = L.DeviceArray(64 * n)
             tm = op.time(x, y, reps=reps, nrhs=64)
@@ -533,7 +574,9 @@ def mrhs_compare(L, meshes, level=6, reps=2):
             del op
         a, b = ys["target"], ys["pairs"]
         e = max(np.linalg.norm(a[r] - b[r]) / np.linalg.norm(a[r]) for r in range(64))
-        print(json.dumps({"precision": prec, "pairs_vs_target_max_rel": e}), flush=True)
+        print(json.dumps({"precision": prec, "pairs_vs_target_max_rel": e,
+                          "resident_bitwise_equal": bool(np.array_equal(
+                              ys["resident"], ys["pairs"]))}), flush=True)
 
 
 def main():

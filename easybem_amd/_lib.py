@@ -73,6 +73,8 @@ def lib():
         L.ebh_op_get_reduction.argtypes = [vp, C.POINTER(C.c_int)]
         L.ebh_op_set_mrhs_schedule.argtypes = [vp, C.c_int]
         L.ebh_op_get_mrhs_schedule.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ebh_op_set_resident_tiles.argtypes = [vp, C.c_int64]
+        L.ebh_op_get_resident_tiles.argtypes = [vp, C.POINTER(C.c_int), i64p]
         L.ebh_op_time.argtypes = [vp, vp, vp, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
         L.ebh_op_work.argtypes = [vp, i64p]
@@ -281,6 +283,19 @@ class Operator:
         v, k = C.c_int(), C.c_int()
         check(lib().ebh_op_get_mrhs_schedule(self.h, C.byref(v), C.byref(k)))
         return {1: "target", 2: "pairs"}[v.value], k.value
+
+    def set_resident_tiles(self, max_bytes):
+        """Keep the "pairs" schedule's tiles in device memory (formed once,
+        now) when they and the item sums fit in max_bytes; 0 turns it off.
+        The far field is then stored, not recomputed per apply."""
+        check(lib().ebh_op_set_resident_tiles(self.h, int(max_bytes)))
+
+    @property
+    def resident_tiles(self):
+        """(tiles resident, their bytes)."""
+        v, nb = C.c_int(), C.c_int64()
+        check(lib().ebh_op_get_resident_tiles(self.h, C.byref(v), C.byref(nb)))
+        return bool(v.value), nb.value
 
     @property
     def fused(self):

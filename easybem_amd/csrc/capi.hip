@@ -477,6 +477,34 @@ int ebh_op_set_mrhs_schedule(ebh_op *op, int schedule)
   return EBH_OK;
 }
 
+int ebh_op_set_resident_tiles(ebh_op *op, int64_t max_bytes)
+{
+  if (!op) { set_error("null op"); return EBH_ERR_ARG; }
+  if (max_bytes < 0) { set_error("resident tile limit must be >= 0"); return EBH_ERR_ARG; }
+  if (no_far(op)) return EBH_ERR_UNSUPPORTED;
+  ebh_op *f = far_op(op);
+  if (max_bytes > 0 && !(f->rb && f->mts)) {
+    set_error("resident tiles need the pair-symmetric multi-RHS schedule (a P0 single "
+              "layer with max_rhs > 1 and a far rule of at most 7 points)");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  f->mt_res_limit = (double)max_bytes;
+  if (f->rb) EBH_TRY(op_configure(f, f->block, f->tile, f->sym ? f->ksplit : f->chunks, f->sym));
+  if (f != op) op->gen++;
+  return EBH_OK;
+}
+
+int ebh_op_get_resident_tiles(const ebh_op *op, int *resident, int64_t *bytes)
+{
+  if (!op || !resident) { set_error("null argument"); return EBH_ERR_ARG; }
+  const ebh_op *f = far_op(op);
+  *resident = f->mt_res ? 1 : 0;
+  if (bytes)
+    *bytes = f->mt_res ? (int64_t)(f->mt_tile32.bytes() + f->mt_tile64.bytes()) : 0;
+  return EBH_OK;
+}
+
 int ebh_op_get_mrhs_schedule(const ebh_op *op, int *schedule, int *passes)
 {
   if (!op || !schedule) { set_error("null argument"); return EBH_ERR_ARG; }

@@ -228,9 +228,13 @@ struct ebh_op {
   // mt_rows[p+1]-1 form pass p, whose items are mt_ioff[p] .. mt_ioff[p+1]-1
   // of mt_item, with per-group item pointers at mt_koff[p] of mt_kptr;
   // the first tile of pass p is mt_t0[p]; mt_tiles / mt_items: the largest
-  // pass's tiles and items
+  // pass's tiles and items.  Resident tiles: when every tile and item fits
+  // in mt_res_limit bytes (0: off), one pass whose tiles are formed once at
+  // configuration (mt_res once they are) and reused by every apply
   int mrhs_sched = 0;
   bool mts = false;
+  double mt_res_limit = 0.0;
+  bool mt_res = false;
   std::vector<int> mt_rows, mt_t0, mt_ioff, mt_koff;
   size_t mt_tiles = 0, mt_items = 0;
   ebh::DevBuf<int> mt_rowoff, mt_item, mt_kptr, mt_order;

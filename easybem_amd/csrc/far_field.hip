@@ -1273,17 +1273,26 @@ __global__ __launch_bounds__(256) void k_far_mrhs(FarMArgs<R> a)
 // r^2 of each point pair from triangle-pair terms as in the pair-symmetric
 // kernel, |D|^2 + E_q + E'_b + c_q D.F_k - c'_b D.F'_l - (c_q c'_b / 2) F_k.F'_l
 // with D = C - C' and F = 2 (P - C): one add and one FMA per point pair.
-// The source record of triangle j is then C', F'_0, F'_1, E'_b, w_b
-// ([ne][MedRec]) instead of its points.
-template <int NT> struct MedRec { static constexpr int W = (9 + 2 * NT + 3) / 4 * 4; };
+// The source record of triangle j is then C', F'_0, F'_1, E'_b, w_b and, in
+// FP32, the rounding remainder C' - (float)C' ([ne][MedRec]) instead of its
+// points; FP32 forms D as (C_hi - C'_hi) + (C_lo - C'_lo), so that D carries
+// no rounding of the positions (systematic for each triangle pair).  The
+// record fits the 4 values per point of the point form.
+template <int NT> struct MedRec {
+  static constexpr int LO = 9 + 2 * NT;
+  static constexpr int W = (LO + 3 + 3) / 4 * 4;
+  static_assert(LO + 3 <= W && W <= 4 * NT, "median record width");
+};
 
 // A target triangle of the triangle-pair kernels: its points (MED: its
 // frame and the E_q, c_q of its points) and weights in registers.  sum(a, j,
 // keep) is G of the pair with source triangle j, 0 when keep is 0.
 template <typename R, int NT, bool MED>
 struct TriTarget {
+  static constexpr bool HL = MED && sizeof(R) == 4;
   R X[NT], Y[NT], Z[NT], W[NT];
   R Cx = 0, Cy = 0, Cz = 0, F0x = 0, F0y = 0, F0z = 0, F1x = 0, F1y = 0, F1z = 0;
+  R Lx = 0, Ly = 0, Lz = 0;  // FP32: C - (float)C
 
   __device__ __forceinline__ void load(const FarMArgs<R> &a, int tt)
   {
@@ -1312,6 +1321,11 @@ struct TriTarget {
       F1y = a.tsl[7 * (size_t)ne + tt];
       F1z = a.tsl[8 * (size_t)ne + tt];
     }
+    if constexpr (HL) {
+      Lx = a.tsl[(9 + NT) * (size_t)ne + tt];
+      Ly = a.tsl[(10 + NT) * (size_t)ne + tt];
+      Lz = a.tsl[(11 + NT) * (size_t)ne + tt];
+    }
   }
 
   __device__ __forceinline__ R sum(const FarMArgs<R> &a, int j, R keep) const
@@ -1320,7 +1334,13 @@ struct TriTarget {
     if constexpr (MED) {
       constexpr int C0 = NT % 3;
       const R *gp = a.geom + (size_t)j * MedRec<NT>::W;
-      const R Dx = Cx - gp[0], Dy = Cy - gp[1], Dz = Cz - gp[2];
+      R Dx = Cx - gp[0], Dy = Cy - gp[1], Dz = Cz - gp[2];
+      if constexpr (HL) {
+        constexpr int LO = MedRec<NT>::LO;
+        Dx += Lx - gp[LO];
+        Dy += Ly - gp[LO + 1];
+        Dz += Lz - gp[LO + 2];
+      }
       // a touching pair's expanded r^2 can round below zero at coincident
       // points: lift it far away before the rsqrt (and multiply by 0 after)
       const R excl = keep == R(0) ? R(1e30) : R(0);
@@ -1618,8 +1638,13 @@ __global__ void k_fold_mrhs(int ne, int nt, int nrhs, int groups,
     // median record of source triangle j (k_far_mrhs_tri, MED)
     if (g == 0) {
       R *o = geom + (size_t)j * med;
-      if (b == 0)
+      if (b == 0) {
         for (int f = 0; f < 9; f++) o[f] = tsl[(size_t)f * ne + j];
+        // FP32: the centroid's rounding remainders (target-frame rows
+        // 9 + nt .. 11 + nt) after the weights
+        if constexpr (sizeof(R) == 4)
+          for (int k = 0; k < 3; k++) o[9 + 2 * nt + k] = tsl[(size_t)(9 + nt + k) * ne + j];
+      }
       o[9 + b] = tsl[(size_t)(9 + b) * ne + j];
       o[9 + nt + b] = (R)swt[p];
     }
@@ -2432,8 +2457,30 @@ int launch_mt_apply(dim3 ga, dim3 gr, hipStream_t s, const FarMArgs<R> &a, const
   return EBH_OK;
 }
 
+// Arguments of pass p of the pair-symmetric multi-RHS schedule.
+MtArgs mt_args(const ebh_op *op, int p, int groups)
+{
+  MtArgs m;
+  m.ng = (op->mesh->T.ne + 63) / 64;
+  m.I0 = op->mt_rows[p];
+  m.I1 = op->mt_rows[p + 1];
+  m.rowoff = op->mt_rowoff.p;
+  m.t0 = op->mt_t0[p];
+  m.ntiles = op->mt_t0[p + 1] - m.t0;
+  m.items = op->mt_ioff[p + 1] - op->mt_ioff[p];
+  m.groups = groups;
+  m.first = p == 0;
+  m.item = reinterpret_cast<const int4 *>(op->mt_item.p) + op->mt_ioff[p];
+  m.order = op->mt_order.p + op->mt_ioff[p];
+  m.kptr = op->mt_kptr.p + op->mt_koff[p];
+  m.part = op->mt_part.p;
+  return m;
+}
+
 // The passes of the pair-symmetric multi-RHS schedule: per pass the tiles,
-// the items and their reduction into tri_out ([nrhs][ne] far sums).
+// the items and their reduction into tri_out ([nrhs][ne] far sums).  With
+// resident tiles (one pass, formed when the operator was configured) only
+// the products and the reduction run.
 template <typename R>
 int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
 {
@@ -2443,26 +2490,15 @@ int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
   else tiles = op->mt_tile64.p;
   const int np = (int)op->mt_rows.size() - 1;
   for (int p = 0; p < np; p++) {
-    MtArgs m;
-    m.ng = ng;
-    m.I0 = op->mt_rows[p];
-    m.I1 = op->mt_rows[p + 1];
-    m.rowoff = op->mt_rowoff.p;
-    m.t0 = op->mt_t0[p];
-    m.ntiles = op->mt_t0[p + 1] - m.t0;
-    m.items = op->mt_ioff[p + 1] - op->mt_ioff[p];
-    m.groups = a.groups;
-    m.first = p == 0;
-    m.item = reinterpret_cast<const int4 *>(op->mt_item.p) + op->mt_ioff[p];
-    m.order = op->mt_order.p + op->mt_ioff[p];
-    m.kptr = op->mt_kptr.p + op->mt_koff[p];
-    m.part = op->mt_part.p;
-    // far-kernel profiling records the tile launches (the VALU-bound kernel);
-    // the matrix-core products and reductions follow outside the pairs
+    const MtArgs m = mt_args(op, p, a.groups);
+    // far-kernel profiling records the tile launches (the VALU-bound kernel)
+    // and the matrix-core products and reductions follow outside the pairs;
+    // with resident tiles it records the products and reductions
     const bool prof = op->prof_on && 2 * (op->prof_count + 1) <= (int)op->prof_ev.size();
     if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count], s));
-    EBH_TRY(launch_mt_tiles<R>(op->rule.nt, med, dim3((m.ntiles + 3) / 4), s, a, m, tiles));
-    if (prof) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
+    if (!op->mt_res)
+      EBH_TRY(launch_mt_tiles<R>(op->rule.nt, med, dim3((m.ntiles + 3) / 4), s, a, m, tiles));
+    if (prof && !op->mt_res) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
     const dim3 ga((m.items + 3) / 4, a.groups), gr((ng - m.I0) * (rb / 4), a.groups);
     switch (rb) {
     case 64: EBH_TRY((launch_mt_apply<R, 64>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
@@ -2470,6 +2506,7 @@ int run_mrhs_tiles(ebh_op *op, const FarMArgs<R> &a, bool med, hipStream_t s)
     case 16: EBH_TRY((launch_mt_apply<R, 16>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
     default: EBH_TRY((launch_mt_apply<R, 32>(ga, gr, s, a, m, tiles, op->tri_out.p))); break;
     }
+    if (prof && op->mt_res) EBH_HIP(hipEventRecord(op->prof_ev[2 * op->prof_count++ + 1], s));
   }
   return EBH_OK;
 }
@@ -2493,11 +2530,14 @@ int launch_mrhs_nt(int nt, dim3 grid, hipStream_t s, const FarMArgs<R> &a)
   return EBH_OK;
 }
 
+// The fold of the multi-RHS kernels (source records, strengths of the nrhs
+// right-hand sides; nrhs 0 writes the records and zero strengths and reads
+// no x) and their arguments.
 template <typename R>
-int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
+int mrhs_setup(ebh_op *op, const double *x, int nrhs, hipStream_t s, FarMArgs<R> &a)
 {
   const int ne = op->mesh->T.ne, nt = op->rule.nt, rb = op->rb;
-  const int groups = (nrhs + rb - 1) / rb;
+  const int groups = std::max(1, (nrhs + rb - 1) / rb);
   R *geom, *sig;
   if constexpr (sizeof(R) == 4) { geom = op->mgeom32.p; sig = op->msig32.p; }
   else { geom = op->mgeom64.p; sig = op->msig64.p; }
@@ -2519,7 +2559,6 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
   }
 #undef EBH_FOLD
   EBH_HIP(hipGetLastError());
-  FarMArgs<R> a;
   a.ne = ne;
   a.chunks = op->mchunks;
   a.chunk = op->mchunk;
@@ -2551,6 +2590,35 @@ int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
     a.phi[q][1] = in ? op->rule.x[q] : 0.0;
     a.phi[q][2] = in ? op->rule.y[q] : 0.0;
   }
+  return EBH_OK;
+}
+
+// Resident tiles: the one pass's tiles formed once, on the operator's
+// stream, for every later apply to reuse.
+template <typename R>
+int build_resident_tiles(ebh_op *op)
+{
+  FarMArgs<R> a;
+  EBH_TRY(mrhs_setup<R>(op, nullptr, 0, op->stream, a));
+  R *tiles;
+  if constexpr (sizeof(R) == 4) tiles = op->mt_tile32.p;
+  else tiles = op->mt_tile64.p;
+  const MtArgs m = mt_args(op, 0, a.groups);
+  EBH_TRY(launch_mt_tiles<R>(op->rule.nt, mrhs_med(op) > 0, dim3((m.ntiles + 3) / 4),
+                             op->stream, a, m, tiles));
+  EBH_HIP(hipStreamSynchronize(op->stream));
+  op->mt_res = true;
+  return EBH_OK;
+}
+
+template <typename R>
+int run_mrhs(ebh_op *op, const double *x, int nrhs, hipStream_t s)
+{
+  const int ne = op->mesh->T.ne, nt = op->rule.nt, rb = op->rb;
+  const int groups = (nrhs + rb - 1) / rb;
+  const int med = mrhs_med(op);
+  FarMArgs<R> a;
+  EBH_TRY(mrhs_setup<R>(op, x, nrhs, s, a));
   const int tpb = mrhs_tri(op) ? 256 : 256 / nt;
   const int tbn = (ne + tpb - 1) / tpb;
   dim3 grid((unsigned)(tbn * groups * op->mchunks));
@@ -2670,14 +2738,24 @@ bool mts_eligible(const ebh_op *op)
 // kMtSeg tile rows within the operator's partial limit, column items on the
 // same grid, each output's items in increasing source group, so every
 // output sees the same sequence of additions whatever the passes and y does
-// not depend on the limit.
-static int plan_mrhs_tiles(ebh_op *op)
+// not depend on the limit.  *resident: every tile and item fits in the
+// resident limit, planned as one pass.
+static int plan_mrhs_tiles(ebh_op *op, bool *resident)
 {
   const bool f32 = op->d.far_precision == EBH_FP32;
   const size_t groups = (op->d.max_rhs + op->rb - 1) / op->rb;
+  const double tile_bytes = 4096.0 * (f32 ? 4 : 8);
+  const double item_bytes = 8.0 * (double)(groups * op->rb * 64);
   MtPlan p;
-  mt_plan(op->mesh->T.ne, 4096.0 * (f32 ? 4 : 8), 8.0 * (double)(groups * op->rb * 64),
-          op->partial_limit, kMtSeg, p);
+  *resident = false;
+  if (op->mt_res_limit > 0) {
+    mt_plan(op->mesh->T.ne, tile_bytes, item_bytes, op->mt_res_limit, kMtSeg, p);
+    const double all = (double)p.rowoff.back() * tile_bytes +
+                       (double)(p.item.size() / 4) * item_bytes;
+    *resident = p.rows.size() == 2 && all <= op->mt_res_limit;
+  }
+  if (!*resident)
+    mt_plan(op->mesh->T.ne, tile_bytes, item_bytes, op->partial_limit, kMtSeg, p);
   op->mt_rows = p.rows;
   op->mt_t0 = p.t0;
   op->mt_ioff = p.ioff;
@@ -2701,6 +2779,7 @@ static int plan_mrhs_tiles(ebh_op *op)
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
 {
   const int ne = op->mesh->T.ne;
+  bool resident = false;
   if (sym) {
     if (!sym_eligible(op)) {
       set_error("the pair-symmetric schedule needs a Laplace or Kelvin single layer with "
@@ -2772,8 +2851,9 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     op->mchunk = (ne + mc - 1) / mc;
     op->mchunks = (ne + op->mchunk - 1) / op->mchunk;
     op->mts = op->mrhs_sched != EBH_MRHS_TARGET && mts_eligible(op);
+    op->mt_res = false;
     if (op->mts) {
-      EBH_TRY(plan_mrhs_tiles(op));
+      EBH_TRY(plan_mrhs_tiles(op, &resident));
     } else {
       op->mt_tile32.release();
       op->mt_tile64.release();
@@ -2791,6 +2871,10 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
   }
   EBH_TRY(op->partial.alloc(need));
   op->gen++;
+  if (resident) {
+    if (op->d.far_precision == EBH_FP32) EBH_TRY(build_resident_tiles<float>(op));
+    else EBH_TRY(build_resident_tiles<double>(op));
+  }
   return EBH_OK;
 }
 

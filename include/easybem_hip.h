@@ -198,6 +198,19 @@ int ebh_op_get_partial(const ebh_op *op, int64_t *limit, int *passes, int64_t *b
 #define EBH_MRHS_PAIRS 2
 int ebh_op_set_mrhs_schedule(ebh_op *op, int schedule);
 int ebh_op_get_mrhs_schedule(const ebh_op *op, int *schedule, int *passes);
+/* Resident tiles of the EBH_MRHS_PAIRS schedule: when every tile and item
+ * sum of the operator fits in max_bytes, the tiles are formed once, here
+ * (synchronously, on the op's stream), and kept in device memory; every
+ * later apply runs only the tile products and their reduction, in one pass,
+ * and y is bitwise that of the apply without them.  The far field is then
+ * a stored operator, not a matrix-free one: 8 (FP32) or 16 (FP64) bytes per
+ * unordered triangle pair, 13.4 / 26.9 GB for 81920 triangles.  0 turns it
+ * off (the default); a limit the tiles do not fit in leaves the passes of
+ * the partial limit.  Fails with EBH_ERR_UNSUPPORTED for an operator
+ * without that schedule.  get: whether the tiles are resident and their
+ * bytes.  No counterpart in the reference, which is matrix-free. */
+int ebh_op_set_resident_tiles(ebh_op *op, int64_t max_bytes);
+int ebh_op_get_resident_tiles(const ebh_op *op, int *resident, int64_t *bytes);
 /* Averages over the last `ebh_op_time` calls: milliseconds per apply of
  * [0] fold, [1] far field, [2] reduce + near field, [3] whole apply;
  * measured with HIP events on the op's stream over `reps` applies of the

@@ -150,3 +150,73 @@ def test_pairs_schedule_rejected_where_it_does_not_apply():
     big_rule = L.Operator(m, "V", 0, 0, "fp64", max_rhs=8, line_rule=4, tri_rule=6)
     with pytest.raises(RuntimeError):
         big_rule.set_mrhs_schedule("pairs")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_resident_tiles_equal_per_apply_tiles(prec):
+    """Resident tiles (formed once, when enabled) against tiles formed in
+    every apply: y bitwise equal in one pass and against a 10-pass apply,
+    for repeated applies and a different number of right-hand sides; a
+    limit the tiles do not fit in leaves the passes; 0 turns it off."""
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(5)
+    m = L.Mesh(P, T)
+    ref = L.Operator(m, "V", 0, 0, prec, max_rhs=16)
+    ref.set_mrhs_schedule("pairs")
+    ref.set_partial_limit(1 << 20)
+    op = L.Operator(m, "V", 0, 0, prec, max_rhs=16)
+    assert op.resident_tiles == (False, 0)
+    op.set_resident_tiles(1 << 40)
+    res, nbytes = op.resident_tiles
+    ng = (T.shape[0] + 63) // 64
+    print(f"case {prec}: resident {res}, {nbytes} bytes, passes {op.mrhs_schedule[1]}",
+          flush=True)
+    assert res and op.mrhs_schedule == ("pairs", 1)
+    assert nbytes == ng * (ng + 1) // 2 * 4096 * (4 if prec == "fp32" else 8)
+    rng = np.random.default_rng(17)
+    for nrhs in (16, 16, 5):
+        X = rng.uniform(-1, 1, (nrhs, T.shape[0]))
+        assert np.array_equal(op(X.ravel()), ref(X.ravel()))
+    # a limit the tiles do not fit in: per-apply tiles in passes again
+    op.set_partial_limit(1 << 20)
+    op.set_resident_tiles(1 << 24)
+    assert op.resident_tiles == (False, 0) and op.mrhs_schedule == ("pairs", 10)
+    X = rng.uniform(-1, 1, (16, T.shape[0]))
+    Y = ref(X.ravel())
+    assert np.array_equal(op(X.ravel()), Y)
+    op.set_resident_tiles(1 << 40)
+    assert op.resident_tiles[0]
+    assert np.array_equal(op(X.ravel()), Y)
+    op.set_resident_tiles(0)
+    assert op.resident_tiles == (False, 0) and op.mrhs_schedule[1] == 10
+    assert np.array_equal(op(X.ravel()), Y)
+
+
+def test_resident_tiles_in_block_solves():
+    """A block GMRES solve (captured block steps, replayed) with resident
+    tiles returns bitwise the iterates of the solve that forms its tiles in
+    every apply, in FP64 and in mixed precision."""
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    B = np.random.default_rng(23).uniform(-1, 1, (8, T.shape[0]))
+    out = {}
+    for res in (False, True):
+        V64 = L.Operator(m, "V", 0, 0, "fp64", max_rhs=8)
+        V32 = L.Operator(m, "V", 0, 0, "fp32", max_rhs=8)
+        if res:
+            V64.set_resident_tiles(1 << 34)
+            V32.set_resident_tiles(1 << 34)
+            assert V64.resident_tiles[0] and V32.resident_tiles[0]
+        for mixed in (False, True):
+            s = L.Solver(V64, V32 if mixed else None, batch=8)
+            s.set_block(True)
+            for k in range(2):  # the second solve replays the captured steps
+                X, st = s.solve(B.ravel(), tol=1e-10, mixed=mixed)
+                assert all(v["converged"] for v in st)
+                out[res, mixed, k] = X
+            s.close()
+    for mixed in (False, True):
+        for k in range(2):
+            print(f"case mixed={mixed} solve {k}", flush=True)
+            assert np.array_equal(out[True, mixed, k], out[False, mixed, k])
