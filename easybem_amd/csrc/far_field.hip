@@ -1135,9 +1135,12 @@ __global__ __launch_bounds__(256) void k_far_sym_kel(FarArgs<R> a)
 // Persistent multi-mesh form of the symmetric schedule: the workgroups of
 // several operators' launches are items of one queue (operator k owns items
 // off[k] .. off[k+1]-1, item off[k] + b being its workgroup b), taken from
-// an atomic counter until the queue is empty.
+// an atomic counter until the queue is empty.  At least 2 waves per SIMD, as
+// k_far_sym: FP64 at 7 points would otherwise take 261 registers and run
+// one wave per SIMD.
 template <typename R, int NT>
-__global__ __launch_bounds__(256) void k_far_sym_multi(const FarArgs<R> *__restrict__ descs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_far_sym_multi(
+    const FarArgs<R> *__restrict__ descs,
                                                        const int *__restrict__ off,
                                                        int nops, int *__restrict__ counter)
 {
