@@ -675,16 +675,18 @@ def main():
     elif args.precond == "calderon":
         # Q (D + m m^T / (4|Gamma|)) Q: D through curls, 3-point Gauss near
         # field, 1-point far rule; Q five Chebyshev steps for the P1 mass
-        # inverse.  FP64 also inside the FP32 inner solves: an FP32 one is
-        # linear only to its rounding, and the inner residual then stalls
-        # near 1.6e-8, so one refinement pass could not reach 1e-8
+        # inverse.  The FP32 inner solves take D's far field in FP32: GMRES
+        # keeps each preconditioned vector (flexible), so P need not be
+        # linear beyond its rounding
         P64 = L.precond_calderon(mesh, "fp64", line_rule=2, tri_rule=0)
-        solver.set_precond(P64, P64)
+        P32 = L.precond_calderon(mesh, "fp32", line_rule=2, tri_rule=0)
+        solver.set_precond(P64, P32)
         pc = ("Calderon, right: Q (D + m m^T/(4|Gamma|)) Q, D the hypersingular "
               "operator through curls with a 1-point far rule and 3-point Gauss "
               "near rule, Q five Chebyshev steps for the consistent P1 mass "
-              "inverse (lumped-mass preconditioned), in FP64 for the FP32 inner "
-              "solves too; the residual is that of the unpreconditioned system")
+              "inverse (lumped-mass preconditioned); D's far field in FP32 for the "
+              "FP32 inner solves (flexible GMRES); the residual is that of the "
+              "unpreconditioned system")
 
     def solve(mixed=True, s=None):
         L.check(L.lib().ebh_memcpy_h2d(dx.ptr, zeros.ctypes.data, zeros.nbytes))

@@ -364,6 +364,11 @@ struct ebh_solver {
   // right preconditioners for the FP64 and FP32 operators (may be null)
   ebh_op *p64 = nullptr, *p32 = nullptr;
   DevBuf<double> Zt, Ut;             // nbmax x n: preconditioned vectors
+  // with a preconditioner, GMRES is flexible: z_j = P v_j is kept (m x
+  // nbmax x n) and the update is Z += sum_j y_j z_j, so P need not be
+  // linear (an FP32 P is linear only to its rounding) and the update needs
+  // no further P apply
+  DevBuf<double> ZV;
   int precond_applies = 0;
   int n = 0, m = 50, nbmax = 1;
   hipStream_t stream = nullptr;
@@ -438,15 +443,17 @@ struct ebh_solver {
   ebh_op *precond_for(const ebh_op *A) const { return A == a64 ? p64 : p32; }
 
   // One Arnoldi step j for nb systems (kernels only; graph-capturable):
-  // w = A P v_j (P the right preconditioner, if any), orthogonalised.
+  // w = A z_j with z_j = P v_j (P the right preconditioner, if any, z_j kept
+  // for the update), orthogonalised.
   int step_kernels(ebh_op *A, int j, int nb)
   {
     const size_t vs = vstride();
     double *vj = V.p + (size_t)j * vs, *w = V.p + (size_t)(j + 1) * vs;
     ebh_op *P = precond_for(A);
     if (P) {
-      EBH_TRY(op_apply_multi(P, vj, Zt.p, nb, stream, false));
-      EBH_TRY(op_apply_multi(A, Zt.p, w, nb, stream, false));
+      double *zj = ZV.p + (size_t)j * vs;
+      EBH_TRY(op_apply_multi(P, vj, zj, nb, stream, false));
+      EBH_TRY(op_apply_multi(A, zj, w, nb, stream, false));
     } else {
       EBH_TRY(op_apply_multi(A, vj, w, nb, stream, false));
     }
@@ -864,6 +871,9 @@ struct ebh_solver {
                            hipMemcpyHostToDevice, stream));
     EBH_HIP(hipStreamSynchronize(stream));
     iters.assign(nb, 0);
+    // the kept z_j of flexible GMRES (allocated once, before any step is
+    // captured, so captured steps keep their addresses)
+    if (precond_for(A) && !ZV.p) EBH_TRY(ZV.alloc((size_t)m * vstride()));
     int total = 0;
     while (true) {
       EBH_TRY(residual(A, B, Z, z_zero, nb));
@@ -893,22 +903,13 @@ struct ebh_solver {
         iters[r] += h_jdone[r];
         all_met &= h_est[r] <= atol_h[r];
       }
-      // Z += P V y with each system's own number of steps
+      // Z += sum_j y_j z_j (z_j = P v_j, or v_j without a preconditioner)
+      // with each system's own number of steps
       hipLaunchKernelGGL(k_bbacksolve, dim3((nb + 63) / 64), dim3(64), 0, stream, nb, m,
                          Rm.p, g.p, jdone.p, yv.p);
-      ebh_op *P = precond_for(A);
-      if (P) {
-        const size_t tot = (size_t)nb * n;
-        hipLaunchKernelGGL(k_fill, grid1(tot), dim3(256), 0, stream, tot, 0.0, Ut.p);
-        hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
-                           jdone.p, V.p, vstride(), yv.p, m, 1.0, Ut.p, allon.p);
-        EBH_TRY(op_apply_multi(P, Ut.p, Zt.p, nb, stream, false));
-        precond_applies += nb;
-        hipLaunchKernelGGL(k_axpby, grid1(tot), dim3(256), 0, stream, tot, 1.0, Zt.p, 1.0, Z);
-      } else {
-        hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
-                           jdone.p, V.p, vstride(), yv.p, m, 1.0, Z, allon.p);
-      }
+      hipLaunchKernelGGL(k_bgemv, dim3((n + 255) / 256, nb), dim3(256), 0, stream, n, m,
+                         jdone.p, precond_for(A) ? ZV.p : V.p, vstride(), yv.p, m, 1.0, Z,
+                         allon.p);
       EBH_HIP(hipGetLastError());
       // the Arnoldi estimate met every tolerance: no restart residual needed
       // (callers that need the true residual compute it themselves)
@@ -1147,6 +1148,8 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32)
   if (p64 || p32) {
     EBH_TRY(s->Zt.alloc(tot));
     EBH_TRY(s->Ut.alloc(tot));
+  } else {
+    s->ZV.release();
   }
   s->p64 = p64;
   s->p32 = p32;
