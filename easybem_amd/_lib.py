@@ -648,7 +648,7 @@ class ServerOpts(C.Structure):
                 ("line_rule", C.c_int), ("tri_rule", C.c_int),
                 ("max_iters", C.c_int), ("dirichlet_p1", C.c_int),
                 ("no_precond", C.c_int), ("no_graphs", C.c_int),
-                ("partial_limit", C.c_size_t)]
+                ("partial_limit", C.c_size_t), ("inner_tol", C.c_double)]
 
 
 class RequestResult(C.Structure):
@@ -668,7 +668,8 @@ class Server:
 
     def __init__(self, max_slots=4, hbm_budget=0, restart=50, tol=1e-8,
                  mixed=True, line_rule=4, tri_rule=4, max_iters=2000,
-                 dirichlet_p1=False, precond=True, graphs=True, partial_limit=0):
+                 dirichlet_p1=False, precond=True, graphs=True, partial_limit=0,
+                 inner_tol=0.0):
         L = lib()
         vp = C.c_void_p
         L.ebh_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -684,7 +685,8 @@ class Server:
         L.ebh_server_destroy.restype = None
         o = ServerOpts(max_slots, hbm_budget, restart, tol, int(mixed),
                        line_rule, tri_rule, max_iters, int(dirichlet_p1),
-                       int(not precond), int(not graphs), int(partial_limit))
+                       int(not precond), int(not graphs), int(partial_limit),
+                       float(inner_tol))
         self.dirichlet_p1 = bool(dirichlet_p1)
         h = vp()
         check(L.ebh_server_create(C.byref(o), C.byref(h)))

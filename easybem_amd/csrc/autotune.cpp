@@ -187,12 +187,13 @@ int ebh_op_autotune(ebh_op *op, double budget_ms, ebh_tune_result *out)
   const int start_ks = op->ksplit;
   int bb = start_sym ? 256 : op->block, bt = start_sym ? 64 : op->tile;
   int bg = start_sym ? 4096 : grid_of(op);
-  double best = 0.0;
-  EBH_TRY(eval(bb, bt, bg, &best));
+  double best = 1e300;
+  const bool chunked = chunked_ok(op);
+  if (chunked) EBH_TRY(eval(bb, bt, bg, &best));
   const int blocks[] = {64, 128, 256};
   const int tiles[] = {16, 32, 64};
   const int grids[] = {1024, 2048, 4096, 8192, 16384, 32768};
-  for (int round = 0; round < 2 && spent < budget_ms; round++) {
+  for (int round = 0; round < 2 && chunked && spent < budget_ms; round++) {
     for (int g : grids) {
       if (spent >= budget_ms) break;
       double ms;

@@ -241,10 +241,8 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
     set_error("the hypersingular operator needs P1 test and trial spaces");
     return EBH_ERR_UNSUPPORTED;
   }
-  if (d.kernel == EBH_LAPLACE_ADL &&
-      (d.test_basis != EBH_P1 || d.far_precision != EBH_FP64)) {
-    set_error("the adjoint double layer needs a P1 test space and the FP64 far "
-              "field ((y-x).n cancels to O(r^2) on smooth surfaces)");
+  if (d.kernel == EBH_LAPLACE_ADL && d.test_basis != EBH_P1) {
+    set_error("the adjoint double layer needs a P1 test space");
     return EBH_ERR_UNSUPPORTED;
   }
   if (d.kernel == EBH_LAPLACE_HS ? !std::isfinite(d.stab) : d.stab != 0.0) {
@@ -293,6 +291,13 @@ int ebh_op_create(ebh_mesh *mesh, const ebh_op_desc *desc, ebh_op **out)
   if (op->rule.nt <= 7 && !op->ray) {
     delete op;
     set_error("far-field rules of up to 7 points must be median-symmetric");
+    return EBH_ERR_UNSUPPORTED;
+  }
+  if (d.kernel == EBH_LAPLACE_ADL && d.far_precision == EBH_FP32 &&
+      (op->rule.nt > 7 || op->d.max_rhs > 1)) {
+    delete op;
+    set_error("the FP32 adjoint double layer runs on the pair-symmetric schedule: far "
+              "rules of at most 7 points (tri_rule 0..4) and max_rhs 1");
     return EBH_ERR_UNSUPPORTED;
   }
   op->nc = elastic ? 3 : 1;

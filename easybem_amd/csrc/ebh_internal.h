@@ -308,6 +308,9 @@ int op_build_far(ebh_op *op);
 // the mesh); only for operators that sym_eligible accepts
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
+// false for operators that run on the pair-symmetric schedule only (the
+// FP32 adjoint double layer)
+bool chunked_ok(const ebh_op *op);
 // the pair-symmetric multi-RHS schedule applies (a P0 single layer with
 // max_rhs > 1 and a rule of at most 7 points)
 bool mts_eligible(const ebh_op *op);

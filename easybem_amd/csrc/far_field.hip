@@ -631,10 +631,11 @@ struct SymLayout {
   static constexpr int LDS = 64 * GW;
 };
 // the layout of sym_block<R, NT, NR, DK> (DK 1, 2: double layers with the
-// normal; 0 and 3 (Kelvin) in FP32: the centroid remainder)
+// normal; FP32 records read from LDS by index, every kernel but the
+// one-point single layers: the centroid remainder)
 template <typename R, int NT, int NR, int DK>
 using SymLay = SymLayout<NT, NR, DK == 1 || DK == 2,
-                         sizeof(R) == 4 && (DK == 0 || DK == 3) && NT != 1>;
+                         sizeof(R) == 4 && (NT != 1 || DK == 1 || DK == 2)>;
 
 // Workgroup bid of the symmetric schedule (see above); lds holds the source
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
@@ -661,15 +662,17 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   using RecT = SymRec<R, NR>;
   using Lay = SymLay<R, NT, NR, DK>;
   constexpr int C0 = Lay::C0, NO = Lay::NO, GW = Lay::GW;
-  // FP32 single layer: the centroid difference D = C - C' is formed from
-  // hi and lo parts, (C_hi - C'_hi) + (C_lo - C'_lo), so that it carries no
-  // rounding of the positions (which would be systematic for each triangle
-  // pair), and with one right-hand side every step's point sums (FP32 over
-  // the step's 7 source points) go to FP64 accumulators.  Measured at L5 P1:
-  // FP32-vs-FP64 apply error 6e-8 -> 8e-9 (random density) and 7e-9 (the
-  // solution of the bench's solve); flushing every 1, 4 or 8 steps costs the
-  // same time, and an FP32 Newton step on rsqrt costs 60% and helps nothing.
-  constexpr bool HL = sizeof(R) == 4 && !DL && !Lay::ROT;
+  // FP32 (records read from LDS): the centroid difference D = C - C' is
+  // formed from hi and lo parts, (C_hi - C'_hi) + (C_lo - C'_lo), so that it
+  // carries no rounding of the positions (which would be systematic for each
+  // triangle pair; for the double layers D.n is a near-cancelling difference
+  // between nearby panels), and with one right-hand side every step's point
+  // sums (FP32 over the step's source points) go to FP64 accumulators.
+  // Measured at L5 P1 single layer: FP32-vs-FP64 apply error 6e-8 -> 8e-9
+  // (random density) and 7e-9 (the solution of the bench's solve); flushing
+  // every 1, 4 or 8 steps costs the same time, and an FP32 Newton step on
+  // rsqrt costs 60% and helps nothing.
+  constexpr bool HL = sizeof(R) == 4 && !Lay::ROT;
   constexpr bool FL = HL && NR == 1;
   constexpr int KF = 1;
   // One-point single-layer rules move the source records between lanes (DPP
@@ -1111,6 +1114,9 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 
 // at least 2 waves per SIMD: the FP64 7-point single layer sits at the
 // 256-register edge, and one register more would halve its occupancy
+// (the FP32 7-point double layer takes 129 registers, 3 waves per SIMD;
+// asking for 4 spills two values and measures the same: 3.01 against 3.00
+// ms at L5, 41.4 against 42.2 ms at L6)
 template <typename R, int NT, int NR, int DK = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_far_sym(
     FarArgs<R> a)
@@ -2200,14 +2206,9 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
       EBH_HIP(hipGetLastError());
       return EBH_OK;
     }
-    if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) {
-      // the symmetric double layers exist in FP64 only (sym_eligible)
-      if constexpr (sizeof(R) == 8)
-        return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, b)
-                                              : launch_sym<R, 1, 2>(op->rule.nt, grid, s, b);
-      set_error("no FP32 pair-symmetric double layer");
-      return EBH_ERR_UNSUPPORTED;
-    }
+    if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL)
+      return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, b)
+                                            : launch_sym<R, 1, 2>(op->rule.nt, grid, s, b);
     return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, b)
                    : launch_sym<R, 1>(op->rule.nt, grid, s, b);
   };
@@ -2680,18 +2681,23 @@ int sym_default_split(int ng)
 
 bool sym_eligible(const ebh_op *op)
 {
-  // the single layer with equal bases, or the FP64 double layer or its
-  // adjoint (their point sums are weighted by the test functions and their
+  // the single layer with equal bases, or the double layer or its adjoint
+  // (their point sums are weighted by the test functions and their
   // strengths by the trial density, so the bases may differ)
   const bool sl = op->d.kernel == EBH_LAPLACE_SL && op->d.test_basis == op->d.trial_basis;
   const bool dl = (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL) &&
-                  op->d.far_precision == EBH_FP64 && op->nr_max == 1;
+                  op->nr_max == 1;
   // the Kelvin single layer with equal bases (its three components are the
   // launch's three right-hand-side planes)
   const bool kel = op->d.kernel == EBH_KELVIN_SL && op->d.test_basis == op->d.trial_basis &&
                    op->nr_max == 1 &&
                    (op->d.far_precision == EBH_FP32 || op->rule.nt <= 4);
   return (sl || dl || kel) && op->ray && op->rule.nt <= 7;
+}
+
+bool chunked_ok(const ebh_op *op)
+{
+  return !(op->d.kernel == EBH_LAPLACE_ADL && op->d.far_precision == EBH_FP32);
 }
 
 // Passes of the pair-symmetric schedule for partial sums above the
@@ -2783,11 +2789,17 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
 {
   const int ne = op->mesh->T.ne;
   bool resident = false;
+  if (!sym && !chunked_ok(op)) {
+    // the chunked kernel forms (y - x).n_x from rounded FP32 positions
+    set_error("the FP32 adjoint double layer runs on the pair-symmetric schedule only");
+    return EBH_ERR_UNSUPPORTED;
+  }
   if (sym) {
     if (!sym_eligible(op)) {
       set_error("the pair-symmetric schedule needs a Laplace or Kelvin single layer with "
-                "equal test and trial bases or an FP64 Laplace double layer, and a "
-                "median-symmetric far rule");
+                "equal test and trial bases or a Laplace double layer (or its adjoint) "
+                "with one right-hand side, and a median-symmetric far rule of at most 7 "
+                "points");
       return EBH_ERR_UNSUPPORTED;
     }
     // groups of 64 triangles; chunks (when given) is the number of step
@@ -2941,11 +2953,11 @@ int op_build_far(ebh_op *op)
   }
   if (reg_rule(nt) &&
       (op->d.kernel == EBH_LAPLACE_SL || op->d.kernel == EBH_LAPLACE_ADL ||
-       ((op->d.kernel == EBH_LAPLACE_DL && op->d.far_precision == EBH_FP64) ||
-        op->d.kernel == EBH_KELVIN_SL) && op->ray && nt <= 7)) {
+       (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_KELVIN_SL) && op->ray &&
+           nt <= 7)) {
     // single-layer target frame: centroid, then the median form (ray rules)
-    // or the offsets of every point from the centroid; the FP64 double layer
-    // and the Kelvin single layer use it on the symmetric schedule
+    // or the offsets of every point from the centroid; the double layer and
+    // the Kelvin single layer use it on the symmetric schedule
     const bool ray = op->ray && nt <= 7;
     // FP32 median rules: three more rows, the rounding remainders of the
     // centroid, C - (float)C (the symmetric kernel's hi/lo differences)
@@ -3570,6 +3582,12 @@ int ebh_batch_create(ebh_op *const *ops, int nops, ebh_batch **out)
     if (!reg_rule(ops[k]->rule.nt)) {
       set_error("batches take far rules of at most 13 points (triangle rule "
                 "indices 0..6)");
+      return EBH_ERR_UNSUPPORTED;
+    }
+    if (!chunked_ok(ops[k])) {
+      // a batch runs double layers on the chunked schedule
+      set_error("the FP32 adjoint double layer runs on the pair-symmetric schedule "
+                "only, which batches take for single layers");
       return EBH_ERR_UNSUPPORTED;
     }
     if (ops[k]->d.kernel != ops[0]->d.kernel ||

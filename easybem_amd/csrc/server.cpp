@@ -140,17 +140,18 @@ double padded_p1_row(int np, int ne, const int *elems)
 // under.  Operators with partial sums:
 //   P0 Dirichlet: V (FP64, FP32) and the FP64 double layer, 1 value each;
 //   P1 Dirichlet: V (FP64, FP32) and the double layer with 3 values, the
-//     Calderon preconditioners (FP64, FP32) with a P0 V on 3 densities;
+//     Calderon preconditioner (FP32 in a mixed solve, else FP64) with a P0
+//     V on 3 densities;
 //   Neumann: D (FP64, FP32) with a P0 V on 3 densities, the adjoint double
-//     layer with 3 values, the dual Calderon preconditioners (FP64, FP32)
-//     with a P1 V.
+//     layer with 3 values, the dual Calderon preconditioner (FP32 in a
+//     mixed solve, else FP64) with a P1 V.
 size_t estimate_bytes(int np, int ne, int restart, bool mixed, bool neumann, bool p1,
                       bool precond, double p1_row, double limit)
 {
   const double pairs = 14.0 * ne;
   const double near = pairs * (9 * 8 + 6 * 4 + 12 + 8) + 2 * pairs * 12;
   const double far_one = ne * 7.0 * (3 * 12 + 64 + 16) + 4096.0 * 256 * 8 * 2;
-  const int nprec = precond && (neumann || p1) ? (mixed ? 2 : 1) : 0;
+  const int nprec = precond && (neumann || p1) ? 1 : 0;
   const int nops = (mixed ? 3 : 2) + nprec;
   const double ops = nops * (near / 2 + far_one);
   const double krylov = (restart + 6.0) * ne * 8;
@@ -341,20 +342,20 @@ struct ebh_server {
       return rc;
     });
     const int cal = o.no_precond ? 0 : neu ? 2 : p1 ? 1 : 0;
+    // Krylov steps run on the FP32 operator in a mixed solve (the FP64 one
+    // only forms residuals), so it needs the FP32 preconditioner alone
+    ebh_op **pc = o.mixed ? &en->P32 : &en->P64;
+    const int pprec = o.mixed ? EBH_FP32 : EBH_FP64;
     if (cal == 2) {
-      // dual Calderon preconditioners Q V Q (Q the Chebyshev inverse of the
+      // dual Calderon preconditioner Q V Q (Q the Chebyshev inverse of the
       // P1 mass) with a 3-point far and 4-point Gauss near rule: D's small
       // eigenvalues are its smooth modes, which a local inverse cannot reach
       // (its sparse inverse needs more iterations from L5 up)
-      spawn([=]() { return ebh_precond_calderon_hs(mesh, EBH_FP64, 3, 1, &en->P64); });
-      if (o.mixed)
-        spawn([=]() { return ebh_precond_calderon_hs(mesh, EBH_FP32, 3, 1, &en->P32); });
+      spawn([=]() { return ebh_precond_calderon_hs(mesh, pprec, 3, 1, pc); });
     } else if (cal == 1) {
       // P1 Dirichlet: the Calderon preconditioner Q (D + m m^T/(4|Gamma|)) Q
       // (Q the Chebyshev inverse of the P1 mass) with a 1-point far rule
-      spawn([=]() { return ebh_precond_calderon(mesh, EBH_FP64, 2, 0, 1, &en->P64); });
-      if (o.mixed)
-        spawn([=]() { return ebh_precond_calderon(mesh, EBH_FP32, 2, 0, 1, &en->P32); });
+      spawn([=]() { return ebh_precond_calderon(mesh, pprec, 2, 0, 1, pc); });
     }
     int rc = ebh_op_create(mesh, &d, &e->A64);
     std::string msg = rc == EBH_OK ? std::string() : std::string(ebh_last_error());
@@ -371,7 +372,7 @@ struct ebh_server {
     }
     EBH_TRY(ebh_solver_create(e->A64, e->A32, o.restart, &e->solver));
     if (cal) {
-      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, o.mixed ? e->P32 : nullptr));
+      EBH_TRY(ebh_solver_set_precond(e->solver, e->P64, e->P32));
     } else if (!o.no_precond) {
       // P0 Dirichlet: near-field sparse approximate inverse of V (2 triangle
       // rings), one FP64 operator for both precisions
@@ -482,7 +483,11 @@ struct ebh_server {
     so.max_iters = o.max_iters;
     so.tol = o.tol;
     so.mixed = o.mixed;
-    so.inner_tol = 1e-5;
+    // Dirichlet: the FP32 pass to 1e-9 leaves one FP64 residual per solve
+    // (L5 31 -> 25 ms, L6 412 -> 358 ms, one request at a time); Neumann:
+    // two passes to 1e-5 take one Arnoldi step fewer than one pass to 1e-9
+    // (L6 720 against 770 ms), so it keeps them
+    so.inner_tol = o.inner_tol > 0 ? o.inner_tol : neu ? 1e-5 : 1e-9;
     so.max_outer = 10;
     ebh_solve_stats st{};
     const int rc = ebh_solver_solve(e->solver, db.p, dx.p, &so, &st);

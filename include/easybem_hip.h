@@ -54,8 +54,9 @@ enum {
 /* EBH_LAPLACE_ADL: the adjoint double layer, kernel (y - x).n_x / (4 pi
  * r^3), whose Galerkin matrix is the transpose of the double layer's
  * (getLaplace3d's K, src/BEM3dLaplace.cpp:386-437,647); it gives the
- * Neumann right-hand side (M^T/2 - K^T) t.  P1 test, P0 or P1 trial,
- * FP64 far field only. */
+ * Neumann right-hand side (M^T/2 - K^T) t.  P1 test, P0 or P1 trial; in
+ * FP32 on the pair-symmetric schedule only (far rules of at most 7 points,
+ * max_rhs 1), whose hi/lo centroid differences keep (y-x).n accurate. */
 /* EBH_KELVIN_DL, EBH_KELVIN_HS: the elasticity double layer (P0 test x P1
  * trial, 3 components each) and hypersingular operator (P1 x P1) of
  * getElasticity3d (src/BEM3dElasticity.cpp:760-943), applied through the
@@ -514,6 +515,10 @@ typedef struct {
   size_t partial_limit; /* far-field partial sums per operator, bytes
                           (ebh_op_set_partial_limit; 0 = 512 MiB): larger
                           meshes run the symmetric schedule in passes       */
+  double inner_tol;    /* mixed solves: relative tolerance of the first FP32
+                          pass; below 0.1 tol the first pass solves to it,
+                          so that one FP64 residual can certify the solve
+                          (0: 1e-9 for Dirichlet, 1e-5 for Neumann)         */
 } ebh_server_opts;
 
 typedef struct {

@@ -121,13 +121,26 @@ def test_adjoint_double_layer_p1_trial(L, oracle):
     assert err <= 1e-12
 
 
-def test_adjoint_double_layer_rejects_fp32_and_p0_test(L):
+def test_adjoint_double_layer_limits(L):
+    """A P0 test space is rejected; the FP32 adjoint runs on the
+    pair-symmetric schedule only (rules of at most 7 points, one right-hand
+    side, no chunked schedule), where its (y - x).n comes from hi/lo
+    centroid differences."""
     P, T = meshes.icosphere(1)
     m = L.Mesh(P, T)
     with pytest.raises(L.EbhError):
-        L.Operator(m, "Kt", 1, 0, "fp32")
-    with pytest.raises(L.EbhError):
         L.Operator(m, "Kt", 0, 0, "fp64")
+    with pytest.raises(L.EbhError, match="pair-symmetric"):
+        L.Operator(m, "Kt", 1, 0, "fp32", line_rule=4, tri_rule=5)
+    with pytest.raises(L.EbhError, match="pair-symmetric"):
+        L.Operator(m, "Kt", 1, 0, "fp32", max_rhs=2)
+    op = L.Operator(m, "Kt", 1, 0, "fp32")
+    assert op.schedule == "symmetric"
+    with pytest.raises(L.EbhError, match="pair-symmetric"):
+        op.set_schedule("chunked")
+    with pytest.raises(L.EbhError, match="pair-symmetric"):
+        op.set_launch(256, 64, 64)
+    assert op.schedule == "symmetric"
 
 
 # ------------------------------------- dual Calderon preconditioner of D

@@ -106,7 +106,7 @@ def test_symmetric_split_rejected(L):
 
 
 @pytest.mark.parametrize("kernel,test,trial,tri_rule,prec", [
-    ("K", 0, 1, 4, "fp32"), ("K", 1, 1, 5, "fp64"), ("V", 0, 1, 4, "fp64"),
+    ("K", 0, 1, 5, "fp32"), ("K", 1, 1, 5, "fp64"), ("V", 0, 1, 4, "fp64"),
     ("V", 1, 0, 4, "fp64"), ("kelvin", 0, 0, 4, "fp64"), ("V", 0, 0, 5, "fp64")])
 def test_symmetric_rejected_where_it_does_not_apply(L, kernel, test, trial,
                                                     tri_rule, prec):
@@ -297,3 +297,68 @@ def test_symmetric_adjoint_double_layer_l5(L, split):
     err = rel(op(t), y0)
     print("Kt L5 symmetric vs chunked", err)
     assert err <= 1e-13
+
+# ------------------------------------------- FP32 double layers, symmetric
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("kernel,test,trial", [("K", 0, 1), ("K", 1, 1),
+                                               ("Kt", 1, 0), ("Kt", 1, 1)])
+def test_fp32_symmetric_double_layers(L, oracle, mesh, kernel, test, trial):
+    """FP32 double layer and adjoint on the pair-symmetric schedule (the
+    default for them): against the oracle's double layer (K), its
+    transpose (Kt, P0 trial: the columns of the oracle's P1 x P0 matrix are
+    not available, so against the FP64 adjoint, itself checked against the
+    reference's K^T), to the FP32 bar 1e-6; bitwise repeatable."""
+    P, T = MESHES[mesh]()
+    print(f"case {kernel}{test}{trial} fp32 mesh={mesh} ne={T.shape[0]}", flush=True)
+    x = _x(trial, P, T, 31)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, kernel, test, trial, "fp32", line_rule=4, tri_rule=4)
+    assert op.schedule == "symmetric"
+    y = op(x)
+    if kernel == "K":
+        yr = oracle.apply("K", test, 1, P, T, x, line_idx=4, tri_idx=4).ravel()
+    else:
+        yr = L.Operator(m, "Kt", test, trial, "fp64", line_rule=4, tri_rule=4)(x)
+    err = rel(y, yr)
+    print("rel err", err)
+    assert err <= 1e-6
+    assert np.array_equal(op(x), y)
+    if kernel == "K":
+        # the chunked FP32 double layer (hi/lo points) agrees to the same bar
+        op.set_schedule("chunked")
+        assert rel(op(x), yr) <= 1e-6
+
+
+@pytest.mark.parametrize("kernel,test,trial", [("K", 1, 1), ("Kt", 1, 0)])
+@pytest.mark.parametrize("tri_rule", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("split", [1, 4, 64])
+def test_fp32_symmetric_double_layer_rules_and_splits(L, kernel, test, trial,
+                                                      tri_rule, split):
+    P, T = meshes.torus(1.0, 0.4, 40, 20)
+    m = L.Mesh(P, T)
+    x = _x(trial, P, T, 32)
+    y64 = L.Operator(m, kernel, test, trial, "fp64", line_rule=3, tri_rule=tri_rule)(x)
+    op = L.Operator(m, kernel, test, trial, "fp32", line_rule=3, tri_rule=tri_rule)
+    op.set_schedule("symmetric", split)
+    assert op.split == split
+    err = rel(op(x), y64)
+    print(f"{kernel} fp32 rule {tri_rule} split {split}: rel err vs fp64 {err}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("kernel,test,trial", [("K", 1, 1), ("Kt", 1, 0)])
+def test_fp32_symmetric_double_layer_l5(L, kernel, test, trial):
+    """Full size (20480 triangles): the FP32 double layers against FP64 on
+    the bench's Dirichlet data and on a random Neumann density."""
+    P, T = meshes.icosphere(5)
+    m = L.Mesh(P, T)
+    if kernel == "K":
+        x = 1.0 / np.linalg.norm(P - np.array([0.0, 0.0, 2.0]), axis=1)
+    else:
+        x = np.random.default_rng(33).uniform(-1, 1, T.shape[0])
+    y64 = L.Operator(m, kernel, test, trial, "fp64")(x)
+    op = L.Operator(m, kernel, test, trial, "fp32")
+    assert op.schedule == "symmetric"
+    err = rel(op(x), y64)
+    print(f"{kernel} L5 fp32 symmetric vs fp64: {err}")
+    assert err <= 1e-6
