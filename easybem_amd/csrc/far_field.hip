@@ -116,6 +116,8 @@ struct FarArgs {
   // spart = partial, sprow = prow.
   int m0, t0, c0, tcols, scols, sprow;
   double *spart;
+  int pf;                             // pair-symmetric single layer, P0 test
+                                      // and trial: the P0-factored form
   double phi[16][3];                  // test shape functions at the points
 };
 
@@ -615,16 +617,18 @@ struct alignas(4 * sizeof(R)) SymRec {
 // double layer its unit normal n', then the NT x NR strengths.  GW is a
 // multiple of 4 with GW / 4 odd, so the 16-byte reads of 16 lanes at
 // rotating records spread over the LDS banks.
-// HL (the FP32 single layer): the record also holds the rounding remainder
-// of the source centroid, C' - (float)C', at offset LO.
-template <int NT, int NR, bool DL = false, bool HL = false>
+// HL (FP32): the record also holds the rounding remainder of the source
+// centroid, C' - (float)C', at offset LO.  PF (the P0-factored single
+// layer): in place of the NT x NR strengths, the NT point weights w_b J'
+// and then the NR densities (times 1/4 pi) of the triangle.
+template <int NT, int NR, bool DL = false, bool HL = false, bool PF = false>
 struct SymLayout {
   static constexpr bool ROT = NT == 1 && !DL;
   static constexpr int C0 = NT % 3;
   static constexpr int NO = NT - C0;
   static constexpr int NRM = 9 + NO;              // offset of n' (double layer)
   static constexpr int SIG = NRM + (DL ? 3 : 0);  // offset of the strengths
-  static constexpr int LO = SIG + NT * NR;        // centroid remainder (HL)
+  static constexpr int LO = SIG + (PF ? NT + NR : NT * NR);  // centroid remainder (HL)
   static constexpr int USED = ROT ? (NR == 1 ? 4 : 8) : LO + (HL ? 3 : 0);
   static constexpr int Q4 = (USED + 3) / 4;
   static constexpr int GW = ROT ? USED : 4 * (Q4 % 2 == 1 ? Q4 : Q4 + 1);
@@ -632,10 +636,11 @@ struct SymLayout {
 };
 // the layout of sym_block<R, NT, NR, DK> (DK 1, 2: double layers with the
 // normal; FP32 records read from LDS by index, every kernel but the
-// one-point single layers: the centroid remainder)
+// one-point single layers: the centroid remainder; DK 4: the P0-factored
+// single layer)
 template <typename R, int NT, int NR, int DK>
 using SymLay = SymLayout<NT, NR, DK == 1 || DK == 2,
-                         sizeof(R) == 4 && (NT != 1 || DK == 1 || DK == 2)>;
+                         sizeof(R) == 4 && (NT != 1 || DK == 1 || DK == 2), DK == 4>;
 
 // Workgroup bid of the symmetric schedule (see above); lds holds the source
 // group in the SymLayout form.  Every thread of the workgroup calls it (one
@@ -655,9 +660,12 @@ template <typename R, int NT, int NR, int DK = 0>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                                           R *__restrict__ lds)
 {
-  constexpr bool DL = DK == 1 || DK == 2, ADJ = DK == 2, KEL = DK == 3;
+  constexpr bool DL = DK == 1 || DK == 2, ADJ = DK == 2, KEL = DK == 3, PF = DK == 4;
   static_assert(!DL || NR == 1, "double layer: one right-hand side");
   static_assert(!KEL || NR == 3, "Kelvin: three components");
+  static_assert(!PF || NT > 1, "P0-factored form: rules of several points");
+  // point sums per side: NT, or one per triangle in the P0-factored form
+  constexpr int NS = PF ? 1 : NT;
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
   using Lay = SymLay<R, NT, NR, DK>;
@@ -733,8 +741,21 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
         } else if (f < Lay::SIG) {
           v = (R)a.tn[3 * (size_t)j + (f - Lay::NRM)];
         } else if (f < Lay::LO) {
-          const int sr = f - Lay::SIG, b = sr / NR, r = sr - b * NR;
-          v = a.src[RW * ((size_t)j * NT + b) + 3 + r];
+          const int sr = f - Lay::SIG;
+          if constexpr (PF) {
+            // point weights w_b J', then the densities: point 0's strength
+            // over its weight (the fold's strengths are w_b J' sigma / 4 pi)
+            if (sr < NT) {
+              v = (R)a.tw[(size_t)sr * ne + j];
+            } else {
+              const double w0 = a.tw[j];
+              v = w0 != 0.0 ? (R)((double)a.src[RW * (size_t)j * NT + 3 + (sr - NT)] / w0)
+                            : R(0);
+            }
+          } else {
+            const int b = sr / NR, r = sr - b * NR;
+            v = a.src[RW * ((size_t)j * NT + b) + 3 + r];
+          }
         } else {
           v = a.tsl[(9 + NT + f - Lay::LO) * (size_t)ne + j];
         }
@@ -763,14 +784,27 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     Clz = a.tsl[(9 + NT + 2) * (size_t)ne + tt];
   }
   R E[NT], sq[NT][NR], rc[NT], hc[NT];
+  // P0-factored form: the target's point weights w_q J and densities
+  R wt[PF ? NT : 1], st[PF ? NR : 1];
 #pragma unroll
   for (int q = 0; q < NT; q++) {
     E[q] = ps * ps * a.tsl[(9 + q) * (size_t)ne + tt];
     rc[q] = ps * a.rc[q];
     hc[q] = R(0.5) * a.rc[q];  // double layer: c_q m.n = (c_q / 2) F.n
+    if constexpr (PF) {
+      wt[q] = (R)a.tw[(size_t)q * ne + tt];
+    } else {
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
+    }
+  }
+  if constexpr (PF) {
+    const double w0 = a.tw[tt];
 #pragma unroll
     for (int r = 0; r < NR; r++)
-      sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
+      st[r] = t < ne && w0 != 0.0
+                  ? (R)((double)a.src[RW * (size_t)tt * NT + 3 + r] / w0) : R(0);
   }
   // touching triangles of group J (a lane past the mesh excludes them all)
   uint64_t mask = ~0ull;
@@ -806,12 +840,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       for (int r = 0; r < 3; r++) ksq[q][r] = a.kappa * sq[q][r];
     }
   }
-  R acc[NT][NR], S[NT][NR];
+  R acc[NS][NR], S[NS][NR];
   // FP64 sums of the flushed steps (FL); the source side's stay aligned
   // with S: after a flush at step k, lane L holds source (L + k) mod 64 in both
-  double acc64[NT][NR], S64[NT][NR];
+  double acc64[NS][NR], S64[NS][NR];
 #pragma unroll
-  for (int q = 0; q < NT; q++)
+  for (int q = 0; q < NS; q++)
 #pragma unroll
     for (int r = 0; r < NR; r++) {
       acc[q][r] = S[q][r] = R(0);
@@ -823,7 +857,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     constexpr bool SYM = decltype(sym_tag)::value;
     const int from = ((L + shift) & 63) << 2;
 #pragma unroll
-    for (int q = 0; q < NT; q++)
+    for (int q = 0; q < NS; q++)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         acc64[q][r] += (double)acc[q][r];
@@ -843,10 +877,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   // source point, then both directions' sums.  Double layer: it[q] gathers
   // sigma_b / r^3 for target point q over the step's source points, and
   // source point b's sum of sigma_q / r^3 is scaled by its factor sfb.
-  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag, R *it, R sfb,
-                          const R *sqs) {
-    constexpr bool SYM = decltype(sym_tag)::value;
-    R ri[NT];
+  auto inv_r = [&](const R *r2, R *ri) {
     if constexpr (F64) {
       // r2 = 4 r^2: h = rsqrt(r2) ~ 1/(2r), and the Newton step for 1/r
       // from 2h is h (3 - r2 h^2): 3 instructions after rsqrt.  All
@@ -861,6 +892,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
 #pragma unroll
       for (int q = 0; q < NT; q++) ri[q] = rsq(r2[q]);
     }
+  };
+  auto finish_point = [&](const R *r2, const R *sig, int b, auto sym_tag, R *it, R sfb,
+                          const R *sqs) {
+    constexpr bool SYM = decltype(sym_tag)::value;
+    R ri[NT];
+    inv_r(r2, ri);
     if constexpr (ADJ) {
       const R sg = sig[0] * sfb;
 #pragma unroll
@@ -1016,6 +1053,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
             if constexpr (ADJ) sqs[q] = sq[q][0] * tf[q];
           }
         }
+        // P0-factored form: G = sum_q w_q J sum_b w'_b J' / r_qb of the
+        // triangle pair (NT independent sums over b, then one over q), then
+        // one FMA per density and direction
+        R hq[PF ? NT : 1];
+#pragma unroll
+        for (int q = 0; q < (PF ? NT : 1); q++) hq[q] = R(0);
 #pragma unroll
         for (int b = 0; b < NT; b++) {
           const int lb = (b - C0) % 3, ob = (b - C0) / 3;
@@ -1028,7 +1071,27 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                     : b < C0 ? Ab + B[q]
                              : fma(a.kap[2 * oq + ob], W[kq][lb], Ab + B[q]);
           }
-          finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag, it, DL ? sf[b] : R(0), sqs);
+          if constexpr (PF) {
+            R ri[NT];
+            inv_r(r2, ri);
+            const R wb = g[Lay::SIG + b];
+#pragma unroll
+            for (int q = 0; q < NT; q++) hq[q] = fma(wb, ri[q], hq[q]);
+          } else {
+            finish_point(r2, g + Lay::SIG + b * NR, b, sym_tag, it, DL ? sf[b] : R(0), sqs);
+          }
+        }
+        if constexpr (PF) {
+          constexpr bool SYM = decltype(sym_tag)::value;
+          R G = R(0);
+#pragma unroll
+          for (int q = 0; q < NT; q++) G = fma(wt[q], hq[q], G);
+          const R *sj = g + Lay::SIG + NT;
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            acc[0][r] = fma(G, sj[r], acc[0][r]);
+            if constexpr (SYM) S[0][r] = fma(G, st[r], S[0][r]);
+          }
         }
         if constexpr (DL && !ADJ) {
 #pragma unroll
@@ -1038,7 +1101,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       }
       if constexpr (SYM) {
 #pragma unroll
-        for (int b = 0; b < NT; b++)
+        for (int b = 0; b < NS; b++)
 #pragma unroll
           for (int r = 0; r < NR; r++) {
             if constexpr (ROT) {
@@ -1064,7 +1127,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   else steps(std::false_type{});
   if constexpr (!FL) {
 #pragma unroll
-    for (int q = 0; q < NT; q++)
+    for (int q = 0; q < NS; q++)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         acc64[q][r] = (double)acc[q][r];
@@ -1085,7 +1148,10 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     double *row = base + (size_t)(c - a.c0) * NR * rowv;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      if (a.nout == 1) {
+      if constexpr (PF) {
+        // the weights are inside G
+        row[(size_t)r * rowv + tl] = v[0][r];
+      } else if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
@@ -1150,7 +1216,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                        const int *__restrict__ off,
                                                        int nops, int *__restrict__ counter)
 {
-  __shared__ __align__(16) R lds[SymLay<R, NT, 1, 0>::LDS];
+  constexpr int L0 = SymLay<R, NT, 1, 0>::LDS, L4 = SymLay<R, NT, 1, 4>::LDS;
+  __shared__ __align__(16) R lds[L0 > L4 ? L0 : L4];
   __shared__ int sh_item;
   const int total = off[nops];
   while (true) {
@@ -1161,6 +1228,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (item >= total) break;
     int k = 0;
     while (item >= off[k + 1]) k++;
+    if constexpr (NT > 1)
+      if (descs[k].pf) {  // P0 members, as their own applies
+        sym_block<R, NT, 1, 4>(descs[k], item - off[k], lds);
+        continue;
+      }
     sym_block<R, NT, 1>(descs[k], item - off[k], lds);
   }
 }
@@ -2103,6 +2175,10 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
   a.tile = op->tile;
   a.ksplit = op->sym ? op->ksplit : 1;
   a.nout = op->d.test_basis ? 3 : 1;
+  // P0 test and trial functions of the single layer on the symmetric
+  // schedule: one kernel sum per triangle pair (rules of several points)
+  a.pf = op->sym && op->d.kernel == EBH_LAPLACE_SL && !op->d.test_basis &&
+         !op->d.trial_basis && nt > 1 ? 1 : 0;
   a.kappa = (R)(3.0 - 4.0 * op->d.nu);
   if constexpr (sizeof(R) == 4) {
     a.tq = op->tq32.p;
@@ -2159,7 +2235,11 @@ int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
 {
   const dim3 blk(256);
   switch (nt) {
-  case 1: hipLaunchKernelGGL((k_far_sym<R, 1, NR, DK>), grid, blk, 0, s, a); break;
+  case 1:
+    // (the P0-factored form is for rules of several points: fill_far_args
+    // sets pf only there)
+    hipLaunchKernelGGL((k_far_sym<R, 1, NR, DK == 4 ? 0 : DK>), grid, blk, 0, s, a);
+    break;
   case 3: hipLaunchKernelGGL((k_far_sym<R, 3, NR, DK>), grid, blk, 0, s, a); break;
   case 4: hipLaunchKernelGGL((k_far_sym<R, 4, NR, DK>), grid, blk, 0, s, a); break;
   case 6: hipLaunchKernelGGL((k_far_sym<R, 6, NR, DK>), grid, blk, 0, s, a); break;
@@ -2209,6 +2289,9 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
     if (op->d.kernel == EBH_LAPLACE_DL || op->d.kernel == EBH_LAPLACE_ADL)
       return op->d.kernel == EBH_LAPLACE_DL ? launch_sym<R, 1, 1>(op->rule.nt, grid, s, b)
                                             : launch_sym<R, 1, 2>(op->rule.nt, grid, s, b);
+    if (b.pf)
+      return nr == 3 ? launch_sym<R, 3, 4>(op->rule.nt, grid, s, b)
+                     : launch_sym<R, 1, 4>(op->rule.nt, grid, s, b);
     return nr == 3 ? launch_sym<R, 3>(op->rule.nt, grid, s, b)
                    : launch_sym<R, 1>(op->rule.nt, grid, s, b);
   };

@@ -307,12 +307,15 @@ def test_served_solves_replay_graphs_with_identical_results():
     out = {}
     for graphs in (False, True):
         srv = L.Server(max_slots=3, graphs=graphs)
-        ids = []
+        res = []
         for rep in range(2):  # the second pass hits the resident meshes
+            # (each pass ends before the next starts: a request whose mesh
+            # is still in use by another request builds its own copy)
+            ids = []
             for k, (P, T, x0) in enumerate(reqs):
                 pb = ("dirichlet", "neumann")[k % 2]
                 ids.append(srv.submit(P, T, x0 + 0.1 * rep, key=100 + k, problem=pb))
-        res = [srv.wait(r) for r in ids]
+            res += [srv.wait(r) for r in ids]
         srv.close()
         for r, _ in res:
             assert r["status"] == 0 and r["relres"] <= 1e-8, r

@@ -362,3 +362,66 @@ def test_fp32_symmetric_double_layer_l5(L, kernel, test, trial):
     err = rel(op(x), y64)
     print(f"{kernel} L5 fp32 symmetric vs fp64: {err}")
     assert err <= 1e-6
+
+
+# --------------------------------- P0-factored single layer (symmetric)
+@pytest.mark.parametrize("tri_rule", [1, 2, 3, 4])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("split", [1, 4])
+def test_p0_factored_every_rule(L, oracle, tri_rule, prec, split):
+    """P0 test and trial functions on the symmetric schedule sum each
+    triangle pair's kernel values once, G = sum_b w'_b J' sum_q w_q J / r,
+    and apply G to the densities in both directions (rules of 3 to 7
+    points; the one-point rule keeps its own form): against the oracle and
+    the chunked FP64 schedule."""
+    P, T = meshes.torus(1.0, 0.4, 40, 20)
+    print(f"case P0 torus tri_rule={tri_rule} {prec} split={split}", flush=True)
+    x = _x(0, P, T, 41)
+    m = L.Mesh(P, T)
+    ref = L.Operator(m, "V", 0, 0, "fp64", line_rule=3, tri_rule=tri_rule)
+    ref.set_schedule("chunked")
+    y_ref = ref(x)
+    op = L.Operator(m, "V", 0, 0, prec, line_rule=3, tri_rule=tri_rule)
+    op.set_schedule("symmetric", split)
+    y = op(x)
+    yo = oracle.apply("V", 0, 0, P, T, x, line_idx=3, tri_idx=tri_rule).ravel()
+    print("rel err vs chunked", rel(y, y_ref), "vs oracle", rel(y, yo))
+    assert rel(y, y_ref) <= (1e-14 if prec == "fp64" else 1e-6)
+    assert rel(y, yo) <= (1e-12 if prec == "fp64" else 1e-6)
+    assert np.array_equal(op(x), y)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_p0_factored_three_densities_through_d(L, prec):
+    """The hypersingular operator applies the P0 single layer to three curl
+    densities in one symmetric launch (the factored form with three
+    right-hand sides): equal to the same operator on the chunked
+    schedule."""
+    P, T, _ = meshes.perturbed_icosphere(3, 5)
+    m = L.Mesh(P, T)
+    u = np.random.default_rng(42).uniform(-1, 1, P.shape[0])
+    D64 = L.Operator(m, "D", 1, 1, "fp64", stab=1.0)
+    y0 = D64(u)
+    D = L.Operator(m, "D", 1, 1, prec, stab=1.0)
+    assert D.schedule == "symmetric"
+    y = D(u)
+    D.set_schedule("chunked")
+    yc = D(u)
+    print("symmetric vs fp64", rel(y, y0), "chunked vs fp64", rel(yc, y0))
+    assert rel(y, y0) <= (1e-14 if prec == "fp64" else 1e-6)
+    assert rel(yc, y0) <= (1e-14 if prec == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_p0_factored_in_passes(L, prec):
+    """Bounded partial sums (passes of block rows) give the one-pass
+    result bitwise in the factored form too."""
+    P, T = meshes.icosphere(4)
+    m = L.Mesh(P, T)
+    x = _x(0, P, T, 43)
+    op = L.Operator(m, "V", 0, 0, prec)
+    op.set_schedule("symmetric", 1)
+    y1 = op(x)
+    op.set_partial_limit(1 << 20)
+    assert op.partial["passes"] > 1
+    assert np.array_equal(op(x), y1)
