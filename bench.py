@@ -579,6 +579,35 @@ def mrhs_compare(L, meshes, level=6, reps=2):
                               ys["resident"], ys["pairs"]))}), flush=True)
 
 
+def operator_probe(L, meshes, level=6, reps=3):
+    """The double layer, its adjoint, the P0 single layer and the
+    hypersingular operator (a P0 single layer on three curl densities) in
+    both precisions on the icosphere: reps applies each after one warm-up,
+    wall time per apply from host timers around synchronized loops, one
+    JSON line per operator (the kernel trace of this workload is the
+    profile of the symmetric double-layer and P0 kernels)."""
+    P, T = meshes.icosphere(level)
+    m = L.Mesh(P, T)
+    rng = np.random.default_rng(3)
+    for name, kern, te, tr in (("K P1xP1", "K", 1, 1), ("Kt P1xP0", "Kt", 1, 0),
+                               ("V P0", "V", 0, 0), ("D P1", "D", 1, 1)):
+        for prec in ("fp32", "fp64"):
+            op = L.Operator(m, kern, te, tr, prec, line_rule=4, tri_rule=4,
+                            stab=1.0 if kern == "D" else 0.0)
+            x = L.DeviceArray.from_host(rng.uniform(-1, 1, op.ncols))
+            y = L.DeviceArray(op.nrows)
+            op.apply(x, y)
+            L.check(L.lib().ebh_synchronize())
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                op.apply(x, y)
+            L.check(L.lib().ebh_synchronize())
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            print(json.dumps({"operator": name, "precision": prec, "level": level,
+                              "schedule": op.schedule, "ms_per_apply": ms}), flush=True)
+            del op
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -611,6 +640,10 @@ def main():
     ap.add_argument("--mrhs-compare", action="store_true",
                     help="run only config 4's 64-RHS block matvec under both "
                          "multi-RHS schedules (no JSON bench line)")
+    ap.add_argument("--operator-probe", action="store_true",
+                    help="run only the double layer, adjoint, P0 single layer "
+                         "and hypersingular applies at icosphere L6 in both "
+                         "precisions (no JSON bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
@@ -625,6 +658,9 @@ def main():
         return
     if args.mrhs_compare:
         mrhs_compare(L, meshes)
+        return
+    if args.operator_probe:
+        operator_probe(L, meshes)
         return
     P, T = meshes.icosphere(args.level)
     x0 = np.array([0.0, 0.0, 2.0])
