@@ -616,10 +616,10 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--inner-tol", type=float, default=1e-9,
-                    help="tolerance of the FP32 inner solves (the default, below "
-                         "the target, lets one refinement pass reach it: the FP64 "
-                         "residual is then applied once, to certify the result)")
+    ap.add_argument("--inner-tol", type=float, default=1e-8,
+                    help="tolerance of the first FP32 inner solve (default: the "
+                         "target itself; its FP64 residual certifies the result "
+                         "or starts a second pass)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip configs 1, 3 and 4 (matvecs, batched solves), "
                          "which the default run times")

@@ -518,7 +518,7 @@ typedef struct {
   double inner_tol;    /* mixed solves: relative tolerance of the first FP32
                           pass; below 0.1 tol the first pass solves to it,
                           so that one FP64 residual can certify the solve
-                          (0: 1e-9 for Dirichlet, 1e-5 for Neumann)         */
+                          (0: 5e-9 for Dirichlet, 1e-5 for Neumann)         */
 } ebh_server_opts;
 
 typedef struct {

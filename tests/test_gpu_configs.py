@@ -6,6 +6,12 @@ apply against the golden vectors the compiled reference wrote
 centroids), and the Rayleigh quotient t'Vt / t'Mt of Y_2^0 approaching the
 sphere eigenvalue 1/5 from levels 2 to 4.
 
+Config 2 (icosphere L5 P1 Dirichlet, the headline): the mixed solve with
+the benchmark's settings (Calderon preconditioner, first FP32 pass to the
+target), its residual recomputed with a separate FP64 operator, its
+solution against the plain FP64 solve's, and a second refinement pass when
+the first pass is cut short.
+
 Config 4 (icosphere L6, 81920 triangles, P0 V, 64 right-hand sides in one
 launch): the block apply against the single-RHS apply column by column, the
 FP64 block apply against oracle rows, and the batched 64-RHS mixed solve to
@@ -29,6 +35,53 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+# ---------------------------------------------------------------- config 2
+@pytest.fixture(scope="module")
+def headline():
+    from easybem_amd import _lib as L
+    P, T = meshes.icosphere(5)
+    u = 1.0 / np.linalg.norm(P - np.array([0.0, 0.0, 2.0]), axis=1)
+    m = L.Mesh(P, T)
+    V64 = L.Operator(m, "V", 1, 1, "fp64")
+    V32 = L.Operator(m, "V", 1, 1, "fp32")
+    b = 0.5 * m.mass(u, 1) + L.Operator(m, "K", 1, 1, "fp64")(u)
+    s = L.Solver(V64, V32, restart=50)
+    s.set_precond(L.precond_calderon(m, "fp64", line_rule=2, tri_rule=0),
+                  L.precond_calderon(m, "fp32", line_rule=2, tri_rule=0))
+    return L, m, b, s
+
+
+def test_config2_mixed_solve_certified_by_fp64_residual(headline):
+    """First FP32 pass to the target (the benchmark's setting): one FP64
+    residual; the residual recomputed with an FP64 operator built here is
+    at most 1e-8, and the solution equals the FP64 solve's to the accuracy
+    both residuals allow."""
+    L, m, b, s = headline
+    x, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=1e-8)
+    print("mixed", st)
+    assert st["converged"] and st["relres"] <= 1e-8
+    V = L.Operator(m, "V", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    r = rel(V(x), b)
+    print("independent FP64 residual", r)
+    assert r <= 1e-8
+    x64, st64 = s.solve(b, tol=1e-8, mixed=False)
+    print("fp64", st64, "solutions differ by", rel(x, x64))
+    assert st64["converged"] and rel(x, x64) <= 1e-5
+
+
+def test_config2_second_pass_when_the_first_is_cut_short(headline):
+    """A first pass stopped at 1e-4 leaves an FP64 residual above the
+    target; the refinement then runs more passes until the FP64 residual
+    certifies the solve."""
+    L, m, b, s = headline
+    x, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=1e-4)
+    print(st)
+    assert st["converged"] and st["relres"] <= 1e-8
+    assert st["outer"] >= 2 and st["matvec64"] >= 2
+    V = L.Operator(m, "V", 1, 1, "fp64", line_rule=4, tri_rule=4)
+    assert rel(V(x), b) <= 1e-8
 
 
 # ---------------------------------------------------------------- config 1

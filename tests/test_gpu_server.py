@@ -22,7 +22,7 @@ def stream(n, seed=0, levels=(2, 3, 4)):
 def test_server_result_matches_direct_solve(inner_tol):
     """A served P0 Dirichlet solve equals the same mixed solve run
     directly, with the server's default first-pass tolerance for Dirichlet
-    requests (1e-9: one FP64 residual) and with one given."""
+    requests (5e-9: one FP64 residual) and with one given."""
     from easybem_amd import _lib as L
     P, T, x0 = stream(1, seed=5, levels=(3,))[0]
     srv = L.Server(max_slots=1, inner_tol=inner_tol)
@@ -40,7 +40,7 @@ def test_server_result_matches_direct_solve(inner_tol):
     S = L.precond_spai(V64, rings=2)
     s = L.Solver(V64, V32)
     s.set_precond(S, S)
-    t2, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=inner_tol or 1e-9)
+    t2, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=inner_tol or 5e-9)
     assert np.array_equal(t, t2)
 
 
@@ -185,8 +185,8 @@ def test_dirichlet_p1_server_matches_preconditioned_direct_solve():
     s = L.Solver(V64, V32)
     s.set_precond(L.precond_calderon(m, "fp64", line_rule=2, tri_rule=0),
                   L.precond_calderon(m, "fp32", line_rule=2, tri_rule=0))
-    # (the server's first FP32 pass of a Dirichlet solve goes to 1e-9)
-    t2, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=1e-9)
+    # (the server's first FP32 pass of a Dirichlet solve goes to 5e-9)
+    t2, st = s.solve(b, tol=1e-8, mixed=True, inner_tol=5e-9)
     # right-hand sides formed by the server and here differ in the last bits
     err = np.linalg.norm(t - t2) / np.linalg.norm(t2)
     print("server vs direct", err)
