@@ -117,6 +117,58 @@ def _sampled_matvec(o, P, T, threads, seconds, seed, op="V", basis=1, nu=0.3):
     return elapsed / nrows * ntest, nrows, elapsed
 
 
+def _quiet_fd1(fn):
+    """Runs fn with the process's file descriptor 1 on /dev/null (the
+    reference's C++ code prints its assembly progress there)."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    null = os.open(os.devnull, os.O_WRONLY)
+    try:
+        os.dup2(null, 1)
+        return fn()
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(null)
+        os.close(saved)
+
+
+def reference_cpu():
+    """The reference's own CPU path (getLaplace3d compiled from its sources
+    into oracle/_ref; single-threaded, as written): dense assembly of V
+    (P0), K, D and M at order 4 measured whole on icosphere L3 and
+    extrapolated to config 2's mesh (L5) by the triangle-pair count, plus
+    a dense LU solve of an L5-sized P0 system extrapolated from one of
+    L4's size.  Its CPU code has no P1 single layer, so this is the P0
+    formulation of config 2's Dirichlet problem (its CPU example's)."""
+    from oracle import Reference
+    from easybem_amd import meshes
+    if not Reference.available():
+        return {"available": False,
+                "note": "oracle/_ref not built (the reference sources were absent)"}
+    r = Reference()
+    P3, T3 = meshes.icosphere(3)
+    t0 = time.perf_counter()
+    _quiet_fd1(lambda: r.laplace3d(P3, T3, 4))
+    t_asm3 = time.perf_counter() - t0
+    ne3, ne4, ne5 = T3.shape[0], 4 * T3.shape[0], 16 * T3.shape[0]
+    A = np.random.default_rng(5).standard_normal((ne4, ne4)) + ne4 * np.eye(ne4)
+    t0 = time.perf_counter()
+    np.linalg.solve(A, np.ones(ne4))
+    t_lu4 = time.perf_counter() - t0
+    t_asm5 = t_asm3 * (ne5 / ne3) ** 2
+    t_lu5 = t_lu4 * (ne5 / ne4) ** 3
+    return {"kind": "reference", "cores": 1, "value": 1.0 / (t_asm5 + t_lu5),
+            "unit": "solves/s", "assembly_s_ico3_measured": t_asm3,
+            "assembly_s_ico5_extrapolated": t_asm5, "lu_s_ico5_extrapolated": t_lu5,
+            "sample": f"getLaplace3d (V P0, K, D, M, order 4) on icosphere L3 "
+                      f"({ne3} triangles, {t_asm3:.2f} s), times {(ne5 / ne3) ** 2:.0f} "
+                      f"pairs for L5; numpy LU of a {ne4}-square system "
+                      f"({t_lu4:.2f} s), times {(ne5 / ne4) ** 3:.0f} for L5's P0 system"}
+
+
 def cpu_baseline(P, T, n_matvec, seconds=10.0):
     """The oracle (FP64 C restatement of the reference CPU operator) on the
     host cores, 1 thread (as the single-threaded reference) and all cores
@@ -171,7 +223,7 @@ def cpu_baseline(P, T, n_matvec, seconds=10.0):
                       "GPU solve's operator and preconditioner applies); config 1 "
                       "full matvecs and dense solve measured whole; configs 3 "
                       "and 4 row-sampled on all cores, extrapolated",
-            "config1_ico3_p0_V": c1, **c34}
+            "config1_ico3_p0_V": c1, **c34, "reference_cpu_path": reference_cpu()}
 
 
 def secondary(L, meshes, solve_batched=False):
