@@ -403,11 +403,20 @@ struct ebh_solver {
   // starting at the column's own row) and the reduced right-hand sides
   std::vector<double> hR, hV, hbeta, hG, hH, hS;
 
+  // Captured steps hold the addresses of the operators' and the solver's
+  // buffers (the kept z_j among them); they go whenever the
+  // preconditioners change.  The stream must be idle.
+  void drop_graphs()
+  {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+
   ~ebh_solver()
   {
     if (stream) (void)hipStreamSynchronize(stream);
     if (blas) (void)rocblas_destroy_handle(blas);
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    drop_graphs();
     if (h_est) (void)gated([&] { return hipHostFree(h_est); });
     if (h_active) (void)gated([&] { return hipHostFree(h_active); });
     if (h_jdone) (void)gated([&] { return hipHostFree(h_jdone); });
@@ -1144,6 +1153,10 @@ int ebh_solver_set_precond(ebh_solver *s, ebh_op *p64, ebh_op *p32)
     }
   }
   EBH_HIP(hipStreamSynchronize(s->stream));
+  // steps captured with the old preconditioners (or the old z_j buffer,
+  // released below) must not replay: a preconditioner set again, or a new
+  // operator at a freed one's address, would match their keys
+  s->drop_graphs();
   const size_t tot = (size_t)s->nbmax * s->n;
   if (p64 || p32) {
     EBH_TRY(s->Zt.alloc(tot));

@@ -198,6 +198,39 @@ def test_preconditioned_graph_replays_match_eager():
         assert np.array_equal(x, out[0])
 
 
+def test_graphs_dropped_when_preconditioners_change():
+    """Captured Arnoldi steps hold the addresses of the preconditioner's
+    buffers and of the kept z_j (released when the preconditioners are
+    removed): after removing the preconditioner and setting it again, and
+    after switching to another one and back, the solves replay fresh
+    captures and equal the first solve bit for bit."""
+    from easybem_amd import _lib as L
+    from easybem_amd import meshes
+    P, T = meshes.icosphere(3)
+    m = L.Mesh(P, T)
+    V64 = L.Operator(m, "V", 1, 1, "fp64")
+    b = np.random.default_rng(4).uniform(-1, 1, P.shape[0])
+    pc = L.precond_calderon(m, "fp64", line_rule=2, tri_rule=0)
+    pc2 = L.precond_calderon(m, "fp64", line_rule=3, tri_rule=1)
+    s = L.Solver(V64, graphs=True)
+    s.set_precond(pc, None)
+    x0, st0 = s.solve(b, tol=1e-10, mixed=False)
+    s.set_precond(None, None)
+    x1, st1 = s.solve(b, tol=1e-10, mixed=False)
+    assert st1["iters"] > st0["iters"]
+    s.set_precond(pc, None)
+    x2, st2 = s.solve(b, tol=1e-10, mixed=False)
+    s.set_precond(pc2, None)
+    s.solve(b, tol=1e-10, mixed=False)
+    s.set_precond(pc, None)
+    x3, st3 = s.solve(b, tol=1e-10, mixed=False)
+    x4, st4 = s.solve(b, tol=1e-10, mixed=False)
+    print(st0, st2, st3, st4)
+    assert st4["graph_steps"] > 0
+    for x in (x2, x3, x4):
+        assert np.array_equal(x, x0)
+
+
 def test_batched_calderon_solve_matches_individual_solves():
     """Four right-hand sides in one batched mixed solve with the Calderon
     preconditioner built for four (its Chebyshev mass inverse and the
