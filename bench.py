@@ -305,7 +305,8 @@ def secondary(L, meshes, solve_batched=False):
                 "ms": mfma_ms, "bound": "mfma",
                 "tflops": flop_mfma / mfma_ms / 1e9,
                 "frac": flop_mfma / mfma_ms / 1e9 / PEAK_FP64_MFMA_TFLOPS},
-            # BASELINE's convention (9 + 2 nrhs FLOP per ordered point pair)
+            # the multi-RHS counting convention (9 + 2 nrhs FLOP per ordered
+            # point pair)
             # divided by the far time: above the peaks because the P0
             # densities factor per triangle pair and each unordered pair's
             # point sums are formed once
