@@ -716,7 +716,7 @@ def main():
                          "precisions (no JSON bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must "
+        print(json.dumps({"error": "easybem-hip runs on one GPU; --gpus must "
                           "be 1 (multi-GPU sharding is a design note only)"}))
         sys.exit(2)
 
