@@ -45,6 +45,21 @@ def test_apply_matches_oracle_on_shaft(L, oracle, kernel, test, trial, prec, tol
     assert err <= tol
 
 
+@pytest.mark.parametrize("trial", [0, 1])
+def test_fp32_adjoint_double_layer_on_shaft(L, trial):
+    """The FP32 adjoint double layer (pair-symmetric, hi/lo centroid
+    differences) on a surface with sharp edges: against the FP64 adjoint,
+    itself checked against the reference's K^T elsewhere."""
+    P, T = shaft(1)
+    x = np.random.default_rng(3).uniform(-1, 1, P.shape[0] if trial else T.shape[0])
+    m = L.Mesh(P, T)
+    y64 = L.Operator(m, "Kt", 1, trial, "fp64", line_rule=4, tri_rule=4)(x)
+    y32 = L.Operator(m, "Kt", 1, trial, "fp32", line_rule=4, tri_rule=4)(x)
+    err = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
+    print("Kt fp32 vs fp64 on shaft1", err)
+    assert err <= 1e-6
+
+
 def test_dirichlet_and_neumann_solves_on_shaft(L):
     """u = 1/|x - x0| with x0 just outside the shaft (6 units above its top
     face; far sources make u nearly constant, whose zero flux the P0
