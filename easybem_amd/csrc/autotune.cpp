@@ -87,10 +87,10 @@ int time_apply(ebh_op *op, const double *x, double *y, bool multi, int reps,
 int grid_of(const ebh_op *op)
 {
   if (op->sym) {
-    const int ng = op->chunks / op->ksplit;
+    const int ng = op->chunks / sym_slots(op);
     int nblk = 0;
     for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
-    return nblk * op->ksplit;
+    return nblk * sym_slots(op);
   }
   const int tb = (op->mesh->T.ne + op->block - 1) / op->block;
   return tb * op->chunks;

@@ -269,6 +269,8 @@ struct ebh_op {
   int block = 256, tile = 64, chunks = 1, chunk = 0;
   bool sym = false;  // pair-symmetric far-field schedule (k_far_sym)
   int ksplit = 1;    // its step ranges per group pair
+  int qsplit = 1;    // target-point halves per group pair (FP64 Kelvin with
+                     // 6 or 7 points: 2); ksplit * qsplit slots per pair
   // its partial sums reduced inside the near-field pass (k_reduce_finish)
   // instead of by k_reduce_wide before k_finish (ebh_op_set_reduction)
   bool fused_req = false;
@@ -308,6 +310,8 @@ int op_build_far(ebh_op *op);
 // the mesh); only for operators that sym_eligible accepts
 int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym = false);
 bool sym_eligible(const ebh_op *op);
+// partial-sum slots per group pair of the pair-symmetric schedule
+inline int sym_slots(const ebh_op *op) { return op->ksplit * op->qsplit; }
 // false for operators that run on the pair-symmetric schedule only (the
 // FP32 adjoint double layer)
 bool chunked_ok(const ebh_op *op);

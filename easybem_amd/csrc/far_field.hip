@@ -656,7 +656,14 @@ using SymLay = SymLayout<NT, NR, DK == 1 || DK == 2,
 // with the roles swapped: target point q takes sigma_b sf_b / r^3, source
 // point b takes sigma_q tf_q / r^3, so the strengths are scaled once per
 // step.  The strengths are the single layer's (w J u_h / 4 pi per point).
-template <typename R, int NT, int NR, int DK = 0>
+//
+// QS = 2 (the FP64 Kelvin single layer with 6 or 7 points, whose target and
+// source sums would not fit in 256 registers): a workgroup takes the target
+// points Q0 .. Q0 + NQ - 1 of its triangles (QH = 0: the first half, 1: the
+// rest) against every source point; the two halves are two step-range slots
+// of the partial sums (slot index rng = step range * QS + QH), which the
+// reduction adds like any other.
+template <typename R, int NT, int NR, int DK = 0, int QS = 1, int QH = 0>
 __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
                                           R *__restrict__ lds)
 {
@@ -664,8 +671,12 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   static_assert(!DL || NR == 1, "double layer: one right-hand side");
   static_assert(!KEL || NR == 3, "Kelvin: three components");
   static_assert(!PF || NT > 1, "P0-factored form: rules of several points");
-  // point sums per side: NT, or one per triangle in the P0-factored form
-  constexpr int NS = PF ? 1 : NT;
+  static_assert(QS == 1 || (KEL && NT > 1), "target-point halves: Kelvin only");
+  // target points of this workgroup: Q0 .. Q0 + NQ - 1 (those below NT)
+  constexpr int NQ = (NT + QS - 1) / QS, Q0 = QH * NQ;
+  // point sums per side: NQ target and NT source points, or one per triangle
+  // in the P0-factored form
+  constexpr int NA = PF ? 1 : NQ, NB = PF ? 1 : NT;
   constexpr int RW = NR == 1 ? 4 : 8;
   using RecT = SymRec<R, NR>;
   using Lay = SymLay<R, NT, NR, DK>;
@@ -695,7 +706,8 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   const int rng = bid - rem * ks;
   while (rem >= ng - 4 * M) { rem -= ng - 4 * M; M++; }
   const int J = 4 * M + rem;
-  const int k0 = rng * 64 / ks, k1 = (rng + 1) * 64 / ks;
+  // step range rng / QS of ks / QS (slot rng: the range and the target half)
+  const int k0 = rng / QS * 64 / (ks / QS), k1 = (rng / QS + 1) * 64 / (ks / QS);
   // FP64 works with doubled positions, so the distances it forms are 2r and
   // rsqrt gives 1/(2r); a Newton step from that estimate needs one multiply
   // fewer (see below).  FP32 uses the positions as they are.  Centroids and
@@ -783,7 +795,8 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     Cly = a.tsl[(9 + NT + 1) * (size_t)ne + tt];
     Clz = a.tsl[(9 + NT + 2) * (size_t)ne + tt];
   }
-  R E[NT], sq[NT][NR], rc[NT], hc[NT];
+  // (sq: the strengths of this workgroup's target points Q0 + qi)
+  R E[NT], sq[NQ][NR], rc[NT], hc[NT];
   // P0-factored form: the target's point weights w_q J and densities
   R wt[PF ? NT : 1], st[PF ? NR : 1];
 #pragma unroll
@@ -791,12 +804,15 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
     E[q] = ps * ps * a.tsl[(9 + q) * (size_t)ne + tt];
     rc[q] = ps * a.rc[q];
     hc[q] = R(0.5) * a.rc[q];  // double layer: c_q m.n = (c_q / 2) F.n
-    if constexpr (PF) {
-      wt[q] = (R)a.tw[(size_t)q * ne + tt];
-    } else {
+    if constexpr (PF) wt[q] = (R)a.tw[(size_t)q * ne + tt];
+  }
+  if constexpr (!PF) {
+#pragma unroll
+    for (int qi = 0; qi < NQ; qi++) {
+      const int q = Q0 + qi;
 #pragma unroll
       for (int r = 0; r < NR; r++)
-        sq[q][r] = t < ne ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
+        sq[qi][r] = t < ne && q < NT ? a.src[RW * ((size_t)tt * NT + q) + 3 + r] : R(0);
     }
   }
   if constexpr (PF) {
@@ -827,37 +843,45 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   // Kelvin: target point q = C + o_q with o_q = (c_q / 2) F_k (doubled in
   // FP64, as C), and kappa times its strengths (the source side's
   // kappa-diagonal term)
-  R oq[KEL ? NT : 1][3], ksq[KEL ? NT : 1][3];
+  R oq[KEL ? NQ : 1][3], ksq[KEL ? NQ : 1][3];
   if constexpr (KEL) {
 #pragma unroll
-    for (int q = 0; q < NT; q++) {
+    for (int qi = 0; qi < NQ; qi++) {
+      const int q = Q0 + qi < NT ? Q0 + qi : NT - 1;
       const int k = q < C0 ? 0 : (q - C0) % 3;
       const R h = q < C0 ? R(0) : R(0.5) * rc[q];
-      oq[q][0] = h * (k == 0 ? F0x : k == 1 ? F1x : -(F0x + F1x));
-      oq[q][1] = h * (k == 0 ? F0y : k == 1 ? F1y : -(F0y + F1y));
-      oq[q][2] = h * (k == 0 ? F0z : k == 1 ? F1z : -(F0z + F1z));
+      oq[qi][0] = h * (k == 0 ? F0x : k == 1 ? F1x : -(F0x + F1x));
+      oq[qi][1] = h * (k == 0 ? F0y : k == 1 ? F1y : -(F0y + F1y));
+      oq[qi][2] = h * (k == 0 ? F0z : k == 1 ? F1z : -(F0z + F1z));
 #pragma unroll
-      for (int r = 0; r < 3; r++) ksq[q][r] = a.kappa * sq[q][r];
+      for (int r = 0; r < 3; r++) ksq[qi][r] = a.kappa * sq[qi][r];
     }
   }
-  R acc[NS][NR], S[NS][NR];
+  R acc[NA][NR], S[NB][NR];
   // FP64 sums of the flushed steps (FL); the source side's stay aligned
   // with S: after a flush at step k, lane L holds source (L + k) mod 64 in both
-  double acc64[NS][NR], S64[NS][NR];
+  double acc64[NA][NR], S64[NB][NR];
 #pragma unroll
-  for (int q = 0; q < NS; q++)
+  for (int r = 0; r < NR; r++) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-      acc[q][r] = S[q][r] = R(0);
-      acc64[q][r] = S64[q][r] = 0.0;
+    for (int q = 0; q < NA; q++) {
+      acc[q][r] = R(0);
+      acc64[q][r] = 0.0;
     }
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      S[b][r] = R(0);
+      S64[b][r] = 0.0;
+    }
+  }
   // FL: moves the FP32 sums into the FP64 ones; the source side's FP64 sums
   // first move down `shift` lanes (the steps since the last flush)
+  static_assert(!FL || NA == NB, "flushed sums: one target and source point set");
   auto flush = [&](int shift, auto sym_tag) {
     constexpr bool SYM = decltype(sym_tag)::value;
     const int from = ((L + shift) & 63) << 2;
 #pragma unroll
-    for (int q = 0; q < NS; q++)
+    for (int q = 0; q < NA; q++)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         acc64[q][r] += (double)acc[q][r];
@@ -1001,10 +1025,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
             const R *sb = g + Lay::SIG + b * NR;
             const R kb[3] = {a.kappa * sb[0], a.kappa * sb[1], a.kappa * sb[2]};
 #pragma unroll
-            for (int q = 0; q < NT; q++) {
-              const R dx = Dbx + oq[q][0], dy = Dby + oq[q][1], dz = Dbz + oq[q][2];
+            for (int qi = 0; qi < NQ; qi++) {
+              if (Q0 + qi >= NT) continue;  // (compile-time: the second half's end)
+              const R dx = Dbx + oq[qi][0], dy = Dby + oq[qi][1], dz = Dbz + oq[qi][2];
               kelvin_pair(dx, dy, dz, fma(dx, dx, fma(dy, dy, fma(dz, dz, excl))), sb, kb,
-                          q, b, sym_tag);
+                          qi, b, sym_tag);
             }
           }
         } else {
@@ -1101,7 +1126,7 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       }
       if constexpr (SYM) {
 #pragma unroll
-        for (int b = 0; b < NS; b++)
+        for (int b = 0; b < NB; b++)
 #pragma unroll
           for (int r = 0; r < NR; r++) {
             if constexpr (ROT) {
@@ -1127,24 +1152,29 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
   else steps(std::false_type{});
   if constexpr (!FL) {
 #pragma unroll
-    for (int q = 0; q < NS; q++)
+    for (int r = 0; r < NR; r++) {
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        acc64[q][r] = (double)acc[q][r];
-        S64[q][r] = (double)S[q][r];
-      }
+      for (int q = 0; q < NA; q++) acc64[q][r] = (double)acc[q][r];
+#pragma unroll
+      for (int b = 0; b < NB; b++) S64[b][r] = (double)S[b][r];
+    }
   }
 
   // point sums of triangle tri to partial row c: the target side's
   // (partial[c - c0][r][.][tri - t0]) or the source side's (spart[...])
-  auto store = [&](const double (*v)[NR], int tri, int c, bool src_side) {
+  // (v: NV point sums of points q0 + qi -- the target side's NA from Q0,
+  // the source side's NB from 0)
+  auto store = [&](const auto &v, auto q0_tag, int tri, int c, bool src_side) {
+    constexpr int NV = std::extent<std::remove_reference_t<decltype(v)>>::value;
+    constexpr int q0 = decltype(q0_tag)::value;
     const size_t stride = (size_t)ne;
     double *base = src_side ? a.spart : a.partial;
     const int rowv = src_side ? a.sprow : a.prow, cols = src_side ? a.scols : a.tcols;
     const int tl = tri - a.t0;
-    double wq[NT];
+    double wq[NV];
 #pragma unroll
-    for (int q = 0; q < NT; q++) wq[q] = a.tw[q * stride + tri];
+    for (int qi = 0; qi < NV; qi++)
+      wq[qi] = q0 + qi < NT ? a.tw[(q0 + qi) * stride + tri] : 0.0;
     double *row = base + (size_t)(c - a.c0) * NR * rowv;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -1154,14 +1184,15 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       } else if (a.nout == 1) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < NT; q++) s = fma(wq[q], v[q][r], s);
+        for (int qi = 0; qi < NV; qi++) s = fma(wq[qi], v[qi][r], s);
         row[(size_t)r * rowv + tl] = s;
       } else {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           double s = 0.0;
 #pragma unroll
-          for (int q = 0; q < NT; q++) s = fma(wq[q] * a.phi[q][k], v[q][r], s);
+          for (int qi = 0; qi < NV; qi++)
+            if (q0 + qi < NT) s = fma(wq[qi] * a.phi[q0 + qi][k], v[qi][r], s);
           // planes: right-hand side r, then output k; Kelvin: output k, then
           // component r (the chunked kernel's order)
           if constexpr (KEL) row[((size_t)k * NR + r) * cols + tl] = s;
@@ -1170,11 +1201,11 @@ __device__ __forceinline__ void sym_block(const FarArgs<R> &a, int bid,
       }
     }
   };
-  if (t < ne) store(acc64, t, J * ks + rng, false);
+  if (t < ne) store(acc64, std::integral_constant<int, Q0>{}, t, J * ks + rng, false);
   if (I < J) {
     // after the steps k0 .. k1-1 lane L holds source (L + k1) mod 64
     const int j = J * 64 + ((L + k1) & 63);
-    if (j < ne) store(S64, j, I * ks + rng, true);
+    if (j < ne) store(S64, std::integral_constant<int, 0>{}, j, I * ks + rng, true);
   }
 }
 
@@ -1194,14 +1225,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // The Kelvin single layer (three components) on the symmetric schedule.
 // FP64 holds 7 x 3 target sums, strengths, kappa-scaled strengths and point
 // offsets plus the source side's sums in registers; with 6 or 7 points that
-// is more than 256 per lane (one wave per SIMD, spills at 7), so FP64 takes
-// it for rules of at most 4 points (sym_eligible) and the chunked kernel
-// otherwise.
+// is more than 256 per lane (391 at 7: one wave per SIMD and spills), so
+// FP64 takes those rules in two target-point halves (sym_block QS = 2), each
+// workgroup one half: slot rng = step range * 2 + half.
 template <typename R, int NT>
-__global__ __launch_bounds__(256) void k_far_sym_kel(FarArgs<R> a)
+struct KelQS { static constexpr int v = sizeof(R) == 8 && NT >= 6 ? 2 : 1; };
+template <typename R, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_far_sym_kel(
+    FarArgs<R> a)
 {
   __shared__ __align__(16) R lds[SymLay<R, NT, 3, 3>::LDS];
-  sym_block<R, NT, 3, 3>(a, (int)blockIdx.x, lds);
+  constexpr int QS = KelQS<R, NT>::v;
+  if constexpr (QS == 1) {
+    sym_block<R, NT, 3, 3>(a, (int)blockIdx.x, lds);
+  } else {
+    if ((int)blockIdx.x % a.ksplit % QS == 0)
+      sym_block<R, NT, 3, 3, QS, 0>(a, (int)blockIdx.x, lds);
+    else
+      sym_block<R, NT, 3, 3, QS, 1>(a, (int)blockIdx.x, lds);
+  }
 }
 
 // Persistent multi-mesh form of the symmetric schedule: the workgroups of
@@ -2173,7 +2215,7 @@ void fill_far_args(const ebh_op *op, FarArgs<R> &a)
   a.chunks = op->chunks;
   a.chunk = op->chunk;
   a.tile = op->tile;
-  a.ksplit = op->sym ? op->ksplit : 1;
+  a.ksplit = op->sym ? sym_slots(op) : 1;
   a.nout = op->d.test_basis ? 3 : 1;
   // P0 test and trial functions of the single layer on the symmetric
   // schedule: one kernel sum per triangle pair (rules of several points)
@@ -2258,7 +2300,7 @@ int launch_sym(int nt, dim3 grid, hipStream_t s, const FarArgs<R> &a)
 template <typename R>
 int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
 {
-  const int ks = op->ksplit, ng = op->chunks / ks;
+  const int ks = sym_slots(op), ng = op->chunks / ks;
   if (op->d.kernel == EBH_KELVIN_SL) nr = 3;  // the three components
   auto launch = [&](dim3 grid, const FarArgs<R> &b) -> int {
     if (op->d.kernel == EBH_KELVIN_SL) {
@@ -2266,18 +2308,8 @@ int run_far_sym(ebh_op *op, hipStream_t s, const FarArgs<R> &a, int nr = 1)
       case 1: hipLaunchKernelGGL((k_far_sym_kel<R, 1>), grid, dim3(256), 0, s, b); break;
       case 3: hipLaunchKernelGGL((k_far_sym_kel<R, 3>), grid, dim3(256), 0, s, b); break;
       case 4: hipLaunchKernelGGL((k_far_sym_kel<R, 4>), grid, dim3(256), 0, s, b); break;
-      case 6:
-        if constexpr (sizeof(R) == 4) {
-          hipLaunchKernelGGL((k_far_sym_kel<R, 6>), grid, dim3(256), 0, s, b);
-          break;
-        }
-        [[fallthrough]];
-      case 7:
-        if constexpr (sizeof(R) == 4) {
-          hipLaunchKernelGGL((k_far_sym_kel<R, 7>), grid, dim3(256), 0, s, b);
-          break;
-        }
-        [[fallthrough]];
+      case 6: hipLaunchKernelGGL((k_far_sym_kel<R, 6>), grid, dim3(256), 0, s, b); break;
+      case 7: hipLaunchKernelGGL((k_far_sym_kel<R, 7>), grid, dim3(256), 0, s, b); break;
       default:
         set_error("no pair-symmetric Kelvin kernel for rule size " +
                   std::to_string(op->rule.nt));
@@ -2773,8 +2805,7 @@ bool sym_eligible(const ebh_op *op)
   // the Kelvin single layer with equal bases (its three components are the
   // launch's three right-hand-side planes)
   const bool kel = op->d.kernel == EBH_KELVIN_SL && op->d.test_basis == op->d.trial_basis &&
-                   op->nr_max == 1 &&
-                   (op->d.far_precision == EBH_FP32 || op->rule.nt <= 4);
+                   op->nr_max == 1;
   return (sl || dl || kel) && op->ray && op->rule.nt <= 7;
 }
 
@@ -2791,7 +2822,7 @@ bool chunked_ok(const ebh_op *op)
 // pass_tsz / pass_ssz: the largest buffers, values per right-hand side.
 static void plan_passes(ebh_op *op, size_t nout)
 {
-  const size_t ne = op->mesh->T.ne, ks = op->ksplit;
+  const size_t ne = op->mesh->T.ne, ks = sym_slots(op);
   const size_t ng = op->chunks / ks, nrow = (ng + 3) / 4;
   auto sizes = [&](size_t m0, size_t m1, size_t &tsz, size_t &ssz) {
     const size_t t0 = 256 * m0;
@@ -2897,7 +2928,12 @@ int op_configure(ebh_op *op, int block, int tile, int chunks, bool sym)
     block = 256;
     tile = 64;
     op->ksplit = ks;
-    chunks = ng * ks;
+    // FP64 Kelvin with 6 or 7 points: two target-point halves per pair
+    op->qsplit = op->d.kernel == EBH_KELVIN_SL && op->d.far_precision == EBH_FP64 &&
+                         op->rule.nt >= 6 ? 2 : 1;
+    chunks = ng * ks * op->qsplit;
+  } else {
+    op->qsplit = 1;
   }
   op->sym = sym;
   if (block) {
@@ -3541,9 +3577,9 @@ int batch_prepare(ebh_batch *b)
     const ebh_op *op = b->ops[k];
     int nblk = 0;
     if (op->sym) {
-      const int ng = op->chunks / op->ksplit;
+      const int ng = op->chunks / sym_slots(op);
       for (int M = 0; 4 * M < ng; M++) nblk += ng - 4 * M;
-      nblk *= op->ksplit;
+      nblk *= sym_slots(op);
     }
     soff[k + 1] = soff[k] + nblk;
   }

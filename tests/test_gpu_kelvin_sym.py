@@ -3,8 +3,8 @@
 The Kelvin kernel kappa I / r + d d^T / r^3 (src/BEM3dElasticity.cpp:289-348
 and 838-846 for its blocks) is even in d = x - y, so one rsqrt and one d per
 point pair serve both directions.  The symmetric kernel
-(k_far_sym_kel) is the default for equal bases in FP32 with every median
-rule and in FP64 with rules of at most 4 points.  Checked against the
+(k_far_sym_kel) is the default for equal bases with every median rule (FP64
+with 6 or 7 points in two target-point halves).  Checked against the
 chunked kernel (FP64 <= 1e-13, FP32 <= 2e-6) and against oracle rows
 (FP64 <= 1e-12, FP32 <= 1e-6) for P0 and P1, with passes (bitwise equal to
 one pass), and on the benchmark's 256 x 128 torus with P1 functions.
@@ -34,7 +34,7 @@ MESHES = {
     "ico2": lambda: meshes.icosphere(2),
     "blob3": lambda: meshes.perturbed_icosphere(3, 7)[:2],
 }
-CASES = [("fp64", r) for r in (0, 1, 2)] + [("fp32", r) for r in (0, 1, 2, 3, 4)]
+CASES = [("fp64", r) for r in (0, 1, 2, 3, 4)] + [("fp32", r) for r in (0, 1, 2, 3, 4)]
 
 
 @pytest.mark.parametrize("mesh", list(MESHES))
@@ -62,16 +62,28 @@ def test_kelvin_symmetric_matches_chunked_and_oracle(L, oracle, mesh, basis, pre
     assert e_o <= (1e-12 if prec == "fp64" else 1e-6)
 
 
-def test_kelvin_fp64_large_rules_stay_chunked(L):
-    P, T = meshes.icosphere(2)
+@pytest.mark.parametrize("rule", [3, 4])
+@pytest.mark.parametrize("split", [1, 2, 8])
+def test_kelvin_fp64_target_halves(L, rule, split):
+    """FP64 with 6 or 7 points runs the symmetric schedule in two
+    target-point halves (two partial slots per step range): every step
+    split gives the chunked kernel's apply to 1e-13, and the split reads
+    back as set."""
+    P, T = meshes.torus(1.0, 0.4, 24, 12)
     m = L.Mesh(P, T)
-    op = L.Operator(m, "kelvin", 1, 1, "fp64", line_rule=4, tri_rule=4, nu=NU)
-    assert op.schedule == "chunked"
-    with pytest.raises(L.EbhError):
-        op.set_schedule("symmetric")
+    x = np.random.default_rng(6).uniform(-1, 1, 3 * P.shape[0])
+    op = L.Operator(m, "kelvin", 1, 1, "fp64", line_rule=4, tri_rule=rule, nu=NU)
+    op.set_schedule("chunked")
+    yc = op(x)
+    op.set_schedule("symmetric", split)
+    assert op.schedule == "symmetric" and op.split == split
+    y = op(x)
+    print(f"rule {rule} split {split}: symmetric vs chunked {rel(y, yc):.2e}")
+    assert rel(y, yc) <= 1e-13
+    assert np.array_equal(op(x), y)
 
 
-@pytest.mark.parametrize("prec,rule", [("fp32", 4), ("fp64", 2)])
+@pytest.mark.parametrize("prec,rule", [("fp32", 4), ("fp64", 2), ("fp64", 4)])
 def test_kelvin_passes_bitwise_equal_one_pass(L, prec, rule):
     P, T = meshes.torus(1.0, 0.4, 40, 20)
     m = L.Mesh(P, T)

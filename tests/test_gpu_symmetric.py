@@ -107,7 +107,7 @@ def test_symmetric_split_rejected(L):
 
 @pytest.mark.parametrize("kernel,test,trial,tri_rule,prec", [
     ("K", 0, 1, 5, "fp32"), ("K", 1, 1, 5, "fp64"), ("V", 0, 1, 4, "fp64"),
-    ("V", 1, 0, 4, "fp64"), ("kelvin", 0, 0, 4, "fp64"), ("V", 0, 0, 5, "fp64")])
+    ("V", 1, 0, 4, "fp64"), ("kelvin", 0, 0, 5, "fp64"), ("V", 0, 0, 5, "fp64")])
 def test_symmetric_rejected_where_it_does_not_apply(L, kernel, test, trial,
                                                     tri_rule, prec):
     P, T = meshes.icosphere(1)
