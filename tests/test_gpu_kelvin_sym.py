@@ -116,3 +116,25 @@ def test_kelvin_config3_torus(L, oracle):
     e_c, e_o = rel(y, yc), rel(yr, yo)
     print(f"config 3: symmetric vs chunked {e_c:.2e}, vs oracle rows {e_o:.2e}")
     assert e_c <= 2e-6 and e_o <= 1e-6
+
+
+def test_kelvin_config3_torus_fp64(L, oracle):
+    """The same torus in FP64 with the 7-point rule: the symmetric schedule
+    (two target-point halves) against chunked and oracle rows."""
+    P, T = meshes.torus(1.0, 0.4, 256, 128)
+    m = L.Mesh(P, T)
+    op = L.Operator(m, "kelvin", 1, 1, "fp64", line_rule=4, tri_rule=4, nu=NU)
+    assert op.schedule == "symmetric"
+    n = P.shape[0]
+    x = np.random.default_rng(4).uniform(-1, 1, 3 * n)
+    print(f"case torus fp64, partial {op.partial}", flush=True)
+    y = op(x)
+    op.set_schedule("chunked")
+    yc = op(x)
+    rows = np.arange(0, n, 8209, dtype=np.int32)
+    yo = np.asarray(oracle.apply("kelvin", 1, 1, P, T, x, rows=rows, line_idx=4, tri_idx=4,
+                                 nu=NU))
+    yr = np.stack([y[c * n + rows] for c in range(3)])
+    e_c, e_o = rel(y, yc), rel(yr, yo)
+    print(f"torus fp64: symmetric vs chunked {e_c:.2e}, vs oracle rows {e_o:.2e}")
+    assert e_c <= 1e-13 and e_o <= 1e-12
