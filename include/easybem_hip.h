@@ -145,8 +145,9 @@ int ebh_op_get_launch(const ebh_op *op, int *block, int *tile, int *chunks);
 /* Far-field schedule.  EBH_SCHED_CHUNKED: every target triangle against
  * source chunks (the shape set by ebh_op_set_launch).  EBH_SCHED_SYMMETRIC:
  * each unordered pair of 64-triangle groups once, one rsqrt per point pair
- * serving both directions; only for EBH_LAPLACE_SL with equal test and trial
- * bases and a median-symmetric far rule (triangle rule indices 0-4),
+ * serving both directions; for a median-symmetric far rule (triangle rule
+ * indices 0-4) with EBH_LAPLACE_SL or EBH_KELVIN_SL on equal test and trial
+ * bases, and EBH_LAPLACE_DL (P1 x P1) or EBH_LAPLACE_ADL (P1 x P0);
  * EBH_ERR_UNSUPPORTED otherwise.  split: the symmetric schedule's step
  * ranges per group pair (workgroups per pair; a power of 2 up to 64, 0 for
  * a default from the mesh size), ignored for EBH_SCHED_CHUNKED.  Both
