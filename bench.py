@@ -678,6 +678,37 @@ def operator_probe(L, meshes, level=6, reps=3):
             del op
 
 
+def kelvin_probe(L, meshes, reps=3):
+    """The Kelvin single layer on the 256 x 128 torus (65536 triangles, P1,
+    7-point far rule, nu 0.3) in both precisions on both far schedules: the
+    default (pair-symmetric) and the chunked kernel at the packaged table's
+    chunked shape (256-thread blocks, 32-triangle target tiles, 8 source
+    chunks: 2048 tiles x 8 = 16384 workgroups).
+    One warm-up and reps applies each, one JSON line per case (the kernel
+    trace of this workload is the profile of the Kelvin kernels)."""
+    P, T = meshes.torus(1.0, 0.4, 256, 128)
+    m = L.Mesh(P, T)
+    rng = np.random.default_rng(3)
+    for prec in ("fp32", "fp64"):
+        for sched in ("default", "chunked"):
+            op = L.Operator(m, "kelvin", 1, 1, prec, line_rule=4, tri_rule=4, nu=0.3)
+            if sched == "chunked":
+                op.set_launch(256, 32, 8)
+            x = L.DeviceArray.from_host(rng.uniform(-1, 1, op.ncols))
+            y = L.DeviceArray(op.nrows)
+            op.apply(x, y)
+            L.check(L.lib().ebh_synchronize())
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                op.apply(x, y)
+            L.check(L.lib().ebh_synchronize())
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            print(json.dumps({"operator": "kelvin P1", "precision": prec,
+                              "schedule": op.schedule, "partial": op.partial,
+                              "ms_per_apply": ms}), flush=True)
+            del op
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -714,6 +745,10 @@ def main():
                     help="run only the double layer, adjoint, P0 single layer "
                          "and hypersingular applies at icosphere L6 in both "
                          "precisions (no JSON bench line)")
+    ap.add_argument("--kelvin-probe", action="store_true",
+                    help="run only the Kelvin single layer on the 256 x 128 "
+                         "torus, both precisions and far schedules (no JSON "
+                         "bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "easybem-hip runs on one GPU; --gpus must "
@@ -731,6 +766,9 @@ def main():
         return
     if args.operator_probe:
         operator_probe(L, meshes)
+        return
+    if args.kelvin_probe:
+        kelvin_probe(L, meshes)
         return
     P, T = meshes.icosphere(args.level)
     x0 = np.array([0.0, 0.0, 2.0])
